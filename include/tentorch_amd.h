@@ -1,0 +1,139 @@
+/*
+ * tentorch_amd: the C ABI of the MI355X device library for TenTorch's block-sparse contraction path.
+ *
+ * The host engine (legs, fusion trees, leg-move and contraction planning) compiles every leg move
+ * and every contraction into a descriptor table; this library uploads the tables once per plan and
+ * runs them with hand-written HIP kernels for gfx950.  No torch types cross this boundary: device
+ * buffers are plain pointers, streams are `hipStream_t` passed as `void *`, plans are opaque handles
+ * owned by the library.  Every entry point returns 0 on success and a negative status on failure;
+ * `tt_last_error` then holds the message (thread-local).  Nothing here throws.
+ *
+ * Reference interfaces each entry point replaces (paths relative to the reference checkout):
+ *   tt_gss_*     the leg-move execution of `Tensor::place`: the per-block chain walk `carried`
+ *                (tentorch/tensors/place.cpp:127-180) with the host pointer walk / per-block device
+ *                matmul of `TensorCaches::apply` (tentorch/tensors/cache.cpp:66-116,297-319);
+ *   tt_ggemm_*   the per-root block products `out.block(r, s).addmm_(L, R)` of `sym::contract`
+ *                (tentorch/linalg/graded/contract.cpp:144-163);
+ *   tt_dot       the reduced scalar product `inner` and `norm`
+ *                (tentorch/linalg/graded/compare.cpp:31-45), with the scalar left on the device;
+ *   tt_axpby / tt_scale   the per-block `add_/sub_/mul_/div_` of `Tensor` arithmetic
+ *                (tentorch/tensors/tensor.cpp:138-181) on a whole flat tensor at once;
+ *   tt_graph_*   no counterpart: capture of a launch sequence (the Lanczos matvec) into a hipGraph.
+ */
+#ifndef TENTORCH_AMD_H
+#define TENTORCH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Element types of device buffers. */
+enum tt_dtype { TT_F64 = 0, TT_F32 = 1, TT_C128 = 2 };
+
+/* Largest rank of a gather-scale-scatter box (tensors of up to eight legs). */
+#define TT_MAX_DIMS 8
+
+/*
+ * One target tile of a leg move: a dense box of the target buffer, `extent[d]` wide along axis d
+ * and `dst_stride[d]` apart (elements), starting at `dst_offset`.  Its value is the sum, in source
+ * order, of `coef * source box` over the `num_sources` sources starting at `first_source`; a tile
+ * with no source is written with zeros.  Axes are listed with the destination's slowest first.
+ */
+typedef struct tt_gss_task {
+  int64_t dst_offset;
+  int32_t ndim;
+  int32_t first_source;
+  int32_t num_sources;
+  int32_t group;          /* launch group of the per-block (plain) form */
+  int64_t extent[TT_MAX_DIMS];
+  int64_t dst_stride[TT_MAX_DIMS];
+} tt_gss_task;
+
+/* One source box of a tile: same extents as its task, its own offset and strides. */
+typedef struct tt_gss_source {
+  int64_t src_offset;
+  int64_t src_stride[TT_MAX_DIMS];
+  double coef_re;
+  double coef_im;
+} tt_gss_source;
+
+typedef struct tt_gss_plan tt_gss_plan;
+
+int tt_gss_plan_create(const tt_gss_task *tasks, int64_t num_tasks, const tt_gss_source *sources,
+                       int64_t num_sources, tt_gss_plan **plan);
+int tt_gss_plan_destroy(tt_gss_plan *plan);
+/* Number of destination elements the plan writes and of source elements it reads. */
+int tt_gss_plan_counts(const tt_gss_plan *plan, int64_t *written, int64_t *read);
+/* All tiles in one launch.  `src` and `dst` must not overlap. */
+int tt_gss_execute(const tt_gss_plan *plan, int dtype, const void *src, void *dst, void *stream);
+/* The per-block form: one launch per launch group (one source block into one target block). */
+int tt_gss_execute_plain(const tt_gss_plan *plan, int dtype, const void *src, void *dst, void *stream);
+
+/*
+ * One output block of a grouped GEMM: C[c_offset : m x n, row-major, ldc = n] =
+ * sum over its terms of A_t (m x k_t, row-major, lda = k_t) times B_t (k_t x n, row-major, ldb = n),
+ * the terms added in list order; a block with no term is written with zeros.
+ */
+typedef struct tt_gemm_group {
+  int64_t c_offset;
+  int64_t m;
+  int64_t n;
+  int32_t first_term;
+  int32_t num_terms;
+} tt_gemm_group;
+
+typedef struct tt_gemm_term {
+  int64_t a_offset;
+  int64_t b_offset;
+  int64_t k;
+} tt_gemm_term;
+
+typedef struct tt_ggemm_plan tt_ggemm_plan;
+
+int tt_ggemm_plan_create(const tt_gemm_group *groups, int64_t num_groups, const tt_gemm_term *terms,
+                         int64_t num_terms, tt_ggemm_plan **plan);
+int tt_ggemm_plan_destroy(tt_ggemm_plan *plan);
+/* Real multiply-add count of the plan (2 m n k summed over terms), and tile count of one launch. */
+int tt_ggemm_plan_counts(const tt_ggemm_plan *plan, double *flops, int64_t *tiles);
+/*
+ * One launch for every output block.  `a_type`, `b_type`, `c_type` are the element types of the
+ * three buffers (TT_F64 or TT_F32); `compute_type` TT_F64 runs FP64 MFMA with FP64 accumulation,
+ * TT_F32 runs FP32 MFMA with FP32 accumulation (operands rounded to FP32 on load).
+ */
+int tt_ggemm_execute(const tt_ggemm_plan *plan, int a_type, int b_type, int c_type, int compute_type,
+                     const void *a, const void *b, void *c, void *stream);
+/* The per-block form: one launch per (output block, term), the first term overwriting. */
+int tt_ggemm_execute_plain(const tt_ggemm_plan *plan, int dtype, const void *a, const void *b, void *c,
+                           void *stream);
+
+/* result[0] = sum_i conj(x_i) * y_i over n elements (TT_F64 or TT_F32 inputs; the result is one
+ * double, or two for TT_C128), summed in a fixed order, left in device memory. */
+int tt_dot(int dtype, int64_t n, const void *x, const void *y, double *result, void *stream);
+/* y = alpha * x + beta * y, with alpha and beta read from device memory when the pointers are
+ * non-null (`alpha_dev`, `beta_dev`, one double each) and from the host values otherwise. */
+int tt_axpby(int dtype, int64_t n, double alpha, const double *alpha_dev, const void *x, double beta,
+             const double *beta_dev, void *y, void *stream);
+/* y = x converted between TT_F64 and TT_F32. */
+int tt_convert(int from_type, int to_type, int64_t n, const void *x, void *y, void *stream);
+
+typedef struct tt_graph tt_graph;
+/* Begin capturing every launch on `stream` into a graph. */
+int tt_graph_begin(void *stream);
+/* End the capture and instantiate it. */
+int tt_graph_end(void *stream, tt_graph **graph);
+int tt_graph_launch(const tt_graph *graph, void *stream);
+int tt_graph_destroy(tt_graph *graph);
+
+/* Copy the last error of this thread into `buffer` (NUL-terminated, truncated to `size`). */
+int tt_last_error(char *buffer, size_t size);
+/* Library version, for a loader to check. */
+int tt_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TENTORCH_AMD_H */

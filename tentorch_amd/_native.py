@@ -1,0 +1,124 @@
+'''
+Loader of the two native libraries: the host engine (`_engine`, planning) and the device library
+(`libtentorch_hip.so`, the C ABI of `include/tentorch_amd.h`).  There is no fallback: when either
+library is missing, or no GPU is visible when device work is asked for, this module raises.
+'''
+
+import ctypes
+import os
+
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+try:
+    from . import _engine as engine  # noqa: F401
+except ImportError as error:  # pragma: no cover - exercised only on a broken install
+    raise ImportError('tentorch_amd: the host engine is not built; run `python -m tentorch_amd.build` '
+                      f'({error})') from error
+
+_lib = ctypes.CDLL(os.path.join(HERE, 'libtentorch_hip.so'))
+
+TT_F64, TT_F32, TT_C128 = 0, 1, 2
+
+_c_void_p, _i64, _int, _dbl = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+_lib.tt_dot.argtypes = [_int, _i64, _c_void_p, _c_void_p, _c_void_p, _c_void_p]
+_lib.tt_axpby.argtypes = [_int, _i64, _dbl, _c_void_p, _c_void_p, _dbl, _c_void_p, _c_void_p, _c_void_p]
+_lib.tt_convert.argtypes = [_int, _int, _i64, _c_void_p, _c_void_p, _c_void_p]
+_lib.tt_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+_lib.tt_version.argtypes = []
+_lib.tt_graph_begin.argtypes = [_c_void_p]
+_lib.tt_graph_end.argtypes = [_c_void_p, ctypes.POINTER(_c_void_p)]
+_lib.tt_graph_launch.argtypes = [_c_void_p, _c_void_p]
+_lib.tt_graph_destroy.argtypes = [_c_void_p]
+
+EXPORTED = ['tt_gss_plan_create', 'tt_gss_plan_destroy', 'tt_gss_plan_counts', 'tt_gss_execute',
+            'tt_gss_execute_plain', 'tt_ggemm_plan_create', 'tt_ggemm_plan_destroy', 'tt_ggemm_plan_counts',
+            'tt_ggemm_execute', 'tt_ggemm_execute_plain', 'tt_dot', 'tt_axpby', 'tt_convert', 'tt_graph_begin',
+            'tt_graph_end', 'tt_graph_launch', 'tt_graph_destroy', 'tt_last_error', 'tt_version']
+
+
+def library() -> ctypes.CDLL:
+    return _lib
+
+
+def last_error() -> str:
+    buffer = ctypes.create_string_buffer(1024)
+    _lib.tt_last_error(buffer, 1024)
+    return buffer.value.decode(errors='replace')
+
+
+def check(status: int, what: str) -> None:
+    if status != 0:
+        raise RuntimeError(f'{what}: {last_error()}')
+
+
+def dtype_code(dtype: torch.dtype) -> int:
+    if dtype == torch.float64:
+        return TT_F64
+    if dtype == torch.float32:
+        return TT_F32
+    if dtype == torch.complex128:
+        return TT_C128
+    raise TypeError(f'the HIP backend does not run tensors of {dtype}')
+
+
+def require_device(t: torch.Tensor) -> None:
+    if t.device.type != 'cuda':
+        raise RuntimeError('the HIP backend runs on the GPU only: this tensor lives on '
+                           f'{t.device}, and there is no CPU path')
+
+
+def stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def dot(dtype: torch.dtype, x: torch.Tensor, y: torch.Tensor, out: torch.Tensor) -> None:
+    '''out[0] (out[0:2] for complex) = sum conj(x) y, fixed summation order, on the device.'''
+    check(_lib.tt_dot(dtype_code(dtype), x.numel(), x.data_ptr(), y.data_ptr(), out.data_ptr(), stream()), 'dot')
+
+
+def axpby(dtype: torch.dtype, alpha, x, beta, y: torch.Tensor, alpha_dev=None, beta_dev=None) -> None:
+    '''y = alpha x + beta y on the device (alpha/beta from device scalars when given).'''
+    code = dtype_code(dtype)
+    n = y.numel()
+    if code == TT_C128:
+        if alpha_dev is not None or beta_dev is not None or isinstance(alpha, complex) or isinstance(beta, complex):
+            raise TypeError('complex scaling is not a real axpby')
+        code, n = TT_F64, 2 * n
+    check(_lib.tt_axpby(code, n, float(alpha), None if alpha_dev is None else alpha_dev.data_ptr(),
+                        None if x is None else x.data_ptr(), float(beta),
+                        None if beta_dev is None else beta_dev.data_ptr(), y.data_ptr(), stream()), 'axpby')
+
+
+def convert(x: torch.Tensor, y: torch.Tensor) -> None:
+    check(_lib.tt_convert(dtype_code(x.dtype), dtype_code(y.dtype), x.numel(), x.data_ptr(), y.data_ptr(),
+                          stream()), 'convert')
+
+
+class Graph:
+    '''A captured launch sequence (hipGraph) replayed on the current stream.'''
+
+    def __init__(self, handle):
+        self._handle = handle
+
+    def launch(self) -> None:
+        check(_lib.tt_graph_launch(self._handle, stream()), 'graph launch')
+
+    def __del__(self):
+        if getattr(self, '_handle', None):
+            _lib.tt_graph_destroy(self._handle)
+            self._handle = None
+
+
+def capture(fn) -> Graph:
+    '''Run `fn` under stream capture and return the instantiated graph.'''
+    s = stream()
+    check(_lib.tt_graph_begin(s), 'graph capture')
+    try:
+        fn()
+    finally:
+        handle = ctypes.c_void_p()
+        status = _lib.tt_graph_end(s, ctypes.byref(handle))
+    check(status, 'graph capture')
+    return Graph(handle)

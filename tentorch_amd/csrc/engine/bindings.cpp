@@ -1,0 +1,315 @@
+// Python binding of the host engine: categories, legs, fusion trees, tensor structures, and the
+// planned programs (einsum, contract, place, conj) that run on the device through the C ABI.
+// Mirrors the reference's module surface (tentorch/_ext/bindings.cpp:166-471) for the objects the
+// contraction path needs; data pointers and streams cross as integers.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include "program.h"
+
+namespace py = pybind11;
+using namespace tt;
+
+namespace {
+
+Sector to_sector(const py::handle &h) {
+  std::vector<int64_t> v;
+  for (const py::handle item : h) v.push_back(item.cast<int64_t>());
+  return Sector(v);
+}
+
+py::tuple from_sector(const Sector &s) {
+  py::tuple t(s.n);
+  for (int i = 0; i < s.n; ++i) t[static_cast<size_t>(i)] = py::int_(s.v[i]);
+  return t;
+}
+
+py::list sectors_list(const std::vector<Sector> &ss) {
+  py::list out;
+  for (const Sector &s : ss) out.append(from_sector(s));
+  return out;
+}
+
+py::array_t<std::complex<double>> matrix_array(const Matrix &m) {
+  py::array_t<std::complex<double>> a({m.rows, m.cols});
+  auto r = a.mutable_unchecked<2>();
+  for (int64_t i = 0; i < m.rows; ++i)
+    for (int64_t j = 0; j < m.cols; ++j) r(i, j) = m(i, j);
+  return a;
+}
+
+struct PyCategory {
+  CategoryPtr cat;
+};
+
+struct PyLeg {
+  LegPtr leg;
+};
+
+struct PyStructure {
+  StructurePtr s;
+};
+
+struct PyProgram {
+  ProgramPtr p;
+};
+
+Legs to_legs(const std::vector<PyLeg> &legs) {
+  Legs out;
+  for (const PyLeg &l : legs) out.push_back(l.leg);
+  return out;
+}
+
+py::object tree_object(const FusionTree &t) {
+  py::dict d;
+  return py::make_tuple(sectors_list(t.sectors), sectors_list(t.inner), py::cast(t.copies), from_sector(t.coupled));
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_engine, m) {
+  m.doc() = "TenTorch MI355X host engine";
+
+  py::register_exception_translator([](std::exception_ptr p) {
+    try {
+      if (p) std::rethrow_exception(p);
+    } catch (const std::out_of_range &e) {
+      PyErr_SetString(PyExc_IndexError, e.what());
+    }
+  });
+
+  py::class_<PyCategory>(m, "FusionCategory")
+      .def_property_readonly("name", [](const PyCategory &c) { return c.cat->name(); })
+      .def_property_readonly("abelian", [](const PyCategory &c) { return c.cat->abelian(); })
+      .def_property_readonly("real_symbols", [](const PyCategory &c) { return c.cat->real_symbols(); })
+      .def("outcomes",
+           [](const PyCategory &c, py::handle a, py::handle b) {
+             py::list out;
+             for (const Outcome &o : c.cat->outcomes(to_sector(a), to_sector(b)))
+               out.append(py::make_tuple(from_sector(o.sector), o.multiplicity));
+             return out;
+           })
+      .def("n_symbol", [](const PyCategory &c, py::handle a, py::handle b, py::handle x) {
+        return c.cat->n_symbol(to_sector(a), to_sector(b), to_sector(x));
+      })
+      .def("can_fuse_to", [](const PyCategory &c, py::handle a, py::handle b, py::handle x) {
+        return c.cat->n_symbol(to_sector(a), to_sector(b), to_sector(x)) > 0;
+      })
+      .def("F_symbol", [](const PyCategory &c, py::handle a, py::handle b, py::handle x, py::handle d) {
+        return matrix_array(c.cat->F(to_sector(a), to_sector(b), to_sector(x), to_sector(d)));
+      })
+      .def("R_symbol", [](const PyCategory &c, py::handle b, py::handle x) {
+        return matrix_array(c.cat->R(to_sector(b), to_sector(x)));
+      })
+      .def("q_dim", [](const PyCategory &c, py::handle a) { return c.cat->q_dim(to_sector(a)); })
+      .def("dual", [](const PyCategory &c, py::handle a) { return from_sector(c.cat->dual(to_sector(a))); })
+      .def("frobenius_schur_sign", [](const PyCategory &c, py::handle a) { return c.cat->frobenius_schur_sign(to_sector(a)); })
+      .def("trivial", [](const PyCategory &c) { return from_sector(c.cat->trivial()); })
+      .def("clear_cache", [](const PyCategory &c) { c.cat->clear_cache(); })
+      .def("__eq__", [](const PyCategory &a, const PyCategory &b) { return a.cat == b.cat; })
+      .def("__hash__", [](const PyCategory &c) { return reinterpret_cast<uintptr_t>(c.cat.get()); })
+      .def("__repr__", [](const PyCategory &c) { return c.cat->name(); });
+
+  m.def("no_symmetry", [] { return PyCategory{no_symmetry()}; });
+  m.def("u1", [] { return PyCategory{u1_symmetry()}; });
+  m.def("z_n", [](int64_t n) { return PyCategory{z_n_symmetry(n)}; });
+  m.def("fermion_parity", [] { return PyCategory{fermion_parity()}; });
+  m.def("su2", [] { return PyCategory{su2_symmetry()}; });
+  m.def("fibonacci", [] { return PyCategory{fibonacci_category()}; });
+
+  py::class_<PyLeg>(m, "Leg")
+      .def(py::init([](const PyCategory &cat, py::iterable sectors, py::iterable mults) {
+             std::vector<Sector> ss;
+             for (const py::handle s : sectors) ss.push_back(to_sector(s));
+             std::vector<int64_t> ms;
+             for (const py::handle x : mults) ms.push_back(x.cast<int64_t>());
+             return PyLeg{Leg::make(cat.cat, ss, ms)};
+           }),
+           py::arg("cat"), py::arg("sectors"), py::arg("multiplicities"))
+      .def_property_readonly("cat", [](const PyLeg &l) { return PyCategory{l.leg->cat()}; })
+      .def_property_readonly("sectors", [](const PyLeg &l) { return sectors_list(l.leg->sectors()); })
+      .def_property_readonly("multiplicities", [](const PyLeg &l) { return l.leg->multiplicities(); })
+      .def("nblocks", [](const PyLeg &l) { return l.leg->nblocks(); })
+      .def("sector", [](const PyLeg &l, int64_t b) {
+        if (b < 0 || b >= l.leg->nblocks()) throw std::out_of_range("block index out of range");
+        return from_sector(l.leg->sectors()[static_cast<size_t>(b)]);
+      })
+      .def("multiplicity", [](const PyLeg &l, int64_t b) {
+        if (b < 0 || b >= l.leg->nblocks()) throw std::out_of_range("block index out of range");
+        return l.leg->multiplicities()[static_cast<size_t>(b)];
+      })
+      .def("multiplets", [](const PyLeg &l) { return l.leg->multiplets(); })
+      .def("fuse", [](const PyLeg &l, const PyLeg &o) { return PyLeg{l.leg->fuse(*o.leg)}; })
+      .def("conj", [](const PyLeg &l) { return PyLeg{l.leg->conj()}; })
+      .def("capped", [](const PyLeg &l, int64_t n) { return PyLeg{l.leg->capped(n)}; })
+      .def("__eq__", [](const PyLeg &a, const PyLeg &b) { return a.leg == b.leg; })
+      .def("__ne__", [](const PyLeg &a, const PyLeg &b) { return a.leg != b.leg; })
+      .def("__hash__", [](const PyLeg &l) { return l.leg->hash(); })
+      .def("_address", [](const PyLeg &l) { return reinterpret_cast<uintptr_t>(l.leg.get()); })
+      .def("__repr__", [](const PyLeg &l) {
+        std::string out = "Leg(" + l.leg->cat()->name() + ", [";
+        for (int64_t b = 0; b < l.leg->nblocks(); ++b) {
+          if (b) out += ", ";
+          out += render(l.leg->sectors()[static_cast<size_t>(b)]) + ": " + std::to_string(l.leg->multiplicities()[static_cast<size_t>(b)]);
+        }
+        return out + "])";
+      });
+
+  m.def("fusion_trees", [](const PyCategory &cat, const std::vector<PyLeg> &legs, py::handle coupled) {
+    py::list out;
+    for (const FusionTree &t : fusion_trees(*cat.cat, to_legs(legs), to_sector(coupled))) out.append(tree_object(t));
+    return out;
+  });
+  m.def("reachable_charges", [](const PyCategory &cat, const std::vector<PyLeg> &legs) {
+    return sectors_list(reachable_charges(*cat.cat, to_legs(legs)));
+  });
+  m.def("block_keys", [](const PyCategory &cat, const std::vector<PyLeg> &cod, const std::vector<PyLeg> &dom, py::handle q) {
+    py::list out;
+    for (const BlockKey &k : block_keys(*cat.cat, to_legs(cod), to_legs(dom), to_sector(q)))
+      out.append(py::make_tuple(from_sector(k.codomain), from_sector(k.domain), k.copy));
+    return out;
+  });
+  m.def("side_layout", [](const PyCategory &cat, const std::vector<PyLeg> &legs, py::handle root) {
+    LayoutPtr l = layout_of(cat.cat, to_legs(legs), to_sector(root));
+    py::list trees;
+    for (const FusionTree &t : l->trees) trees.append(tree_object(t));
+    return py::make_tuple(trees, l->offsets);
+  });
+
+  py::class_<PyStructure>(m, "Structure")
+      .def(py::init([](const PyCategory &cat, const std::vector<PyLeg> &cod, const std::vector<PyLeg> &dom, py::handle q) {
+             return PyStructure{Structure::make(cat.cat, to_legs(cod), to_legs(dom), to_sector(q))};
+           }),
+           py::arg("cat"), py::arg("codomain"), py::arg("domain"), py::arg("qtotal"))
+      .def_property_readonly("cat", [](const PyStructure &s) { return PyCategory{s.s->cat()}; })
+      .def_property_readonly("codomain", [](const PyStructure &s) {
+        std::vector<PyLeg> out;
+        for (const LegPtr &l : s.s->codomain()) out.push_back(PyLeg{l});
+        return out;
+      })
+      .def_property_readonly("domain", [](const PyStructure &s) {
+        std::vector<PyLeg> out;
+        for (const LegPtr &l : s.s->domain()) out.push_back(PyLeg{l});
+        return out;
+      })
+      .def_property_readonly("qtotal", [](const PyStructure &s) { return from_sector(s.s->qtotal()); })
+      .def_property_readonly("size", [](const PyStructure &s) { return s.s->size(); })
+      .def_property_readonly("id", [](const PyStructure &s) { return s.s->id(); })
+      .def("nlegs", [](const PyStructure &s) { return s.s->nlegs(); })
+      .def("nblocks", [](const PyStructure &s) { return static_cast<int64_t>(s.s->blocks().size()); })
+      .def("blocks", [](const PyStructure &s) {
+        py::list out;
+        for (const BlockInfo &b : s.s->blocks())
+          out.append(py::make_tuple(py::make_tuple(from_sector(b.key.codomain), from_sector(b.key.domain), b.key.copy),
+                                    b.rows, b.cols, b.offset));
+        return out;
+      })
+      .def("find", [](const PyStructure &s, py::handle r, py::handle d, int64_t copy) {
+        return s.s->find(to_sector(r), to_sector(d), copy);
+      })
+      .def("__eq__", [](const PyStructure &a, const PyStructure &b) { return a.s == b.s; })
+      .def("__hash__", [](const PyStructure &s) { return s.s->id(); });
+
+  py::class_<PyProgram>(m, "Program")
+      .def_property_readonly("out", [](const PyProgram &p) { return PyStructure{p.p->out}; })
+      .def_property_readonly("workspace", [](const PyProgram &p) { return p.p->workspace; })
+      .def_property_readonly("plain_workspace", [](const PyProgram &p) { return p.p->plain_workspace; })
+      .def_property_readonly("flops", [](const PyProgram &p) { return p.p->flops; })
+      .def_property_readonly("move_elements", [](const PyProgram &p) { return p.p->move_elements; })
+      .def_property_readonly("hoistable", [](const PyProgram &p) { return p.p->hoistable(); })
+      .def_property_readonly("id", [](const PyProgram &p) { return reinterpret_cast<uintptr_t>(p.p.get()); })
+      .def("op_kinds", [](const PyProgram &p) {
+        std::vector<std::string> out;
+        for (const Op &op : p.p->ops)
+          out.push_back(op.kind == OpKind::move ? "move" : op.kind == OpKind::gemm ? "gemm" : op.kind == OpKind::copy ? "copy" : "add");
+        return out;
+      })
+      .def("ops",
+           [](const PyProgram &p) {
+             py::list out;
+             for (const Op &op : p.p->ops)
+               out.append(py::make_tuple(op.kind == OpKind::move ? "move" : op.kind == OpKind::gemm ? "gemm"
+                                         : op.kind == OpKind::copy ? "copy" : "add",
+                                         op.a, op.b, op.dst));
+             return out;
+           })
+      .def("slots",
+           [](const PyProgram &p) {
+             py::list out;
+             for (const Slot &s : p.p->slots)
+               out.append(py::make_tuple(s.kind == Slot::operand ? "operand" : s.kind == Slot::output ? "output" : "workspace",
+                                         s.index, s.offset, s.structure->size()));
+             return out;
+           })
+      .def("move_tables",
+           [](const PyProgram &p, int64_t which) {
+             // The descriptor table of the which-th leg move, for inspection.
+             int64_t seen = 0;
+             for (const Op &op : p.p->ops) {
+               if (op.kind != OpKind::move) continue;
+               if (seen++ != which) continue;
+               py::list tasks;
+               for (const tt_gss_task &t : op.move->tasks) {
+                 std::vector<int64_t> ext(t.extent, t.extent + t.ndim), ds(t.dst_stride, t.dst_stride + t.ndim);
+                 tasks.append(py::make_tuple(t.dst_offset, ext, ds, t.first_source, t.num_sources));
+               }
+               py::list sources;
+               for (const tt_gss_source &s : op.move->sources) {
+                 std::vector<int64_t> ss(s.src_stride, s.src_stride + TT_MAX_DIMS);
+                 sources.append(py::make_tuple(s.src_offset, ss, std::complex<double>(s.coef_re, s.coef_im)));
+               }
+               return py::make_tuple(tasks, sources);
+             }
+             throw std::out_of_range("the program has no such leg move");
+           })
+      .def("gemm_tables",
+           [](const PyProgram &p, int64_t which) {
+             int64_t seen = 0;
+             for (const Op &op : p.p->ops) {
+               if (op.kind != OpKind::gemm) continue;
+               if (seen++ != which) continue;
+               py::list groups, terms;
+               for (const tt_gemm_group &g : op.gemm->groups)
+                 groups.append(py::make_tuple(g.c_offset, g.m, g.n, g.first_term, g.num_terms));
+               for (const tt_gemm_term &t : op.gemm->terms) terms.append(py::make_tuple(t.a_offset, t.b_offset, t.k));
+               return py::make_tuple(groups, terms);
+             }
+             throw std::out_of_range("the program has no such block product");
+           })
+      .def("run",
+           [](const PyProgram &p, int dtype, const std::vector<uintptr_t> &operands, uintptr_t out, uintptr_t workspace,
+              uintptr_t stream, int mode, uint32_t skip) {
+             std::vector<const void *> ptrs;
+             for (uintptr_t x : operands) ptrs.push_back(reinterpret_cast<const void *>(x));
+             py::gil_scoped_release release;
+             p.p->run(dtype, ptrs, reinterpret_cast<void *>(out), reinterpret_cast<void *>(workspace),
+                      reinterpret_cast<void *>(stream), mode == 1 ? Mode::plain : Mode::batched, skip);
+           },
+           py::arg("dtype"), py::arg("operands"), py::arg("out"), py::arg("workspace"), py::arg("stream"),
+           py::arg("mode") = 0, py::arg("skip") = 0);
+
+  m.def("plan_einsum", [](const std::string &spec, const std::vector<PyStructure> &ops, py::object charge) {
+    std::vector<StructurePtr> ss;
+    for (const PyStructure &s : ops) ss.push_back(s.s);
+    if (charge.is_none()) return PyProgram{plan_einsum(spec, ss, nullptr)};
+    const Sector q = to_sector(charge);
+    return PyProgram{plan_einsum(spec, ss, &q)};
+  });
+  m.def("plan_contract", [](const PyStructure &l, const PyStructure &r, const std::vector<int64_t> &ll,
+                            const std::vector<int64_t> &rl, py::object charge) {
+    if (charge.is_none()) return PyProgram{plan_contract(l.s, r.s, ll, rl, nullptr)};
+    const Sector q = to_sector(charge);
+    return PyProgram{plan_contract(l.s, r.s, ll, rl, &q)};
+  });
+  m.def("plan_place", [](const PyStructure &s, const std::vector<int64_t> &above, py::object below) {
+    const std::vector<int64_t> b = below.is_none() ? complement(s.s->nlegs(), above) : below.cast<std::vector<int64_t>>();
+    return PyProgram{plan_place(s.s, above, b)};
+  });
+  m.def("plan_conj", [](const PyStructure &s) { return PyProgram{plan_conj(s.s)}; });
+  m.def("clear_caches", [] {
+    clear_program_cache();
+    clear_move_cache();
+    clear_engine_caches();
+  });
+}

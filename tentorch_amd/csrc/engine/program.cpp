@@ -1,0 +1,656 @@
+#include "program.h"
+
+#include <algorithm>
+#include <cctype>
+#include <functional>
+#include <limits>
+#include <map>
+#include <stdexcept>
+#include <tuple>
+
+namespace tt {
+
+std::string device_error();
+
+namespace {
+
+constexpr int64_t kMaxCharges = 4096;  // einsum.cpp:25
+
+int64_t element_size(int dtype) { return dtype == TT_F32 ? 4 : dtype == TT_C128 ? 16 : 8; }
+
+void check(int status, const char *what) {
+  if (status != 0) throw std::runtime_error(std::string(what) + ": " + device_error());
+}
+
+// contract.cpp:95-106
+void require_single_closing(const Category &cat, const Sector &above, const Sector &below, const Sector &charge,
+                            const std::string &subject = "the roots", const std::string &what = "a contraction") {
+  if (cat.n_symbol(above, cat.dual(below), charge) > 1)
+    throw std::invalid_argument(subject + " " + render(above) + " and " + render(below) + " close to " + render(charge) +
+                                " in more than one way, which " + what + " does not recouple yet");
+}
+
+std::vector<int64_t> without(int64_t total, const std::vector<int64_t> &taken) {
+  std::vector<bool> named(static_cast<size_t>(total), false);
+  for (int64_t leg : taken) named[static_cast<size_t>(leg)] = true;
+  std::vector<int64_t> out;
+  for (int64_t leg = 0; leg < total; ++leg)
+    if (!named[static_cast<size_t>(leg)]) out.push_back(leg);
+  return out;
+}
+
+// contract.cpp:47-77
+void check_pairs(const Structure &left, const Structure &right, const std::vector<int64_t> &left_legs,
+                 const std::vector<int64_t> &right_legs) {
+  if (left.cat() != right.cat()) throw std::invalid_argument("a contraction needs both tensors over one category");
+  if (left_legs.size() != right_legs.size())
+    throw std::invalid_argument("the two legs of a contraction must be given in pairs, got " +
+                                std::to_string(left_legs.size()) + " and " + std::to_string(right_legs.size()));
+  auto check_side = [](const Structure &t, const std::vector<int64_t> &legs, const char *side) {
+    std::vector<bool> named(static_cast<size_t>(t.nlegs()), false);
+    for (int64_t leg : legs) {
+      if (leg < 0 || leg >= t.nlegs())
+        throw std::invalid_argument("the leg " + std::to_string(leg) + " of the " + side +
+                                    " tensor is out of range of its " + std::to_string(t.nlegs()) + " legs");
+      if (named[static_cast<size_t>(leg)])
+        throw std::invalid_argument("the leg " + std::to_string(leg) + " of the " + side + " tensor is summed twice");
+      named[static_cast<size_t>(leg)] = true;
+    }
+  };
+  check_side(left, left_legs, "first");
+  check_side(right, right_legs, "second");
+  const Legs la = left.legs(), ra = right.legs();
+  for (size_t at = 0; at < left_legs.size(); ++at)
+    if (la[static_cast<size_t>(left_legs[at])] != ra[static_cast<size_t>(right_legs[at])])
+      throw std::invalid_argument("the pair at position " + std::to_string(at) + " joins the leg " +
+                                  std::to_string(left_legs[at]) + " of the first tensor to the leg " +
+                                  std::to_string(right_legs[at]) + " of the second, and the two do not name one space");
+}
+
+// The block products of two aligned operands (contract.cpp:128-163).
+GemmPlanPtr plan_gemm(const StructurePtr &above, const StructurePtr &under, const Sector &charge) {
+  const Category &cat = *above->cat();
+  auto g = std::make_shared<GemmPlan>();
+  g->above = above;
+  g->under = under;
+  g->out = Structure::make(above->cat(), above->codomain(), under->domain(), charge);
+  for (const BlockInfo &b : above->blocks()) require_single_closing(cat, b.key.codomain, b.key.domain, above->qtotal());
+  for (const BlockInfo &b : under->blocks()) require_single_closing(cat, b.key.codomain, b.key.domain, under->qtotal());
+  for (const BlockInfo &b : g->out->blocks()) require_single_closing(cat, b.key.codomain, b.key.domain, charge);
+  std::map<Sector, std::vector<int64_t>> rows;
+  for (int64_t j = 0; j < static_cast<int64_t>(under->blocks().size()); ++j)
+    rows[under->blocks()[static_cast<size_t>(j)].key.codomain].push_back(j);
+  std::vector<std::vector<tt_gemm_term>> per_block(g->out->blocks().size());
+  for (const BlockInfo &lb : above->blocks()) {
+    const auto found = rows.find(lb.key.domain);
+    if (found == rows.end()) continue;
+    for (int64_t j : found->second) {
+      const BlockInfo &rb = under->blocks()[static_cast<size_t>(j)];
+      if (cat.n_symbol(lb.key.codomain, cat.dual(rb.key.domain), charge) == 0) continue;
+      if (lb.cols != rb.rows)
+        throw std::logic_error("the two blocks over the roots " + render(lb.key.domain) + " do not line up: " +
+                               std::to_string(lb.cols) + " against " + std::to_string(rb.rows));
+      const int64_t ob = g->out->find(lb.key.codomain, rb.key.domain, 0);
+      if (ob < 0)
+        throw std::out_of_range("the tensor has no block at " + render(lb.key.codomain) + " x " + render(rb.key.domain) +
+                                " of copy 0");
+      per_block[static_cast<size_t>(ob)].push_back(tt_gemm_term{lb.offset, rb.offset, lb.cols});
+      g->flops += 2.0 * static_cast<double>(lb.rows) * static_cast<double>(rb.cols) * static_cast<double>(lb.cols);
+    }
+  }
+  for (size_t ob = 0; ob < per_block.size(); ++ob) {
+    const BlockInfo &b = g->out->blocks()[ob];
+    tt_gemm_group grp{};
+    grp.c_offset = b.offset;
+    grp.m = b.rows;
+    grp.n = b.cols;
+    grp.first_term = static_cast<int32_t>(g->terms.size());
+    grp.num_terms = static_cast<int32_t>(per_block[ob].size());
+    g->terms.insert(g->terms.end(), per_block[ob].begin(), per_block[ob].end());
+    g->groups.push_back(grp);
+  }
+  return g;
+}
+
+struct Builder {
+  ProgramPtr p = std::make_shared<Program>();
+  std::vector<uint32_t> deps;
+
+  int32_t operand(int32_t index, const StructurePtr &s) {
+    p->slots.push_back(Slot{Slot::operand, index, 0, s});
+    deps.push_back(1u << (index < 32 ? index : 31));
+    return static_cast<int32_t>(p->slots.size() - 1);
+  }
+  int32_t fresh(const StructurePtr &s, uint32_t dep) {
+    p->slots.push_back(Slot{Slot::workspace, 0, 0, s});
+    deps.push_back(dep);
+    return static_cast<int32_t>(p->slots.size() - 1);
+  }
+  const StructurePtr &structure(int32_t v) const { return p->slots[static_cast<size_t>(v)].structure; }
+  int32_t move(int32_t v, const std::vector<int64_t> &above, const std::vector<int64_t> &below) {
+    MovePlanPtr plan = plan_move(structure(v), above, below);
+    if (plan->identity) return v;
+    const int32_t d = fresh(plan->target, deps[static_cast<size_t>(v)]);
+    Op op{OpKind::move, plan, nullptr, v, -1, d, deps[static_cast<size_t>(v)]};
+    p->ops.push_back(op);
+    p->move_elements += plan->elements();
+    return d;
+  }
+  int32_t gemm(int32_t a, int32_t b, const GemmPlanPtr &g) {
+    const uint32_t dep = deps[static_cast<size_t>(a)] | deps[static_cast<size_t>(b)];
+    const int32_t d = fresh(g->out, dep);
+    p->ops.push_back(Op{OpKind::gemm, nullptr, g, a, b, d, dep});
+    p->flops += g->flops;
+    return d;
+  }
+  // Route the given result values into the output buffer: the first one is written there by its
+  // producer (or copied when it is an operand), the rest are added.
+  void finish(const std::vector<int32_t> &results) {
+    const int32_t first = results.front();
+    p->out = structure(first);
+    Slot &slot = p->slots[static_cast<size_t>(first)];
+    if (slot.kind == Slot::operand) {
+      const int32_t d = fresh(p->out, deps[static_cast<size_t>(first)]);
+      p->slots[static_cast<size_t>(d)].kind = Slot::output;
+      p->ops.push_back(Op{OpKind::copy, nullptr, nullptr, first, -1, d, deps[static_cast<size_t>(first)]});
+    } else {
+      slot.kind = Slot::output;
+    }
+    int32_t out_value = -1;
+    for (size_t i = 0; i < p->slots.size(); ++i)
+      if (p->slots[i].kind == Slot::output) out_value = static_cast<int32_t>(i);
+    for (size_t r = 1; r < results.size(); ++r)
+      p->ops.push_back(Op{OpKind::add, nullptr, nullptr, results[r], -1, out_value, ~0u});
+    // Workspace: every value that is neither an operand nor the output gets its own region.
+    int64_t offset = 0;
+    for (Slot &s : p->slots)
+      if (s.kind == Slot::workspace) {
+        s.offset = offset;
+        offset += s.structure->size();
+      }
+    p->workspace = offset;
+    int64_t widest = 0;
+    for (const Op &op : p->ops)
+      if (op.kind == OpKind::move) widest = std::max(widest, op.move->max_intermediate);
+    p->plain_workspace = 2 * widest;
+  }
+};
+
+struct Side {
+  std::vector<char> labels;
+  std::vector<bool> above;
+};
+
+Side side_of(const std::string &text) {
+  Side out;
+  for (char letter : text) {
+    if (letter == ' ') continue;
+    out.labels.push_back(static_cast<char>(std::tolower(static_cast<unsigned char>(letter))));
+    out.above.push_back(letter >= 'A' && letter <= 'Z');
+  }
+  return out;
+}
+
+bool holds(const std::vector<char> &labels, char label) {
+  return std::find(labels.begin(), labels.end(), label) != labels.end();
+}
+
+int64_t position_of(const std::vector<char> &labels, char label) {
+  auto it = std::find(labels.begin(), labels.end(), label);
+  if (it == labels.end()) throw std::logic_error(std::string("the label '") + label + "' left the equation");
+  return it - labels.begin();
+}
+
+std::vector<char> union_of(const std::vector<char> &l, const std::vector<char> &r) {
+  std::vector<char> out = l;
+  for (char c : r)
+    if (!holds(out, c)) out.push_back(c);
+  return out;
+}
+
+std::vector<char> shared_of(const std::vector<char> &l, const std::vector<char> &r) {
+  std::vector<char> out;
+  for (char c : l)
+    if (holds(r, c)) out.push_back(c);
+  return out;
+}
+
+std::vector<char> product_labels(const std::vector<char> &l, const std::vector<char> &r) {
+  const std::vector<char> shared = shared_of(l, r);
+  std::vector<char> out;
+  for (char c : union_of(l, r))
+    if (!holds(shared, c)) out.push_back(c);
+  return out;
+}
+
+double tensor_size(const std::vector<char> &labels, const std::map<char, double> &dims) {
+  double out = 1;
+  for (char c : labels) out *= dims.at(c);
+  return out;
+}
+
+// einsum.cpp:216-301
+std::vector<int64_t> fold_order(const std::vector<std::vector<char>> &inputs, const std::map<char, double> &dims) {
+  const int64_t count = static_cast<int64_t>(inputs.size());
+  std::vector<int64_t> order;
+  if (count <= 12) {
+    const int64_t masks = int64_t(1) << (count - 1);
+    std::vector<std::vector<char>> labels(static_cast<size_t>(masks), inputs.front());
+    for (int64_t mask = 1; mask < masks; ++mask) {
+      const int64_t bit = __builtin_ctzll(static_cast<unsigned long long>(mask));
+      labels[static_cast<size_t>(mask)] = union_of(labels[static_cast<size_t>(mask ^ (int64_t(1) << bit))], inputs[static_cast<size_t>(bit + 1)]);
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+      std::vector<double> cost(static_cast<size_t>(masks), std::numeric_limits<double>::infinity());
+      std::vector<int64_t> summed(static_cast<size_t>(masks), 0), last(static_cast<size_t>(masks), 0);
+      cost[0] = 0;
+      for (int64_t mask = 1; mask < masks; ++mask)
+        for (int64_t bit = 0; bit < count - 1; ++bit)
+          if (mask & (int64_t(1) << bit)) {
+            const auto shared = shared_of(labels[static_cast<size_t>(mask ^ (int64_t(1) << bit))], inputs[static_cast<size_t>(bit + 1)]);
+            if (pass == 0 && shared.empty()) continue;
+            const double candidate = cost[static_cast<size_t>(mask ^ (int64_t(1) << bit))] + tensor_size(labels[static_cast<size_t>(mask)], dims);
+            const size_t m = static_cast<size_t>(mask);
+            const int64_t ns = static_cast<int64_t>(shared.size());
+            const bool better = candidate < cost[m] || (candidate == cost[m] && ns > summed[m]) ||
+                                (candidate == cost[m] && ns == summed[m] && bit > last[m]);
+            if (better) {
+              cost[m] = candidate;
+              summed[m] = ns;
+              last[m] = bit;
+            }
+          }
+      if (cost[static_cast<size_t>(masks - 1)] == std::numeric_limits<double>::infinity()) continue;
+      for (int64_t mask = masks - 1; mask; mask ^= int64_t(1) << last[static_cast<size_t>(mask)])
+        order.push_back(last[static_cast<size_t>(mask)] + 1);
+      order.push_back(0);
+      std::reverse(order.begin(), order.end());
+      return order;
+    }
+  }
+  std::vector<bool> taken(inputs.size(), false);
+  std::vector<char> live = inputs.front();
+  order.push_back(0);
+  taken[0] = true;
+  for (int64_t step = 1; step < count; ++step) {
+    int64_t best = -1;
+    double best_size = 0;
+    bool found = false;
+    for (int pass = 0; pass < 2 && !found; ++pass)
+      for (int64_t at = 0; at < count; ++at) {
+        if (taken[static_cast<size_t>(at)]) continue;
+        if (shared_of(live, inputs[static_cast<size_t>(at)]).empty() != (pass == 1)) continue;
+        const double size = tensor_size(union_of(live, inputs[static_cast<size_t>(at)]), dims);
+        if (!found || size < best_size) {
+          found = true;
+          best = at;
+          best_size = size;
+        }
+      }
+    order.push_back(best);
+    taken[static_cast<size_t>(best)] = true;
+    live = union_of(live, inputs[static_cast<size_t>(best)]);
+  }
+  return order;
+}
+
+double leg_size(const Category &cat, const Leg &leg) {
+  double out = 0;
+  for (int64_t b = 0; b < leg.nblocks(); ++b) out += static_cast<double>(leg.multiplicities()[static_cast<size_t>(b)]) * cat.q_dim(leg.sectors()[static_cast<size_t>(b)]);
+  return out;
+}
+
+// align + contract of two values, one result per closing charge (einsum.cpp:306-345).
+std::vector<int32_t> combine(Builder &bld, const std::vector<int32_t> &lefts, const std::vector<char> &left_labels,
+                             const std::vector<int32_t> &rights, const std::vector<char> &right_labels) {
+  const auto shared = shared_of(left_labels, right_labels);
+  std::vector<int64_t> left_legs, right_legs;
+  for (char c : shared) {
+    left_legs.push_back(position_of(left_labels, c));
+    right_legs.push_back(position_of(right_labels, c));
+  }
+  std::vector<int32_t> folded;
+  for (int32_t l : lefts)
+    for (int32_t r : rights) {
+      const StructurePtr ls = bld.structure(l), rs = bld.structure(r);
+      check_pairs(*ls, *rs, left_legs, right_legs);
+      const int32_t above = bld.move(l, without(ls->nlegs(), left_legs), left_legs);
+      const int32_t under = bld.move(r, right_legs, without(rs->nlegs(), right_legs));
+      const Category &cat = *ls->cat();
+      std::vector<Sector> closings;
+      for (const Outcome &o : cat.outcomes(ls->qtotal(), rs->qtotal()))
+        if (std::find(closings.begin(), closings.end(), o.sector) == closings.end()) closings.push_back(o.sector);
+      for (const Sector &closed : closings) {
+        require_single_closing(cat, ls->qtotal(), rs->qtotal(), closed, "the charges", "a contraction");
+        folded.push_back(bld.gemm(above, under, plan_gemm(bld.structure(above), bld.structure(under), closed)));
+      }
+    }
+  return folded;
+}
+
+struct ProgramKey {
+  int kind;
+  std::string spec;
+  std::vector<uint64_t> structures;
+  std::vector<int64_t> a, b;
+  bool has_charge;
+  std::vector<int64_t> charge;
+  bool operator<(const ProgramKey &o) const {
+    return std::tie(kind, spec, structures, a, b, has_charge, charge) <
+           std::tie(o.kind, o.spec, o.structures, o.a, o.b, o.has_charge, o.charge);
+  }
+};
+
+std::map<ProgramKey, ProgramPtr> &program_cache() {
+  static std::map<ProgramKey, ProgramPtr> cache;
+  return cache;
+}
+
+ProgramPtr cached(ProgramKey key, const std::function<ProgramPtr()> &make) {
+  auto &cache = program_cache();
+  auto found = cache.find(key);
+  if (found != cache.end()) return found->second;
+  if (cache.size() > 4096) cache.clear();
+  ProgramPtr p = make();
+  cache.emplace(std::move(key), p);
+  return p;
+}
+
+ProgramPtr build_einsum(const std::string &spec, const std::vector<StructurePtr> &operands, const Sector *outcharge) {
+  if (operands.empty()) throw std::invalid_argument("an einsum needs at least one operand");
+  const size_t arrow = spec.find("->");
+  if (arrow == std::string::npos) throw std::invalid_argument("the equation '" + spec + "' does not say what it leaves");
+  std::vector<Side> groups;
+  const std::string left = spec.substr(0, arrow);
+  size_t start = 0;
+  while (true) {
+    const size_t comma = left.find(',', start);
+    groups.push_back(side_of(left.substr(start, comma == std::string::npos ? std::string::npos : comma - start)));
+    if (comma == std::string::npos) break;
+    start = comma + 1;
+  }
+  const Side output = side_of(spec.substr(arrow + 2));
+  if (groups.size() != operands.size())
+    throw std::invalid_argument("the equation '" + spec + "' names " + std::to_string(groups.size()) +
+                                " operands but got " + std::to_string(operands.size()));
+  const CategoryPtr &catp = operands.front()->cat();
+  const Category &cat = *catp;
+  const Sector charge = outcharge ? *outcharge : cat.trivial();
+  for (size_t at = 0; at < operands.size(); ++at) {
+    if (operands[at]->cat() != catp) throw std::invalid_argument("an einsum needs every operand over one category");
+    if (groups[at].labels.size() != static_cast<size_t>(operands[at]->nlegs()))
+      throw std::invalid_argument("the group " + std::to_string(at) + " of '" + spec + "' names " +
+                                  std::to_string(groups[at].labels.size()) + " legs but the tensor has " +
+                                  std::to_string(operands[at]->nlegs()));
+  }
+  std::vector<char> seen;
+  for (const Side &g : groups)
+    for (char c : g.labels) {
+      if (std::count(seen.begin(), seen.end(), c) >= 2)
+        throw std::invalid_argument(std::string("the label '") + c + "' of '" + spec + "' names more than two legs");
+      seen.push_back(c);
+    }
+  for (char c : seen) {
+    const auto count = std::count(seen.begin(), seen.end(), c);
+    if (count == 1 && !holds(output.labels, c))
+      throw std::invalid_argument(std::string("the label '") + c + "' of '" + spec +
+                                  "' names one leg but is not kept in the output, which would leave it open");
+    if (count == 2 && holds(output.labels, c))
+      throw std::invalid_argument(std::string("the label '") + c + "' of '" + spec + "' is summed over and still kept");
+  }
+  for (char c : output.labels)
+    if (std::count(seen.begin(), seen.end(), c) != 1)
+      throw std::invalid_argument(std::string("the output label '") + c + "' of '" + spec +
+                                  "' does not name a leg of the operands");
+  std::vector<char> summed;
+  for (char c : seen)
+    if (std::count(seen.begin(), seen.end(), c) == 2 && !holds(summed, c)) summed.push_back(c);
+
+  Builder bld;
+  std::vector<std::vector<int32_t>> prepared;
+  std::vector<std::vector<char>> labels;
+  for (size_t at = 0; at < operands.size(); ++at) {
+    const int32_t v = bld.operand(static_cast<int32_t>(at), operands[at]);
+    const int64_t ncod = static_cast<int64_t>(operands[at]->codomain().size());
+    std::vector<int64_t> above, below;
+    for (int64_t leg = 0; leg < operands[at]->nlegs(); ++leg) {
+      const char label = groups[at].labels[static_cast<size_t>(leg)];
+      const bool stays = holds(summed, label) ? leg < ncod : groups[at].above[static_cast<size_t>(leg)];
+      (stays ? above : below).push_back(leg);
+    }
+    std::vector<char> lab;
+    for (int64_t leg : above) lab.push_back(groups[at].labels[static_cast<size_t>(leg)]);
+    for (int64_t leg : below) lab.push_back(groups[at].labels[static_cast<size_t>(leg)]);
+    for (size_t i = 0; i < lab.size(); ++i)
+      for (size_t j = i + 1; j < lab.size(); ++j)
+        if (lab[i] == lab[j])
+          throw std::invalid_argument(std::string("the label '") + lab[i] + "' of '" + spec +
+                                      "' is traced within one operand, which this backend does not run yet");
+    prepared.push_back({bld.move(v, above, below)});
+    labels.push_back(lab);
+  }
+  // The output legs have to close to the charge (einsum.cpp:523-544).
+  Legs above_legs, below_legs;
+  for (size_t one = 0; one < output.labels.size(); ++one)
+    for (size_t at = 0; at < labels.size(); ++at)
+      if (holds(labels[at], output.labels[one])) {
+        const LegPtr leg = bld.structure(prepared[at].front())->legs()[static_cast<size_t>(position_of(labels[at], output.labels[one]))];
+        (output.above[one] ? above_legs : below_legs).push_back(leg);
+        break;
+      }
+  bool closes = output.labels.empty();
+  for (const Sector &a : reachable_charges(cat, above_legs))
+    for (const Sector &b : reachable_charges(cat, below_legs))
+      if (cat.n_symbol(a, cat.dual(b), charge) > 0) closes = true;
+  if (!closes) throw std::invalid_argument("the output legs of '" + spec + "' cannot close to " + render(charge));
+  std::map<char, double> dims;
+  for (size_t at = 0; at < labels.size(); ++at) {
+    const Legs legs = bld.structure(prepared[at].front())->legs();
+    for (size_t leg = 0; leg < labels[at].size(); ++leg) dims[labels[at][leg]] = leg_size(cat, *legs[leg]);
+  }
+  const std::vector<int64_t> order = fold_order(labels, dims);
+  std::vector<char> live = labels[static_cast<size_t>(order[0])];
+  std::vector<int32_t> states = prepared[static_cast<size_t>(order[0])];
+  for (size_t step = 1; step < order.size(); ++step) {
+    const size_t at = static_cast<size_t>(order[step]);
+    std::vector<int32_t> folded = combine(bld, states, live, prepared[at], labels[at]);
+    if (static_cast<int64_t>(folded.size()) > kMaxCharges)
+      throw std::invalid_argument("the products of the equation leave more than " + std::to_string(kMaxCharges) +
+                                  " charges to sum, which is more than this takes");
+    live = product_labels(live, labels[at]);
+    states = std::move(folded);
+  }
+  std::vector<int64_t> above, below;
+  for (size_t at = 0; at < output.labels.size(); ++at)
+    (output.above[at] ? above : below).push_back(position_of(live, output.labels[at]));
+  std::vector<int32_t> results;
+  for (int32_t s : states) {
+    const int32_t placed = bld.move(s, above, below);
+    const StructurePtr &st = bld.structure(placed);
+    if (!(st->qtotal() == charge) || st->blocks().empty()) continue;
+    results.push_back(placed);
+  }
+  if (results.empty())
+    throw std::invalid_argument("no way for the charges of '" + spec + "' to fuse to " + render(charge));
+  bld.p->operands = operands;
+  bld.finish(results);
+  return bld.p;
+}
+
+}  // namespace
+
+GemmPlan::~GemmPlan() {
+  if (device_) tt_ggemm_plan_destroy(device_);
+}
+
+tt_ggemm_plan *GemmPlan::device() {
+  if (!device_)
+    check(tt_ggemm_plan_create(groups.data(), static_cast<int64_t>(groups.size()), terms.data(),
+                               static_cast<int64_t>(terms.size()), &device_),
+          "block product upload");
+  return device_;
+}
+
+ProgramPtr plan_einsum(const std::string &spec, const std::vector<StructurePtr> &operands, const Sector *charge) {
+  ProgramKey key{0, spec, {}, {}, {}, charge != nullptr, charge ? charge->to_vector() : std::vector<int64_t>{}};
+  for (const auto &s : operands) key.structures.push_back(s->id());
+  return cached(std::move(key), [&] { return build_einsum(spec, operands, charge); });
+}
+
+ProgramPtr plan_contract(const StructurePtr &left, const StructurePtr &right, const std::vector<int64_t> &left_legs,
+                         const std::vector<int64_t> &right_legs, const Sector *charge) {
+  ProgramKey key{1, "", {left->id(), right->id()}, left_legs, right_legs, charge != nullptr,
+                 charge ? charge->to_vector() : std::vector<int64_t>{}};
+  return cached(std::move(key), [&] {
+    const Category &cat = *left->cat();
+    Sector q;
+    if (charge) {
+      q = *charge;
+    } else {
+      if (!(right->qtotal() == cat.trivial()))
+        throw std::invalid_argument("the second tensor of a contraction must carry the trivial charge");
+      q = left->qtotal();
+    }
+    check_pairs(*left, *right, left_legs, right_legs);
+    if (cat.n_symbol(left->qtotal(), right->qtotal(), q) == 0)
+      throw std::invalid_argument("the charge " + render(q) + " is not an outcome of " + render(left->qtotal()) + " and " +
+                                  render(right->qtotal()));
+    require_single_closing(cat, left->qtotal(), right->qtotal(), q, "the charges", "a contraction");
+    Builder bld;
+    const int32_t l = bld.operand(0, left), r = bld.operand(1, right);
+    const int32_t above = bld.move(l, without(left->nlegs(), left_legs), left_legs);
+    const int32_t under = bld.move(r, right_legs, without(right->nlegs(), right_legs));
+    const int32_t out = bld.gemm(above, under, plan_gemm(bld.structure(above), bld.structure(under), q));
+    bld.p->operands = {left, right};
+    bld.finish({out});
+    return bld.p;
+  });
+}
+
+ProgramPtr plan_place(const StructurePtr &source, const std::vector<int64_t> &above, const std::vector<int64_t> &below) {
+  ProgramKey key{2, "", {source->id()}, above, below, false, {}};
+  return cached(std::move(key), [&] {
+    Builder bld;
+    const int32_t v = bld.operand(0, source);
+    const int32_t placed = bld.move(v, above, below);
+    bld.p->operands = {source};
+    bld.finish({placed});
+    return bld.p;
+  });
+}
+
+// The conjugate's blocks are the transposed blocks under swapped roots (tensor.cpp:105-113).
+ProgramPtr plan_conj(const StructurePtr &source) {
+  ProgramKey key{3, "", {source->id()}, {}, {}, false, {}};
+  return cached(std::move(key), [&] {
+    const Category &cat = *source->cat();
+    auto move = std::make_shared<MovePlan>();
+    move->source = source;
+    move->target = Structure::make(source->cat(), source->domain(), source->codomain(), cat.dual(source->qtotal()));
+    for (const BlockInfo &b : source->blocks()) {
+      const int64_t tb = move->target->find(b.key.domain, b.key.codomain, b.key.copy);
+      if (tb < 0) throw std::logic_error("the conjugate has no block for a key of the tensor");
+      const BlockInfo &t = move->target->blocks()[static_cast<size_t>(tb)];
+      if (t.rows != b.cols || t.cols != b.rows) throw std::logic_error("the conjugate's block does not transpose");
+      if (b.rows == 0 || b.cols == 0) continue;
+      tt_gss_task task{};
+      task.dst_offset = t.offset;
+      task.ndim = 2;
+      task.first_source = static_cast<int32_t>(move->sources.size());
+      task.num_sources = 1;
+      task.extent[0] = t.rows;
+      task.extent[1] = t.cols;
+      task.dst_stride[0] = t.cols;
+      task.dst_stride[1] = 1;
+      move->tasks.push_back(task);
+      tt_gss_source s{};
+      s.src_offset = b.offset;
+      s.src_stride[0] = 1;
+      s.src_stride[1] = b.cols;
+      s.coef_re = 1.0;
+      move->sources.push_back(s);
+    }
+    Builder bld;
+    const int32_t v = bld.operand(0, source);
+    const int32_t d = bld.fresh(move->target, 1u);
+    bld.p->ops.push_back(Op{OpKind::move, move, nullptr, v, -1, d, 1u});
+    bld.p->move_elements += move->elements();
+    bld.p->operands = {source};
+    bld.finish({d});
+    return bld.p;
+  });
+}
+
+uint32_t Program::hoistable() const {
+  uint32_t mask = 0;
+  for (const Op &op : ops)
+    if (op.kind == OpKind::move && op.inputs && (op.inputs & (op.inputs - 1)) == 0 &&
+        slots[static_cast<size_t>(op.dst)].kind == Slot::workspace)
+      mask |= op.inputs;
+  return mask;
+}
+
+void Program::run(int dtype, const std::vector<const void *> &operand_ptrs, void *out_ptr, void *workspace_ptr,
+                  void *stream, Mode mode, uint32_t skip) const {
+  if (operand_ptrs.size() != operands.size()) throw std::invalid_argument("a program got the wrong number of operands");
+  const int64_t es = element_size(dtype);
+  char *ws = static_cast<char *>(workspace_ptr);
+  auto ptr = [&](int32_t v) -> void * {
+    const Slot &s = slots[static_cast<size_t>(v)];
+    switch (s.kind) {
+      case Slot::operand: return const_cast<void *>(operand_ptrs[static_cast<size_t>(s.index)]);
+      case Slot::output: return out_ptr;
+      default: return ws + s.offset * es;
+    }
+  };
+  if (workspace > 0 && !workspace_ptr) throw std::invalid_argument("a program needs its workspace");
+  for (const Op &op : ops) {
+    const bool hoisted = op.kind == OpKind::move && op.inputs && (op.inputs & (op.inputs - 1)) == 0 &&
+                         (op.inputs & skip) && slots[static_cast<size_t>(op.dst)].kind == Slot::workspace;
+    if (hoisted && mode == Mode::batched) continue;
+    switch (op.kind) {
+      case OpKind::move:
+        if (mode == Mode::plain && op.move->chained) {
+          // Per-step form: intermediates ping-pong through the extra workspace after the main one.
+          char *extra = ws + workspace * es;
+          const MovePlanPtr plain = plan_move_plain(op.move->source, op.move->above, op.move->below);
+          const auto &steps = plain->elementary;
+          const void *from = ptr(op.a);
+          const int64_t half = plain_workspace / 2;
+          for (size_t k = 0; k < steps.size(); ++k) {
+            void *to = k + 1 == steps.size() ? ptr(op.dst) : extra + static_cast<int64_t>(k % 2) * half * es;
+            check(tt_gss_execute_plain(steps[k]->device(), dtype, from, to, stream), "leg move");
+            from = to;
+          }
+        } else {
+          check(tt_gss_execute(op.move->device(), dtype, ptr(op.a), ptr(op.dst), stream), "leg move");
+        }
+        break;
+      case OpKind::gemm:
+        if (mode == Mode::plain)
+          check(tt_ggemm_execute_plain(op.gemm->device(), dtype, ptr(op.a), ptr(op.b), ptr(op.dst), stream), "block products");
+        else
+          check(tt_ggemm_execute(op.gemm->device(), dtype, dtype, dtype, dtype, ptr(op.a), ptr(op.b), ptr(op.dst), stream),
+                "block products");
+        break;
+      case OpKind::copy:
+        if (dtype == TT_C128) {
+          check(tt_axpby(TT_F64, 2 * out->size(), 1.0, nullptr, ptr(op.a), 0.0, nullptr, ptr(op.dst), stream), "copy");
+        } else {
+          check(tt_axpby(dtype, out->size(), 1.0, nullptr, ptr(op.a), 0.0, nullptr, ptr(op.dst), stream), "copy");
+        }
+        break;
+      case OpKind::add:
+        if (dtype == TT_C128) {
+          check(tt_axpby(TT_F64, 2 * out->size(), 1.0, nullptr, ptr(op.a), 1.0, nullptr, ptr(op.dst), stream), "sum");
+        } else {
+          check(tt_axpby(dtype, out->size(), 1.0, nullptr, ptr(op.a), 1.0, nullptr, ptr(op.dst), stream), "sum");
+        }
+        break;
+    }
+  }
+}
+
+void clear_program_cache() { program_cache().clear(); }
+
+}  // namespace tt

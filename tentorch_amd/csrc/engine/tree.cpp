@@ -1,0 +1,324 @@
+#include "tree.h"
+
+#include <algorithm>
+#include <functional>
+#include <map>
+#include <numeric>
+#include <set>
+
+namespace tt {
+
+namespace {
+
+void mix(size_t &seed, size_t value) { seed ^= value + 0x9e3779b97f4a7c15ULL + (seed << 6) + (seed >> 2); }
+
+struct LegKey {
+  const Category *cat;
+  std::vector<int64_t> data;
+  bool operator<(const LegKey &o) const {
+    if (cat != o.cat) return cat < o.cat;
+    return data < o.data;
+  }
+};
+
+std::map<LegKey, LegPtr> &leg_table() {
+  static std::map<LegKey, LegPtr> table;
+  return table;
+}
+
+struct LayoutKey {
+  const Category *cat;
+  std::vector<const Leg *> legs;
+  Sector root;
+  bool operator<(const LayoutKey &o) const {
+    if (cat != o.cat) return cat < o.cat;
+    if (legs != o.legs) return legs < o.legs;
+    return root < o.root;
+  }
+};
+
+std::map<LayoutKey, LayoutPtr> &layout_table() {
+  static std::map<LayoutKey, LayoutPtr> table;
+  return table;
+}
+
+struct StructureKey {
+  const Category *cat;
+  std::vector<const Leg *> codomain, domain;
+  Sector q;
+  bool operator<(const StructureKey &o) const {
+    if (cat != o.cat) return cat < o.cat;
+    if (codomain != o.codomain) return codomain < o.codomain;
+    if (domain != o.domain) return domain < o.domain;
+    return q < o.q;
+  }
+};
+
+std::map<StructureKey, StructurePtr> &structure_table() {
+  static std::map<StructureKey, StructurePtr> table;
+  return table;
+}
+
+uint64_t next_structure_id() {
+  static uint64_t id = 0;
+  return ++id;
+}
+
+}  // namespace
+
+// Sorted, unique, positive multiplicities (tree.cpp:90-140).
+LegPtr Leg::make(CategoryPtr cat, std::vector<Sector> sectors, std::vector<int64_t> mults) {
+  if (!cat) throw std::invalid_argument("A leg needs a category.");
+  if (sectors.size() != mults.size())
+    throw std::invalid_argument("A leg needs one multiplicity per sector, got " + std::to_string(mults.size()) +
+                                " for " + std::to_string(sectors.size()) + " sectors.");
+  for (int64_t m : mults)
+    if (m <= 0) throw std::invalid_argument("A block multiplicity must be positive, got " + std::to_string(m) + ".");
+  std::vector<size_t> order(sectors.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return sectors[a] < sectors[b]; });
+  std::vector<Sector> ss;
+  std::vector<int64_t> ms;
+  for (size_t i : order) {
+    ss.push_back(sectors[i]);
+    ms.push_back(mults[i]);
+  }
+  for (size_t i = 1; i < ss.size(); ++i)
+    if (ss[i] == ss[i - 1]) throw std::invalid_argument("A leg cannot hold the sector " + render(ss[i]) + " twice.");
+  for (const Sector &s : ss)
+    if (cat->q_dim(s) <= 0.0)
+      throw std::invalid_argument("The sector " + render(s) + " has no components in " + cat->name() + ".");
+  LegKey key{cat.get(), {}};
+  size_t hash = std::hash<std::string>{}(cat->name());
+  for (size_t i = 0; i < ss.size(); ++i) {
+    key.data.push_back(ss[i].n);
+    for (int j = 0; j < ss[i].n; ++j) {
+      key.data.push_back(ss[i].v[j]);
+      mix(hash, std::hash<int64_t>{}(ss[i].v[j]));
+    }
+    key.data.push_back(ms[i]);
+    mix(hash, std::hash<int64_t>{}(ms[i]));
+  }
+  auto &table = leg_table();
+  auto found = table.find(key);
+  if (found != table.end()) return found->second;
+  LegPtr leg = std::make_shared<Leg>(cat, std::move(ss), std::move(ms), hash);
+  table.emplace(std::move(key), leg);
+  return leg;
+}
+
+int64_t Leg::multiplets() const { return std::accumulate(mults_.begin(), mults_.end(), int64_t{0}); }
+
+int64_t Leg::find(const Sector &s) const {
+  auto it = std::lower_bound(sectors_.begin(), sectors_.end(), s);
+  if (it == sectors_.end() || *it != s) return -1;
+  return it - sectors_.begin();
+}
+
+int64_t Leg::block_of(const Sector &s) const {
+  const int64_t at = find(s);
+  if (at < 0)
+    throw std::invalid_argument("a leg of " + std::to_string(nblocks()) + " blocks carries none in sector " + render(s));
+  return at;
+}
+
+LegPtr Leg::fuse(const Leg &other) const {
+  if (cat_ != other.cat_)
+    throw std::invalid_argument("Cannot fuse a leg of " + cat_->name() + " with one of " + other.cat_->name() + ".");
+  std::map<Sector, int64_t> fused;
+  for (int64_t a = 0; a < nblocks(); ++a)
+    for (int64_t b = 0; b < other.nblocks(); ++b)
+      for (const Outcome &o : cat_->outcomes(sectors_[a], other.sectors_[b]))
+        fused[o.sector] += mults_[a] * other.mults_[b] * o.multiplicity;
+  std::vector<Sector> ss;
+  std::vector<int64_t> ms;
+  for (const auto &[s, m] : fused) {
+    ss.push_back(s);
+    ms.push_back(m);
+  }
+  return make(cat_, ss, ms);
+}
+
+LegPtr Leg::conj() const {
+  std::vector<Sector> ss;
+  for (const Sector &s : sectors_) ss.push_back(cat_->dual(s));
+  return make(cat_, ss, mults_);
+}
+
+LegPtr Leg::capped(int64_t max_multiplets) const {
+  if (max_multiplets <= 0)
+    throw std::invalid_argument("A cap must keep at least one multiplet, got " + std::to_string(max_multiplets) + ".");
+  std::vector<Sector> ss;
+  std::vector<int64_t> ms;
+  int64_t kept = 0;
+  for (int64_t b = 0; b < nblocks() && kept < max_multiplets; ++b) {
+    const int64_t take = std::min(mults_[b], max_multiplets - kept);
+    ss.push_back(sectors_[b]);
+    ms.push_back(take);
+    kept += take;
+  }
+  return make(cat_, ss, ms);
+}
+
+bool FusionTree::operator<(const FusionTree &o) const {
+  if (sectors != o.sectors) return sectors < o.sectors;
+  if (inner != o.inner) return inner < o.inner;
+  if (copies != o.copies) return copies < o.copies;
+  return coupled < o.coupled;
+}
+
+size_t TreeHash::operator()(const FusionTree &t) const {
+  size_t seed = 0;
+  for (const Sector &s : t.sectors) mix(seed, SectorHash{}(s));
+  for (const Sector &s : t.inner) mix(seed, SectorHash{}(s) * 31u);
+  for (int64_t c : t.copies) mix(seed, std::hash<int64_t>{}(c));
+  mix(seed, SectorHash{}(t.coupled));
+  return seed;
+}
+
+// Insertion order of first appearance (tree.cpp:375-398).
+std::vector<Sector> reachable_charges(const Category &cat, const Legs &legs) {
+  std::vector<Sector> charges{cat.trivial()};
+  for (const LegPtr &leg : legs) {
+    std::vector<Sector> next;
+    std::set<Sector> seen;
+    for (const Sector &charge : charges)
+      for (const Sector &s : leg->sectors())
+        for (const Outcome &o : cat.outcomes(charge, s))
+          if (seen.insert(o.sector).second) next.push_back(o.sector);
+    charges = std::move(next);
+  }
+  return charges;
+}
+
+// Depth first over leg blocks in sector order and outcomes in category order (tree.cpp:400-471).
+std::vector<FusionTree> fusion_trees(const Category &cat, const Legs &legs, const Sector &coupled) {
+  const int64_t count = static_cast<int64_t>(legs.size());
+  if (count == 0) {
+    if (coupled == cat.trivial()) return {FusionTree{{}, {}, {}, coupled}};
+    return {};
+  }
+  std::vector<FusionTree> out;
+  FusionTree tree;
+  tree.sectors.resize(static_cast<size_t>(count));
+  tree.inner.resize(static_cast<size_t>(count >= 2 ? count - 2 : 0));
+  tree.copies.resize(static_cast<size_t>(count >= 2 ? count - 1 : 0));
+  tree.coupled = coupled;
+  std::function<void(int64_t, const Sector &)> walk = [&](int64_t leg, const Sector &previous) {
+    if (leg == 0) {
+      for (const Sector &s : legs[0]->sectors()) {
+        tree.sectors[0] = s;
+        if (count == 1) {
+          if (s == coupled) out.push_back(tree);
+        } else {
+          walk(1, s);
+        }
+      }
+      return;
+    }
+    for (const Sector &s : legs[static_cast<size_t>(leg)]->sectors()) {
+      tree.sectors[static_cast<size_t>(leg)] = s;
+      for (const Outcome &o : cat.outcomes(previous, s)) {
+        if (leg + 1 == count && o.sector != coupled) continue;
+        if (leg + 1 < count) tree.inner[static_cast<size_t>(leg - 1)] = o.sector;
+        for (int64_t copy = 0; copy < o.multiplicity; ++copy) {
+          tree.copies[static_cast<size_t>(leg - 1)] = copy;
+          if (leg + 1 == count)
+            out.push_back(tree);
+          else
+            walk(leg + 1, o.sector);
+        }
+      }
+    }
+  };
+  walk(0, cat.trivial());
+  return out;
+}
+
+FusionTree trimmed(const Category &cat, const FusionTree &tree) {
+  FusionTree out = tree;
+  out.sectors.pop_back();
+  if (!out.inner.empty()) out.inner.pop_back();
+  if (!out.copies.empty()) out.copies.pop_back();
+  const int64_t n = tree.size();
+  out.coupled = n <= 1 ? cat.trivial() : n == 2 ? tree.sectors.at(0) : tree.inner.at(static_cast<size_t>(n - 3));
+  return out;
+}
+
+LayoutPtr layout_of(const CategoryPtr &cat, const Legs &legs, const Sector &root) {
+  LayoutKey key{cat.get(), {}, root};
+  for (const LegPtr &l : legs) key.legs.push_back(l.get());
+  auto &table = layout_table();
+  auto found = table.find(key);
+  if (found != table.end()) return found->second;
+  auto layout = std::make_shared<SideLayout>();
+  layout->trees = fusion_trees(*cat, legs, root);
+  layout->offsets.push_back(0);
+  for (size_t t = 0; t < layout->trees.size(); ++t) {
+    int64_t rows = 1;
+    for (size_t axis = 0; axis < legs.size(); ++axis) rows *= legs[axis]->mult_of(layout->trees[t].sectors[axis]);
+    layout->offsets.push_back(layout->offsets.back() + rows);
+    layout->index.emplace(layout->trees[t], static_cast<int64_t>(t));
+  }
+  table.emplace(std::move(key), layout);
+  return layout;
+}
+
+std::vector<BlockKey> block_keys(const Category &cat, const Legs &codomain, const Legs &domain, const Sector &q) {
+  std::vector<BlockKey> keys;
+  for (const Sector &r : reachable_charges(cat, codomain))
+    for (const Sector &s : reachable_charges(cat, domain))
+      for (int64_t copy = 0; copy < cat.n_symbol(r, cat.dual(s), q); ++copy) keys.push_back(BlockKey{r, s, copy});
+  return keys;
+}
+
+Structure::Structure(CategoryPtr cat, Legs codomain, Legs domain, Sector qtotal, uint64_t id)
+    : cat_(std::move(cat)), codomain_(std::move(codomain)), domain_(std::move(domain)), qtotal_(qtotal), id_(id) {
+  std::vector<BlockKey> keys = block_keys(*cat_, codomain_, domain_, qtotal_);
+  std::sort(keys.begin(), keys.end());
+  int64_t offset = 0;
+  for (const BlockKey &k : keys) {
+    LayoutPtr rl = layout_of(cat_, codomain_, k.codomain);
+    LayoutPtr cl = layout_of(cat_, domain_, k.domain);
+    BlockInfo info{k, rl->rows(), cl->rows(), offset, rl, cl};
+    index_.emplace(k, static_cast<int64_t>(blocks_.size()));
+    blocks_.push_back(info);
+    offset += info.rows * info.cols;
+  }
+  size_ = offset;
+}
+
+StructurePtr Structure::make(CategoryPtr cat, Legs codomain, Legs domain, Sector qtotal) {
+  if (!cat) throw std::invalid_argument("a tensor needs a category");
+  for (const LegPtr &l : codomain)
+    if (l->cat() != cat) throw std::invalid_argument("a codomain leg is over a different category");
+  for (const LegPtr &l : domain)
+    if (l->cat() != cat) throw std::invalid_argument("a domain leg is over a different category");
+  StructureKey key{cat.get(), {}, {}, qtotal};
+  for (const LegPtr &l : codomain) key.codomain.push_back(l.get());
+  for (const LegPtr &l : domain) key.domain.push_back(l.get());
+  auto &table = structure_table();
+  auto found = table.find(key);
+  if (found != table.end()) return found->second;
+  auto s = std::make_shared<Structure>(cat, std::move(codomain), std::move(domain), qtotal, next_structure_id());
+  table.emplace(std::move(key), s);
+  return s;
+}
+
+Legs Structure::legs() const {
+  Legs out = codomain_;
+  out.insert(out.end(), domain_.begin(), domain_.end());
+  return out;
+}
+
+int64_t Structure::find(const BlockKey &k) const {
+  auto f = index_.find(k);
+  return f == index_.end() ? -1 : f->second;
+}
+
+void clear_engine_caches() {
+  structure_table().clear();
+  layout_table().clear();
+}
+
+}  // namespace tt

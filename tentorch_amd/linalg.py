@@ -1,0 +1,216 @@
+'''
+The splittings and the Krylov eigensolver of the MI355X backend.
+
+`svd`/`svdvals`/`qr`/`rq` restate `tentorch/linalg/graded/svd.cpp:41-257`: one factorisation per
+block (on the device, through torch's linear algebra), a global ranking of the singular values on
+the host with the tolerance, the cap and the sector/dual re-pairing, then the factors packed along a
+new bond leg that carries the domain root of each block.  Ties in the ranking are broken by (block,
+index) order, a deterministic choice where the reference's `std::sort` leaves it open
+(`svd.cpp:79-80`).
+
+`lanczos` restates `tentorch/linalg/lanczos.py:8-64`; the scalar products run on the device
+(`tt_dot`) and the Krylov vectors are updated with `tt_axpby`.
+'''
+
+from typing import Callable, List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _native
+from ._native import engine
+from .tensor import Tensor, inner, inner_device, norm
+
+
+def _arranged(tensor: Tensor, left: Optional[Sequence[int]]) -> Tensor:
+    return tensor.place(list(left)) if left is not None else tensor
+
+
+def _bond_leg(cat, mults: dict):
+    if not mults:
+        mults = {cat.trivial(): 1}
+    keys = sorted(mults)
+    return engine.Leg(cat, keys, [mults[k] for k in keys])
+
+
+def svd(tensor: Tensor, left: Optional[Sequence[int]] = None, max_bond: Optional[int] = None, tol: float = 0.0,
+        absorb: int = 0) -> Tuple[Tensor, torch.Tensor, Tensor]:
+    '''`split_over` (`svd.cpp:41-177`).  Returns (U, values, V) with U over (codomain | bond) and V over
+    (bond | domain).'''
+    if absorb not in (0, 1, 2, 3):
+        raise ValueError(f'absorb has to be 0, 1, 2 or 3, got {absorb}')
+    if max_bond is not None and max_bond < 1:
+        raise ValueError(f'a bond needs at least one multiplet, got {max_bond}')
+    arranged = _arranged(tensor, left)
+    cat = arranged.cat
+    factors = []
+    for key, block in arranged.blocks().items():
+        if block.numel() == 0:
+            continue
+        u, v, vh = torch.linalg.svd(block, full_matrices=False)
+        factors.append({'key': key, 'u': u, 'v': v, 'vh': vh, 'kept': 0, 'offset': 0})
+    host = torch.cat([f['v'] for f in factors]).double().cpu().numpy() if factors else np.zeros(0)
+    candidates = []
+    at = 0
+    for fi, f in enumerate(factors):
+        n = f['v'].numel()
+        for index in range(n):
+            candidates.append((float(host[at + index]), fi, index))
+        at += n
+    candidates.sort(key=lambda c: -c[0])
+    cut = len(candidates)
+    for i, c in enumerate(candidates):
+        if c[0] <= tol:
+            cut = i
+            break
+    candidates = candidates[:cut]
+    if max_bond is not None and len(candidates) > max_bond:
+        candidates = candidates[:max_bond]
+    for value, fi, index in candidates:
+        factors[fi]['kept'] = max(factors[fi]['kept'], index + 1)
+    mults = {}
+    for f in factors:
+        if f['kept']:
+            s = f['key'].domain
+            mults[s] = mults.get(s, 0) + f['kept']
+    paired = {}
+    for sector, count in mults.items():
+        other = mults.get(cat.dual(sector))
+        if other is not None:
+            paired[sector] = min(count, other)
+    if paired and paired != mults:
+        for f in factors:
+            f['kept'] = 0
+        used = {}
+        for value, fi, index in candidates:
+            f = factors[fi]
+            s = f['key'].domain
+            room = paired.get(s)
+            if room is None or used.get(s, 0) >= room:
+                continue
+            used[s] = used.get(s, 0) + 1
+            f['kept'] = max(f['kept'], index + 1)
+        mults = paired
+    nxt = {}
+    for f in factors:
+        s = f['key'].domain
+        f['offset'] = nxt.get(s, 0)
+        nxt[s] = f['offset'] + f['kept']
+    placed, run = {}, 0
+    for sector in sorted(mults):
+        placed[sector] = run
+        run += mults[sector]
+    bond = _bond_leg(cat, mults)
+    first = Tensor(cat, arranged.codomain, [bond], arranged.qtotal, arranged.dtype, arranged.device)
+    second = Tensor(cat, [bond], arranged.domain, cat.trivial(), arranged.dtype, arranged.device)
+    values = torch.zeros(bond.multiplets(), dtype=arranged.dtype, device=arranged.device)
+    for f in factors:
+        k = f['kept']
+        if not k:
+            continue
+        r, s, c = f['key']
+        u, vh, v = f['u'][:, :k], f['vh'][:k], f['v'][:k]
+        if absorb == 3:
+            u = u * v.sqrt()
+            vh = v.sqrt().unsqueeze(1) * vh
+        elif absorb == 1:
+            u = u * v
+        elif absorb == 2:
+            vh = v.unsqueeze(1) * vh
+        o = f['offset']
+        first.block(r, s, c)[:, o:o + k] = u
+        second.block(s, s)[o:o + k, :] = vh
+        values[placed[s] + o:placed[s] + o + k] = v.to(arranged.dtype)
+    return first, values, second
+
+
+def svdvals(tensor: Tensor, left: Optional[Sequence[int]] = None) -> torch.Tensor:
+    return svd(tensor, left)[1]
+
+
+def _split_blocks(tensor: Tensor, pieces) -> Tuple[Tensor, Tensor]:
+    '''`split_blocks` (`svd.cpp:196-223`).'''
+    cat = tensor.cat
+    mults, nxt, placed = {}, {}, []
+    for key, lft, rgt, kept in pieces:
+        if not kept:
+            continue
+        s = key.domain
+        offset = nxt.get(s, 0)
+        nxt[s] = offset + kept
+        mults[s] = mults.get(s, 0) + kept
+        placed.append((key, lft, rgt, kept, offset))
+    bond = _bond_leg(cat, mults)
+    first = Tensor(cat, tensor.codomain, [bond], tensor.qtotal, tensor.dtype, tensor.device)
+    second = Tensor(cat, [bond], tensor.domain, cat.trivial(), tensor.dtype, tensor.device)
+    for (r, s, c), lft, rgt, kept, offset in placed:
+        first.block(r, s, c)[:, offset:offset + kept] = lft
+        second.block(s, s)[offset:offset + kept, :] = rgt
+    return first, second
+
+
+def qr(tensor: Tensor, left: Optional[Sequence[int]] = None) -> Tuple[Tensor, Tensor]:
+    '''`qr_over` (`svd.cpp:225-236`).'''
+    arranged = _arranged(tensor, left)
+    pieces = []
+    for key, block in arranged.blocks().items():
+        if block.numel() == 0:
+            continue
+        q, r = torch.linalg.qr(block, mode='reduced')
+        pieces.append((key, q, r, q.shape[1]))
+    return _split_blocks(arranged, pieces)
+
+
+def rq(tensor: Tensor, left: Optional[Sequence[int]] = None) -> Tuple[Tensor, Tensor]:
+    '''`rq_over` (`svd.cpp:238-249`).'''
+    arranged = _arranged(tensor, left)
+    pieces = []
+    for key, block in arranged.blocks().items():
+        if block.numel() == 0:
+            continue
+        q, r = torch.linalg.qr(block.transpose(0, 1), mode='reduced')
+        pieces.append((key, r.transpose(0, 1), q.transpose(0, 1), q.shape[1]))
+    return _split_blocks(arranged, pieces)
+
+
+def lanczos(action: Callable[[Tensor], Tensor], start: Tensor, n_iter: int = 5,
+            tol: float = 1e-12) -> Tuple[float, Tensor]:
+    '''
+    `lanczos` (`tentorch/linalg/lanczos.py:8-64`): n_iter Krylov steps with two full
+    reorthogonalisation passes, the tridiagonal solved in FP64 on the host, the Ritz vector normalised.
+    '''
+    if n_iter < 1:
+        raise ValueError(f'Lanczos needs at least one step, not {n_iter}.')
+    scale = norm(start)
+    if scale == 0.0:
+        raise ValueError('Lanczos needs a starting vector of non-zero norm.')
+    basis = [start * (1.0 / scale)]
+    alphas: List[float] = []
+    betas: List[float] = []
+    for step in range(n_iter):
+        image = action(basis[-1])
+        alphas.append(inner(basis[-1], image).real)
+        image -= basis[-1] * alphas[-1]
+        if step > 0:
+            image -= basis[-2] * betas[-1]
+        for _ in range(2):
+            for vector in basis:
+                image -= vector * inner(vector, image)
+        beta = norm(image)
+        if beta < tol:
+            break
+        betas.append(beta)
+        basis.append(image * (1.0 / beta))
+    size = len(alphas)
+    matrix = np.zeros((size, size))
+    for step in range(size):
+        matrix[step, step] = alphas[step]
+    for step, beta in enumerate(betas[:size - 1]):
+        matrix[step, step + 1] = beta
+        matrix[step + 1, step] = beta
+    values, vectors = np.linalg.eigh(matrix)
+    coefficients = vectors[:, 0]
+    vector = basis[0] * float(coefficients[0])
+    for step in range(1, size):
+        vector += basis[step] * float(coefficients[step])
+    return float(values[0]), vector * (1.0 / norm(vector))
