@@ -434,6 +434,15 @@ void compose(const StructurePtr &source, const std::vector<Step> &steps, const s
           task.extent[d] = ext[d];
           task.dst_stride[d] = dstr[d];
         }
+        // Every address the tile touches stays inside its buffer (checked here, on the host, so a
+        // planning mistake raises instead of reaching the device).
+        auto span = [&](int64_t base, const std::vector<int64_t> &strides, int64_t size, const char *what) {
+          int64_t hi = base;
+          for (size_t d = 0; d < ext.size(); ++d) hi += (ext[d] - 1) * strides[d];
+          if (base < 0 || hi >= size) throw std::logic_error(std::string("a leg move tile leaves its ") + what + " buffer");
+        };
+        span(doff, dstr, target->size(), "target");
+        for (size_t s = 0; s < list.size(); ++s) span(soff[s], src[s], source->size(), "source");
         plan.tasks.push_back(task);
         for (size_t s = 0; s < list.size(); ++s) {
           tt_gss_source source_box{};

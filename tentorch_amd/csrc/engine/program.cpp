@@ -94,6 +94,8 @@ GemmPlanPtr plan_gemm(const StructurePtr &above, const StructurePtr &under, cons
       if (ob < 0)
         throw std::out_of_range("the tensor has no block at " + render(lb.key.codomain) + " x " + render(rb.key.domain) +
                                 " of copy 0");
+      const BlockInfo &obk = g->out->blocks()[static_cast<size_t>(ob)];
+      if (obk.rows != lb.rows || obk.cols != rb.cols) throw std::logic_error("a block product does not fit its result block");
       per_block[static_cast<size_t>(ob)].push_back(tt_gemm_term{lb.offset, rb.offset, lb.cols});
       g->flops += 2.0 * static_cast<double>(lb.rows) * static_cast<double>(rb.cols) * static_cast<double>(lb.cols);
     }

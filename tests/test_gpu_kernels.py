@@ -1,0 +1,148 @@
+'''
+The device library through its C ABI (include/tentorch_amd.h), against numpy: the grouped block
+GEMM (FP64 MFMA within 1e-12 relative, FP32 MFMA within 2e-5 relative to the operand scale), the
+gather-scale-scatter tables the host engine plans (bit-exact against the oracle's leg moves), and the
+fixed-order scalar product.
+'''
+
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import cabi
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def lib():
+    return cabi.load()
+
+
+def _stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _random_gemm(rng, shapes, multi_term=False):
+    '''Flat A, B buffers and the tables of one grouped GEMM over the given (m, n, k) blocks.'''
+    groups, terms = [], []
+    a_parts, b_parts = [], []
+    a_at = b_at = c_at = 0
+    for m, n, k in shapes:
+        first = len(terms)
+        ks = [k, max(1, k // 3)] if multi_term and k else ([k] if k else [])
+        for kk in ks:
+            a_parts.append(rng.standard_normal(m * kk))
+            b_parts.append(rng.standard_normal(kk * n))
+            terms.append((a_at, b_at, kk))
+            a_at += m * kk
+            b_at += kk * n
+        groups.append((c_at, m, n, first, len(terms) - first))
+        c_at += m * n
+    a = np.concatenate(a_parts) if a_parts else np.zeros(1)
+    b = np.concatenate(b_parts) if b_parts else np.zeros(1)
+    return a, b, c_at, groups, terms
+
+
+def _reference_gemm(a, b, c_size, groups, terms):
+    c = np.zeros(c_size)
+    for c_off, m, n, first, count in groups:
+        acc = np.zeros((m, n))
+        for t in range(first, first + count):
+            ao, bo, k = terms[t]
+            acc += a[ao:ao + m * k].reshape(m, k) @ b[bo:bo + k * n].reshape(k, n)
+        c[c_off:c_off + m * n] = acc.reshape(-1)
+    return c
+
+
+SHAPES = [(1, 1, 1), (6, 1, 1), (64, 64, 16), (65, 63, 17), (130, 7, 200), (3, 129, 5), (200, 190, 70), (17, 33, 0)]
+
+
+@pytest.mark.parametrize('multi_term', [False, True])
+@pytest.mark.parametrize('precision', ['f64', 'f32'])
+def test_grouped_gemm_matches_numpy(lib, multi_term, precision):
+    rng = np.random.default_rng(3)
+    a, b, c_size, groups, terms = _random_gemm(rng, SHAPES, multi_term)
+    ref = _reference_gemm(a, b, c_size, groups, terms)
+    g, t = cabi.gemm_tables(groups, terms)
+    plan = ctypes.c_void_p()
+    assert lib.tt_ggemm_plan_create(g, len(groups), t, len(terms), ctypes.byref(plan)) == 0, cabi.error(lib)
+    dtype = torch.float64 if precision == 'f64' else torch.float32
+    code = cabi.TT_F64 if precision == 'f64' else cabi.TT_F32
+    da, db = torch.tensor(a, dtype=dtype, device='cuda'), torch.tensor(b, dtype=dtype, device='cuda')
+    dc = torch.full((c_size,), float('nan'), dtype=dtype, device='cuda')
+    assert lib.tt_ggemm_execute(plan, code, code, code, code, da.data_ptr(), db.data_ptr(), dc.data_ptr(), _stream()) == 0, cabi.error(lib)
+    got = dc.double().cpu().numpy()
+    scale = np.abs(ref).max()
+    tol = 1e-12 if precision == 'f64' else 2e-5
+    assert np.all(np.isfinite(got))
+    assert np.abs(got - ref).max() <= tol * scale
+    # the per-block form gives the same blocks
+    dc2 = torch.full((c_size,), float('nan'), dtype=dtype, device='cuda')
+    assert lib.tt_ggemm_execute_plain(plan, code, da.data_ptr(), db.data_ptr(), dc2.data_ptr(), _stream()) == 0, cabi.error(lib)
+    assert np.abs(dc2.double().cpu().numpy() - ref).max() <= tol * scale
+    lib.tt_ggemm_plan_destroy(plan)
+
+
+def test_mixed_precision_gemm_reads_fp64_writes_fp64(lib):
+    rng = np.random.default_rng(4)
+    a, b, c_size, groups, terms = _random_gemm(rng, [(100, 80, 60), (33, 5, 7)])
+    ref = _reference_gemm(a, b, c_size, groups, terms)
+    g, t = cabi.gemm_tables(groups, terms)
+    plan = ctypes.c_void_p()
+    assert lib.tt_ggemm_plan_create(g, len(groups), t, len(terms), ctypes.byref(plan)) == 0
+    da = torch.tensor(a, dtype=torch.float64, device='cuda')
+    db = torch.tensor(b, dtype=torch.float64, device='cuda')
+    dc = torch.zeros(c_size, dtype=torch.float64, device='cuda')
+    assert lib.tt_ggemm_execute(plan, cabi.TT_F64, cabi.TT_F64, cabi.TT_F64, cabi.TT_F32,
+                                da.data_ptr(), db.data_ptr(), dc.data_ptr(), _stream()) == 0, cabi.error(lib)
+    assert np.abs(dc.cpu().numpy() - ref).max() <= 2e-5 * np.abs(ref).max()
+    lib.tt_ggemm_plan_destroy(plan)
+
+
+def test_dot_is_fixed_order_and_accurate(lib):
+    rng = np.random.default_rng(5)
+    for n in (0, 1, 1000, 1 << 20):
+        x, y = rng.standard_normal(n), rng.standard_normal(n)
+        dx, dy = torch.tensor(x, device='cuda'), torch.tensor(y, device='cuda')
+        out = torch.zeros(2, dtype=torch.float64, device='cuda')
+        assert lib.tt_dot(cabi.TT_F64, n, dx.data_ptr(), dy.data_ptr(), out.data_ptr(), _stream()) == 0
+        first = out[0].item()
+        assert lib.tt_dot(cabi.TT_F64, n, dx.data_ptr(), dy.data_ptr(), out.data_ptr(), _stream()) == 0
+        assert out[0].item() == first
+        exact = float(np.dot(x, y))
+        assert abs(first - exact) <= 1e-13 * max(1.0, np.abs(x).dot(np.abs(y)))
+
+
+def test_leg_move_tables_are_bit_exact_against_the_oracle(lib):
+    '''Every placement of a 4-leg U(1) tensor: the engine's table run on the GPU equals the oracle's
+    per-entry chain walk bit for bit.'''
+    import itertools
+    from tentorch_amd._native import engine
+    from tentorch_amd import u1, Leg
+    import oracle.symmetry as osym
+    import oracle.tensor as ot
+    rng = np.random.default_rng(0)
+    bond = ([(-2,), (0,), (2,)], [2, 3, 2])
+    phys = ([(-1,), (1,)], [1, 1])
+    o = ot.random_tensor(osym.u1, [osym.Leg(osym.u1, *bond), osym.Leg(osym.u1, *phys)],
+                         [osym.Leg(osym.u1, *phys), osym.Leg(osym.u1, *bond)], (0,), rng)
+    s = engine.Structure(u1, [Leg(u1, *bond), Leg(u1, *phys)], [Leg(u1, *phys), Leg(u1, *bond)], (0,))
+    src = torch.tensor(o.flat(), device='cuda')
+    for n_above in range(5):
+        for perm in itertools.permutations(range(4)):
+            above, below = list(perm[:n_above]), list(perm[n_above:])
+            program = engine.plan_place(s, above, below)
+            ref = o.place(above, below).flat()
+            if program.op_kinds() == []:
+                continue
+            tasks, sources = program.move_tables(0)
+            tt, ss = cabi.gss_tables(tasks, sources)
+            plan = ctypes.c_void_p()
+            assert lib.tt_gss_plan_create(tt, len(tasks), ss, len(sources), ctypes.byref(plan)) == 0, cabi.error(lib)
+            dst = torch.full((program.out.size,), float('nan'), dtype=torch.float64, device='cuda')
+            assert lib.tt_gss_execute(plan, cabi.TT_F64, src.data_ptr(), dst.data_ptr(), _stream()) == 0
+            assert np.array_equal(dst.cpu().numpy(), ref), (above, below)
+            lib.tt_gss_plan_destroy(plan)
