@@ -1,0 +1,172 @@
+'''
+Seconds per two-site DMRG sweep of the U(1) Heisenberg chain on one MI355X: the batched form
+(one composed leg move and one grouped FP64-MFMA launch per contraction step, head/tail moves
+hoisted out of the Krylov loop) against the plain per-block form (the reference's per-step,
+per-block launches), at L = 64, chi = 1024, n_iter = 5, FP64, seed 0.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--length L] [--bond CHI]
+
+A step is one sweep.  W warmup sweeps ramp the bond (the state starts with one multiplet per
+sector), then K sweeps are timed between synchronisations.  The plain form then sweeps once from the
+same state, and |dE| between the two forms' first timed sweep is reported.  Every block-product
+launch of the timed sweeps is timed with HIP events on its stream; the grouped GEMM is the dominant
+kernel and its roofline line uses the algorithmic 2mnk of its descriptor table over its average
+launch time.  The CPU baseline times the oracle (the numpy restatement of the reference engine) on
+the reference's own CPU case, L = 16, chi = 32, 3 sweeps, on one host thread.
+
+One JSON line goes to stdout; progress goes to stderr.
+'''
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def log(message: str) -> None:
+    print(message, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(length=16, bond=32, sweeps=3):
+    '''The oracle on one host thread: s/sweep of the reference's CPU case.'''
+    from threadpoolctl import threadpool_limits
+    import oracle.symmetry as osym
+    from oracle.dmrg import DMRG as OracleDMRG, Model as OracleModel
+    with threadpool_limits(limits=1):
+        driver = OracleDMRG(OracleModel(length, cat=osym.u1), max_bond_dim=bond, n_iter=5, seed=0)
+        times = []
+        for sweep in range(sweeps):
+            start = time.perf_counter()
+            energy, _ = driver.sweep()
+            times.append(time.perf_counter() - start)
+            log(f'[cpu] oracle sweep {sweep + 1}: {times[-1]:.2f} s energy {energy:.12f}')
+    return sum(times) / len(times), times, energy
+
+
+def main() -> int:
+    parser = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    parser.add_argument('--gpus', type=int, default=1)
+    parser.add_argument('--steps', type=int, default=2, help='timed sweeps')
+    parser.add_argument('--warmup', type=int, default=2, help='untimed sweeps that ramp the bond')
+    parser.add_argument('--length', type=int, default=64)
+    parser.add_argument('--bond', type=int, default=1024)
+    parser.add_argument('--iterations', type=int, default=5)
+    parser.add_argument('--no-plain', action='store_true', help='skip the plain per-block sweep')
+    parser.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
+    parser.add_argument('--no-profile', action='store_true', help='do not time launches with events')
+    args = parser.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py measures one GPU: every box here has one, and this path has no multi-GPU form '
+              f'(asked for {args.gpus}).', file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+    if not torch.cuda.is_available():
+        print('bench.py needs a GPU', file=sys.stderr)
+        return 1
+    import tentorch_amd as tt
+    from tentorch_amd.tensor import Profiler, Settings, workspaces
+
+    log(f'[bench] device {torch.cuda.get_device_name(0)}; L={args.length} chi={args.bond} '
+        f'n_iter={args.iterations} warmup={args.warmup} steps={args.steps}')
+    driver = tt.DMRG(tt.Model(args.length, cat=tt.u1), max_bond_dim=args.bond, n_iter=args.iterations, seed=0)
+    for sweep in range(args.warmup):
+        start = time.perf_counter()
+        energy, lost = driver.sweep()
+        torch.cuda.synchronize()
+        log(f'[bench] warmup sweep {sweep + 1}: {time.perf_counter() - start:.2f} s energy {energy:.12f} '
+            f'max bond {max(driver.state.bond_dims)}')
+    snapshot = ([t.clone() for t in driver.state.tensors], driver.state.center)
+
+    profiler = Profiler()
+    times, energies = [], []
+    torch.cuda.synchronize()
+    total_start = time.perf_counter()
+    for sweep in range(args.steps):
+        start = time.perf_counter()
+        if args.no_profile:
+            energy, lost = driver.sweep()
+        else:
+            with profiler:
+                energy, lost = driver.sweep()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - start)
+        energies.append(energy)
+        log(f'[bench] timed sweep {sweep + 1}: {times[-1]:.2f} s energy {energy:.12f} truncated {lost:.2e} '
+            f'max bond {max(driver.state.bond_dims)}')
+    total = time.perf_counter() - total_start
+    per_sweep = total / args.steps
+
+    plain_per_sweep, plain_energy, delta = None, None, None
+    if not args.no_plain:
+        state = driver.state
+        state.tensors = [t.clone() for t in snapshot[0]]
+        state.center = snapshot[1]
+        workspaces.clear()
+        Settings.mode = 1
+        try:
+            torch.cuda.synchronize()
+            start = time.perf_counter()
+            plain_energy, _ = driver.sweep()
+            torch.cuda.synchronize()
+            plain_per_sweep = time.perf_counter() - start
+        finally:
+            Settings.mode = 0
+        delta = abs(plain_energy - energies[0])
+        log(f'[bench] plain sweep: {plain_per_sweep:.2f} s energy {plain_energy:.12f} |dE| {delta:.2e}')
+
+    roofline = None
+    summary = profiler.summary() if not args.no_profile else {}
+    if 'gemm' in summary and summary['gemm']['ms'] > 0:
+        g = summary['gemm']
+        achieved = g['flops'] / (g['ms'] * 1e-3) / 1e12
+        roofline = {'bound': 'mfma', 'kernel': 'ggemm_kernel<double,double,double,double>',
+                    'achieved': round(achieved, 3), 'peak': PEAK_FP64_TFLOPS, 'unit': 'TFLOP/s',
+                    'frac': round(achieved / PEAK_FP64_TFLOPS, 4), 'traffic': None,
+                    'launches': g['launches'], 'avg_launch_us': round(g['ms'] * 1e3 / g['launches'], 2),
+                    'flops_per_launch': g['flops'] / g['launches']}
+        log(f'[bench] grouped GEMM: {g["launches"]} launches, {g["ms"]:.1f} ms, {achieved:.2f} TFLOP/s')
+    if 'move' in summary and summary['move']['ms'] > 0:
+        m = summary['move']
+        gbs = m['bytes'] / (m['ms'] * 1e-3) / 1e9
+        log(f'[bench] leg moves: {m["launches"]} launches, {m["ms"]:.1f} ms, {gbs:.1f} GB/s')
+        if roofline is not None:
+            roofline['moves'] = {'bound': 'hbm', 'achieved': round(gbs, 1), 'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
+                                 'frac': round(gbs / PEAK_HBM_GBS, 4), 'launches': m['launches'],
+                                 'avg_launch_us': round(m['ms'] * 1e3 / m['launches'], 2)}
+
+    baseline = None
+    if not args.no_cpu:
+        value, cpu_times, cpu_energy = cpu_baseline()
+        baseline = {'value': round(value, 4), 'unit': 's/sweep', 'cores': 1, 'kind': 'port',
+                    'sample': 'oracle (numpy restatement of the reference engine), U(1) Heisenberg L=16 chi=32 '
+                              'n_iter=5, 3 sweeps from seed 0, one thread; the reference CPU case, not L=64 chi=1024'}
+
+    line = {
+        'metric': 's/sweep U(1) Heisenberg L=64 chi=1024, batched vs plain, one GPU, |dE|<=1e-10',
+        'value': round(per_sweep, 4), 'unit': 's/sweep', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(per_sweep * 1e3, 1), 'higher_is_better': False, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic (random initial MPS, seed 0)',
+        'config': {'workload': f'two-site DMRG sweep, U(1) Heisenberg L={args.length} chi={args.bond} '
+                               f'n_iter={args.iterations} FP64, batched form',
+                   'length': args.length, 'bond': args.bond, 'n_iter': args.iterations},
+        'sweeps_s': [round(t, 4) for t in times], 'energies': energies,
+        'plain_s_per_sweep': None if plain_per_sweep is None else round(plain_per_sweep, 4),
+        'speedup_vs_plain': None if plain_per_sweep is None else round(plain_per_sweep / per_sweep, 3),
+        'dE_plain_vs_batched': delta,
+        'roofline': roofline, 'cpu_baseline': baseline,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == '__main__':
+    sys.exit(main())

@@ -231,7 +231,7 @@ PYBIND11_MODULE(_engine, m) {
              for (const Op &op : p.p->ops)
                out.append(py::make_tuple(op.kind == OpKind::move ? "move" : op.kind == OpKind::gemm ? "gemm"
                                          : op.kind == OpKind::copy ? "copy" : "add",
-                                         op.a, op.b, op.dst));
+                                         op.a, op.b, op.dst, op.inputs));
              return out;
            })
       .def("slots",
@@ -279,15 +279,29 @@ PYBIND11_MODULE(_engine, m) {
            })
       .def("run",
            [](const PyProgram &p, int dtype, const std::vector<uintptr_t> &operands, uintptr_t out, uintptr_t workspace,
-              uintptr_t stream, int mode, uint32_t skip) {
+              uintptr_t stream, int mode, uint32_t skip, int64_t first, int64_t last) {
              std::vector<const void *> ptrs;
              for (uintptr_t x : operands) ptrs.push_back(reinterpret_cast<const void *>(x));
              py::gil_scoped_release release;
              p.p->run(dtype, ptrs, reinterpret_cast<void *>(out), reinterpret_cast<void *>(workspace),
-                      reinterpret_cast<void *>(stream), mode == 1 ? Mode::plain : Mode::batched, skip);
+                      reinterpret_cast<void *>(stream), mode == 1 ? Mode::plain : Mode::batched, skip, first, last);
            },
            py::arg("dtype"), py::arg("operands"), py::arg("out"), py::arg("workspace"), py::arg("stream"),
-           py::arg("mode") = 0, py::arg("skip") = 0);
+           py::arg("mode") = 0, py::arg("skip") = 0, py::arg("first") = 0, py::arg("last") = -1)
+      .def("op_costs",
+           [](const PyProgram &p) {
+             // Per op: (kind, algorithmic flops, elements written, elements read).
+             py::list out;
+             for (const Op &op : p.p->ops) {
+               if (op.kind == OpKind::gemm)
+                 out.append(py::make_tuple("gemm", op.gemm->flops, op.gemm->out->size(), op.gemm->above->size() + op.gemm->under->size()));
+               else if (op.kind == OpKind::move)
+                 out.append(py::make_tuple("move", 0.0, op.move->target->size(), op.move->source->size()));
+               else
+                 out.append(py::make_tuple(op.kind == OpKind::copy ? "copy" : "add", 0.0, p.p->out->size(), p.p->out->size()));
+             }
+             return out;
+           });
 
   m.def("plan_einsum", [](const std::string &spec, const std::vector<PyStructure> &ops, py::object charge) {
     std::vector<StructurePtr> ss;

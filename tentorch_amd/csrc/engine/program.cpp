@@ -593,7 +593,7 @@ uint32_t Program::hoistable() const {
 }
 
 void Program::run(int dtype, const std::vector<const void *> &operand_ptrs, void *out_ptr, void *workspace_ptr,
-                  void *stream, Mode mode, uint32_t skip) const {
+                  void *stream, Mode mode, uint32_t skip, int64_t first, int64_t last) const {
   if (operand_ptrs.size() != operands.size()) throw std::invalid_argument("a program got the wrong number of operands");
   const int64_t es = element_size(dtype);
   char *ws = static_cast<char *>(workspace_ptr);
@@ -606,7 +606,9 @@ void Program::run(int dtype, const std::vector<const void *> &operand_ptrs, void
     }
   };
   if (workspace > 0 && !workspace_ptr) throw std::invalid_argument("a program needs its workspace");
-  for (const Op &op : ops) {
+  const int64_t end = last < 0 ? static_cast<int64_t>(ops.size()) : std::min<int64_t>(last, static_cast<int64_t>(ops.size()));
+  for (int64_t at = std::max<int64_t>(first, 0); at < end; ++at) {
+    const Op &op = ops[static_cast<size_t>(at)];
     const bool hoisted = op.kind == OpKind::move && op.inputs && (op.inputs & (op.inputs - 1)) == 0 &&
                          (op.inputs & skip) && slots[static_cast<size_t>(op.dst)].kind == Slot::workspace;
     if (hoisted && mode == Mode::batched) continue;

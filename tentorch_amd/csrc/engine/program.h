@@ -73,8 +73,9 @@ class Program {
 
   // Run with device pointers.  `skip` is a mask of operands whose hoistable moves are still valid in
   // the workspace from the previous run of this program on the same workspace.
+  // Ops [first, last) only when `last` >= 0 (for instrumented runs that time each launch).
   void run(int dtype, const std::vector<const void *> &operand_ptrs, void *out_ptr, void *workspace_ptr,
-           void *stream, Mode mode, uint32_t skip) const;
+           void *stream, Mode mode, uint32_t skip, int64_t first = 0, int64_t last = -1) const;
   // Operands whose moves can be hoisted (moves that read one operand and write the workspace).
   uint32_t hoistable() const;
 };

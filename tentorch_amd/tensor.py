@@ -300,9 +300,57 @@ def run(program, operands: Sequence[Tensor], out: Optional[torch.Tensor] = None)
         ws = entry[0].data_ptr()
     else:
         ws = 0
-    program.run(_native.dtype_code(dtype), [t.data.data_ptr() for t in operands], out.data_ptr(), ws,
-                _native.stream(), mode, skip)
+    args = (_native.dtype_code(dtype), [t.data.data_ptr() for t in operands], out.data_ptr(), ws, _native.stream(),
+            mode, skip)
+    if Profiler.active is None:
+        program.run(*args)
+    else:
+        Profiler.active.run(program, args, skip, mode)
     return Tensor._wrap(result, out)
+
+
+class Profiler:
+    '''Times every block-product and leg-move launch with HIP events on the launch stream, op by op,
+    while active; `summary()` synchronises and returns per-kind totals with their algorithmic work.'''
+
+    active = None
+
+    def __init__(self):
+        self.records = []   # (kind, flops, elements written, elements read, dtype size, start, end)
+
+    def __enter__(self):
+        Profiler.active = self
+        return self
+
+    def __exit__(self, *exc):
+        Profiler.active = None
+
+    def run(self, program, args, skip, mode):
+        costs = program.op_costs()
+        ops = program.ops()
+        for i, ((kind, flops, written, read), (_, _, _, _, inputs)) in enumerate(zip(costs, ops)):
+            single = inputs and not (inputs & (inputs - 1))
+            hoisted = mode == 0 and kind == 'move' and single and (inputs & skip)
+            if kind in ('gemm', 'move') and not hoisted:
+                start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                start.record()
+                program.run(*args, i, i + 1)
+                end.record()
+                size = torch.empty((), dtype=torch.float64 if args[0] != 1 else torch.float32).element_size()
+                self.records.append((kind, flops, written, read, size, start, end))
+            else:
+                program.run(*args, i, i + 1)
+
+    def summary(self):
+        torch.cuda.synchronize()
+        out = {}
+        for kind, flops, written, read, size, start, end in self.records:
+            entry = out.setdefault(kind, {'launches': 0, 'ms': 0.0, 'flops': 0.0, 'bytes': 0.0})
+            entry['launches'] += 1
+            entry['ms'] += start.elapsed_time(end)
+            entry['flops'] += flops
+            entry['bytes'] += (written + read) * size
+        return out
 
 
 def einsum(spec: str, *tensors: Tensor, outcharge=None) -> Tensor:

@@ -49,7 +49,7 @@ def run_program(program, operands, dtype=np.float64) -> np.ndarray:
         else:
             buffers.append(np.zeros(size, dtype=dtype))
     moves = gemms = 0
-    for kind, a, b, d in program.ops():
+    for kind, a, b, d, _ in program.ops():
         if kind == 'move':
             tasks, sources = program.move_tables(moves)
             moves += 1
