@@ -50,6 +50,44 @@ def cpu_baseline(length=16, bond=32, sweeps=3):
     return sum(times) / len(times), times, energy
 
 
+def time_breakdown(driver):
+    '''One sweep with the main phases wrapped in synchronised timers (diagnostic only).'''
+    import torch
+    import tentorch_amd.algorithms as alg
+    totals = {}
+
+    def timed(name, fn):
+        def wrapper(*a, **k):
+            torch.cuda.synchronize()
+            start = time.perf_counter()
+            out = fn(*a, **k)
+            torch.cuda.synchronize()
+            totals[name] = totals.get(name, 0.0) + time.perf_counter() - start
+            return out
+        return wrapper
+
+    saved = (alg.lanczos, alg.svd, alg.einsum, alg.contract, driver._advance_left, driver._advance_right,
+             driver.state.move_center, driver.state.normalize)
+    alg.lanczos = timed('lanczos', alg.lanczos)
+    alg.svd = timed('svd', alg.svd)
+    alg.einsum = timed('einsum (head, tail, energy)', alg.einsum)
+    alg.contract = timed('contract (two-site)', alg.contract)
+    driver._advance_left = timed('environments', driver._advance_left)
+    driver._advance_right = timed('environments', driver._advance_right)
+    driver.state.move_center = timed('canonical moves', driver.state.move_center)
+    driver.state.normalize = timed('normalize', driver.state.normalize)
+    try:
+        torch.cuda.synchronize()
+        start = time.perf_counter()
+        driver.sweep()
+        torch.cuda.synchronize()
+        totals['sweep'] = time.perf_counter() - start
+    finally:
+        (alg.lanczos, alg.svd, alg.einsum, alg.contract, driver._advance_left, driver._advance_right,
+         driver.state.move_center, driver.state.normalize) = saved
+    return {k: round(v, 3) for k, v in totals.items()}
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     parser.add_argument('--gpus', type=int, default=1)
@@ -61,6 +99,8 @@ def main() -> int:
     parser.add_argument('--no-plain', action='store_true', help='skip the plain per-block sweep')
     parser.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     parser.add_argument('--no-profile', action='store_true', help='do not time launches with events')
+    parser.add_argument('--breakdown', action='store_true',
+                        help='after the timed sweeps, one extra sweep with a synchronised time breakdown')
     args = parser.parse_args()
     if args.gpus != 1:
         print(f'bench.py measures one GPU: every box here has one, and this path has no multi-GPU form '
@@ -78,6 +118,17 @@ def main() -> int:
     log(f'[bench] device {torch.cuda.get_device_name(0)}; L={args.length} chi={args.bond} '
         f'n_iter={args.iterations} warmup={args.warmup} steps={args.steps}')
     driver = tt.DMRG(tt.Model(args.length, cat=tt.u1), max_bond_dim=args.bond, n_iter=args.iterations, seed=0)
+    # A progress line every few windows, so a long sweep is visibly alive.
+    update = driver.update
+    windows = {'count': 0, 'start': time.perf_counter()}
+
+    def logged_update(direction):
+        result = update(direction)
+        windows['count'] += 1
+        if windows['count'] % 16 == 0:
+            log(f'[bench]   {windows["count"]} windows, {time.perf_counter() - windows["start"]:.1f} s')
+        return result
+    driver.update = logged_update
     for sweep in range(args.warmup):
         start = time.perf_counter()
         energy, lost = driver.sweep()
@@ -104,6 +155,11 @@ def main() -> int:
             f'max bond {max(driver.state.bond_dims)}')
     total = time.perf_counter() - total_start
     per_sweep = total / args.steps
+
+    breakdown = None
+    if args.breakdown:
+        breakdown = time_breakdown(driver)
+        log(f'[bench] breakdown of one sweep: {json.dumps(breakdown)}')
 
     plain_per_sweep, plain_energy, delta = None, None, None
     if not args.no_plain:
@@ -162,7 +218,7 @@ def main() -> int:
         'plain_s_per_sweep': None if plain_per_sweep is None else round(plain_per_sweep, 4),
         'speedup_vs_plain': None if plain_per_sweep is None else round(plain_per_sweep / per_sweep, 3),
         'dE_plain_vs_batched': delta,
-        'roofline': roofline, 'cpu_baseline': baseline,
+        'roofline': roofline, 'cpu_baseline': baseline, 'breakdown_s': breakdown,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -249,15 +249,19 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 template <typename TC>
 struct Mfma;
+// Result layout of one 16x16x4 MFMA (measured on gfx950): lane l holds column l % 16 in its four
+// result registers r; the f64 form holds rows 4 r + l / 16, the f32 form rows 4 (l / 16) + r.
 template <>
 struct Mfma<double> {
   using acc_t = d4;
   static __device__ acc_t step(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  static __device__ int row(int lk, int r) { return 4 * r + lk; }
 };
 template <>
 struct Mfma<float> {
   using acc_t = f4;
   static __device__ acc_t step(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  static __device__ int row(int lk, int r) { return 4 * lk + r; }
 };
 
 // Output tile of one workgroup: which block, and the tile's first row and column.
@@ -274,8 +278,7 @@ constexpr int kGemmThreads = 256;
 // A 64x64 tile per workgroup: four waves in a 2x2 grid, each wave 32x32 as 2x2 MFMA 16x16x4 tiles.
 // Operands are staged through LDS (A transposed to k-major) with the next k-slab prefetched into
 // registers while the current one is multiplied; the terms of a block are summed in list order.
-// Lane l of an MFMA supplies A[l % 16][l / 16] and B[l / 16][l % 16] and holds
-// D[4 (l / 16) + j][l % 16] for j = 0..3.
+// Lane l of an MFMA supplies A[l % 16][l / 16] and B[l / 16][l % 16]; its results follow Mfma::row.
 template <typename TA, typename TB, typename TO, typename TC>
 __global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__restrict__ tiles,
                                                              const tt_gemm_group *__restrict__ groups,
@@ -390,7 +393,7 @@ __global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__r
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t row = m0 + wm * 32 + i * 16 + lk * 4 + r;
+        const int64_t row = m0 + wm * 32 + i * 16 + M::row(lk, r);
         const int64_t col = n0 + wn * 32 + j * 16 + li;
         if (row < m && col < n) {
           TO *out = C + group.c_offset + row * n + col;

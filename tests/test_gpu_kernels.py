@@ -136,7 +136,7 @@ def test_leg_move_tables_are_bit_exact_against_the_oracle(lib):
             above, below = list(perm[:n_above]), list(perm[n_above:])
             program = engine.plan_place(s, above, below)
             ref = o.place(above, below).flat()
-            if program.op_kinds() == []:
+            if 'move' not in program.op_kinds():
                 continue
             tasks, sources = program.move_tables(0)
             tt, ss = cabi.gss_tables(tasks, sources)
