@@ -66,8 +66,10 @@ def time_breakdown(driver):
             return out
         return wrapper
 
+    import tentorch_amd.linalg as lin
     saved = (alg.lanczos, alg.svd, alg.einsum, alg.contract, driver._advance_left, driver._advance_right,
-             driver.state.move_center, driver.state.normalize)
+             driver.state.move_center, driver.state.normalize, lin._gram_factors)
+    lin._gram_factors = timed('svd: gram eigh', lin._gram_factors)
     alg.lanczos = timed('lanczos', alg.lanczos)
     alg.svd = timed('svd', alg.svd)
     alg.einsum = timed('einsum (head, tail, energy)', alg.einsum)
@@ -84,7 +86,7 @@ def time_breakdown(driver):
         totals['sweep'] = time.perf_counter() - start
     finally:
         (alg.lanczos, alg.svd, alg.einsum, alg.contract, driver._advance_left, driver._advance_right,
-         driver.state.move_center, driver.state.normalize) = saved
+         driver.state.move_center, driver.state.normalize, lin._gram_factors) = saved
     return {k: round(v, 3) for k, v in totals.items()}
 
 
@@ -180,24 +182,30 @@ def main() -> int:
         log(f'[bench] plain sweep: {plain_per_sweep:.2f} s energy {plain_energy:.12f} |dE| {delta:.2e}')
 
     roofline = None
-    summary = profiler.summary() if not args.no_profile else {}
+    matvec = 'EPaqj,JRSFb,AQsb->EPrf'
+    summary = profiler.summary(matvec) if not args.no_profile else {}
+    overall = profiler.summary() if not args.no_profile else {}
+    for name, part in (('all', overall), ('matvec', summary)):
+        for kind in ('gemm', 'move'):
+            if kind in part and part[kind]['ms'] > 0:
+                k = part[kind]
+                log(f'[bench] {name} {kind}: {k["launches"]} launches, {k["ms"]:.1f} ms, '
+                    f'{k["flops"] / (k["ms"] * 1e-3) / 1e12:.2f} TFLOP/s, {k["bytes"] / (k["ms"] * 1e-3) / 1e9:.1f} GB/s')
     if 'gemm' in summary and summary['gemm']['ms'] > 0:
         g = summary['gemm']
         achieved = g['flops'] / (g['ms'] * 1e-3) / 1e12
-        roofline = {'bound': 'mfma', 'kernel': 'ggemm_kernel<double,double,double,double>',
+        roofline = {'bound': 'mfma', 'kernel': 'ggemm_kernel<double,double,double,double> (matvec block products)',
                     'achieved': round(achieved, 3), 'peak': PEAK_FP64_TFLOPS, 'unit': 'TFLOP/s',
                     'frac': round(achieved / PEAK_FP64_TFLOPS, 4), 'traffic': None,
                     'launches': g['launches'], 'avg_launch_us': round(g['ms'] * 1e3 / g['launches'], 2),
                     'flops_per_launch': g['flops'] / g['launches']}
-        log(f'[bench] grouped GEMM: {g["launches"]} launches, {g["ms"]:.1f} ms, {achieved:.2f} TFLOP/s')
-    if 'move' in summary and summary['move']['ms'] > 0:
+    if 'move' in summary and summary['move']['ms'] > 0 and roofline is not None:
         m = summary['move']
         gbs = m['bytes'] / (m['ms'] * 1e-3) / 1e9
-        log(f'[bench] leg moves: {m["launches"]} launches, {m["ms"]:.1f} ms, {gbs:.1f} GB/s')
-        if roofline is not None:
-            roofline['moves'] = {'bound': 'hbm', 'achieved': round(gbs, 1), 'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
-                                 'frac': round(gbs / PEAK_HBM_GBS, 4), 'launches': m['launches'],
-                                 'avg_launch_us': round(m['ms'] * 1e3 / m['launches'], 2)}
+        roofline['moves'] = {'bound': 'hbm', 'kernel': 'gss_kernel<double> (matvec leg move)', 'achieved': round(gbs, 1),
+                             'peak': PEAK_HBM_GBS, 'unit': 'GB/s', 'frac': round(gbs / PEAK_HBM_GBS, 4),
+                             'launches': m['launches'], 'avg_launch_us': round(m['ms'] * 1e3 / m['launches'], 2),
+                             'bytes_per_launch': m['bytes'] / m['launches']}
 
     baseline = None
     if not args.no_cpu:

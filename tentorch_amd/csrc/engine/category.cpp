@@ -15,9 +15,17 @@ std::string render(const Sector &s) {
 }
 
 const std::vector<Outcome> &Category::outcomes(const Sector &a, const Sector &b) const {
+  const bool small = a.n == 1 && b.n == 1 && a.v[0] >= -kSmall && a.v[0] < kSmall && b.v[0] >= -kSmall && b.v[0] < kSmall;
+  size_t slot = 0;
+  if (small) {
+    if (small_.empty()) small_.assign(static_cast<size_t>(4 * kSmall * kSmall), nullptr);
+    slot = static_cast<size_t>((a.v[0] + kSmall) * 2 * kSmall + (b.v[0] + kSmall));
+    if (small_[slot]) return *small_[slot];
+  }
   const auto key = std::make_pair(a, b);
   auto found = outcomes_.find(key);
   if (found == outcomes_.end()) found = outcomes_.emplace(key, compute_outcomes(a, b)).first;
+  if (small) small_[slot] = &found->second;  // node-based map: the element never moves
   return found->second;
 }
 
@@ -72,6 +80,7 @@ const Matrix &Category::R(const Sector &b, const Sector &c) const {
 }
 
 void Category::clear_cache() const {
+  small_.clear();
   outcomes_.clear();
   f_.clear();
   r_.clear();

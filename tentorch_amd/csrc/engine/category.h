@@ -125,6 +125,10 @@ class Category {
   };
   std::string name_;
   mutable std::unordered_map<std::pair<Sector, Sector>, std::vector<Outcome>, PairHash> outcomes_;
+  // Direct-mapped view of outcomes_ for one-integer sectors in [-kSmall, kSmall): the fusion tables
+  // of a contraction are looked up hundreds of thousands of times while it is planned.
+  static constexpr int64_t kSmall = 64;
+  mutable std::vector<const std::vector<Outcome> *> small_;
   mutable std::map<std::tuple<Sector, Sector, Sector, Sector>, Matrix> f_;
   mutable std::map<std::pair<Sector, Sector>, Matrix> r_;
 };

@@ -301,11 +301,9 @@ std::vector<Terms> step_map(const PairSpace &from, const PairSpace &to, const St
 // Multiplicities of the legs at a tree pair, codomain legs then domain legs.
 void pair_extents(const Structure &S, int64_t block, int64_t row, int64_t col, std::vector<int64_t> &ext) {
   const BlockInfo &b = S.blocks()[static_cast<size_t>(block)];
-  const FusionTree &rt = b.row_layout->trees[static_cast<size_t>(row)];
-  const FusionTree &ct = b.col_layout->trees[static_cast<size_t>(col)];
-  ext.clear();
-  for (size_t a = 0; a < S.codomain().size(); ++a) ext.push_back(S.codomain()[a]->mult_of(rt.sectors[a]));
-  for (size_t a = 0; a < S.domain().size(); ++a) ext.push_back(S.domain()[a]->mult_of(ct.sectors[a]));
+  const SideLayout &rl = *b.row_layout, &cl = *b.col_layout;
+  ext.assign(rl.mults.begin() + row * rl.nlegs, rl.mults.begin() + (row + 1) * rl.nlegs);
+  ext.insert(ext.end(), cl.mults.begin() + col * cl.nlegs, cl.mults.begin() + (col + 1) * cl.nlegs);
 }
 
 // Element offset and per-axis strides of one tree pair's box inside the flat buffer.
@@ -328,18 +326,105 @@ int64_t pair_strides(const Structure &S, int64_t block, int64_t row, int64_t col
   return b.offset + b.row_layout->offsets[static_cast<size_t>(row)] * b.cols + b.col_layout->offsets[static_cast<size_t>(col)];
 }
 
+// The leg permutation of a chain: target axis j carries source leg perm[j].
+std::vector<int64_t> chain_permutation(int64_t ncod0, int64_t nlegs, const std::vector<Step> &steps) {
+  std::vector<int64_t> perm(static_cast<size_t>(nlegs));
+  std::iota(perm.begin(), perm.end(), 0);
+  int64_t ncod = ncod0;
+  for (const Step &step : steps) {
+    if (step.kind == StepKind::exchange) {
+      std::swap(perm[static_cast<size_t>(step.position)], perm[static_cast<size_t>(step.position + 1)]);
+    } else if (step.kind == StepKind::to_domain) {
+      const int64_t leg = perm[static_cast<size_t>(ncod - 1)];
+      perm.erase(perm.begin() + (ncod - 1));
+      perm.push_back(leg);
+      --ncod;
+    } else {
+      const int64_t leg = perm.back();
+      perm.pop_back();
+      perm.insert(perm.begin() + ncod, leg);
+      ++ncod;
+    }
+  }
+  return perm;
+}
+
+// For abelian categories every vertex is the number 1, so each elementary step maps one tree pair to
+// exactly one tree pair: an exchange swaps two sectors with the factor R(b, c) (a sign for fermion
+// parity, 1 for the groups), a bend moves the last sector of one side to the end of the other as its
+// dual with F = 1 (regroup.cpp:25-417 reduce to this).  The composed map is therefore the chain
+// replayed on the sector lists of each source pair; only the final structure is built.
+std::vector<Terms> compose_abelian(const StructurePtr &source, const std::vector<Step> &steps, const PairSpace &target) {
+  const Category &cat = *source->cat();
+  const Structure &T = *target.s;
+  PairSpace space(source);
+  std::vector<Terms> comp(static_cast<size_t>(space.total));
+  // Target tree lookup by sector list, per side layout.
+  std::map<const SideLayout *, std::map<std::vector<Sector>, int64_t>> lookup;
+  auto index_of = [&](const SideLayout &layout, const std::vector<Sector> &sectors) -> int64_t {
+    auto &table = lookup[&layout];
+    if (table.empty())
+      for (size_t t = 0; t < layout.trees.size(); ++t) table.emplace(layout.trees[t].sectors, static_cast<int64_t>(t));
+    auto found = table.find(sectors);
+    return found == table.end() ? -1 : found->second;
+  };
+  auto root_of = [&](const std::vector<Sector> &sectors) {
+    Sector r = cat.trivial();
+    for (const Sector &s : sectors) r = cat.outcomes(r, s).front().sector;
+    return r;
+  };
+  std::vector<Sector> cod, dom;
+  for (int64_t b = 0; b < static_cast<int64_t>(source->blocks().size()); ++b) {
+    const BlockInfo &blk = source->blocks()[static_cast<size_t>(b)];
+    for (int64_t row = 0; row < static_cast<int64_t>(blk.row_layout->trees.size()); ++row)
+      for (int64_t col = 0; col < static_cast<int64_t>(blk.col_layout->trees.size()); ++col) {
+        cod = blk.row_layout->trees[static_cast<size_t>(row)].sectors;
+        dom = blk.col_layout->trees[static_cast<size_t>(col)].sectors;
+        cplx coef(1.0, 0.0);
+        for (const Step &step : steps) {
+          if (step.kind == StepKind::exchange) {
+            const size_t p = static_cast<size_t>(step.position);
+            coef *= cat.R(cod[p], cod[p + 1])(0, 0);
+            std::swap(cod[p], cod[p + 1]);
+          } else if (step.kind == StepKind::to_domain) {
+            dom.push_back(cat.dual(cod.back()));
+            cod.pop_back();
+          } else {
+            cod.push_back(cat.dual(dom.back()));
+            dom.pop_back();
+          }
+        }
+        const int64_t tb = T.find(root_of(cod), root_of(dom), blk.key.copy);
+        if (tb < 0) continue;
+        const BlockInfo &tblk = T.blocks()[static_cast<size_t>(tb)];
+        const int64_t trow = index_of(*tblk.row_layout, cod);
+        const int64_t tcol = index_of(*tblk.col_layout, dom);
+        if (trow < 0 || tcol < 0) continue;  // the dual sector is not on the leg (regroup.cpp:383)
+        comp[static_cast<size_t>(space.id(b, row, col))].emplace_back(target.id(tb, trow, tcol), coef);
+      }
+  }
+  return comp;
+}
+
 // Compose the steps from `source` and emit one tile per target tree pair.
 void compose(const StructurePtr &source, const std::vector<Step> &steps, const std::vector<StructurePtr> &after,
              MovePlan &plan, bool per_block_groups) {
   const int64_t nlegs = source->nlegs();
   if (nlegs > TT_MAX_DIMS) throw std::invalid_argument("a leg move handles at most " + std::to_string(TT_MAX_DIMS) + " legs");
   PairSpace space(source);
-  std::vector<Terms> comp(static_cast<size_t>(space.total));
-  for (int64_t i = 0; i < space.total; ++i) comp[static_cast<size_t>(i)].emplace_back(i, cplx(1.0, 0.0));
-  std::vector<int64_t> perm(static_cast<size_t>(nlegs));
-  std::iota(perm.begin(), perm.end(), 0);
+  std::vector<Terms> comp;
+  std::vector<int64_t> perm = chain_permutation(static_cast<int64_t>(source->codomain().size()), nlegs, steps);
   const StructurePtr *current = &source;
-  for (size_t k = 0; k < steps.size(); ++k) {
+  if (source->cat()->abelian()) {
+    PairSpace target_space(after.back());
+    comp = compose_abelian(source, steps, target_space);
+    space = std::move(target_space);
+    current = &after.back();
+  } else {
+    comp.resize(static_cast<size_t>(space.total));
+    for (int64_t i = 0; i < space.total; ++i) comp[static_cast<size_t>(i)].emplace_back(i, cplx(1.0, 0.0));
+  }
+  for (size_t k = 0; k < steps.size() && !source->cat()->abelian(); ++k) {
     PairSpace next(after[k]);
     const std::vector<Terms> map = step_map(space, next, steps[k]);
     for (Terms &terms : comp) {
@@ -353,18 +438,6 @@ void compose(const StructurePtr &source, const std::vector<Step> &steps, const s
             it->second += coef * c2;
         }
       terms = std::move(out);
-    }
-    const int64_t ncod = static_cast<int64_t>((*current)->codomain().size());
-    if (steps[k].kind == StepKind::exchange) {
-      std::swap(perm[static_cast<size_t>(steps[k].position)], perm[static_cast<size_t>(steps[k].position + 1)]);
-    } else if (steps[k].kind == StepKind::to_domain) {
-      const int64_t leg = perm[static_cast<size_t>(ncod - 1)];
-      perm.erase(perm.begin() + (ncod - 1));
-      perm.push_back(leg);
-    } else {
-      const int64_t leg = perm.back();
-      perm.pop_back();
-      perm.insert(perm.begin() + ncod, leg);
     }
     space = std::move(next);
     current = &after[k];
@@ -481,13 +554,19 @@ MovePlanPtr build(const StructurePtr &source, const std::vector<int64_t> &above,
   auto plan = std::make_shared<MovePlan>();
   plan->source = source;
   std::vector<StructurePtr> after;
-  for (const auto &[cod, dom] : chain.sides) after.push_back(Structure::make(source->cat(), cod, dom, source->qtotal()));
+  // An abelian chain composes without its intermediate structures; the per-step form needs them.
+  const bool direct = source->cat()->abelian() && !plain;
+  for (size_t k = direct && !chain.sides.empty() ? chain.sides.size() - 1 : 0; k < chain.sides.size(); ++k)
+    after.push_back(Structure::make(source->cat(), chain.sides[k].first, chain.sides[k].second, source->qtotal()));
   plan->target = after.empty() ? source : after.back();
   plan->identity = chain.steps.empty();
   plan->above = above;
   plan->below = below;
   plan->chained = true;
-  for (size_t k = 0; k + 1 < after.size(); ++k) plan->max_intermediate = std::max(plan->max_intermediate, after[k]->size());
+  if (direct)
+    plan->max_intermediate = std::max(source->size(), plan->target->size());
+  else
+    for (size_t k = 0; k + 1 < after.size(); ++k) plan->max_intermediate = std::max(plan->max_intermediate, after[k]->size());
   if (!plan->identity) {
     compose(source, chain.steps, after, *plan, false);
     if (plain) {

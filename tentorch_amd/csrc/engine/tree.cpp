@@ -176,8 +176,26 @@ size_t TreeHash::operator()(const FusionTree &t) const {
   return seed;
 }
 
-// Insertion order of first appearance (tree.cpp:375-398).
+namespace {
+std::map<std::pair<const Category *, std::vector<const Leg *>>, std::vector<Sector>> &reachable_table() {
+  static std::map<std::pair<const Category *, std::vector<const Leg *>>, std::vector<Sector>> table;
+  return table;
+}
+std::vector<Sector> compute_reachable(const Category &cat, const Legs &legs);
+}  // namespace
+
+// Insertion order of first appearance (tree.cpp:375-398), kept per (category, legs).
 std::vector<Sector> reachable_charges(const Category &cat, const Legs &legs) {
+  std::pair<const Category *, std::vector<const Leg *>> key{&cat, {}};
+  for (const LegPtr &l : legs) key.second.push_back(l.get());
+  auto &table = reachable_table();
+  auto found = table.find(key);
+  if (found != table.end()) return found->second;
+  return table.emplace(std::move(key), compute_reachable(cat, legs)).first->second;
+}
+
+namespace {
+std::vector<Sector> compute_reachable(const Category &cat, const Legs &legs) {
   std::vector<Sector> charges{cat.trivial()};
   for (const LegPtr &leg : legs) {
     std::vector<Sector> next;
@@ -190,6 +208,7 @@ std::vector<Sector> reachable_charges(const Category &cat, const Legs &legs) {
   }
   return charges;
 }
+}  // namespace
 
 // Depth first over leg blocks in sector order and outcomes in category order (tree.cpp:400-471).
 std::vector<FusionTree> fusion_trees(const Category &cat, const Legs &legs, const Sector &coupled) {
@@ -253,10 +272,16 @@ LayoutPtr layout_of(const CategoryPtr &cat, const Legs &legs, const Sector &root
   if (found != table.end()) return found->second;
   auto layout = std::make_shared<SideLayout>();
   layout->trees = fusion_trees(*cat, legs, root);
+  layout->nlegs = static_cast<int64_t>(legs.size());
   layout->offsets.push_back(0);
+  layout->mults.reserve(layout->trees.size() * legs.size());
   for (size_t t = 0; t < layout->trees.size(); ++t) {
     int64_t rows = 1;
-    for (size_t axis = 0; axis < legs.size(); ++axis) rows *= legs[axis]->mult_of(layout->trees[t].sectors[axis]);
+    for (size_t axis = 0; axis < legs.size(); ++axis) {
+      const int64_t m = legs[axis]->mult_of(layout->trees[t].sectors[axis]);
+      layout->mults.push_back(m);
+      rows *= m;
+    }
     layout->offsets.push_back(layout->offsets.back() + rows);
     layout->index.emplace(layout->trees[t], static_cast<int64_t>(t));
   }
@@ -317,6 +342,7 @@ int64_t Structure::find(const BlockKey &k) const {
 }
 
 void clear_engine_caches() {
+  reachable_table().clear();
   structure_table().clear();
   layout_table().clear();
 }

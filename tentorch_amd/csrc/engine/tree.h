@@ -80,6 +80,8 @@ FusionTree trimmed(const Category &cat, const FusionTree &tree);
 struct SideLayout {
   std::vector<FusionTree> trees;
   std::vector<int64_t> offsets;   // size trees + 1
+  std::vector<int64_t> mults;     // per tree, the multiplicity each leg carries (trees x legs)
+  int64_t nlegs = 0;
   std::unordered_map<FusionTree, int64_t, TreeHash> index;
   int64_t rows() const { return offsets.back(); }
   int64_t find(const FusionTree &t) const {

@@ -274,7 +274,7 @@ class Settings:
     mode = 0
 
 
-def run(program, operands: Sequence[Tensor], out: Optional[torch.Tensor] = None) -> Tensor:
+def run(program, operands: Sequence[Tensor], out: Optional[torch.Tensor] = None, label: str = '') -> Tensor:
     '''Execute a planned program on device operands; returns the result tensor.'''
     first = operands[0]
     dtype, device = first.dtype, first.device
@@ -305,7 +305,7 @@ def run(program, operands: Sequence[Tensor], out: Optional[torch.Tensor] = None)
     if Profiler.active is None:
         program.run(*args)
     else:
-        Profiler.active.run(program, args, skip, mode)
+        Profiler.active.run(program, args, skip, mode, label)
     return Tensor._wrap(result, out)
 
 
@@ -325,7 +325,7 @@ class Profiler:
     def __exit__(self, *exc):
         Profiler.active = None
 
-    def run(self, program, args, skip, mode):
+    def run(self, program, args, skip, mode, label=''):
         costs = program.op_costs()
         ops = program.ops()
         for i, ((kind, flops, written, read), (_, _, _, _, inputs)) in enumerate(zip(costs, ops)):
@@ -337,14 +337,17 @@ class Profiler:
                 program.run(*args, i, i + 1)
                 end.record()
                 size = torch.empty((), dtype=torch.float64 if args[0] != 1 else torch.float32).element_size()
-                self.records.append((kind, flops, written, read, size, start, end))
+                self.records.append((kind, flops, written, read, size, start, end, label))
             else:
                 program.run(*args, i, i + 1)
 
-    def summary(self):
+    def summary(self, label=None):
+        '''Totals per kind, over every record or over the records of one einsum equation.'''
         torch.cuda.synchronize()
         out = {}
-        for kind, flops, written, read, size, start, end in self.records:
+        for kind, flops, written, read, size, start, end, lab in self.records:
+            if label is not None and lab != label:
+                continue
             entry = out.setdefault(kind, {'launches': 0, 'ms': 0.0, 'flops': 0.0, 'bytes': 0.0})
             entry['launches'] += 1
             entry['ms'] += start.elapsed_time(end)
@@ -359,7 +362,7 @@ def einsum(spec: str, *tensors: Tensor, outcharge=None) -> Tensor:
         raise ValueError('an einsum needs at least one operand')
     program = engine.plan_einsum(spec, [t.structure for t in tensors],
                                  None if outcharge is None else tuple(outcharge))
-    return run(program, tensors)
+    return run(program, tensors, label=spec)
 
 
 def contract(left: Tensor, right: Tensor, left_legs, right_legs, charge=None) -> Tensor:
