@@ -95,6 +95,15 @@ typedef struct tt_ggemm_plan tt_ggemm_plan;
 
 int tt_ggemm_plan_create(const tt_gemm_group *groups, int64_t num_groups, const tt_gemm_term *terms,
                          int64_t num_terms, tt_ggemm_plan **plan);
+/* The same with an explicit tile shape (0 <= config < tt_ggemm_num_configs(); a negative config lets
+ * the library choose from the block shapes, as tt_ggemm_plan_create does).  For an autotuner. */
+int tt_ggemm_plan_create_config(const tt_gemm_group *groups, int64_t num_groups, const tt_gemm_term *terms,
+                                int64_t num_terms, int config, tt_ggemm_plan **plan);
+/* Tile shapes the library is built with: rows x cols of the output tile, depth of one k-slab. */
+int tt_ggemm_num_configs(void);
+int tt_ggemm_config_shape(int config, int *rows, int *cols, int *depth);
+/* The tile shape a plan uses. */
+int tt_ggemm_plan_config(const tt_ggemm_plan *plan);
 int tt_ggemm_plan_destroy(tt_ggemm_plan *plan);
 /* Real multiply-add count of the plan (2 m n k summed over terms), and tile count of one launch. */
 int tt_ggemm_plan_counts(const tt_ggemm_plan *plan, double *flops, int64_t *tiles);

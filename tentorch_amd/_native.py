@@ -34,7 +34,8 @@ _lib.tt_graph_destroy.argtypes = [_c_void_p]
 
 EXPORTED = ['tt_gss_plan_create', 'tt_gss_plan_destroy', 'tt_gss_plan_counts', 'tt_gss_execute',
             'tt_gss_execute_plain', 'tt_ggemm_plan_create', 'tt_ggemm_plan_destroy', 'tt_ggemm_plan_counts',
-            'tt_ggemm_execute', 'tt_ggemm_execute_plain', 'tt_dot', 'tt_axpby', 'tt_convert', 'tt_graph_begin',
+            'tt_ggemm_execute', 'tt_ggemm_execute_plain', 'tt_ggemm_plan_create_config', 'tt_ggemm_num_configs',
+            'tt_ggemm_config_shape', 'tt_ggemm_plan_config', 'tt_dot', 'tt_axpby', 'tt_convert', 'tt_graph_begin',
             'tt_graph_end', 'tt_graph_launch', 'tt_graph_destroy', 'tt_last_error', 'tt_version']
 
 

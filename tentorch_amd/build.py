@@ -21,6 +21,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, 'csrc')
 BUILD = os.path.join(ROOT, '_build')
 HIP_LIB = os.path.join(HERE, 'libtentorch_hip.so')
+HIP_SOURCES = ['device.hip', 'ggemm.hip']
 ENGINE_SOURCES = ['category.cpp', 'tree.cpp', 'move.cpp', 'program.cpp', 'bindings.cpp']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 HIP_FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared',
@@ -42,7 +43,7 @@ def build_hip() -> str:
     '''Compile the device library for gfx950.'''
     os.makedirs(BUILD, exist_ok=True)
     out = HIP_LIB + '.tmp'
-    _run([HIPCC, *HIP_FLAGS, '-o', out, os.path.join(CSRC, 'hip', 'device.hip')])
+    _run([HIPCC, *HIP_FLAGS, '-o', out, *[os.path.join(CSRC, 'hip', name) for name in HIP_SOURCES]])
     os.replace(out, HIP_LIB)
     return HIP_LIB
 

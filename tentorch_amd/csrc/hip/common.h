@@ -1,0 +1,36 @@
+// Shared helpers of the device library's translation units: the thread-local error text behind
+// tt_last_error, status checks, and the stream cast.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../../include/tentorch_amd.h"
+
+namespace ttdev {
+
+// Records `message` as this thread's last error and returns the failure status.
+int fail(const std::string &message);
+
+inline hipStream_t as_stream(void *stream) { return reinterpret_cast<hipStream_t>(stream); }
+
+struct cdouble {
+  double re, im;
+};
+
+}  // namespace ttdev
+
+#define TT_HIP(call)                                                                           \
+  do {                                                                                         \
+    hipError_t status_ = (call);                                                               \
+    if (status_ != hipSuccess)                                                                 \
+      return ::ttdev::fail(std::string(#call) + " failed: " + hipGetErrorString(status_));     \
+  } while (0)
+
+#define TT_LAUNCHED()                                                                          \
+  do {                                                                                         \
+    hipError_t status_ = hipGetLastError();                                                    \
+    if (status_ != hipSuccess)                                                                 \
+      return ::ttdev::fail(std::string("kernel launch failed: ") + hipGetErrorString(status_)); \
+  } while (0)

@@ -1,8 +1,6 @@
-// The MI355X device library behind include/tentorch_amd.h: descriptor-table upload, the
-// gather-scale-scatter leg-move kernel, the grouped block GEMM on FP64/FP32 MFMA, fixed-order
-// reductions and hipGraph capture.  gfx950 only.
-#include <hip/hip_runtime.h>
-
+// The MI355X device library behind include/tentorch_amd.h (part one): descriptor-table upload and
+// the gather-scale-scatter leg-move kernel, fixed-order reductions, elementwise updates, hipGraph
+// capture and the error channel.  The grouped block GEMM lives in ggemm.hip.  gfx950 only.
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -10,35 +8,26 @@
 #include <string>
 #include <vector>
 
-#include "../../../include/tentorch_amd.h"
+#include "common.h"
+
+namespace ttdev {
 
 namespace {
-
 thread_local std::string g_error;
+}
 
 int fail(const std::string &message) {
   g_error = message;
   return -1;
 }
 
-#define TT_HIP(call)                                                                           \
-  do {                                                                                         \
-    hipError_t status_ = (call);                                                               \
-    if (status_ != hipSuccess)                                                                 \
-      return fail(std::string(#call) + " failed: " + hipGetErrorString(status_));              \
-  } while (0)
+}  // namespace ttdev
 
-#define TT_LAUNCHED()                                                                          \
-  do {                                                                                         \
-    hipError_t status_ = hipGetLastError();                                                    \
-    if (status_ != hipSuccess) return fail(std::string("kernel launch failed: ") + hipGetErrorString(status_)); \
-  } while (0)
+using ttdev::as_stream;
+using ttdev::cdouble;
+using ttdev::fail;
 
-hipStream_t as_stream(void *stream) { return reinterpret_cast<hipStream_t>(stream); }
-
-struct cdouble {
-  double re, im;
-};
+namespace {
 
 // ---------------------------------------------------------------------------------------------
 // Gather-scale-scatter
@@ -178,7 +167,7 @@ extern "C" int tt_gss_plan_create(const tt_gss_task *tasks, int64_t num_tasks, c
   if (upload(reinterpret_cast<void **>(&p->tasks), tasks, sizeof(tt_gss_task) * num_tasks) ||
       upload(reinterpret_cast<void **>(&p->sources), sources, sizeof(tt_gss_source) * num_sources) ||
       upload(reinterpret_cast<void **>(&p->chunks), chunks.data(), sizeof(GssChunk) * chunks.size())) {
-    std::string keep = g_error;
+    std::string keep = ttdev::g_error;
     tt_gss_plan_destroy(p);
     return fail(keep);
   }
@@ -236,320 +225,6 @@ extern "C" int tt_gss_execute_plain(const tt_gss_plan *plan, int dtype, const vo
   if (!src || !dst) return fail("tt_gss_execute_plain: null buffer");
   for (const auto &g : plan->groups)
     if (int rc = gss_dispatch(plan, dtype, src, dst, as_stream(stream), g.first, g.second)) return rc;
-  return 0;
-}
-
-// ---------------------------------------------------------------------------------------------
-// Grouped block GEMM on MFMA
-
-namespace {
-
-typedef double d4 __attribute__((ext_vector_type(4)));
-typedef float f4 __attribute__((ext_vector_type(4)));
-
-template <typename TC>
-struct Mfma;
-// Result layout of one 16x16x4 MFMA (measured on gfx950): lane l holds column l % 16 in its four
-// result registers r; the f64 form holds rows 4 r + l / 16, the f32 form rows 4 (l / 16) + r.
-template <>
-struct Mfma<double> {
-  using acc_t = d4;
-  static __device__ acc_t step(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-  static __device__ int row(int lk, int r) { return 4 * r + lk; }
-};
-template <>
-struct Mfma<float> {
-  using acc_t = f4;
-  static __device__ acc_t step(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-  static __device__ int row(int lk, int r) { return 4 * lk + r; }
-};
-
-// Output tile of one workgroup: which block, and the tile's first row and column.
-struct GemmTile {
-  int32_t group;
-  int32_t m0;
-  int32_t n0;
-  int32_t pad;
-};
-
-constexpr int kBM = 64, kBN = 64, kBK = 16, kPad = 1;
-constexpr int kGemmThreads = 256;
-
-// A 64x64 tile per workgroup: four waves in a 2x2 grid, each wave 32x32 as 2x2 MFMA 16x16x4 tiles.
-// Operands are staged through LDS (A transposed to k-major) with the next k-slab prefetched into
-// registers while the current one is multiplied; the terms of a block are summed in list order.
-// Lane l of an MFMA supplies A[l % 16][l / 16] and B[l / 16][l % 16]; its results follow Mfma::row.
-template <typename TA, typename TB, typename TO, typename TC>
-__global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__restrict__ tiles,
-                                                             const tt_gemm_group *__restrict__ groups,
-                                                             const tt_gemm_term *__restrict__ terms,
-                                                             const TA *__restrict__ A, const TB *__restrict__ B,
-                                                             TO *__restrict__ C, int32_t plain_group,
-                                                             int32_t plain_term, int32_t accumulate) {
-  using M = Mfma<TC>;
-  __shared__ TC As[2][kBK][kBM + kPad];
-  __shared__ TC Bs[2][kBK][kBN + kPad];
-
-  int32_t gi, m0, n0, t_begin, t_end;
-  if (plain_group < 0) {
-    const GemmTile tile = tiles[blockIdx.x];
-    gi = tile.group;
-    m0 = tile.m0;
-    n0 = tile.n0;
-    t_begin = groups[gi].first_term;
-    t_end = t_begin + groups[gi].num_terms;
-  } else {
-    gi = plain_group;
-    const int32_t tiles_n = static_cast<int32_t>((groups[gi].n + kBN - 1) / kBN);
-    m0 = static_cast<int32_t>(blockIdx.x / tiles_n) * kBM;
-    n0 = static_cast<int32_t>(blockIdx.x % tiles_n) * kBN;
-    t_begin = plain_term < 0 ? 0 : plain_term;
-    t_end = plain_term < 0 ? 0 : plain_term + 1;
-  }
-  const tt_gemm_group group = groups[gi];
-  const int64_t m = group.m, n = group.n;
-
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  const int li = lane & 15, lk = lane >> 4;
-
-  typename M::acc_t acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = typename M::acc_t{0, 0, 0, 0};
-
-  // Loader coordinates: A slab 64 rows x 16 k (4 k per thread), B slab 16 k x 64 cols (4 cols).
-  const int a_row = tid >> 2, a_k = (tid & 3) * 4;
-  const int b_k = tid >> 4, b_col = (tid & 15) * 4;
-
-  // Flattened (term, k0) steps.
-  int32_t term = t_begin;
-  int64_t k0 = 0;
-  // Skip empty terms.
-  while (term < t_end && terms[term].k <= 0) ++term;
-
-  TC ra[4], rb[4];
-  auto load = [&](int32_t t, int64_t kb) {
-    const tt_gemm_term tm = terms[t];
-    const int64_t K = tm.k;
-    const int64_t gr = m0 + a_row;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int64_t kk = kb + a_k + j;
-      ra[j] = (gr < m && kk < K) ? static_cast<TC>(A[tm.a_offset + gr * K + kk]) : TC(0);
-    }
-    const int64_t kr = kb + b_k;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int64_t gc = n0 + b_col + j;
-      rb[j] = (kr < K && gc < n) ? static_cast<TC>(B[tm.b_offset + kr * n + gc]) : TC(0);
-    }
-  };
-  auto advance = [&](int32_t &t, int64_t &kb) {
-    kb += kBK;
-    if (kb >= terms[t].k) {
-      kb = 0;
-      ++t;
-      while (t < t_end && terms[t].k <= 0) ++t;
-    }
-  };
-
-  int buf = 0;
-  if (term < t_end) load(term, k0);
-  while (term < t_end) {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) As[buf][a_k + j][a_row] = ra[j];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) Bs[buf][b_k][b_col + j] = rb[j];
-    __syncthreads();
-    int32_t next_term = term;
-    int64_t next_k = k0;
-    advance(next_term, next_k);
-    if (next_term < t_end) load(next_term, next_k);
-#pragma unroll
-    for (int kk = 0; kk < kBK / 4; ++kk) {
-      const int k = kk * 4 + lk;
-      TC af[2], bf[2];
-#pragma unroll
-      for (int i = 0; i < 2; ++i) af[i] = As[buf][k][wm * 32 + i * 16 + li];
-#pragma unroll
-      for (int j = 0; j < 2; ++j) bf[j] = Bs[buf][k][wn * 32 + j * 16 + li];
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = M::step(af[i], bf[j], acc[i][j]);
-    }
-    term = next_term;
-    k0 = next_k;
-    buf ^= 1;
-  }
-
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t row = m0 + wm * 32 + i * 16 + M::row(lk, r);
-        const int64_t col = n0 + wn * 32 + j * 16 + li;
-        if (row < m && col < n) {
-          TO *out = C + group.c_offset + row * n + col;
-          const TC value = acc[i][j][r];
-          if (accumulate)
-            *out = static_cast<TO>(static_cast<TC>(*out) + value);
-          else
-            *out = static_cast<TO>(value);
-        }
-      }
-}
-
-}  // namespace
-
-struct tt_ggemm_plan {
-  tt_gemm_group *groups = nullptr;
-  tt_gemm_term *terms = nullptr;
-  GemmTile *tiles = nullptr;
-  int64_t num_groups = 0, num_terms = 0, num_tiles = 0;
-  double flops = 0.0;
-  std::vector<tt_gemm_group> host_groups;
-};
-
-extern "C" int tt_ggemm_plan_create(const tt_gemm_group *groups, int64_t num_groups, const tt_gemm_term *terms,
-                                    int64_t num_terms, tt_ggemm_plan **plan) {
-  if (!plan) return fail("tt_ggemm_plan_create: null output handle");
-  if (num_groups < 0 || num_terms < 0) return fail("tt_ggemm_plan_create: negative count");
-  struct Costed {
-    double cost;
-    GemmTile tile;
-  };
-  std::vector<Costed> costed;
-  double flops = 0.0;
-  for (int64_t g = 0; g < num_groups; ++g) {
-    const tt_gemm_group &grp = groups[g];
-    if (grp.m < 0 || grp.n < 0) return fail("tt_ggemm_plan_create: negative block shape");
-    if (grp.num_terms < 0 || grp.first_term < 0 || grp.first_term + grp.num_terms > num_terms)
-      return fail("tt_ggemm_plan_create: group terms out of range");
-    if (grp.m >= (int64_t(1) << 31) || grp.n >= (int64_t(1) << 31)) return fail("tt_ggemm_plan_create: block too large");
-    int64_t ksum = 0;
-    for (int32_t t = grp.first_term; t < grp.first_term + grp.num_terms; ++t) {
-      if (terms[t].k < 0) return fail("tt_ggemm_plan_create: negative k");
-      ksum += terms[t].k;
-    }
-    flops += 2.0 * static_cast<double>(grp.m) * static_cast<double>(grp.n) * static_cast<double>(ksum);
-    for (int64_t mt = 0; mt < grp.m; mt += kBM)
-      for (int64_t nt = 0; nt < grp.n; nt += kBN)
-        costed.push_back({static_cast<double>(ksum) + 1.0,
-                          GemmTile{static_cast<int32_t>(g), static_cast<int32_t>(mt), static_cast<int32_t>(nt), 0}});
-  }
-  // Longest tiles first, so the last wave of the launch is made of short ones.
-  std::stable_sort(costed.begin(), costed.end(), [](const Costed &a, const Costed &b) { return a.cost > b.cost; });
-  std::vector<GemmTile> tiles;
-  tiles.reserve(costed.size());
-  for (const Costed &c : costed) tiles.push_back(c.tile);
-  auto *p = new tt_ggemm_plan();
-  p->num_groups = num_groups;
-  p->num_terms = num_terms;
-  p->num_tiles = static_cast<int64_t>(tiles.size());
-  p->flops = flops;
-  p->host_groups.assign(groups, groups + num_groups);
-  auto upload = [&](void **where, const void *data, size_t bytes) -> int {
-    if (!bytes) return 0;
-    TT_HIP(hipMalloc(where, bytes));
-    TT_HIP(hipMemcpy(*where, data, bytes, hipMemcpyHostToDevice));
-    return 0;
-  };
-  if (upload(reinterpret_cast<void **>(&p->groups), groups, sizeof(tt_gemm_group) * num_groups) ||
-      upload(reinterpret_cast<void **>(&p->terms), terms, sizeof(tt_gemm_term) * num_terms) ||
-      upload(reinterpret_cast<void **>(&p->tiles), tiles.data(), sizeof(GemmTile) * tiles.size())) {
-    std::string keep = g_error;
-    tt_ggemm_plan_destroy(p);
-    return fail(keep);
-  }
-  *plan = p;
-  return 0;
-}
-
-extern "C" int tt_ggemm_plan_destroy(tt_ggemm_plan *plan) {
-  if (!plan) return 0;
-  if (plan->groups) (void)hipFree(plan->groups);
-  if (plan->terms) (void)hipFree(plan->terms);
-  if (plan->tiles) (void)hipFree(plan->tiles);
-  delete plan;
-  return 0;
-}
-
-extern "C" int tt_ggemm_plan_counts(const tt_ggemm_plan *plan, double *flops, int64_t *tiles) {
-  if (!plan) return fail("tt_ggemm_plan_counts: null plan");
-  if (flops) *flops = plan->flops;
-  if (tiles) *tiles = plan->num_tiles;
-  return 0;
-}
-
-namespace {
-template <typename TA, typename TB, typename TO, typename TC>
-int ggemm_launch(const tt_ggemm_plan *plan, const void *a, const void *b, void *c, hipStream_t stream, int64_t grid,
-                 int32_t plain_group, int32_t plain_term, int32_t accumulate) {
-  if (grid <= 0) return 0;
-  hipLaunchKernelGGL((ggemm_kernel<TA, TB, TO, TC>), dim3(static_cast<uint32_t>(grid)), dim3(kGemmThreads), 0, stream,
-                     plan->tiles, plan->groups, plan->terms, static_cast<const TA *>(a), static_cast<const TB *>(b),
-                     static_cast<TO *>(c), plain_group, plain_term, accumulate);
-  TT_LAUNCHED();
-  return 0;
-}
-
-int ggemm_dispatch(const tt_ggemm_plan *plan, int at, int bt, int ct, int comp, const void *a, const void *b, void *c,
-                   hipStream_t stream, int64_t grid, int32_t pg, int32_t pt, int32_t acc) {
-#define TT_CASE(A_, B_, C_, K_, TA, TB, TO, TC) \
-  if (at == A_ && bt == B_ && ct == C_ && comp == K_) return ggemm_launch<TA, TB, TO, TC>(plan, a, b, c, stream, grid, pg, pt, acc);
-  TT_CASE(TT_F64, TT_F64, TT_F64, TT_F64, double, double, double, double)
-  TT_CASE(TT_F32, TT_F32, TT_F32, TT_F32, float, float, float, float)
-  TT_CASE(TT_F32, TT_F64, TT_F32, TT_F32, float, double, float, float)
-  TT_CASE(TT_F32, TT_F32, TT_F64, TT_F32, float, float, double, float)
-  TT_CASE(TT_F64, TT_F64, TT_F64, TT_F32, double, double, double, float)
-  TT_CASE(TT_F64, TT_F64, TT_F32, TT_F32, double, double, float, float)
-  TT_CASE(TT_F32, TT_F64, TT_F64, TT_F32, float, double, double, float)
-#undef TT_CASE
-  return fail("grouped GEMM: unsupported type combination (a " + std::to_string(at) + ", b " + std::to_string(bt) +
-              ", c " + std::to_string(ct) + ", compute " + std::to_string(comp) + ")");
-}
-}  // namespace
-
-extern "C" int tt_ggemm_execute(const tt_ggemm_plan *plan, int a_type, int b_type, int c_type, int compute_type,
-                                const void *a, const void *b, void *c, void *stream) {
-  if (!plan) return fail("tt_ggemm_execute: null plan");
-  if (plan->num_tiles == 0) return 0;
-  if (!c || ((!a || !b) && plan->num_terms)) return fail("tt_ggemm_execute: null buffer");
-  return ggemm_dispatch(plan, a_type, b_type, c_type, compute_type, a, b, c, as_stream(stream), plan->num_tiles, -1, 0, 0);
-}
-
-extern "C" int tt_ggemm_execute_plain(const tt_ggemm_plan *plan, int dtype, const void *a, const void *b, void *c,
-                                      void *stream) {
-  if (!plan) return fail("tt_ggemm_execute_plain: null plan");
-  if (dtype != TT_F64 && dtype != TT_F32) return fail("tt_ggemm_execute_plain: unsupported dtype");
-  for (int64_t g = 0; g < plan->num_groups; ++g) {
-    const tt_gemm_group &grp = plan->host_groups[static_cast<size_t>(g)];
-    const int64_t grid = ((grp.m + kBM - 1) / kBM) * ((grp.n + kBN - 1) / kBN);
-    if (grp.num_terms == 0) {
-      // An empty term list still owes the block its zeros: one launch over an empty K.
-      continue;
-    }
-    for (int32_t t = 0; t < grp.num_terms; ++t)
-      if (int rc = ggemm_dispatch(plan, dtype, dtype, dtype, dtype, a, b, c, as_stream(stream), grid,
-                                  static_cast<int32_t>(g), grp.first_term + t, t > 0 ? 1 : 0))
-        return rc;
-  }
-  // Blocks without terms: zero them with the batched kernel restricted by an empty term range.
-  for (int64_t g = 0; g < plan->num_groups; ++g) {
-    const tt_gemm_group &grp = plan->host_groups[static_cast<size_t>(g)];
-    if (grp.num_terms != 0) continue;
-    const int64_t grid = ((grp.m + kBM - 1) / kBM) * ((grp.n + kBN - 1) / kBN);
-    if (int rc = ggemm_dispatch(plan, dtype, dtype, dtype, dtype, a, b, c, as_stream(stream), grid,
-                                static_cast<int32_t>(g), -1, 0))
-      return rc;
-  }
   return 0;
 }
 
@@ -781,7 +456,7 @@ extern "C" int tt_graph_destroy(tt_graph *graph) {
 
 extern "C" int tt_last_error(char *buffer, size_t size) {
   if (!buffer || size == 0) return -1;
-  std::snprintf(buffer, size, "%s", g_error.c_str());
+  std::snprintf(buffer, size, "%s", ttdev::g_error.c_str());
   return 0;
 }
 

@@ -1,0 +1,410 @@
+// The MI355X device library behind include/tentorch_amd.h (part two): the grouped block GEMM on
+// FP64 and FP32 MFMA.  One launch computes every output block of a contraction step; each
+// workgroup owns one BM x BN output tile of one block and sums that block's terms in list order.
+// The tile shape is a property of the plan (chosen by the host from the block shapes, or by an
+// autotuner through tt_ggemm_plan_create_config).  gfx950 only.
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+using ttdev::as_stream;
+using ttdev::fail;
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// Result layout of one 16x16x4 MFMA (measured on gfx950): lane l holds column l % 16 in its four
+// result registers r; the f64 form holds rows 4 r + l / 16, the f32 form rows 4 (l / 16) + r.
+// Operands: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16].
+template <typename TC>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  using acc_t = d4;
+  static __device__ acc_t step(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  static __device__ int row(int lk, int r) { return 4 * r + lk; }
+};
+template <>
+struct Mfma<float> {
+  using acc_t = f4;
+  static __device__ acc_t step(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  static __device__ int row(int lk, int r) { return 4 * lk + r; }
+};
+
+// Output tile of one workgroup: which block, and the tile's first row and column.
+struct GemmTile {
+  int32_t group;
+  int32_t m0;
+  int32_t n0;
+  int32_t pad;
+};
+
+constexpr int kGemmThreads = 256;  // four waves in a 2 x 2 grid over the tile
+constexpr int kPad = 1;
+
+// The tile shapes the library is built with: (BM, BN, BK).
+struct Shape {
+  int bm, bn, bk;
+};
+constexpr Shape kShapes[] = {{64, 64, 16}, {32, 32, 16}, {128, 64, 16}, {64, 32, 16}, {32, 64, 16}, {128, 128, 16}, {64, 64, 32}};
+constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+
+// A BM x BN tile per workgroup: four waves in a 2 x 2 grid, each wave (BM/2) x (BN/2) as
+// 16 x 16 x 4 MFMA tiles.  Operands are staged through LDS (A transposed to k-major) with the next
+// k-slab prefetched into registers while the current one is multiplied; one barrier per slab.
+template <typename TA, typename TB, typename TO, typename TC, int BM, int BN, int BK>
+__global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__restrict__ tiles,
+                                                             const tt_gemm_group *__restrict__ groups,
+                                                             const tt_gemm_term *__restrict__ terms,
+                                                             const TA *__restrict__ A, const TB *__restrict__ B,
+                                                             TO *__restrict__ C, int32_t plain_group,
+                                                             int32_t plain_term, int32_t accumulate) {
+  using M = Mfma<TC>;
+  constexpr int TM = BM / 2, TN = BN / 2, FM = TM / 16, FN = TN / 16;
+  constexpr int A_PER = BM * BK / kGemmThreads, B_PER = BK * BN / kGemmThreads;
+  constexpr int A_TPR = BK / A_PER, B_TPR = BN / B_PER;
+  static_assert(FM >= 1 && FN >= 1 && A_PER >= 1 && B_PER >= 1, "tile too small for four waves");
+  static_assert(BK % A_PER == 0 && BN % B_PER == 0 && BK % 4 == 0, "tile does not split over the threads");
+  __shared__ TC As[2][BK][BM + kPad];
+  __shared__ TC Bs[2][BK][BN + kPad];
+
+  int32_t gi, m0, n0, t_begin, t_end;
+  if (plain_group < 0) {
+    const GemmTile tile = tiles[blockIdx.x];
+    gi = tile.group;
+    m0 = tile.m0;
+    n0 = tile.n0;
+    t_begin = groups[gi].first_term;
+    t_end = t_begin + groups[gi].num_terms;
+  } else {
+    gi = plain_group;
+    const int32_t tiles_n = static_cast<int32_t>((groups[gi].n + BN - 1) / BN);
+    m0 = static_cast<int32_t>(blockIdx.x / tiles_n) * BM;
+    n0 = static_cast<int32_t>(blockIdx.x % tiles_n) * BN;
+    t_begin = plain_term < 0 ? 0 : plain_term;
+    t_end = plain_term < 0 ? 0 : plain_term + 1;
+  }
+  const tt_gemm_group group = groups[gi];
+  const int64_t m = group.m, n = group.n;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int li = lane & 15, lk = lane >> 4;
+
+  typename M::acc_t acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = typename M::acc_t{0, 0, 0, 0};
+
+  const int a_row = tid / A_TPR, a_k = (tid % A_TPR) * A_PER;
+  const int b_k = tid / B_TPR, b_col = (tid % B_TPR) * B_PER;
+
+  int32_t term = t_begin;
+  int64_t k0 = 0;
+  while (term < t_end && terms[term].k <= 0) ++term;
+
+  TC ra[A_PER], rb[B_PER];
+  auto load = [&](int32_t t, int64_t kb) {
+    const tt_gemm_term tm = terms[t];
+    const int64_t K = tm.k;
+    const int64_t gr = m0 + a_row;
+#pragma unroll
+    for (int j = 0; j < A_PER; ++j) {
+      const int64_t kk = kb + a_k + j;
+      ra[j] = (gr < m && kk < K) ? static_cast<TC>(A[tm.a_offset + gr * K + kk]) : TC(0);
+    }
+    const int64_t kr = kb + b_k;
+#pragma unroll
+    for (int j = 0; j < B_PER; ++j) {
+      const int64_t gc = n0 + b_col + j;
+      rb[j] = (kr < K && gc < n) ? static_cast<TC>(B[tm.b_offset + kr * n + gc]) : TC(0);
+    }
+  };
+  auto advance = [&](int32_t &t, int64_t &kb) {
+    kb += BK;
+    if (kb >= terms[t].k) {
+      kb = 0;
+      ++t;
+      while (t < t_end && terms[t].k <= 0) ++t;
+    }
+  };
+
+  int buf = 0;
+  if (term < t_end) load(term, k0);
+  while (term < t_end) {
+#pragma unroll
+    for (int j = 0; j < A_PER; ++j) As[buf][a_k + j][a_row] = ra[j];
+#pragma unroll
+    for (int j = 0; j < B_PER; ++j) Bs[buf][b_k][b_col + j] = rb[j];
+    __syncthreads();
+    int32_t next_term = term;
+    int64_t next_k = k0;
+    advance(next_term, next_k);
+    if (next_term < t_end) load(next_term, next_k);
+#pragma unroll
+    for (int kk = 0; kk < BK / 4; ++kk) {
+      const int k = kk * 4 + lk;
+      TC af[FM], bf[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) af[i] = As[buf][k][wm * TM + i * 16 + li];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) bf[j] = Bs[buf][k][wn * TN + j * 16 + li];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = M::step(af[i], bf[j], acc[i][j]);
+    }
+    term = next_term;
+    k0 = next_k;
+    buf ^= 1;
+  }
+
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = m0 + wm * TM + i * 16 + M::row(lk, r);
+        const int64_t col = n0 + wn * TN + j * 16 + li;
+        if (row < m && col < n) {
+          TO *out = C + group.c_offset + row * n + col;
+          const TC value = acc[i][j][r];
+          if (accumulate)
+            *out = static_cast<TO>(static_cast<TC>(*out) + value);
+          else
+            *out = static_cast<TO>(value);
+        }
+      }
+}
+
+// Host-side tile count and cost of a group list under one tile shape.
+void tile_list(const tt_gemm_group *groups, int64_t num_groups, const tt_gemm_term *terms, const Shape &shape,
+               std::vector<GemmTile> &tiles, double *max_cost) {
+  struct Costed {
+    double cost;
+    GemmTile tile;
+  };
+  std::vector<Costed> costed;
+  *max_cost = 0.0;
+  for (int64_t g = 0; g < num_groups; ++g) {
+    const tt_gemm_group &grp = groups[g];
+    int64_t ksum = 0;
+    for (int32_t t = grp.first_term; t < grp.first_term + grp.num_terms; ++t) ksum += terms[t].k;
+    const double cost = static_cast<double>((ksum + shape.bk - 1) / shape.bk) + 1.0;
+    *max_cost = std::max(*max_cost, cost);
+    for (int64_t mt = 0; mt < grp.m; mt += shape.bm)
+      for (int64_t nt = 0; nt < grp.n; nt += shape.bn)
+        costed.push_back({cost, GemmTile{static_cast<int32_t>(g), static_cast<int32_t>(mt), static_cast<int32_t>(nt), 0}});
+  }
+  // Longest tiles first, so the last wave of the launch is made of short ones.
+  std::stable_sort(costed.begin(), costed.end(), [](const Costed &a, const Costed &b) { return a.cost > b.cost; });
+  tiles.clear();
+  for (const Costed &c : costed) tiles.push_back(c.tile);
+}
+
+// The default tile shape: the one whose estimated makespan is smallest, with a tile's time taken as
+// its k-slab count times the slab cost of its shape (relative MFMA work per slab, with a floor for
+// the per-slab overhead), and the launch spread over 256 CUs at two workgroups each.
+int choose_shape(const tt_gemm_group *groups, int64_t num_groups, const tt_gemm_term *terms) {
+  const int candidates[] = {0, 1, 2, 3, 4};
+  int best = 0;
+  double best_time = 0.0;
+  for (int c : candidates) {
+    const Shape &s = kShapes[c];
+    std::vector<GemmTile> tiles;
+    double max_cost = 0.0;
+    tile_list(groups, num_groups, terms, s, tiles, &max_cost);
+    double work = 0.0;
+    for (int64_t g = 0; g < num_groups; ++g) {
+      int64_t ksum = 0;
+      for (int32_t t = groups[g].first_term; t < groups[g].first_term + groups[g].num_terms; ++t) ksum += terms[t].k;
+      const double slabs = static_cast<double>((ksum + s.bk - 1) / s.bk) + 1.0;
+      const double ntiles = static_cast<double>(((groups[g].m + s.bm - 1) / s.bm) * ((groups[g].n + s.bn - 1) / s.bn));
+      work += ntiles * slabs;
+    }
+    const double slab = std::max(1.0, s.bm * s.bn / 4096.0) ;
+    const double slots = 512.0;
+    const double time = std::max(work / slots, max_cost) * slab;
+    if (c == candidates[0] || time < best_time) {
+      best = c;
+      best_time = time;
+    }
+  }
+  return best;
+}
+
+}  // namespace
+
+struct tt_ggemm_plan {
+  tt_gemm_group *groups = nullptr;
+  tt_gemm_term *terms = nullptr;
+  GemmTile *tiles = nullptr;
+  int64_t num_groups = 0, num_terms = 0, num_tiles = 0;
+  int config = 0;
+  double flops = 0.0;
+  std::vector<tt_gemm_group> host_groups;
+};
+
+extern "C" int tt_ggemm_num_configs(void) { return kNumShapes; }
+
+extern "C" int tt_ggemm_config_shape(int config, int *rows, int *cols, int *depth) {
+  if (config < 0 || config >= kNumShapes) return fail("tt_ggemm_config_shape: no such tile shape");
+  if (rows) *rows = kShapes[config].bm;
+  if (cols) *cols = kShapes[config].bn;
+  if (depth) *depth = kShapes[config].bk;
+  return 0;
+}
+
+extern "C" int tt_ggemm_plan_create_config(const tt_gemm_group *groups, int64_t num_groups, const tt_gemm_term *terms,
+                                           int64_t num_terms, int config, tt_ggemm_plan **plan) {
+  if (!plan) return fail("tt_ggemm_plan_create: null output handle");
+  if (num_groups < 0 || num_terms < 0) return fail("tt_ggemm_plan_create: negative count");
+  double flops = 0.0;
+  for (int64_t g = 0; g < num_groups; ++g) {
+    const tt_gemm_group &grp = groups[g];
+    if (grp.m < 0 || grp.n < 0) return fail("tt_ggemm_plan_create: negative block shape");
+    if (grp.num_terms < 0 || grp.first_term < 0 || grp.first_term + grp.num_terms > num_terms)
+      return fail("tt_ggemm_plan_create: group terms out of range");
+    if (grp.m >= (int64_t(1) << 31) || grp.n >= (int64_t(1) << 31)) return fail("tt_ggemm_plan_create: block too large");
+    for (int32_t t = grp.first_term; t < grp.first_term + grp.num_terms; ++t) {
+      if (terms[t].k < 0) return fail("tt_ggemm_plan_create: negative k");
+      flops += 2.0 * static_cast<double>(grp.m) * static_cast<double>(grp.n) * static_cast<double>(terms[t].k);
+    }
+  }
+  if (config < 0) config = choose_shape(groups, num_groups, terms);
+  if (config >= kNumShapes) return fail("tt_ggemm_plan_create: no such tile shape");
+  std::vector<GemmTile> tiles;
+  double max_cost = 0.0;
+  tile_list(groups, num_groups, terms, kShapes[config], tiles, &max_cost);
+  auto *p = new tt_ggemm_plan();
+  p->num_groups = num_groups;
+  p->num_terms = num_terms;
+  p->num_tiles = static_cast<int64_t>(tiles.size());
+  p->config = config;
+  p->flops = flops;
+  p->host_groups.assign(groups, groups + num_groups);
+  auto upload = [&](void **where, const void *data, size_t bytes) -> int {
+    if (!bytes) return 0;
+    TT_HIP(hipMalloc(where, bytes));
+    TT_HIP(hipMemcpy(*where, data, bytes, hipMemcpyHostToDevice));
+    return 0;
+  };
+  if (upload(reinterpret_cast<void **>(&p->groups), groups, sizeof(tt_gemm_group) * num_groups) ||
+      upload(reinterpret_cast<void **>(&p->terms), terms, sizeof(tt_gemm_term) * num_terms) ||
+      upload(reinterpret_cast<void **>(&p->tiles), tiles.data(), sizeof(GemmTile) * tiles.size())) {
+    char keep[1024];
+    tt_last_error(keep, sizeof(keep));
+    tt_ggemm_plan_destroy(p);
+    return fail(keep);
+  }
+  *plan = p;
+  return 0;
+}
+
+extern "C" int tt_ggemm_plan_create(const tt_gemm_group *groups, int64_t num_groups, const tt_gemm_term *terms,
+                                    int64_t num_terms, tt_ggemm_plan **plan) {
+  return tt_ggemm_plan_create_config(groups, num_groups, terms, num_terms, -1, plan);
+}
+
+extern "C" int tt_ggemm_plan_destroy(tt_ggemm_plan *plan) {
+  if (!plan) return 0;
+  if (plan->groups) (void)hipFree(plan->groups);
+  if (plan->terms) (void)hipFree(plan->terms);
+  if (plan->tiles) (void)hipFree(plan->tiles);
+  delete plan;
+  return 0;
+}
+
+extern "C" int tt_ggemm_plan_counts(const tt_ggemm_plan *plan, double *flops, int64_t *tiles) {
+  if (!plan) return fail("tt_ggemm_plan_counts: null plan");
+  if (flops) *flops = plan->flops;
+  if (tiles) *tiles = plan->num_tiles;
+  return 0;
+}
+
+extern "C" int tt_ggemm_plan_config(const tt_ggemm_plan *plan) { return plan ? plan->config : -1; }
+
+namespace {
+
+template <typename TA, typename TB, typename TO, typename TC, int BM, int BN, int BK>
+int ggemm_launch(const tt_ggemm_plan *plan, const void *a, const void *b, void *c, hipStream_t stream, int64_t grid,
+                 int32_t plain_group, int32_t plain_term, int32_t accumulate) {
+  if (grid <= 0) return 0;
+  hipLaunchKernelGGL((ggemm_kernel<TA, TB, TO, TC, BM, BN, BK>), dim3(static_cast<uint32_t>(grid)), dim3(kGemmThreads),
+                     0, stream, plan->tiles, plan->groups, plan->terms, static_cast<const TA *>(a),
+                     static_cast<const TB *>(b), static_cast<TO *>(c), plain_group, plain_term, accumulate);
+  TT_LAUNCHED();
+  return 0;
+}
+
+template <typename TA, typename TB, typename TO, typename TC>
+int ggemm_shape(const tt_ggemm_plan *plan, const void *a, const void *b, void *c, hipStream_t stream, int64_t grid,
+                int32_t pg, int32_t pt, int32_t acc) {
+  switch (plan->config) {
+    case 0: return ggemm_launch<TA, TB, TO, TC, 64, 64, 16>(plan, a, b, c, stream, grid, pg, pt, acc);
+    case 1: return ggemm_launch<TA, TB, TO, TC, 32, 32, 16>(plan, a, b, c, stream, grid, pg, pt, acc);
+    case 2: return ggemm_launch<TA, TB, TO, TC, 128, 64, 16>(plan, a, b, c, stream, grid, pg, pt, acc);
+    case 3: return ggemm_launch<TA, TB, TO, TC, 64, 32, 16>(plan, a, b, c, stream, grid, pg, pt, acc);
+    case 4: return ggemm_launch<TA, TB, TO, TC, 32, 64, 16>(plan, a, b, c, stream, grid, pg, pt, acc);
+    case 5: return ggemm_launch<TA, TB, TO, TC, 128, 128, 16>(plan, a, b, c, stream, grid, pg, pt, acc);
+    case 6: return ggemm_launch<TA, TB, TO, TC, 64, 64, 32>(plan, a, b, c, stream, grid, pg, pt, acc);
+    default: return fail("grouped GEMM: no such tile shape");
+  }
+}
+
+int ggemm_dispatch(const tt_ggemm_plan *plan, int at, int bt, int ct, int comp, const void *a, const void *b, void *c,
+                   hipStream_t stream, int64_t grid, int32_t pg, int32_t pt, int32_t acc) {
+#define TT_CASE(A_, B_, C_, K_, TA, TB, TO, TC) \
+  if (at == A_ && bt == B_ && ct == C_ && comp == K_) return ggemm_shape<TA, TB, TO, TC>(plan, a, b, c, stream, grid, pg, pt, acc);
+  TT_CASE(TT_F64, TT_F64, TT_F64, TT_F64, double, double, double, double)
+  TT_CASE(TT_F32, TT_F32, TT_F32, TT_F32, float, float, float, float)
+  TT_CASE(TT_F32, TT_F64, TT_F32, TT_F32, float, double, float, float)
+  TT_CASE(TT_F32, TT_F32, TT_F64, TT_F32, float, float, double, float)
+  TT_CASE(TT_F64, TT_F64, TT_F64, TT_F32, double, double, double, float)
+  TT_CASE(TT_F64, TT_F64, TT_F32, TT_F32, double, double, float, float)
+  TT_CASE(TT_F32, TT_F64, TT_F64, TT_F32, float, double, double, float)
+#undef TT_CASE
+  return fail("grouped GEMM: unsupported type combination (a " + std::to_string(at) + ", b " + std::to_string(bt) +
+              ", c " + std::to_string(ct) + ", compute " + std::to_string(comp) + ")");
+}
+
+}  // namespace
+
+extern "C" int tt_ggemm_execute(const tt_ggemm_plan *plan, int a_type, int b_type, int c_type, int compute_type,
+                                const void *a, const void *b, void *c, void *stream) {
+  if (!plan) return fail("tt_ggemm_execute: null plan");
+  if (plan->num_tiles == 0) return 0;
+  if (!c || ((!a || !b) && plan->num_terms)) return fail("tt_ggemm_execute: null buffer");
+  return ggemm_dispatch(plan, a_type, b_type, c_type, compute_type, a, b, c, as_stream(stream), plan->num_tiles, -1, 0, 0);
+}
+
+extern "C" int tt_ggemm_execute_plain(const tt_ggemm_plan *plan, int dtype, const void *a, const void *b, void *c,
+                                      void *stream) {
+  if (!plan) return fail("tt_ggemm_execute_plain: null plan");
+  if (dtype != TT_F64 && dtype != TT_F32) return fail("tt_ggemm_execute_plain: unsupported dtype");
+  const Shape &s = kShapes[plan->config];
+  for (int64_t g = 0; g < plan->num_groups; ++g) {
+    const tt_gemm_group &grp = plan->host_groups[static_cast<size_t>(g)];
+    const int64_t grid = ((grp.m + s.bm - 1) / s.bm) * ((grp.n + s.bn - 1) / s.bn);
+    if (grp.num_terms == 0) {
+      // A block with no term still owes its zeros: one launch over an empty term range.
+      if (int rc = ggemm_dispatch(plan, dtype, dtype, dtype, dtype, a, b, c, as_stream(stream), grid,
+                                  static_cast<int32_t>(g), -1, 0))
+        return rc;
+      continue;
+    }
+    for (int32_t t = 0; t < grp.num_terms; ++t)
+      if (int rc = ggemm_dispatch(plan, dtype, dtype, dtype, dtype, a, b, c, as_stream(stream), grid,
+                                  static_cast<int32_t>(g), grp.first_term + t, t > 0 ? 1 : 0))
+        return rc;
+  }
+  return 0;
+}

@@ -44,6 +44,9 @@ def load():
     lib.tt_gss_execute.argtypes = [vp, i32, vp, vp, vp]
     lib.tt_gss_execute_plain.argtypes = [vp, i32, vp, vp, vp]
     lib.tt_ggemm_plan_create.argtypes = [ctypes.POINTER(GemmGroup), i64, ctypes.POINTER(GemmTerm), i64, ctypes.POINTER(vp)]
+    lib.tt_ggemm_plan_create_config.argtypes = [ctypes.POINTER(GemmGroup), i64, ctypes.POINTER(GemmTerm), i64, i32, ctypes.POINTER(vp)]
+    lib.tt_ggemm_config_shape.argtypes = [i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    lib.tt_ggemm_plan_config.argtypes = [vp]
     lib.tt_ggemm_plan_destroy.argtypes = [vp]
     lib.tt_ggemm_plan_counts.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]
     lib.tt_ggemm_execute.argtypes = [vp, i32, i32, i32, i32, vp, vp, vp, vp]
