@@ -39,8 +39,28 @@ struct GssChunk {
   int64_t start;
 };
 
+// The device form of a tile: extents with their division constants (n / d == (n * magic) >> shift
+// for every n < 2^31, Granlund-Montgomery with N = 31), and the first source resolved in place so
+// a single-source tile (every abelian move) reads no second record.
+struct DevTask {
+  int64_t dst_offset;
+  int64_t src_offset;
+  int32_t ndim;
+  int32_t num_sources;
+  int32_t first_source;
+  int32_t pad;
+  uint32_t extent[TT_MAX_DIMS];
+  uint32_t shift[TT_MAX_DIMS];
+  uint64_t magic[TT_MAX_DIMS];
+  int64_t dst_stride[TT_MAX_DIMS];
+  int64_t src_stride[TT_MAX_DIMS];
+  double coef_re;
+  double coef_im;
+};
+
 constexpr int kGssThreads = 256;
-constexpr int kGssChunk = 4096;
+constexpr int kGssUnroll = 4;
+constexpr int kGssChunk = kGssThreads * kGssUnroll * 4;
 
 template <typename T>
 struct Scale;
@@ -62,57 +82,77 @@ struct Scale<cdouble> {
   }
 };
 
-// Each workgroup takes one chunk of one tile; threads stride the chunk in destination order, so
-// consecutive lanes write consecutive destination elements along the tile's fastest axis.
+// Each workgroup takes one chunk of one tile.  A thread handles kGssUnroll elements per pass, 256
+// apart, so consecutive lanes write consecutive destination elements along the tile's fastest axis
+// and each thread keeps several independent loads in flight.
 template <typename T>
 __global__ __launch_bounds__(kGssThreads) void gss_kernel(const GssChunk *__restrict__ chunks,
-                                                          const tt_gss_task *__restrict__ tasks,
+                                                          const DevTask *__restrict__ tasks,
                                                           const tt_gss_source *__restrict__ sources,
                                                           const T *__restrict__ src, T *__restrict__ dst,
                                                           int32_t chunk_base) {
   const GssChunk chunk = chunks[chunk_base + blockIdx.x];
-  const tt_gss_task &task = tasks[chunk.task];
+  const DevTask &task = tasks[chunk.task];
   const int nd = task.ndim;
-  uint32_t ext[TT_MAX_DIMS];
-  int64_t dstr[TT_MAX_DIMS];
-#pragma unroll
-  for (int d = 0; d < TT_MAX_DIMS; ++d) {
-    ext[d] = d < nd ? static_cast<uint32_t>(task.extent[d]) : 1u;
-    dstr[d] = d < nd ? task.dst_stride[d] : 0;
-  }
   const int ns = task.num_sources;
-  const tt_gss_source *first = sources + task.first_source;
-  for (int32_t at = threadIdx.x; at < chunk.count; at += kGssThreads) {
-    uint32_t rem = static_cast<uint32_t>(chunk.start + at);
-    uint32_t idx[TT_MAX_DIMS];
-    int64_t doff = task.dst_offset;
+  for (int32_t at = threadIdx.x; at < chunk.count; at += kGssThreads * kGssUnroll) {
+    int64_t doff[kGssUnroll], soff[kGssUnroll];
+    uint32_t idx[kGssUnroll][TT_MAX_DIMS];
 #pragma unroll
-    for (int d = TT_MAX_DIMS - 1; d >= 0; --d) {
-      if (d < nd) {
-        idx[d] = rem % ext[d];
-        rem /= ext[d];
-        doff += static_cast<int64_t>(idx[d]) * dstr[d];
-      } else {
-        idx[d] = 0;
+    for (int u = 0; u < kGssUnroll; ++u) {
+      uint32_t rem = static_cast<uint32_t>(chunk.start + at + u * kGssThreads);
+      doff[u] = task.dst_offset;
+      soff[u] = task.src_offset;
+#pragma unroll
+      for (int d = TT_MAX_DIMS - 1; d >= 0; --d) {
+        if (d < nd) {
+          const uint32_t q = static_cast<uint32_t>((static_cast<uint64_t>(rem) * task.magic[d]) >> task.shift[d]);
+          idx[u][d] = rem - q * task.extent[d];
+          rem = q;
+          doff[u] += static_cast<int64_t>(idx[u][d]) * task.dst_stride[d];
+          soff[u] += static_cast<int64_t>(idx[u][d]) * task.src_stride[d];
+        } else {
+          idx[u][d] = 0;
+        }
       }
     }
-    T acc = Scale<T>::zero();
-    for (int s = 0; s < ns; ++s) {
-      const tt_gss_source &source = first[s];
-      int64_t soff = source.src_offset;
+    T value[kGssUnroll];
 #pragma unroll
-      for (int d = 0; d < TT_MAX_DIMS; ++d)
-        if (d < nd) soff += static_cast<int64_t>(idx[d]) * source.src_stride[d];
-      acc = Scale<T>::fma(acc, source.coef_re, source.coef_im, src[soff]);
+    for (int u = 0; u < kGssUnroll; ++u)
+      if (at + u * kGssThreads < chunk.count)
+        value[u] = ns > 0 ? Scale<T>::fma(Scale<T>::zero(), task.coef_re, task.coef_im, src[soff[u]]) : Scale<T>::zero();
+    for (int s = 1; s < ns; ++s) {
+      const tt_gss_source &source = sources[task.first_source + s];
+#pragma unroll
+      for (int u = 0; u < kGssUnroll; ++u) {
+        if (at + u * kGssThreads >= chunk.count) continue;
+        int64_t off = source.src_offset;
+#pragma unroll
+        for (int d = 0; d < TT_MAX_DIMS; ++d)
+          if (d < nd) off += static_cast<int64_t>(idx[u][d]) * source.src_stride[d];
+        value[u] = Scale<T>::fma(value[u], source.coef_re, source.coef_im, src[off]);
+      }
     }
-    dst[doff] = acc;
+#pragma unroll
+    for (int u = 0; u < kGssUnroll; ++u)
+      if (at + u * kGssThreads < chunk.count) dst[doff[u]] = value[u];
   }
+}
+
+// The division constants of one extent d (1 <= d < 2^31): shift = 31 + ceil(log2 d),
+// magic = ceil(2^shift / d); then (n * magic) >> shift == n / d for all n < 2^31.
+void division_constants(uint32_t d, uint64_t *magic, uint32_t *shift) {
+  uint32_t l = 0;
+  while ((uint64_t(1) << l) < d) ++l;
+  *shift = 31 + l;
+  const unsigned __int128 numerator = static_cast<unsigned __int128>(1) << *shift;
+  *magic = static_cast<uint64_t>((numerator + d - 1) / d);
 }
 
 }  // namespace
 
 struct tt_gss_plan {
-  tt_gss_task *tasks = nullptr;
+  DevTask *tasks = nullptr;
   tt_gss_source *sources = nullptr;
   GssChunk *chunks = nullptr;
   int64_t num_tasks = 0, num_sources = 0, num_chunks = 0;
@@ -126,6 +166,7 @@ extern "C" int tt_gss_plan_create(const tt_gss_task *tasks, int64_t num_tasks, c
   if (!plan) return fail("tt_gss_plan_create: null output handle");
   if (num_tasks < 0 || num_sources < 0) return fail("tt_gss_plan_create: negative count");
   std::vector<GssChunk> chunks;
+  std::vector<DevTask> dev(static_cast<size_t>(num_tasks));
   std::vector<std::pair<int32_t, int32_t>> groups;
   int64_t written = 0, read = 0;
   int32_t current_group = -1;
@@ -140,6 +181,24 @@ extern "C" int tt_gss_plan_create(const tt_gss_task *tasks, int64_t num_tasks, c
       elements *= task.extent[d];
     }
     if (elements >= (int64_t(1) << 31)) return fail("tt_gss_plan_create: a tile exceeds 2^31 elements");
+    DevTask &d = dev[static_cast<size_t>(t)];
+    std::memset(&d, 0, sizeof(DevTask));
+    d.dst_offset = task.dst_offset;
+    d.ndim = task.ndim;
+    d.num_sources = task.num_sources;
+    d.first_source = task.first_source;
+    for (int k = 0; k < task.ndim; ++k) {
+      d.extent[k] = static_cast<uint32_t>(task.extent[k]);
+      division_constants(d.extent[k], &d.magic[k], &d.shift[k]);
+      d.dst_stride[k] = task.dst_stride[k];
+    }
+    if (task.num_sources > 0) {
+      const tt_gss_source &s0 = sources[task.first_source];
+      d.src_offset = s0.src_offset;
+      for (int k = 0; k < task.ndim; ++k) d.src_stride[k] = s0.src_stride[k];
+      d.coef_re = s0.coef_re;
+      d.coef_im = s0.coef_im;
+    }
     written += elements;
     read += elements * task.num_sources;
     if (task.group != current_group || groups.empty()) {
@@ -164,7 +223,7 @@ extern "C" int tt_gss_plan_create(const tt_gss_task *tasks, int64_t num_tasks, c
     TT_HIP(hipMemcpy(*where, data, bytes, hipMemcpyHostToDevice));
     return 0;
   };
-  if (upload(reinterpret_cast<void **>(&p->tasks), tasks, sizeof(tt_gss_task) * num_tasks) ||
+  if (upload(reinterpret_cast<void **>(&p->tasks), dev.data(), sizeof(DevTask) * dev.size()) ||
       upload(reinterpret_cast<void **>(&p->sources), sources, sizeof(tt_gss_source) * num_sources) ||
       upload(reinterpret_cast<void **>(&p->chunks), chunks.data(), sizeof(GssChunk) * chunks.size())) {
     std::string keep = ttdev::g_error;
