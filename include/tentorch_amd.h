@@ -71,6 +71,10 @@ int tt_gss_plan_counts(const tt_gss_plan *plan, int64_t *written, int64_t *read)
 int tt_gss_execute(const tt_gss_plan *plan, int dtype, const void *src, void *dst, void *stream);
 /* The per-block form: one launch per launch group (one source block into one target block). */
 int tt_gss_execute_plain(const tt_gss_plan *plan, int dtype, const void *src, void *dst, void *stream);
+/* All tiles in one launch, converting between element types on the way (TT_F64 <-> TT_F32, or the
+ * same type): the FP32 form stages FP64 operands into FP32 workspaces and back. */
+int tt_gss_execute_typed(const tt_gss_plan *plan, int src_type, int dst_type, const void *src, void *dst,
+                         void *stream);
 
 /*
  * One output block of a grouped GEMM: C[c_offset : m x n, row-major, ldc = n] =
@@ -127,6 +131,13 @@ int tt_axpby(int dtype, int64_t n, double alpha, const double *alpha_dev, const 
              const double *beta_dev, void *y, void *stream);
 /* y = x converted between TT_F64 and TT_F32. */
 int tt_convert(int from_type, int to_type, int64_t n, const void *x, void *y, void *stream);
+/* Krylov basis (FP64, rows of length n, `ldv` apart): out[i] = sum_j V[i][j] w[j] for i < k <= 32,
+ * each in a fixed order, left on the device.  Replaces the k separate `inner` calls of one
+ * reorthogonalisation pass (tentorch/linalg/lanczos.py:43-45). */
+int tt_multidot(int64_t n, int k, const double *V, int64_t ldv, const double *w, double *out, void *stream);
+/* w += scale * sum_{i < k} coef[i] V[i][:], coefficients read from device memory. */
+int tt_multiaxpy(int64_t n, int k, const double *coef, double scale, const double *V, int64_t ldv, double *w,
+                 void *stream);
 
 typedef struct tt_graph tt_graph;
 /* Begin capturing every launch on `stream` into a graph. */

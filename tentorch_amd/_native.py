@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 try:
     from . import _engine as engine  # noqa: F401
 except ImportError as error:  # pragma: no cover - exercised only on a broken install
-    raise ImportError('tentorch_amd: the host engine is not built; run `python -m tentorch_amd.build` '
+    raise ImportError('tentorch_amd: the host engine is not built; run `python tentorch_amd/build.py` '
                       f'({error})') from error
 
 _lib = ctypes.CDLL(os.path.join(HERE, 'libtentorch_hip.so'))
@@ -27,6 +27,8 @@ _lib.tt_axpby.argtypes = [_int, _i64, _dbl, _c_void_p, _c_void_p, _dbl, _c_void_
 _lib.tt_convert.argtypes = [_int, _int, _i64, _c_void_p, _c_void_p, _c_void_p]
 _lib.tt_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
 _lib.tt_version.argtypes = []
+_lib.tt_multidot.argtypes = [_i64, _int, _c_void_p, _i64, _c_void_p, _c_void_p, _c_void_p]
+_lib.tt_multiaxpy.argtypes = [_i64, _int, _c_void_p, _dbl, _c_void_p, _i64, _c_void_p, _c_void_p]
 _lib.tt_graph_begin.argtypes = [_c_void_p]
 _lib.tt_graph_end.argtypes = [_c_void_p, ctypes.POINTER(_c_void_p)]
 _lib.tt_graph_launch.argtypes = [_c_void_p, _c_void_p]
@@ -35,7 +37,8 @@ _lib.tt_graph_destroy.argtypes = [_c_void_p]
 EXPORTED = ['tt_gss_plan_create', 'tt_gss_plan_destroy', 'tt_gss_plan_counts', 'tt_gss_execute',
             'tt_gss_execute_plain', 'tt_ggemm_plan_create', 'tt_ggemm_plan_destroy', 'tt_ggemm_plan_counts',
             'tt_ggemm_execute', 'tt_ggemm_execute_plain', 'tt_ggemm_plan_create_config', 'tt_ggemm_num_configs',
-            'tt_ggemm_config_shape', 'tt_ggemm_plan_config', 'tt_dot', 'tt_axpby', 'tt_convert', 'tt_graph_begin',
+            'tt_ggemm_config_shape', 'tt_ggemm_plan_config', 'tt_dot', 'tt_axpby', 'tt_convert', 'tt_multidot',
+            'tt_multiaxpy', 'tt_graph_begin',
             'tt_graph_end', 'tt_graph_launch', 'tt_graph_destroy', 'tt_last_error', 'tt_version']
 
 
@@ -90,6 +93,18 @@ def axpby(dtype: torch.dtype, alpha, x, beta, y: torch.Tensor, alpha_dev=None, b
     check(_lib.tt_axpby(code, n, float(alpha), None if alpha_dev is None else alpha_dev.data_ptr(),
                         None if x is None else x.data_ptr(), float(beta),
                         None if beta_dev is None else beta_dev.data_ptr(), y.data_ptr(), stream()), 'axpby')
+
+
+def multidot(V: torch.Tensor, k: int, w: torch.Tensor, out: torch.Tensor) -> None:
+    '''out[:k] = V[:k] @ w (FP64 Krylov rows, fixed summation order, on the device).'''
+    check(_lib.tt_multidot(w.numel(), k, V.data_ptr(), V.stride(0), w.data_ptr(), out.data_ptr(), stream()),
+          'multidot')
+
+
+def multiaxpy(V: torch.Tensor, k: int, coef: torch.Tensor, scale: float, w: torch.Tensor) -> None:
+    '''w += scale * coef[:k] @ V[:k] with the coefficients on the device.'''
+    check(_lib.tt_multiaxpy(w.numel(), k, coef.data_ptr(), float(scale), V.data_ptr(), V.stride(0), w.data_ptr(),
+                            stream()), 'multiaxpy')
 
 
 def convert(x: torch.Tensor, y: torch.Tensor) -> None:

@@ -14,14 +14,18 @@ import torch
 from .linalg import lanczos, svd
 from .models import Model, heisenberg
 from .networks import MPS, identity
-from .tensor import Tensor, contract, domain_legs, einsum, inner
+from .tensor import Settings, Tensor, contract, domain_legs, einsum, inner
 
 
 class DMRG:
     '''`DMRG` (`tentorch/algorithms/dmrg.py:13-212`).'''
 
     def __init__(self, model: Model, max_bond_dim: int = 50, svd_tol: float = 1e-14, n_iter: int = 5,
-                 seed: int = 0, dtype=None, device=None) -> None:
+                 seed: int = 0, dtype=None, device=None, precision: str = 'fp64', refine: int = 2) -> None:
+        if precision not in ('fp64', 'fp32'):
+            raise ValueError(f"precision has to be 'fp64' or 'fp32', not {precision!r}")
+        self.precision = precision
+        self.refine = refine
         self.model = model
         self.max_bond_dim = max_bond_dim
         self.svd_tol = svd_tol
@@ -73,7 +77,15 @@ class DMRG:
         def action(vector: Tensor) -> Tensor:
             return einsum('EPaqj,JRSFb,AQsb->EPrf', head, tail, vector)
 
-        energy, vector = lanczos(action, two, self.n_iter)
+        if self.precision == 'fp32':
+            # The Krylov steps run their block products on FP32 MFMA; the Ritz vector is then
+            # corrected in FP64 by a short Lanczos restart from it with FP64 block products.
+            with Settings.using(precision='fp32'):
+                energy, vector = lanczos(action, two, self.n_iter)
+            if self.refine:
+                energy, vector = lanczos(action, vector, self.refine)
+        else:
+            energy, vector = lanczos(action, two, self.n_iter)
         factor = 2 if direction == 'right' else 1
         left, values, right = svd(vector, None, self.max_bond_dim, self.svd_tol, factor)
         self.state.tensors[at], self.state.tensors[at + 1] = left, right.place([0, 1])

@@ -6,8 +6,8 @@ In-tree build of the two native libraries of the package:
 - `_engine*.so`: the C++ host engine (categories, legs, trees, leg-move and contraction planning)
   with its Python binding, compiled by g++ and linked against the first.
 
-Both land next to this file, so they travel with the source tree.  `python -m tentorch_amd.build`
-rebuilds them.
+Both land next to this file, so they travel with the source tree.  `python tentorch_amd/build.py`
+rebuilds them (run it by path: importing the package loads the libraries being rebuilt).
 '''
 
 import os

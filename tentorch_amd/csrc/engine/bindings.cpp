@@ -279,15 +279,17 @@ PYBIND11_MODULE(_engine, m) {
            })
       .def("run",
            [](const PyProgram &p, int dtype, const std::vector<uintptr_t> &operands, uintptr_t out, uintptr_t workspace,
-              uintptr_t stream, int mode, uint32_t skip, int64_t first, int64_t last) {
+              uintptr_t stream, int mode, uint32_t skip, int64_t first, int64_t last, int ws_dtype, int compute) {
              std::vector<const void *> ptrs;
              for (uintptr_t x : operands) ptrs.push_back(reinterpret_cast<const void *>(x));
              py::gil_scoped_release release;
              p.p->run(dtype, ptrs, reinterpret_cast<void *>(out), reinterpret_cast<void *>(workspace),
-                      reinterpret_cast<void *>(stream), mode == 1 ? Mode::plain : Mode::batched, skip, first, last);
+                      reinterpret_cast<void *>(stream), mode == 1 ? Mode::plain : Mode::batched, skip, first, last,
+                      ws_dtype, compute);
            },
            py::arg("dtype"), py::arg("operands"), py::arg("out"), py::arg("workspace"), py::arg("stream"),
-           py::arg("mode") = 0, py::arg("skip") = 0, py::arg("first") = 0, py::arg("last") = -1)
+           py::arg("mode") = 0, py::arg("skip") = 0, py::arg("first") = 0, py::arg("last") = -1,
+           py::arg("ws_dtype") = -1, py::arg("compute") = -1)
       .def("op_costs",
            [](const PyProgram &p) {
              // Per op: (kind, algorithmic flops, elements written, elements read).

@@ -593,9 +593,13 @@ uint32_t Program::hoistable() const {
 }
 
 void Program::run(int dtype, const std::vector<const void *> &operand_ptrs, void *out_ptr, void *workspace_ptr,
-                  void *stream, Mode mode, uint32_t skip, int64_t first, int64_t last) const {
+                  void *stream, Mode mode, uint32_t skip, int64_t first, int64_t last, int ws_dtype, int compute) const {
   if (operand_ptrs.size() != operands.size()) throw std::invalid_argument("a program got the wrong number of operands");
-  const int64_t es = element_size(dtype);
+  if (ws_dtype < 0) ws_dtype = dtype;
+  if (compute < 0) compute = dtype;
+  if (mode == Mode::plain && (ws_dtype != dtype || compute != dtype))
+    throw std::invalid_argument("the per-block form runs in one precision");
+  const int64_t es = element_size(ws_dtype);
   char *ws = static_cast<char *>(workspace_ptr);
   auto ptr = [&](int32_t v) -> void * {
     const Slot &s = slots[static_cast<size_t>(v)];
@@ -605,6 +609,7 @@ void Program::run(int dtype, const std::vector<const void *> &operand_ptrs, void
       default: return ws + s.offset * es;
     }
   };
+  auto type_of = [&](int32_t v) { return slots[static_cast<size_t>(v)].kind == Slot::workspace ? ws_dtype : dtype; };
   if (workspace > 0 && !workspace_ptr) throw std::invalid_argument("a program needs its workspace");
   const int64_t end = last < 0 ? static_cast<int64_t>(ops.size()) : std::min<int64_t>(last, static_cast<int64_t>(ops.size()));
   for (int64_t at = std::max<int64_t>(first, 0); at < end; ++at) {
@@ -627,30 +632,32 @@ void Program::run(int dtype, const std::vector<const void *> &operand_ptrs, void
             from = to;
           }
         } else {
-          check(tt_gss_execute(op.move->device(), dtype, ptr(op.a), ptr(op.dst), stream), "leg move");
+          check(tt_gss_execute_typed(op.move->device(), type_of(op.a), type_of(op.dst), ptr(op.a), ptr(op.dst), stream),
+                "leg move");
         }
         break;
       case OpKind::gemm:
         if (mode == Mode::plain)
           check(tt_ggemm_execute_plain(op.gemm->device(), dtype, ptr(op.a), ptr(op.b), ptr(op.dst), stream), "block products");
         else
-          check(tt_ggemm_execute(op.gemm->device(), dtype, dtype, dtype, dtype, ptr(op.a), ptr(op.b), ptr(op.dst), stream),
+          check(tt_ggemm_execute(op.gemm->device(), type_of(op.a), type_of(op.b), type_of(op.dst), compute, ptr(op.a),
+                                 ptr(op.b), ptr(op.dst), stream),
                 "block products");
         break;
       case OpKind::copy:
-        if (dtype == TT_C128) {
-          check(tt_axpby(TT_F64, 2 * out->size(), 1.0, nullptr, ptr(op.a), 0.0, nullptr, ptr(op.dst), stream), "copy");
+      case OpKind::add: {
+        const int from = type_of(op.a), to = type_of(op.dst);
+        const double keep = op.kind == OpKind::add ? 1.0 : 0.0;
+        if (from != to) {
+          if (op.kind == OpKind::add) throw std::invalid_argument("a sum over charges runs in one precision");
+          check(tt_convert(from, to, out->size(), ptr(op.a), ptr(op.dst), stream), "copy");
+        } else if (dtype == TT_C128) {
+          check(tt_axpby(TT_F64, 2 * out->size(), 1.0, nullptr, ptr(op.a), keep, nullptr, ptr(op.dst), stream), "copy");
         } else {
-          check(tt_axpby(dtype, out->size(), 1.0, nullptr, ptr(op.a), 0.0, nullptr, ptr(op.dst), stream), "copy");
+          check(tt_axpby(from, out->size(), 1.0, nullptr, ptr(op.a), keep, nullptr, ptr(op.dst), stream), "copy");
         }
         break;
-      case OpKind::add:
-        if (dtype == TT_C128) {
-          check(tt_axpby(TT_F64, 2 * out->size(), 1.0, nullptr, ptr(op.a), 1.0, nullptr, ptr(op.dst), stream), "sum");
-        } else {
-          check(tt_axpby(dtype, out->size(), 1.0, nullptr, ptr(op.a), 1.0, nullptr, ptr(op.dst), stream), "sum");
-        }
-        break;
+      }
     }
   }
 }

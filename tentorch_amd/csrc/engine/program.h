@@ -74,8 +74,12 @@ class Program {
   // Run with device pointers.  `skip` is a mask of operands whose hoistable moves are still valid in
   // the workspace from the previous run of this program on the same workspace.
   // Ops [first, last) only when `last` >= 0 (for instrumented runs that time each launch).
+  // `dtype` is the type of the operands and of the output; `ws_dtype` the type of every workspace
+  // value and `compute` the MFMA type of the block products (both `dtype` when negative).  The FP32
+  // form runs FP64 operands with FP32 workspaces and FP32 block products.
   void run(int dtype, const std::vector<const void *> &operand_ptrs, void *out_ptr, void *workspace_ptr,
-           void *stream, Mode mode, uint32_t skip, int64_t first = 0, int64_t last = -1) const;
+           void *stream, Mode mode, uint32_t skip, int64_t first = 0, int64_t last = -1, int ws_dtype = -1,
+           int compute = -1) const;
   // Operands whose moves can be hoisted (moves that read one operand and write the workspace).
   uint32_t hoistable() const;
 };

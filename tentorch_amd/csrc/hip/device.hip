@@ -85,11 +85,11 @@ struct Scale<cdouble> {
 // Each workgroup takes one chunk of one tile.  A thread handles kGssUnroll elements per pass, 256
 // apart, so consecutive lanes write consecutive destination elements along the tile's fastest axis
 // and each thread keeps several independent loads in flight.
-template <typename T>
+template <typename TS, typename T>
 __global__ __launch_bounds__(kGssThreads) void gss_kernel(const GssChunk *__restrict__ chunks,
                                                           const DevTask *__restrict__ tasks,
                                                           const tt_gss_source *__restrict__ sources,
-                                                          const T *__restrict__ src, T *__restrict__ dst,
+                                                          const TS *__restrict__ src, T *__restrict__ dst,
                                                           int32_t chunk_base) {
   const GssChunk chunk = chunks[chunk_base + blockIdx.x];
   const DevTask &task = tasks[chunk.task];
@@ -120,7 +120,8 @@ __global__ __launch_bounds__(kGssThreads) void gss_kernel(const GssChunk *__rest
 #pragma unroll
     for (int u = 0; u < kGssUnroll; ++u)
       if (at + u * kGssThreads < chunk.count)
-        value[u] = ns > 0 ? Scale<T>::fma(Scale<T>::zero(), task.coef_re, task.coef_im, src[soff[u]]) : Scale<T>::zero();
+        value[u] = ns > 0 ? Scale<T>::fma(Scale<T>::zero(), task.coef_re, task.coef_im, static_cast<T>(src[soff[u]]))
+                          : Scale<T>::zero();
     for (int s = 1; s < ns; ++s) {
       const tt_gss_source &source = sources[task.first_source + s];
 #pragma unroll
@@ -130,7 +131,7 @@ __global__ __launch_bounds__(kGssThreads) void gss_kernel(const GssChunk *__rest
 #pragma unroll
         for (int d = 0; d < TT_MAX_DIMS; ++d)
           if (d < nd) off += static_cast<int64_t>(idx[u][d]) * source.src_stride[d];
-        value[u] = Scale<T>::fma(value[u], source.coef_re, source.coef_im, src[off]);
+        value[u] = Scale<T>::fma(value[u], source.coef_re, source.coef_im, static_cast<T>(src[off]));
       }
     }
 #pragma unroll
@@ -251,25 +252,38 @@ extern "C" int tt_gss_plan_counts(const tt_gss_plan *plan, int64_t *written, int
 }
 
 namespace {
-template <typename T>
+template <typename TS, typename T>
 int gss_launch(const tt_gss_plan *plan, const void *src, void *dst, hipStream_t stream, int32_t first, int32_t count) {
   if (count <= 0) return 0;
-  hipLaunchKernelGGL(gss_kernel<T>, dim3(count), dim3(kGssThreads), 0, stream, plan->chunks, plan->tasks,
-                     plan->sources, static_cast<const T *>(src), static_cast<T *>(dst), first);
+  hipLaunchKernelGGL((gss_kernel<TS, T>), dim3(count), dim3(kGssThreads), 0, stream, plan->chunks, plan->tasks,
+                     plan->sources, static_cast<const TS *>(src), static_cast<T *>(dst), first);
   TT_LAUNCHED();
   return 0;
 }
 
+int gss_dispatch(const tt_gss_plan *plan, int src_type, int dst_type, const void *src, void *dst, hipStream_t stream,
+                 int32_t first, int32_t count) {
+  if (src_type == TT_F64 && dst_type == TT_F64) return gss_launch<double, double>(plan, src, dst, stream, first, count);
+  if (src_type == TT_F32 && dst_type == TT_F32) return gss_launch<float, float>(plan, src, dst, stream, first, count);
+  if (src_type == TT_C128 && dst_type == TT_C128) return gss_launch<cdouble, cdouble>(plan, src, dst, stream, first, count);
+  if (src_type == TT_F64 && dst_type == TT_F32) return gss_launch<double, float>(plan, src, dst, stream, first, count);
+  if (src_type == TT_F32 && dst_type == TT_F64) return gss_launch<float, double>(plan, src, dst, stream, first, count);
+  return fail("gather-scale-scatter: unsupported types " + std::to_string(src_type) + " -> " + std::to_string(dst_type));
+}
+
 int gss_dispatch(const tt_gss_plan *plan, int dtype, const void *src, void *dst, hipStream_t stream, int32_t first,
                  int32_t count) {
-  switch (dtype) {
-    case TT_F64: return gss_launch<double>(plan, src, dst, stream, first, count);
-    case TT_F32: return gss_launch<float>(plan, src, dst, stream, first, count);
-    case TT_C128: return gss_launch<cdouble>(plan, src, dst, stream, first, count);
-    default: return fail("gather-scale-scatter: unsupported dtype " + std::to_string(dtype));
-  }
+  return gss_dispatch(plan, dtype, dtype, src, dst, stream, first, count);
 }
 }  // namespace
+
+extern "C" int tt_gss_execute_typed(const tt_gss_plan *plan, int src_type, int dst_type, const void *src, void *dst,
+                                    void *stream) {
+  if (!plan) return fail("tt_gss_execute_typed: null plan");
+  if (plan->num_chunks == 0) return 0;
+  if (!src || !dst) return fail("tt_gss_execute_typed: null buffer");
+  return gss_dispatch(plan, src_type, dst_type, src, dst, as_stream(stream), 0, static_cast<int32_t>(plan->num_chunks));
+}
 
 extern "C" int tt_gss_execute(const tt_gss_plan *plan, int dtype, const void *src, void *dst, void *stream) {
   if (!plan) return fail("tt_gss_execute: null plan");
@@ -464,6 +478,105 @@ extern "C" int tt_convert(int from_type, int to_type, int64_t n, const void *x, 
                        static_cast<const float *>(x), static_cast<double *>(y));
   else
     return fail("tt_convert: unsupported conversion");
+  TT_LAUNCHED();
+  return 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Krylov-basis kernels: the k scalar products of one vector with k basis rows, and the update of a
+// vector by a combination of basis rows, both with coefficients that stay on the device.
+
+namespace {
+
+constexpr int kMultiBlocks = 256;
+constexpr int kMultiMax = 32;
+
+std::vector<double *> g_multi_scratch;
+
+int multi_scratch(double **out) {
+  int device = 0;
+  TT_HIP(hipGetDevice(&device));
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (static_cast<int>(g_multi_scratch.size()) <= device) g_multi_scratch.resize(static_cast<size_t>(device) + 1, nullptr);
+  double *&s = g_multi_scratch[static_cast<size_t>(device)];
+  if (!s) TT_HIP(hipMalloc(reinterpret_cast<void **>(&s), sizeof(double) * kMultiMax * kMultiBlocks));
+  *out = s;
+  return 0;
+}
+
+// Block b sums V[i, j] w[j] over its strided share of j for every row i < k (fixed order), then a
+// fixed LDS tree per row; partial[i * kMultiBlocks + b].
+__global__ __launch_bounds__(kDotThreads) void multidot_stage1(int64_t n, int k, const double *__restrict__ V, int64_t ldv,
+                                                               const double *__restrict__ w, double *__restrict__ partial) {
+  __shared__ double sh[kDotThreads];
+  double acc[kMultiMax];
+#pragma unroll
+  for (int i = 0; i < kMultiMax; ++i) acc[i] = 0.0;
+  for (int64_t j = static_cast<int64_t>(blockIdx.x) * kDotThreads + threadIdx.x; j < n;
+       j += static_cast<int64_t>(kMultiBlocks) * kDotThreads) {
+    const double x = w[j];
+#pragma unroll
+    for (int i = 0; i < kMultiMax; ++i)
+      if (i < k) acc[i] += V[i * ldv + j] * x;
+  }
+  for (int i = 0; i < k; ++i) {
+    sh[threadIdx.x] = acc[i];
+    __syncthreads();
+    for (int s = kDotThreads / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[i * kMultiBlocks + blockIdx.x] = sh[0];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kDotThreads) void multidot_stage2(const double *__restrict__ partial, double *__restrict__ out) {
+  __shared__ double sh[kDotThreads];
+  const int i = blockIdx.x;
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < kMultiBlocks; b += kDotThreads) acc += partial[i * kMultiBlocks + b];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kDotThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[i] = sh[0];
+}
+
+__global__ void multiaxpy_kernel(int64_t n, int k, const double *__restrict__ coef, double scale,
+                                 const double *__restrict__ V, int64_t ldv, double *__restrict__ w) {
+  for (int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < n;
+       j += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    double acc = 0.0;
+    for (int i = 0; i < k; ++i) acc += coef[i] * V[i * ldv + j];
+    w[j] += scale * acc;
+  }
+}
+
+}  // namespace
+
+extern "C" int tt_multidot(int64_t n, int k, const double *V, int64_t ldv, const double *w, double *out, void *stream) {
+  if (n < 0 || k < 0 || k > kMultiMax) return fail("tt_multidot: k must lie in [0, 32] and n be non-negative");
+  if (k == 0) return 0;
+  if (!V || !w || !out) return fail("tt_multidot: null buffer");
+  double *partial = nullptr;
+  if (int rc = multi_scratch(&partial)) return rc;
+  hipLaunchKernelGGL(multidot_stage1, dim3(kMultiBlocks), dim3(kDotThreads), 0, as_stream(stream), n, k, V, ldv, w, partial);
+  TT_LAUNCHED();
+  hipLaunchKernelGGL(multidot_stage2, dim3(k), dim3(kDotThreads), 0, as_stream(stream), partial, out);
+  TT_LAUNCHED();
+  return 0;
+}
+
+extern "C" int tt_multiaxpy(int64_t n, int k, const double *coef, double scale, const double *V, int64_t ldv, double *w,
+                            void *stream) {
+  if (n < 0 || k < 0) return fail("tt_multiaxpy: negative size");
+  if (n == 0 || k == 0) return 0;
+  if (!V || !w || !coef) return fail("tt_multiaxpy: null buffer");
+  hipLaunchKernelGGL(multiaxpy_kernel, dim3(elementwise_grid(n)), dim3(256), 0, as_stream(stream), n, k, coef, scale, V,
+                     ldv, w);
   TT_LAUNCHED();
   return 0;
 }

@@ -350,6 +350,8 @@ int ggemm_dispatch(const tt_ggemm_plan *plan, int at, int bt, int ct, int comp, 
   TT_CASE(TT_F64, TT_F64, TT_F64, TT_F32, double, double, double, float)
   TT_CASE(TT_F64, TT_F64, TT_F32, TT_F32, double, double, float, float)
   TT_CASE(TT_F32, TT_F64, TT_F64, TT_F32, float, double, double, float)
+  TT_CASE(TT_F64, TT_F32, TT_F32, TT_F32, double, float, float, float)
+  TT_CASE(TT_F64, TT_F32, TT_F64, TT_F32, double, float, double, float)
 #undef TT_CASE
   return fail("grouped GEMM: unsupported type combination (a " + std::to_string(at) + ", b " + std::to_string(bt) +
               ", c " + std::to_string(ct) + ", compute " + std::to_string(comp) + ")");

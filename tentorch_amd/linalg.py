@@ -229,6 +229,13 @@ def rq(tensor: Tensor, left: Optional[Sequence[int]] = None) -> Tuple[Tensor, Te
     return _split_blocks(arranged, pieces)
 
 
+class LanczosSettings:
+    '''`device` runs the iteration with the Krylov basis and its scalars on the GPU (one host
+    synchronisation per call besides the start-norm check); `host` follows the reference's loop with
+    one synchronisation per scalar product.'''
+    mode = 'device'
+
+
 def lanczos(action: Callable[[Tensor], Tensor], start: Tensor, n_iter: int = 5,
             tol: float = 1e-12) -> Tuple[float, Tensor]:
     '''
@@ -237,6 +244,69 @@ def lanczos(action: Callable[[Tensor], Tensor], start: Tensor, n_iter: int = 5,
     '''
     if n_iter < 1:
         raise ValueError(f'Lanczos needs at least one step, not {n_iter}.')
+    if LanczosSettings.mode == 'device' and start.dtype == torch.float64 and n_iter + 1 <= 32:
+        return _lanczos_device(action, start, n_iter, tol)
+    return _lanczos_host(action, start, n_iter, tol)
+
+
+def _lanczos_device(action, start: Tensor, n_iter: int, tol: float) -> Tuple[float, Tensor]:
+    '''The same recurrence with every scalar on the device.  Each Krylov step: the action, alpha =
+    <v_k, Hv_k>, w -= alpha v_k + beta_{k-1} v_{k-1}, two classical Gram-Schmidt passes against the
+    basis (the reference runs modified Gram-Schmidt, `lanczos.py:43-45`: equal up to rounding), beta =
+    |w|, v_{k+1} = w / beta.  The alphas and betas are read back once; the break at beta < tol
+    (`lanczos.py:46-48`) is applied to what was read.'''
+    scale = norm(start)
+    if scale == 0.0:
+        raise ValueError('Lanczos needs a starting vector of non-zero norm.')
+    size_n = start.structure.size
+    device = start.device
+    V = torch.empty((n_iter + 1, size_n), dtype=torch.float64, device=device)
+    scalars = torch.zeros(4 * (n_iter + 2), dtype=torch.float64, device=device)
+    alphas, betas = scalars[:n_iter], scalars[n_iter:2 * n_iter]
+    coef = scalars[2 * n_iter:3 * n_iter + 1]
+    work = scalars[3 * n_iter + 2:]
+    _native.axpby(torch.float64, 1.0 / scale, start.data, 0.0, V[0])
+    for k in range(n_iter):
+        vector = Tensor._wrap(start.structure, V[k])
+        image = action(vector)
+        if image.structure != start.structure:
+            raise ValueError('the action has to map the legs of the start vector to themselves')
+        w = image.data if image.data.data_ptr() != V[k].data_ptr() else image.data.clone()
+        _native.dot(torch.float64, V[k], w, alphas[k:k + 1])
+        _native.multiaxpy(V[k:k + 1], 1, alphas[k:k + 1], -1.0, w)
+        if k > 0:
+            _native.multiaxpy(V[k - 1:k], 1, betas[k - 1:k], -1.0, w)
+        for _ in range(2):
+            _native.multidot(V, k + 1, w, coef)
+            _native.multiaxpy(V, k + 1, coef, -1.0, w)
+        _native.dot(torch.float64, w, w, work[0:1])
+        torch.sqrt(work[0:1], out=betas[k:k + 1])
+        torch.reciprocal(betas[k:k + 1], out=work[1:2])
+        _native.axpby(torch.float64, 0.0, w, 0.0, V[k + 1], alpha_dev=work[1:2])
+    host = scalars[:2 * n_iter].cpu().numpy()
+    a, b = host[:n_iter], host[n_iter:]
+    size = n_iter
+    for k in range(n_iter):
+        if b[k] < tol:
+            size = k + 1
+            break
+    matrix = np.zeros((size, size))
+    for step in range(size):
+        matrix[step, step] = a[step]
+    for step in range(size - 1):
+        matrix[step, step + 1] = matrix[step + 1, step] = b[step]
+    values, vectors = np.linalg.eigh(matrix)
+    coef[:size].copy_(torch.from_numpy(vectors[:, 0].copy()))
+    out = torch.zeros(size_n, dtype=torch.float64, device=device)
+    _native.multiaxpy(V, size, coef, 1.0, out)
+    _native.dot(torch.float64, out, out, work[0:1])
+    torch.rsqrt(work[0:1], out=work[1:2])
+    _native.axpby(torch.float64, 0.0, out, 0.0, out, alpha_dev=work[1:2])
+    return float(values[0]), Tensor._wrap(start.structure, out)
+
+
+def _lanczos_host(action, start: Tensor, n_iter: int, tol: float) -> Tuple[float, Tensor]:
+    '''The reference's loop with host scalars (`lanczos.py:27-64`).'''
     scale = norm(start)
     if scale == 0.0:
         raise ValueError('Lanczos needs a starting vector of non-zero norm.')

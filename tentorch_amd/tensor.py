@@ -270,8 +270,26 @@ workspaces = _Workspaces()
 
 class Settings:
     '''How programs run: `mode` 0 batched (composed moves, grouped GEMM, hoisting), 1 plain (the
-    reference's per-step, per-block launches, nothing hoisted).'''
+    reference's per-step, per-block launches, nothing hoisted); `precision` 'fp64', or 'fp32' to run
+    the block products of FP64 operands on FP32 MFMA with FP32 workspaces (the result stays FP64).'''
     mode = 0
+    precision = 'fp64'
+
+    class using:
+        '''Context manager that sets the fields given and restores them on exit.'''
+
+        def __init__(self, **fields):
+            self.fields = fields
+            self.saved = {}
+
+        def __enter__(self):
+            for name, value in self.fields.items():
+                self.saved[name] = getattr(Settings, name)
+                setattr(Settings, name, value)
+
+        def __exit__(self, *exc):
+            for name, value in self.saved.items():
+                setattr(Settings, name, value)
 
 
 def run(program, operands: Sequence[Tensor], out: Optional[torch.Tensor] = None, label: str = '') -> Tensor:
@@ -286,10 +304,12 @@ def run(program, operands: Sequence[Tensor], out: Optional[torch.Tensor] = None,
     if out is None:
         out = torch.empty(result.size, dtype=dtype, device=device)
     mode = Settings.mode
+    mixed = Settings.precision == 'fp32' and dtype == torch.float64 and mode == 0
+    ws_dtype = torch.float32 if mixed else dtype
     elements = program.workspace + (program.plain_workspace if mode == 1 else 0)
     skip = 0
     if elements:
-        entry = workspaces.get(program, dtype, device, elements)
+        entry = workspaces.get(program, ws_dtype, device, elements)
         hoist = program.hoistable
         identity = tuple((t._uid, t._version) for t in operands)
         if mode == 0 and hoist and entry[1] is not None:
@@ -302,6 +322,8 @@ def run(program, operands: Sequence[Tensor], out: Optional[torch.Tensor] = None,
         ws = 0
     args = (_native.dtype_code(dtype), [t.data.data_ptr() for t in operands], out.data_ptr(), ws, _native.stream(),
             mode, skip)
+    if mixed:
+        args = args + (0, -1, _native.TT_F32, _native.TT_F32)
     if Profiler.active is None:
         program.run(*args)
     else:
@@ -334,12 +356,17 @@ class Profiler:
             if kind in ('gemm', 'move') and not hoisted:
                 start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 start.record()
-                program.run(*args, i, i + 1)
+                self._one(program, args, i)
                 end.record()
                 size = torch.empty((), dtype=torch.float64 if args[0] != 1 else torch.float32).element_size()
                 self.records.append((kind, flops, written, read, size, start, end, label))
             else:
-                program.run(*args, i, i + 1)
+                self._one(program, args, i)
+
+    @staticmethod
+    def _one(program, args, i):
+        base, extra = args[:7], args[9:] if len(args) > 7 else ()
+        program.run(*base, i, i + 1, *extra)
 
     def summary(self, label=None):
         '''Totals per kind, over every record or over the records of one einsum equation.'''

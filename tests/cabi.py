@@ -43,6 +43,7 @@ def load():
     lib.tt_gss_plan_counts.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
     lib.tt_gss_execute.argtypes = [vp, i32, vp, vp, vp]
     lib.tt_gss_execute_plain.argtypes = [vp, i32, vp, vp, vp]
+    lib.tt_gss_execute_typed.argtypes = [vp, i32, i32, vp, vp, vp]
     lib.tt_ggemm_plan_create.argtypes = [ctypes.POINTER(GemmGroup), i64, ctypes.POINTER(GemmTerm), i64, ctypes.POINTER(vp)]
     lib.tt_ggemm_plan_create_config.argtypes = [ctypes.POINTER(GemmGroup), i64, ctypes.POINTER(GemmTerm), i64, i32, ctypes.POINTER(vp)]
     lib.tt_ggemm_config_shape.argtypes = [i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
@@ -54,6 +55,8 @@ def load():
     lib.tt_dot.argtypes = [i32, i64, vp, vp, vp, vp]
     lib.tt_axpby.argtypes = [i32, i64, ctypes.c_double, vp, vp, ctypes.c_double, vp, vp, vp]
     lib.tt_convert.argtypes = [i32, i32, i64, vp, vp, vp]
+    lib.tt_multidot.argtypes = [i64, i32, vp, i64, vp, vp, vp]
+    lib.tt_multiaxpy.argtypes = [i64, i32, vp, ctypes.c_double, vp, i64, vp, vp]
     lib.tt_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
     return lib
 

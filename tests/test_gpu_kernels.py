@@ -146,3 +146,42 @@ def test_leg_move_tables_are_bit_exact_against_the_oracle(lib):
             assert lib.tt_gss_execute(plan, cabi.TT_F64, src.data_ptr(), dst.data_ptr(), _stream()) == 0
             assert np.array_equal(dst.cpu().numpy(), ref), (above, below)
             lib.tt_gss_plan_destroy(plan)
+
+
+def test_krylov_kernels_match_numpy(lib):
+    '''tt_multidot / tt_multiaxpy against numpy on a 7-row basis.'''
+    rng = np.random.default_rng(8)
+    n, k = 100003, 7
+    V = torch.tensor(rng.standard_normal((k + 2, n)), device='cuda')
+    w = torch.tensor(rng.standard_normal(n), device='cuda')
+    out = torch.zeros(k, dtype=torch.float64, device='cuda')
+    assert lib.tt_multidot(n, k, V.data_ptr(), V.stride(0), w.data_ptr(), out.data_ptr(), _stream()) == 0, cabi.error(lib)
+    ref = V[:k].cpu().numpy() @ w.cpu().numpy()
+    assert np.abs(out.cpu().numpy() - ref).max() <= 1e-12 * np.abs(ref).max()
+    before = w.clone()
+    assert lib.tt_multiaxpy(n, k, out.data_ptr(), -0.5, V.data_ptr(), V.stride(0), w.data_ptr(), _stream()) == 0
+    ref_w = before.cpu().numpy() - 0.5 * (out.cpu().numpy() @ V[:k].cpu().numpy())
+    assert np.abs(w.cpu().numpy() - ref_w).max() <= 1e-12 * np.abs(ref_w).max()
+
+
+def test_typed_leg_move_converts_between_fp64_and_fp32(lib):
+    '''A composed move that reads FP64 and writes FP32 equals the FP64 move rounded to FP32.'''
+    from tentorch_amd._native import engine
+    from tentorch_amd import u1, Leg
+    rng = np.random.default_rng(9)
+    bond = ([(-2,), (0,), (2,)], [2, 3, 2])
+    phys = ([(-1,), (1,)], [1, 1])
+    s = engine.Structure(u1, [Leg(u1, *bond), Leg(u1, *phys)], [Leg(u1, *phys), Leg(u1, *bond)], (0,))
+    program = engine.plan_place(s, [3, 0], [1, 2])
+    tasks, sources = program.move_tables(0)
+    tt, ss = cabi.gss_tables(tasks, sources)
+    plan = ctypes.c_void_p()
+    assert lib.tt_gss_plan_create(tt, len(tasks), ss, len(sources), ctypes.byref(plan)) == 0
+    src = torch.tensor(rng.standard_normal(s.size), device='cuda')
+    full = torch.zeros(program.out.size, dtype=torch.float64, device='cuda')
+    low = torch.zeros(program.out.size, dtype=torch.float32, device='cuda')
+    back = torch.zeros(s.size, dtype=torch.float64, device='cuda')
+    assert lib.tt_gss_execute(plan, cabi.TT_F64, src.data_ptr(), full.data_ptr(), _stream()) == 0
+    assert lib.tt_gss_execute_typed(plan, cabi.TT_F64, cabi.TT_F32, src.data_ptr(), low.data_ptr(), _stream()) == 0
+    assert torch.equal(low, full.float())
+    lib.tt_gss_plan_destroy(plan)
