@@ -1,14 +1,17 @@
 '''
 Seconds per two-site DMRG sweep of the U(1) Heisenberg chain on one MI355X: the batched form
-(one composed leg move and one grouped FP64-MFMA launch per contraction step, head/tail moves
-hoisted out of the Krylov loop) against the plain per-block form (the reference's per-step,
-per-block launches), at L = 64, chi = 1024, n_iter = 5, FP64, seed 0.
+(one composed leg move and one grouped MFMA launch per contraction step, head/tail moves hoisted out
+of the Krylov loop) against the plain per-block FP64 form (the reference's per-step, per-block
+launches), at L = 64, chi = 1024, n_iter = 5, seed 0.  `--precision fp32` runs the batched form's
+Lanczos block products in FP32 on MFMA, followed by a two-step FP64 Lanczos restart from the Ritz
+vector (defect correction); `fp64` keeps every product in FP64.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--length L] [--bond CHI]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--length L] [--bond CHI] [--precision P]
 
 A step is one sweep.  W warmup sweeps ramp the bond (the state starts with one multiplet per
-sector), then K sweeps are timed between synchronisations.  The plain form then sweeps once from the
-same state, and |dE| between the two forms' first timed sweep is reported.  Every block-product
+sector), then K sweeps are timed between synchronisations.  The plain FP64 form then sweeps once from
+the same state, and |dE| between the two forms' first timed sweep is reported (for the FP32 form this
+is |E - E_FP64|), with the truncated weight of both.  Every block-product
 launch of the timed sweeps is timed with HIP events on its stream; the grouped GEMM is the dominant
 kernel and its roofline line uses the algorithmic 2mnk of its descriptor table over its average
 launch time.  The CPU baseline times the oracle (the numpy restatement of the reference engine) on
@@ -27,6 +30,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
 
@@ -98,6 +102,9 @@ def main() -> int:
     parser.add_argument('--length', type=int, default=64)
     parser.add_argument('--bond', type=int, default=1024)
     parser.add_argument('--iterations', type=int, default=5)
+    parser.add_argument('--precision', choices=('fp64', 'fp32'), default='fp32',
+                        help='block products of the batched form: fp32 (with FP64 defect correction) or fp64')
+    parser.add_argument('--refine', type=int, default=2, help='FP64 Lanczos steps after the FP32 solve')
     parser.add_argument('--no-plain', action='store_true', help='skip the plain per-block sweep')
     parser.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     parser.add_argument('--no-profile', action='store_true', help='do not time launches with events')
@@ -118,8 +125,9 @@ def main() -> int:
     from tentorch_amd.tensor import Profiler, Settings, workspaces
 
     log(f'[bench] device {torch.cuda.get_device_name(0)}; L={args.length} chi={args.bond} '
-        f'n_iter={args.iterations} warmup={args.warmup} steps={args.steps}')
-    driver = tt.DMRG(tt.Model(args.length, cat=tt.u1), max_bond_dim=args.bond, n_iter=args.iterations, seed=0)
+        f'n_iter={args.iterations} precision={args.precision} warmup={args.warmup} steps={args.steps}')
+    driver = tt.DMRG(tt.Model(args.length, cat=tt.u1), max_bond_dim=args.bond, n_iter=args.iterations, seed=0,
+                     precision=args.precision, refine=args.refine)
     # A progress line every few windows, so a long sweep is visibly alive.
     update = driver.update
     windows = {'count': 0, 'start': time.perf_counter()}
@@ -140,7 +148,7 @@ def main() -> int:
     snapshot = ([t.clone() for t in driver.state.tensors], driver.state.center)
 
     profiler = Profiler()
-    times, energies = [], []
+    times, energies, truncated = [], [], []
     torch.cuda.synchronize()
     total_start = time.perf_counter()
     for sweep in range(args.steps):
@@ -153,6 +161,7 @@ def main() -> int:
         torch.cuda.synchronize()
         times.append(time.perf_counter() - start)
         energies.append(energy)
+        truncated.append(lost)
         log(f'[bench] timed sweep {sweep + 1}: {times[-1]:.2f} s energy {energy:.12f} truncated {lost:.2e} '
             f'max bond {max(driver.state.bond_dims)}')
     total = time.perf_counter() - total_start
@@ -163,29 +172,36 @@ def main() -> int:
         breakdown = time_breakdown(driver)
         log(f'[bench] breakdown of one sweep: {json.dumps(breakdown)}')
 
-    plain_per_sweep, plain_energy, delta = None, None, None
+    plain_per_sweep, plain_energy, delta, plain_lost = None, None, None, None
     if not args.no_plain:
         state = driver.state
         state.tensors = [t.clone() for t in snapshot[0]]
         state.center = snapshot[1]
         workspaces.clear()
         Settings.mode = 1
+        precision = driver.precision
+        driver.precision = 'fp64'
         try:
             torch.cuda.synchronize()
             start = time.perf_counter()
-            plain_energy, _ = driver.sweep()
+            plain_energy, plain_lost = driver.sweep()
             torch.cuda.synchronize()
             plain_per_sweep = time.perf_counter() - start
         finally:
             Settings.mode = 0
+            driver.precision = precision
         delta = abs(plain_energy - energies[0])
-        log(f'[bench] plain sweep: {plain_per_sweep:.2f} s energy {plain_energy:.12f} |dE| {delta:.2e}')
+        log(f'[bench] plain sweep: {plain_per_sweep:.2f} s energy {plain_energy:.12f} |dE| {delta:.2e} '
+            f'truncated {plain_lost:.2e}')
 
     roofline = None
     matvec = 'EPaqj,JRSFb,AQsb->EPrf'
-    summary = profiler.summary(matvec) if not args.no_profile else {}
+    compute = 'f32' if args.precision == 'fp32' else 'f64'
+    peak = PEAK_FP32_TFLOPS if compute == 'f32' else PEAK_FP64_TFLOPS
+    summary = profiler.summary(matvec, compute) if not args.no_profile else {}
     overall = profiler.summary() if not args.no_profile else {}
-    for name, part in (('all', overall), ('matvec', summary)):
+    refine = profiler.summary(matvec, 'f64') if not args.no_profile and compute == 'f32' else {}
+    for name, part in (('all', overall), ('matvec', summary), ('matvec fp64 refine', refine)):
         for kind in ('gemm', 'move'):
             if kind in part and part[kind]['ms'] > 0:
                 k = part[kind]
@@ -194,15 +210,17 @@ def main() -> int:
     if 'gemm' in summary and summary['gemm']['ms'] > 0:
         g = summary['gemm']
         achieved = g['flops'] / (g['ms'] * 1e-3) / 1e12
-        roofline = {'bound': 'mfma', 'kernel': 'ggemm_kernel<double,double,double,double> (matvec block products)',
-                    'achieved': round(achieved, 3), 'peak': PEAK_FP64_TFLOPS, 'unit': 'TFLOP/s',
-                    'frac': round(achieved / PEAK_FP64_TFLOPS, 4), 'traffic': None,
+        kernel = ('ggemm_kernel<float,float,float,float>' if compute == 'f32' else
+                  'ggemm_kernel<double,double,double,double>')
+        roofline = {'bound': 'mfma', 'kernel': f'{kernel} (matvec block products)',
+                    'achieved': round(achieved, 3), 'peak': peak, 'unit': 'TFLOP/s',
+                    'frac': round(achieved / peak, 4), 'traffic': None,
                     'launches': g['launches'], 'avg_launch_us': round(g['ms'] * 1e3 / g['launches'], 2),
                     'flops_per_launch': g['flops'] / g['launches']}
     if 'move' in summary and summary['move']['ms'] > 0 and roofline is not None:
         m = summary['move']
         gbs = m['bytes'] / (m['ms'] * 1e-3) / 1e9
-        roofline['moves'] = {'bound': 'hbm', 'kernel': 'gss_kernel<double> (matvec leg move)', 'achieved': round(gbs, 1),
+        roofline['moves'] = {'bound': 'hbm', 'kernel': 'gss_kernel (matvec leg moves)', 'achieved': round(gbs, 1),
                              'peak': PEAK_HBM_GBS, 'unit': 'GB/s', 'frac': round(gbs / PEAK_HBM_GBS, 4),
                              'launches': m['launches'], 'avg_launch_us': round(m['ms'] * 1e3 / m['launches'], 2),
                              'bytes_per_launch': m['bytes'] / m['launches']}
@@ -218,11 +236,15 @@ def main() -> int:
         'metric': 's/sweep U(1) Heisenberg L=64 chi=1024, batched vs plain, one GPU, |dE|<=1e-10',
         'value': round(per_sweep, 4), 'unit': 's/sweep', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
         'ms_per_step': round(per_sweep * 1e3, 1), 'higher_is_better': False, 'scaling': 'weak',
-        'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic (random initial MPS, seed 0)',
+        'vs_baseline': None, 'dtype': compute, 'data': 'synthetic (random initial MPS, seed 0)',
         'config': {'workload': f'two-site DMRG sweep, U(1) Heisenberg L={args.length} chi={args.bond} '
-                               f'n_iter={args.iterations} FP64, batched form',
-                   'length': args.length, 'bond': args.bond, 'n_iter': args.iterations},
-        'sweeps_s': [round(t, 4) for t in times], 'energies': energies,
+                               f'n_iter={args.iterations}, batched form, '
+                               + ('FP32 block products + FP64 Lanczos/SVD/defect correction'
+                                  if compute == 'f32' else 'FP64 throughout'),
+                   'length': args.length, 'bond': args.bond, 'n_iter': args.iterations,
+                   'precision': args.precision, 'refine': args.refine if compute == 'f32' else 0},
+        'sweeps_s': [round(t, 4) for t in times], 'energies': energies, 'truncated': truncated,
+        'plain_truncated': plain_lost,
         'plain_s_per_sweep': None if plain_per_sweep is None else round(plain_per_sweep, 4),
         'speedup_vs_plain': None if plain_per_sweep is None else round(plain_per_sweep / per_sweep, 3),
         'dE_plain_vs_batched': delta,

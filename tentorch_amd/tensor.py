@@ -350,7 +350,17 @@ class Profiler:
     def run(self, program, args, skip, mode, label=''):
         costs = program.op_costs()
         ops = program.ops()
-        for i, ((kind, flops, written, read), (_, _, _, _, inputs)) in enumerate(zip(costs, ops)):
+        slots = program.slots()
+        mixed = len(args) > 10
+        size_of = {_native.TT_F64: 8, _native.TT_F32: 4, _native.TT_C128: 16}
+        outer = size_of[args[0]]
+        inner = size_of[args[9]] if mixed else outer
+        compute = 'f32' if mixed and args[10] == _native.TT_F32 else {8: 'f64', 4: 'f32', 16: 'c128'}[outer]
+
+        def size(value):
+            return inner if value >= 0 and slots[value][0] == 'workspace' else outer
+
+        for i, ((kind, flops, written, read), (_, a, b, dst, inputs)) in enumerate(zip(costs, ops)):
             single = inputs and not (inputs & (inputs - 1))
             hoisted = mode == 0 and kind == 'move' and single and (inputs & skip)
             if kind in ('gemm', 'move') and not hoisted:
@@ -358,8 +368,8 @@ class Profiler:
                 start.record()
                 self._one(program, args, i)
                 end.record()
-                size = torch.empty((), dtype=torch.float64 if args[0] != 1 else torch.float32).element_size()
-                self.records.append((kind, flops, written, read, size, start, end, label))
+                moved = written * size(dst) + read * size(a)
+                self.records.append((kind, flops, moved, compute if kind == 'gemm' else '', start, end, label))
             else:
                 self._one(program, args, i)
 
@@ -368,18 +378,21 @@ class Profiler:
         base, extra = args[:7], args[9:] if len(args) > 7 else ()
         program.run(*base, i, i + 1, *extra)
 
-    def summary(self, label=None):
-        '''Totals per kind, over every record or over the records of one einsum equation.'''
+    def summary(self, label=None, compute=None):
+        '''Totals per kind, over every record or over the records of one einsum equation (and, for
+        block products, one compute type).'''
         torch.cuda.synchronize()
         out = {}
-        for kind, flops, written, read, size, start, end, lab in self.records:
+        for kind, flops, moved, kind_compute, start, end, lab in self.records:
             if label is not None and lab != label:
+                continue
+            if compute is not None and kind == 'gemm' and kind_compute != compute:
                 continue
             entry = out.setdefault(kind, {'launches': 0, 'ms': 0.0, 'flops': 0.0, 'bytes': 0.0})
             entry['launches'] += 1
             entry['ms'] += start.elapsed_time(end)
             entry['flops'] += flops
-            entry['bytes'] += (written + read) * size
+            entry['bytes'] += moved
         return out
 
 
