@@ -151,6 +151,9 @@ int tt_graph_destroy(tt_graph *graph);
 int tt_last_error(char *buffer, size_t size);
 /* Library version, for a loader to check. */
 int tt_version(void);
+/* Leg-move and block-product kernel launches issued by this process so far (diagnostics: the
+ * per-block form's launch count against the grouped form's).  No counterpart in the reference. */
+int64_t tt_launch_count(void);
 
 #ifdef __cplusplus
 }

@@ -27,6 +27,8 @@ _lib.tt_axpby.argtypes = [_int, _i64, _dbl, _c_void_p, _c_void_p, _dbl, _c_void_
 _lib.tt_convert.argtypes = [_int, _int, _i64, _c_void_p, _c_void_p, _c_void_p]
 _lib.tt_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
 _lib.tt_version.argtypes = []
+_lib.tt_launch_count.argtypes = []
+_lib.tt_launch_count.restype = ctypes.c_int64
 _lib.tt_multidot.argtypes = [_i64, _int, _c_void_p, _i64, _c_void_p, _c_void_p, _c_void_p]
 _lib.tt_multiaxpy.argtypes = [_i64, _int, _c_void_p, _dbl, _c_void_p, _i64, _c_void_p, _c_void_p]
 _lib.tt_graph_begin.argtypes = [_c_void_p]
@@ -39,11 +41,17 @@ EXPORTED = ['tt_gss_plan_create', 'tt_gss_plan_destroy', 'tt_gss_plan_counts', '
             'tt_ggemm_execute', 'tt_ggemm_execute_plain', 'tt_ggemm_plan_create_config', 'tt_ggemm_num_configs',
             'tt_ggemm_config_shape', 'tt_ggemm_plan_config', 'tt_dot', 'tt_axpby', 'tt_convert', 'tt_multidot',
             'tt_multiaxpy', 'tt_graph_begin',
-            'tt_graph_end', 'tt_graph_launch', 'tt_graph_destroy', 'tt_last_error', 'tt_version']
+            'tt_graph_end', 'tt_graph_launch', 'tt_graph_destroy', 'tt_last_error', 'tt_version',
+            'tt_launch_count']
 
 
 def library() -> ctypes.CDLL:
     return _lib
+
+
+def launch_count() -> int:
+    '''Leg-move and block-product kernel launches issued so far.'''
+    return int(_lib.tt_launch_count())
 
 
 def last_error() -> str:
@@ -128,7 +136,11 @@ class Graph:
 
 
 def capture(fn) -> Graph:
-    '''Run `fn` under stream capture and return the instantiated graph.'''
+    '''Run `fn` under stream capture and return the instantiated graph.  Capture needs a stream of
+    its own (`torch.cuda.stream(torch.cuda.Stream())`): the legacy default stream cannot be captured.
+    Everything `fn` launches must already have its device tables uploaded (run it once first).'''
+    if torch.cuda.current_stream() == torch.cuda.default_stream():
+        raise RuntimeError('graph capture needs a non-default current stream')
     s = stream()
     check(_lib.tt_graph_begin(s), 'graph capture')
     try:

@@ -13,6 +13,9 @@ namespace ttdev {
 // Records `message` as this thread's last error and returns the failure status.
 int fail(const std::string &message);
 
+// Counts one leg-move or block-product kernel launch (read back by tt_launch_count).
+void note_launch();
+
 inline hipStream_t as_stream(void *stream) { return reinterpret_cast<hipStream_t>(stream); }
 
 struct cdouble {

@@ -2,6 +2,7 @@
 // the gather-scale-scatter leg-move kernel, fixed-order reductions, elementwise updates, hipGraph
 // capture and the error channel.  The grouped block GEMM lives in ggemm.hip.  gfx950 only.
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -258,6 +259,7 @@ int gss_launch(const tt_gss_plan *plan, const void *src, void *dst, hipStream_t 
   hipLaunchKernelGGL((gss_kernel<TS, T>), dim3(count), dim3(kGssThreads), 0, stream, plan->chunks, plan->tasks,
                      plan->sources, static_cast<const TS *>(src), static_cast<T *>(dst), first);
   TT_LAUNCHED();
+  ttdev::note_launch();
   return 0;
 }
 
@@ -633,3 +635,11 @@ extern "C" int tt_last_error(char *buffer, size_t size) {
 }
 
 extern "C" int tt_version(void) { return 1; }
+
+namespace {
+std::atomic<int64_t> g_launches{0};
+}
+
+void ttdev::note_launch() { g_launches.fetch_add(1, std::memory_order_relaxed); }
+
+extern "C" int64_t tt_launch_count(void) { return g_launches.load(std::memory_order_relaxed); }

@@ -321,6 +321,7 @@ int ggemm_launch(const tt_ggemm_plan *plan, const void *a, const void *b, void *
                      0, stream, plan->tiles, plan->groups, plan->terms, static_cast<const TA *>(a),
                      static_cast<const TB *>(b), static_cast<TO *>(c), plain_group, plain_term, accumulate);
   TT_LAUNCHED();
+  ttdev::note_launch();
   return 0;
 }
 

@@ -185,3 +185,37 @@ def test_contract_errors_follow_the_reference():
         engine.plan_contract(a, a, [0], [1], None)
     with pytest.raises(ValueError, match='summed twice'):
         engine.plan_contract(a, a, [0, 0], [0, 2], None)
+
+
+def test_microbench_matvec_plan_matches_the_independent_counts():
+    '''The block-einsum microbenchmark's bond (27 sectors, chi = 1024) and its matvec plan: 29 block
+    products per contraction step, the largest (946, 198, 198), 0.898 GFLOP in all, theta of 286,650
+    entries and head/tail of 1,400,610 -- figures counted independently of this engine from the
+    sector arithmetic alone.'''
+    from tentorch_amd.microbench import EQ3, microbench_bond
+    from tentorch_amd.symmetry import Leg, u1
+    sectors, mults = microbench_bond()
+    assert len(sectors) == 27 and sum(mults) == 1024
+    assert dict((s[0], m) for s, m in zip(sectors, mults))[0] == 102
+    assert [dict((s[0], m) for s, m in zip(sectors, mults))[q] for q in (4, -4, 8, -8, 12, -12)] == [62, 62, 14, 14, 1, 1]
+    b = Leg(u1, sectors, mults)
+    M, P = Leg(u1, [(0,), (2,), (-2,)], (3, 1, 1)), Leg(u1, [(1,), (-1,)], (1, 1))
+
+    def S(c, d):
+        return engine.Structure(u1, c, d, (0,))
+
+    W = S([M, P], [P, M])
+    head = engine.plan_einsum('EIa,IPqj->EPaqj', [S([b, M], [b]), W], None).out
+    tail = engine.plan_einsum('JRsk,Fkb->JRSFb', [W, S([b], [M, b])], None).out
+    theta = S([b, P], [P, b])
+    program = engine.plan_einsum(EQ3, [head, tail, theta], None)
+    assert (theta.size, head.size, tail.size) == (286650, 1400610, 1400610)
+    assert program.op_kinds().count('gemm') == 2
+    shapes = []
+    for i in range(2):
+        groups, terms = program.gemm_tables(i)
+        assert len(groups) == 29 and len(terms) == 29
+        shapes += [(g[1], g[2], t[2]) for g in groups for t in terms[g[3]:g[3] + g[4]]]
+    assert max(shapes, key=lambda s: s[0] * s[1] * s[2]) in ((946, 198, 198), (198, 198, 946))
+    flops = sum(c[1] for c in program.op_costs() if c[0] == 'gemm')
+    assert abs(flops - 0.898e9) < 0.001e9

@@ -124,3 +124,31 @@ def test_fp32_form_with_defect_correction_tracks_fp64():
         energies[precision] = driver.energy().real
     assert abs(energies['fp32'] - exact_energy(8)) < 1e-10
     assert abs(energies['fp32'] - energies['fp64']) < 1e-10
+
+
+def test_captured_matvec_replays_match_eager_runs():
+    '''The grouped matvec captured into a hipGraph, replayed three times with new Krylov vectors
+    written into its input between replays (and eager work and synchronisations in between): every
+    replay equals an eager einsum of the same operands bit for bit.'''
+    from tentorch_amd import _native
+    from tentorch_amd.tensor import run
+    from tentorch_amd._native import engine
+    rng = np.random.default_rng(5)
+    _, head = _pair(rng, [BOND, PHYS], [BOND, PHYS, MPO])
+    _, tail = _pair(rng, [MPO, PHYS, PHYS, BOND], [BOND])
+    _, vec = _pair(rng, [BOND, PHYS], [PHYS, BOND])
+    eq = 'EPaqj,JRSFb,AQsb->EPrf'
+    program = engine.plan_einsum(eq, [head.structure, tail.structure, vec.structure], None)
+    out = torch.empty(program.out.size, dtype=torch.float64, device='cuda')
+    torch.cuda.synchronize()
+    with torch.cuda.stream(torch.cuda.Stream()):
+        run(program, (head, tail, vec), out=out)
+        graph = _native.capture(lambda: run(program, (head, tail, vec), out=out))
+        for replay in range(3):
+            vec.data.copy_(torch.from_numpy(rng.standard_normal(vec.data.numel())))
+            graph.launch()
+            torch.cuda.synchronize()
+            got = out.clone()
+            eager = tt.einsum(eq, head, tail, vec).data
+            torch.cuda.synchronize()
+            assert torch.equal(got, eager), f'replay {replay}'
