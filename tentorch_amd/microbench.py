@@ -12,7 +12,9 @@ Two forms of the same product:
   contracted beforehand (their layout moves are hoisted after the first call, as inside Lanczos);
 - `5`: the five-operand einsum `AIe,IPqj,JRsk,Dkf,EQSf->APrd` in one call.
 
-Each runs in the per-block form (one launch per block product and per target block of every
+Every call hands the product a new vector (the last operand), as each Krylov step does, so its
+layout move runs every time while the moves of the fixed operands are hoisted after the first
+call.  Each runs in the per-block form (one launch per block product and per target block of every
 elementary leg move, the reference's execution) and in the grouped form (one composed leg move and
 one grouped MFMA launch per contraction step), and, for the grouped 3-operand form, replayed from a
 captured hipGraph.  Times are HIP events on the launch stream around `--reps` calls after one warm
@@ -87,6 +89,14 @@ def _timed(fn, reps: int):
     return start.elapsed_time(end) / reps, (_native.launch_count() - before) / reps
 
 
+def fresh(ops):
+    '''The operands with the last one (the Krylov vector) wrapped anew, as every Lanczos step hands
+    the matvec a new vector: its layout move runs on every call, while the moves of the fixed
+    operands stay hoisted.'''
+    *fixed, vector = ops
+    return (*fixed, Tensor._wrap(vector.structure, vector.data))
+
+
 def case(form: int, precision: str, mode: str, reps: int, seed: int = 0) -> dict:
     '''One measured configuration; `mode` is plain, grouped or graph.'''
     dtype = torch.float32 if precision == 'fp32' else torch.float64
@@ -112,12 +122,12 @@ def case(form: int, precision: str, mode: str, reps: int, seed: int = 0) -> dict
             with torch.cuda.stream(torch.cuda.Stream()):
                 run(program, ops, out=out)          # uploads the tables, fills the hoisted moves
                 before = _native.launch_count()
-                graph = _native.capture(lambda: run(program, ops, out=out))
+                graph = _native.capture(lambda: run(program, fresh(ops), out=out))
                 launches = _native.launch_count() - before   # kernels recorded into the graph
                 ms, _ = _timed(graph.launch, reps)
             torch.cuda.synchronize()
         else:
-            ms, launches = _timed(lambda: run(program, ops, out=out), reps)
+            ms, launches = _timed(lambda: run(program, fresh(ops), out=out), reps)
         result = out.double().cpu().numpy()
     finally:
         Settings.mode = 0
