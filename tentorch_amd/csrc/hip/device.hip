@@ -591,6 +591,72 @@ struct tt_graph {
   hipGraphExec_t exec = nullptr;
 };
 
+// ---------------------------------------------------------------------------------------------
+// Launch gate: holds a stream until the host releases it, so that a start event, the kernel under
+// measurement and an end event enqueued meanwhile run back to back on the GPU (an event pair around
+// a lone launch on an idle GPU otherwise also times the host's launch latency).  The wait is bounded
+// by a wall-clock limit; a gate that times out lets the stream go and counts itself as expired.
+
+namespace {
+
+struct Gate {
+  unsigned int *ticket = nullptr;  // host memory the device polls (coherent, pinned)
+  unsigned int armed = 0;
+  unsigned int *expired = nullptr;  // device counter of gates that hit the limit
+};
+std::mutex g_gate_mutex;
+Gate g_gate;
+
+__global__ void gate_kernel(const unsigned int *ticket, unsigned int wanted, long long limit, unsigned int *expired) {
+  if (threadIdx.x != 0) return;
+  const long long start = wall_clock64();
+  while (__hip_atomic_load(ticket, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) < wanted) {
+    if (wall_clock64() - start > limit) {
+      atomicAdd(expired, 1u);
+      return;
+    }
+    __builtin_amdgcn_s_sleep(4);
+  }
+}
+
+}  // namespace
+
+extern "C" int tt_gate_arm(void *stream) {
+  std::lock_guard<std::mutex> lock(g_gate_mutex);
+  if (!g_gate.ticket) {
+    TT_HIP(hipHostMalloc(reinterpret_cast<void **>(&g_gate.ticket), sizeof(unsigned int),
+                         hipHostMallocCoherent | hipHostMallocMapped));
+    __atomic_store_n(g_gate.ticket, 0u, __ATOMIC_RELEASE);
+    TT_HIP(hipMalloc(reinterpret_cast<void **>(&g_gate.expired), sizeof(unsigned int)));
+    TT_HIP(hipMemset(g_gate.expired, 0, sizeof(unsigned int)));
+  }
+  unsigned int *device_ticket = nullptr;
+  TT_HIP(hipHostGetDevicePointer(reinterpret_cast<void **>(&device_ticket), g_gate.ticket, 0));
+  int rate_khz = 0;
+  TT_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, 0));
+  const long long limit = static_cast<long long>(rate_khz) * 50;  // 50 ms of wall clock
+  ++g_gate.armed;
+  hipLaunchKernelGGL(gate_kernel, dim3(1), dim3(64), 0, as_stream(stream), device_ticket, g_gate.armed, limit,
+                     g_gate.expired);
+  TT_LAUNCHED();
+  return 0;
+}
+
+extern "C" int tt_gate_release(void) {
+  std::lock_guard<std::mutex> lock(g_gate_mutex);
+  if (!g_gate.ticket) return fail("tt_gate_release: no gate was armed");
+  __atomic_store_n(g_gate.ticket, g_gate.armed, __ATOMIC_RELEASE);
+  return 0;
+}
+
+extern "C" int tt_gate_expired(unsigned int *count) {
+  if (!count) return fail("tt_gate_expired: null output");
+  *count = 0;
+  if (!g_gate.expired) return 0;
+  TT_HIP(hipMemcpy(count, g_gate.expired, sizeof(unsigned int), hipMemcpyDeviceToHost));
+  return 0;
+}
+
 extern "C" int tt_graph_begin(void *stream) {
   TT_HIP(hipStreamBeginCapture(as_stream(stream), hipStreamCaptureModeThreadLocal));
   return 0;

@@ -333,12 +333,16 @@ def run(program, operands: Sequence[Tensor], out: Optional[torch.Tensor] = None,
 
 class Profiler:
     '''Times every block-product and leg-move launch with HIP events on the launch stream, op by op,
-    while active; `summary()` synchronises and returns per-kind totals with their algorithmic work.'''
+    while active; `summary()` synchronises and returns per-kind totals with their algorithmic work.
+    Each timed launch is enqueued behind a launch gate, so that the start event, the kernel and the
+    end event run back to back on the device and the interval is the kernel's own duration (on an
+    idle device an event pair around a lone launch also spans the host's launch latency).'''
 
     active = None
 
-    def __init__(self):
-        self.records = []   # (kind, flops, elements written, elements read, dtype size, start, end)
+    def __init__(self, gated: bool = True):
+        self.gated = gated
+        self.records = []   # (kind, flops, bytes moved, compute type, start, end, label)
 
     def __enter__(self):
         Profiler.active = self
@@ -365,9 +369,13 @@ class Profiler:
             hoisted = mode == 0 and kind == 'move' and single and (inputs & skip)
             if kind in ('gemm', 'move') and not hoisted:
                 start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                if self.gated:
+                    _native.gate_arm()
                 start.record()
                 self._one(program, args, i)
                 end.record()
+                if self.gated:
+                    _native.gate_release()
                 moved = written * size(dst) + read * size(a)
                 self.records.append((kind, flops, moved, compute if kind == 'gemm' else '', start, end, label))
             else:
