@@ -100,7 +100,8 @@ def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     parser.add_argument('--gpus', type=int, default=1)
     parser.add_argument('--steps', type=int, default=2, help='timed sweeps')
-    parser.add_argument('--warmup', type=int, default=2, help='untimed sweeps that ramp the bond')
+    parser.add_argument('--warmup', type=int, default=4,
+                        help='untimed sweeps that ramp the bond and converge the state')
     parser.add_argument('--length', type=int, default=64)
     parser.add_argument('--bond', type=int, default=1024)
     parser.add_argument('--iterations', type=int, default=5)

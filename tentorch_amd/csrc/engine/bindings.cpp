@@ -290,6 +290,7 @@ PYBIND11_MODULE(_engine, m) {
            py::arg("dtype"), py::arg("operands"), py::arg("out"), py::arg("workspace"), py::arg("stream"),
            py::arg("mode") = 0, py::arg("skip") = 0, py::arg("first") = 0, py::arg("last") = -1,
            py::arg("ws_dtype") = -1, py::arg("compute") = -1)
+      .def("prepare", [](const PyProgram &p) { p.p->prepare(); })
       .def("op_costs",
            [](const PyProgram &p) {
              // Per op: (kind, algorithmic flops, elements written, elements read).

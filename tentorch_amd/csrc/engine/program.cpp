@@ -592,6 +592,13 @@ uint32_t Program::hoistable() const {
   return mask;
 }
 
+void Program::prepare() const {
+  for (const Op &op : ops) {
+    if (op.kind == OpKind::move) op.move->device();
+    if (op.kind == OpKind::gemm) op.gemm->device();
+  }
+}
+
 void Program::run(int dtype, const std::vector<const void *> &operand_ptrs, void *out_ptr, void *workspace_ptr,
                   void *stream, Mode mode, uint32_t skip, int64_t first, int64_t last, int ws_dtype, int compute) const {
   if (operand_ptrs.size() != operands.size()) throw std::invalid_argument("a program got the wrong number of operands");

@@ -82,6 +82,9 @@ class Program {
            int compute = -1) const;
   // Operands whose moves can be hoisted (moves that read one operand and write the workspace).
   uint32_t hoistable() const;
+  // Uploads the device tables of every grouped op now (they are otherwise uploaded on first run):
+  // before stream capture, and before timing single launches.
+  void prepare() const;
 };
 using ProgramPtr = std::shared_ptr<Program>;
 

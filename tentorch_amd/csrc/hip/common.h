@@ -16,6 +16,11 @@ int fail(const std::string &message);
 // Counts one leg-move or block-product kernel launch (read back by tt_launch_count).
 void note_launch();
 
+// Allocates `bytes` of device memory at *where and copies `data` there through a pinned staging
+// buffer on the library's own non-blocking upload stream, waiting for that copy alone: a plan
+// upload neither drains nor waits for the work queued on the caller's streams.
+int upload(void **where, const void *data, size_t bytes);
+
 inline hipStream_t as_stream(void *stream) { return reinterpret_cast<hipStream_t>(stream); }
 
 struct cdouble {

@@ -219,12 +219,7 @@ extern "C" int tt_gss_plan_create(const tt_gss_task *tasks, int64_t num_tasks, c
   p->written = written;
   p->read = read;
   p->groups = std::move(groups);
-  auto upload = [&](void **where, const void *data, size_t bytes) -> int {
-    if (!bytes) return 0;
-    TT_HIP(hipMalloc(where, bytes));
-    TT_HIP(hipMemcpy(*where, data, bytes, hipMemcpyHostToDevice));
-    return 0;
-  };
+  using ttdev::upload;
   if (upload(reinterpret_cast<void **>(&p->tasks), dev.data(), sizeof(DevTask) * dev.size()) ||
       upload(reinterpret_cast<void **>(&p->sources), sources, sizeof(tt_gss_source) * num_sources) ||
       upload(reinterpret_cast<void **>(&p->chunks), chunks.data(), sizeof(GssChunk) * chunks.size())) {
@@ -707,5 +702,40 @@ std::atomic<int64_t> g_launches{0};
 }
 
 void ttdev::note_launch() { g_launches.fetch_add(1, std::memory_order_relaxed); }
+
+namespace {
+
+struct Uploader {
+  hipStream_t stream = nullptr;
+  void *staging = nullptr;  // pinned host memory
+  size_t capacity = 0;
+};
+std::mutex g_upload_mutex;
+std::vector<Uploader> g_uploaders;  // one per device
+
+}  // namespace
+
+int ttdev::upload(void **where, const void *data, size_t bytes) {
+  if (!bytes) return 0;
+  TT_HIP(hipMalloc(where, bytes));
+  int device = 0;
+  TT_HIP(hipGetDevice(&device));
+  std::lock_guard<std::mutex> lock(g_upload_mutex);
+  if (g_uploaders.size() <= static_cast<size_t>(device)) g_uploaders.resize(static_cast<size_t>(device) + 1);
+  Uploader &u = g_uploaders[static_cast<size_t>(device)];
+  if (!u.stream) TT_HIP(hipStreamCreateWithFlags(&u.stream, hipStreamNonBlocking));
+  if (u.capacity < bytes) {
+    if (u.staging) TT_HIP(hipHostFree(u.staging));
+    u.staging = nullptr;
+    u.capacity = 0;
+    const size_t grown = std::max<size_t>(bytes, size_t(1) << 20);
+    TT_HIP(hipHostMalloc(&u.staging, grown, hipHostMallocDefault));
+    u.capacity = grown;
+  }
+  std::memcpy(u.staging, data, bytes);
+  TT_HIP(hipMemcpyAsync(*where, u.staging, bytes, hipMemcpyHostToDevice, u.stream));
+  TT_HIP(hipStreamSynchronize(u.stream));
+  return 0;
+}
 
 extern "C" int64_t tt_launch_count(void) { return g_launches.load(std::memory_order_relaxed); }

@@ -270,12 +270,7 @@ extern "C" int tt_ggemm_plan_create_config(const tt_gemm_group *groups, int64_t 
   p->config = config;
   p->flops = flops;
   p->host_groups.assign(groups, groups + num_groups);
-  auto upload = [&](void **where, const void *data, size_t bytes) -> int {
-    if (!bytes) return 0;
-    TT_HIP(hipMalloc(where, bytes));
-    TT_HIP(hipMemcpy(*where, data, bytes, hipMemcpyHostToDevice));
-    return 0;
-  };
+  using ttdev::upload;
   if (upload(reinterpret_cast<void **>(&p->groups), groups, sizeof(tt_gemm_group) * num_groups) ||
       upload(reinterpret_cast<void **>(&p->terms), terms, sizeof(tt_gemm_term) * num_terms) ||
       upload(reinterpret_cast<void **>(&p->tiles), tiles.data(), sizeof(GemmTile) * tiles.size())) {

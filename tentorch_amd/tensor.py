@@ -352,6 +352,8 @@ class Profiler:
         Profiler.active = None
 
     def run(self, program, args, skip, mode, label=''):
+        if self.gated and mode == 0:
+            program.prepare()   # table uploads wait on the device; none may happen behind a gate
         costs = program.op_costs()
         ops = program.ops()
         slots = program.slots()
@@ -369,12 +371,12 @@ class Profiler:
             hoisted = mode == 0 and kind == 'move' and single and (inputs & skip)
             if kind in ('gemm', 'move') and not hoisted:
                 start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                if self.gated:
+                if self.gated and mode == 0:
                     _native.gate_arm()
                 start.record()
                 self._one(program, args, i)
                 end.record()
-                if self.gated:
+                if self.gated and mode == 0:
                     _native.gate_release()
                 moved = written * size(dst) + read * size(a)
                 self.records.append((kind, flops, moved, compute if kind == 'gemm' else '', start, end, label))
