@@ -147,13 +147,14 @@ int tt_graph_end(void *stream, tt_graph **graph);
 int tt_graph_launch(const tt_graph *graph, void *stream);
 int tt_graph_destroy(tt_graph *graph);
 
-/* Launch gate for timing (no counterpart in the reference): `tt_gate_arm` enqueues a one-wave
- * kernel that holds `stream` until `tt_gate_release` (or 50 ms of wall clock, whichever comes
- * first), so that events and a kernel enqueued in between run back to back on the device.
- * `tt_gate_expired` reads how many gates hit the time limit (synchronises the device). */
-int tt_gate_arm(void *stream);
-int tt_gate_release(void);
-int tt_gate_expired(unsigned int *count);
+/* Kernel timing (no counterpart in the reference): while enabled on the calling thread, every
+ * leg-move and block-product launch is issued with a start and a stop event attached to the kernel
+ * dispatch itself (hipExtLaunchKernelGGL), so the interval is the kernel's execution alone.
+ * `tt_timing_collect` sets `count` to the number recorded; given an output of at least that many,
+ * it synchronises on them, writes the durations in launch order and forgets them (with a null output
+ * it only counts). */
+int tt_timing_enable(int on);
+int tt_timing_collect(double *milliseconds, int64_t capacity, int64_t *count);
 
 /* Copy the last error of this thread into `buffer` (NUL-terminated, truncated to `size`). */
 int tt_last_error(char *buffer, size_t size);

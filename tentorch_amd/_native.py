@@ -28,9 +28,8 @@ _lib.tt_convert.argtypes = [_int, _int, _i64, _c_void_p, _c_void_p, _c_void_p]
 _lib.tt_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
 _lib.tt_version.argtypes = []
 _lib.tt_launch_count.argtypes = []
-_lib.tt_gate_arm.argtypes = [_c_void_p]
-_lib.tt_gate_release.argtypes = []
-_lib.tt_gate_expired.argtypes = [ctypes.POINTER(ctypes.c_uint)]
+_lib.tt_timing_enable.argtypes = [_int]
+_lib.tt_timing_collect.argtypes = [ctypes.POINTER(_dbl), _i64, ctypes.POINTER(_i64)]
 _lib.tt_launch_count.restype = ctypes.c_int64
 _lib.tt_multidot.argtypes = [_i64, _int, _c_void_p, _i64, _c_void_p, _c_void_p, _c_void_p]
 _lib.tt_multiaxpy.argtypes = [_i64, _int, _c_void_p, _dbl, _c_void_p, _i64, _c_void_p, _c_void_p]
@@ -45,7 +44,7 @@ EXPORTED = ['tt_gss_plan_create', 'tt_gss_plan_destroy', 'tt_gss_plan_counts', '
             'tt_ggemm_config_shape', 'tt_ggemm_plan_config', 'tt_dot', 'tt_axpby', 'tt_convert', 'tt_multidot',
             'tt_multiaxpy', 'tt_graph_begin',
             'tt_graph_end', 'tt_graph_launch', 'tt_graph_destroy', 'tt_last_error', 'tt_version',
-            'tt_launch_count', 'tt_gate_arm', 'tt_gate_release', 'tt_gate_expired']
+            'tt_launch_count', 'tt_timing_enable', 'tt_timing_collect']
 
 
 def library() -> ctypes.CDLL:
@@ -57,20 +56,18 @@ def launch_count() -> int:
     return int(_lib.tt_launch_count())
 
 
-def gate_arm() -> None:
-    '''Hold the current stream until `gate_release` (at most 50 ms).'''
-    check(_lib.tt_gate_arm(stream()), 'gate')
+def timing(on: bool) -> None:
+    '''Attach kernel-dispatch events to every leg-move and block-product launch of this thread.'''
+    check(_lib.tt_timing_enable(1 if on else 0), 'timing')
 
 
-def gate_release() -> None:
-    check(_lib.tt_gate_release(), 'gate')
-
-
-def gates_expired() -> int:
-    '''Gates that hit their time limit so far (synchronises the device).'''
-    count = ctypes.c_uint(0)
-    check(_lib.tt_gate_expired(ctypes.byref(count)), 'gate')
-    return int(count.value)
+def timing_collect() -> list:
+    '''Durations (ms) of the launches timed since the last collection, in launch order.'''
+    count = _i64(0)
+    check(_lib.tt_timing_collect(None, 0, ctypes.byref(count)), 'timing')
+    buffer = (_dbl * max(1, count.value))()
+    check(_lib.tt_timing_collect(buffer, count.value, ctypes.byref(count)), 'timing')
+    return list(buffer[:count.value])
 
 
 def last_error() -> str:

@@ -2,6 +2,7 @@
 // tt_last_error, status checks, and the stream cast.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -15,6 +16,25 @@ int fail(const std::string &message);
 
 // Counts one leg-move or block-product kernel launch (read back by tt_launch_count).
 void note_launch();
+
+// Kernel timing (tt_timing_enable): when on for this thread, returns a fresh start/stop event pair
+// for the next launch and remembers it; otherwise both are null.
+struct TimedPair {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+int timed_pair(TimedPair *pair);
+
+// Launches `kernel` on `stream`, with the kernel-dispatch events of timing mode when it is on.
+template <typename Kernel, typename... Args>
+int launch(Kernel kernel, dim3 grid, dim3 block, uint32_t lds, hipStream_t stream, Args... args) {
+  TimedPair pair;
+  if (int rc = timed_pair(&pair)) return rc;
+  if (pair.start)
+    hipExtLaunchKernelGGL(kernel, grid, block, lds, stream, pair.start, pair.stop, 0, args...);
+  else
+    hipLaunchKernelGGL(kernel, grid, block, lds, stream, args...);
+  return 0;
+}
 
 // Allocates `bytes` of device memory at *where and copies `data` there through a pinned staging
 // buffer on the library's own non-blocking upload stream, waiting for that copy alone: a plan

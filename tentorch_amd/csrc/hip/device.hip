@@ -251,8 +251,9 @@ namespace {
 template <typename TS, typename T>
 int gss_launch(const tt_gss_plan *plan, const void *src, void *dst, hipStream_t stream, int32_t first, int32_t count) {
   if (count <= 0) return 0;
-  hipLaunchKernelGGL((gss_kernel<TS, T>), dim3(count), dim3(kGssThreads), 0, stream, plan->chunks, plan->tasks,
-                     plan->sources, static_cast<const TS *>(src), static_cast<T *>(dst), first);
+  if (int rc = ttdev::launch(gss_kernel<TS, T>, dim3(count), dim3(kGssThreads), 0, stream, plan->chunks, plan->tasks,
+                             plan->sources, static_cast<const TS *>(src), static_cast<T *>(dst), first))
+    return rc;
   TT_LAUNCHED();
   ttdev::note_launch();
   return 0;
@@ -586,72 +587,6 @@ struct tt_graph {
   hipGraphExec_t exec = nullptr;
 };
 
-// ---------------------------------------------------------------------------------------------
-// Launch gate: holds a stream until the host releases it, so that a start event, the kernel under
-// measurement and an end event enqueued meanwhile run back to back on the GPU (an event pair around
-// a lone launch on an idle GPU otherwise also times the host's launch latency).  The wait is bounded
-// by a wall-clock limit; a gate that times out lets the stream go and counts itself as expired.
-
-namespace {
-
-struct Gate {
-  unsigned int *ticket = nullptr;  // host memory the device polls (coherent, pinned)
-  unsigned int armed = 0;
-  unsigned int *expired = nullptr;  // device counter of gates that hit the limit
-};
-std::mutex g_gate_mutex;
-Gate g_gate;
-
-__global__ void gate_kernel(const unsigned int *ticket, unsigned int wanted, long long limit, unsigned int *expired) {
-  if (threadIdx.x != 0) return;
-  const long long start = wall_clock64();
-  while (__hip_atomic_load(ticket, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) < wanted) {
-    if (wall_clock64() - start > limit) {
-      atomicAdd(expired, 1u);
-      return;
-    }
-    __builtin_amdgcn_s_sleep(4);
-  }
-}
-
-}  // namespace
-
-extern "C" int tt_gate_arm(void *stream) {
-  std::lock_guard<std::mutex> lock(g_gate_mutex);
-  if (!g_gate.ticket) {
-    TT_HIP(hipHostMalloc(reinterpret_cast<void **>(&g_gate.ticket), sizeof(unsigned int),
-                         hipHostMallocCoherent | hipHostMallocMapped));
-    __atomic_store_n(g_gate.ticket, 0u, __ATOMIC_RELEASE);
-    TT_HIP(hipMalloc(reinterpret_cast<void **>(&g_gate.expired), sizeof(unsigned int)));
-    TT_HIP(hipMemset(g_gate.expired, 0, sizeof(unsigned int)));
-  }
-  unsigned int *device_ticket = nullptr;
-  TT_HIP(hipHostGetDevicePointer(reinterpret_cast<void **>(&device_ticket), g_gate.ticket, 0));
-  int rate_khz = 0;
-  TT_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, 0));
-  const long long limit = static_cast<long long>(rate_khz) * 50;  // 50 ms of wall clock
-  ++g_gate.armed;
-  hipLaunchKernelGGL(gate_kernel, dim3(1), dim3(64), 0, as_stream(stream), device_ticket, g_gate.armed, limit,
-                     g_gate.expired);
-  TT_LAUNCHED();
-  return 0;
-}
-
-extern "C" int tt_gate_release(void) {
-  std::lock_guard<std::mutex> lock(g_gate_mutex);
-  if (!g_gate.ticket) return fail("tt_gate_release: no gate was armed");
-  __atomic_store_n(g_gate.ticket, g_gate.armed, __ATOMIC_RELEASE);
-  return 0;
-}
-
-extern "C" int tt_gate_expired(unsigned int *count) {
-  if (!count) return fail("tt_gate_expired: null output");
-  *count = 0;
-  if (!g_gate.expired) return 0;
-  TT_HIP(hipMemcpy(count, g_gate.expired, sizeof(unsigned int), hipMemcpyDeviceToHost));
-  return 0;
-}
-
 extern "C" int tt_graph_begin(void *stream) {
   TT_HIP(hipStreamBeginCapture(as_stream(stream), hipStreamCaptureModeThreadLocal));
   return 0;
@@ -739,3 +674,45 @@ int ttdev::upload(void **where, const void *data, size_t bytes) {
 }
 
 extern "C" int64_t tt_launch_count(void) { return g_launches.load(std::memory_order_relaxed); }
+
+namespace {
+thread_local bool g_timing = false;
+thread_local std::vector<ttdev::TimedPair> g_timed;
+}  // namespace
+
+int ttdev::timed_pair(TimedPair *pair) {
+  if (!g_timing) return 0;
+  TT_HIP(hipEventCreate(&pair->start));
+  if (hipEventCreate(&pair->stop) != hipSuccess) {
+    (void)hipEventDestroy(pair->start);
+    return fail("hipEventCreate failed");
+  }
+  g_timed.push_back(*pair);
+  return 0;
+}
+
+extern "C" int tt_timing_enable(int on) {
+  g_timing = on != 0;
+  return 0;
+}
+
+extern "C" int tt_timing_collect(double *milliseconds, int64_t capacity, int64_t *count) {
+  if (count) *count = static_cast<int64_t>(g_timed.size());
+  if (!milliseconds) return 0;
+  if (capacity < static_cast<int64_t>(g_timed.size())) return fail("tt_timing_collect: the output is too small");
+  int status = 0;
+  for (size_t i = 0; i < g_timed.size(); ++i) {
+    ttdev::TimedPair &p = g_timed[i];
+    float ms = 0.0f;
+    if (status == 0) {
+      hipError_t st = hipEventSynchronize(p.stop);
+      if (st == hipSuccess) st = hipEventElapsedTime(&ms, p.start, p.stop);
+      if (st != hipSuccess) status = fail(std::string("kernel timing: ") + hipGetErrorString(st));
+    }
+    milliseconds[i] = ms;
+    (void)hipEventDestroy(p.start);
+    (void)hipEventDestroy(p.stop);
+  }
+  g_timed.clear();
+  return status;
+}

@@ -312,9 +312,11 @@ template <typename TA, typename TB, typename TO, typename TC, int BM, int BN, in
 int ggemm_launch(const tt_ggemm_plan *plan, const void *a, const void *b, void *c, hipStream_t stream, int64_t grid,
                  int32_t plain_group, int32_t plain_term, int32_t accumulate) {
   if (grid <= 0) return 0;
-  hipLaunchKernelGGL((ggemm_kernel<TA, TB, TO, TC, BM, BN, BK>), dim3(static_cast<uint32_t>(grid)), dim3(kGemmThreads),
-                     0, stream, plan->tiles, plan->groups, plan->terms, static_cast<const TA *>(a),
-                     static_cast<const TB *>(b), static_cast<TO *>(c), plain_group, plain_term, accumulate);
+  if (int rc = ttdev::launch(ggemm_kernel<TA, TB, TO, TC, BM, BN, BK>, dim3(static_cast<uint32_t>(grid)),
+                             dim3(kGemmThreads), 0, stream, plan->tiles, plan->groups, plan->terms,
+                             static_cast<const TA *>(a), static_cast<const TB *>(b), static_cast<TO *>(c), plain_group,
+                             plain_term, accumulate))
+    return rc;
   TT_LAUNCHED();
   ttdev::note_launch();
   return 0;

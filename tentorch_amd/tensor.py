@@ -332,17 +332,16 @@ def run(program, operands: Sequence[Tensor], out: Optional[torch.Tensor] = None,
 
 
 class Profiler:
-    '''Times every block-product and leg-move launch with HIP events on the launch stream, op by op,
-    while active; `summary()` synchronises and returns per-kind totals with their algorithmic work.
-    Each timed launch is enqueued behind a launch gate, so that the start event, the kernel and the
-    end event run back to back on the device and the interval is the kernel's own duration (on an
-    idle device an event pair around a lone launch also spans the host's launch latency).'''
+    '''Times every block-product and leg-move launch of the grouped form while active, op by op,
+    with a start and a stop event attached to the kernel dispatch itself (so the interval is the
+    kernel's execution, not the host's launch latency); `summary()` synchronises and returns per-kind
+    totals with their algorithmic work.'''
 
     active = None
 
-    def __init__(self, gated: bool = True):
-        self.gated = gated
-        self.records = []   # (kind, flops, bytes moved, compute type, start, end, label)
+    def __init__(self):
+        self.records = []   # (kind, flops, bytes moved, compute type, label, launches)
+        self.ms = []        # the kernel time of each record
 
     def __enter__(self):
         Profiler.active = self
@@ -350,10 +349,23 @@ class Profiler:
 
     def __exit__(self, *exc):
         Profiler.active = None
+        self._collect()
+
+    def _collect(self):
+        durations = _native.timing_collect()
+        expected = sum(r[5] for r in self.records[len(self.ms):])
+        if len(durations) != expected:
+            raise RuntimeError(f'{expected} timed launches produced {len(durations)} durations')
+        at = 0
+        for record in self.records[len(self.ms):]:
+            self.ms.append(sum(durations[at:at + record[5]]))
+            at += record[5]
 
     def run(self, program, args, skip, mode, label=''):
-        if self.gated and mode == 0:
-            program.prepare()   # table uploads wait on the device; none may happen behind a gate
+        if mode != 0:
+            program.run(*args)      # the per-block form is timed as a whole, not launch by launch
+            return
+        program.prepare()
         costs = program.op_costs()
         ops = program.ops()
         slots = program.slots()
@@ -368,20 +380,20 @@ class Profiler:
 
         for i, ((kind, flops, written, read), (_, a, b, dst, inputs)) in enumerate(zip(costs, ops)):
             single = inputs and not (inputs & (inputs - 1))
-            hoisted = mode == 0 and kind == 'move' and single and (inputs & skip)
-            if kind in ('gemm', 'move') and not hoisted:
-                start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                if self.gated and mode == 0:
-                    _native.gate_arm()
-                start.record()
+            hoisted = kind == 'move' and single and (inputs & skip)
+            timed = kind in ('gemm', 'move') and not hoisted
+            before = _native.launch_count()
+            if timed:
+                _native.timing(True)
+            try:
                 self._one(program, args, i)
-                end.record()
-                if self.gated and mode == 0:
-                    _native.gate_release()
-                moved = written * size(dst) + read * size(a)
-                self.records.append((kind, flops, moved, compute if kind == 'gemm' else '', start, end, label))
-            else:
-                self._one(program, args, i)
+            finally:
+                if timed:
+                    _native.timing(False)
+            launches = _native.launch_count() - before
+            if timed and launches:      # an op with no work launches nothing
+                self.records.append((kind, flops, written * size(dst) + read * size(a),
+                                     compute if kind == 'gemm' else '', label, launches))
 
     @staticmethod
     def _one(program, args, i):
@@ -391,16 +403,15 @@ class Profiler:
     def summary(self, label=None, compute=None):
         '''Totals per kind, over every record or over the records of one einsum equation (and, for
         block products, one compute type).'''
-        torch.cuda.synchronize()
         out = {}
-        for kind, flops, moved, kind_compute, start, end, lab in self.records:
+        for (kind, flops, moved, kind_compute, lab, launches), ms in zip(self.records, self.ms):
             if label is not None and lab != label:
                 continue
             if compute is not None and kind == 'gemm' and kind_compute != compute:
                 continue
             entry = out.setdefault(kind, {'launches': 0, 'ms': 0.0, 'flops': 0.0, 'bytes': 0.0})
-            entry['launches'] += 1
-            entry['ms'] += start.elapsed_time(end)
+            entry['launches'] += launches
+            entry['ms'] += ms
             entry['flops'] += flops
             entry['bytes'] += moved
         return out
