@@ -104,6 +104,9 @@ int tt_ggemm_plan_create(const tt_gemm_group *groups, int64_t num_groups, const 
 int tt_ggemm_plan_create_config(const tt_gemm_group *groups, int64_t num_groups, const tt_gemm_term *terms,
                                 int64_t num_terms, int config, tt_ggemm_plan **plan);
 /* Tile shapes the library is built with: rows x cols of the output tile, depth of one k-slab. */
+/* Configurations: c % 7 picks the tile shape (BM, BN, BK), c / 7 the packing of tiles into the
+ * launch: 0 longest K first, 1 all tiles of a block product on one XCD (workgroup i runs on XCD
+ * i % 8), block products spread over the XCDs by cost. */
 int tt_ggemm_num_configs(void);
 int tt_ggemm_config_shape(int config, int *rows, int *cols, int *depth);
 /* The tile shape a plan uses. */
@@ -146,6 +149,20 @@ int tt_graph_begin(void *stream);
 int tt_graph_end(void *stream, tt_graph **graph);
 int tt_graph_launch(const tt_graph *graph, void *stream);
 int tt_graph_destroy(tt_graph *graph);
+
+/* The learned tile schedule's cost model (no counterpart in the reference): a multilayer
+ * perceptron x (features) -> relu(W1^T x + b1) (hidden) -> relu(W2^T h1 + b2) (hidden) -> w3 . h2 + b3
+ * in FP32, parameters packed as W1 [features][hidden], b1, W2 [hidden][hidden], b2, w3, b3.
+ * hidden must be 64 and features 1..64.  `x` is [batch][features] row-major.
+ * tt_mlp_gradient writes the gradient of mean((y - target)^2) over the batch (fixed summation order)
+ * and, when `loss` is not null, that mean; tt_mlp_adam takes one Adam step (step counts from 1). */
+int64_t tt_mlp_param_count(int features, int hidden);
+int tt_mlp_forward(int features, int hidden, int64_t batch, const float *x, const float *params, float *y,
+                   void *stream);
+int tt_mlp_gradient(int features, int hidden, int64_t batch, const float *x, const float *target,
+                    const float *params, float *gradient, float *loss, void *stream);
+int tt_mlp_adam(int64_t n, float *params, const float *gradient, float *m, float *v, float lr, float beta1,
+                float beta2, float eps, int step, void *stream);
 
 /* Kernel timing (no counterpart in the reference): while enabled on the calling thread, every
  * leg-move and block-product launch is issued with a start and a stop event attached to the kernel

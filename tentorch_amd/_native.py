@@ -28,6 +28,13 @@ _lib.tt_convert.argtypes = [_int, _int, _i64, _c_void_p, _c_void_p, _c_void_p]
 _lib.tt_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
 _lib.tt_version.argtypes = []
 _lib.tt_launch_count.argtypes = []
+_lib.tt_mlp_param_count.argtypes = [_int, _int]
+_lib.tt_mlp_param_count.restype = ctypes.c_int64
+_lib.tt_mlp_forward.argtypes = [_int, _int, _i64, _c_void_p, _c_void_p, _c_void_p, _c_void_p]
+_lib.tt_mlp_gradient.argtypes = [_int, _int, _i64, _c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_void_p]
+_lib.tt_mlp_adam.argtypes = [_i64, _c_void_p, _c_void_p, _c_void_p, _c_void_p, ctypes.c_float, ctypes.c_float,
+                             ctypes.c_float, ctypes.c_float, _int, _c_void_p]
+_lib.tt_ggemm_num_configs.argtypes = []
 _lib.tt_timing_enable.argtypes = [_int]
 _lib.tt_timing_collect.argtypes = [ctypes.POINTER(_dbl), _i64, ctypes.POINTER(_i64)]
 _lib.tt_launch_count.restype = ctypes.c_int64
@@ -39,12 +46,13 @@ _lib.tt_graph_launch.argtypes = [_c_void_p, _c_void_p]
 _lib.tt_graph_destroy.argtypes = [_c_void_p]
 
 EXPORTED = ['tt_gss_plan_create', 'tt_gss_plan_destroy', 'tt_gss_plan_counts', 'tt_gss_execute',
-            'tt_gss_execute_plain', 'tt_ggemm_plan_create', 'tt_ggemm_plan_destroy', 'tt_ggemm_plan_counts',
+            'tt_gss_execute_plain', 'tt_gss_execute_typed', 'tt_ggemm_plan_create', 'tt_ggemm_plan_destroy', 'tt_ggemm_plan_counts',
             'tt_ggemm_execute', 'tt_ggemm_execute_plain', 'tt_ggemm_plan_create_config', 'tt_ggemm_num_configs',
             'tt_ggemm_config_shape', 'tt_ggemm_plan_config', 'tt_dot', 'tt_axpby', 'tt_convert', 'tt_multidot',
             'tt_multiaxpy', 'tt_graph_begin',
             'tt_graph_end', 'tt_graph_launch', 'tt_graph_destroy', 'tt_last_error', 'tt_version',
-            'tt_launch_count', 'tt_timing_enable', 'tt_timing_collect']
+            'tt_launch_count', 'tt_timing_enable', 'tt_timing_collect', 'tt_mlp_param_count', 'tt_mlp_forward',
+            'tt_mlp_gradient', 'tt_mlp_adam']
 
 
 def library() -> ctypes.CDLL:

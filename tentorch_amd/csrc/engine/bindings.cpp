@@ -55,6 +55,14 @@ struct PyProgram {
   ProgramPtr p;
 };
 
+// The block-product op at index `op` of a program.
+const Op &gemm_op(const PyProgram &p, int64_t op) {
+  if (op < 0 || op >= static_cast<int64_t>(p.p->ops.size())) throw std::out_of_range("the program has no op " + std::to_string(op));
+  const Op &o = p.p->ops[static_cast<size_t>(op)];
+  if (o.kind != OpKind::gemm) throw std::invalid_argument("op " + std::to_string(op) + " is not a block product");
+  return o;
+}
+
 Legs to_legs(const std::vector<PyLeg> &legs) {
   Legs out;
   for (const PyLeg &l : legs) out.push_back(l.leg);
@@ -291,6 +299,20 @@ PYBIND11_MODULE(_engine, m) {
            py::arg("mode") = 0, py::arg("skip") = 0, py::arg("first") = 0, py::arg("last") = -1,
            py::arg("ws_dtype") = -1, py::arg("compute") = -1)
       .def("prepare", [](const PyProgram &p) { p.p->prepare(); })
+      .def("gemm_config",
+           [](const PyProgram &p, int64_t op) {
+             const Op &o = gemm_op(p, op);
+             return tt_ggemm_plan_config(o.gemm->device());
+           })
+      .def("set_gemm_config", [](const PyProgram &p, int64_t op, int config) { gemm_op(p, op).gemm->set_config(config); })
+      .def("gemm_features",
+           [](const PyProgram &p, int64_t op, int element_size) {
+             const std::vector<float> f = gemm_op(p, op).gemm->features(element_size);
+             const int64_t rows = static_cast<int64_t>(f.size()) / kGemmFeatures;
+             py::array_t<float> out({rows, static_cast<int64_t>(kGemmFeatures)});
+             std::copy(f.begin(), f.end(), out.mutable_data());
+             return out;
+           })
       .def("op_costs",
            [](const PyProgram &p) {
              // Per op: (kind, algorithmic flops, elements written, elements read).

@@ -56,6 +56,13 @@ class MovePlan {
   // The device table, uploaded on first use.
   tt_gss_plan *device();
   int64_t elements() const { return target ? target->size() : 0; }
+  // Whether any coefficient has an imaginary part (R or F symbols with phases): such a move cannot
+  // act on a real tensor.
+  bool complex_coefficients() const {
+    for (const tt_gss_source &s : sources)
+      if (s.coef_im != 0.0) return true;
+    return false;
+  }
 
  private:
   tt_gss_plan *device_ = nullptr;

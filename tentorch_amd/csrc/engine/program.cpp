@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <cctype>
+#include <cmath>
 #include <functional>
 #include <limits>
 #include <map>
@@ -487,10 +488,86 @@ GemmPlan::~GemmPlan() {
 
 tt_ggemm_plan *GemmPlan::device() {
   if (!device_)
-    check(tt_ggemm_plan_create(groups.data(), static_cast<int64_t>(groups.size()), terms.data(),
-                               static_cast<int64_t>(terms.size()), &device_),
+    check(tt_ggemm_plan_create_config(groups.data(), static_cast<int64_t>(groups.size()), terms.data(),
+                                      static_cast<int64_t>(terms.size()), config_, &device_),
           "block product upload");
   return device_;
+}
+
+void GemmPlan::set_config(int config) {
+  if (config < -1 || config >= tt_ggemm_num_configs())
+    throw std::invalid_argument("no block-product configuration " + std::to_string(config));
+  if (config == config_) return;
+  if (device_) tt_ggemm_plan_destroy(device_);
+  device_ = nullptr;
+  config_ = config;
+}
+
+std::vector<float> GemmPlan::features(int element_size) const {
+  // Plan-level inputs: sizes and counts on log2 scales, and flop-weighted histograms of log2 m, n
+  // and k over six bins of width 2 (the last open).  Configuration inputs: the tile shape (one-hot
+  // and BK), the packing, and what the shape does to this plan: tile count, padding, the longest
+  // tile's k-slab count and the work per CU.
+  auto lg = [](double v) { return static_cast<float>(std::log2(1.0 + v)); };
+  double bytes = 0.0, mmax = 0, nmax = 0, kmax = 0;
+  float hist[3][6] = {{0}};
+  for (const tt_gemm_group &g : groups)
+    for (int32_t t = g.first_term; t < g.first_term + g.num_terms; ++t) {
+      const double m = static_cast<double>(g.m), n = static_cast<double>(g.n), k = static_cast<double>(terms[t].k);
+      bytes += (m * k + k * n + m * n) * element_size;
+      mmax = std::max(mmax, m), nmax = std::max(nmax, n), kmax = std::max(kmax, k);
+      const double share = flops > 0 ? 2.0 * m * n * k / flops : 0.0;
+      const double dims[3] = {m, n, k};
+      for (int d = 0; d < 3; ++d) {
+        const int bin = std::min(5, static_cast<int>(std::log2(std::max(1.0, dims[d])) / 2.0));
+        hist[d][bin] += static_cast<float>(share);
+      }
+    }
+  std::vector<float> base = {lg(static_cast<double>(groups.size())), lg(static_cast<double>(terms.size())), lg(flops),
+                             lg(bytes), static_cast<float>(element_size) / 8.0f};
+  for (int d = 0; d < 3; ++d) base.insert(base.end(), hist[d], hist[d] + 6);
+  base.push_back(lg(mmax));
+  base.push_back(lg(nmax));
+  base.push_back(lg(kmax));
+  const int configs = tt_ggemm_num_configs();
+  int shapes = 0;
+  for (int c = 0; c < configs; ++c) {
+    int bm0 = 0, bn0 = 0, bk0 = 0, bm = 0, bn = 0, bk = 0;
+    tt_ggemm_config_shape(0, &bm0, &bn0, &bk0);
+    tt_ggemm_config_shape(c, &bm, &bn, &bk);
+    if (c > 0 && bm == bm0 && bn == bn0 && bk == bk0) break;
+    ++shapes;
+  }
+  std::vector<float> out;
+  out.reserve(static_cast<size_t>(configs) * kGemmFeatures);
+  for (int c = 0; c < configs; ++c) {
+    int bm = 0, bn = 0, bk = 0;
+    tt_ggemm_config_shape(c, &bm, &bn, &bk);
+    double tiles = 0.0, padded = 0.0, area = 0.0, longest = 0.0, work = 0.0;
+    for (const tt_gemm_group &g : groups) {
+      int64_t ksum = 0;
+      for (int32_t t = g.first_term; t < g.first_term + g.num_terms; ++t) ksum += terms[t].k;
+      const double slabs = static_cast<double>((ksum + bk - 1) / bk) + 1.0;
+      const double count = static_cast<double>(((g.m + bm - 1) / bm) * ((g.n + bn - 1) / bn));
+      tiles += count;
+      padded += count * bm * bn;
+      area += static_cast<double>(g.m) * static_cast<double>(g.n);
+      longest = std::max(longest, slabs);
+      work += count * slabs;
+    }
+    std::vector<float> row = base;
+    for (int k = 0; k < 7; ++k) row.push_back(c % shapes == k ? 1.0f : 0.0f);
+    row.push_back(static_cast<float>(c / shapes));
+    row.push_back(lg(tiles));
+    row.push_back(area > 0 ? static_cast<float>(padded / area - 1.0) : 0.0f);
+    row.push_back(lg(longest));
+    row.push_back(lg(work / 256.0));
+    row.push_back(lg(tiles / 256.0));
+    row.push_back(static_cast<float>(bk) / 16.0f);
+    if (row.size() != static_cast<size_t>(kGemmFeatures)) throw std::logic_error("cost-model feature count");
+    out.insert(out.end(), row.begin(), row.end());
+  }
+  return out;
 }
 
 ProgramPtr plan_einsum(const std::string &spec, const std::vector<StructurePtr> &operands, const Sector *charge) {
@@ -606,6 +683,12 @@ void Program::run(int dtype, const std::vector<const void *> &operand_ptrs, void
   if (compute < 0) compute = dtype;
   if (mode == Mode::plain && (ws_dtype != dtype || compute != dtype))
     throw std::invalid_argument("the per-block form runs in one precision");
+  if (dtype != TT_C128 && first == 0)
+    for (const Op &op : ops)
+      if (op.kind == OpKind::move && op.move->complex_coefficients())
+        throw std::invalid_argument("a leg move of this program recouples with complex coefficients (" +
+                                    op.move->source->cat()->name() + " R or F symbols), which a real tensor "
+                                    "cannot hold: use complex128");
   const int64_t es = element_size(ws_dtype);
   char *ws = static_cast<char *>(workspace_ptr);
   auto ptr = [&](int32_t v) -> void * {
@@ -631,6 +714,11 @@ void Program::run(int dtype, const std::vector<const void *> &operand_ptrs, void
           char *extra = ws + workspace * es;
           const MovePlanPtr plain = plan_move_plain(op.move->source, op.move->above, op.move->below);
           const auto &steps = plain->elementary;
+          if (dtype != TT_C128)
+            for (const auto &step : steps)
+              if (step->complex_coefficients())
+                throw std::invalid_argument("a step of this leg move recouples with complex coefficients, which a real "
+                                            "tensor cannot hold in the per-block form: use complex128");
           const void *from = ptr(op.a);
           const int64_t half = plain_workspace / 2;
           for (size_t k = 0; k < steps.size(); ++k) {

@@ -25,10 +25,18 @@ class GemmPlan {
   double flops = 0.0;
   ~GemmPlan();
   tt_ggemm_plan *device();
+  // The launch configuration (tile shape and packing, tt_ggemm_num_configs of them); -1 lets the
+  // device library pick from the block shapes.  Changing it drops the uploaded tables.
+  int config() const { return config_; }
+  void set_config(int config);
+  // Cost-model inputs for each configuration: one row of kGemmFeatures values per configuration.
+  std::vector<float> features(int element_size) const;
 
  private:
   tt_ggemm_plan *device_ = nullptr;
+  int config_ = -1;
 };
+constexpr int kGemmFeatures = 40;
 using GemmPlanPtr = std::shared_ptr<GemmPlan>;
 
 enum class OpKind { move, gemm, copy, add };

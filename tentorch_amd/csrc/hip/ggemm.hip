@@ -185,14 +185,25 @@ __global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__r
       }
 }
 
-// Host-side tile count and cost of a group list under one tile shape.
+// How the tiles of one launch are ordered.  Workgroup i of a launch runs on XCD i % 8, and each XCD
+// has an L2 of its own.
+enum Packing {
+  kLongestFirst = 0,  // by decreasing K, so the last wave of the launch is made of short tiles
+  kXcdGrouped = 1,    // every tile of a block product on one XCD (its A and B panels then share an
+                      // L2); block products spread over the XCDs by largest total cost first
+};
+constexpr int kXcds = 8;
+constexpr int kNumPackings = 2;
+
+// Host-side tile list and cost of a group list under one tile shape and packing.
 void tile_list(const tt_gemm_group *groups, int64_t num_groups, const tt_gemm_term *terms, const Shape &shape,
-               std::vector<GemmTile> &tiles, double *max_cost) {
+               int packing, std::vector<GemmTile> &tiles, double *max_cost) {
   struct Costed {
     double cost;
     GemmTile tile;
   };
   std::vector<Costed> costed;
+  std::vector<double> group_cost(static_cast<size_t>(num_groups), 0.0);
   *max_cost = 0.0;
   for (int64_t g = 0; g < num_groups; ++g) {
     const tt_gemm_group &grp = groups[g];
@@ -201,13 +212,51 @@ void tile_list(const tt_gemm_group *groups, int64_t num_groups, const tt_gemm_te
     const double cost = static_cast<double>((ksum + shape.bk - 1) / shape.bk) + 1.0;
     *max_cost = std::max(*max_cost, cost);
     for (int64_t mt = 0; mt < grp.m; mt += shape.bm)
-      for (int64_t nt = 0; nt < grp.n; nt += shape.bn)
+      for (int64_t nt = 0; nt < grp.n; nt += shape.bn) {
         costed.push_back({cost, GemmTile{static_cast<int32_t>(g), static_cast<int32_t>(mt), static_cast<int32_t>(nt), 0}});
+        group_cost[static_cast<size_t>(g)] += cost;
+      }
   }
-  // Longest tiles first, so the last wave of the launch is made of short ones.
   std::stable_sort(costed.begin(), costed.end(), [](const Costed &a, const Costed &b) { return a.cost > b.cost; });
   tiles.clear();
-  for (const Costed &c : costed) tiles.push_back(c.tile);
+  if (packing == kLongestFirst) {
+    for (const Costed &c : costed) tiles.push_back(c.tile);
+    return;
+  }
+  // XCD-grouped: assign block products to XCDs by largest total cost first (each to the least
+  // loaded XCD), keep each XCD's tiles longest first, then lay them out so that position p holds a
+  // tile of XCD p % 8; when an XCD runs out, the position takes a tile from the fullest one.
+  std::vector<int64_t> order(static_cast<size_t>(num_groups));
+  for (int64_t g = 0; g < num_groups; ++g) order[static_cast<size_t>(g)] = g;
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+    return group_cost[static_cast<size_t>(a)] > group_cost[static_cast<size_t>(b)];
+  });
+  std::vector<int> xcd_of(static_cast<size_t>(num_groups), 0);
+  double load[kXcds] = {0.0};
+  for (int64_t g : order) {
+    int best = 0;
+    for (int x = 1; x < kXcds; ++x)
+      if (load[x] < load[best]) best = x;
+    xcd_of[static_cast<size_t>(g)] = best;
+    load[best] += group_cost[static_cast<size_t>(g)];
+  }
+  std::vector<std::vector<GemmTile>> bins(kXcds);
+  for (const Costed &c : costed) bins[static_cast<size_t>(xcd_of[static_cast<size_t>(c.tile.group)])].push_back(c.tile);
+  std::vector<size_t> next(kXcds, 0);
+  for (size_t p = 0; p < costed.size(); ++p) {
+    int x = static_cast<int>(p % kXcds);
+    if (next[static_cast<size_t>(x)] >= bins[static_cast<size_t>(x)].size()) {
+      size_t most = 0;
+      for (int y = 0; y < kXcds; ++y) {
+        const size_t left = bins[static_cast<size_t>(y)].size() - next[static_cast<size_t>(y)];
+        if (left > most) {
+          most = left;
+          x = y;
+        }
+      }
+    }
+    tiles.push_back(bins[static_cast<size_t>(x)][next[static_cast<size_t>(x)]++]);
+  }
 }
 
 // The default tile shape.  Measured on the DMRG matvec products (16 blocks, 0.97 GFLOP per step):
@@ -232,13 +281,14 @@ struct tt_ggemm_plan {
   std::vector<tt_gemm_group> host_groups;
 };
 
-extern "C" int tt_ggemm_num_configs(void) { return kNumShapes; }
+extern "C" int tt_ggemm_num_configs(void) { return kNumShapes * kNumPackings; }
 
 extern "C" int tt_ggemm_config_shape(int config, int *rows, int *cols, int *depth) {
-  if (config < 0 || config >= kNumShapes) return fail("tt_ggemm_config_shape: no such tile shape");
-  if (rows) *rows = kShapes[config].bm;
-  if (cols) *cols = kShapes[config].bn;
-  if (depth) *depth = kShapes[config].bk;
+  if (config < 0 || config >= kNumShapes * kNumPackings) return fail("tt_ggemm_config_shape: no such configuration");
+  const Shape &shape = kShapes[config % kNumShapes];
+  if (rows) *rows = shape.bm;
+  if (cols) *cols = shape.bn;
+  if (depth) *depth = shape.bk;
   return 0;
 }
 
@@ -259,10 +309,10 @@ extern "C" int tt_ggemm_plan_create_config(const tt_gemm_group *groups, int64_t 
     }
   }
   if (config < 0) config = choose_shape(groups, num_groups, terms);
-  if (config >= kNumShapes) return fail("tt_ggemm_plan_create: no such tile shape");
+  if (config >= kNumShapes * kNumPackings) return fail("tt_ggemm_plan_create: no such configuration");
   std::vector<GemmTile> tiles;
   double max_cost = 0.0;
-  tile_list(groups, num_groups, terms, kShapes[config], tiles, &max_cost);
+  tile_list(groups, num_groups, terms, kShapes[config % kNumShapes], config / kNumShapes, tiles, &max_cost);
   auto *p = new tt_ggemm_plan();
   p->num_groups = num_groups;
   p->num_terms = num_terms;
@@ -325,7 +375,7 @@ int ggemm_launch(const tt_ggemm_plan *plan, const void *a, const void *b, void *
 template <typename TA, typename TB, typename TO, typename TC>
 int ggemm_shape(const tt_ggemm_plan *plan, const void *a, const void *b, void *c, hipStream_t stream, int64_t grid,
                 int32_t pg, int32_t pt, int32_t acc) {
-  switch (plan->config) {
+  switch (plan->config % kNumShapes) {
     case 0: return ggemm_launch<TA, TB, TO, TC, 64, 64, 16>(plan, a, b, c, stream, grid, pg, pt, acc);
     case 1: return ggemm_launch<TA, TB, TO, TC, 32, 32, 16>(plan, a, b, c, stream, grid, pg, pt, acc);
     case 2: return ggemm_launch<TA, TB, TO, TC, 128, 64, 16>(plan, a, b, c, stream, grid, pg, pt, acc);
@@ -369,7 +419,7 @@ extern "C" int tt_ggemm_execute_plain(const tt_ggemm_plan *plan, int dtype, cons
                                       void *stream) {
   if (!plan) return fail("tt_ggemm_execute_plain: null plan");
   if (dtype != TT_F64 && dtype != TT_F32) return fail("tt_ggemm_execute_plain: unsupported dtype");
-  const Shape &s = kShapes[plan->config];
+  const Shape &s = kShapes[plan->config % kNumShapes];
   for (int64_t g = 0; g < plan->num_groups; ++g) {
     const tt_gemm_group &grp = plan->host_groups[static_cast<size_t>(g)];
     const int64_t grid = ((grp.m + s.bm - 1) / s.bm) * ((grp.n + s.bn - 1) / s.bn);

@@ -271,9 +271,13 @@ workspaces = _Workspaces()
 class Settings:
     '''How programs run: `mode` 0 batched (composed moves, grouped GEMM, hoisting), 1 plain (the
     reference's per-step, per-block launches, nothing hoisted); `precision` 'fp64', or 'fp32' to run
-    the block products of FP64 operands on FP32 MFMA with FP32 workspaces (the result stays FP64).'''
+    the block products of FP64 operands on FP32 MFMA with FP32 workspaces (the result stays FP64);
+    `tiles` how each block product's launch configuration is picked: 'default' (the device
+    library's rule from block shapes), 'model' (the learned cost model) or 'exhaustive' (time every
+    configuration on the program's first run and keep the fastest).'''
     mode = 0
     precision = 'fp64'
+    tiles = 'default'
 
     class using:
         '''Context manager that sets the fields given and restores them on exit.'''
@@ -324,10 +328,19 @@ def run(program, operands: Sequence[Tensor], out: Optional[torch.Tensor] = None,
             mode, skip)
     if mixed:
         args = args + (0, -1, _native.TT_F32, _native.TT_F32)
+    measure = False
+    if Settings.tiles != 'default' and mode == 0:
+        from .tuning import tuner
+        measure = tuner.before(program, args, Settings.tiles)
     if Profiler.active is None:
         program.run(*args)
     else:
         Profiler.active.run(program, args, skip, mode, label)
+    if measure:
+        # Every block product is timed under each configuration on the data of this run; the run is
+        # then repeated with the chosen ones, so the result never depends on what the timing left.
+        tuner.after_first_run(program, args)
+        program.run(*args)
     return Tensor._wrap(result, out)
 
 
