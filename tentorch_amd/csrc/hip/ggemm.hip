@@ -1,5 +1,5 @@
 // The MI355X device library behind include/tentorch_amd.h (part two): the grouped block GEMM on
-// FP64 and FP32 MFMA.  One launch computes every output block of a contraction step; each
+// FP64 and FP32 MFMA (complex128 as four FP64 products per step).  One launch computes every output block of a contraction step; each
 // workgroup owns one BM x BN output tile of one block and sums that block's terms in list order.
 // The tile shape is a property of the plan (chosen by the host from the block shapes, or by an
 // autotuner through tt_ggemm_plan_create_config).  gfx950 only.
@@ -10,6 +10,7 @@
 #include "common.h"
 
 using ttdev::as_stream;
+using ttdev::cdouble;
 using ttdev::fail;
 
 namespace {
@@ -25,15 +26,54 @@ struct Mfma;
 template <>
 struct Mfma<double> {
   using acc_t = d4;
+  static __device__ acc_t zero() { return acc_t{0, 0, 0, 0}; }
   static __device__ acc_t step(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  static __device__ double get(const acc_t &c, int r) { return c[r]; }
   static __device__ int row(int lk, int r) { return 4 * r + lk; }
 };
 template <>
 struct Mfma<float> {
   using acc_t = f4;
+  static __device__ acc_t zero() { return acc_t{0, 0, 0, 0}; }
   static __device__ acc_t step(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  static __device__ float get(const acc_t &c, int r) { return c[r]; }
   static __device__ int row(int lk, int r) { return 4 * lk + r; }
 };
+// Complex: (a_re + i a_im)(b_re + i b_im) as four FP64 MFMAs into a real and an imaginary
+// accumulator, which share the f64 result layout.
+struct cd4 {
+  d4 re, im;
+};
+template <>
+struct Mfma<cdouble> {
+  using acc_t = cd4;
+  static __device__ acc_t zero() { return acc_t{d4{0, 0, 0, 0}, d4{0, 0, 0, 0}}; }
+  static __device__ acc_t step(cdouble a, cdouble b, acc_t c) {
+    c.re = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.re, c.re, 0, 0, 0);
+    c.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.im, b.im, c.re, 0, 0, 0);
+    c.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.im, c.im, 0, 0, 0);
+    c.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, b.re, c.im, 0, 0, 0);
+    return c;
+  }
+  static __device__ cdouble get(const acc_t &c, int r) { return cdouble{c.re[r], c.im[r]}; }
+  static __device__ int row(int lk, int r) { return 4 * r + lk; }
+};
+
+// Element conversions between storage and compute types (real types convert, complex stays).
+template <typename To>
+struct Conv {
+  template <typename From>
+  static __device__ To from(From x) { return static_cast<To>(x); }
+  static __device__ To zero() { return To(0); }
+};
+template <>
+struct Conv<cdouble> {
+  static __device__ cdouble from(cdouble x) { return x; }
+  static __device__ cdouble zero() { return cdouble{0.0, 0.0}; }
+};
+template <typename T>
+__device__ T add(T a, T b) { return a + b; }
+__device__ cdouble add(cdouble a, cdouble b) { return cdouble{a.re + b.re, a.im + b.im}; }
 
 // Output tile of one workgroup: which block, and the tile's first row and column.
 struct GemmTile {
@@ -101,7 +141,7 @@ __global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__r
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < FN; ++j) acc[i][j] = typename M::acc_t{0, 0, 0, 0};
+    for (int j = 0; j < FN; ++j) acc[i][j] = M::zero();
 
   const int a_row = tid / A_TPR, a_k = (tid % A_TPR) * A_PER;
   const int b_k = tid / B_TPR, b_col = (tid % B_TPR) * B_PER;
@@ -118,13 +158,13 @@ __global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__r
 #pragma unroll
     for (int j = 0; j < A_PER; ++j) {
       const int64_t kk = kb + a_k + j;
-      ra[j] = (gr < m && kk < K) ? static_cast<TC>(A[tm.a_offset + gr * K + kk]) : TC(0);
+      ra[j] = (gr < m && kk < K) ? Conv<TC>::from(A[tm.a_offset + gr * K + kk]) : Conv<TC>::zero();
     }
     const int64_t kr = kb + b_k;
 #pragma unroll
     for (int j = 0; j < B_PER; ++j) {
       const int64_t gc = n0 + b_col + j;
-      rb[j] = (kr < K && gc < n) ? static_cast<TC>(B[tm.b_offset + kr * n + gc]) : TC(0);
+      rb[j] = (kr < K && gc < n) ? Conv<TC>::from(B[tm.b_offset + kr * n + gc]) : Conv<TC>::zero();
     }
   };
   auto advance = [&](int32_t &t, int64_t &kb) {
@@ -176,11 +216,11 @@ __global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__r
         const int64_t col = n0 + wn * TN + j * 16 + li;
         if (row < m && col < n) {
           TO *out = C + group.c_offset + row * n + col;
-          const TC value = acc[i][j][r];
+          const TC value = M::get(acc[i][j], r);
           if (accumulate)
-            *out = static_cast<TO>(static_cast<TC>(*out) + value);
+            *out = Conv<TO>::from(add(Conv<TC>::from(*out), value));
           else
-            *out = static_cast<TO>(value);
+            *out = Conv<TO>::from(value);
         }
       }
 }
@@ -400,6 +440,7 @@ int ggemm_dispatch(const tt_ggemm_plan *plan, int at, int bt, int ct, int comp, 
   TT_CASE(TT_F32, TT_F64, TT_F64, TT_F32, float, double, double, float)
   TT_CASE(TT_F64, TT_F32, TT_F32, TT_F32, double, float, float, float)
   TT_CASE(TT_F64, TT_F32, TT_F64, TT_F32, double, float, double, float)
+  TT_CASE(TT_C128, TT_C128, TT_C128, TT_C128, cdouble, cdouble, cdouble, cdouble)
 #undef TT_CASE
   return fail("grouped GEMM: unsupported type combination (a " + std::to_string(at) + ", b " + std::to_string(bt) +
               ", c " + std::to_string(ct) + ", compute " + std::to_string(comp) + ")");
@@ -418,7 +459,7 @@ extern "C" int tt_ggemm_execute(const tt_ggemm_plan *plan, int a_type, int b_typ
 extern "C" int tt_ggemm_execute_plain(const tt_ggemm_plan *plan, int dtype, const void *a, const void *b, void *c,
                                       void *stream) {
   if (!plan) return fail("tt_ggemm_execute_plain: null plan");
-  if (dtype != TT_F64 && dtype != TT_F32) return fail("tt_ggemm_execute_plain: unsupported dtype");
+  if (dtype != TT_F64 && dtype != TT_F32 && dtype != TT_C128) return fail("tt_ggemm_execute_plain: unsupported dtype");
   const Shape &s = kShapes[plan->config % kNumShapes];
   for (int64_t g = 0; g < plan->num_groups; ++g) {
     const tt_gemm_group &grp = plan->host_groups[static_cast<size_t>(g)];

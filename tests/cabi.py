@@ -48,6 +48,12 @@ def load():
     lib.tt_ggemm_plan_create_config.argtypes = [ctypes.POINTER(GemmGroup), i64, ctypes.POINTER(GemmTerm), i64, i32, ctypes.POINTER(vp)]
     lib.tt_ggemm_config_shape.argtypes = [i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
     lib.tt_ggemm_plan_config.argtypes = [vp]
+    lib.tt_mlp_param_count.argtypes = [i32, i32]
+    lib.tt_mlp_param_count.restype = i64
+    lib.tt_mlp_forward.argtypes = [i32, i32, i64, vp, vp, vp, vp]
+    lib.tt_mlp_gradient.argtypes = [i32, i32, i64, vp, vp, vp, vp, vp, vp]
+    lib.tt_mlp_adam.argtypes = [i64, vp, vp, vp, vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                i32, vp]
     lib.tt_ggemm_plan_destroy.argtypes = [vp]
     lib.tt_ggemm_plan_counts.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]
     lib.tt_ggemm_execute.argtypes = [vp, i32, i32, i32, i32, vp, vp, vp, vp]

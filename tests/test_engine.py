@@ -219,3 +219,29 @@ def test_microbench_matvec_plan_matches_the_independent_counts():
     assert max(shapes, key=lambda s: s[0] * s[1] * s[2]) in ((946, 198, 198), (198, 198, 946))
     flops = sum(c[1] for c in program.op_costs() if c[0] == 'gemm')
     assert abs(flops - 0.898e9) < 0.001e9
+
+
+@pytest.mark.parametrize('name', ['su2', 'fibonacci'])
+def test_recoupling_einsum_program_matches_the_oracle(name):
+    '''`ABCd,Be->ACde` takes the middle leg of a three-leg codomain, so its first move recouples two
+    legs (multi-source tiles with F and R coefficients): the planned program, interpreted on the CPU,
+    against the oracle.'''
+    legs = {'su2': [([(0,), (2,), (4,)], [2, 2, 1]), ([(0,), (2,)], [1, 2]), ([(2,), (4,)], [2, 1]),
+                    ([(0,), (2,)], [2, 1]), ([(0,), (2,), (4,)], [1, 1, 1])],
+            'fibonacci': [([(0,), (1,)], [2, 3]), ([(0,), (1,)], [1, 2]), ([(0,), (1,)], [2, 1]),
+                          ([(0,), (1,)], [1, 1]), ([(0,), (1,)], [2, 2])]}[name]
+    dtype = np.complex128 if name == 'fibonacci' else np.float64
+    cat_p, cat_o = CATS[name]
+    rng = np.random.default_rng(7)
+    operands, structures = [], []
+    for cod, dom in (((0, 1, 2), (3,)), ((1,), (4,))):
+        operands.append(ot.random_tensor(cat_o, [osym.Leg(cat_o, *legs[i]) for i in cod],
+                                         [osym.Leg(cat_o, *legs[i]) for i in dom], cat_o.trivial(), rng, dtype))
+        structures.append(engine.Structure(cat_p, [tt.Leg(cat_p, *legs[i]) for i in cod],
+                                           [tt.Leg(cat_p, *legs[i]) for i in dom], cat_p.trivial()))
+    program = engine.plan_einsum('ABCd,Be->ACde', structures, None)
+    tasks, _ = program.move_tables(0)
+    assert any(t[4] > 1 for t in tasks), 'the first move should recouple'
+    got = run_program(program, [o.flat() for o in operands], dtype)
+    ref = ot.einsum('ABCd,Be->ACde', *operands).flat()
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()

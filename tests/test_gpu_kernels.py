@@ -185,3 +185,58 @@ def test_typed_leg_move_converts_between_fp64_and_fp32(lib):
     assert lib.tt_gss_execute_typed(plan, cabi.TT_F64, cabi.TT_F32, src.data_ptr(), low.data_ptr(), _stream()) == 0
     assert torch.equal(low, full.float())
     lib.tt_gss_plan_destroy(plan)
+
+
+def test_every_launch_configuration_gives_identical_products(lib):
+    '''All tile shapes and both packings: the same values bit for bit (each output element sums its
+    terms and k-slabs in one order whatever the tile), within 1e-12 of numpy.'''
+    rng = np.random.default_rng(11)
+    a, b, c_size, groups, terms = _random_gemm(rng, SHAPES, multi_term=True)
+    ref = _reference_gemm(a, b, c_size, groups, terms)
+    g, t = cabi.gemm_tables(groups, terms)
+    da, db = torch.tensor(a, device='cuda'), torch.tensor(b, device='cuda')
+    first = None
+    for config in range(lib.tt_ggemm_num_configs()):
+        plan = ctypes.c_void_p()
+        assert lib.tt_ggemm_plan_create_config(g, len(groups), t, len(terms), config, ctypes.byref(plan)) == 0, cabi.error(lib)
+        assert lib.tt_ggemm_plan_config(plan) == config
+        dc = torch.full((c_size,), float('nan'), dtype=torch.float64, device='cuda')
+        assert lib.tt_ggemm_execute(plan, cabi.TT_F64, cabi.TT_F64, cabi.TT_F64, cabi.TT_F64, da.data_ptr(), db.data_ptr(),
+                                    dc.data_ptr(), _stream()) == 0, cabi.error(lib)
+        lib.tt_ggemm_plan_destroy(plan)
+        got = dc.cpu().numpy()
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max(), config
+        if first is None:
+            first = got
+        assert np.array_equal(got, first), config
+
+
+@pytest.mark.parametrize('config', [0, 1, 5, 13])
+def test_complex_grouped_gemm_matches_numpy(lib, config):
+    '''complex128 block products (four FP64 MFMAs per step), grouped and per block.'''
+    rng = np.random.default_rng(12)
+    _, _, c_size, groups, terms = _random_gemm(rng, SHAPES, multi_term=True)
+    a_size = max(t[0] + g[1] * t[2] for g in groups for t in terms[g[3]:g[3] + g[4]])
+    b_size = max(t[1] + t[2] * g[2] for g in groups for t in terms[g[3]:g[3] + g[4]])
+    a = rng.standard_normal(a_size) + 1j * rng.standard_normal(a_size)
+    b = rng.standard_normal(b_size) + 1j * rng.standard_normal(b_size)
+    ref = np.zeros(c_size, dtype=np.complex128)
+    for c_off, m, n, first, count in groups:
+        acc = np.zeros((m, n), dtype=np.complex128)
+        for ao, bo, k in terms[first:first + count]:
+            acc += a[ao:ao + m * k].reshape(m, k) @ b[bo:bo + k * n].reshape(k, n)
+        ref[c_off:c_off + m * n] = acc.reshape(-1)
+    g, t = cabi.gemm_tables(groups, terms)
+    plan = ctypes.c_void_p()
+    assert lib.tt_ggemm_plan_create_config(g, len(groups), t, len(terms), config, ctypes.byref(plan)) == 0, cabi.error(lib)
+    da, db = torch.tensor(a, device='cuda'), torch.tensor(b, device='cuda')
+    for plain in (False, True):
+        dc = torch.full((c_size,), complex('nan'), dtype=torch.complex128, device='cuda')
+        if plain:
+            status = lib.tt_ggemm_execute_plain(plan, cabi.TT_C128, da.data_ptr(), db.data_ptr(), dc.data_ptr(), _stream())
+        else:
+            status = lib.tt_ggemm_execute(plan, cabi.TT_C128, cabi.TT_C128, cabi.TT_C128, cabi.TT_C128, da.data_ptr(),
+                                          db.data_ptr(), dc.data_ptr(), _stream())
+        assert status == 0, cabi.error(lib)
+        assert np.abs(dc.cpu().numpy() - ref).max() <= 1e-12 * np.abs(ref).max(), plain
+    lib.tt_ggemm_plan_destroy(plan)
