@@ -214,6 +214,9 @@ class Tuner:
             best = int(np.argmin(times))
             program.set_gemm_config(op, best)
             choices.append(best)
+            if len(self.samples) % 250 == 0:
+                print(f'[tuning] {len(self.samples)} block products timed under {NUM_CONFIGS} configurations, '
+                      f'{self.seconds + time.perf_counter() - start:.1f} s', file=sys.stderr, flush=True)
         self.done[program.id] = choices
         self.seconds += time.perf_counter() - start
 
@@ -294,7 +297,7 @@ def uneven_contractions(seed: int = 0, trials: int = 4):
     (labels 2j) with multiplicities ~ Poisson(12 exp(-j/2)), and Fibonacci tensors with 1 and tau
     multiplicities ~ Poisson(40), N(0, 1) entries in FP64.  Each contraction takes the middle leg of
     a three-leg codomain, so the operand's move recouples two legs through F and R symbols before the
-    block products.  Yields (name, thunk); a thunk builds its tensors and runs the einsum.'''
+    block products.  Fibonacci symbols carry phases, so those tensors are complex128.  Yields (name, thunk); a thunk builds its tensors and runs the einsum.'''
     from .symmetry import Leg, fibonacci, su2
     from .tensor import Tensor, einsum
     rng = np.random.default_rng(seed)
@@ -311,11 +314,11 @@ def uneven_contractions(seed: int = 0, trials: int = 4):
     def fib_leg():
         return Leg(fibonacci, [(0,), (1,)], [max(1, int(rng.poisson(40))), max(1, int(rng.poisson(40)))])
 
-    def run(cat, legs, spec, shapes, generator_seed):
+    def run(cat, legs, spec, shapes, generator_seed, dtype=torch.float64):
         def thunk():
             g = np.random.default_rng(generator_seed)
-            tensors = [Tensor.random(cat, [legs[i] for i in cod], [legs[i] for i in dom], cat.trivial(), generator=g)
-                       for cod, dom in shapes]
+            tensors = [Tensor.random(cat, [legs[i] for i in cod], [legs[i] for i in dom], cat.trivial(), dtype=dtype,
+                                     generator=g) for cod, dom in shapes]
             out = einsum(spec, *tensors)
             torch.cuda.synchronize()
             del out, tensors
@@ -329,7 +332,8 @@ def uneven_contractions(seed: int = 0, trials: int = 4):
                                                      [((0, 1, 2), (3,)), ((2, 4), (5,))], int(rng.integers(1 << 30))))
         legs = [fib_leg() for _ in range(5)]
         yield (f'Fibonacci {trial} ABCd,Be->ACde', run(fibonacci, legs, 'ABCd,Be->ACde',
-                                                       [((0, 1, 2), (3,)), ((1,), (4,))], int(rng.integers(1 << 30))))
+                                                       [((0, 1, 2), (3,)), ((1,), (4,))], int(rng.integers(1 << 30)),
+                                                       torch.complex128))
 
 
 def main(argv=None) -> int:
@@ -341,7 +345,7 @@ def main(argv=None) -> int:
     t = sub.add_parser('train')
     t.add_argument('--data', required=True)
     t.add_argument('--out', required=True)
-    t.add_argument('--epochs', type=int, default=400)
+    t.add_argument('--epochs', type=int, default=150)
     r = sub.add_parser('regret')
     r.add_argument('--model', default=DEFAULT_MODEL)
     r.add_argument('--out', default=None)
