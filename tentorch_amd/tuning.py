@@ -398,4 +398,7 @@ def main(argv=None) -> int:
 
 
 if __name__ == '__main__':
-    sys.exit(main())
+    # Run from the package's own module: tensor.run reports to `tentorch_amd.tuning.tuner`, which
+    # under `python -m` is not this __main__ module's copy.
+    from tentorch_amd.tuning import main as package_main
+    sys.exit(package_main())
