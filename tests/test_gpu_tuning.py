@@ -122,3 +122,16 @@ def test_autotuned_einsum_equals_the_default_one():
     features, times, default = tuner.samples[-1]
     assert features.shape[0] == times.shape[0] and np.all(np.isfinite(times)) and np.all(times > 0)
     assert torch.equal(got, ref) and torch.equal(again, ref)
+
+
+def test_device_inference_makes_the_cpu_forward_choices():
+    '''The committed cost model run by `tt_mlp_forward` agrees with a float64 CPU forward pass on the
+    unseen contractions' feature rows (predictions within 1e-3 in log time).'''
+    import os
+    from tentorch_amd.tuning import CostModel, DEFAULT_MODEL
+    from test_tuning import model_forward
+    data = np.load(os.path.join(os.path.dirname(__file__), 'data', 'uneven_contraction_timings.npz'))
+    model = CostModel.load(DEFAULT_MODEL)
+    for rows in data['features']:
+        got, ref = model.predict(rows), model_forward(DEFAULT_MODEL, rows)
+        assert np.abs(got - ref).max() <= 1e-3 * max(1.0, np.abs(ref).max())
