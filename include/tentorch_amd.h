@@ -150,6 +150,22 @@ int tt_graph_end(void *stream, tt_graph **graph);
 int tt_graph_launch(const tt_graph *graph, void *stream);
 int tt_graph_destroy(tt_graph *graph);
 
+/* Batched symmetric eigendecomposition in FP64 (the split of the two-site tensor: the Gram matrix
+ * of each block).  Replaces the per-block `at::linalg_svd` of `split_over`
+ * (tentorch/linalg/graded/svd.cpp:41-177) with one tridiagonalisation, one divide-and-conquer
+ * and one back-transformation launch for all blocks.  Matrix b is the symmetric n x n at
+ * `matrices + matrix_offset` (overwritten: it keeps the Householder reflectors); its eigenvalues go
+ * to `values + value_offset` in ascending order, and eigenvector j to row j of the n x n at
+ * `vectors + matrix_offset`.  n is 1..1024.  `workspace` holds tt_eigh_workspace(...) doubles. */
+typedef struct tt_eigh_problem {
+  int64_t matrix_offset;
+  int64_t value_offset;
+  int64_t n;
+} tt_eigh_problem;
+int64_t tt_eigh_workspace(const tt_eigh_problem *problems, int64_t count);
+int tt_eigh_batched(const tt_eigh_problem *problems, int64_t count, double *matrices, double *values,
+                    double *vectors, double *workspace, void *stream);
+
 /* The learned tile schedule's cost model (no counterpart in the reference): a multilayer
  * perceptron x (features) -> relu(W1^T x + b1) (hidden) -> relu(W2^T h1 + b2) (hidden) -> w3 . h2 + b3
  * in FP32, parameters packed as W1 [features][hidden], b1, W2 [hidden][hidden], b2, w3, b3.

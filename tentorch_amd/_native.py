@@ -35,6 +35,16 @@ _lib.tt_mlp_gradient.argtypes = [_int, _int, _i64, _c_void_p, _c_void_p, _c_void
 _lib.tt_mlp_adam.argtypes = [_i64, _c_void_p, _c_void_p, _c_void_p, _c_void_p, ctypes.c_float, ctypes.c_float,
                              ctypes.c_float, ctypes.c_float, _int, _c_void_p]
 _lib.tt_ggemm_num_configs.argtypes = []
+
+
+class EighProblem(ctypes.Structure):
+    _fields_ = [('matrix_offset', ctypes.c_int64), ('value_offset', ctypes.c_int64), ('n', ctypes.c_int64)]
+
+
+_lib.tt_eigh_workspace.argtypes = [ctypes.POINTER(EighProblem), _i64]
+_lib.tt_eigh_workspace.restype = ctypes.c_int64
+_lib.tt_eigh_batched.argtypes = [ctypes.POINTER(EighProblem), _i64, _c_void_p, _c_void_p, _c_void_p, _c_void_p,
+                                 _c_void_p]
 _lib.tt_timing_enable.argtypes = [_int]
 _lib.tt_timing_collect.argtypes = [ctypes.POINTER(_dbl), _i64, ctypes.POINTER(_i64)]
 _lib.tt_launch_count.restype = ctypes.c_int64
@@ -52,7 +62,7 @@ EXPORTED = ['tt_gss_plan_create', 'tt_gss_plan_destroy', 'tt_gss_plan_counts', '
             'tt_multiaxpy', 'tt_graph_begin',
             'tt_graph_end', 'tt_graph_launch', 'tt_graph_destroy', 'tt_last_error', 'tt_version',
             'tt_launch_count', 'tt_timing_enable', 'tt_timing_collect', 'tt_mlp_param_count', 'tt_mlp_forward',
-            'tt_mlp_gradient', 'tt_mlp_adam']
+            'tt_mlp_gradient', 'tt_mlp_adam', 'tt_eigh_workspace', 'tt_eigh_batched']
 
 
 def library() -> ctypes.CDLL:
@@ -137,6 +147,35 @@ def multiaxpy(V: torch.Tensor, k: int, coef: torch.Tensor, scale: float, w: torc
     '''w += scale * coef[:k] @ V[:k] with the coefficients on the device.'''
     check(_lib.tt_multiaxpy(w.numel(), k, coef.data_ptr(), float(scale), V.data_ptr(), V.stride(0), w.data_ptr(),
                             stream()), 'multiaxpy')
+
+
+def eigh_batched(matrices: torch.Tensor, sizes, offsets=None):
+    '''Eigenvalues (ascending) and eigenvectors of the symmetric FP64 matrices packed in `matrices`
+    (matrix b is sizes[b] x sizes[b], at offsets[b], default back to back), on the device.  Returns
+    (values, vectors): values packed back to back, vectors with eigenvector j of matrix b as row j
+    of its n x n at the matrix's offset.  `matrices` is overwritten.'''
+    if matrices.dtype != torch.float64:
+        raise TypeError('the batched eigensolver runs in float64')
+    require_device(matrices)
+    count = len(sizes)
+    problems = (EighProblem * max(count, 1))()
+    at, value_at = 0, 0
+    for b, n in enumerate(sizes):
+        offset = at if offsets is None else offsets[b]
+        problems[b] = EighProblem(offset, value_at, n)
+        at = offset + n * n
+        value_at += n
+    if offsets is None and at > matrices.numel():
+        raise ValueError('the matrices do not fit the buffer')
+    values = torch.empty(max(value_at, 1), dtype=torch.float64, device=matrices.device)
+    vectors = torch.empty_like(matrices)
+    size = _lib.tt_eigh_workspace(problems, count)
+    if size < 0:
+        raise RuntimeError(last_error())
+    workspace = torch.empty(max(size, 1), dtype=torch.float64, device=matrices.device)
+    check(_lib.tt_eigh_batched(problems, count, matrices.data_ptr(), values.data_ptr(), vectors.data_ptr(),
+                               workspace.data_ptr(), stream()), 'eigh')
+    return values[:value_at], vectors
 
 
 def convert(x: torch.Tensor, y: torch.Tensor) -> None:

@@ -1,0 +1,505 @@
+// The device library, part four: batched symmetric eigendecomposition in FP64 for the split of the
+// two-site tensor (the Gram matrices of its blocks), in three launches for a whole batch of
+// matrices of different sizes:
+//
+//   1. Householder tridiagonalisation, one workgroup per matrix.  The rank-2 update of step k is
+//      deferred into step k+1's symmetric matrix-vector pass, so each step reads and writes the
+//      trailing matrix once.  The reflectors stay in the matrix's lower part.
+//   2. Cuppen divide and conquer on the tridiagonal, one workgroup per matrix, bottom up from 1 x 1
+//      leaves: deflation of small coupling components and of close poles (Givens rotations), the
+//      secular roots by safeguarded Newton from the nearer pole (one wave per root), eigenvectors
+//      by the Gu-Eisenstat formula (orthogonal to working precision however close the roots), and
+//      the merge product through LDS tiles.
+//   3. Back-transformation of the eigenvectors by the reflectors, one workgroup per 16-column
+//      panel held in LDS.
+//
+// Eigenvalues come out ascending; eigenvector j is row j of the matrix's n x n output.  gfx950 only.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "common.h"
+
+using ttdev::as_stream;
+using ttdev::fail;
+
+namespace {
+
+constexpr int kEigThreads = 256;
+constexpr int kEigWaves = kEigThreads / 64;
+constexpr int kEigMaxN = 1024;      // largest matrix (LDS vectors are sized for it)
+constexpr int kEigBatch = 64;       // problems per launch (kernel-argument table)
+constexpr int kPanel = 16;          // columns per back-transformation workgroup
+constexpr int kRowsPerThread = (kEigMaxN + kEigThreads - 1) / kEigThreads;
+constexpr int kPerLane = (kEigMaxN + 63) / 64;
+constexpr double kEps = 2.220446049250313e-16;
+
+struct Problem {
+  int64_t matrix;       // offset of the n x n matrix (and of its eigenvectors)
+  int64_t values;       // offset of its n eigenvalues
+  int64_t work;         // offset of its workspace (3 n^2 + 3 n doubles)
+  int32_t n;
+  int32_t first_panel;  // index of its first back-transformation panel in the launch
+};
+
+struct Batch {
+  Problem p[kEigBatch];
+  int32_t count;
+};
+
+__device__ double wave_sum(double v) {
+#pragma unroll
+  for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_xor(v, offset, 64);
+  return v;
+}
+
+// Sum over the workgroup in a fixed order; every thread gets it.
+__device__ double block_sum(double v, double *scratch) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) scratch[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double total = 0.0;
+  for (int w = 0; w < kEigWaves; ++w) total += scratch[w];
+  return total;
+}
+
+// ---------------------------------------------------------------------------------------------
+// 1. Tridiagonalisation.  A is column-major n x n (symmetric, so either order), in place.
+//    Output: diag[n], off[n-1], tau[n-1]; reflector k is A[k+1.., k] with A[k+1, k] = 1.
+
+__global__ __launch_bounds__(kEigThreads) void tridiag_kernel(Batch batch, double *__restrict__ matrices,
+                                                               double *__restrict__ work) {
+  const Problem pr = batch.p[blockIdx.x];
+  const int n = pr.n;
+  double *A = matrices + pr.matrix;
+  double *diag = work + pr.work + 3 * int64_t(n) * n, *off = diag + n, *tau = off + n;
+  __shared__ double vp[kEigMaxN], wp[kEigMaxN], vc[kEigMaxN], wc[kEigMaxN];
+  __shared__ double red[kEigWaves];
+  __shared__ double s_scale, s_tau;
+  const int t = threadIdx.x;
+  for (int i = t; i < n; i += kEigThreads) vp[i] = wp[i] = 0.0;
+  __syncthreads();
+  for (int k = 0; k + 1 < n; ++k) {
+    double *col = A + int64_t(k) * n;
+    // (a) column k, rows k.., brought up to date with the previous step's update
+    double sigma = 0.0;
+    for (int i = k + t; i < n; i += kEigThreads) {
+      const double a = col[i] - (vp[i] * wp[k] + wp[i] * vp[k]);
+      col[i] = a;
+      if (i >= k + 2) sigma += a * a;
+    }
+    sigma = block_sum(sigma, red);
+    if (t == 0) {
+      const double alpha = col[k + 1];
+      diag[k] = col[k];
+      if (sigma == 0.0) {
+        off[k] = alpha;
+        s_tau = 0.0;
+        s_scale = 0.0;
+      } else {
+        const double norm = sqrt(alpha * alpha + sigma);
+        const double beta = alpha >= 0.0 ? -norm : norm;
+        off[k] = beta;
+        s_tau = (beta - alpha) / beta;
+        s_scale = 1.0 / (alpha - beta);
+      }
+      tau[k] = s_tau;
+      col[k + 1] = 1.0;
+    }
+    __syncthreads();
+    const double tk = s_tau, scale = s_scale;
+    for (int i = k + 1 + t; i < n; i += kEigThreads) {
+      double v = 1.0;
+      if (i >= k + 2) {
+        v = col[i] * scale;
+        col[i] = v;
+      }
+      vc[i] = v;
+    }
+    __syncthreads();
+    // (b) one pass over the trailing matrix: apply the previous update, form p = tau A v
+    double p[kRowsPerThread], vpi[kRowsPerThread], wpi[kRowsPerThread];
+#pragma unroll
+    for (int r = 0; r < kRowsPerThread; ++r) {
+      const int i = k + 1 + t + r * kEigThreads;
+      p[r] = 0.0;
+      vpi[r] = i < n ? vp[i] : 0.0;
+      wpi[r] = i < n ? wp[i] : 0.0;
+    }
+    for (int j = k + 1; j < n; ++j) {
+      double *cj = A + int64_t(j) * n;
+      const double vpj = vp[j], wpj = wp[j], vcj = vc[j];
+#pragma unroll
+      for (int r = 0; r < kRowsPerThread; ++r) {
+        const int i = k + 1 + t + r * kEigThreads;
+        if (i < n) {
+          const double a = cj[i] - (vpi[r] * wpj + wpi[r] * vpj);
+          cj[i] = a;
+          p[r] += a * vcj;
+        }
+      }
+    }
+    double pv = 0.0;
+#pragma unroll
+    for (int r = 0; r < kRowsPerThread; ++r) {
+      const int i = k + 1 + t + r * kEigThreads;
+      if (i < n) {
+        p[r] *= tk;
+        pv += p[r] * vc[i];
+      }
+    }
+    pv = block_sum(pv, red);
+#pragma unroll
+    for (int r = 0; r < kRowsPerThread; ++r) {
+      const int i = k + 1 + t + r * kEigThreads;
+      if (i < n) wc[i] = p[r] - 0.5 * tk * pv * vc[i];
+    }
+    __syncthreads();
+    for (int i = t; i < n; i += kEigThreads) {
+      vp[i] = i > k ? vc[i] : 0.0;
+      wp[i] = i > k ? wc[i] : 0.0;
+    }
+    __syncthreads();
+  }
+  if (t == 0) diag[n - 1] = A[int64_t(n - 1) * n + n - 1] - 2.0 * vp[n - 1] * wp[n - 1];
+}
+
+// ---------------------------------------------------------------------------------------------
+// 2. Divide and conquer.  Q and Qt are column-major n x n (Q[r + c n]); U is a merge's k x k.
+
+struct Rotation {
+  int32_t a, b;  // local columns
+  double c, s;
+};
+
+// Offsets of the kept poles from pole o, in the lane's registers (kept pole i = lane + 64 c).
+__device__ void offsets_from(const double *dk, int k, int o, double *local) {
+  const int lane = threadIdx.x & 63;
+  for (int c = 0, i = lane; c < kPerLane; ++c, i += 64) local[c] = i < k ? dk[i] - dk[o] : 0.0;
+}
+
+// f(x) = 1 + rho sum z2_i / (delta_i - x) and f'(x) over the wave.
+__device__ void secular(const double *local, const double *z2, int k, double rho, double x, double *f, double *df) {
+  const int lane = threadIdx.x & 63;
+  double sf = 0.0, sd = 0.0;
+  for (int c = 0, i = lane; c < kPerLane; ++c, i += 64)
+    if (i < k) {
+      const double inv = 1.0 / (local[c] - x);
+      sf += z2[i] * inv;
+      sd += z2[i] * inv * inv;
+    }
+  *f = 1.0 + rho * wave_sum(sf);
+  *df = rho * wave_sum(sd);
+}
+
+__global__ __launch_bounds__(kEigThreads) void dc_kernel(Batch batch, double *__restrict__ values,
+                                                          double *__restrict__ work) {
+  const Problem pr = batch.p[blockIdx.x];
+  const int n = pr.n;
+  double *Q = work + pr.work, *Qt = Q + int64_t(n) * n, *U = Q + 2 * int64_t(n) * n;
+  const double *diag = Q + 3 * int64_t(n) * n, *off = diag + n;
+  double *lam = values + pr.values;
+  __shared__ double dloc[kEigMaxN], zloc[kEigMaxN], pole[kEigMaxN], zsort[kEigMaxN];
+  __shared__ double dk[kEigMaxN], z2[kEigMaxN], zk[kEigMaxN], tau[kEigMaxN], zh[kEigMaxN], val[kEigMaxN];
+  __shared__ int32_t perm[kEigMaxN], defl[kEigMaxN], keep[kEigMaxN], org[kEigMaxN], src[kEigMaxN], rnk[kEigMaxN];
+  __shared__ Rotation rot[kEigMaxN];  // each rotation deflates one pole: at most s - 1
+  __shared__ double tile_a[32][33], tile_b[32][33];
+  __shared__ int s_k, s_rot;
+  __shared__ double s_rho;
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  // leaves: d_i minus the coupling of the cut on either side (Cuppen); Q = I
+  for (int i = t; i < n; i += kEigThreads) {
+    double v = diag[i];
+    if (i > 0) v -= off[i - 1];
+    if (i + 1 < n) v -= off[i];
+    lam[i] = v;
+  }
+  for (int64_t x = t; x < int64_t(n) * n; x += kEigThreads) Q[x] = (x % (n + 1) == 0) ? 1.0 : 0.0;
+  __syncthreads();
+  for (int width = 1; width < n; width *= 2) {
+    for (int lo = 0; lo + width < n; lo += 2 * width) {
+      const int mid = lo + width, hi = min(lo + 2 * width, n), s = hi - lo, left = mid - lo;
+      const double beta = off[mid - 1];
+      const double sign = beta < 0.0 ? -1.0 : 1.0;
+      // (a) coupling vector (last row of the left block, first row of the right) and poles
+      for (int i = t; i < s; i += kEigThreads) {
+        const double q = i < left ? Q[(mid - 1) + int64_t(lo + i) * n] : Q[mid + int64_t(lo + i) * n];
+        zloc[i] = q * 0.7071067811865476;
+        dloc[i] = sign * lam[lo + i];
+      }
+      __syncthreads();
+      // (b) one thread: merge the sorted halves, deflate small z and close poles
+      if (t == 0) {
+        const double rho = 2.0 * beta * sign;
+        int a = 0, b = 0;
+        for (int p = 0; p < s; ++p) {
+          // ascending in dloc: a negated half is descending, so it is walked from its end
+          const int ia = sign > 0 ? a : left - 1 - a, ib = sign > 0 ? left + b : s - 1 - b;
+          const bool take_a = b >= s - left || (a < left && dloc[ia] <= dloc[ib]);
+          perm[p] = take_a ? ia : ib;
+          if (take_a) ++a; else ++b;
+        }
+        double dmax = 0.0;
+        for (int p = 0; p < s; ++p) {
+          pole[p] = dloc[perm[p]];
+          zsort[p] = zloc[perm[p]];
+          dmax = fmax(dmax, fabs(pole[p]));
+        }
+        const double tol = 8.0 * kEps * fmax(dmax, rho);
+        int nrot = 0, prev = -1;
+        for (int p = 0; p < s; ++p) {
+          defl[p] = rho * fabs(zsort[p]) <= tol ? 1 : 0;
+          if (defl[p]) continue;
+          if (prev >= 0) {
+            const double r = hypot(zsort[p], zsort[prev]);
+            const double c = zsort[p] / r, sn = -zsort[prev] / r;
+            if (fabs((pole[p] - pole[prev]) * c * sn) <= tol) {
+              zsort[p] = r;
+              zsort[prev] = 0.0;
+              rot[nrot++] = Rotation{perm[prev], perm[p], c, sn};
+              const double dp = pole[prev] * c * c + pole[p] * sn * sn;
+              pole[p] = pole[prev] * sn * sn + pole[p] * c * c;
+              pole[prev] = dp;
+              defl[prev] = 1;
+            }
+          }
+          prev = p;
+        }
+        int k = 0;
+        for (int p = 0; p < s; ++p)
+          if (!defl[p]) keep[k++] = p;
+        s_k = k;
+        s_rot = nrot;
+        s_rho = rho;
+      }
+      __syncthreads();
+      const int k = s_k, nrot = s_rot;
+      const double rho = s_rho;
+      // (c) the deflating rotations on Q's columns, in order; each thread owns its rows
+      for (int r = lo + t; r < hi; r += kEigThreads)
+        for (int q = 0; q < nrot; ++q) {
+          double *ca = Q + r + int64_t(lo + rot[q].a) * n, *cb = Q + r + int64_t(lo + rot[q].b) * n;
+          const double x = *ca, y = *cb;
+          *ca = rot[q].c * x + rot[q].s * y;
+          *cb = -rot[q].s * x + rot[q].c * y;
+        }
+      for (int j = t; j < k; j += kEigThreads) {
+        dk[j] = pole[keep[j]];
+        zk[j] = zsort[keep[j]];
+        z2[j] = zk[j] * zk[j];
+      }
+      __syncthreads();
+      // (d) secular roots: one wave per root, safeguarded Newton from the nearer pole
+      for (int j = wave; j < k; j += kEigWaves) {
+        double local[kPerLane];
+        int o = j;
+        double a = 0.0, b;
+        offsets_from(dk, k, j, local);
+        if (j + 1 < k) {
+          const double gap = dk[j + 1] - dk[j];
+          double f, df;
+          secular(local, z2, k, rho, 0.5 * gap, &f, &df);
+          if (f > 0.0) {
+            b = 0.5 * gap;
+          } else {
+            o = j + 1;
+            a = -0.5 * gap;
+            b = 0.0;
+            offsets_from(dk, k, o, local);
+          }
+        } else {
+          double zs = 0.0;
+          for (int i = lane; i < k; i += 64) zs += z2[i];
+          b = rho * wave_sum(zs);
+        }
+        double x = 0.5 * (a + b);
+        for (int it = 0; it < 120; ++it) {
+          double f, df;
+          secular(local, z2, k, rho, x, &f, &df);
+          if (f > 0.0) b = x; else a = x;
+          if (f == 0.0 || b - a <= 4.0 * kEps * fmax(fabs(a), fabs(b))) break;
+          double next = x - f / df;
+          if (!(next > a && next < b)) next = 0.5 * (a + b);
+          if (next == x) break;
+          x = next;
+        }
+        if (lane == 0) {
+          org[j] = o;
+          tau[j] = x;
+        }
+      }
+      __syncthreads();
+      // (e) Gu-Eisenstat: zh_i^2 = (lam_i - d_i)/rho prod_{j != i} (lam_j - d_i)/(d_j - d_i)
+      for (int i = t; i < k; i += kEigThreads) {
+        double w = ((dk[org[i]] - dk[i]) + tau[i]) / rho;
+        for (int j = 0; j < k; ++j)
+          if (j != i) w *= ((dk[org[j]] - dk[i]) + tau[j]) / (dk[j] - dk[i]);
+        zh[i] = copysign(sqrt(fabs(w)), zk[i]);
+      }
+      __syncthreads();
+      // (f) U[:, j] = zh / (d - lam_j), normalised (k x k, column-major)
+      for (int j = t; j < k; j += kEigThreads) {
+        double norm = 0.0;
+        for (int i = 0; i < k; ++i) {
+          const double u = zh[i] / ((dk[i] - dk[org[j]]) - tau[j]);
+          U[i + int64_t(j) * k] = u;
+          norm += u * u;
+        }
+        const double inv = 1.0 / sqrt(norm);
+        for (int i = 0; i < k; ++i) U[i + int64_t(j) * k] *= inv;
+      }
+      // (g) slots: 0..k-1 the roots, then the deflated poles; their values and sources
+      if (t == 0) {
+        int slot = k;
+        for (int p = 0; p < s; ++p)
+          if (defl[p]) {
+            val[slot] = pole[p];
+            src[slot] = perm[p];
+            ++slot;
+          }
+      }
+      for (int j = t; j < k; j += kEigThreads) {
+        val[j] = dk[org[j]] + tau[j];
+        src[j] = -1;
+      }
+      __syncthreads();
+      // (h) ascending rank of every slot (ties by slot), and its final column
+      for (int x = t; x < s; x += kEigThreads) {
+        int r = 0;
+        for (int y = 0; y < s; ++y) r += (val[y] < val[x] || (val[y] == val[x] && y < x)) ? 1 : 0;
+        rnk[x] = sign > 0 ? r : s - 1 - r;
+        lam[lo + rnk[x]] = sign * val[x];
+      }
+      __syncthreads();
+      // (i) deflated columns copied, root columns = Q[:, kept] U, into Qt
+      for (int x = k; x < s; ++x)
+        for (int r = lo + t; r < hi; r += kEigThreads)
+          Qt[r + int64_t(lo + rnk[x]) * n] = Q[r + int64_t(lo + src[x]) * n];
+      const int ty = t >> 3, tx = (t & 7) * 4;
+      for (int r0 = 0; r0 < s; r0 += 32)
+        for (int j0 = 0; j0 < k; j0 += 32) {
+          double acc[4] = {0.0, 0.0, 0.0, 0.0};
+          for (int i0 = 0; i0 < k; i0 += 32) {
+            __syncthreads();
+            for (int e = t; e < 32 * 32; e += kEigThreads) {
+              const int rr = e >> 5, ii = e & 31;
+              const int row = r0 + rr, kk = i0 + ii;
+              tile_a[rr][ii] = (row < s && kk < k) ? Q[lo + row + int64_t(lo + perm[keep[kk]]) * n] : 0.0;
+              const int kb = i0 + rr, jb = j0 + ii;
+              tile_b[rr][ii] = (kb < k && jb < k) ? U[kb + int64_t(jb) * k] : 0.0;
+            }
+            __syncthreads();
+#pragma unroll 8
+            for (int ii = 0; ii < 32; ++ii) {
+              const double a = tile_a[ty][ii];
+#pragma unroll
+              for (int c = 0; c < 4; ++c) acc[c] += a * tile_b[ii][tx + c];
+            }
+          }
+          const int row = r0 + ty;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const int j = j0 + tx + c;
+            if (row < s && j < k) Qt[lo + row + int64_t(lo + rnk[j]) * n] = acc[c];
+          }
+        }
+      __syncthreads();
+      // (j) the merged block back into Q
+      for (int64_t e = t; e < int64_t(s) * s; e += kEigThreads) {
+        const int r = static_cast<int>(e % s), c = static_cast<int>(e / s);
+        Q[lo + r + int64_t(lo + c) * n] = Qt[lo + r + int64_t(lo + c) * n];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// 3. Back-transformation: Y = H_0 H_1 ... H_{n-2} Z on a panel of 16 columns of Z held in LDS.
+//    Eigenvector c is written as row c of the output.
+
+__global__ __launch_bounds__(kEigThreads) void back_kernel(Batch batch, const double *__restrict__ matrices,
+                                                            double *__restrict__ vectors, const double *__restrict__ work) {
+  // the problem this panel belongs to
+  int b = 0;
+  while (b + 1 < batch.count && batch.p[b + 1].first_panel <= static_cast<int>(blockIdx.x)) ++b;
+  const Problem pr = batch.p[b];
+  const int n = pr.n;
+  const int c0 = (static_cast<int>(blockIdx.x) - pr.first_panel) * kPanel;
+  const int width = min(kPanel, n - c0);
+  const double *A = matrices + pr.matrix;
+  const double *Z = work + pr.work;
+  const double *tau = Z + 3 * int64_t(n) * n + 2 * n;
+  __shared__ double Y[kEigMaxN][kPanel];
+  __shared__ double part[kEigThreads / kPanel][kPanel];
+  __shared__ double w[kPanel];
+  const int t = threadIdx.x, c = t % kPanel, g = t / kPanel, groups = kEigThreads / kPanel;
+  for (int i = g; i < n; i += groups) Y[i][c] = c < width ? Z[i + int64_t(c0 + c) * n] : 0.0;
+  __syncthreads();
+  for (int k = n - 2; k >= 0; --k) {
+    const double tk = tau[k];
+    if (tk == 0.0) continue;
+    const double *v = A + int64_t(k) * n;   // v[k+1] = 1, v[k+2..] stored
+    double sum = 0.0;
+    for (int i = k + 1 + g; i < n; i += groups) sum += v[i] * Y[i][c];
+    part[g][c] = sum;
+    __syncthreads();
+    if (t < kPanel) {
+      double total = 0.0;
+      for (int q = 0; q < groups; ++q) total += part[q][t];
+      w[t] = tk * total;
+    }
+    __syncthreads();
+    for (int i = k + 1 + g; i < n; i += groups) Y[i][c] -= v[i] * w[c];
+    __syncthreads();
+  }
+  double *out = vectors + pr.matrix;
+  if (c < width)
+    for (int i = g; i < n; i += groups) out[int64_t(c0 + c) * n + i] = Y[i][c];
+}
+
+}  // namespace
+
+extern "C" int64_t tt_eigh_workspace(const tt_eigh_problem *problems, int64_t count) {
+  if (count < 0 || (count > 0 && !problems)) {
+    fail("tt_eigh_workspace: bad problem list");
+    return -1;
+  }
+  int64_t total = 0;
+  for (int64_t b = 0; b < count; ++b) total += 3 * problems[b].n * problems[b].n + 3 * problems[b].n;
+  return total;
+}
+
+extern "C" int tt_eigh_batched(const tt_eigh_problem *problems, int64_t count, double *matrices, double *values,
+                               double *vectors, double *workspace, void *stream) {
+  if (count < 0 || (count > 0 && !problems)) return fail("tt_eigh_batched: bad problem list");
+  if (count == 0) return 0;
+  if (!matrices || !values || !vectors || !workspace) return fail("tt_eigh_batched: null buffer");
+  for (int64_t b = 0; b < count; ++b) {
+    const tt_eigh_problem &p = problems[b];
+    if (p.n < 1 || p.n > kEigMaxN)
+      return fail("tt_eigh_batched: matrix order " + std::to_string(p.n) + " outside 1.." + std::to_string(kEigMaxN));
+    if (p.matrix_offset < 0 || p.value_offset < 0) return fail("tt_eigh_batched: negative offset");
+  }
+  int64_t work = 0;
+  for (int64_t first = 0; first < count; first += kEigBatch) {
+    Batch batch{};
+    batch.count = static_cast<int32_t>(std::min<int64_t>(kEigBatch, count - first));
+    int32_t panels = 0;
+    for (int32_t b = 0; b < batch.count; ++b) {
+      const tt_eigh_problem &p = problems[first + b];
+      batch.p[b] = Problem{p.matrix_offset, p.value_offset, work, static_cast<int32_t>(p.n), panels};
+      work += 3 * p.n * p.n + 3 * p.n;
+      panels += static_cast<int32_t>((p.n + kPanel - 1) / kPanel);
+    }
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(tridiag_kernel, dim3(batch.count), dim3(kEigThreads), 0, s, batch, matrices, workspace);
+    TT_LAUNCHED();
+    hipLaunchKernelGGL(dc_kernel, dim3(batch.count), dim3(kEigThreads), 0, s, batch, values, workspace);
+    TT_LAUNCHED();
+    hipLaunchKernelGGL(back_kernel, dim3(panels), dim3(kEigThreads), 0, s, batch, matrices, vectors, workspace);
+    TT_LAUNCHED();
+  }
+  return 0;
+}

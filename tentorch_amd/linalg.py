@@ -36,11 +36,48 @@ def _bond_leg(cat, mults: dict):
 class SvdSettings:
     '''How `svd` factorises the blocks: 'gram' (the default on the GPU for absorb 1 and 2) takes the
     isometric factor from a batched symmetric eigendecomposition of each block's Gram matrix and the
-    other factor as the exact projection of the block on it; 'exact' runs one SVD per block.'''
+    other factor as the exact projection of the block on it; 'exact' runs one SVD per block.
+    `eigh` picks the Gram eigensolver: 'device' (this library's batched tridiagonalisation, divide
+    and conquer and back-transformation, three launches for all blocks) or 'torch'
+    (`torch.linalg.eigh`, batched by size class).'''
     method = 'gram'
+    eigh = 'device'
 
 
 def _gram_factors(blocks, left_isometric: bool):
+    if SvdSettings.eigh == 'device':
+        return _gram_factors_device(blocks, left_isometric)
+    return _gram_factors_torch(blocks, left_isometric)
+
+
+def _gram_factors_device(blocks, left_isometric: bool):
+    '''For each (m x n) block: the isometric factor (m x r or n x r, r = min(m, n)) and the singular
+    values in descending order, from the device library's batched eigendecomposition of the Gram
+    matrices B B^T (left) or B^T B (right), all packed in one buffer.'''
+    if not blocks:
+        return []
+    sizes = [b.shape[0] if left_isometric else b.shape[1] for b in blocks]
+    offsets = np.concatenate([[0], np.cumsum([g * g for g in sizes])]).tolist()
+    packed = torch.empty(max(offsets[-1], 1), dtype=torch.float64, device=blocks[0].device)
+    for b, g, at in zip(blocks, sizes, offsets):
+        gram = packed[at:at + g * g].view(g, g)
+        if left_isometric:
+            torch.matmul(b, b.transpose(0, 1), out=gram)
+        else:
+            torch.matmul(b.transpose(0, 1), b, out=gram)
+    values, vectors = _native.eigh_batched(packed, sizes)
+    out = []
+    value_at = 0
+    for b, g, at in zip(blocks, sizes, offsets):
+        r = min(b.shape)
+        lam = values[value_at + g - r:value_at + g].flip(0)
+        rows = vectors[at:at + g * g].view(g, g)[g - r:]      # eigenvector j is row j
+        out.append((rows.flip(0).transpose(0, 1), lam.clamp(min=0).sqrt()))
+        value_at += g
+    return out
+
+
+def _gram_factors_torch(blocks, left_isometric: bool):
     '''For each (m x n) block: the isometric factor Q (m x r or n x r, r = min(m, n)) with the
     singular values in descending order, from batched eigh of B B^T (left) or B^T B (right).  Blocks
     are batched by Gram size in buckets that differ by at most 25 %; each Gram matrix is padded with

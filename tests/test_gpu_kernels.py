@@ -240,3 +240,51 @@ def test_complex_grouped_gemm_matches_numpy(lib, config):
         assert status == 0, cabi.error(lib)
         assert np.abs(dc.cpu().numpy() - ref).max() <= 1e-12 * np.abs(ref).max(), plain
     lib.tt_ggemm_plan_destroy(plan)
+
+
+def _eigh_cases(rng):
+    '''Symmetric matrices of every kind the split meets: random, Gram matrices with geometric
+    spectra and rank deficits, exact degeneracies, diagonal, zero, and sizes 1 to 700.'''
+    cases = []
+    for n in (1, 2, 3, 5, 17, 64, 130):
+        a = rng.standard_normal((n, n))
+        cases.append(a + a.T)
+    for n in (50, 384, 700):
+        s = np.geomspace(1, 1e-9, n)
+        s[(2 * n) // 3:] = 0
+        u, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        v, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        b = (u * s) @ v.T
+        cases.append(b @ b.T)
+    half = rng.standard_normal((100, 100))
+    half = half + half.T
+    p, _ = np.linalg.qr(rng.standard_normal((200, 200)))
+    cases.append(p @ np.kron(np.eye(2), half) @ p.T)
+    cases.append(np.diag(rng.standard_normal(40)))
+    cases.append(np.zeros((9, 9)))
+    cases.append(np.eye(30))
+    return cases
+
+
+def test_batched_eigh_matches_numpy():
+    '''tt_eigh_batched on one batch of mixed sizes: eigenvalues within 1e-12 of numpy's (relative to
+    the matrix scale), eigenvectors orthonormal to 1e-12 with residuals below 1e-12 of the scale.'''
+    from tentorch_amd import _native
+    rng = np.random.default_rng(21)
+    cases = _eigh_cases(rng)
+    packed = torch.tensor(np.concatenate([c.reshape(-1) for c in cases]), device='cuda')
+    values, vectors = _native.eigh_batched(packed, [c.shape[0] for c in cases])
+    values, vectors = values.cpu().numpy(), vectors.cpu().numpy()
+    at = vat = 0
+    for c in cases:
+        n = c.shape[0]
+        lam = values[vat:vat + n]
+        rows = vectors[at:at + n * n].reshape(n, n)      # eigenvector j is row j
+        ref = np.linalg.eigvalsh(c)
+        scale = max(1.0, np.abs(ref).max())
+        assert np.all(np.diff(lam) >= 0), n
+        assert np.abs(lam - ref).max() <= 1e-12 * scale, n
+        assert np.abs(rows @ rows.T - np.eye(n)).max() <= 1e-12, n
+        assert np.abs(c @ rows.T - rows.T * lam).max() <= 1e-12 * scale, n
+        at += n * n
+        vat += n

@@ -197,3 +197,27 @@ def test_complex_recoupling_on_a_real_tensor_is_refused():
                          tt.fibonacci.trivial(), generator=rng)
     with pytest.raises(ValueError, match='complex'):
         tt.einsum('ABCd,Be->ACde', x, y)
+
+
+def test_split_by_the_device_eigensolver_matches_the_exact_svd():
+    '''The DMRG split (absorb 2: U isometric, values into V) through the batched device eigensolver
+    and through one exact SVD per block: the same singular values and the same reconstruction.'''
+    from tentorch_amd.linalg import SvdSettings
+    rng = np.random.default_rng(31)
+    bond = tt.Leg(tt.u1, [(-2,), (0,), (2,)], [9, 14, 9])
+    phys = tt.Leg(tt.u1, [(-1,), (1,)], [1, 1])
+    two = tt.Tensor.random(tt.u1, [bond, phys], [phys, bond], (0,), generator=rng)
+    results = {}
+    for method, eigh in (('gram', 'device'), ('gram', 'torch'), ('exact', 'device')):
+        SvdSettings.method, SvdSettings.eigh = method, eigh
+        try:
+            results[(method, eigh)] = tt.svd(two, None, 20, 1e-14, 2)
+        finally:
+            SvdSettings.method, SvdSettings.eigh = 'gram', 'device'
+    ref_u, ref_s, ref_v = results[('exact', 'device')]
+    ref_rec = tt.contract(ref_u, ref_v, tt.domain_legs(ref_u), [0]).data
+    for key in (('gram', 'device'), ('gram', 'torch')):
+        u, s, v = results[key]
+        assert torch.allclose(s.sort().values, ref_s.sort().values, rtol=1e-10, atol=1e-13), key
+        rec = tt.contract(u, v, tt.domain_legs(u), [0]).data
+        assert (rec - ref_rec).abs().max() <= 1e-12 * ref_rec.abs().max(), key
