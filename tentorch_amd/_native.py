@@ -6,6 +6,7 @@ library is missing, or no GPU is visible when device work is asked for, this mod
 
 import ctypes
 import os
+from typing import Optional
 
 import torch
 
@@ -149,11 +150,12 @@ def multiaxpy(V: torch.Tensor, k: int, coef: torch.Tensor, scale: float, w: torc
                             stream()), 'multiaxpy')
 
 
-def eigh_batched(matrices: torch.Tensor, sizes, offsets=None):
+def eigh_batched(matrices: torch.Tensor, sizes, offsets=None, work: Optional[torch.Tensor] = None):
     '''Eigenvalues (ascending) and eigenvectors of the symmetric FP64 matrices packed in `matrices`
     (matrix b is sizes[b] x sizes[b], at offsets[b], default back to back), on the device.  Returns
     (values, vectors): values packed back to back, vectors with eigenvector j of matrix b as row j
-    of its n x n at the matrix's offset.  `matrices` is overwritten.'''
+    of its n x n at the matrix's offset.  `matrices` is overwritten; `work` may supply the workspace
+    (any contents).'''
     if matrices.dtype != torch.float64:
         raise TypeError('the batched eigensolver runs in float64')
     require_device(matrices)
@@ -172,7 +174,9 @@ def eigh_batched(matrices: torch.Tensor, sizes, offsets=None):
     size = _lib.tt_eigh_workspace(problems, count)
     if size < 0:
         raise RuntimeError(last_error())
-    workspace = torch.empty(max(size, 1), dtype=torch.float64, device=matrices.device)
+    workspace = torch.empty(max(size, 1), dtype=torch.float64, device=matrices.device) if work is None else work
+    if workspace.numel() < size or workspace.dtype != torch.float64:
+        raise ValueError(f'the eigensolver workspace holds {workspace.numel()} doubles, {size} needed')
     check(_lib.tt_eigh_batched(problems, count, matrices.data_ptr(), values.data_ptr(), vectors.data_ptr(),
                                workspace.data_ptr(), stream()), 'eigh')
     return values[:value_at], vectors

@@ -7,7 +7,7 @@
 //      trailing matrix once.  The reflectors stay in the matrix's lower part.
 //   2. Cuppen divide and conquer on the tridiagonal, one workgroup per matrix, bottom up from 1 x 1
 //      leaves: deflation of small coupling components and of close poles (Givens rotations), the
-//      secular roots by safeguarded Newton from the nearer pole (one wave per root), eigenvectors
+//      secular roots by rational ("middle way") steps from the nearer pole (one thread per root), eigenvectors
 //      by the Gu-Eisenstat formula (orthogonal to working precision however close the roots), and
 //      the merge product through LDS tiles.
 //   3. Back-transformation of the eigenvectors by the reflectors, one workgroup per 16-column
@@ -30,8 +30,6 @@ constexpr int kEigWaves = kEigThreads / 64;
 constexpr int kEigMaxN = 1024;      // largest matrix (LDS vectors are sized for it)
 constexpr int kEigBatch = 64;       // problems per launch (kernel-argument table)
 constexpr int kPanel = 16;          // columns per back-transformation workgroup
-constexpr int kRowsPerThread = (kEigMaxN + kEigThreads - 1) / kEigThreads;
-constexpr int kPerLane = (kEigMaxN + 63) / 64;
 constexpr double kEps = 2.220446049250313e-16;
 
 struct Problem {
@@ -53,43 +51,50 @@ __device__ double wave_sum(double v) {
   return v;
 }
 
-// Sum over the workgroup in a fixed order; every thread gets it.
-__device__ double block_sum(double v, double *scratch) {
+// ---------------------------------------------------------------------------------------------
+// 1. Tridiagonalisation.  A is column-major n x n (symmetric, so either order), in place.
+//    Output: diag[n], off[n-1], tau[n-1]; reflector k is A[k+1.., k] with A[k+1, k] = 1.
+//    1024 threads: in the trailing pass thread t takes rows k+1 + t % 256 (+ 256 r) of every fourth
+//    column (group t / 256), four columns at a time, so each thread keeps sixteen loads in flight;
+//    the four groups' partial products are summed in LDS in group order.
+
+constexpr int kTriThreads = 1024;
+constexpr int kTriGroups = kTriThreads / 256;
+constexpr int kTriRows = (kEigMaxN + 255) / 256;
+
+__device__ double tri_block_sum(double v, double *scratch) {
   v = wave_sum(v);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) scratch[threadIdx.x >> 6] = v;
   __syncthreads();
   double total = 0.0;
-  for (int w = 0; w < kEigWaves; ++w) total += scratch[w];
+  for (int w = 0; w < kTriThreads / 64; ++w) total += scratch[w];
   return total;
 }
 
-// ---------------------------------------------------------------------------------------------
-// 1. Tridiagonalisation.  A is column-major n x n (symmetric, so either order), in place.
-//    Output: diag[n], off[n-1], tau[n-1]; reflector k is A[k+1.., k] with A[k+1, k] = 1.
-
-__global__ __launch_bounds__(kEigThreads) void tridiag_kernel(Batch batch, double *__restrict__ matrices,
+__global__ __launch_bounds__(kTriThreads) void tridiag_kernel(Batch batch, double *__restrict__ matrices,
                                                                double *__restrict__ work) {
   const Problem pr = batch.p[blockIdx.x];
   const int n = pr.n;
-  double *A = matrices + pr.matrix;
+  double *__restrict__ A = matrices + pr.matrix;
   double *diag = work + pr.work + 3 * int64_t(n) * n, *off = diag + n, *tau = off + n;
   __shared__ double vp[kEigMaxN], wp[kEigMaxN], vc[kEigMaxN], wc[kEigMaxN];
-  __shared__ double red[kEigWaves];
+  __shared__ double part[kTriGroups][kEigMaxN];
+  __shared__ double red[kTriThreads / 64];
   __shared__ double s_scale, s_tau;
-  const int t = threadIdx.x;
-  for (int i = t; i < n; i += kEigThreads) vp[i] = wp[i] = 0.0;
+  const int t = threadIdx.x, row0 = t & 255, group = t >> 8;
+  for (int i = t; i < n; i += kTriThreads) vp[i] = wp[i] = 0.0;
   __syncthreads();
   for (int k = 0; k + 1 < n; ++k) {
     double *col = A + int64_t(k) * n;
     // (a) column k, rows k.., brought up to date with the previous step's update
     double sigma = 0.0;
-    for (int i = k + t; i < n; i += kEigThreads) {
+    for (int i = k + t; i < n; i += kTriThreads) {
       const double a = col[i] - (vp[i] * wp[k] + wp[i] * vp[k]);
       col[i] = a;
       if (i >= k + 2) sigma += a * a;
     }
-    sigma = block_sum(sigma, red);
+    sigma = tri_block_sum(sigma, red);
     if (t == 0) {
       const double alpha = col[k + 1];
       diag[k] = col[k];
@@ -109,7 +114,7 @@ __global__ __launch_bounds__(kEigThreads) void tridiag_kernel(Batch batch, doubl
     }
     __syncthreads();
     const double tk = s_tau, scale = s_scale;
-    for (int i = k + 1 + t; i < n; i += kEigThreads) {
+    for (int i = k + 1 + t; i < n; i += kTriThreads) {
       double v = 1.0;
       if (i >= k + 2) {
         v = col[i] * scale;
@@ -119,44 +124,59 @@ __global__ __launch_bounds__(kEigThreads) void tridiag_kernel(Batch batch, doubl
     }
     __syncthreads();
     // (b) one pass over the trailing matrix: apply the previous update, form p = tau A v
-    double p[kRowsPerThread], vpi[kRowsPerThread], wpi[kRowsPerThread];
+    double p[kTriRows], vpi[kTriRows], wpi[kTriRows];
 #pragma unroll
-    for (int r = 0; r < kRowsPerThread; ++r) {
-      const int i = k + 1 + t + r * kEigThreads;
+    for (int r = 0; r < kTriRows; ++r) {
+      const int i = k + 1 + row0 + r * 256;
       p[r] = 0.0;
       vpi[r] = i < n ? vp[i] : 0.0;
       wpi[r] = i < n ? wp[i] : 0.0;
     }
-    for (int j = k + 1; j < n; ++j) {
-      double *cj = A + int64_t(j) * n;
-      const double vpj = vp[j], wpj = wp[j], vcj = vc[j];
+    for (int j0 = k + 1 + group; j0 < n; j0 += 4 * kTriGroups) {
+      double a[4][kTriRows];
 #pragma unroll
-      for (int r = 0; r < kRowsPerThread; ++r) {
-        const int i = k + 1 + t + r * kEigThreads;
-        if (i < n) {
-          const double a = cj[i] - (vpi[r] * wpj + wpi[r] * vpj);
-          cj[i] = a;
-          p[r] += a * vcj;
+      for (int u = 0; u < 4; ++u) {
+        const int j = j0 + u * kTriGroups;
+#pragma unroll
+        for (int r = 0; r < kTriRows; ++r) {
+          const int i = k + 1 + row0 + r * 256;
+          a[u][r] = (j < n && i < n) ? A[i + int64_t(j) * n] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int j = j0 + u * kTriGroups;
+        if (j >= n) continue;
+        const double vpj = vp[j], wpj = wp[j], vcj = vc[j];
+#pragma unroll
+        for (int r = 0; r < kTriRows; ++r) {
+          const int i = k + 1 + row0 + r * 256;
+          if (i < n) {
+            const double x = a[u][r] - (vpi[r] * wpj + wpi[r] * vpj);
+            A[i + int64_t(j) * n] = x;
+            p[r] += x * vcj;
+          }
         }
       }
     }
-    double pv = 0.0;
 #pragma unroll
-    for (int r = 0; r < kRowsPerThread; ++r) {
-      const int i = k + 1 + t + r * kEigThreads;
-      if (i < n) {
-        p[r] *= tk;
-        pv += p[r] * vc[i];
-      }
-    }
-    pv = block_sum(pv, red);
-#pragma unroll
-    for (int r = 0; r < kRowsPerThread; ++r) {
-      const int i = k + 1 + t + r * kEigThreads;
-      if (i < n) wc[i] = p[r] - 0.5 * tk * pv * vc[i];
+    for (int r = 0; r < kTriRows; ++r) {
+      const int i = k + 1 + row0 + r * 256;
+      if (i < n) part[group][i] = p[r];
     }
     __syncthreads();
-    for (int i = t; i < n; i += kEigThreads) {
+    double pv = 0.0;
+    for (int i = k + 1 + t; i < n; i += kTriThreads) {
+      double total = 0.0;
+      for (int g = 0; g < kTriGroups; ++g) total += part[g][i];
+      total *= tk;
+      wc[i] = total;   // p, for now
+      pv += total * vc[i];
+    }
+    pv = tri_block_sum(pv, red);
+    for (int i = k + 1 + t; i < n; i += kTriThreads) wc[i] = wc[i] - 0.5 * tk * pv * vc[i];
+    __syncthreads();
+    for (int i = t; i < n; i += kTriThreads) {
       vp[i] = i > k ? vc[i] : 0.0;
       wp[i] = i > k ? wc[i] : 0.0;
     }
@@ -173,32 +193,41 @@ struct Rotation {
   double c, s;
 };
 
-// Offsets of the kept poles from pole o, in the lane's registers (kept pole i = lane + 64 c).
-__device__ void offsets_from(const double *dk, int k, int o, double *local) {
-  const int lane = threadIdx.x & 63;
-  for (int c = 0, i = lane; c < kPerLane; ++c, i += 64) local[c] = i < k ? dk[i] - dk[o] : 0.0;
-}
+// The secular function f(x) = 1 + psi(x) + phi(x) of root j, with x measured from pole o: psi sums
+// over the kept poles 0..j (at or left of the root's interval), phi over the rest; both with their
+// derivatives.  One thread evaluates it serially; every thread of the workgroup reads the same pole
+// at the same time, an LDS broadcast.
+struct Secular {
+  double f, psi, phi, dpsi, dphi;
+};
 
-// f(x) = 1 + rho sum z2_i / (delta_i - x) and f'(x) over the wave.
-__device__ void secular(const double *local, const double *z2, int k, double rho, double x, double *f, double *df) {
-  const int lane = threadIdx.x & 63;
-  double sf = 0.0, sd = 0.0;
-  for (int c = 0, i = lane; c < kPerLane; ++c, i += 64)
-    if (i < k) {
-      const double inv = 1.0 / (local[c] - x);
-      sf += z2[i] * inv;
-      sd += z2[i] * inv * inv;
-    }
-  *f = 1.0 + rho * wave_sum(sf);
-  *df = rho * wave_sum(sd);
+__device__ Secular secular(const double *dk, const double *z2, int k, int j, int o, double rho, double x) {
+  const double origin = dk[o];
+  double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0;
+  for (int i = 0; i <= j; ++i) {
+    const double inv = 1.0 / ((dk[i] - origin) - x);
+    const double w = z2[i] * inv;
+    psi += w;
+    dpsi += w * inv;
+  }
+  for (int i = j + 1; i < k; ++i) {
+    const double inv = 1.0 / ((dk[i] - origin) - x);
+    const double w = z2[i] * inv;
+    phi += w;
+    dphi += w * inv;
+  }
+  return Secular{1.0 + rho * (psi + phi), rho * psi, rho * phi, rho * dpsi, rho * dphi};
 }
 
 __global__ __launch_bounds__(kEigThreads) void dc_kernel(Batch batch, double *__restrict__ values,
                                                           double *__restrict__ work) {
   const Problem pr = batch.p[blockIdx.x];
   const int n = pr.n;
-  double *Q = work + pr.work, *Qt = Q + int64_t(n) * n, *U = Q + 2 * int64_t(n) * n;
-  const double *diag = Q + 3 * int64_t(n) * n, *off = diag + n;
+  double *const base = work + pr.work;
+  double *__restrict__ U = base + 2 * int64_t(n) * n;
+  const double *diag = base + 3 * int64_t(n) * n, *off = diag + n;
+  // each level reads its blocks from Q and writes the merged blocks to Qt; the two then swap
+  double *Q = base, *Qt = base + int64_t(n) * n;
   double *lam = values + pr.values;
   __shared__ double dloc[kEigMaxN], zloc[kEigMaxN], pole[kEigMaxN], zsort[kEigMaxN];
   __shared__ double dk[kEigMaxN], z2[kEigMaxN], zk[kEigMaxN], tau[kEigMaxN], zh[kEigMaxN], val[kEigMaxN];
@@ -215,9 +244,23 @@ __global__ __launch_bounds__(kEigThreads) void dc_kernel(Batch batch, double *__
     if (i + 1 < n) v -= off[i];
     lam[i] = v;
   }
-  for (int64_t x = t; x < int64_t(n) * n; x += kEigThreads) Q[x] = (x % (n + 1) == 0) ? 1.0 : 0.0;
+  // Both buffers start as I and 0: a merge reads its whole block, whose off-diagonal parts (rows of
+  // one child, columns of the other) no merge below writes and must be zero.
+  for (int64_t x = t; x < int64_t(n) * n; x += kEigThreads) {
+    Q[x] = (x % (n + 1) == 0) ? 1.0 : 0.0;
+    Qt[x] = 0.0;
+  }
   __syncthreads();
   for (int width = 1; width < n; width *= 2) {
+    // a block with no partner at this level moves over unchanged
+    {
+      const int lo = ((n - 1) / (2 * width)) * (2 * width);
+      if (lo + width >= n) {
+        const int s = n - lo;
+        for (int64_t e = t; e < int64_t(s) * s; e += kEigThreads)
+          Qt[lo + e % s + int64_t(lo + e / s) * n] = Q[lo + e % s + int64_t(lo + e / s) * n];
+      }
+    }
     for (int lo = 0; lo + width < n; lo += 2 * width) {
       const int mid = lo + width, hi = min(lo + 2 * width, n), s = hi - lo, left = mid - lo;
       const double beta = off[mid - 1];
@@ -290,44 +333,66 @@ __global__ __launch_bounds__(kEigThreads) void dc_kernel(Batch batch, double *__
         z2[j] = zk[j] * zk[j];
       }
       __syncthreads();
-      // (d) secular roots: one wave per root, safeguarded Newton from the nearer pole
-      for (int j = wave; j < k; j += kEigWaves) {
-        double local[kPerLane];
+      // (d) secular roots, one thread per root, from the nearer pole of its interval.  Each step
+      //     models the sums left and right of the interval by one pole each, matching value and
+      //     slope at the iterate, and takes the root of that model (the rational "middle way"
+      //     step); steps that leave the bracket bisect it.
+      for (int j = t; j < k; j += kEigThreads) {
         int o = j;
         double a = 0.0, b;
-        offsets_from(dk, k, j, local);
         if (j + 1 < k) {
           const double gap = dk[j + 1] - dk[j];
-          double f, df;
-          secular(local, z2, k, rho, 0.5 * gap, &f, &df);
-          if (f > 0.0) {
+          if (secular(dk, z2, k, j, j, rho, 0.5 * gap).f > 0.0) {
             b = 0.5 * gap;
           } else {
             o = j + 1;
             a = -0.5 * gap;
             b = 0.0;
-            offsets_from(dk, k, o, local);
           }
         } else {
           double zs = 0.0;
-          for (int i = lane; i < k; i += 64) zs += z2[i];
-          b = rho * wave_sum(zs);
+          for (int i = 0; i < k; ++i) zs += z2[i];
+          b = rho * zs;
         }
+        const double lo_pole = dk[j] - dk[o];
+        const double hi_pole = j + 1 < k ? dk[j + 1] - dk[o] : 0.0;
         double x = 0.5 * (a + b);
-        for (int it = 0; it < 120; ++it) {
-          double f, df;
-          secular(local, z2, k, rho, x, &f, &df);
-          if (f > 0.0) b = x; else a = x;
-          if (f == 0.0 || b - a <= 4.0 * kEps * fmax(fabs(a), fabs(b))) break;
-          double next = x - f / df;
+        for (int it = 0; it < 80; ++it) {
+          const Secular v = secular(dk, z2, k, j, o, rho, x);
+          if (v.f > 0.0) b = x; else a = x;
+          // converged when f is within its own rounding error, or the bracket is at the
+          // resolution of x
+          if (fabs(v.f) <= 8.0 * kEps * k * (1.0 + fabs(v.psi) + fabs(v.phi)) ||
+              b - a <= 4.0 * kEps * fmax(fabs(a), fabs(b)))
+            break;
+          const double dlo = lo_pole - x;
+          double eta;
+          if (j + 1 < k) {
+            const double dhi = hi_pole - x;
+            const double S = dlo * dlo * v.dpsi, T = dhi * dhi * v.dphi;
+            const double c = v.f - dlo * v.dpsi - dhi * v.dphi;
+            const double B = c * (dlo + dhi) + S + T, C = c * dlo * dhi + S * dhi + T * dlo;
+            const double root = sqrt(fmax(B * B - 4.0 * c * C, 0.0));
+            if (B >= 0.0)
+              eta = 2.0 * C / (B + root);
+            else if (c != 0.0)
+              eta = (B - root) / (2.0 * c);
+            else
+              eta = C / B;
+          } else {
+            const double c = v.f - dlo * v.dpsi;
+            eta = c != 0.0 ? dlo + dlo * dlo * v.dpsi / c : 0.5 * (b - a);
+          }
+          double next = x + eta;
           if (!(next > a && next < b)) next = 0.5 * (a + b);
-          if (next == x) break;
+          if (fabs(next - x) <= 2.0 * kEps * fabs(x)) {
+            x = next;
+            break;
+          }
           x = next;
         }
-        if (lane == 0) {
-          org[j] = o;
-          tau[j] = x;
-        }
+        org[j] = o;
+        tau[j] = x;
       }
       __syncthreads();
       // (e) Gu-Eisenstat: zh_i^2 = (lam_i - d_i)/rho prod_{j != i} (lam_j - d_i)/(d_j - d_i)
@@ -338,16 +403,16 @@ __global__ __launch_bounds__(kEigThreads) void dc_kernel(Batch batch, double *__
         zh[i] = copysign(sqrt(fabs(w)), zk[i]);
       }
       __syncthreads();
-      // (f) U[:, j] = zh / (d - lam_j), normalised (k x k, column-major)
-      for (int j = t; j < k; j += kEigThreads) {
+      // (f) U[:, j] = zh / (d - lam_j), normalised (k x k, column-major): one wave per column
+      for (int j = wave; j < k; j += kEigWaves) {
+        const double dj = dk[org[j]], tj = tau[j];
         double norm = 0.0;
-        for (int i = 0; i < k; ++i) {
-          const double u = zh[i] / ((dk[i] - dk[org[j]]) - tau[j]);
-          U[i + int64_t(j) * k] = u;
+        for (int i = lane; i < k; i += 64) {
+          const double u = zh[i] / ((dk[i] - dj) - tj);
           norm += u * u;
         }
-        const double inv = 1.0 / sqrt(norm);
-        for (int i = 0; i < k; ++i) U[i + int64_t(j) * k] *= inv;
+        const double inv = 1.0 / sqrt(wave_sum(norm));
+        for (int i = lane; i < k; i += 64) U[i + int64_t(j) * k] = inv * zh[i] / ((dk[i] - dj) - tj);
       }
       // (g) slots: 0..k-1 the roots, then the deflated poles; their values and sources
       if (t == 0) {
@@ -373,9 +438,10 @@ __global__ __launch_bounds__(kEigThreads) void dc_kernel(Batch batch, double *__
       }
       __syncthreads();
       // (i) deflated columns copied, root columns = Q[:, kept] U, into Qt
-      for (int x = k; x < s; ++x)
-        for (int r = lo + t; r < hi; r += kEigThreads)
-          Qt[r + int64_t(lo + rnk[x]) * n] = Q[r + int64_t(lo + src[x]) * n];
+      for (int64_t e = t; e < int64_t(s - k) * s; e += kEigThreads) {
+        const int x = k + static_cast<int>(e / s), r = static_cast<int>(e % s);
+        Qt[lo + r + int64_t(lo + rnk[x]) * n] = Q[lo + r + int64_t(lo + src[x]) * n];
+      }
       const int ty = t >> 3, tx = (t & 7) * 4;
       for (int r0 = 0; r0 < s; r0 += 32)
         for (int j0 = 0; j0 < k; j0 += 32) {
@@ -383,11 +449,11 @@ __global__ __launch_bounds__(kEigThreads) void dc_kernel(Batch batch, double *__
           for (int i0 = 0; i0 < k; i0 += 32) {
             __syncthreads();
             for (int e = t; e < 32 * 32; e += kEigThreads) {
-              const int rr = e >> 5, ii = e & 31;
-              const int row = r0 + rr, kk = i0 + ii;
-              tile_a[rr][ii] = (row < s && kk < k) ? Q[lo + row + int64_t(lo + perm[keep[kk]]) * n] : 0.0;
-              const int kb = i0 + rr, jb = j0 + ii;
-              tile_b[rr][ii] = (kb < k && jb < k) ? U[kb + int64_t(jb) * k] : 0.0;
+              const int fast = e & 31, slow = e >> 5;   // consecutive threads walk down a column
+              const int row = r0 + fast, kk = i0 + slow;
+              tile_a[fast][slow] = (row < s && kk < k) ? Q[lo + row + int64_t(lo + perm[keep[kk]]) * n] : 0.0;
+              const int kb = i0 + fast, jb = j0 + slow;
+              tile_b[fast][slow] = (kb < k && jb < k) ? U[kb + int64_t(jb) * k] : 0.0;
             }
             __syncthreads();
 #pragma unroll 8
@@ -405,14 +471,15 @@ __global__ __launch_bounds__(kEigThreads) void dc_kernel(Batch batch, double *__
           }
         }
       __syncthreads();
-      // (j) the merged block back into Q
-      for (int64_t e = t; e < int64_t(s) * s; e += kEigThreads) {
-        const int r = static_cast<int>(e % s), c = static_cast<int>(e / s);
-        Q[lo + r + int64_t(lo + c) * n] = Qt[lo + r + int64_t(lo + c) * n];
-      }
-      __syncthreads();
     }
+    double *swap = Q;
+    Q = Qt;
+    Qt = swap;
+    __syncthreads();
   }
+  // the result must end in the first buffer, where the back-transformation reads it
+  if (Q != base)
+    for (int64_t e = t; e < int64_t(n) * n; e += kEigThreads) base[e] = Q[e];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -494,7 +561,7 @@ extern "C" int tt_eigh_batched(const tt_eigh_problem *problems, int64_t count, d
       panels += static_cast<int32_t>((p.n + kPanel - 1) / kPanel);
     }
     hipStream_t s = as_stream(stream);
-    hipLaunchKernelGGL(tridiag_kernel, dim3(batch.count), dim3(kEigThreads), 0, s, batch, matrices, workspace);
+    hipLaunchKernelGGL(tridiag_kernel, dim3(batch.count), dim3(kTriThreads), 0, s, batch, matrices, workspace);
     TT_LAUNCHED();
     hipLaunchKernelGGL(dc_kernel, dim3(batch.count), dim3(kEigThreads), 0, s, batch, values, workspace);
     TT_LAUNCHED();

@@ -273,7 +273,10 @@ def test_batched_eigh_matches_numpy():
     rng = np.random.default_rng(21)
     cases = _eigh_cases(rng)
     packed = torch.tensor(np.concatenate([c.reshape(-1) for c in cases]), device='cuda')
-    values, vectors = _native.eigh_batched(packed, [c.shape[0] for c in cases])
+    # a workspace full of garbage: nothing may depend on its contents
+    size = sum(3 * c.shape[0] ** 2 + 3 * c.shape[0] for c in cases)
+    work = torch.full((size,), float('nan'), dtype=torch.float64, device='cuda')
+    values, vectors = _native.eigh_batched(packed, [c.shape[0] for c in cases], work=work)
     values, vectors = values.cpu().numpy(), vectors.cpu().numpy()
     at = vat = 0
     for c in cases:
