@@ -117,7 +117,8 @@ def require_device(t: torch.Tensor) -> None:
 
 
 def stream() -> int:
-    return torch.cuda.current_stream().cuda_stream
+    '''The current HIP stream of the current device, as a raw handle.'''
+    return torch._C._cuda_getCurrentRawStream(torch._C._cuda_getDevice())
 
 
 def dot(dtype: torch.dtype, x: torch.Tensor, y: torch.Tensor, out: torch.Tensor) -> None:

@@ -1,15 +1,16 @@
 '''
 The splittings and the Krylov eigensolver of the MI355X backend.
 
-`svd`/`svdvals`/`qr`/`rq` restate `tentorch/linalg/graded/svd.cpp:41-257`: one factorisation per
-block (on the device, through torch's linear algebra), a global ranking of the singular values on
-the host with the tolerance, the cap and the sector/dual re-pairing, then the factors packed along a
-new bond leg that carries the domain root of each block.  Ties in the ranking are broken by (block,
+`svd`/`svdvals`/`qr`/`rq` restate `tentorch/linalg/graded/svd.cpp:41-257`: the factorisation of
+every block on the device (for the DMRG split, one batched eigendecomposition of the blocks' Gram
+matrices by the device library; otherwise one torch factorisation per block), a global ranking of
+the singular values on the host with the tolerance, the cap and the sector/dual re-pairing, then the
+factors packed along a new bond leg that carries the domain root of each block.  Ties in the ranking are broken by (block,
 index) order, a deterministic choice where the reference's `std::sort` leaves it open
 (`svd.cpp:79-80`).
 
-`lanczos` restates `tentorch/linalg/lanczos.py:8-64`; the scalar products run on the device
-(`tt_dot`) and the Krylov vectors are updated with `tt_axpby`.
+`lanczos` restates `tentorch/linalg/lanczos.py:8-64`; the Krylov basis and its scalars stay on the
+device (`tt_dot`, `tt_multidot`, `tt_multiaxpy`, `tt_axpby`), with one readback per call.
 '''
 
 from typing import Callable, List, Optional, Sequence, Tuple
@@ -135,24 +136,20 @@ def svd(tensor: Tensor, left: Optional[Sequence[int]] = None, max_bond: Optional
             u, v, vh = torch.linalg.svd(block, full_matrices=False)
             factors.append({'key': key, 'u': u, 'v': v, 'vh': vh, 'kept': 0, 'offset': 0})
     host = torch.cat([f['v'] for f in factors]).double().cpu().numpy() if factors else np.zeros(0)
-    candidates = []
-    at = 0
+    # global ranking, largest first; a stable sort keeps ties in (block, index) order
+    sizes = [f['v'].numel() for f in factors]
+    owner = np.repeat(np.arange(len(factors)), sizes)
+    position = np.concatenate([np.arange(n) for n in sizes]) if factors else np.zeros(0, dtype=np.int64)
+    order = np.argsort(-host, kind='stable')
+    above = np.nonzero(host[order] <= tol)[0]
+    order = order[:above[0] if above.size else order.size]
+    if max_bond is not None:
+        order = order[:max_bond]
+    candidates = list(zip(owner[order].tolist(), position[order].tolist()))
+    kept = np.zeros(len(factors), dtype=np.int64)
+    np.maximum.at(kept, owner[order], position[order] + 1)
     for fi, f in enumerate(factors):
-        n = f['v'].numel()
-        for index in range(n):
-            candidates.append((float(host[at + index]), fi, index))
-        at += n
-    candidates.sort(key=lambda c: -c[0])
-    cut = len(candidates)
-    for i, c in enumerate(candidates):
-        if c[0] <= tol:
-            cut = i
-            break
-    candidates = candidates[:cut]
-    if max_bond is not None and len(candidates) > max_bond:
-        candidates = candidates[:max_bond]
-    for value, fi, index in candidates:
-        factors[fi]['kept'] = max(factors[fi]['kept'], index + 1)
+        f['kept'] = int(kept[fi])
     mults = {}
     for f in factors:
         if f['kept']:
@@ -167,7 +164,7 @@ def svd(tensor: Tensor, left: Optional[Sequence[int]] = None, max_bond: Optional
         for f in factors:
             f['kept'] = 0
         used = {}
-        for value, fi, index in candidates:
+        for fi, index in candidates:
             f = factors[fi]
             s = f['key'].domain
             room = paired.get(s)

@@ -10,8 +10,7 @@ the host engine plans once per shape and the device library executes (include/te
 
 Differences from the reference, by design:
 - the default scalar type is float64 (the reference's categories default to complex128,
-  `tentorch/symmetry/cat/abelian/abelian.cpp:47`); complex128 is supported by the leg moves and the
-  reductions, float32 by everything;
+  `tentorch/symmetry/cat/abelian/abelian.cpp:47`); complex128 and float32 are supported throughout;
 - data must live on the GPU for any compute: there is no CPU path.
 '''
 
@@ -430,12 +429,43 @@ class Profiler:
         return out
 
 
+class _Plans:
+    '''Programs by (request, interned structure ids): a lookup that skips building the engine's key.
+    The engine keeps its own cache; this one holds at most `limit` programs, least recent first out.'''
+    limit = 4096
+    table: 'OrderedDict' = OrderedDict()
+
+    @classmethod
+    def get(cls, key, make):
+        program = cls.table.get(key)
+        if program is None:
+            program = make()
+            cls.table[key] = program
+            if len(cls.table) > cls.limit:
+                cls.table.popitem(last=False)
+        else:
+            cls.table.move_to_end(key)
+        return program
+
+    @classmethod
+    def clear(cls):
+        cls.table.clear()
+
+
+def clear_caches() -> None:
+    '''Drop every cached plan and program (the engine's and this module's) and the workspaces.'''
+    _Plans.clear()
+    workspaces.clear()
+    engine.clear_caches()
+
+
 def einsum(spec: str, *tensors: Tensor, outcharge=None) -> Tensor:
     '''`einsum` (`tentorch/linalg/graded/einsum.cpp:444-598`, binding `bindings.cpp:347-361`).'''
     if not tensors:
         raise ValueError('an einsum needs at least one operand')
-    program = engine.plan_einsum(spec, [t.structure for t in tensors],
-                                 None if outcharge is None else tuple(outcharge))
+    charge = None if outcharge is None else tuple(outcharge)
+    key = ('einsum', spec, tuple(t.structure.id for t in tensors), charge)
+    program = _Plans.get(key, lambda: engine.plan_einsum(spec, [t.structure for t in tensors], charge))
     return run(program, tensors, label=spec)
 
 
@@ -443,8 +473,10 @@ def contract(left: Tensor, right: Tensor, left_legs, right_legs, charge=None) ->
     '''`contract` (`tentorch/linalg/graded/contract.cpp:117-175`).'''
     if left.dtype != right.dtype:
         raise ValueError('a contraction needs both tensors of one scalar type')
-    program = engine.plan_contract(left.structure, right.structure, list(left_legs), list(right_legs),
-                                   None if charge is None else tuple(charge))
+    charge = None if charge is None else tuple(charge)
+    key = ('contract', left.structure.id, right.structure.id, tuple(left_legs), tuple(right_legs), charge)
+    program = _Plans.get(key, lambda: engine.plan_contract(left.structure, right.structure, list(left_legs),
+                                                           list(right_legs), charge))
     return run(program, [left, right])
 
 
