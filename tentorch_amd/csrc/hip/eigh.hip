@@ -78,42 +78,46 @@ __global__ __launch_bounds__(kTriThreads) void tridiag_kernel(Batch batch, doubl
   const int n = pr.n;
   double *__restrict__ A = matrices + pr.matrix;
   double *diag = work + pr.work + 3 * int64_t(n) * n, *off = diag + n, *tau = off + n;
-  __shared__ double vp[kEigMaxN], wp[kEigMaxN], vc[kEigMaxN], wc[kEigMaxN];
+  // (v, w) of the previous step and of this one, swapped by index after every step
+  __shared__ double vbuf[2][kEigMaxN], wbuf[2][kEigMaxN];
   __shared__ double part[kTriGroups][kEigMaxN];
   __shared__ double red[kTriThreads / 64];
-  __shared__ double s_scale, s_tau;
+  __shared__ double s_alpha, s_diag;
   const int t = threadIdx.x, row0 = t & 255, group = t >> 8;
-  for (int i = t; i < n; i += kTriThreads) vp[i] = wp[i] = 0.0;
+  for (int i = t; i < n; i += kTriThreads) vbuf[0][i] = wbuf[0][i] = 0.0;
   __syncthreads();
+  int cur = 0;
   for (int k = 0; k + 1 < n; ++k) {
+    const double *vp = vbuf[cur], *wp = wbuf[cur];
+    double *vc = vbuf[cur ^ 1], *wc = wbuf[cur ^ 1];
     double *col = A + int64_t(k) * n;
     // (a) column k, rows k.., brought up to date with the previous step's update
     double sigma = 0.0;
     for (int i = k + t; i < n; i += kTriThreads) {
       const double a = col[i] - (vp[i] * wp[k] + wp[i] * vp[k]);
-      col[i] = a;
-      if (i >= k + 2) sigma += a * a;
+      if (i == k) s_diag = a;
+      if (i == k + 1) s_alpha = a;
+      if (i >= k + 2) {
+        col[i] = a;
+        sigma += a * a;
+      }
     }
     sigma = tri_block_sum(sigma, red);
+    // every thread forms the reflector's scalars from the same values
+    const double alpha = s_alpha;
+    double beta = alpha, tk = 0.0, scale = 0.0;
+    if (sigma != 0.0) {
+      const double norm = sqrt(alpha * alpha + sigma);
+      beta = alpha >= 0.0 ? -norm : norm;
+      tk = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
     if (t == 0) {
-      const double alpha = col[k + 1];
-      diag[k] = col[k];
-      if (sigma == 0.0) {
-        off[k] = alpha;
-        s_tau = 0.0;
-        s_scale = 0.0;
-      } else {
-        const double norm = sqrt(alpha * alpha + sigma);
-        const double beta = alpha >= 0.0 ? -norm : norm;
-        off[k] = beta;
-        s_tau = (beta - alpha) / beta;
-        s_scale = 1.0 / (alpha - beta);
-      }
-      tau[k] = s_tau;
+      diag[k] = s_diag;
+      off[k] = beta;
+      tau[k] = tk;
       col[k + 1] = 1.0;
     }
-    __syncthreads();
-    const double tk = s_tau, scale = s_scale;
     for (int i = k + 1 + t; i < n; i += kTriThreads) {
       double v = 1.0;
       if (i >= k + 2) {
@@ -176,13 +180,9 @@ __global__ __launch_bounds__(kTriThreads) void tridiag_kernel(Batch batch, doubl
     pv = tri_block_sum(pv, red);
     for (int i = k + 1 + t; i < n; i += kTriThreads) wc[i] = wc[i] - 0.5 * tk * pv * vc[i];
     __syncthreads();
-    for (int i = t; i < n; i += kTriThreads) {
-      vp[i] = i > k ? vc[i] : 0.0;
-      wp[i] = i > k ? wc[i] : 0.0;
-    }
-    __syncthreads();
+    cur ^= 1;
   }
-  if (t == 0) diag[n - 1] = A[int64_t(n - 1) * n + n - 1] - 2.0 * vp[n - 1] * wp[n - 1];
+  if (t == 0) diag[n - 1] = A[int64_t(n - 1) * n + n - 1] - 2.0 * vbuf[cur][n - 1] * wbuf[cur][n - 1];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -201,20 +201,24 @@ struct Secular {
   double f, psi, phi, dpsi, dphi;
 };
 
+// 1/d from the hardware reciprocal estimate and one Newton-Raphson correction r + r (1 - d r).
+__device__ inline double reciprocal(double d) {
+  const double r = __builtin_amdgcn_rcp(d);
+  return fma(r, fma(-d, r, 1.0), r);
+}
+
 __device__ Secular secular(const double *dk, const double *z2, int k, int j, int o, double rho, double x) {
   const double origin = dk[o];
   double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0;
-  for (int i = 0; i <= j; ++i) {
-    const double inv = 1.0 / ((dk[i] - origin) - x);
+  // one loop for both sums, so lanes working on different roots do not diverge
+  for (int i = 0; i < k; ++i) {
+    const double inv = reciprocal((dk[i] - origin) - x);
     const double w = z2[i] * inv;
-    psi += w;
-    dpsi += w * inv;
-  }
-  for (int i = j + 1; i < k; ++i) {
-    const double inv = 1.0 / ((dk[i] - origin) - x);
-    const double w = z2[i] * inv;
-    phi += w;
-    dphi += w * inv;
+    const bool left = i <= j;
+    psi += left ? w : 0.0;
+    dpsi += left ? w * inv : 0.0;
+    phi += left ? 0.0 : w;
+    dphi += left ? 0.0 : w * inv;
   }
   return Secular{1.0 + rho * (psi + phi), rho * psi, rho * phi, rho * dpsi, rho * dphi};
 }
