@@ -152,8 +152,8 @@ int tt_graph_destroy(tt_graph *graph);
 
 /* Batched symmetric eigendecomposition in FP64 (the split of the two-site tensor: the Gram matrix
  * of each block).  Replaces the per-block `at::linalg_svd` of `split_over`
- * (tentorch/linalg/graded/svd.cpp:41-177) with one tridiagonalisation, one divide-and-conquer
- * and one back-transformation launch for all blocks.  Matrix b is the symmetric n x n at
+ * (tentorch/linalg/graded/svd.cpp:41-177) with one tridiagonalisation, two divide-and-conquer
+ * (leaves, merges) and one back-transformation launch for all blocks.  Matrix b is the symmetric n x n at
  * `matrices + matrix_offset` (overwritten: it keeps the Householder reflectors); its eigenvalues go
  * to `values + value_offset` in ascending order, and eigenvector j to row j of the n x n at
  * `vectors + matrix_offset`.  n is 1..1024.  `workspace` holds tt_eigh_workspace(...) doubles. */

@@ -1,12 +1,12 @@
 // The device library, part four: batched symmetric eigendecomposition in FP64 for the split of the
-// two-site tensor (the Gram matrices of its blocks), in three launches for a whole batch of
+// two-site tensor (the Gram matrices of its blocks), in four launches for a whole batch of
 // matrices of different sizes:
 //
 //   1. Householder tridiagonalisation, one workgroup per matrix.  The rank-2 update of step k is
 //      deferred into step k+1's symmetric matrix-vector pass, so each step reads and writes the
 //      trailing matrix once.  The reflectors stay in the matrix's lower part.
-//   2. Cuppen divide and conquer on the tridiagonal, one workgroup per matrix, bottom up from 1 x 1
-//      leaves: deflation of small coupling components and of close poles (Givens rotations), the
+//   2. Cuppen divide and conquer on the tridiagonal: leaves of 32 rows by implicit QL (one wave
+//      each), then merges bottom up, one workgroup per matrix: deflation of small coupling components and of close poles (Givens rotations), the
 //      secular roots by rational ("middle way") steps from the nearer pole (one thread per root), eigenvectors
 //      by the Gu-Eisenstat formula (orthogonal to working precision however close the roots), and
 //      the merge product through LDS tiles.
@@ -30,6 +30,7 @@ constexpr int kEigWaves = kEigThreads / 64;
 constexpr int kEigMaxN = 1024;      // largest matrix (LDS vectors are sized for it)
 constexpr int kEigBatch = 64;       // problems per launch (kernel-argument table)
 constexpr int kPanel = 16;          // columns per back-transformation workgroup
+constexpr int kLeaf = 32;           // order of the divide-and-conquer leaves
 constexpr double kEps = 2.220446049250313e-16;
 
 struct Problem {
@@ -38,6 +39,8 @@ struct Problem {
   int64_t work;         // offset of its workspace (3 n^2 + 3 n doubles)
   int32_t n;
   int32_t first_panel;  // index of its first back-transformation panel in the launch
+  int32_t first_leaf;   // index of its first divide-and-conquer leaf in the launch
+  int32_t pad;
 };
 
 struct Batch {
@@ -186,7 +189,113 @@ __global__ __launch_bounds__(kTriThreads) void tridiag_kernel(Batch batch, doubl
 }
 
 // ---------------------------------------------------------------------------------------------
-// 2. Divide and conquer.  Q and Qt are column-major n x n (Q[r + c n]); U is a merge's k x k.
+// 2a. Leaves of the divide and conquer: every block of kLeaf rows of the tridiagonal (with the
+//     couplings of its two cuts taken off its end diagonals, Cuppen's split) solved by implicit QL
+//     with Wilkinson-type shifts and eigenvector accumulation, one wave per leaf.  Every lane runs
+//     the scalar recurrence on the wave's copy of the leaf in LDS; lane r applies the rotations to
+//     row r of the leaf's eigenvectors.  The eigenpairs are then sorted ascending and written to the
+//     leaf's diagonal block of Q, whose other entries in the leaf's rows are zeroed (and the whole
+//     rows of the second buffer too).
+
+__global__ __launch_bounds__(kEigThreads) void leaf_kernel(Batch batch, double *__restrict__ values,
+                                                            double *__restrict__ work, int32_t leaves) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int leaf = static_cast<int>(blockIdx.x) * kEigWaves + wave;
+  __shared__ double dl[kEigWaves][kLeaf], el[kEigWaves][kLeaf], Z[kEigWaves][kLeaf][kLeaf + 1];
+  if (leaf >= leaves) return;
+  int b = 0;
+  while (b + 1 < batch.count && batch.p[b + 1].first_leaf <= leaf) ++b;
+  const Problem pr = batch.p[b];
+  const int n = pr.n;
+  const int lo = (leaf - pr.first_leaf) * kLeaf, hi = min(lo + kLeaf, n), m = hi - lo;
+  double *Q = work + pr.work, *Qt = Q + int64_t(n) * n;
+  const double *diag = Q + 3 * int64_t(n) * n, *off = diag + n;
+  double *d = dl[wave], *e = el[wave];
+  double (*z)[kLeaf + 1] = Z[wave];
+  if (lane < m) {
+    double v = diag[lo + lane];
+    if (lane == 0 && lo > 0) v -= off[lo - 1];
+    if (lane == m - 1 && hi < n) v -= off[hi - 1];
+    d[lane] = v;
+    e[lane] = lane + 1 < m ? off[lo + lane] : 0.0;
+    for (int c = 0; c < m; ++c) z[lane][c] = lane == c ? 1.0 : 0.0;
+  }
+  __builtin_amdgcn_wave_barrier();
+  for (int l = 0; l < m; ++l) {
+    for (int iter = 0; iter < 60; ++iter) {
+      int mm = l;
+      for (; mm < m - 1; ++mm)
+        if (fabs(e[mm]) <= kEps * (fabs(d[mm]) + fabs(d[mm + 1]))) break;
+      if (mm == l) break;
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = hypot(g, 1.0);
+      g = d[mm] - d[l] + e[l] / (g + copysign(r, g));
+      double sn = 1.0, cs = 1.0, p = 0.0;
+      bool split = false;
+      for (int i = mm - 1; i >= l; --i) {
+        const double f = sn * e[i], bb = cs * e[i];
+        r = hypot(f, g);
+        __builtin_amdgcn_wave_barrier();
+        e[i + 1] = r;
+        if (r == 0.0) {
+          d[i + 1] -= p;
+          e[mm] = 0.0;
+          split = true;
+          break;
+        }
+        sn = f / r;
+        cs = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * sn + 2.0 * cs * bb;
+        p = sn * r;
+        __builtin_amdgcn_wave_barrier();
+        d[i + 1] = g + p;
+        g = cs * r - bb;
+        if (lane < m) {
+          const double zi = z[lane][i], zi1 = z[lane][i + 1];
+          z[lane][i + 1] = sn * zi + cs * zi1;
+          z[lane][i] = cs * zi - sn * zi1;
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (split) continue;
+      d[l] -= p;
+      e[l] = g;
+      e[mm] = 0.0;
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  // ascending order, vectors along
+  for (int i = 0; i < m; ++i) {
+    int best = i;
+    for (int j = i + 1; j < m; ++j)
+      if (d[j] < d[best]) best = j;
+    __builtin_amdgcn_wave_barrier();
+    if (best != i) {
+      const double di = d[i], db = d[best];
+      __builtin_amdgcn_wave_barrier();
+      d[i] = db;
+      d[best] = di;
+      if (lane < m) {
+        const double a = z[lane][i];
+        z[lane][i] = z[lane][best];
+        z[lane][best] = a;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  double *lam = values + pr.values;
+  if (lane < m) lam[lo + lane] = d[lane];
+  // the leaf's rows of both buffers: zero, then the leaf's eigenvectors in its diagonal block
+  for (int c = 0; c < n; ++c)
+    if (lane < m) {
+      Q[lo + lane + int64_t(c) * n] = (c >= lo && c < hi) ? z[lane][c - lo] : 0.0;
+      Qt[lo + lane + int64_t(c) * n] = 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// 2b. Divide and conquer.  Q and Qt are column-major n x n (Q[r + c n]); U is a merge's k x k.
 
 struct Rotation {
   int32_t a, b;  // local columns
@@ -241,21 +350,8 @@ __global__ __launch_bounds__(kEigThreads) void dc_kernel(Batch batch, double *__
   __shared__ int s_k, s_rot;
   __shared__ double s_rho;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-  // leaves: d_i minus the coupling of the cut on either side (Cuppen); Q = I
-  for (int i = t; i < n; i += kEigThreads) {
-    double v = diag[i];
-    if (i > 0) v -= off[i - 1];
-    if (i + 1 < n) v -= off[i];
-    lam[i] = v;
-  }
-  // Both buffers start as I and 0: a merge reads its whole block, whose off-diagonal parts (rows of
-  // one child, columns of the other) no merge below writes and must be zero.
-  for (int64_t x = t; x < int64_t(n) * n; x += kEigThreads) {
-    Q[x] = (x % (n + 1) == 0) ? 1.0 : 0.0;
-    Qt[x] = 0.0;
-  }
-  __syncthreads();
-  for (int width = 1; width < n; width *= 2) {
+  // the leaves (blocks of kLeaf rows) were solved by leaf_kernel, which also cleared both buffers
+  for (int width = kLeaf; width < n; width *= 2) {
     // a block with no partner at this level moves over unchanged
     {
       const int lo = ((n - 1) / (2 * width)) * (2 * width);
@@ -557,15 +653,19 @@ extern "C" int tt_eigh_batched(const tt_eigh_problem *problems, int64_t count, d
   for (int64_t first = 0; first < count; first += kEigBatch) {
     Batch batch{};
     batch.count = static_cast<int32_t>(std::min<int64_t>(kEigBatch, count - first));
-    int32_t panels = 0;
+    int32_t panels = 0, leaves = 0;
     for (int32_t b = 0; b < batch.count; ++b) {
       const tt_eigh_problem &p = problems[first + b];
-      batch.p[b] = Problem{p.matrix_offset, p.value_offset, work, static_cast<int32_t>(p.n), panels};
+      batch.p[b] = Problem{p.matrix_offset, p.value_offset, work, static_cast<int32_t>(p.n), panels, leaves, 0};
       work += 3 * p.n * p.n + 3 * p.n;
       panels += static_cast<int32_t>((p.n + kPanel - 1) / kPanel);
+      leaves += static_cast<int32_t>((p.n + kLeaf - 1) / kLeaf);
     }
     hipStream_t s = as_stream(stream);
     hipLaunchKernelGGL(tridiag_kernel, dim3(batch.count), dim3(kTriThreads), 0, s, batch, matrices, workspace);
+    TT_LAUNCHED();
+    hipLaunchKernelGGL(leaf_kernel, dim3((leaves + kEigWaves - 1) / kEigWaves), dim3(kEigThreads), 0, s, batch, values,
+                       workspace, leaves);
     TT_LAUNCHED();
     hipLaunchKernelGGL(dc_kernel, dim3(batch.count), dim3(kEigThreads), 0, s, batch, values, workspace);
     TT_LAUNCHED();
