@@ -58,12 +58,13 @@ __device__ double wave_sum(double v) {
 // 1. Tridiagonalisation.  A is column-major n x n (symmetric, so either order), in place.
 //    Output: diag[n], off[n-1], tau[n-1]; reflector k is A[k+1.., k] with A[k+1, k] = 1.
 //    1024 threads: in the trailing pass thread t takes rows k+1 + t % 256 (+ 256 r) of every fourth
-//    column (group t / 256), four columns at a time, so each thread keeps sixteen loads in flight;
-//    the four groups' partial products are summed in LDS in group order.
+//    column (group t / 256), eight columns at a time (four with four rows), so each thread keeps 16
+//    loads in flight;
+//    the four groups' partial products are summed in LDS in group order.  kTriRows (rows per thread:
+//    1, 2 or 4) is the smallest that covers the batch's largest matrix, to keep the loads in registers.
 
 constexpr int kTriThreads = 1024;
 constexpr int kTriGroups = kTriThreads / 256;
-constexpr int kTriRows = (kEigMaxN + 255) / 256;
 
 __device__ double tri_block_sum(double v, double *scratch) {
   v = wave_sum(v);
@@ -75,8 +76,10 @@ __device__ double tri_block_sum(double v, double *scratch) {
   return total;
 }
 
+template <int kTriRows>
 __global__ __launch_bounds__(kTriThreads) void tridiag_kernel(Batch batch, double *__restrict__ matrices,
                                                                double *__restrict__ work) {
+  constexpr int kTriUnroll = kTriRows == 4 ? 4 : 8;   // columns per thread per batch of loads
   const Problem pr = batch.p[blockIdx.x];
   const int n = pr.n;
   double *__restrict__ A = matrices + pr.matrix;
@@ -139,10 +142,10 @@ __global__ __launch_bounds__(kTriThreads) void tridiag_kernel(Batch batch, doubl
       vpi[r] = i < n ? vp[i] : 0.0;
       wpi[r] = i < n ? wp[i] : 0.0;
     }
-    for (int j0 = k + 1 + group; j0 < n; j0 += 4 * kTriGroups) {
-      double a[4][kTriRows];
+    for (int j0 = k + 1 + group; j0 < n; j0 += kTriUnroll * kTriGroups) {
+      double a[kTriUnroll][kTriRows];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < kTriUnroll; ++u) {
         const int j = j0 + u * kTriGroups;
 #pragma unroll
         for (int r = 0; r < kTriRows; ++r) {
@@ -151,7 +154,7 @@ __global__ __launch_bounds__(kTriThreads) void tridiag_kernel(Batch batch, doubl
         }
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < kTriUnroll; ++u) {
         const int j = j0 + u * kTriGroups;
         if (j >= n) continue;
         const double vpj = vp[j], wpj = wp[j], vcj = vc[j];
@@ -662,7 +665,14 @@ extern "C" int tt_eigh_batched(const tt_eigh_problem *problems, int64_t count, d
       leaves += static_cast<int32_t>((p.n + kLeaf - 1) / kLeaf);
     }
     hipStream_t s = as_stream(stream);
-    hipLaunchKernelGGL(tridiag_kernel, dim3(batch.count), dim3(kTriThreads), 0, s, batch, matrices, workspace);
+    int64_t largest = 0;
+    for (int32_t b = 0; b < batch.count; ++b) largest = std::max<int64_t>(largest, batch.p[b].n);
+    if (largest <= 257)
+      hipLaunchKernelGGL(tridiag_kernel<1>, dim3(batch.count), dim3(kTriThreads), 0, s, batch, matrices, workspace);
+    else if (largest <= 513)
+      hipLaunchKernelGGL(tridiag_kernel<2>, dim3(batch.count), dim3(kTriThreads), 0, s, batch, matrices, workspace);
+    else
+      hipLaunchKernelGGL(tridiag_kernel<4>, dim3(batch.count), dim3(kTriThreads), 0, s, batch, matrices, workspace);
     TT_LAUNCHED();
     hipLaunchKernelGGL(leaf_kernel, dim3((leaves + kEigWaves - 1) / kEigWaves), dim3(kEigThreads), 0, s, batch, values,
                        workspace, leaves);
