@@ -162,6 +162,8 @@ typedef struct tt_eigh_problem {
   int64_t value_offset;
   int64_t n;
 } tt_eigh_problem;
+/* The largest matrix order tt_eigh_batched takes (1024). */
+int64_t tt_eigh_max_order(void);
 int64_t tt_eigh_workspace(const tt_eigh_problem *problems, int64_t count);
 int tt_eigh_batched(const tt_eigh_problem *problems, int64_t count, double *matrices, double *values,
                     double *vectors, double *workspace, void *stream);

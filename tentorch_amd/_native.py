@@ -42,6 +42,9 @@ class EighProblem(ctypes.Structure):
     _fields_ = [('matrix_offset', ctypes.c_int64), ('value_offset', ctypes.c_int64), ('n', ctypes.c_int64)]
 
 
+_lib.tt_eigh_max_order.argtypes = []
+_lib.tt_eigh_max_order.restype = ctypes.c_int64
+EIGH_MAX_ORDER = int(_lib.tt_eigh_max_order())
 _lib.tt_eigh_workspace.argtypes = [ctypes.POINTER(EighProblem), _i64]
 _lib.tt_eigh_workspace.restype = ctypes.c_int64
 _lib.tt_eigh_batched.argtypes = [ctypes.POINTER(EighProblem), _i64, _c_void_p, _c_void_p, _c_void_p, _c_void_p,
@@ -63,7 +66,7 @@ EXPORTED = ['tt_gss_plan_create', 'tt_gss_plan_destroy', 'tt_gss_plan_counts', '
             'tt_multiaxpy', 'tt_graph_begin',
             'tt_graph_end', 'tt_graph_launch', 'tt_graph_destroy', 'tt_last_error', 'tt_version',
             'tt_launch_count', 'tt_timing_enable', 'tt_timing_collect', 'tt_mlp_param_count', 'tt_mlp_forward',
-            'tt_mlp_gradient', 'tt_mlp_adam', 'tt_eigh_workspace', 'tt_eigh_batched']
+            'tt_mlp_gradient', 'tt_mlp_adam', 'tt_eigh_max_order', 'tt_eigh_workspace', 'tt_eigh_batched']
 
 
 def library() -> ctypes.CDLL:

@@ -631,6 +631,8 @@ __global__ __launch_bounds__(kEigThreads) void back_kernel(Batch batch, const do
 
 }  // namespace
 
+extern "C" int64_t tt_eigh_max_order(void) { return kEigMaxN; }
+
 extern "C" int64_t tt_eigh_workspace(const tt_eigh_problem *problems, int64_t count) {
   if (count < 0 || (count > 0 && !problems)) {
     fail("tt_eigh_workspace: bad problem list");

@@ -46,9 +46,18 @@ class SvdSettings:
 
 
 def _gram_factors(blocks, left_isometric: bool):
-    if SvdSettings.eigh == 'device':
-        return _gram_factors_device(blocks, left_isometric)
-    return _gram_factors_torch(blocks, left_isometric)
+    if SvdSettings.eigh != 'device':
+        return _gram_factors_torch(blocks, left_isometric)
+    # the device eigensolver takes Gram matrices up to its order limit; larger ones go to torch
+    order = [b.shape[0] if left_isometric else b.shape[1] for b in blocks]
+    small = [i for i, g in enumerate(order) if g <= _native.EIGH_MAX_ORDER]
+    large = [i for i, g in enumerate(order) if g > _native.EIGH_MAX_ORDER]
+    out = [None] * len(blocks)
+    for group, solve in ((small, _gram_factors_device), (large, _gram_factors_torch)):
+        if group:
+            for i, factor in zip(group, solve([blocks[i] for i in group], left_isometric)):
+                out[i] = factor
+    return out
 
 
 def _gram_factors_device(blocks, left_isometric: bool):
