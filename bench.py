@@ -74,8 +74,11 @@ def time_breakdown(driver):
 
     import tentorch_amd.linalg as lin
     saved = (alg.lanczos, alg.svd, alg.einsum, alg.contract, driver._advance_left, driver._advance_right,
-             driver.state.move_center, driver.state.normalize, lin._gram_factors)
+             driver.state.move_center, driver.state.normalize, lin._gram_factors, lin._gram_factors_device,
+             lin._gram_factors_torch)
     lin._gram_factors = timed('svd: gram eigh', lin._gram_factors)
+    lin._gram_factors_device = timed('svd: gram eigh on device', lin._gram_factors_device)
+    lin._gram_factors_torch = timed('svd: gram eigh by torch', lin._gram_factors_torch)
     alg.lanczos = timed('lanczos', alg.lanczos)
     alg.svd = timed('svd', alg.svd)
     alg.einsum = timed('einsum (head, tail, energy)', alg.einsum)
@@ -92,7 +95,8 @@ def time_breakdown(driver):
         totals['sweep'] = time.perf_counter() - start
     finally:
         (alg.lanczos, alg.svd, alg.einsum, alg.contract, driver._advance_left, driver._advance_right,
-         driver.state.move_center, driver.state.normalize, lin._gram_factors) = saved
+         driver.state.move_center, driver.state.normalize, lin._gram_factors, lin._gram_factors_device,
+         lin._gram_factors_torch) = saved
     return {k: round(v, 3) for k, v in totals.items()}
 
 

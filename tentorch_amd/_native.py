@@ -154,6 +154,17 @@ def multiaxpy(V: torch.Tensor, k: int, coef: torch.Tensor, scale: float, w: torc
                             stream()), 'multiaxpy')
 
 
+def eigh_workspace(sizes) -> int:
+    '''Doubles of workspace tt_eigh_batched needs for matrices of these orders.'''
+    problems = (EighProblem * max(len(sizes), 1))()
+    for b, n in enumerate(sizes):
+        problems[b] = EighProblem(0, 0, n)
+    size = _lib.tt_eigh_workspace(problems, len(sizes))
+    if size < 0:
+        raise RuntimeError(last_error())
+    return size
+
+
 def eigh_batched(matrices: torch.Tensor, sizes, offsets=None, work: Optional[torch.Tensor] = None):
     '''Eigenvalues (ascending) and eigenvectors of the symmetric FP64 matrices packed in `matrices`
     (matrix b is sizes[b] x sizes[b], at offsets[b], default back to back), on the device.  Returns

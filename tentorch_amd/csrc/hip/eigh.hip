@@ -1,17 +1,21 @@
 // The device library, part four: batched symmetric eigendecomposition in FP64 for the split of the
-// two-site tensor (the Gram matrices of its blocks), in four launches for a whole batch of
-// matrices of different sizes:
+// two-site tensor (the Gram matrices of its blocks), for a whole batch of matrices of different
+// sizes at once:
 //
-//   1. Householder tridiagonalisation, one workgroup per matrix.  The rank-2 update of step k is
-//      deferred into step k+1's symmetric matrix-vector pass, so each step reads and writes the
-//      trailing matrix once.  The reflectors stay in the matrix's lower part.
+//   1. Householder tridiagonalisation.  The rank-2 update of step k is deferred into step k+1's
+//      symmetric matrix-vector pass, so each step reads and writes the trailing matrix once.  The
+//      reflectors stay in the matrix's lower part.  Batches of small matrices run one workgroup per
+//      matrix in a single launch; a batch with a matrix of order kStepMinOrder or more runs one
+//      launch per step, with every matrix spread over one workgroup per 16 trailing columns.
 //   2. Cuppen divide and conquer on the tridiagonal: leaves of 32 rows by implicit QL (one wave
-//      each), then merges bottom up, one workgroup per matrix: deflation of small coupling components and of close poles (Givens rotations), the
-//      secular roots by rational ("middle way") steps from the nearer pole (one thread per root), eigenvectors
-//      by the Gu-Eisenstat formula (orthogonal to working precision however close the roots), and
-//      the merge product through LDS tiles.
-//   3. Back-transformation of the eigenvectors by the reflectors, one workgroup per 16-column
-//      panel held in LDS.
+//      each), then merges bottom up, two launches per level for every matrix of the batch.  One
+//      workgroup per merge deflates small coupling components and close poles (Givens rotations),
+//      finds the secular roots by rational ("middle way") steps from the nearer pole (one thread
+//      per root) and their eigenvectors by the Gu-Eisenstat formula (orthogonal to working
+//      precision however close the roots); then the merge products run as 64 x 64 FP64 MFMA
+//      tiles spread over the GPU.
+//   3. Back-transformation of the eigenvectors by the reflectors in blocks of 32 (compact WY
+//      form), one workgroup per 16-column panel held in registers, on FP64 MFMA.
 //
 // Eigenvalues come out ascending; eigenvector j is row j of the matrix's n x n output.  gfx950 only.
 #include <algorithm>
@@ -36,7 +40,7 @@ constexpr double kEps = 2.220446049250313e-16;
 struct Problem {
   int64_t matrix;       // offset of the n x n matrix (and of its eigenvectors)
   int64_t values;       // offset of its n eigenvalues
-  int64_t work;         // offset of its workspace (3 n^2 + 3 n doubles)
+  int64_t work;         // offset of its workspace (3 n^2 + 7 n + 1024 ceil((n - 1) / 32) doubles)
   int32_t n;
   int32_t first_panel;  // index of its first back-transformation panel in the launch
   int32_t first_leaf;   // index of its first divide-and-conquer leaf in the launch
@@ -47,6 +51,8 @@ struct Batch {
   Problem p[kEigBatch];
   int32_t count;
 };
+
+typedef double d4 __attribute__((ext_vector_type(4)));
 
 __device__ double wave_sum(double v) {
 #pragma unroll
@@ -60,8 +66,8 @@ __device__ double wave_sum(double v) {
 //    1024 threads: in the trailing pass thread t takes rows k+1 + t % 256 (+ 256 r) of every fourth
 //    column (group t / 256), eight columns at a time (four with four rows), so each thread keeps 16
 //    loads in flight;
-//    the four groups' partial products are summed in LDS in group order.  kTriRows (rows per thread:
-//    1, 2 or 4) is the smallest that covers the batch's largest matrix, to keep the loads in registers.
+//    the four groups' partial products are summed in LDS in group order.  kTriRows (rows per thread)
+//    must cover the batch's largest matrix; it is launched with one (orders up to 256).
 
 constexpr int kTriThreads = 1024;
 constexpr int kTriGroups = kTriThreads / 256;
@@ -189,6 +195,122 @@ __global__ __launch_bounds__(kTriThreads) void tridiag_kernel(Batch batch, doubl
     cur ^= 1;
   }
   if (t == 0) diag[n - 1] = A[int64_t(n - 1) * n + n - 1] - 2.0 * vbuf[cur][n - 1] * wbuf[cur][n - 1];
+}
+
+// 1'. The same reduction in steps, for batches with a matrix above kStepMinOrder: one launch per
+//     Householder step k, one workgroup per kStepCols trailing columns of every matrix (one column
+//     per wave), so a large matrix is spread over many CUs instead of one.  Every workgroup of a
+//     matrix redoes the step's O(n) part itself: w of step k-1 from that step's p and v, column k
+//     with step k-1's update, the reflector of step k.  Then each wave applies step k-1's update
+//     to its column (rows k+1..) and forms that column's entry of p = tau A v by a column dot
+//     product (A is symmetric).  Between launches the state lives in the workspace: v and p of the
+//     last two steps (by step parity, so a launch never writes what it reads).  The first
+//     workgroup of a matrix writes the step's scalars, v, and the reflector of step k-1 into column
+//     k-1, which no workgroup reads at step k; a last launch (k = n-1, one workgroup) forms the
+//     last diagonal entry and stores the last reflector.
+
+constexpr int kStepThreads = 1024;
+constexpr int kStepCols = kStepThreads / 64;
+constexpr int kStepChunks = kEigMaxN / 64;   // 64-row chunks of a column per lane
+constexpr int kStepMinOrder = 257;           // smaller batches stay in one workgroup per matrix
+static_assert(kEigMaxN <= kStepThreads, "one row of the step vectors per thread");
+
+// workgroups of a matrix of order n at step k
+__host__ __device__ inline int step_blocks(int n, int k) {
+  return k + 1 < n ? (n - k - 1 + kStepCols - 1) / kStepCols : (k + 1 == n ? 1 : 0);
+}
+
+__global__ __launch_bounds__(kStepThreads) void tridiag_step_kernel(Batch batch, double *__restrict__ matrices,
+                                                                    double *__restrict__ work, int k) {
+  int b = 0, first = 0;
+  for (; b < batch.count; ++b) {
+    const int count = step_blocks(batch.p[b].n, k);
+    if (static_cast<int>(blockIdx.x) < first + count) break;
+    first += count;
+  }
+  if (b == batch.count) return;
+  const Problem pr = batch.p[b];
+  const int n = pr.n, block = static_cast<int>(blockIdx.x) - first;
+  double *__restrict__ A = matrices + pr.matrix;
+  double *diag = work + pr.work + 3 * int64_t(n) * n, *off = diag + n, *tau = off + n;
+  double *vbuf = tau + n, *pbuf = vbuf + 2 * n;   // [2][n] each, by step parity
+  __shared__ double vp[kEigMaxN], wp[kEigMaxN], vc[kEigMaxN];
+  __shared__ double red[kStepThreads / 64];
+  __shared__ double s_alpha, s_diag;
+  const int t = threadIdx.x, i = k + t;
+  const bool live = i < n;
+  // v and p of step k-1 (zero before the first step), and column k as it stands
+  double v = 0.0, p = 0.0, a = 0.0;
+  if (live) {
+    a = A[i + int64_t(k) * n];
+    if (k > 0) {
+      v = vbuf[((k - 1) & 1) * n + i];
+      p = pbuf[((k - 1) & 1) * n + i];
+    }
+  }
+  const double tprev = k > 0 ? tau[k - 1] : 0.0;
+  const double pv = tri_block_sum(p * v, red);
+  const double w = p - 0.5 * tprev * pv * v;
+  if (live) {
+    vp[i] = v;
+    wp[i] = w;
+  }
+  __syncthreads();
+  const double x = live ? a - (v * wp[k] + w * vp[k]) : 0.0;
+  if (i == k) s_diag = x;
+  if (i == k + 1) s_alpha = x;
+  const double sigma = tri_block_sum(live && i >= k + 2 ? x * x : 0.0, red);
+  if (k + 1 == n) {
+    if (t == 0) diag[k] = s_diag;
+    if (k > 0 && live) A[i + int64_t(k - 1) * n] = v;
+    return;
+  }
+  const double alpha = s_alpha;
+  double beta = alpha, tk = 0.0, scale = 0.0;
+  if (sigma != 0.0) {
+    const double norm = sqrt(alpha * alpha + sigma);
+    beta = alpha >= 0.0 ? -norm : norm;
+    tk = (beta - alpha) / beta;
+    scale = 1.0 / (alpha - beta);
+  }
+  const double vnew = i == k + 1 ? 1.0 : (live && i >= k + 2 ? x * scale : 0.0);
+  if (live) vc[i] = vnew;
+  if (block == 0) {
+    if (t == 0) {
+      diag[k] = s_diag;
+      off[k] = beta;
+      tau[k] = tk;
+    }
+    if (live) {
+      vbuf[(k & 1) * n + i] = vnew;
+      if (k > 0) A[i + int64_t(k - 1) * n] = v;
+    }
+  }
+  __syncthreads();
+  // this wave's column: step k-1's update on rows k+1.., and its entry of p
+  const int wave = t >> 6, lane = t & 63;
+  const int c = k + 1 + block * kStepCols + wave;
+  if (c >= n) return;
+  double *__restrict__ col = A + int64_t(c) * n;
+  const double vpc = vp[c], wpc = wp[c];
+  double y[kStepChunks];
+#pragma unroll
+  for (int q = 0; q < kStepChunks; ++q) {
+    const int r = k + 1 + lane + 64 * q;
+    y[q] = r < n ? col[r] : 0.0;
+  }
+  double acc = 0.0;
+#pragma unroll
+  for (int q = 0; q < kStepChunks; ++q) {
+    const int r = k + 1 + lane + 64 * q;
+    if (r < n) {
+      const double z = y[q] - (vp[r] * wpc + wp[r] * vpc);
+      col[r] = z;
+      acc += z * vc[r];
+    }
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) pbuf[(k & 1) * n + c] = tk * acc;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -335,259 +457,436 @@ __device__ Secular secular(const double *dk, const double *z2, int k, int j, int
   return Secular{1.0 + rho * (psi + phi), rho * psi, rho * phi, rho * dpsi, rho * dphi};
 }
 
-__global__ __launch_bounds__(kEigThreads) void dc_kernel(Batch batch, double *__restrict__ values,
-                                                          double *__restrict__ work) {
-  const Problem pr = batch.p[blockIdx.x];
+// A level of the divide and conquer is a list of units, blocks of up to 2 * width rows: the merge of
+// two solved halves, or (the last block of a matrix, when it has no partner at this level) a copy.
+// Level l reads the eigenvectors from buffer l % 2 (Q or Qt) and writes the other.  A unit's
+// preparation (one workgroup) leaves in the workspace its k x k matrix U (the eigenvectors of the
+// rank-one update in the basis of the kept poles, at 2 n^2 + lo * min(2 width, n)) and, per output
+// column j of the unit, the source column colsrc[lo + j] and the final column dst[lo + j]: columns
+// j < k are Q[:, colsrc] U[:, j], the others copies of Q[:, colsrc[j]]; unitk[lo] = k.
+
+__host__ __device__ inline int dc_units(int n, int width) { return n > width ? (n + 2 * width - 1) / (2 * width) : 0; }
+
+// merge levels a matrix of order n goes through (the widths kLeaf, 2 kLeaf, ... below n)
+__device__ inline int dc_levels(int n) {
+  int levels = 0;
+  for (int width = kLeaf; width < n; width *= 2) ++levels;
+  return levels;
+}
+
+struct DcMaps {
+  int32_t *colsrc, *dst, *unitk;
+};
+
+// integer maps of a level, in the step vectors' space (free once the matrix is tridiagonal)
+__device__ inline DcMaps dc_maps(double *base, int n) {
+  int32_t *ints = reinterpret_cast<int32_t *>(base + 3 * int64_t(n) * n + 3 * n);
+  return DcMaps{ints, ints + n, ints + 2 * n};
+}
+
+__global__ __launch_bounds__(kEigThreads) void dc_prep_kernel(Batch batch, double *__restrict__ values,
+                                                               double *__restrict__ work, int width, int parity) {
+  int b = 0, first = 0;
+  for (; b < batch.count; ++b) {
+    const int units = dc_units(batch.p[b].n, width);
+    if (static_cast<int>(blockIdx.x) < first + units) break;
+    first += units;
+  }
+  if (b == batch.count) return;
+  const Problem pr = batch.p[b];
   const int n = pr.n;
   double *const base = work + pr.work;
-  double *__restrict__ U = base + 2 * int64_t(n) * n;
+  double *__restrict__ Q = base + (parity ? int64_t(n) * n : 0);
   const double *diag = base + 3 * int64_t(n) * n, *off = diag + n;
-  // each level reads its blocks from Q and writes the merged blocks to Qt; the two then swap
-  double *Q = base, *Qt = base + int64_t(n) * n;
+  const DcMaps maps = dc_maps(base, n);
   double *lam = values + pr.values;
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  const int lo = (static_cast<int>(blockIdx.x) - first) * 2 * width;
+  const int mid = lo + width, hi = min(lo + 2 * width, n), s = hi - lo, left = mid - lo;
+  if (mid >= n) {   // no partner at this level: the block moves over unchanged
+    for (int j = t; j < s; j += kEigThreads) {
+      maps.colsrc[lo + j] = j;
+      maps.dst[lo + j] = j;
+    }
+    if (t == 0) maps.unitk[lo] = 0;
+    return;
+  }
+  double *__restrict__ U = base + 2 * int64_t(n) * n + int64_t(lo) * min(2 * width, n);
   __shared__ double dloc[kEigMaxN], zloc[kEigMaxN], pole[kEigMaxN], zsort[kEigMaxN];
   __shared__ double dk[kEigMaxN], z2[kEigMaxN], zk[kEigMaxN], tau[kEigMaxN], zh[kEigMaxN], val[kEigMaxN];
   __shared__ int32_t perm[kEigMaxN], defl[kEigMaxN], keep[kEigMaxN], org[kEigMaxN], src[kEigMaxN], rnk[kEigMaxN];
   __shared__ Rotation rot[kEigMaxN];  // each rotation deflates one pole: at most s - 1
-  __shared__ double tile_a[32][33], tile_b[32][33];
   __shared__ int s_k, s_rot;
   __shared__ double s_rho;
-  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-  // the leaves (blocks of kLeaf rows) were solved by leaf_kernel, which also cleared both buffers
-  for (int width = kLeaf; width < n; width *= 2) {
-    // a block with no partner at this level moves over unchanged
-    {
-      const int lo = ((n - 1) / (2 * width)) * (2 * width);
-      if (lo + width >= n) {
-        const int s = n - lo;
-        for (int64_t e = t; e < int64_t(s) * s; e += kEigThreads)
-          Qt[lo + e % s + int64_t(lo + e / s) * n] = Q[lo + e % s + int64_t(lo + e / s) * n];
-      }
-    }
-    for (int lo = 0; lo + width < n; lo += 2 * width) {
-      const int mid = lo + width, hi = min(lo + 2 * width, n), s = hi - lo, left = mid - lo;
-      const double beta = off[mid - 1];
-      const double sign = beta < 0.0 ? -1.0 : 1.0;
-      // (a) coupling vector (last row of the left block, first row of the right) and poles
-      for (int i = t; i < s; i += kEigThreads) {
-        const double q = i < left ? Q[(mid - 1) + int64_t(lo + i) * n] : Q[mid + int64_t(lo + i) * n];
-        zloc[i] = q * 0.7071067811865476;
-        dloc[i] = sign * lam[lo + i];
-      }
-      __syncthreads();
-      // (b) one thread: merge the sorted halves, deflate small z and close poles
-      if (t == 0) {
-        const double rho = 2.0 * beta * sign;
-        int a = 0, b = 0;
-        for (int p = 0; p < s; ++p) {
-          // ascending in dloc: a negated half is descending, so it is walked from its end
-          const int ia = sign > 0 ? a : left - 1 - a, ib = sign > 0 ? left + b : s - 1 - b;
-          const bool take_a = b >= s - left || (a < left && dloc[ia] <= dloc[ib]);
-          perm[p] = take_a ? ia : ib;
-          if (take_a) ++a; else ++b;
-        }
-        double dmax = 0.0;
-        for (int p = 0; p < s; ++p) {
-          pole[p] = dloc[perm[p]];
-          zsort[p] = zloc[perm[p]];
-          dmax = fmax(dmax, fabs(pole[p]));
-        }
-        const double tol = 8.0 * kEps * fmax(dmax, rho);
-        int nrot = 0, prev = -1;
-        for (int p = 0; p < s; ++p) {
-          defl[p] = rho * fabs(zsort[p]) <= tol ? 1 : 0;
-          if (defl[p]) continue;
-          if (prev >= 0) {
-            const double r = hypot(zsort[p], zsort[prev]);
-            const double c = zsort[p] / r, sn = -zsort[prev] / r;
-            if (fabs((pole[p] - pole[prev]) * c * sn) <= tol) {
-              zsort[p] = r;
-              zsort[prev] = 0.0;
-              rot[nrot++] = Rotation{perm[prev], perm[p], c, sn};
-              const double dp = pole[prev] * c * c + pole[p] * sn * sn;
-              pole[p] = pole[prev] * sn * sn + pole[p] * c * c;
-              pole[prev] = dp;
-              defl[prev] = 1;
-            }
-          }
-          prev = p;
-        }
-        int k = 0;
-        for (int p = 0; p < s; ++p)
-          if (!defl[p]) keep[k++] = p;
-        s_k = k;
-        s_rot = nrot;
-        s_rho = rho;
-      }
-      __syncthreads();
-      const int k = s_k, nrot = s_rot;
-      const double rho = s_rho;
-      // (c) the deflating rotations on Q's columns, in order; each thread owns its rows
-      for (int r = lo + t; r < hi; r += kEigThreads)
-        for (int q = 0; q < nrot; ++q) {
-          double *ca = Q + r + int64_t(lo + rot[q].a) * n, *cb = Q + r + int64_t(lo + rot[q].b) * n;
-          const double x = *ca, y = *cb;
-          *ca = rot[q].c * x + rot[q].s * y;
-          *cb = -rot[q].s * x + rot[q].c * y;
-        }
-      for (int j = t; j < k; j += kEigThreads) {
-        dk[j] = pole[keep[j]];
-        zk[j] = zsort[keep[j]];
-        z2[j] = zk[j] * zk[j];
-      }
-      __syncthreads();
-      // (d) secular roots, one thread per root, from the nearer pole of its interval.  Each step
-      //     models the sums left and right of the interval by one pole each, matching value and
-      //     slope at the iterate, and takes the root of that model (the rational "middle way"
-      //     step); steps that leave the bracket bisect it.
-      for (int j = t; j < k; j += kEigThreads) {
-        int o = j;
-        double a = 0.0, b;
-        if (j + 1 < k) {
-          const double gap = dk[j + 1] - dk[j];
-          if (secular(dk, z2, k, j, j, rho, 0.5 * gap).f > 0.0) {
-            b = 0.5 * gap;
-          } else {
-            o = j + 1;
-            a = -0.5 * gap;
-            b = 0.0;
-          }
-        } else {
-          double zs = 0.0;
-          for (int i = 0; i < k; ++i) zs += z2[i];
-          b = rho * zs;
-        }
-        const double lo_pole = dk[j] - dk[o];
-        const double hi_pole = j + 1 < k ? dk[j + 1] - dk[o] : 0.0;
-        double x = 0.5 * (a + b);
-        for (int it = 0; it < 80; ++it) {
-          const Secular v = secular(dk, z2, k, j, o, rho, x);
-          if (v.f > 0.0) b = x; else a = x;
-          // converged when f is within its own rounding error, or the bracket is at the
-          // resolution of x
-          if (fabs(v.f) <= 8.0 * kEps * k * (1.0 + fabs(v.psi) + fabs(v.phi)) ||
-              b - a <= 4.0 * kEps * fmax(fabs(a), fabs(b)))
-            break;
-          const double dlo = lo_pole - x;
-          double eta;
-          if (j + 1 < k) {
-            const double dhi = hi_pole - x;
-            const double S = dlo * dlo * v.dpsi, T = dhi * dhi * v.dphi;
-            const double c = v.f - dlo * v.dpsi - dhi * v.dphi;
-            const double B = c * (dlo + dhi) + S + T, C = c * dlo * dhi + S * dhi + T * dlo;
-            const double root = sqrt(fmax(B * B - 4.0 * c * C, 0.0));
-            if (B >= 0.0)
-              eta = 2.0 * C / (B + root);
-            else if (c != 0.0)
-              eta = (B - root) / (2.0 * c);
-            else
-              eta = C / B;
-          } else {
-            const double c = v.f - dlo * v.dpsi;
-            eta = c != 0.0 ? dlo + dlo * dlo * v.dpsi / c : 0.5 * (b - a);
-          }
-          double next = x + eta;
-          if (!(next > a && next < b)) next = 0.5 * (a + b);
-          if (fabs(next - x) <= 2.0 * kEps * fabs(x)) {
-            x = next;
-            break;
-          }
-          x = next;
-        }
-        org[j] = o;
-        tau[j] = x;
-      }
-      __syncthreads();
-      // (e) Gu-Eisenstat: zh_i^2 = (lam_i - d_i)/rho prod_{j != i} (lam_j - d_i)/(d_j - d_i)
-      for (int i = t; i < k; i += kEigThreads) {
-        double w = ((dk[org[i]] - dk[i]) + tau[i]) / rho;
-        for (int j = 0; j < k; ++j)
-          if (j != i) w *= ((dk[org[j]] - dk[i]) + tau[j]) / (dk[j] - dk[i]);
-        zh[i] = copysign(sqrt(fabs(w)), zk[i]);
-      }
-      __syncthreads();
-      // (f) U[:, j] = zh / (d - lam_j), normalised (k x k, column-major): one wave per column
-      for (int j = wave; j < k; j += kEigWaves) {
-        const double dj = dk[org[j]], tj = tau[j];
-        double norm = 0.0;
-        for (int i = lane; i < k; i += 64) {
-          const double u = zh[i] / ((dk[i] - dj) - tj);
-          norm += u * u;
-        }
-        const double inv = 1.0 / sqrt(wave_sum(norm));
-        for (int i = lane; i < k; i += 64) U[i + int64_t(j) * k] = inv * zh[i] / ((dk[i] - dj) - tj);
-      }
-      // (g) slots: 0..k-1 the roots, then the deflated poles; their values and sources
-      if (t == 0) {
-        int slot = k;
-        for (int p = 0; p < s; ++p)
-          if (defl[p]) {
-            val[slot] = pole[p];
-            src[slot] = perm[p];
-            ++slot;
-          }
-      }
-      for (int j = t; j < k; j += kEigThreads) {
-        val[j] = dk[org[j]] + tau[j];
-        src[j] = -1;
-      }
-      __syncthreads();
-      // (h) ascending rank of every slot (ties by slot), and its final column
-      for (int x = t; x < s; x += kEigThreads) {
-        int r = 0;
-        for (int y = 0; y < s; ++y) r += (val[y] < val[x] || (val[y] == val[x] && y < x)) ? 1 : 0;
-        rnk[x] = sign > 0 ? r : s - 1 - r;
-        lam[lo + rnk[x]] = sign * val[x];
-      }
-      __syncthreads();
-      // (i) deflated columns copied, root columns = Q[:, kept] U, into Qt
-      for (int64_t e = t; e < int64_t(s - k) * s; e += kEigThreads) {
-        const int x = k + static_cast<int>(e / s), r = static_cast<int>(e % s);
-        Qt[lo + r + int64_t(lo + rnk[x]) * n] = Q[lo + r + int64_t(lo + src[x]) * n];
-      }
-      const int ty = t >> 3, tx = (t & 7) * 4;
-      for (int r0 = 0; r0 < s; r0 += 32)
-        for (int j0 = 0; j0 < k; j0 += 32) {
-          double acc[4] = {0.0, 0.0, 0.0, 0.0};
-          for (int i0 = 0; i0 < k; i0 += 32) {
-            __syncthreads();
-            for (int e = t; e < 32 * 32; e += kEigThreads) {
-              const int fast = e & 31, slow = e >> 5;   // consecutive threads walk down a column
-              const int row = r0 + fast, kk = i0 + slow;
-              tile_a[fast][slow] = (row < s && kk < k) ? Q[lo + row + int64_t(lo + perm[keep[kk]]) * n] : 0.0;
-              const int kb = i0 + fast, jb = j0 + slow;
-              tile_b[fast][slow] = (kb < k && jb < k) ? U[kb + int64_t(jb) * k] : 0.0;
-            }
-            __syncthreads();
-#pragma unroll 8
-            for (int ii = 0; ii < 32; ++ii) {
-              const double a = tile_a[ty][ii];
-#pragma unroll
-              for (int c = 0; c < 4; ++c) acc[c] += a * tile_b[ii][tx + c];
-            }
-          }
-          const int row = r0 + ty;
-#pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            const int j = j0 + tx + c;
-            if (row < s && j < k) Qt[lo + row + int64_t(lo + rnk[j]) * n] = acc[c];
-          }
-        }
-      __syncthreads();
-    }
-    double *swap = Q;
-    Q = Qt;
-    Qt = swap;
-    __syncthreads();
+  const double beta = off[mid - 1];
+  const double sign = beta < 0.0 ? -1.0 : 1.0;
+  // (a) coupling vector (last row of the left block, first row of the right) and poles
+  for (int i = t; i < s; i += kEigThreads) {
+    const double q = i < left ? Q[(mid - 1) + int64_t(lo + i) * n] : Q[mid + int64_t(lo + i) * n];
+    zloc[i] = q * 0.7071067811865476;
+    dloc[i] = sign * lam[lo + i];
   }
-  // the result must end in the first buffer, where the back-transformation reads it
-  if (Q != base)
-    for (int64_t e = t; e < int64_t(n) * n; e += kEigThreads) base[e] = Q[e];
+  __syncthreads();
+  // (b) one thread: merge the sorted halves, deflate small z and close poles
+  if (t == 0) {
+    const double rho = 2.0 * beta * sign;
+    int a = 0, bb = 0;
+    for (int p = 0; p < s; ++p) {
+      // ascending in dloc: a negated half is descending, so it is walked from its end
+      const int ia = sign > 0 ? a : left - 1 - a, ib = sign > 0 ? left + bb : s - 1 - bb;
+      const bool take_a = bb >= s - left || (a < left && dloc[ia] <= dloc[ib]);
+      perm[p] = take_a ? ia : ib;
+      if (take_a) ++a; else ++bb;
+    }
+    double dmax = 0.0;
+    for (int p = 0; p < s; ++p) {
+      pole[p] = dloc[perm[p]];
+      zsort[p] = zloc[perm[p]];
+      dmax = fmax(dmax, fabs(pole[p]));
+    }
+    const double tol = 8.0 * kEps * fmax(dmax, rho);
+    int nrot = 0, prev = -1;
+    for (int p = 0; p < s; ++p) {
+      defl[p] = rho * fabs(zsort[p]) <= tol ? 1 : 0;
+      if (defl[p]) continue;
+      if (prev >= 0) {
+        const double r = hypot(zsort[p], zsort[prev]);
+        const double c = zsort[p] / r, sn = -zsort[prev] / r;
+        if (fabs((pole[p] - pole[prev]) * c * sn) <= tol) {
+          zsort[p] = r;
+          zsort[prev] = 0.0;
+          rot[nrot++] = Rotation{perm[prev], perm[p], c, sn};
+          const double dp = pole[prev] * c * c + pole[p] * sn * sn;
+          pole[p] = pole[prev] * sn * sn + pole[p] * c * c;
+          pole[prev] = dp;
+          defl[prev] = 1;
+        }
+      }
+      prev = p;
+    }
+    int k = 0;
+    for (int p = 0; p < s; ++p)
+      if (!defl[p]) keep[k++] = p;
+    s_k = k;
+    s_rot = nrot;
+    s_rho = rho;
+  }
+  __syncthreads();
+  const int k = s_k, nrot = s_rot;
+  const double rho = s_rho;
+  // (c) the deflating rotations on Q's columns, in order; each thread owns its rows
+  for (int r = lo + t; r < hi; r += kEigThreads)
+    for (int q = 0; q < nrot; ++q) {
+      double *ca = Q + r + int64_t(lo + rot[q].a) * n, *cb = Q + r + int64_t(lo + rot[q].b) * n;
+      const double x = *ca, y = *cb;
+      *ca = rot[q].c * x + rot[q].s * y;
+      *cb = -rot[q].s * x + rot[q].c * y;
+    }
+  for (int j = t; j < k; j += kEigThreads) {
+    dk[j] = pole[keep[j]];
+    zk[j] = zsort[keep[j]];
+    z2[j] = zk[j] * zk[j];
+  }
+  __syncthreads();
+  // (d) secular roots, one thread per root, from the nearer pole of its interval.  Each step
+  //     models the sums left and right of the interval by one pole each, matching value and
+  //     slope at the iterate, and takes the root of that model (the rational "middle way"
+  //     step); steps that leave the bracket bisect it.
+  for (int j = t; j < k; j += kEigThreads) {
+    int o = j;
+    double a = 0.0, bnd;
+    if (j + 1 < k) {
+      const double gap = dk[j + 1] - dk[j];
+      if (secular(dk, z2, k, j, j, rho, 0.5 * gap).f > 0.0) {
+        bnd = 0.5 * gap;
+      } else {
+        o = j + 1;
+        a = -0.5 * gap;
+        bnd = 0.0;
+      }
+    } else {
+      double zs = 0.0;
+      for (int i = 0; i < k; ++i) zs += z2[i];
+      bnd = rho * zs;
+    }
+    const double lo_pole = dk[j] - dk[o];
+    const double hi_pole = j + 1 < k ? dk[j + 1] - dk[o] : 0.0;
+    double x = 0.5 * (a + bnd);
+    for (int it = 0; it < 80; ++it) {
+      const Secular v = secular(dk, z2, k, j, o, rho, x);
+      if (v.f > 0.0) bnd = x; else a = x;
+      // converged when f is within its own rounding error, or the bracket is at the
+      // resolution of x
+      if (fabs(v.f) <= 8.0 * kEps * k * (1.0 + fabs(v.psi) + fabs(v.phi)) ||
+          bnd - a <= 4.0 * kEps * fmax(fabs(a), fabs(bnd)))
+        break;
+      const double dlo = lo_pole - x;
+      double eta;
+      if (j + 1 < k) {
+        const double dhi = hi_pole - x;
+        const double S = dlo * dlo * v.dpsi, T = dhi * dhi * v.dphi;
+        const double c = v.f - dlo * v.dpsi - dhi * v.dphi;
+        const double B = c * (dlo + dhi) + S + T, C = c * dlo * dhi + S * dhi + T * dlo;
+        const double root = sqrt(fmax(B * B - 4.0 * c * C, 0.0));
+        if (B >= 0.0)
+          eta = 2.0 * C / (B + root);
+        else if (c != 0.0)
+          eta = (B - root) / (2.0 * c);
+        else
+          eta = C / B;
+      } else {
+        const double c = v.f - dlo * v.dpsi;
+        eta = c != 0.0 ? dlo + dlo * dlo * v.dpsi / c : 0.5 * (bnd - a);
+      }
+      double next = x + eta;
+      if (!(next > a && next < bnd)) next = 0.5 * (a + bnd);
+      if (fabs(next - x) <= 2.0 * kEps * fabs(x)) {
+        x = next;
+        break;
+      }
+      x = next;
+    }
+    org[j] = o;
+    tau[j] = x;
+  }
+  __syncthreads();
+  // (e) Gu-Eisenstat: zh_i^2 = (lam_i - d_i)/rho prod_{j != i} (lam_j - d_i)/(d_j - d_i)
+  for (int i = t; i < k; i += kEigThreads) {
+    double w = ((dk[org[i]] - dk[i]) + tau[i]) / rho;
+    for (int j = 0; j < k; ++j)
+      if (j != i) w *= ((dk[org[j]] - dk[i]) + tau[j]) / (dk[j] - dk[i]);
+    zh[i] = copysign(sqrt(fabs(w)), zk[i]);
+  }
+  __syncthreads();
+  // (f) U[:, j] = zh / (d - lam_j), normalised (k x k, column-major): one wave per column
+  for (int j = wave; j < k; j += kEigWaves) {
+    const double dj = dk[org[j]], tj = tau[j];
+    double norm = 0.0;
+    for (int i = lane; i < k; i += 64) {
+      const double u = zh[i] / ((dk[i] - dj) - tj);
+      norm += u * u;
+    }
+    const double inv = 1.0 / sqrt(wave_sum(norm));
+    for (int i = lane; i < k; i += 64) U[i + int64_t(j) * k] = inv * zh[i] / ((dk[i] - dj) - tj);
+  }
+  // (g) slots: 0..k-1 the roots, then the deflated poles; their values and sources
+  if (t == 0) {
+    int slot = k;
+    for (int p = 0; p < s; ++p)
+      if (defl[p]) {
+        val[slot] = pole[p];
+        src[slot] = perm[p];
+        ++slot;
+      }
+  }
+  for (int j = t; j < k; j += kEigThreads) {
+    val[j] = dk[org[j]] + tau[j];
+    src[j] = perm[keep[j]];
+  }
+  __syncthreads();
+  // (h) ascending rank of every slot (ties by slot): its final column and eigenvalue
+  for (int x = t; x < s; x += kEigThreads) {
+    int r = 0;
+    for (int y = 0; y < s; ++y) r += (val[y] < val[x] || (val[y] == val[x] && y < x)) ? 1 : 0;
+    r = sign > 0 ? r : s - 1 - r;
+    lam[lo + r] = sign * val[x];
+    maps.colsrc[lo + x] = src[x];
+    maps.dst[lo + x] = r;
+  }
+  if (t == 0) maps.unitk[lo] = k;
+}
+
+// (i) the unit's output columns: one workgroup per 64 x 64 tile of (rows, output columns).  Copies
+//     move over directly; root columns are the product Q[:, colsrc[0..k)] U on FP64 MFMA
+//     (16x16x4: four waves, each a 32 x 32 quarter of the tile), with the next 16 columns of the
+//     sum loaded into registers while the current ones are multiplied from LDS.
+
+constexpr int kMergeTile = 64;
+constexpr int kMergeK = 16;
+
+__host__ __device__ inline int merge_tiles(int s) {
+  const int per = (s + kMergeTile - 1) / kMergeTile;
+  return per * per;
+}
+
+// workgroups of a matrix of order n at a level: full units, then the last
+__host__ __device__ inline int dc_tiles(int n, int width) {
+  const int units = dc_units(n, width);
+  if (units == 0) return 0;
+  const int last = n - (units - 1) * 2 * width;
+  return (units - 1) * merge_tiles(2 * width) + merge_tiles(last);
+}
+
+__global__ __launch_bounds__(kEigThreads) void dc_gemm_kernel(Batch batch, double *__restrict__ work, int width,
+                                                               int parity) {
+  int b = 0, first = 0;
+  for (; b < batch.count; ++b) {
+    const int tiles = dc_tiles(batch.p[b].n, width);
+    if (static_cast<int>(blockIdx.x) < first + tiles) break;
+    first += tiles;
+  }
+  if (b == batch.count) return;
+  const Problem pr = batch.p[b];
+  const int n = pr.n;
+  double *const base = work + pr.work;
+  const double *__restrict__ Q = base + (parity ? int64_t(n) * n : 0);
+  double *__restrict__ Qt = base + (parity ? 0 : int64_t(n) * n);
+  const DcMaps maps = dc_maps(base, n);
+  const int units = dc_units(n, width), full = merge_tiles(2 * width);
+  int local = static_cast<int>(blockIdx.x) - first, unit = local / full;
+  if (unit >= units - 1) {
+    unit = units - 1;
+    local -= (units - 1) * full;
+  } else {
+    local -= unit * full;
+  }
+  const int lo = unit * 2 * width, s = min(2 * width, n - lo);
+  const int per = (s + kMergeTile - 1) / kMergeTile;
+  const int r0 = (local / per) * kMergeTile, j0 = (local % per) * kMergeTile;
+  const int k = maps.unitk[lo];
+  const double *__restrict__ U = base + 2 * int64_t(n) * n + int64_t(lo) * min(2 * width, n);
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  __shared__ int32_t src[kEigMaxN];
+  __shared__ double As[2][kMergeK][kMergeTile + 2], Bs[2][kMergeK][kMergeTile + 2];
+  __shared__ double Cs[kMergeTile][kMergeTile + 1];
+  for (int j = t; j < k; j += kEigThreads) src[j] = maps.colsrc[lo + j];
+  // copies among this tile's output columns
+  for (int e = t; e < kMergeTile * kMergeTile; e += kEigThreads) {
+    const int rr = e & (kMergeTile - 1), j = j0 + e / kMergeTile, r = r0 + rr;
+    if (j >= k && j < s && r < s)
+      Qt[lo + r + int64_t(lo + maps.dst[lo + j]) * n] = Q[lo + r + int64_t(lo + maps.colsrc[lo + j]) * n];
+  }
+  if (j0 >= k) return;
+  __syncthreads();
+  // register staging: A element (row rr, sum index kq + 4 q), B element (sum index t % 16, column jq + 16 q)
+  const int rr = t & 63, kq = t >> 6, kb = t & 15, jq = t >> 4;
+  double ra[4], rb[4];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int kk = k0 + kq + 4 * q;
+      ra[q] = (r0 + rr < s && kk < k) ? Q[lo + r0 + rr + int64_t(lo + src[kk]) * n] : 0.0;
+      const int kk2 = k0 + kb, j = j0 + jq + 16 * q;
+      rb[q] = (kk2 < k && j < k) ? U[kk2 + int64_t(j) * k] : 0.0;
+    }
+  };
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32, lr = lane & 15, lk = lane >> 4;
+  d4 acc[2][2];
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = d4{0.0, 0.0, 0.0, 0.0};
+  fetch(0);
+  int buf = 0;
+  for (int k0 = 0; k0 < k; k0 += kMergeK) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      As[buf][kq + 4 * q][rr] = ra[q];
+      Bs[buf][kb][jq + 16 * q] = rb[q];
+    }
+    __syncthreads();
+    if (k0 + kMergeK < k) fetch(k0 + kMergeK);
+#pragma unroll
+    for (int k4 = 0; k4 < kMergeK; k4 += 4) {
+      double a[2], bv[2];
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi) a[bi] = As[buf][k4 + lk][wr + 16 * bi + lr];
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) bv[bj] = Bs[buf][k4 + lk][wc + 16 * bj + lr];
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+          acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], bv[bj], acc[bi][bj], 0, 0, 0);
+    }
+    buf ^= 1;
+  }
+  // through LDS (column-major) so each output column is written as 64 consecutive rows
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Cs[wc + 16 * bj + lr][wr + 16 * bi + 4 * r + lk] = acc[bi][bj][r];
+  __syncthreads();
+  for (int e = t; e < kMergeTile * kMergeTile; e += kEigThreads) {
+    const int row = e & (kMergeTile - 1), jj = e / kMergeTile, j = j0 + jj, r = r0 + row;
+    if (j < k && r < s) Qt[lo + r + int64_t(lo + maps.dst[lo + j]) * n] = Cs[jj][row];
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
-// 3. Back-transformation: Y = H_0 H_1 ... H_{n-2} Z on a panel of 16 columns of Z held in LDS.
-//    Eigenvector c is written as row c of the output.
+// 3. Back-transformation Y = H_0 H_1 ... H_{n-2} Z, with the reflectors in blocks of kRefl:
+//    H_j ... H_{j+m-1} = I - V T V^T (compact WY form, T upper triangular).  One workgroup per
+//    block forms T from the block's Gram matrix V^T V; then one workgroup per 16-column panel of Z
+//    keeps the panel in registers and applies the blocks last to first, each as W = V^T Y,
+//    W = T W, Y -= V W on FP64 MFMA.  Wave w holds the 16-row blocks w, w + 4, ... of the panel in
+//    the MFMA result layout (lane l: rows 4 r + l / 16 of a block, column l % 16), which is also
+//    the layout of the 4 x 16 right-hand operand of W = V^T Y.  Eigenvector c is written as row c
+//    of the output.
+
+constexpr int kRefl = 32;
+constexpr int kRowBlocks = kEigMaxN / 64;   // 16-row blocks per wave
+
+__host__ __device__ inline int reflector_blocks(int n) { return n > 1 ? (n - 1 + kRefl - 1) / kRefl : 0; }
+
+__global__ __launch_bounds__(kEigThreads) void block_t_kernel(Batch batch, const double *__restrict__ matrices,
+                                                               double *__restrict__ work) {
+  int b = 0, first = 0;
+  for (; b < batch.count; ++b) {
+    const int count = reflector_blocks(batch.p[b].n);
+    if (static_cast<int>(blockIdx.x) < first + count) break;
+    first += count;
+  }
+  if (b == batch.count) return;
+  const Problem pr = batch.p[b];
+  const int n = pr.n, q = static_cast<int>(blockIdx.x) - first;
+  const int j = q * kRefl, m = min(kRefl, n - 1 - j);
+  const double *__restrict__ A = matrices + pr.matrix;
+  const double *tau = work + pr.work + 3 * int64_t(n) * n + 2 * n;
+  double *T = work + pr.work + 3 * int64_t(n) * n + 7 * n + int64_t(q) * kRefl * kRefl;
+  __shared__ double Vs[64][kRefl + 1];
+  __shared__ double G[kRefl][kRefl + 1], Ts[kRefl][kRefl + 1];
+  const int t = threadIdx.x, ga = t >> 3, gc = (t & 7) * 4;
+  // G = V^T V over rows j+1..: reflector a of the block is nonzero from row j + a + 1 (where it is 1)
+  double g[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int i0 = j + 1; i0 < n; i0 += 64) {
+    __syncthreads();
+    for (int e = t; e < 64 * kRefl; e += kEigThreads) {
+      const int row = e & 63, a = e >> 6, i = i0 + row;
+      Vs[row][a] = (a < m && i < n && i > j + a) ? A[i + int64_t(j + a) * n] : 0.0;
+    }
+    __syncthreads();
+    for (int row = 0; row < 64; ++row) {
+      const double va = Vs[row][ga];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) g[u] += va * Vs[row][gc + u];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) G[ga][gc + u] = g[u];
+  __syncthreads();
+  // T column by column (forward order): T[i][i] = tau_i, T[a][i] = -tau_i sum_c T[a][c] G[c][i];
+  // thread a owns row a of T
+  if (t < kRefl) {
+    const int a = t;
+    for (int i = 0; i < kRefl; ++i) {
+      double value = 0.0;
+      if (i < m && a <= i) {
+        const double ti = tau[j + i];
+        if (a == i) {
+          value = ti;
+        } else {
+          double sum = 0.0;
+          for (int c = a; c < i; ++c) sum += Ts[a][c] * G[c][i];
+          value = -ti * sum;
+        }
+      }
+      Ts[a][i] = value;
+    }
+    for (int i = 0; i < kRefl; ++i) T[a * kRefl + i] = Ts[a][i];
+  }
+}
 
 __global__ __launch_bounds__(kEigThreads) void back_kernel(Batch batch, const double *__restrict__ matrices,
                                                             double *__restrict__ vectors, const double *__restrict__ work) {
@@ -597,36 +896,90 @@ __global__ __launch_bounds__(kEigThreads) void back_kernel(Batch batch, const do
   const Problem pr = batch.p[b];
   const int n = pr.n;
   const int c0 = (static_cast<int>(blockIdx.x) - pr.first_panel) * kPanel;
-  const int width = min(kPanel, n - c0);
-  const double *A = matrices + pr.matrix;
-  const double *Z = work + pr.work;
-  const double *tau = Z + 3 * int64_t(n) * n + 2 * n;
-  __shared__ double Y[kEigMaxN][kPanel];
-  __shared__ double part[kEigThreads / kPanel][kPanel];
-  __shared__ double w[kPanel];
-  const int t = threadIdx.x, c = t % kPanel, g = t / kPanel, groups = kEigThreads / kPanel;
-  for (int i = g; i < n; i += groups) Y[i][c] = c < width ? Z[i + int64_t(c0 + c) * n] : 0.0;
-  __syncthreads();
-  for (int k = n - 2; k >= 0; --k) {
-    const double tk = tau[k];
-    if (tk == 0.0) continue;
-    const double *v = A + int64_t(k) * n;   // v[k+1] = 1, v[k+2..] stored
-    double sum = 0.0;
-    for (int i = k + 1 + g; i < n; i += groups) sum += v[i] * Y[i][c];
-    part[g][c] = sum;
+  const double *__restrict__ A = matrices + pr.matrix;
+  const double *__restrict__ Z = work + pr.work + (dc_levels(n) & 1 ? int64_t(n) * n : 0);
+  const double *__restrict__ Tall = work + pr.work + 3 * int64_t(n) * n + 7 * n;
+  __shared__ double Wp[kEigWaves][kRefl][kPanel + 1];
+  __shared__ double Wq[kRefl][kPanel + 1], Wr[kRefl][kPanel + 1];
+  __shared__ double Ts[kRefl][kRefl + 1];
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63, lr = lane & 15, lk = lane >> 4;
+  const int col = c0 + lr;
+  double y[kRowBlocks][4];
+#pragma unroll
+  for (int mb = 0; mb < kRowBlocks; ++mb) {
+    const int base_row = 16 * (4 * mb + wave);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = base_row + 4 * r + lk;
+      y[mb][r] = (row < n && col < n) ? Z[row + int64_t(col) * n] : 0.0;
+    }
+  }
+  for (int q = reflector_blocks(n) - 1; q >= 0; --q) {
+    const int j = q * kRefl, m = min(kRefl, n - 1 - j);
     __syncthreads();
-    if (t < kPanel) {
-      double total = 0.0;
-      for (int q = 0; q < groups; ++q) total += part[q][t];
-      w[t] = tk * total;
+    for (int e = t; e < kRefl * kRefl; e += kEigThreads) Ts[e / kRefl][e % kRefl] = Tall[int64_t(q) * kRefl * kRefl + e];
+    // W = V^T Y: this wave's rows, both halves of the block's reflectors
+    d4 w0 = d4{0.0, 0.0, 0.0, 0.0}, w1 = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int mb = 0; mb < kRowBlocks; ++mb) {
+      const int base_row = 16 * (4 * mb + wave);
+      if (base_row + 16 <= j + 1 || base_row >= n) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = base_row + 4 * r + lk, a0 = lr, a1 = 16 + lr;
+        const double v0 = (a0 < m && i < n && i > j + a0) ? A[i + int64_t(j + a0) * n] : 0.0;
+        const double v1 = (a1 < m && i < n && i > j + a1) ? A[i + int64_t(j + a1) * n] : 0.0;
+        w0 = __builtin_amdgcn_mfma_f64_16x16x4f64(v0, y[mb][r], w0, 0, 0, 0);
+        w1 = __builtin_amdgcn_mfma_f64_16x16x4f64(v1, y[mb][r], w1, 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      Wp[wave][4 * r + lk][lr] = w0[r];
+      Wp[wave][16 + 4 * r + lk][lr] = w1[r];
     }
     __syncthreads();
-    for (int i = k + 1 + g; i < n; i += groups) Y[i][c] -= v[i] * w[c];
+    for (int e = t; e < kRefl * kPanel; e += kEigThreads) {
+      const int a = e / kPanel, c = e % kPanel;
+      Wq[a][c] = (Wp[0][a][c] + Wp[1][a][c]) + (Wp[2][a][c] + Wp[3][a][c]);
+    }
     __syncthreads();
+    // W = T W (T upper triangular)
+    for (int e = t; e < kRefl * kPanel; e += kEigThreads) {
+      const int a = e / kPanel, c = e % kPanel;
+      double sum = 0.0;
+      for (int i = a; i < kRefl; ++i) sum += Ts[a][i] * Wq[i][c];
+      Wr[a][c] = sum;
+    }
+    __syncthreads();
+    // Y -= V W
+#pragma unroll
+    for (int mb = 0; mb < kRowBlocks; ++mb) {
+      const int base_row = 16 * (4 * mb + wave);
+      if (base_row + 16 <= j + 1 || base_row >= n) continue;
+      d4 acc = d4{0.0, 0.0, 0.0, 0.0};
+      const int i = base_row + lr;
+#pragma unroll
+      for (int s4 = 0; s4 < kRefl / 4; ++s4) {
+        const int a = 4 * s4 + lk;
+        const double v = (a < m && i < n && i > j + a) ? A[i + int64_t(j + a) * n] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(v, Wr[a][lr], acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) y[mb][r] -= acc[r];
+    }
   }
   double *out = vectors + pr.matrix;
-  if (c < width)
-    for (int i = g; i < n; i += groups) out[int64_t(c0 + c) * n + i] = Y[i][c];
+  if (col < n)
+#pragma unroll
+    for (int mb = 0; mb < kRowBlocks; ++mb) {
+      const int base_row = 16 * (4 * mb + wave);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = base_row + 4 * r + lk;
+        if (row < n) out[int64_t(col) * n + row] = y[mb][r];
+      }
+    }
 }
 
 }  // namespace
@@ -639,7 +992,10 @@ extern "C" int64_t tt_eigh_workspace(const tt_eigh_problem *problems, int64_t co
     return -1;
   }
   int64_t total = 0;
-  for (int64_t b = 0; b < count; ++b) total += 3 * problems[b].n * problems[b].n + 3 * problems[b].n;
+  for (int64_t b = 0; b < count; ++b) {
+    const int64_t n = problems[b].n;
+    total += 3 * n * n + 7 * n + (n >= 1 && n <= kEigMaxN ? int64_t(kRefl) * kRefl * reflector_blocks(static_cast<int>(n)) : 0);
+  }
   return total;
 }
 
@@ -662,25 +1018,44 @@ extern "C" int tt_eigh_batched(const tt_eigh_problem *problems, int64_t count, d
     for (int32_t b = 0; b < batch.count; ++b) {
       const tt_eigh_problem &p = problems[first + b];
       batch.p[b] = Problem{p.matrix_offset, p.value_offset, work, static_cast<int32_t>(p.n), panels, leaves, 0};
-      work += 3 * p.n * p.n + 3 * p.n;
+      work += 3 * p.n * p.n + 7 * p.n + int64_t(kRefl) * kRefl * reflector_blocks(static_cast<int>(p.n));
       panels += static_cast<int32_t>((p.n + kPanel - 1) / kPanel);
       leaves += static_cast<int32_t>((p.n + kLeaf - 1) / kLeaf);
     }
     hipStream_t s = as_stream(stream);
     int64_t largest = 0;
     for (int32_t b = 0; b < batch.count; ++b) largest = std::max<int64_t>(largest, batch.p[b].n);
-    if (largest <= 257)
+    if (largest >= kStepMinOrder) {
+      for (int k = 0; k < largest; ++k) {
+        int grid = 0;
+        for (int32_t b = 0; b < batch.count; ++b) grid += step_blocks(batch.p[b].n, k);
+        hipLaunchKernelGGL(tridiag_step_kernel, dim3(grid), dim3(kStepThreads), 0, s, batch, matrices, workspace, k);
+        TT_LAUNCHED();
+      }
+    } else {
       hipLaunchKernelGGL(tridiag_kernel<1>, dim3(batch.count), dim3(kTriThreads), 0, s, batch, matrices, workspace);
-    else if (largest <= 513)
-      hipLaunchKernelGGL(tridiag_kernel<2>, dim3(batch.count), dim3(kTriThreads), 0, s, batch, matrices, workspace);
-    else
-      hipLaunchKernelGGL(tridiag_kernel<4>, dim3(batch.count), dim3(kTriThreads), 0, s, batch, matrices, workspace);
-    TT_LAUNCHED();
+      TT_LAUNCHED();
+    }
     hipLaunchKernelGGL(leaf_kernel, dim3((leaves + kEigWaves - 1) / kEigWaves), dim3(kEigThreads), 0, s, batch, values,
                        workspace, leaves);
     TT_LAUNCHED();
-    hipLaunchKernelGGL(dc_kernel, dim3(batch.count), dim3(kEigThreads), 0, s, batch, values, workspace);
-    TT_LAUNCHED();
+    for (int width = kLeaf, level = 0; width < largest; width *= 2, ++level) {
+      int units = 0, tiles = 0;
+      for (int32_t b = 0; b < batch.count; ++b) {
+        units += dc_units(batch.p[b].n, width);
+        tiles += dc_tiles(batch.p[b].n, width);
+      }
+      hipLaunchKernelGGL(dc_prep_kernel, dim3(units), dim3(kEigThreads), 0, s, batch, values, workspace, width, level & 1);
+      TT_LAUNCHED();
+      hipLaunchKernelGGL(dc_gemm_kernel, dim3(tiles), dim3(kEigThreads), 0, s, batch, workspace, width, level & 1);
+      TT_LAUNCHED();
+    }
+    int blocks = 0;
+    for (int32_t b = 0; b < batch.count; ++b) blocks += reflector_blocks(batch.p[b].n);
+    if (blocks > 0) {
+      hipLaunchKernelGGL(block_t_kernel, dim3(blocks), dim3(kEigThreads), 0, s, batch, matrices, workspace);
+      TT_LAUNCHED();
+    }
     hipLaunchKernelGGL(back_kernel, dim3(panels), dim3(kEigThreads), 0, s, batch, matrices, vectors, workspace);
     TT_LAUNCHED();
   }
