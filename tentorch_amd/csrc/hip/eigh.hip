@@ -7,7 +7,7 @@
 //      reflectors stay in the matrix's lower part.  Batches of small matrices run one workgroup per
 //      matrix in a single launch; a batch with a matrix of order kStepMinOrder or more runs one
 //      launch per step, with every matrix spread over one workgroup per 16 trailing columns.
-//   2. Cuppen divide and conquer on the tridiagonal: leaves of 32 rows by implicit QL (one wave
+//   2. Cuppen divide and conquer on the tridiagonal: leaves of 16 rows by implicit QL (one wave
 //      each), then merges bottom up, two launches per level for every matrix of the batch.  One
 //      workgroup per merge deflates small coupling components and close poles (Givens rotations),
 //      finds the secular roots by rational ("middle way") steps from the nearer pole (one thread
@@ -34,7 +34,7 @@ constexpr int kEigWaves = kEigThreads / 64;
 constexpr int kEigMaxN = 1024;      // largest matrix (LDS vectors are sized for it)
 constexpr int kEigBatch = 64;       // problems per launch (kernel-argument table)
 constexpr int kPanel = 16;          // columns per back-transformation workgroup
-constexpr int kLeaf = 32;           // order of the divide-and-conquer leaves
+constexpr int kLeaf = 16;           // order of the divide-and-conquer leaves
 constexpr double kEps = 2.220446049250313e-16;
 
 struct Problem {
@@ -54,10 +54,39 @@ struct Batch {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
+// One DPP move of a double (both halves); lanes of rows outside kRowMask receive 0.
+template <int kCtrl, int kRowMask>
+__device__ inline double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), kCtrl, kRowMask, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), kCtrl, kRowMask, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Largest of non-negative values over the wave's 64 lanes (the same DPP pattern as wave_sum).
+__device__ double wave_max(double v) {
+  v = fmax(v, dpp_move<0xb1, 0xf>(v));
+  v = fmax(v, dpp_move<0x4e, 0xf>(v));
+  v = fmax(v, dpp_move<0x141, 0xf>(v));
+  v = fmax(v, dpp_move<0x140, 0xf>(v));
+  v = fmax(v, dpp_move<0x142, 0xa>(v));
+  v = fmax(v, dpp_move<0x143, 0xc>(v));
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the wave's 64 lanes, the same value in every lane: quad and mirror moves total each
+// row of 16 lanes, two row broadcasts carry the totals into lane 63, which every lane then reads.
 __device__ double wave_sum(double v) {
-#pragma unroll
-  for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_xor(v, offset, 64);
-  return v;
+  v += dpp_move<0xb1, 0xf>(v);    // quad_perm [1, 0, 3, 2]
+  v += dpp_move<0x4e, 0xf>(v);    // quad_perm [2, 3, 0, 1]
+  v += dpp_move<0x141, 0xf>(v);   // row_half_mirror
+  v += dpp_move<0x140, 0xf>(v);   // row_mirror
+  v += dpp_move<0x142, 0xa>(v);   // row_bcast:15, into rows 1 and 3
+  v += dpp_move<0x143, 0xc>(v);   // row_bcast:31, into rows 2 and 3
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  return __hiloint2double(hi, lo);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -67,7 +96,7 @@ __device__ double wave_sum(double v) {
 //    column (group t / 256), eight columns at a time (four with four rows), so each thread keeps 16
 //    loads in flight;
 //    the four groups' partial products are summed in LDS in group order.  kTriRows (rows per thread)
-//    must cover the batch's largest matrix; it is launched with one (orders up to 256).
+//    must cover the batch's largest matrix; it is launched with one (orders up to 128).
 
 constexpr int kTriThreads = 1024;
 constexpr int kTriGroups = kTriThreads / 256;
@@ -212,7 +241,7 @@ __global__ __launch_bounds__(kTriThreads) void tridiag_kernel(Batch batch, doubl
 constexpr int kStepThreads = 1024;
 constexpr int kStepCols = kStepThreads / 64;
 constexpr int kStepChunks = kEigMaxN / 64;   // 64-row chunks of a column per lane
-constexpr int kStepMinOrder = 257;           // smaller batches stay in one workgroup per matrix
+constexpr int kStepMinOrder = 129;           // smaller batches stay in one workgroup per matrix
 static_assert(kEigMaxN <= kStepThreads, "one row of the step vectors per thread");
 
 // workgroups of a matrix of order n at step k
@@ -249,6 +278,15 @@ __global__ __launch_bounds__(kStepThreads) void tridiag_step_kernel(Batch batch,
     }
   }
   const double tprev = k > 0 ? tau[k - 1] : 0.0;
+  // this wave's trailing column, rows k+1.., loaded now so the loads overlap the step's O(n) part
+  const int wave = t >> 6, lane = t & 63;
+  const int c = k + 1 + block * kStepCols + wave;
+  double y[kStepChunks];
+#pragma unroll
+  for (int q = 0; q < kStepChunks; ++q) {
+    const int r = k + 1 + lane + 64 * q;
+    y[q] = (c < n && r < n) ? A[r + int64_t(c) * n] : 0.0;
+  }
   const double pv = tri_block_sum(p * v, red);
   const double w = p - 0.5 * tprev * pv * v;
   if (live) {
@@ -288,17 +326,9 @@ __global__ __launch_bounds__(kStepThreads) void tridiag_step_kernel(Batch batch,
   }
   __syncthreads();
   // this wave's column: step k-1's update on rows k+1.., and its entry of p
-  const int wave = t >> 6, lane = t & 63;
-  const int c = k + 1 + block * kStepCols + wave;
   if (c >= n) return;
   double *__restrict__ col = A + int64_t(c) * n;
   const double vpc = vp[c], wpc = wp[c];
-  double y[kStepChunks];
-#pragma unroll
-  for (int q = 0; q < kStepChunks; ++q) {
-    const int r = k + 1 + lane + 64 * q;
-    y[q] = r < n ? col[r] : 0.0;
-  }
   double acc = 0.0;
 #pragma unroll
   for (int q = 0; q < kStepChunks; ++q) {
@@ -427,43 +457,23 @@ struct Rotation {
   double c, s;
 };
 
-// The secular function f(x) = 1 + psi(x) + phi(x) of root j, with x measured from pole o: psi sums
-// over the kept poles 0..j (at or left of the root's interval), phi over the rest; both with their
-// derivatives.  One thread evaluates it serially; every thread of the workgroup reads the same pole
-// at the same time, an LDS broadcast.
-struct Secular {
-  double f, psi, phi, dpsi, dphi;
-};
-
 // 1/d from the hardware reciprocal estimate and one Newton-Raphson correction r + r (1 - d r).
 __device__ inline double reciprocal(double d) {
   const double r = __builtin_amdgcn_rcp(d);
   return fma(r, fma(-d, r, 1.0), r);
 }
 
-__device__ Secular secular(const double *dk, const double *z2, int k, int j, int o, double rho, double x) {
-  const double origin = dk[o];
-  double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0;
-  // one loop for both sums, so lanes working on different roots do not diverge
-  for (int i = 0; i < k; ++i) {
-    const double inv = reciprocal((dk[i] - origin) - x);
-    const double w = z2[i] * inv;
-    const bool left = i <= j;
-    psi += left ? w : 0.0;
-    dpsi += left ? w * inv : 0.0;
-    phi += left ? 0.0 : w;
-    dphi += left ? 0.0 : w * inv;
-  }
-  return Secular{1.0 + rho * (psi + phi), rho * psi, rho * phi, rho * dpsi, rho * dphi};
-}
-
 // A level of the divide and conquer is a list of units, blocks of up to 2 * width rows: the merge of
 // two solved halves, or (the last block of a matrix, when it has no partner at this level) a copy.
-// Level l reads the eigenvectors from buffer l % 2 (Q or Qt) and writes the other.  A unit's
-// preparation (one workgroup) leaves in the workspace its k x k matrix U (the eigenvectors of the
-// rank-one update in the basis of the kept poles, at 2 n^2 + lo * min(2 width, n)) and, per output
-// column j of the unit, the source column colsrc[lo + j] and the final column dst[lo + j]: columns
-// j < k are Q[:, colsrc] U[:, j], the others copies of Q[:, colsrc[j]]; unitk[lo] = k.
+// Level l reads the eigenvectors from buffer l % 2 (Q or Qt) and writes the other, in four
+// launches: deflation (one workgroup per unit), secular roots (one wave per root), finishing (one
+// workgroup per unit: the eigenvectors of the rank-one update, the merged eigenvalues, the column
+// maps) and the merge products.  A unit leaves in the workspace its k x k matrix U (the
+// eigenvectors of the rank-one update in the basis of the kept poles, at 2 n^2 + lo * min(2 width,
+// n)) and, per output column j of the unit, the source column colsrc[lo + j] and the final column
+// dst[lo + j]: columns j < k are Q[:, colsrc] U[:, j], the others copies of Q[:, colsrc[j]];
+// unitk[lo] = k.  Between the launches a unit's poles, coupling components and roots wait in the
+// scratch arrays of dc_scratch, indexed from lo.
 
 __host__ __device__ inline int dc_units(int n, int width) { return n > width ? (n + 2 * width - 1) / (2 * width) : 0; }
 
@@ -484,24 +494,71 @@ __device__ inline DcMaps dc_maps(double *base, int n) {
   return DcMaps{ints, ints + n, ints + 2 * n};
 }
 
-__global__ __launch_bounds__(kEigThreads) void dc_prep_kernel(Batch batch, double *__restrict__ values,
-                                                               double *__restrict__ work, int width, int parity) {
-  int b = 0, first = 0;
-  for (; b < batch.count; ++b) {
+// Per-unit arrays between the launches of a level, in the space of the back-transformation's T
+// matrices (at least 32 (n - 1) doubles, unused until the divide and conquer is done): kept poles
+// dk, their squared and signed coupling components z2 and zk, root offsets tau, all sorted poles,
+// rho (at lo); root origins org, merge order perm, kept positions keep, deflation flags defl.
+struct DcScratch {
+  double *dk, *z2, *zk, *tau, *pole, *rho;
+  int32_t *org, *perm, *keep, *defl;
+};
+
+__device__ inline DcScratch dc_scratch(double *base, int n) {
+  double *d = base + 3 * int64_t(n) * n + 7 * n;
+  int32_t *i = reinterpret_cast<int32_t *>(d + 6 * n);
+  return DcScratch{d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n, i, i + n, i + 2 * n, i + 3 * n};
+}
+
+// unit of workgroup `block` among the units of a level: problem index, lo (-1 past the batch)
+struct UnitRef {
+  int b, lo;
+};
+
+__device__ inline UnitRef find_unit(const Batch &batch, int width, int block) {
+  int first = 0;
+  for (int b = 0; b < batch.count; ++b) {
     const int units = dc_units(batch.p[b].n, width);
-    if (static_cast<int>(blockIdx.x) < first + units) break;
+    if (block < first + units) return UnitRef{b, (block - first) * 2 * width};
     first += units;
   }
-  if (b == batch.count) return;
-  const Problem pr = batch.p[b];
-  const int n = pr.n;
+  return UnitRef{-1, 0};
+}
+
+// Exclusive prefix sum over the workgroup (kEigThreads threads) of one value per thread; the total
+// is left in *total.  `scratch` holds kEigThreads + 1 ints.
+__device__ int block_exclusive_scan(int value, int *scratch, int *total) {
+  const int t = threadIdx.x;
+  scratch[t] = value;
+  __syncthreads();
+  for (int offset = 1; offset < kEigThreads; offset *= 2) {
+    const int add = t >= offset ? scratch[t - offset] : 0;
+    __syncthreads();
+    scratch[t] += add;
+    __syncthreads();
+  }
+  const int inclusive = scratch[t];
+  *total = scratch[kEigThreads - 1];
+  __syncthreads();
+  return inclusive - value;
+}
+
+// Per thread, the elements [4 t, 4 t + 4) of an s-long list (s <= kEigMaxN = 4 kEigThreads).
+constexpr int kPerThread = kEigMaxN / kEigThreads;
+static_assert(kPerThread * kEigThreads == kEigMaxN, "four list elements per thread");
+
+__global__ __launch_bounds__(kEigThreads) void dc_deflate_kernel(Batch batch, double *__restrict__ values,
+                                                                  double *__restrict__ work, int width, int parity) {
+  const UnitRef u = find_unit(batch, width, static_cast<int>(blockIdx.x));
+  if (u.b < 0) return;
+  const Problem pr = batch.p[u.b];
+  const int n = pr.n, lo = u.lo;
   double *const base = work + pr.work;
   double *__restrict__ Q = base + (parity ? int64_t(n) * n : 0);
   const double *diag = base + 3 * int64_t(n) * n, *off = diag + n;
   const DcMaps maps = dc_maps(base, n);
-  double *lam = values + pr.values;
-  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-  const int lo = (static_cast<int>(blockIdx.x) - first) * 2 * width;
+  const DcScratch sc = dc_scratch(base, n);
+  const double *lam = values + pr.values;
+  const int t = threadIdx.x;
   const int mid = lo + width, hi = min(lo + 2 * width, n), s = hi - lo, left = mid - lo;
   if (mid >= n) {   // no partner at this level: the block moves over unchanged
     for (int j = t; j < s; j += kEigThreads) {
@@ -511,69 +568,107 @@ __global__ __launch_bounds__(kEigThreads) void dc_prep_kernel(Batch batch, doubl
     if (t == 0) maps.unitk[lo] = 0;
     return;
   }
-  double *__restrict__ U = base + 2 * int64_t(n) * n + int64_t(lo) * min(2 * width, n);
   __shared__ double dloc[kEigMaxN], zloc[kEigMaxN], pole[kEigMaxN], zsort[kEigMaxN];
-  __shared__ double dk[kEigMaxN], z2[kEigMaxN], zk[kEigMaxN], tau[kEigMaxN], zh[kEigMaxN], val[kEigMaxN];
-  __shared__ int32_t perm[kEigMaxN], defl[kEigMaxN], keep[kEigMaxN], org[kEigMaxN], src[kEigMaxN], rnk[kEigMaxN];
+  __shared__ int32_t perm[kEigMaxN], defl[kEigMaxN], cand[kEigMaxN], keep[kEigMaxN];
   __shared__ Rotation rot[kEigMaxN];  // each rotation deflates one pole: at most s - 1
-  __shared__ int s_k, s_rot;
-  __shared__ double s_rho;
+  __shared__ int scan[kEigThreads + 1];
+  __shared__ double red[kEigWaves];
+  __shared__ int s_rot;
   const double beta = off[mid - 1];
-  const double sign = beta < 0.0 ? -1.0 : 1.0;
+  const double sign = beta < 0.0 ? -1.0 : 1.0, rho = 2.0 * beta * sign;
   // (a) coupling vector (last row of the left block, first row of the right) and poles
   for (int i = t; i < s; i += kEigThreads) {
     const double q = i < left ? Q[(mid - 1) + int64_t(lo + i) * n] : Q[mid + int64_t(lo + i) * n];
     zloc[i] = q * 0.7071067811865476;
     dloc[i] = sign * lam[lo + i];
+    perm[i] = i;   // overwritten below; keeps every entry a valid index whatever the values
   }
   __syncthreads();
-  // (b) one thread: merge the sorted halves, deflate small z and close poles
+  // (b) merge the two ascending halves (a negated half is descending and walked from its end):
+  //     an element's position is its rank in its half plus the count of the other half's elements
+  //     before it, ties going to the left half first
+  for (int i = t; i < s; i += kEigThreads) {
+    const double x = dloc[i];
+    const bool in_a = i < left;
+    const int rank = in_a ? (sign > 0 ? i : left - 1 - i) : (sign > 0 ? i - left : s - 1 - i);
+    const int other = in_a ? s - left : left;
+    int lo_b = 0, hi_b = other;   // count of the other half's elements before x
+    while (lo_b < hi_b) {
+      const int m = (lo_b + hi_b) >> 1;
+      const int at = in_a ? (sign > 0 ? left + m : s - 1 - m) : (sign > 0 ? m : left - 1 - m);
+      const bool before = in_a ? dloc[at] < x : dloc[at] <= x;
+      if (before) lo_b = m + 1; else hi_b = m;
+    }
+    const int position = rank + lo_b;
+    if (position < s) perm[position] = i;
+  }
+  __syncthreads();
+  double dmax = 0.0;
+  for (int p = t; p < s; p += kEigThreads) {
+    pole[p] = dloc[perm[p]];
+    zsort[p] = zloc[perm[p]];
+    dmax = fmax(dmax, fabs(pole[p]));
+  }
+  dmax = wave_max(dmax);
+  if ((t & 63) == 0) red[t >> 6] = dmax;
+  __syncthreads();
+  dmax = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  const double tol = 8.0 * kEps * fmax(dmax, rho);
+  // small coupling components deflate at once; the rest are candidates for the close-pole test
+  int mine = 0;
+  for (int e = 0; e < kPerThread; ++e) {
+    const int p = kPerThread * t + e;
+    const int d = p < s && rho * fabs(zsort[p]) <= tol ? 1 : 0;
+    if (p < s) defl[p] = d;
+    mine += p < s && !d ? 1 : 0;
+  }
+  int candidates;
+  int at = block_exclusive_scan(mine, scan, &candidates);
+  for (int e = 0; e < kPerThread; ++e) {
+    const int p = kPerThread * t + e;
+    if (p < s && !defl[p]) cand[at++] = p;
+  }
+  __syncthreads();
+  // one thread: close poles, in order; a pair is rotated so one of its components vanishes when
+  // |(d_p - d_prev) c s| <= tol, which is tested after a cheaper bound rules most pairs out
   if (t == 0) {
-    const double rho = 2.0 * beta * sign;
-    int a = 0, bb = 0;
-    for (int p = 0; p < s; ++p) {
-      // ascending in dloc: a negated half is descending, so it is walked from its end
-      const int ia = sign > 0 ? a : left - 1 - a, ib = sign > 0 ? left + bb : s - 1 - bb;
-      const bool take_a = bb >= s - left || (a < left && dloc[ia] <= dloc[ib]);
-      perm[p] = take_a ? ia : ib;
-      if (take_a) ++a; else ++bb;
-    }
-    double dmax = 0.0;
-    for (int p = 0; p < s; ++p) {
-      pole[p] = dloc[perm[p]];
-      zsort[p] = zloc[perm[p]];
-      dmax = fmax(dmax, fabs(pole[p]));
-    }
-    const double tol = 8.0 * kEps * fmax(dmax, rho);
     int nrot = 0, prev = -1;
-    for (int p = 0; p < s; ++p) {
-      defl[p] = rho * fabs(zsort[p]) <= tol ? 1 : 0;
-      if (defl[p]) continue;
+    for (int c = 0; c < candidates; ++c) {
+      const int p = cand[c];
       if (prev >= 0) {
-        const double r = hypot(zsort[p], zsort[prev]);
-        const double c = zsort[p] / r, sn = -zsort[prev] / r;
-        if (fabs((pole[p] - pole[prev]) * c * sn) <= tol) {
-          zsort[p] = r;
-          zsort[prev] = 0.0;
-          rot[nrot++] = Rotation{perm[prev], perm[p], c, sn};
-          const double dp = pole[prev] * c * c + pole[p] * sn * sn;
-          pole[p] = pole[prev] * sn * sn + pole[p] * c * c;
-          pole[prev] = dp;
-          defl[prev] = 1;
+        const double zp = zsort[p], zq = zsort[prev], gap = fabs(pole[p] - pole[prev]);
+        if (gap * fabs(zp * zq) <= 2.0 * tol * (zp * zp + zq * zq)) {
+          const double r = hypot(zp, zq);
+          const double cs = zp / r, sn = -zq / r;
+          if (fabs((pole[p] - pole[prev]) * cs * sn) <= tol) {
+            zsort[p] = r;
+            zsort[prev] = 0.0;
+            rot[nrot++] = Rotation{perm[prev], perm[p], cs, sn};
+            const double dp = pole[prev] * cs * cs + pole[p] * sn * sn;
+            pole[p] = pole[prev] * sn * sn + pole[p] * cs * cs;
+            pole[prev] = dp;
+            defl[prev] = 1;
+          }
         }
       }
       prev = p;
     }
-    int k = 0;
-    for (int p = 0; p < s; ++p)
-      if (!defl[p]) keep[k++] = p;
-    s_k = k;
     s_rot = nrot;
-    s_rho = rho;
   }
   __syncthreads();
-  const int k = s_k, nrot = s_rot;
-  const double rho = s_rho;
+  const int nrot = s_rot;
+  mine = 0;
+  for (int e = 0; e < kPerThread; ++e) {
+    const int p = kPerThread * t + e;
+    mine += p < s && !defl[p] ? 1 : 0;
+  }
+  int k;
+  at = block_exclusive_scan(mine, scan, &k);
+  for (int e = 0; e < kPerThread; ++e) {
+    const int p = kPerThread * t + e;
+    if (p < s && !defl[p]) keep[at++] = p;
+  }
+  __syncthreads();
   // (c) the deflating rotations on Q's columns, in order; each thread owns its rows
   for (int r = lo + t; r < hi; r += kEigThreads)
     for (int q = 0; q < nrot; ++q) {
@@ -582,72 +677,209 @@ __global__ __launch_bounds__(kEigThreads) void dc_prep_kernel(Batch batch, doubl
       *ca = rot[q].c * x + rot[q].s * y;
       *cb = -rot[q].s * x + rot[q].c * y;
     }
+  // what the next launches read
   for (int j = t; j < k; j += kEigThreads) {
-    dk[j] = pole[keep[j]];
-    zk[j] = zsort[keep[j]];
-    z2[j] = zk[j] * zk[j];
+    const double z = zsort[keep[j]];
+    sc.dk[lo + j] = pole[keep[j]];
+    sc.zk[lo + j] = z;
+    sc.z2[lo + j] = z * z;
+    sc.keep[lo + j] = keep[j];
   }
-  __syncthreads();
-  // (d) secular roots, one thread per root, from the nearer pole of its interval.  Each step
-  //     models the sums left and right of the interval by one pole each, matching value and
-  //     slope at the iterate, and takes the root of that model (the rational "middle way"
-  //     step); steps that leave the bracket bisect it.
-  for (int j = t; j < k; j += kEigThreads) {
-    int o = j;
-    double a = 0.0, bnd;
-    if (j + 1 < k) {
-      const double gap = dk[j + 1] - dk[j];
-      if (secular(dk, z2, k, j, j, rho, 0.5 * gap).f > 0.0) {
-        bnd = 0.5 * gap;
+  for (int p = t; p < s; p += kEigThreads) {
+    sc.pole[lo + p] = pole[p];
+    sc.perm[lo + p] = perm[p];
+    sc.defl[lo + p] = defl[p];
+  }
+  if (t == 0) {
+    sc.rho[lo] = rho;
+    maps.unitk[lo] = k;
+  }
+}
+
+// Secular roots, one wave per root j of a unit: lane l holds the kept poles l, l + 64, ... in
+// registers; every evaluation of f(x) = 1 + psi(x) + phi(x) (x measured from the root's origin
+// pole; psi sums over the poles 0..j, phi over the rest) sums them lane-wise and then over the
+// wave, so every lane holds the same f and takes the same step.  The root starts from the nearer
+// pole of its interval; each step models psi and phi by one pole each, matching value and slope
+// at the iterate, and takes the root of that model (the rational "middle way" step); steps that
+// leave the bracket bisect it.  Converged when f is within its own rounding error, the bracket is
+// at the resolution of x, or a step no longer changes x.
+
+constexpr int kPoleChunks = kEigMaxN / 64;
+
+// root blocks (one per workgroup, kEigWaves roots) of a matrix of order n at a level
+__host__ __device__ inline int dc_root_blocks(int n, int width) {
+  const int units = dc_units(n, width);
+  if (units == 0) return 0;
+  const int last = n - (units - 1) * 2 * width;
+  return (units - 1) * ((2 * width + kEigWaves - 1) / kEigWaves) + (last + kEigWaves - 1) / kEigWaves;
+}
+
+struct Secular {
+  double f, psi, phi, dpsi, dphi;
+};
+
+__device__ inline Secular secular_wave(const double (&d)[kPoleChunks], const double (&w2)[kPoleChunks], int k, int j,
+                                       double origin, double rho, double x) {
+  const int lane = threadIdx.x & 63;
+  double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0;
+#pragma unroll
+  for (int m = 0; m < kPoleChunks; ++m) {
+    const int i = lane + 64 * m;
+    if (i < k) {
+      const double inv = reciprocal((d[m] - origin) - x);
+      const double w = w2[m] * inv;
+      if (i <= j) {
+        psi += w;
+        dpsi += w * inv;
       } else {
-        o = j + 1;
-        a = -0.5 * gap;
-        bnd = 0.0;
+        phi += w;
+        dphi += w * inv;
       }
+    }
+  }
+  psi = wave_sum(psi);
+  dpsi = wave_sum(dpsi);
+  phi = wave_sum(phi);
+  dphi = wave_sum(dphi);
+  return Secular{1.0 + rho * (psi + phi), rho * psi, rho * phi, rho * dpsi, rho * dphi};
+}
+
+__global__ __launch_bounds__(kEigThreads) void dc_secular_kernel(Batch batch, double *__restrict__ work, int width) {
+  int b = 0, first = 0;
+  for (; b < batch.count; ++b) {
+    const int blocks = dc_root_blocks(batch.p[b].n, width);
+    if (static_cast<int>(blockIdx.x) < first + blocks) break;
+    first += blocks;
+  }
+  if (b == batch.count) return;
+  const Problem pr = batch.p[b];
+  const int n = pr.n;
+  const int units = dc_units(n, width), full = (2 * width + kEigWaves - 1) / kEigWaves;
+  int local = static_cast<int>(blockIdx.x) - first, unit = local / full;
+  if (unit >= units - 1) {
+    unit = units - 1;
+    local -= (units - 1) * full;
+  } else {
+    local -= unit * full;
+  }
+  const int lo = unit * 2 * width;
+  double *const base = work + pr.work;
+  const DcMaps maps = dc_maps(base, n);
+  const DcScratch sc = dc_scratch(base, n);
+  const int k = maps.unitk[lo];
+  const int j = local * kEigWaves + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (j >= k) return;
+  const double rho = sc.rho[lo];
+  double d[kPoleChunks], w2[kPoleChunks];
+#pragma unroll
+  for (int m = 0; m < kPoleChunks; ++m) {
+    const int i = lane + 64 * m;
+    d[m] = i < k ? sc.dk[lo + i] : 0.0;
+    w2[m] = i < k ? sc.z2[lo + i] : 0.0;
+  }
+  const double dj = sc.dk[lo + j];
+  int o = j;
+  double a = 0.0, bnd;
+  if (j + 1 < k) {
+    const double gap = sc.dk[lo + j + 1] - dj;
+    if (secular_wave(d, w2, k, j, dj, rho, 0.5 * gap).f > 0.0) {
+      bnd = 0.5 * gap;
     } else {
-      double zs = 0.0;
-      for (int i = 0; i < k; ++i) zs += z2[i];
-      bnd = rho * zs;
+      o = j + 1;
+      a = -0.5 * gap;
+      bnd = 0.0;
     }
-    const double lo_pole = dk[j] - dk[o];
-    const double hi_pole = j + 1 < k ? dk[j + 1] - dk[o] : 0.0;
-    double x = 0.5 * (a + bnd);
-    for (int it = 0; it < 80; ++it) {
-      const Secular v = secular(dk, z2, k, j, o, rho, x);
-      if (v.f > 0.0) bnd = x; else a = x;
-      // converged when f is within its own rounding error, or the bracket is at the
-      // resolution of x
-      if (fabs(v.f) <= 8.0 * kEps * k * (1.0 + fabs(v.psi) + fabs(v.phi)) ||
-          bnd - a <= 4.0 * kEps * fmax(fabs(a), fabs(bnd)))
-        break;
-      const double dlo = lo_pole - x;
-      double eta;
-      if (j + 1 < k) {
-        const double dhi = hi_pole - x;
-        const double S = dlo * dlo * v.dpsi, T = dhi * dhi * v.dphi;
-        const double c = v.f - dlo * v.dpsi - dhi * v.dphi;
-        const double B = c * (dlo + dhi) + S + T, C = c * dlo * dhi + S * dhi + T * dlo;
-        const double root = sqrt(fmax(B * B - 4.0 * c * C, 0.0));
-        if (B >= 0.0)
-          eta = 2.0 * C / (B + root);
-        else if (c != 0.0)
-          eta = (B - root) / (2.0 * c);
-        else
-          eta = C / B;
-      } else {
-        const double c = v.f - dlo * v.dpsi;
-        eta = c != 0.0 ? dlo + dlo * dlo * v.dpsi / c : 0.5 * (bnd - a);
-      }
-      double next = x + eta;
-      if (!(next > a && next < bnd)) next = 0.5 * (a + bnd);
-      if (fabs(next - x) <= 2.0 * kEps * fabs(x)) {
-        x = next;
-        break;
-      }
+  } else {
+    double zs = 0.0;
+#pragma unroll
+    for (int m = 0; m < kPoleChunks; ++m) zs += w2[m];
+    bnd = rho * wave_sum(zs);
+  }
+  const double origin = sc.dk[lo + o];
+  const double lo_pole = dj - origin;
+  const double hi_pole = j + 1 < k ? sc.dk[lo + j + 1] - origin : 0.0;
+  double x = 0.5 * (a + bnd);
+  for (int it = 0; it < 80; ++it) {
+    const Secular v = secular_wave(d, w2, k, j, origin, rho, x);
+    if (v.f > 0.0) bnd = x; else a = x;
+    if (fabs(v.f) <= 8.0 * kEps * k * (1.0 + fabs(v.psi) + fabs(v.phi)) ||
+        bnd - a <= 4.0 * kEps * fmax(fabs(a), fabs(bnd)))
+      break;
+    const double dlo = lo_pole - x;
+    double eta;
+    if (j + 1 < k) {
+      const double dhi = hi_pole - x;
+      const double S = dlo * dlo * v.dpsi, T = dhi * dhi * v.dphi;
+      const double c = v.f - dlo * v.dpsi - dhi * v.dphi;
+      const double B = c * (dlo + dhi) + S + T, C = c * dlo * dhi + S * dhi + T * dlo;
+      const double root = sqrt(fmax(B * B - 4.0 * c * C, 0.0));
+      if (B >= 0.0)
+        eta = 2.0 * C / (B + root);
+      else if (c != 0.0)
+        eta = (B - root) / (2.0 * c);
+      else
+        eta = C / B;
+    } else {
+      const double c = v.f - dlo * v.dpsi;
+      eta = c != 0.0 ? dlo + dlo * dlo * v.dpsi / c : 0.5 * (bnd - a);
+    }
+    double next = x + eta;
+    if (!(next > a && next < bnd)) next = 0.5 * (a + bnd);
+    if (fabs(next - x) <= 2.0 * kEps * fabs(x)) {
       x = next;
+      break;
     }
-    org[j] = o;
-    tau[j] = x;
+    x = next;
+  }
+  if (lane == 0) {
+    sc.org[lo + j] = o;
+    sc.tau[lo + j] = x;
+  }
+}
+
+__global__ __launch_bounds__(kEigThreads) void dc_finish_kernel(Batch batch, double *__restrict__ values,
+                                                                 double *__restrict__ work, int width) {
+  const UnitRef u = find_unit(batch, width, static_cast<int>(blockIdx.x));
+  if (u.b < 0) return;
+  const Problem pr = batch.p[u.b];
+  const int n = pr.n, lo = u.lo;
+  const int mid = lo + width, hi = min(lo + 2 * width, n), s = hi - lo;
+  if (mid >= n) return;   // a copy: its maps are written
+  double *const base = work + pr.work;
+  const DcMaps maps = dc_maps(base, n);
+  const DcScratch sc = dc_scratch(base, n);
+  double *lam = values + pr.values;
+  double *__restrict__ U = base + 2 * int64_t(n) * n + int64_t(lo) * min(2 * width, n);
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  __shared__ double dk[kEigMaxN], zk[kEigMaxN], tau[kEigMaxN], zh[kEigMaxN], val[kEigMaxN];
+  __shared__ int32_t org[kEigMaxN], src[kEigMaxN];
+  __shared__ int scan[kEigThreads + 1];
+  const int k = maps.unitk[lo];
+  const double rho = sc.rho[lo];
+  const double *off = base + 3 * int64_t(n) * n + n;
+  const double sign = off[mid - 1] < 0.0 ? -1.0 : 1.0;
+  for (int j = t; j < k; j += kEigThreads) {
+    dk[j] = sc.dk[lo + j];
+    zk[j] = sc.zk[lo + j];
+    tau[j] = sc.tau[lo + j];
+    org[j] = sc.org[lo + j];
+  }
+  // slots: 0..k-1 the roots, then the deflated poles in order
+  int mine = 0;
+  for (int e = 0; e < kPerThread; ++e) {
+    const int p = kPerThread * t + e;
+    mine += p < s && sc.defl[lo + p] ? 1 : 0;
+  }
+  int deflated;
+  int at = k + block_exclusive_scan(mine, scan, &deflated);
+  for (int e = 0; e < kPerThread; ++e) {
+    const int p = kPerThread * t + e;
+    if (p < s && sc.defl[lo + p]) {
+      val[at] = sc.pole[lo + p];
+      src[at] = sc.perm[lo + p];
+      ++at;
+    }
   }
   __syncthreads();
   // (e) Gu-Eisenstat: zh_i^2 = (lam_i - d_i)/rho prod_{j != i} (lam_j - d_i)/(d_j - d_i)
@@ -656,6 +888,8 @@ __global__ __launch_bounds__(kEigThreads) void dc_prep_kernel(Batch batch, doubl
     for (int j = 0; j < k; ++j)
       if (j != i) w *= ((dk[org[j]] - dk[i]) + tau[j]) / (dk[j] - dk[i]);
     zh[i] = copysign(sqrt(fabs(w)), zk[i]);
+    val[i] = dk[org[i]] + tau[i];
+    src[i] = sc.perm[lo + sc.keep[lo + i]];
   }
   __syncthreads();
   // (f) U[:, j] = zh / (d - lam_j), normalised (k x k, column-major): one wave per column
@@ -663,27 +897,12 @@ __global__ __launch_bounds__(kEigThreads) void dc_prep_kernel(Batch batch, doubl
     const double dj = dk[org[j]], tj = tau[j];
     double norm = 0.0;
     for (int i = lane; i < k; i += 64) {
-      const double u = zh[i] / ((dk[i] - dj) - tj);
-      norm += u * u;
+      const double v = zh[i] / ((dk[i] - dj) - tj);
+      norm += v * v;
     }
     const double inv = 1.0 / sqrt(wave_sum(norm));
     for (int i = lane; i < k; i += 64) U[i + int64_t(j) * k] = inv * zh[i] / ((dk[i] - dj) - tj);
   }
-  // (g) slots: 0..k-1 the roots, then the deflated poles; their values and sources
-  if (t == 0) {
-    int slot = k;
-    for (int p = 0; p < s; ++p)
-      if (defl[p]) {
-        val[slot] = pole[p];
-        src[slot] = perm[p];
-        ++slot;
-      }
-  }
-  for (int j = t; j < k; j += kEigThreads) {
-    val[j] = dk[org[j]] + tau[j];
-    src[j] = perm[keep[j]];
-  }
-  __syncthreads();
   // (h) ascending rank of every slot (ties by slot): its final column and eigenvalue
   for (int x = t; x < s; x += kEigThreads) {
     int r = 0;
@@ -693,7 +912,6 @@ __global__ __launch_bounds__(kEigThreads) void dc_prep_kernel(Batch batch, doubl
     maps.colsrc[lo + x] = src[x];
     maps.dst[lo + x] = r;
   }
-  if (t == 0) maps.unitk[lo] = k;
 }
 
 // (i) the unit's output columns: one workgroup per 64 x 64 tile of (rows, output columns).  Copies
@@ -1045,7 +1263,13 @@ extern "C" int tt_eigh_batched(const tt_eigh_problem *problems, int64_t count, d
         units += dc_units(batch.p[b].n, width);
         tiles += dc_tiles(batch.p[b].n, width);
       }
-      hipLaunchKernelGGL(dc_prep_kernel, dim3(units), dim3(kEigThreads), 0, s, batch, values, workspace, width, level & 1);
+      int roots = 0;
+      for (int32_t b = 0; b < batch.count; ++b) roots += dc_root_blocks(batch.p[b].n, width);
+      hipLaunchKernelGGL(dc_deflate_kernel, dim3(units), dim3(kEigThreads), 0, s, batch, values, workspace, width, level & 1);
+      TT_LAUNCHED();
+      hipLaunchKernelGGL(dc_secular_kernel, dim3(roots), dim3(kEigThreads), 0, s, batch, workspace, width);
+      TT_LAUNCHED();
+      hipLaunchKernelGGL(dc_finish_kernel, dim3(units), dim3(kEigThreads), 0, s, batch, values, workspace, width);
       TT_LAUNCHED();
       hipLaunchKernelGGL(dc_gemm_kernel, dim3(tiles), dim3(kEigThreads), 0, s, batch, workspace, width, level & 1);
       TT_LAUNCHED();
