@@ -15,7 +15,10 @@ attached to the kernel dispatch itself (hipExtLaunchKernelGGL), so that each int
 alone; the grouped GEMM of the matvec is the dominant kernel, and its roofline line uses the algorithmic 2mnk of
 its descriptor table over its average launch time.  The plain FP64 form then sweeps once from the
 state the timed sweeps started from, and |dE| between the two forms' first sweep from that state is
-reported (for the FP32 form this is |E - E_FP64|), with the truncated weight of both.  The CPU
+reported (for the FP32 form this is |E - E_FP64|), with the truncated weight of both.  The plain
+form splits the two-site tensor as the reference does, one torch.linalg.svd per block
+(`speedup_vs_plain`); a second plain sweep shares the batched form's device eigensolver for the split,
+so that its ratio isolates the contraction path (`plain_sweeps`).  The CPU
 baseline times the oracle (the numpy restatement of the reference engine) on the reference's own
 CPU case, L = 16, chi = 32, 3 sweeps, on one host thread.
 
@@ -112,7 +115,10 @@ def main() -> int:
     parser.add_argument('--precision', choices=('fp64', 'fp32'), default='fp32',
                         help='block products of the batched form: fp32 (with FP64 defect correction) or fp64')
     parser.add_argument('--refine', type=int, default=2, help='FP64 Lanczos steps after the FP32 solve')
-    parser.add_argument('--no-plain', action='store_true', help='skip the plain per-block sweep')
+    parser.add_argument('--no-plain', action='store_true', help='skip the plain per-block sweeps')
+    parser.add_argument('--plain-split', choices=('reference', 'shared', 'both'), default='both',
+                        help="plain sweeps' split: per-block torch SVD as the reference, the batched "
+                             "form's device eigensolver, or one sweep with each")
     parser.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     parser.add_argument('--no-profile', action='store_true', help='do not time launches with events')
     parser.add_argument('--breakdown', action='store_true',
@@ -130,6 +136,7 @@ def main() -> int:
         return 1
     import tentorch_amd as tt
     from tentorch_amd.tensor import Profiler, Settings, workspaces
+    from tentorch_amd.linalg import SvdSettings
 
     log(f'[bench] device {torch.cuda.get_device_name(0)}; L={args.length} chi={args.bond} '
         f'n_iter={args.iterations} precision={args.precision} warmup={args.warmup} steps={args.steps}')
@@ -184,27 +191,39 @@ def main() -> int:
         breakdown = time_breakdown(driver)
         log(f'[bench] breakdown of one sweep: {json.dumps(breakdown)}')
 
-    plain_per_sweep, plain_energy, delta, plain_lost = None, None, None, None
-    if not args.no_plain:
+    def plain_sweep(split):
+        '''One plain FP64 sweep from the snapshot: per-block launches for every contraction, and the
+        split either per block by torch.linalg.svd, as the reference factorises (svd.cpp:59), or
+        by the batched device eigensolver the batched form uses.'''
         state = driver.state
         state.tensors = [t.clone() for t in snapshot[0]]
         state.center = snapshot[1]
         workspaces.clear()
         Settings.mode = 1
-        precision = driver.precision
+        precision, method = driver.precision, SvdSettings.method
         driver.precision = 'fp64'
+        SvdSettings.method = 'exact' if split == 'reference' else 'gram'
         try:
             torch.cuda.synchronize()
             start = time.perf_counter()
-            plain_energy, plain_lost = driver.sweep()
+            energy, lost = driver.sweep()
             torch.cuda.synchronize()
-            plain_per_sweep = time.perf_counter() - start
+            elapsed = time.perf_counter() - start
         finally:
             Settings.mode = 0
-            driver.precision = precision
-        delta = abs(plain_energy - energies[0])
-        log(f'[bench] plain sweep: {plain_per_sweep:.2f} s energy {plain_energy:.12f} |dE| {delta:.2e} '
-            f'truncated {plain_lost:.2e}')
+            driver.precision, SvdSettings.method = precision, method
+        log(f'[bench] plain sweep ({split} split): {elapsed:.2f} s energy {energy:.12f} '
+            f'|dE| {abs(energy - energies[0]):.2e} truncated {lost:.2e}')
+        return elapsed, energy, lost
+
+    plain = {}
+    if not args.no_plain:
+        for split in ('reference', 'shared'):
+            if args.plain_split in (split, 'both'):
+                plain[split] = plain_sweep(split)
+    first_plain = plain.get('reference') or plain.get('shared')
+    plain_per_sweep, plain_energy, plain_lost = first_plain if first_plain else (None, None, None)
+    delta = None if plain_energy is None else abs(plain_energy - energies[0])
 
     roofline = None
     matvec = 'EPaqj,JRSFb,AQsb->EPrf'
@@ -259,9 +278,13 @@ def main() -> int:
                    'precision': args.precision, 'refine': args.refine if compute == 'f32' else 0},
         'sweeps_s': [round(t, 4) for t in times], 'energies': energies, 'truncated': truncated,
         'plain_truncated': plain_lost,
+        'plain_split': 'reference' if 'reference' in plain else ('shared' if plain else None),
         'plain_s_per_sweep': None if plain_per_sweep is None else round(plain_per_sweep, 4),
         'speedup_vs_plain': None if plain_per_sweep is None else round(plain_per_sweep / per_sweep, 3),
         'dE_plain_vs_batched': delta,
+        'plain_sweeps': {split: {'s_per_sweep': round(t, 4), 'speedup': round(t / per_sweep, 3),
+                                 'dE': abs(e - energies[0]), 'truncated': lost}
+                         for split, (t, e, lost) in plain.items()},
         'roofline': roofline, 'cpu_baseline': baseline, 'breakdown_s': breakdown,
     }
     print(json.dumps(line), flush=True)

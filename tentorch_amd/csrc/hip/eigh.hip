@@ -31,7 +31,7 @@ namespace {
 
 constexpr int kEigThreads = 256;
 constexpr int kEigWaves = kEigThreads / 64;
-constexpr int kEigMaxN = 1024;      // largest matrix (LDS vectors are sized for it)
+constexpr int kEigMaxN = 2048;      // largest matrix (LDS vectors are sized for it)
 constexpr int kEigBatch = 64;       // problems per launch (kernel-argument table)
 constexpr int kPanel = 16;          // columns per back-transformation workgroup
 constexpr int kLeaf = 16;           // order of the divide-and-conquer leaves
@@ -100,6 +100,7 @@ __device__ double wave_sum(double v) {
 
 constexpr int kTriThreads = 1024;
 constexpr int kTriGroups = kTriThreads / 256;
+constexpr int kTriMaxN = 256;   // orders the one-workgroup form takes (one row per thread of a group)
 
 __device__ double tri_block_sum(double v, double *scratch) {
   v = wave_sum(v);
@@ -112,16 +113,16 @@ __device__ double tri_block_sum(double v, double *scratch) {
 }
 
 template <int kTriRows>
-__global__ __launch_bounds__(kTriThreads) void tridiag_kernel(Batch batch, double *__restrict__ matrices,
+__global__ __launch_bounds__(kTriThreads) void tridiag_kernel(const Batch *__restrict__ batch, double *__restrict__ matrices,
                                                                double *__restrict__ work) {
   constexpr int kTriUnroll = kTriRows == 4 ? 4 : 8;   // columns per thread per batch of loads
-  const Problem pr = batch.p[blockIdx.x];
+  const Problem pr = batch->p[blockIdx.x];
   const int n = pr.n;
   double *__restrict__ A = matrices + pr.matrix;
   double *diag = work + pr.work + 3 * int64_t(n) * n, *off = diag + n, *tau = off + n;
   // (v, w) of the previous step and of this one, swapped by index after every step
-  __shared__ double vbuf[2][kEigMaxN], wbuf[2][kEigMaxN];
-  __shared__ double part[kTriGroups][kEigMaxN];
+  __shared__ double vbuf[2][kTriMaxN], wbuf[2][kTriMaxN];
+  __shared__ double part[kTriGroups][kTriMaxN];
   __shared__ double red[kTriThreads / 64];
   __shared__ double s_alpha, s_diag;
   const int t = threadIdx.x, row0 = t & 255, group = t >> 8;
@@ -242,23 +243,25 @@ constexpr int kStepThreads = 1024;
 constexpr int kStepCols = kStepThreads / 64;
 constexpr int kStepChunks = kEigMaxN / 64;   // 64-row chunks of a column per lane
 constexpr int kStepMinOrder = 129;           // smaller batches stay in one workgroup per matrix
-static_assert(kEigMaxN <= kStepThreads, "one row of the step vectors per thread");
+constexpr int kStepRows = kEigMaxN / kStepThreads;   // rows of the step vectors per thread
+static_assert(kStepRows * kStepThreads == kEigMaxN, "whole rows per thread");
+static_assert(kStepMinOrder - 1 <= kTriMaxN, "the one-workgroup form covers the smaller batches");
 
 // workgroups of a matrix of order n at step k
 __host__ __device__ inline int step_blocks(int n, int k) {
   return k + 1 < n ? (n - k - 1 + kStepCols - 1) / kStepCols : (k + 1 == n ? 1 : 0);
 }
 
-__global__ __launch_bounds__(kStepThreads) void tridiag_step_kernel(Batch batch, double *__restrict__ matrices,
+__global__ __launch_bounds__(kStepThreads) void tridiag_step_kernel(const Batch *__restrict__ batch, double *__restrict__ matrices,
                                                                     double *__restrict__ work, int k) {
   int b = 0, first = 0;
-  for (; b < batch.count; ++b) {
-    const int count = step_blocks(batch.p[b].n, k);
+  for (; b < batch->count; ++b) {
+    const int count = step_blocks(batch->p[b].n, k);
     if (static_cast<int>(blockIdx.x) < first + count) break;
     first += count;
   }
-  if (b == batch.count) return;
-  const Problem pr = batch.p[b];
+  if (b == batch->count) return;
+  const Problem pr = batch->p[b];
   const int n = pr.n, block = static_cast<int>(blockIdx.x) - first;
   double *__restrict__ A = matrices + pr.matrix;
   double *diag = work + pr.work + 3 * int64_t(n) * n, *off = diag + n, *tau = off + n;
@@ -266,15 +269,20 @@ __global__ __launch_bounds__(kStepThreads) void tridiag_step_kernel(Batch batch,
   __shared__ double vp[kEigMaxN], wp[kEigMaxN], vc[kEigMaxN];
   __shared__ double red[kStepThreads / 64];
   __shared__ double s_alpha, s_diag;
-  const int t = threadIdx.x, i = k + t;
-  const bool live = i < n;
-  // v and p of step k-1 (zero before the first step), and column k as it stands
-  double v = 0.0, p = 0.0, a = 0.0;
-  if (live) {
-    a = A[i + int64_t(k) * n];
-    if (k > 0) {
-      v = vbuf[((k - 1) & 1) * n + i];
-      p = pbuf[((k - 1) & 1) * n + i];
+  const int t = threadIdx.x;
+  // rows k + t + kStepThreads r of the step vectors: v and p of step k-1 (zero before the first
+  // step), and column k as it stands
+  double v[kStepRows], p[kStepRows], a[kStepRows];
+#pragma unroll
+  for (int r = 0; r < kStepRows; ++r) {
+    const int i = k + t + kStepThreads * r;
+    v[r] = p[r] = a[r] = 0.0;
+    if (i < n) {
+      a[r] = A[i + int64_t(k) * n];
+      if (k > 0) {
+        v[r] = vbuf[((k - 1) & 1) * n + i];
+        p[r] = pbuf[((k - 1) & 1) * n + i];
+      }
     }
   }
   const double tprev = k > 0 ? tau[k - 1] : 0.0;
@@ -287,20 +295,39 @@ __global__ __launch_bounds__(kStepThreads) void tridiag_step_kernel(Batch batch,
     const int r = k + 1 + lane + 64 * q;
     y[q] = (c < n && r < n) ? A[r + int64_t(c) * n] : 0.0;
   }
-  const double pv = tri_block_sum(p * v, red);
-  const double w = p - 0.5 * tprev * pv * v;
-  if (live) {
-    vp[i] = v;
-    wp[i] = w;
+  double partial = 0.0;
+#pragma unroll
+  for (int r = 0; r < kStepRows; ++r) partial += p[r] * v[r];
+  const double pv = tri_block_sum(partial, red);
+  double w[kStepRows];
+#pragma unroll
+  for (int r = 0; r < kStepRows; ++r) {
+    const int i = k + t + kStepThreads * r;
+    w[r] = p[r] - 0.5 * tprev * pv * v[r];
+    if (i < n) {
+      vp[i] = v[r];
+      wp[i] = w[r];
+    }
   }
   __syncthreads();
-  const double x = live ? a - (v * wp[k] + w * vp[k]) : 0.0;
-  if (i == k) s_diag = x;
-  if (i == k + 1) s_alpha = x;
-  const double sigma = tri_block_sum(live && i >= k + 2 ? x * x : 0.0, red);
+  double x[kStepRows];
+  partial = 0.0;
+#pragma unroll
+  for (int r = 0; r < kStepRows; ++r) {
+    const int i = k + t + kStepThreads * r;
+    x[r] = i < n ? a[r] - (v[r] * wp[k] + w[r] * vp[k]) : 0.0;
+    if (i == k) s_diag = x[r];
+    if (i == k + 1) s_alpha = x[r];
+    if (i < n && i >= k + 2) partial += x[r] * x[r];
+  }
+  const double sigma = tri_block_sum(partial, red);
   if (k + 1 == n) {
     if (t == 0) diag[k] = s_diag;
-    if (k > 0 && live) A[i + int64_t(k - 1) * n] = v;
+#pragma unroll
+    for (int r = 0; r < kStepRows; ++r) {
+      const int i = k + t + kStepThreads * r;
+      if (k > 0 && i < n) A[i + int64_t(k - 1) * n] = v[r];
+    }
     return;
   }
   const double alpha = s_alpha;
@@ -311,18 +338,22 @@ __global__ __launch_bounds__(kStepThreads) void tridiag_step_kernel(Batch batch,
     tk = (beta - alpha) / beta;
     scale = 1.0 / (alpha - beta);
   }
-  const double vnew = i == k + 1 ? 1.0 : (live && i >= k + 2 ? x * scale : 0.0);
-  if (live) vc[i] = vnew;
-  if (block == 0) {
-    if (t == 0) {
-      diag[k] = s_diag;
-      off[k] = beta;
-      tau[k] = tk;
+#pragma unroll
+  for (int r = 0; r < kStepRows; ++r) {
+    const int i = k + t + kStepThreads * r;
+    const double vnew = i == k + 1 ? 1.0 : (i < n && i >= k + 2 ? x[r] * scale : 0.0);
+    if (i < n) {
+      vc[i] = vnew;
+      if (block == 0) {
+        vbuf[(k & 1) * n + i] = vnew;
+        if (k > 0) A[i + int64_t(k - 1) * n] = v[r];
+      }
     }
-    if (live) {
-      vbuf[(k & 1) * n + i] = vnew;
-      if (k > 0) A[i + int64_t(k - 1) * n] = v;
-    }
+  }
+  if (block == 0 && t == 0) {
+    diag[k] = s_diag;
+    off[k] = beta;
+    tau[k] = tk;
   }
   __syncthreads();
   // this wave's column: step k-1's update on rows k+1.., and its entry of p
@@ -352,15 +383,15 @@ __global__ __launch_bounds__(kStepThreads) void tridiag_step_kernel(Batch batch,
 //     leaf's diagonal block of Q, whose other entries in the leaf's rows are zeroed (and the whole
 //     rows of the second buffer too).
 
-__global__ __launch_bounds__(kEigThreads) void leaf_kernel(Batch batch, double *__restrict__ values,
+__global__ __launch_bounds__(kEigThreads) void leaf_kernel(const Batch *__restrict__ batch, double *__restrict__ values,
                                                             double *__restrict__ work, int32_t leaves) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int leaf = static_cast<int>(blockIdx.x) * kEigWaves + wave;
   __shared__ double dl[kEigWaves][kLeaf], el[kEigWaves][kLeaf], Z[kEigWaves][kLeaf][kLeaf + 1];
   if (leaf >= leaves) return;
   int b = 0;
-  while (b + 1 < batch.count && batch.p[b + 1].first_leaf <= leaf) ++b;
-  const Problem pr = batch.p[b];
+  while (b + 1 < batch->count && batch->p[b + 1].first_leaf <= leaf) ++b;
+  const Problem pr = batch->p[b];
   const int n = pr.n;
   const int lo = (leaf - pr.first_leaf) * kLeaf, hi = min(lo + kLeaf, n), m = hi - lo;
   double *Q = work + pr.work, *Qt = Q + int64_t(n) * n;
@@ -546,11 +577,11 @@ __device__ int block_exclusive_scan(int value, int *scratch, int *total) {
 constexpr int kPerThread = kEigMaxN / kEigThreads;
 static_assert(kPerThread * kEigThreads == kEigMaxN, "four list elements per thread");
 
-__global__ __launch_bounds__(kEigThreads) void dc_deflate_kernel(Batch batch, double *__restrict__ values,
+__global__ __launch_bounds__(kEigThreads) void dc_deflate_kernel(const Batch *__restrict__ batch, double *__restrict__ values,
                                                                   double *__restrict__ work, int width, int parity) {
-  const UnitRef u = find_unit(batch, width, static_cast<int>(blockIdx.x));
+  const UnitRef u = find_unit(*batch, width, static_cast<int>(blockIdx.x));
   if (u.b < 0) return;
-  const Problem pr = batch.p[u.b];
+  const Problem pr = batch->p[u.b];
   const int n = pr.n, lo = u.lo;
   double *const base = work + pr.work;
   double *__restrict__ Q = base + (parity ? int64_t(n) * n : 0);
@@ -745,15 +776,15 @@ __device__ inline Secular secular_wave(const double (&d)[kPoleChunks], const dou
   return Secular{1.0 + rho * (psi + phi), rho * psi, rho * phi, rho * dpsi, rho * dphi};
 }
 
-__global__ __launch_bounds__(kEigThreads) void dc_secular_kernel(Batch batch, double *__restrict__ work, int width) {
+__global__ __launch_bounds__(kEigThreads) void dc_secular_kernel(const Batch *__restrict__ batch, double *__restrict__ work, int width) {
   int b = 0, first = 0;
-  for (; b < batch.count; ++b) {
-    const int blocks = dc_root_blocks(batch.p[b].n, width);
+  for (; b < batch->count; ++b) {
+    const int blocks = dc_root_blocks(batch->p[b].n, width);
     if (static_cast<int>(blockIdx.x) < first + blocks) break;
     first += blocks;
   }
-  if (b == batch.count) return;
-  const Problem pr = batch.p[b];
+  if (b == batch->count) return;
+  const Problem pr = batch->p[b];
   const int n = pr.n;
   const int units = dc_units(n, width), full = (2 * width + kEigWaves - 1) / kEigWaves;
   int local = static_cast<int>(blockIdx.x) - first, unit = local / full;
@@ -838,11 +869,11 @@ __global__ __launch_bounds__(kEigThreads) void dc_secular_kernel(Batch batch, do
   }
 }
 
-__global__ __launch_bounds__(kEigThreads) void dc_finish_kernel(Batch batch, double *__restrict__ values,
+__global__ __launch_bounds__(kEigThreads) void dc_finish_kernel(const Batch *__restrict__ batch, double *__restrict__ values,
                                                                  double *__restrict__ work, int width) {
-  const UnitRef u = find_unit(batch, width, static_cast<int>(blockIdx.x));
+  const UnitRef u = find_unit(*batch, width, static_cast<int>(blockIdx.x));
   if (u.b < 0) return;
-  const Problem pr = batch.p[u.b];
+  const Problem pr = batch->p[u.b];
   const int n = pr.n, lo = u.lo;
   const int mid = lo + width, hi = min(lo + 2 * width, n), s = hi - lo;
   if (mid >= n) return;   // a copy: its maps are written
@@ -935,16 +966,16 @@ __host__ __device__ inline int dc_tiles(int n, int width) {
   return (units - 1) * merge_tiles(2 * width) + merge_tiles(last);
 }
 
-__global__ __launch_bounds__(kEigThreads) void dc_gemm_kernel(Batch batch, double *__restrict__ work, int width,
+__global__ __launch_bounds__(kEigThreads) void dc_gemm_kernel(const Batch *__restrict__ batch, double *__restrict__ work, int width,
                                                                int parity) {
   int b = 0, first = 0;
-  for (; b < batch.count; ++b) {
-    const int tiles = dc_tiles(batch.p[b].n, width);
+  for (; b < batch->count; ++b) {
+    const int tiles = dc_tiles(batch->p[b].n, width);
     if (static_cast<int>(blockIdx.x) < first + tiles) break;
     first += tiles;
   }
-  if (b == batch.count) return;
-  const Problem pr = batch.p[b];
+  if (b == batch->count) return;
+  const Problem pr = batch->p[b];
   const int n = pr.n;
   double *const base = work + pr.work;
   const double *__restrict__ Q = base + (parity ? int64_t(n) * n : 0);
@@ -1038,26 +1069,28 @@ __global__ __launch_bounds__(kEigThreads) void dc_gemm_kernel(Batch batch, doubl
 //    H_j ... H_{j+m-1} = I - V T V^T (compact WY form, T upper triangular).  One workgroup per
 //    block forms T from the block's Gram matrix V^T V; then one workgroup per 16-column panel of Z
 //    keeps the panel in registers and applies the blocks last to first, each as W = V^T Y,
-//    W = T W, Y -= V W on FP64 MFMA.  Wave w holds the 16-row blocks w, w + 4, ... of the panel in
+//    W = T W, Y -= V W on FP64 MFMA.  Wave w of eight holds the 16-row blocks w, w + 8, ... of the panel in
 //    the MFMA result layout (lane l: rows 4 r + l / 16 of a block, column l % 16), which is also
 //    the layout of the 4 x 16 right-hand operand of W = V^T Y.  Eigenvector c is written as row c
 //    of the output.
 
 constexpr int kRefl = 32;
-constexpr int kRowBlocks = kEigMaxN / 64;   // 16-row blocks per wave
+constexpr int kBackThreads = 512;
+constexpr int kBackWaves = kBackThreads / 64;
+constexpr int kRowBlocks = kEigMaxN / (16 * kBackWaves);   // 16-row blocks per wave
 
 __host__ __device__ inline int reflector_blocks(int n) { return n > 1 ? (n - 1 + kRefl - 1) / kRefl : 0; }
 
-__global__ __launch_bounds__(kEigThreads) void block_t_kernel(Batch batch, const double *__restrict__ matrices,
+__global__ __launch_bounds__(kEigThreads) void block_t_kernel(const Batch *__restrict__ batch, const double *__restrict__ matrices,
                                                                double *__restrict__ work) {
   int b = 0, first = 0;
-  for (; b < batch.count; ++b) {
-    const int count = reflector_blocks(batch.p[b].n);
+  for (; b < batch->count; ++b) {
+    const int count = reflector_blocks(batch->p[b].n);
     if (static_cast<int>(blockIdx.x) < first + count) break;
     first += count;
   }
-  if (b == batch.count) return;
-  const Problem pr = batch.p[b];
+  if (b == batch->count) return;
+  const Problem pr = batch->p[b];
   const int n = pr.n, q = static_cast<int>(blockIdx.x) - first;
   const int j = q * kRefl, m = min(kRefl, n - 1 - j);
   const double *__restrict__ A = matrices + pr.matrix;
@@ -1106,41 +1139,48 @@ __global__ __launch_bounds__(kEigThreads) void block_t_kernel(Batch batch, const
   }
 }
 
-__global__ __launch_bounds__(kEigThreads) void back_kernel(Batch batch, const double *__restrict__ matrices,
+__global__ __launch_bounds__(kBackThreads) void back_kernel(const Batch *__restrict__ batch, const double *__restrict__ matrices,
                                                             double *__restrict__ vectors, const double *__restrict__ work) {
   // the problem this panel belongs to
   int b = 0;
-  while (b + 1 < batch.count && batch.p[b + 1].first_panel <= static_cast<int>(blockIdx.x)) ++b;
-  const Problem pr = batch.p[b];
+  while (b + 1 < batch->count && batch->p[b + 1].first_panel <= static_cast<int>(blockIdx.x)) ++b;
+  const Problem pr = batch->p[b];
   const int n = pr.n;
   const int c0 = (static_cast<int>(blockIdx.x) - pr.first_panel) * kPanel;
   const double *__restrict__ A = matrices + pr.matrix;
   const double *__restrict__ Z = work + pr.work + (dc_levels(n) & 1 ? int64_t(n) * n : 0);
   const double *__restrict__ Tall = work + pr.work + 3 * int64_t(n) * n + 7 * n;
-  __shared__ double Wp[kEigWaves][kRefl][kPanel + 1];
+  __shared__ double Wp[kBackWaves][kRefl][kPanel + 1];
   __shared__ double Wq[kRefl][kPanel + 1], Wr[kRefl][kPanel + 1];
   __shared__ double Ts[kRefl][kRefl + 1];
-  const int t = threadIdx.x, wave = t >> 6, lane = t & 63, lr = lane & 15, lk = lane >> 4;
-  const int col = c0 + lr;
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  const int col = c0 + (lane & 15);
   double y[kRowBlocks][4];
+  const int lk0 = lane >> 4;
 #pragma unroll
   for (int mb = 0; mb < kRowBlocks; ++mb) {
-    const int base_row = 16 * (4 * mb + wave);
+    const int base_row = 16 * (kBackWaves * mb + wave);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int row = base_row + 4 * r + lk;
+      const int row = base_row + 4 * r + lk0;
       y[mb][r] = (row < n && col < n) ? Z[row + int64_t(col) * n] : 0.0;
     }
+    __builtin_amdgcn_sched_barrier(0);
   }
   for (int q = reflector_blocks(n) - 1; q >= 0; --q) {
     const int j = q * kRefl, m = min(kRefl, n - 1 - j);
+    // the lane's row and column indices, recomputed in every block: kept live across the loop
+    // (the compiler's choice otherwise) they do not fit beside the panel and spill
+    int lane_q = lane;
+    asm volatile("" : "+v"(lane_q));
+    const int lr = lane_q & 15, lk = lane_q >> 4;
     __syncthreads();
-    for (int e = t; e < kRefl * kRefl; e += kEigThreads) Ts[e / kRefl][e % kRefl] = Tall[int64_t(q) * kRefl * kRefl + e];
+    for (int e = t; e < kRefl * kRefl; e += kBackThreads) Ts[e / kRefl][e % kRefl] = Tall[int64_t(q) * kRefl * kRefl + e];
     // W = V^T Y: this wave's rows, both halves of the block's reflectors
     d4 w0 = d4{0.0, 0.0, 0.0, 0.0}, w1 = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int mb = 0; mb < kRowBlocks; ++mb) {
-      const int base_row = 16 * (4 * mb + wave);
+      const int base_row = 16 * (kBackWaves * mb + wave);
       if (base_row + 16 <= j + 1 || base_row >= n) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -1150,6 +1190,8 @@ __global__ __launch_bounds__(kEigThreads) void back_kernel(Batch batch, const do
         w0 = __builtin_amdgcn_mfma_f64_16x16x4f64(v0, y[mb][r], w0, 0, 0, 0);
         w1 = __builtin_amdgcn_mfma_f64_16x16x4f64(v1, y[mb][r], w1, 0, 0, 0);
       }
+      // one row block's operands at a time: hoisting the next block's loads would spill
+      __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -1157,13 +1199,16 @@ __global__ __launch_bounds__(kEigThreads) void back_kernel(Batch batch, const do
       Wp[wave][16 + 4 * r + lk][lr] = w1[r];
     }
     __syncthreads();
-    for (int e = t; e < kRefl * kPanel; e += kEigThreads) {
+    for (int e = t; e < kRefl * kPanel; e += kBackThreads) {
       const int a = e / kPanel, c = e % kPanel;
-      Wq[a][c] = (Wp[0][a][c] + Wp[1][a][c]) + (Wp[2][a][c] + Wp[3][a][c]);
+      double sum = 0.0;
+#pragma unroll
+      for (int w = 0; w < kBackWaves; ++w) sum += Wp[w][a][c];
+      Wq[a][c] = sum;
     }
     __syncthreads();
     // W = T W (T upper triangular)
-    for (int e = t; e < kRefl * kPanel; e += kEigThreads) {
+    for (int e = t; e < kRefl * kPanel; e += kBackThreads) {
       const int a = e / kPanel, c = e % kPanel;
       double sum = 0.0;
       for (int i = a; i < kRefl; ++i) sum += Ts[a][i] * Wq[i][c];
@@ -1173,7 +1218,7 @@ __global__ __launch_bounds__(kEigThreads) void back_kernel(Batch batch, const do
     // Y -= V W
 #pragma unroll
     for (int mb = 0; mb < kRowBlocks; ++mb) {
-      const int base_row = 16 * (4 * mb + wave);
+      const int base_row = 16 * (kBackWaves * mb + wave);
       if (base_row + 16 <= j + 1 || base_row >= n) continue;
       d4 acc = d4{0.0, 0.0, 0.0, 0.0};
       const int i = base_row + lr;
@@ -1185,20 +1230,30 @@ __global__ __launch_bounds__(kEigThreads) void back_kernel(Batch batch, const do
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) y[mb][r] -= acc[r];
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
   double *out = vectors + pr.matrix;
   if (col < n)
 #pragma unroll
     for (int mb = 0; mb < kRowBlocks; ++mb) {
-      const int base_row = 16 * (4 * mb + wave);
+      const int base_row = 16 * (kBackWaves * mb + wave);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = base_row + 4 * r + lk;
+        const int row = base_row + 4 * r + lk0;
         if (row < n) out[int64_t(col) * n + row] = y[mb][r];
       }
+      __builtin_amdgcn_sched_barrier(0);
     }
 }
+
+// The batch table, written once per batch into the workspace, where every launch reads it (a
+// kernel argument of its size would be copied for each of the many launches of a batch).
+__global__ void store_batch_kernel(Batch batch, Batch *__restrict__ table) {
+  if (threadIdx.x == 0) *table = batch;
+}
+
+constexpr int64_t kTableDoubles = (sizeof(Batch) + 255) / 256 * 32;   // one table, 256-byte aligned
 
 }  // namespace
 
@@ -1209,7 +1264,7 @@ extern "C" int64_t tt_eigh_workspace(const tt_eigh_problem *problems, int64_t co
     fail("tt_eigh_workspace: bad problem list");
     return -1;
   }
-  int64_t total = 0;
+  int64_t total = kTableDoubles * ((count + kEigBatch - 1) / kEigBatch);
   for (int64_t b = 0; b < count; ++b) {
     const int64_t n = problems[b].n;
     total += 3 * n * n + 7 * n + (n >= 1 && n <= kEigMaxN ? int64_t(kRefl) * kRefl * reflector_blocks(static_cast<int>(n)) : 0);
@@ -1228,8 +1283,10 @@ extern "C" int tt_eigh_batched(const tt_eigh_problem *problems, int64_t count, d
       return fail("tt_eigh_batched: matrix order " + std::to_string(p.n) + " outside 1.." + std::to_string(kEigMaxN));
     if (p.matrix_offset < 0 || p.value_offset < 0) return fail("tt_eigh_batched: negative offset");
   }
-  int64_t work = 0;
+  // the batch tables first, then every problem's workspace
+  int64_t work = kTableDoubles * ((count + kEigBatch - 1) / kEigBatch);
   for (int64_t first = 0; first < count; first += kEigBatch) {
+    Batch *table = reinterpret_cast<Batch *>(workspace + kTableDoubles * (first / kEigBatch));
     Batch batch{};
     batch.count = static_cast<int32_t>(std::min<int64_t>(kEigBatch, count - first));
     int32_t panels = 0, leaves = 0;
@@ -1241,20 +1298,22 @@ extern "C" int tt_eigh_batched(const tt_eigh_problem *problems, int64_t count, d
       leaves += static_cast<int32_t>((p.n + kLeaf - 1) / kLeaf);
     }
     hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(store_batch_kernel, dim3(1), dim3(64), 0, s, batch, table);
+    TT_LAUNCHED();
     int64_t largest = 0;
     for (int32_t b = 0; b < batch.count; ++b) largest = std::max<int64_t>(largest, batch.p[b].n);
     if (largest >= kStepMinOrder) {
       for (int k = 0; k < largest; ++k) {
         int grid = 0;
         for (int32_t b = 0; b < batch.count; ++b) grid += step_blocks(batch.p[b].n, k);
-        hipLaunchKernelGGL(tridiag_step_kernel, dim3(grid), dim3(kStepThreads), 0, s, batch, matrices, workspace, k);
+        hipLaunchKernelGGL(tridiag_step_kernel, dim3(grid), dim3(kStepThreads), 0, s, table, matrices, workspace, k);
         TT_LAUNCHED();
       }
     } else {
-      hipLaunchKernelGGL(tridiag_kernel<1>, dim3(batch.count), dim3(kTriThreads), 0, s, batch, matrices, workspace);
+      hipLaunchKernelGGL(tridiag_kernel<1>, dim3(batch.count), dim3(kTriThreads), 0, s, table, matrices, workspace);
       TT_LAUNCHED();
     }
-    hipLaunchKernelGGL(leaf_kernel, dim3((leaves + kEigWaves - 1) / kEigWaves), dim3(kEigThreads), 0, s, batch, values,
+    hipLaunchKernelGGL(leaf_kernel, dim3((leaves + kEigWaves - 1) / kEigWaves), dim3(kEigThreads), 0, s, table, values,
                        workspace, leaves);
     TT_LAUNCHED();
     for (int width = kLeaf, level = 0; width < largest; width *= 2, ++level) {
@@ -1265,22 +1324,22 @@ extern "C" int tt_eigh_batched(const tt_eigh_problem *problems, int64_t count, d
       }
       int roots = 0;
       for (int32_t b = 0; b < batch.count; ++b) roots += dc_root_blocks(batch.p[b].n, width);
-      hipLaunchKernelGGL(dc_deflate_kernel, dim3(units), dim3(kEigThreads), 0, s, batch, values, workspace, width, level & 1);
+      hipLaunchKernelGGL(dc_deflate_kernel, dim3(units), dim3(kEigThreads), 0, s, table, values, workspace, width, level & 1);
       TT_LAUNCHED();
-      hipLaunchKernelGGL(dc_secular_kernel, dim3(roots), dim3(kEigThreads), 0, s, batch, workspace, width);
+      hipLaunchKernelGGL(dc_secular_kernel, dim3(roots), dim3(kEigThreads), 0, s, table, workspace, width);
       TT_LAUNCHED();
-      hipLaunchKernelGGL(dc_finish_kernel, dim3(units), dim3(kEigThreads), 0, s, batch, values, workspace, width);
+      hipLaunchKernelGGL(dc_finish_kernel, dim3(units), dim3(kEigThreads), 0, s, table, values, workspace, width);
       TT_LAUNCHED();
-      hipLaunchKernelGGL(dc_gemm_kernel, dim3(tiles), dim3(kEigThreads), 0, s, batch, workspace, width, level & 1);
+      hipLaunchKernelGGL(dc_gemm_kernel, dim3(tiles), dim3(kEigThreads), 0, s, table, workspace, width, level & 1);
       TT_LAUNCHED();
     }
     int blocks = 0;
     for (int32_t b = 0; b < batch.count; ++b) blocks += reflector_blocks(batch.p[b].n);
     if (blocks > 0) {
-      hipLaunchKernelGGL(block_t_kernel, dim3(blocks), dim3(kEigThreads), 0, s, batch, matrices, workspace);
+      hipLaunchKernelGGL(block_t_kernel, dim3(blocks), dim3(kEigThreads), 0, s, table, matrices, workspace);
       TT_LAUNCHED();
     }
-    hipLaunchKernelGGL(back_kernel, dim3(panels), dim3(kEigThreads), 0, s, batch, matrices, vectors, workspace);
+    hipLaunchKernelGGL(back_kernel, dim3(panels), dim3(kBackThreads), 0, s, table, matrices, vectors, workspace);
     TT_LAUNCHED();
   }
   return 0;

@@ -145,8 +145,21 @@ def svd(tensor: Tensor, left: Optional[Sequence[int]] = None, max_bond: Optional
             u, v, vh = torch.linalg.svd(block, full_matrices=False)
             factors.append({'key': key, 'u': u, 'v': v, 'vh': vh, 'kept': 0, 'offset': 0})
     host = torch.cat([f['v'] for f in factors]).double().cpu().numpy() if factors else np.zeros(0)
-    # global ranking, largest first; a stable sort keeps ties in (block, index) order
     sizes = [f['v'].numel() for f in factors]
+    # The ranking keeps a leading run of every block, which needs each block's values descending
+    # (svd.cpp:80).  torch.linalg.svd on ROCm can return values at the rounding level (~1e-14 of
+    # the largest) slightly out of order; such a block is re-sorted with its singular vectors.
+    if not gram:
+        at = 0
+        for f, size in zip(factors, sizes):
+            segment = host[at:at + size]
+            if size > 1 and np.any(segment[1:] > segment[:-1]):
+                order = np.argsort(-segment, kind='stable')
+                index = torch.as_tensor(order, device=f['v'].device)
+                f['u'], f['v'], f['vh'] = f['u'][:, index], f['v'][index], f['vh'][index]
+                host[at:at + size] = segment[order]
+            at += size
+    # global ranking, largest first; a stable sort keeps ties in (block, index) order
     owner = np.repeat(np.arange(len(factors)), sizes)
     position = np.concatenate([np.arange(n) for n in sizes]) if factors else np.zeros(0, dtype=np.int64)
     order = np.argsort(-host, kind='stable')

@@ -221,3 +221,33 @@ def test_split_by_the_device_eigensolver_matches_the_exact_svd():
         assert torch.allclose(s.sort().values, ref_s.sort().values, rtol=1e-10, atol=1e-13), key
         rec = tt.contract(u, v, tt.domain_legs(u), [0]).data
         assert (rec - ref_rec).abs().max() <= 1e-12 * ref_rec.abs().max(), key
+
+
+def test_exact_split_reorders_unsorted_singular_values(monkeypatch):
+    '''torch.linalg.svd on ROCm can return a block's smallest singular values slightly out of order;
+    the split re-sorts every block with its singular vectors before ranking (the ranking keeps a
+    leading run of each block).  A per-block SVD whose values come back shuffled gives the same
+    values and the same reconstruction as the sorted one.'''
+    from tentorch_amd.linalg import SvdSettings
+    rng = np.random.default_rng(37)
+    bond = tt.Leg(tt.u1, [(-2,), (0,), (2,)], [9, 14, 9])
+    phys = tt.Leg(tt.u1, [(-1,), (1,)], [1, 1])
+    two = tt.Tensor.random(tt.u1, [bond, phys], [phys, bond], (0,), generator=rng)
+    SvdSettings.method = 'exact'
+    try:
+        ref_u, ref_s, ref_v = tt.svd(two, None, 20, 1e-14, 2)
+        original = torch.linalg.svd
+        shuffle = torch.Generator().manual_seed(5)
+
+        def shuffled(block, full_matrices=False):
+            u, s, vh = original(block, full_matrices=full_matrices)
+            order = torch.randperm(s.numel(), generator=shuffle).to(s.device)
+            return u[:, order], s[order], vh[order]
+        monkeypatch.setattr(torch.linalg, 'svd', shuffled)
+        u, s, v = tt.svd(two, None, 20, 1e-14, 2)
+    finally:
+        SvdSettings.method = 'gram'
+    assert torch.equal(s, ref_s)
+    rec = tt.contract(u, v, tt.domain_legs(u), [0]).data
+    ref_rec = tt.contract(ref_u, ref_v, tt.domain_legs(ref_u), [0]).data
+    assert (rec - ref_rec).abs().max() <= 1e-12 * ref_rec.abs().max()
