@@ -152,17 +152,19 @@ int tt_graph_destroy(tt_graph *graph);
 
 /* Batched symmetric eigendecomposition in FP64 (the split of the two-site tensor: the Gram matrix
  * of each block).  Replaces the per-block `at::linalg_svd` of `split_over`
- * (tentorch/linalg/graded/svd.cpp:41-177) with one tridiagonalisation, two divide-and-conquer
- * (leaves, merges) and one back-transformation launch for all blocks.  Matrix b is the symmetric n x n at
+ * (tentorch/linalg/graded/svd.cpp:41-177) with one eigendecomposition of all blocks at once:
+ * tridiagonalisation (one launch per Householder step when a matrix exceeds order 128),
+ * divide-and-conquer leaves and merge levels, blocked back-transformation; stream-ordered, no host
+ * synchronisation, no allocation.  Matrix b is the symmetric n x n at
  * `matrices + matrix_offset` (overwritten: it keeps the Householder reflectors); its eigenvalues go
  * to `values + value_offset` in ascending order, and eigenvector j to row j of the n x n at
- * `vectors + matrix_offset`.  n is 1..1024.  `workspace` holds tt_eigh_workspace(...) doubles. */
+ * `vectors + matrix_offset`.  n is 1..2048.  `workspace` holds tt_eigh_workspace(...) doubles. */
 typedef struct tt_eigh_problem {
   int64_t matrix_offset;
   int64_t value_offset;
   int64_t n;
 } tt_eigh_problem;
-/* The largest matrix order tt_eigh_batched takes (1024). */
+/* The largest matrix order tt_eigh_batched takes (2048). */
 int64_t tt_eigh_max_order(void);
 int64_t tt_eigh_workspace(const tt_eigh_problem *problems, int64_t count);
 int tt_eigh_batched(const tt_eigh_problem *problems, int64_t count, double *matrices, double *values,

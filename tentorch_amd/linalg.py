@@ -39,8 +39,8 @@ class SvdSettings:
     isometric factor from a batched symmetric eigendecomposition of each block's Gram matrix and the
     other factor as the exact projection of the block on it; 'exact' runs one SVD per block.
     `eigh` picks the Gram eigensolver: 'device' (this library's batched tridiagonalisation, divide
-    and conquer and back-transformation, four launches for all blocks) or 'torch'
-    (`torch.linalg.eigh`, batched by size class).'''
+    and conquer and back-transformation for all blocks at once, orders up to 2048; larger Gram
+    matrices go to torch) or 'torch' (`torch.linalg.eigh`, batched by size class).'''
     method = 'gram'
     eigh = 'device'
 
