@@ -5,6 +5,7 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <initializer_list>
 #include <string>
 
 #include "../../../include/tentorch_amd.h"
@@ -40,6 +41,16 @@ int launch(Kernel kernel, dim3 grid, dim3 block, uint32_t lds, hipStream_t strea
 // buffer on the library's own non-blocking upload stream, waiting for that copy alone: a plan
 // upload neither drains nor waits for the work queued on the caller's streams.
 int upload(void **where, const void *data, size_t bytes);
+
+// Several host tables copied into one device allocation (each at a 256-byte aligned offset) with
+// one allocation, one copy and one synchronisation of the upload stream: *block receives the
+// allocation (to be freed as a whole), each *table.where its table's address (null when empty).
+struct Table {
+  void **where;
+  const void *data;
+  size_t bytes;
+};
+int upload_tables(void **block, std::initializer_list<Table> tables);
 
 inline hipStream_t as_stream(void *stream) { return reinterpret_cast<hipStream_t>(stream); }
 

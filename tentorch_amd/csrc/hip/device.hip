@@ -154,6 +154,7 @@ void division_constants(uint32_t d, uint64_t *magic, uint32_t *shift) {
 }  // namespace
 
 struct tt_gss_plan {
+  void *block = nullptr;   // the one device allocation holding the three tables
   DevTask *tasks = nullptr;
   tt_gss_source *sources = nullptr;
   GssChunk *chunks = nullptr;
@@ -219,10 +220,9 @@ extern "C" int tt_gss_plan_create(const tt_gss_task *tasks, int64_t num_tasks, c
   p->written = written;
   p->read = read;
   p->groups = std::move(groups);
-  using ttdev::upload;
-  if (upload(reinterpret_cast<void **>(&p->tasks), dev.data(), sizeof(DevTask) * dev.size()) ||
-      upload(reinterpret_cast<void **>(&p->sources), sources, sizeof(tt_gss_source) * num_sources) ||
-      upload(reinterpret_cast<void **>(&p->chunks), chunks.data(), sizeof(GssChunk) * chunks.size())) {
+  if (ttdev::upload_tables(&p->block, {{reinterpret_cast<void **>(&p->tasks), dev.data(), sizeof(DevTask) * dev.size()},
+                                        {reinterpret_cast<void **>(&p->sources), sources, sizeof(tt_gss_source) * num_sources},
+                                        {reinterpret_cast<void **>(&p->chunks), chunks.data(), sizeof(GssChunk) * chunks.size()}})) {
     std::string keep = ttdev::g_error;
     tt_gss_plan_destroy(p);
     return fail(keep);
@@ -233,9 +233,7 @@ extern "C" int tt_gss_plan_create(const tt_gss_task *tasks, int64_t num_tasks, c
 
 extern "C" int tt_gss_plan_destroy(tt_gss_plan *plan) {
   if (!plan) return 0;
-  if (plan->tasks) (void)hipFree(plan->tasks);
-  if (plan->sources) (void)hipFree(plan->sources);
-  if (plan->chunks) (void)hipFree(plan->chunks);
+  if (plan->block) (void)hipFree(plan->block);
   delete plan;
   return 0;
 }
@@ -670,6 +668,32 @@ int ttdev::upload(void **where, const void *data, size_t bytes) {
   std::memcpy(u.staging, data, bytes);
   TT_HIP(hipMemcpyAsync(*where, u.staging, bytes, hipMemcpyHostToDevice, u.stream));
   TT_HIP(hipStreamSynchronize(u.stream));
+  return 0;
+}
+
+int ttdev::upload_tables(void **block, std::initializer_list<Table> tables) {
+  *block = nullptr;
+  size_t total = 0;
+  for (const Table &t : tables) {
+    *t.where = nullptr;
+    total = (total + 255) / 256 * 256 + t.bytes;
+  }
+  if (!total) return 0;
+  std::vector<unsigned char> packed(total);
+  size_t at = 0;
+  std::vector<size_t> offsets;
+  for (const Table &t : tables) {
+    at = (at + 255) / 256 * 256;
+    offsets.push_back(at);
+    if (t.bytes) std::memcpy(packed.data() + at, t.data, t.bytes);
+    at += t.bytes;
+  }
+  if (upload(block, packed.data(), total)) return -1;
+  size_t k = 0;
+  for (const Table &t : tables) {
+    if (t.bytes) *t.where = static_cast<unsigned char *>(*block) + offsets[k];
+    ++k;
+  }
   return 0;
 }
 

@@ -4,16 +4,16 @@
 //
 //   1. Householder tridiagonalisation.  The rank-2 update of step k is deferred into step k+1's
 //      symmetric matrix-vector pass, so each step reads and writes the trailing matrix once.  The
-//      reflectors stay in the matrix's lower part.  Batches of small matrices run one workgroup per
-//      matrix in a single launch; a batch with a matrix of order kStepMinOrder or more runs one
-//      launch per step, with every matrix spread over one workgroup per 16 trailing columns.
+//      reflectors stay in the matrix's lower part.  While a trailing matrix exceeds 128 rows, one
+//      launch per step spreads every matrix over one workgroup per 16 trailing columns; the last
+//      128 steps of every matrix run in one launch, one workgroup per matrix, in LDS.
 //   2. Cuppen divide and conquer on the tridiagonal: leaves of 16 rows by implicit QL (one wave
-//      each), then merges bottom up, two launches per level for every matrix of the batch.  One
-//      workgroup per merge deflates small coupling components and close poles (Givens rotations),
-//      finds the secular roots by rational ("middle way") steps from the nearer pole (one thread
-//      per root) and their eigenvectors by the Gu-Eisenstat formula (orthogonal to working
-//      precision however close the roots); then the merge products run as 64 x 64 FP64 MFMA
-//      tiles spread over the GPU.
+//      each), then merges bottom up, four launches per level for every matrix of the batch: one
+//      workgroup per merge deflates small coupling components and close poles (Givens rotations);
+//      one wave per secular root takes rational ("middle way") steps from the nearer pole; one
+//      workgroup per merge forms the eigenvectors of the rank-one update by the Gu-Eisenstat
+//      formula (orthogonal to working precision however close the roots); the merge products run
+//      as 64 x 64 FP64 MFMA tiles spread over the GPU.
 //   3. Back-transformation of the eigenvectors by the reflectors in blocks of 32 (compact WY
 //      form), one workgroup per 16-column panel held in registers, on FP64 MFMA.
 //
@@ -91,16 +91,12 @@ __device__ double wave_sum(double v) {
 
 // ---------------------------------------------------------------------------------------------
 // 1. Tridiagonalisation.  A is column-major n x n (symmetric, so either order), in place.
-//    Output: diag[n], off[n-1], tau[n-1]; reflector k is A[k+1.., k] with A[k+1, k] = 1.
-//    1024 threads: in the trailing pass thread t takes rows k+1 + t % 256 (+ 256 r) of every fourth
-//    column (group t / 256), eight columns at a time (four with four rows), so each thread keeps 16
-//    loads in flight;
-//    the four groups' partial products are summed in LDS in group order.  kTriRows (rows per thread)
-//    must cover the batch's largest matrix; it is launched with one (orders up to 128).
+//    Output: diag[n], off[n-1], tau[n-1]; reflector k is A[k+1.., k] with A[k+1, k] = 1.  The
+//    rank-2 update of step k is deferred into step k+1's pass over the trailing matrix.  While a
+//    trailing matrix exceeds kLdsOrder rows the steps run one launch each over many workgroups
+//    (1'); the last kLdsOrder steps of every matrix run in one launch in LDS (1'').
 
 constexpr int kTriThreads = 1024;
-constexpr int kTriGroups = kTriThreads / 256;
-constexpr int kTriMaxN = 256;   // orders the one-workgroup form takes (one row per thread of a group)
 
 __device__ double tri_block_sum(double v, double *scratch) {
   v = wave_sum(v);
@@ -112,123 +108,7 @@ __device__ double tri_block_sum(double v, double *scratch) {
   return total;
 }
 
-template <int kTriRows>
-__global__ __launch_bounds__(kTriThreads) void tridiag_kernel(const Batch *__restrict__ batch, double *__restrict__ matrices,
-                                                               double *__restrict__ work) {
-  constexpr int kTriUnroll = kTriRows == 4 ? 4 : 8;   // columns per thread per batch of loads
-  const Problem pr = batch->p[blockIdx.x];
-  const int n = pr.n;
-  double *__restrict__ A = matrices + pr.matrix;
-  double *diag = work + pr.work + 3 * int64_t(n) * n, *off = diag + n, *tau = off + n;
-  // (v, w) of the previous step and of this one, swapped by index after every step
-  __shared__ double vbuf[2][kTriMaxN], wbuf[2][kTriMaxN];
-  __shared__ double part[kTriGroups][kTriMaxN];
-  __shared__ double red[kTriThreads / 64];
-  __shared__ double s_alpha, s_diag;
-  const int t = threadIdx.x, row0 = t & 255, group = t >> 8;
-  for (int i = t; i < n; i += kTriThreads) vbuf[0][i] = wbuf[0][i] = 0.0;
-  __syncthreads();
-  int cur = 0;
-  for (int k = 0; k + 1 < n; ++k) {
-    const double *vp = vbuf[cur], *wp = wbuf[cur];
-    double *vc = vbuf[cur ^ 1], *wc = wbuf[cur ^ 1];
-    double *col = A + int64_t(k) * n;
-    // (a) column k, rows k.., brought up to date with the previous step's update
-    double sigma = 0.0;
-    for (int i = k + t; i < n; i += kTriThreads) {
-      const double a = col[i] - (vp[i] * wp[k] + wp[i] * vp[k]);
-      if (i == k) s_diag = a;
-      if (i == k + 1) s_alpha = a;
-      if (i >= k + 2) {
-        col[i] = a;
-        sigma += a * a;
-      }
-    }
-    sigma = tri_block_sum(sigma, red);
-    // every thread forms the reflector's scalars from the same values
-    const double alpha = s_alpha;
-    double beta = alpha, tk = 0.0, scale = 0.0;
-    if (sigma != 0.0) {
-      const double norm = sqrt(alpha * alpha + sigma);
-      beta = alpha >= 0.0 ? -norm : norm;
-      tk = (beta - alpha) / beta;
-      scale = 1.0 / (alpha - beta);
-    }
-    if (t == 0) {
-      diag[k] = s_diag;
-      off[k] = beta;
-      tau[k] = tk;
-      col[k + 1] = 1.0;
-    }
-    for (int i = k + 1 + t; i < n; i += kTriThreads) {
-      double v = 1.0;
-      if (i >= k + 2) {
-        v = col[i] * scale;
-        col[i] = v;
-      }
-      vc[i] = v;
-    }
-    __syncthreads();
-    // (b) one pass over the trailing matrix: apply the previous update, form p = tau A v
-    double p[kTriRows], vpi[kTriRows], wpi[kTriRows];
-#pragma unroll
-    for (int r = 0; r < kTriRows; ++r) {
-      const int i = k + 1 + row0 + r * 256;
-      p[r] = 0.0;
-      vpi[r] = i < n ? vp[i] : 0.0;
-      wpi[r] = i < n ? wp[i] : 0.0;
-    }
-    for (int j0 = k + 1 + group; j0 < n; j0 += kTriUnroll * kTriGroups) {
-      double a[kTriUnroll][kTriRows];
-#pragma unroll
-      for (int u = 0; u < kTriUnroll; ++u) {
-        const int j = j0 + u * kTriGroups;
-#pragma unroll
-        for (int r = 0; r < kTriRows; ++r) {
-          const int i = k + 1 + row0 + r * 256;
-          a[u][r] = (j < n && i < n) ? A[i + int64_t(j) * n] : 0.0;
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < kTriUnroll; ++u) {
-        const int j = j0 + u * kTriGroups;
-        if (j >= n) continue;
-        const double vpj = vp[j], wpj = wp[j], vcj = vc[j];
-#pragma unroll
-        for (int r = 0; r < kTriRows; ++r) {
-          const int i = k + 1 + row0 + r * 256;
-          if (i < n) {
-            const double x = a[u][r] - (vpi[r] * wpj + wpi[r] * vpj);
-            A[i + int64_t(j) * n] = x;
-            p[r] += x * vcj;
-          }
-        }
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < kTriRows; ++r) {
-      const int i = k + 1 + row0 + r * 256;
-      if (i < n) part[group][i] = p[r];
-    }
-    __syncthreads();
-    double pv = 0.0;
-    for (int i = k + 1 + t; i < n; i += kTriThreads) {
-      double total = 0.0;
-      for (int g = 0; g < kTriGroups; ++g) total += part[g][i];
-      total *= tk;
-      wc[i] = total;   // p, for now
-      pv += total * vc[i];
-    }
-    pv = tri_block_sum(pv, red);
-    for (int i = k + 1 + t; i < n; i += kTriThreads) wc[i] = wc[i] - 0.5 * tk * pv * vc[i];
-    __syncthreads();
-    cur ^= 1;
-  }
-  if (t == 0) diag[n - 1] = A[int64_t(n - 1) * n + n - 1] - 2.0 * vbuf[cur][n - 1] * wbuf[cur][n - 1];
-}
-
-// 1'. The same reduction in steps, for batches with a matrix above kStepMinOrder: one launch per
-//     Householder step k, one workgroup per kStepCols trailing columns of every matrix (one column
+// 1'. The steps while trailing matrices are large: one launch per Householder step k, one workgroup per kStepCols trailing columns of every matrix (one column
 //     per wave), so a large matrix is spread over many CUs instead of one.  Every workgroup of a
 //     matrix redoes the step's O(n) part itself: w of step k-1 from that step's p and v, column k
 //     with step k-1's update, the reflector of step k.  Then each wave applies step k-1's update
@@ -242,10 +122,8 @@ __global__ __launch_bounds__(kTriThreads) void tridiag_kernel(const Batch *__res
 constexpr int kStepThreads = 1024;
 constexpr int kStepCols = kStepThreads / 64;
 constexpr int kStepChunks = kEigMaxN / 64;   // 64-row chunks of a column per lane
-constexpr int kStepMinOrder = 129;           // smaller batches stay in one workgroup per matrix
 constexpr int kStepRows = kEigMaxN / kStepThreads;   // rows of the step vectors per thread
 static_assert(kStepRows * kStepThreads == kEigMaxN, "whole rows per thread");
-static_assert(kStepMinOrder - 1 <= kTriMaxN, "the one-workgroup form covers the smaller batches");
 
 // workgroups of a matrix of order n at step k
 __host__ __device__ inline int step_blocks(int n, int k) {
@@ -372,6 +250,105 @@ __global__ __launch_bounds__(kStepThreads) void tridiag_step_kernel(const Batch 
   }
   acc = wave_sum(acc);
   if (lane == 0) pbuf[(k & 1) * n + c] = tk * acc;
+}
+
+// 1''. The last kLdsOrder steps of every matrix in one launch, one workgroup per matrix: the
+//      trailing matrix (order <= kLdsOrder, rows padded to kLdsStride against bank conflicts) is
+//      loaded into LDS with the state the stepped launches left (v, p and tau of step k0-1, whose
+//      rank-2 update is still pending), and the remaining steps run there with no memory traffic
+//      but the reflectors and scalars they produce.  In the trailing pass eight lanes share a row
+//      and sum their partial products by shuffles.  With k0 = 0 it is the whole reduction of a
+//      batch of small matrices.
+
+constexpr int kLdsOrder = 128;
+constexpr int kLdsStride = kLdsOrder + 1;
+static_assert(kLdsOrder * 8 == kTriThreads, "eight lanes per trailing row");
+
+__global__ __launch_bounds__(kTriThreads) void tridiag_lds_kernel(const Batch *__restrict__ batch,
+                                                                   double *__restrict__ matrices,
+                                                                   double *__restrict__ work, int k0) {
+  const Problem pr = batch->p[blockIdx.x];
+  const int n = pr.n;
+  if (n - 1 < k0) return;   // finished by the stepped launches
+  const int m = n - k0;     // at most kLdsOrder (the host starts this launch late enough)
+  double *__restrict__ A = matrices + pr.matrix;
+  double *diag = work + pr.work + 3 * int64_t(n) * n, *off = diag + n, *tau = off + n;
+  const double *vbuf = tau + n, *pbuf = vbuf + 2 * n;
+  __shared__ double T[kLdsOrder * kLdsStride];
+  __shared__ double vp[kLdsOrder], wp[kLdsOrder], vc[kLdsOrder], pc[kLdsOrder];
+  __shared__ double red[kTriThreads / 64];
+  __shared__ double s_alpha, s_diag;
+  const int t = threadIdx.x;
+  for (int e = t; e < m * m; e += kTriThreads) {
+    const int i = e % m, j = e / m;
+    T[i * kLdsStride + j] = A[(k0 + i) + int64_t(k0 + j) * n];
+  }
+  // the pending update of step k0-1 (none before the first step); its reflector goes to column k0-1
+  double v = 0.0, p = 0.0;
+  if (t < m && k0 > 0) {
+    v = vbuf[((k0 - 1) & 1) * n + k0 + t];
+    p = pbuf[((k0 - 1) & 1) * n + k0 + t];
+    A[(k0 + t) + int64_t(k0 - 1) * n] = v;
+  }
+  const double tprev = k0 > 0 ? tau[k0 - 1] : 0.0;
+  const double pv0 = tri_block_sum(p * v, red);
+  if (t < m) {
+    vp[t] = v;
+    wp[t] = p - 0.5 * tprev * pv0 * v;
+  }
+  __syncthreads();
+  for (int kk = 0; kk + 1 < m; ++kk) {
+    const int k = k0 + kk;
+    // (a) column kk (rows kk..) with the pending update, and the reflector of this step
+    double x = 0.0;
+    if (t >= kk && t < m) x = T[t * kLdsStride + kk] - (vp[t] * wp[kk] + wp[t] * vp[kk]);
+    if (t == kk) s_diag = x;
+    if (t == kk + 1) s_alpha = x;
+    const double sigma = tri_block_sum(t >= kk + 2 && t < m ? x * x : 0.0, red);
+    const double alpha = s_alpha;
+    double beta = alpha, tk = 0.0, scale = 0.0;
+    if (sigma != 0.0) {
+      const double norm = sqrt(alpha * alpha + sigma);
+      beta = alpha >= 0.0 ? -norm : norm;
+      tk = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    if (t < m) {
+      const double vnew = t == kk + 1 ? 1.0 : (t >= kk + 2 ? x * scale : 0.0);
+      vc[t] = vnew;
+      if (t >= kk + 1) A[(k0 + t) + int64_t(k) * n] = vnew;
+    }
+    if (t == 0) {
+      diag[k] = s_diag;
+      off[k] = beta;
+      tau[k] = tk;
+    }
+    __syncthreads();
+    // (b) rows and columns kk+1..: the pending update, and p = tau T v
+    const int row = kk + 1 + (t >> 3), sub = t & 7;
+    double acc = 0.0;
+    if (row < m) {
+      const double vpr = vp[row], wpr = wp[row];
+      for (int j = kk + 1 + sub; j < m; j += 8) {
+        const double y = T[row * kLdsStride + j] - (vpr * wp[j] + wpr * vp[j]);
+        T[row * kLdsStride + j] = y;
+        acc += y * vc[j];
+      }
+    }
+    acc += __shfl_xor(acc, 1, 64);
+    acc += __shfl_xor(acc, 2, 64);
+    acc += __shfl_xor(acc, 4, 64);
+    if (sub == 0 && row < m) pc[row] = tk * acc;
+    __syncthreads();
+    // (c) w = p - tau/2 (p . v) v becomes the pending update
+    const double pv = tri_block_sum(t >= kk + 1 && t < m ? pc[t] * vc[t] : 0.0, red);
+    if (t >= kk + 1 && t < m) {
+      wp[t] = pc[t] - 0.5 * tk * pv * vc[t];
+      vp[t] = vc[t];
+    }
+    __syncthreads();
+  }
+  if (t == 0) diag[n - 1] = T[(m - 1) * kLdsStride + (m - 1)] - 2.0 * vp[m - 1] * wp[m - 1];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1302,17 +1279,18 @@ extern "C" int tt_eigh_batched(const tt_eigh_problem *problems, int64_t count, d
     TT_LAUNCHED();
     int64_t largest = 0;
     for (int32_t b = 0; b < batch.count; ++b) largest = std::max<int64_t>(largest, batch.p[b].n);
-    if (largest >= kStepMinOrder) {
-      for (int k = 0; k < largest; ++k) {
-        int grid = 0;
-        for (int32_t b = 0; b < batch.count; ++b) grid += step_blocks(batch.p[b].n, k);
-        hipLaunchKernelGGL(tridiag_step_kernel, dim3(grid), dim3(kStepThreads), 0, s, table, matrices, workspace, k);
-        TT_LAUNCHED();
-      }
-    } else {
-      hipLaunchKernelGGL(tridiag_kernel<1>, dim3(batch.count), dim3(kTriThreads), 0, s, table, matrices, workspace);
+    // steps over many workgroups while the largest trailing matrix exceeds kLdsOrder rows, then
+    // the rest of every matrix in LDS
+    const int lds_from = static_cast<int>(std::max<int64_t>(0, largest - kLdsOrder));
+    for (int k = 0; k < lds_from; ++k) {
+      int grid = 0;
+      for (int32_t b = 0; b < batch.count; ++b) grid += step_blocks(batch.p[b].n, k);
+      hipLaunchKernelGGL(tridiag_step_kernel, dim3(grid), dim3(kStepThreads), 0, s, table, matrices, workspace, k);
       TT_LAUNCHED();
     }
+    hipLaunchKernelGGL(tridiag_lds_kernel, dim3(batch.count), dim3(kTriThreads), 0, s, table, matrices, workspace,
+                       lds_from);
+    TT_LAUNCHED();
     hipLaunchKernelGGL(leaf_kernel, dim3((leaves + kEigWaves - 1) / kEigWaves), dim3(kEigThreads), 0, s, table, values,
                        workspace, leaves);
     TT_LAUNCHED();

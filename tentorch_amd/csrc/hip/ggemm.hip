@@ -312,6 +312,7 @@ int choose_shape(const tt_gemm_group *groups, int64_t num_groups, const tt_gemm_
 }  // namespace
 
 struct tt_ggemm_plan {
+  void *block = nullptr;   // the one device allocation holding the three tables
   tt_gemm_group *groups = nullptr;
   tt_gemm_term *terms = nullptr;
   GemmTile *tiles = nullptr;
@@ -360,10 +361,9 @@ extern "C" int tt_ggemm_plan_create_config(const tt_gemm_group *groups, int64_t 
   p->config = config;
   p->flops = flops;
   p->host_groups.assign(groups, groups + num_groups);
-  using ttdev::upload;
-  if (upload(reinterpret_cast<void **>(&p->groups), groups, sizeof(tt_gemm_group) * num_groups) ||
-      upload(reinterpret_cast<void **>(&p->terms), terms, sizeof(tt_gemm_term) * num_terms) ||
-      upload(reinterpret_cast<void **>(&p->tiles), tiles.data(), sizeof(GemmTile) * tiles.size())) {
+  if (ttdev::upload_tables(&p->block, {{reinterpret_cast<void **>(&p->groups), groups, sizeof(tt_gemm_group) * num_groups},
+                                        {reinterpret_cast<void **>(&p->terms), terms, sizeof(tt_gemm_term) * num_terms},
+                                        {reinterpret_cast<void **>(&p->tiles), tiles.data(), sizeof(GemmTile) * tiles.size()}})) {
     char keep[1024];
     tt_last_error(keep, sizeof(keep));
     tt_ggemm_plan_destroy(p);
@@ -380,9 +380,7 @@ extern "C" int tt_ggemm_plan_create(const tt_gemm_group *groups, int64_t num_gro
 
 extern "C" int tt_ggemm_plan_destroy(tt_ggemm_plan *plan) {
   if (!plan) return 0;
-  if (plan->groups) (void)hipFree(plan->groups);
-  if (plan->terms) (void)hipFree(plan->terms);
-  if (plan->tiles) (void)hipFree(plan->tiles);
+  if (plan->block) (void)hipFree(plan->block);
   delete plan;
   return 0;
 }

@@ -266,20 +266,21 @@ def _eigh_cases(rng):
     return cases
 
 
-@pytest.mark.parametrize('largest', [2048, 128])
+@pytest.mark.parametrize('largest', [2048, 300, 128])
 def test_batched_eigh_matches_numpy(largest):
     '''tt_eigh_batched on one batch of mixed sizes: eigenvalues within 1e-12 of numpy's (relative to
     the matrix scale), eigenvectors orthonormal to 1e-12 with residuals below 1e-12 of the scale.
-    A batch reaching order 2048 (the largest taken) is reduced in one launch per step over many
-    workgroups; one with nothing above 128 in one workgroup per matrix.'''
+    The tridiagonalisation runs one launch per step over many workgroups until the largest trailing
+    matrix has 128 rows, then finishes every matrix in LDS: a batch reaching order 2048 (the largest
+    taken), one reaching 300 (matrices of order 200 and 130 hand over part-way), and one with
+    nothing above 128 (all in LDS).'''
     from tentorch_amd import _native
     assert _native.EIGH_MAX_ORDER == 2048
     rng = np.random.default_rng(21)
     cases = [c for c in _eigh_cases(rng) if c.shape[0] <= largest]
-    if largest > 700:
-        for n in (1024, largest):
-            a = rng.standard_normal((n, n))
-            cases.append(a + a.T)
+    for n in {2048: (1024, 2048), 300: (300,), 128: ()}[largest]:
+        a = rng.standard_normal((n, n))
+        cases.append(a + a.T)
     packed = torch.tensor(np.concatenate([c.reshape(-1) for c in cases]), device='cuda')
     # a workspace full of garbage: nothing may depend on its contents
     size = _native.eigh_workspace([c.shape[0] for c in cases])
