@@ -241,9 +241,10 @@ def main() -> int:
     if 'gemm' in summary and summary['gemm']['ms'] > 0:
         g = summary['gemm']
         achieved = g['flops'] / (g['ms'] * 1e-3) / 1e12
-        kernel = ('ggemm_kernel<float,float,float,float>' if compute == 'f32' else
-                  'ggemm_kernel<double,double,double,double>')
-        roofline = {'bound': 'mfma', 'kernel': f'{kernel} (matvec block products)',
+        kernel = ('ggemm_kernel<float,double,float,float> and <float,float,double,float> (the matvec\'s two '
+                  'FP32-MFMA block-product launches: FP64 operand converted on load, FP64 result)'
+                  if compute == 'f32' else 'ggemm_kernel<double,double,double,double> (matvec block products)')
+        roofline = {'bound': 'mfma', 'kernel': kernel,
                     'achieved': round(achieved, 3), 'peak': peak, 'unit': 'TFLOP/s',
                     'frac': round(achieved / peak, 4), 'traffic': None,
                     'launches': g['launches'], 'avg_launch_us': round(g['ms'] * 1e3 / g['launches'], 2),
