@@ -1117,7 +1117,8 @@ __global__ __launch_bounds__(kEigThreads) void block_t_kernel(const Batch *__res
 }
 
 __global__ __launch_bounds__(kBackThreads) void back_kernel(const Batch *__restrict__ batch, const double *__restrict__ matrices,
-                                                            double *__restrict__ vectors, const double *__restrict__ work) {
+                                                            double *__restrict__ vectors, const double *__restrict__ work,
+                                                            int descending) {
   // the problem this panel belongs to
   int b = 0;
   while (b + 1 < batch->count && batch->p[b + 1].first_panel <= static_cast<int>(blockIdx.x)) ++b;
@@ -1218,10 +1219,28 @@ __global__ __launch_bounds__(kBackThreads) void back_kernel(const Batch *__restr
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = base_row + 4 * r + lk0;
-        if (row < n) out[int64_t(col) * n + row] = y[mb][r];
+        if (row < n) {
+          if (descending)
+            out[int64_t(row) * n + (n - 1 - col)] = y[mb][r];
+          else
+            out[int64_t(col) * n + row] = y[mb][r];
+        }
       }
       __builtin_amdgcn_sched_barrier(0);
     }
+}
+
+// Descending order: every matrix's eigenvalues reversed in place, one workgroup per matrix (one
+// thread per exchanged pair).
+__global__ __launch_bounds__(kEigThreads) void reverse_values_kernel(const Batch *__restrict__ batch,
+                                                                      double *__restrict__ values) {
+  const Problem pr = batch->p[blockIdx.x];
+  double *lam = values + pr.values;
+  for (int i = threadIdx.x; i < pr.n / 2; i += kEigThreads) {
+    const double a = lam[i], b = lam[pr.n - 1 - i];
+    lam[i] = b;
+    lam[pr.n - 1 - i] = a;
+  }
 }
 
 // The batch table, written once per batch into the workspace, where every launch reads it (a
@@ -1249,8 +1268,10 @@ extern "C" int64_t tt_eigh_workspace(const tt_eigh_problem *problems, int64_t co
   return total;
 }
 
-extern "C" int tt_eigh_batched(const tt_eigh_problem *problems, int64_t count, double *matrices, double *values,
-                               double *vectors, double *workspace, void *stream) {
+namespace {
+
+int eigh_batched(const tt_eigh_problem *problems, int64_t count, double *matrices, double *values, double *vectors,
+                 double *workspace, void *stream, int descending) {
   if (count < 0 || (count > 0 && !problems)) return fail("tt_eigh_batched: bad problem list");
   if (count == 0) return 0;
   if (!matrices || !values || !vectors || !workspace) return fail("tt_eigh_batched: null buffer");
@@ -1317,8 +1338,26 @@ extern "C" int tt_eigh_batched(const tt_eigh_problem *problems, int64_t count, d
       hipLaunchKernelGGL(block_t_kernel, dim3(blocks), dim3(kEigThreads), 0, s, table, matrices, workspace);
       TT_LAUNCHED();
     }
-    hipLaunchKernelGGL(back_kernel, dim3(panels), dim3(kBackThreads), 0, s, table, matrices, vectors, workspace);
+    hipLaunchKernelGGL(back_kernel, dim3(panels), dim3(kBackThreads), 0, s, table, matrices, vectors, workspace,
+                       descending);
     TT_LAUNCHED();
+    if (descending) {
+      hipLaunchKernelGGL(reverse_values_kernel, dim3(batch.count), dim3(kEigThreads), 0, s, table, values);
+      TT_LAUNCHED();
+    }
   }
   return 0;
 }
+
+}  // namespace
+
+extern "C" int tt_eigh_batched(const tt_eigh_problem *problems, int64_t count, double *matrices, double *values,
+                               double *vectors, double *workspace, void *stream) {
+  return eigh_batched(problems, count, matrices, values, vectors, workspace, stream, 0);
+}
+
+extern "C" int tt_eigh_batched_descending(const tt_eigh_problem *problems, int64_t count, double *matrices,
+                                          double *values, double *vectors, double *workspace, void *stream) {
+  return eigh_batched(problems, count, matrices, values, vectors, workspace, stream, 1);
+}
+

@@ -244,7 +244,8 @@ def test_complex_grouped_gemm_matches_numpy(lib, config):
 
 def _eigh_cases(rng):
     '''Symmetric matrices of every kind the split meets: random, Gram matrices with geometric
-    spectra and rank deficits, exact degeneracies, diagonal, zero, and sizes 1 to 700.'''
+    spectra and rank deficits, exact degeneracies, diagonal, zero and identity (small and above
+    order 128), sizes 1 to 700.'''
     cases = []
     for n in (1, 2, 3, 5, 17, 64, 130):
         a = rng.standard_normal((n, n))
@@ -263,7 +264,36 @@ def _eigh_cases(rng):
     cases.append(np.diag(rng.standard_normal(40)))
     cases.append(np.zeros((9, 9)))
     cases.append(np.eye(30))
+    # above the LDS tail's 128 rows: every step with nothing to reflect, every merge fully deflated
+    cases.append(np.zeros((260, 260)))
+    cases.append(2.5 * np.eye(300))
     return cases
+
+
+def test_descending_eigh_is_the_ascending_one_reversed():
+    '''tt_eigh_batched_descending (the split's order) returns exactly what tt_eigh_batched does,
+    reordered: each matrix's eigenvalues reversed, and the eigenvector of its c-th largest
+    eigenvalue as column c.'''
+    from tentorch_amd import _native
+    rng = np.random.default_rng(23)
+    cases = []
+    for n in (1, 7, 100, 260):
+        a = rng.standard_normal((n, n))
+        cases.append(a + a.T)
+    sizes = [c.shape[0] for c in cases]
+    packed = torch.tensor(np.concatenate([c.reshape(-1) for c in cases]), device='cuda')
+    up_values, up_vectors = _native.eigh_batched(packed.clone(), sizes)
+    down_values, down_vectors = _native.eigh_batched(packed.clone(), sizes, descending=True)
+    up_values, up_vectors = up_values.cpu().numpy(), up_vectors.cpu().numpy()
+    down_values, down_vectors = down_values.cpu().numpy(), down_vectors.cpu().numpy()
+    at = vat = 0
+    for n in sizes:
+        assert np.array_equal(down_values[vat:vat + n], up_values[vat:vat + n][::-1]), n
+        rows = up_vectors[at:at + n * n].reshape(n, n)          # eigenvector j is row j, ascending
+        columns = down_vectors[at:at + n * n].reshape(n, n)     # eigenvector c is column c, descending
+        assert np.array_equal(columns, rows[::-1].T), n
+        at += n * n
+        vat += n
 
 
 @pytest.mark.parametrize('largest', [2048, 300, 128])
