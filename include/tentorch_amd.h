@@ -141,6 +141,14 @@ int tt_multidot(int64_t n, int k, const double *V, int64_t ldv, const double *w,
 /* w += scale * sum_{i < k} coef[i] V[i][:], coefficients read from device memory. */
 int tt_multiaxpy(int64_t n, int k, const double *coef, double scale, const double *V, int64_t ldv, double *w,
                  void *stream);
+/* One Krylov step's vector work after the action (tentorch/linalg/lanczos.py:36-50): w, the image of
+ * row k of V, is orthogonalised against rows 0..k of V by two classical Gram-Schmidt passes, each
+ * subtraction fused with what follows it (the second projection, then the norm); *alpha = the two
+ * passes' projections on row k, *beta = the norm left, and row k+1 of V = w / beta.  V is row-major
+ * with leading dimension ldv and at least k+2 rows; k + 1 <= 32.  `coef` holds 2 (k + 1) + 1
+ * doubles of device scratch; alpha and beta are device doubles.  Seven launches, no host sync. */
+int tt_lanczos_orthonormalize(int64_t n, int k, double *V, int64_t ldv, double *w, double *coef, double *alpha,
+                              double *beta, void *stream);
 
 typedef struct tt_graph tt_graph;
 /* Begin capturing every launch on `stream` into a graph. */

@@ -56,6 +56,8 @@ _lib.tt_timing_collect.argtypes = [ctypes.POINTER(_dbl), _i64, ctypes.POINTER(_i
 _lib.tt_launch_count.restype = ctypes.c_int64
 _lib.tt_multidot.argtypes = [_i64, _int, _c_void_p, _i64, _c_void_p, _c_void_p, _c_void_p]
 _lib.tt_multiaxpy.argtypes = [_i64, _int, _c_void_p, _dbl, _c_void_p, _i64, _c_void_p, _c_void_p]
+_lib.tt_lanczos_orthonormalize.argtypes = [_i64, _int, _c_void_p, _i64, _c_void_p, _c_void_p, _c_void_p, _c_void_p,
+                                           _c_void_p]
 _lib.tt_graph_begin.argtypes = [_c_void_p]
 _lib.tt_graph_end.argtypes = [_c_void_p, ctypes.POINTER(_c_void_p)]
 _lib.tt_graph_launch.argtypes = [_c_void_p, _c_void_p]
@@ -69,7 +71,7 @@ EXPORTED = ['tt_gss_plan_create', 'tt_gss_plan_destroy', 'tt_gss_plan_counts', '
             'tt_graph_end', 'tt_graph_launch', 'tt_graph_destroy', 'tt_last_error', 'tt_version',
             'tt_launch_count', 'tt_timing_enable', 'tt_timing_collect', 'tt_mlp_param_count', 'tt_mlp_forward',
             'tt_mlp_gradient', 'tt_mlp_adam', 'tt_eigh_max_order', 'tt_eigh_workspace', 'tt_eigh_batched',
-            'tt_eigh_batched_descending']
+            'tt_eigh_batched_descending', 'tt_lanczos_orthonormalize']
 
 
 def library() -> ctypes.CDLL:
@@ -155,6 +157,17 @@ def multiaxpy(V: torch.Tensor, k: int, coef: torch.Tensor, scale: float, w: torc
     '''w += scale * coef[:k] @ V[:k] with the coefficients on the device.'''
     check(_lib.tt_multiaxpy(w.numel(), k, coef.data_ptr(), float(scale), V.data_ptr(), V.stride(0), w.data_ptr(),
                             stream()), 'multiaxpy')
+
+
+def lanczos_orthonormalize(V: torch.Tensor, k: int, w: torch.Tensor, coef: torch.Tensor, alpha: torch.Tensor,
+                           beta: torch.Tensor) -> None:
+    '''w (the image of V[k]) orthogonalised against V[:k+1] by two Gram-Schmidt passes; alpha and beta
+    (device scalars) get its projection on V[k] and the norm left; V[k+1] = w / beta.  `coef` holds
+    2 (k + 1) + 1 doubles of scratch.'''
+    if coef.numel() < 2 * (k + 1) + 1 or V.shape[0] < k + 2:
+        raise ValueError('lanczos_orthonormalize: scratch or basis too small')
+    check(_lib.tt_lanczos_orthonormalize(w.numel(), k, V.data_ptr(), V.stride(0), w.data_ptr(), coef.data_ptr(),
+                                         alpha.data_ptr(), beta.data_ptr(), stream()), 'lanczos orthonormalize')
 
 
 def eigh_workspace(sizes) -> int:

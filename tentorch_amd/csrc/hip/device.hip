@@ -515,15 +515,20 @@ __global__ __launch_bounds__(kDotThreads) void multidot_stage1(int64_t n, int k,
     for (int i = 0; i < kMultiMax; ++i)
       if (i < k) acc[i] += V[i * ldv + j] * x;
   }
-  for (int i = 0; i < k; ++i) {
-    sh[threadIdx.x] = acc[i];
-    __syncthreads();
-    for (int s = kDotThreads / 2; s > 0; s >>= 1) {
-      if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+  // unrolled over the largest k (the guard is uniform), so acc stays in registers
+#pragma unroll
+  for (int i = 0; i < kMultiMax; ++i) {
+    if (i < k) {
+      sh[threadIdx.x] = acc[i];
+      __syncthreads();
+#pragma unroll
+      for (int s = kDotThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) partial[i * kMultiBlocks + blockIdx.x] = sh[0];
       __syncthreads();
     }
-    if (threadIdx.x == 0) partial[i * kMultiBlocks + blockIdx.x] = sh[0];
-    __syncthreads();
   }
 }
 
@@ -551,7 +556,113 @@ __global__ void multiaxpy_kernel(int64_t n, int k, const double *__restrict__ co
   }
 }
 
+// One Gram-Schmidt pass fused with what follows it: w -= V[0..k) coef, then, over the updated w,
+// either the partial products with V[0..k) (next_rows = k, the next pass's projection) or the
+// partial sum of squares (next_rows = 0, the norm); partials laid out as multidot_stage1's.
+__global__ __launch_bounds__(kDotThreads) void project_kernel(int64_t n, int k, const double *__restrict__ V, int64_t ldv,
+                                                              const double *__restrict__ coef, double *__restrict__ w,
+                                                              int next_rows, double *__restrict__ partial) {
+  __shared__ double sh[kDotThreads];
+  double c[kMultiMax], acc[kMultiMax];
+#pragma unroll
+  for (int i = 0; i < kMultiMax; ++i) {
+    c[i] = i < k ? coef[i] : 0.0;
+    acc[i] = 0.0;
+  }
+  double squares = 0.0;
+  for (int64_t j = static_cast<int64_t>(blockIdx.x) * kDotThreads + threadIdx.x; j < n;
+       j += static_cast<int64_t>(kMultiBlocks) * kDotThreads) {
+    double v[kMultiMax];
+    double x = w[j];
+#pragma unroll
+    for (int i = 0; i < kMultiMax; ++i) {
+      v[i] = i < k ? V[i * ldv + j] : 0.0;
+      x -= c[i] * v[i];
+    }
+    w[j] = x;
+    squares += x * x;
+#pragma unroll
+    for (int i = 0; i < kMultiMax; ++i) acc[i] += v[i] * x;
+  }
+  const int rows = next_rows > 0 ? next_rows : 1;
+  // unrolled over the largest k (the guard is uniform), so acc stays in registers
+#pragma unroll
+  for (int i = 0; i < kMultiMax; ++i) {
+    if (i < rows) {
+      sh[threadIdx.x] = next_rows > 0 ? acc[i] : squares;
+      __syncthreads();
+#pragma unroll
+      for (int s = kDotThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) partial[i * kMultiBlocks + blockIdx.x] = sh[0];
+      __syncthreads();
+    }
+  }
+}
+
+// The step's scalars from the norm's partials: beta = |w|, 1/beta, and alpha = the two passes'
+// projections on row k (first[k] + second[k]).
+__global__ __launch_bounds__(kDotThreads) void lanczos_scalars_kernel(const double *__restrict__ partial, int k,
+                                                                      const double *__restrict__ first,
+                                                                      const double *__restrict__ second,
+                                                                      double *__restrict__ alpha,
+                                                                      double *__restrict__ beta,
+                                                                      double *__restrict__ inverse) {
+  __shared__ double sh[kDotThreads];
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < kMultiBlocks; b += kDotThreads) acc += partial[b];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kDotThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double norm = sqrt(sh[0]);
+    *beta = norm;
+    *inverse = 1.0 / norm;
+    *alpha = first[k] + second[k];
+  }
+}
+
 }  // namespace
+
+extern "C" int tt_lanczos_orthonormalize(int64_t n, int k, double *V, int64_t ldv, double *w, double *coef,
+                                         double *alpha, double *beta, void *stream) {
+  if (n < 0 || k < 0 || k + 1 > kMultiMax) return fail("tt_lanczos_orthonormalize: k + 1 must lie in [1, 32]");
+  if (!V || !w || !coef || !alpha || !beta) return fail("tt_lanczos_orthonormalize: null buffer");
+  if (n == 0) return 0;
+  double *partial = nullptr;
+  if (int rc = multi_scratch(&partial)) return rc;
+  hipStream_t s = as_stream(stream);
+  const int rows = k + 1;
+  double *first = coef, *second = coef + rows, *inverse = coef + 2 * rows;
+  // first pass: the projections of w on rows 0..k
+  hipLaunchKernelGGL(multidot_stage1, dim3(kMultiBlocks), dim3(kDotThreads), 0, s, n, rows, V, ldv, w, partial);
+  TT_LAUNCHED();
+  hipLaunchKernelGGL(multidot_stage2, dim3(rows), dim3(kDotThreads), 0, s, partial, first);
+  TT_LAUNCHED();
+  // subtract them, and project the result again
+  hipLaunchKernelGGL(project_kernel, dim3(kMultiBlocks), dim3(kDotThreads), 0, s, n, rows, V, ldv, first, w, rows,
+                     partial);
+  TT_LAUNCHED();
+  hipLaunchKernelGGL(multidot_stage2, dim3(rows), dim3(kDotThreads), 0, s, partial, second);
+  TT_LAUNCHED();
+  // subtract the second projections, with the norm of what remains
+  hipLaunchKernelGGL(project_kernel, dim3(kMultiBlocks), dim3(kDotThreads), 0, s, n, rows, V, ldv, second, w, 0,
+                     partial);
+  TT_LAUNCHED();
+  hipLaunchKernelGGL(lanczos_scalars_kernel, dim3(1), dim3(kDotThreads), 0, s, partial, k, first, second, alpha, beta,
+                     inverse);
+  TT_LAUNCHED();
+  // the next basis vector
+  hipLaunchKernelGGL(axpby_kernel<double>, dim3(elementwise_grid(n)), dim3(256), 0, s, n, 0.0, inverse, w, 0.0,
+                     nullptr, V + rows * ldv);
+  TT_LAUNCHED();
+  return 0;
+}
 
 extern "C" int tt_multidot(int64_t n, int k, const double *V, int64_t ldv, const double *w, double *out, void *stream) {
   if (n < 0 || k < 0 || k > kMultiMax) return fail("tt_multidot: k must lie in [0, 32] and n be non-negative");

@@ -325,21 +325,25 @@ def lanczos(action: Callable[[Tensor], Tensor], start: Tensor, n_iter: int = 5,
 
 
 def _lanczos_device(action, start: Tensor, n_iter: int, tol: float) -> Tuple[float, Tensor]:
-    '''The same recurrence with every scalar on the device.  Each Krylov step: the action, alpha =
-    <v_k, Hv_k>, w -= alpha v_k + beta_{k-1} v_{k-1}, two classical Gram-Schmidt passes against the
-    basis (the reference runs modified Gram-Schmidt, `lanczos.py:43-45`: equal up to rounding), beta =
-    |w|, v_{k+1} = w / beta.  The alphas and betas are read back once; the break at beta < tol
-    (`lanczos.py:46-48`) is applied to what was read.'''
+    '''The same recurrence with every scalar on the device.  Each Krylov step: the action, then one
+    library call (`tt_lanczos_orthonormalize`): two classical Gram-Schmidt passes of w = Hv_k against
+    the whole basis, alpha = its projections on v_k (<v_k, Hv_k> plus the second pass's correction),
+    beta = |w|, v_{k+1} = w / beta.  The reference subtracts alpha v_k + beta_{k-1} v_{k-1} first and
+    then runs two modified Gram-Schmidt passes (`lanczos.py:36-45`); the first full pass removes the
+    same components, so the two agree up to rounding.  The alphas and betas are read back once; the
+    break at beta < tol (`lanczos.py:46-48`) is applied to what was read.'''
     scale = norm(start)
     if scale == 0.0:
         raise ValueError('Lanczos needs a starting vector of non-zero norm.')
     size_n = start.structure.size
     device = start.device
     V = torch.empty((n_iter + 1, size_n), dtype=torch.float64, device=device)
-    scalars = torch.zeros(4 * (n_iter + 2), dtype=torch.float64, device=device)
+    # alphas, betas, the step's scratch (2 (k+1) + 1), the Ritz coefficients, two work scalars
+    scalars = torch.zeros(5 * n_iter + 8, dtype=torch.float64, device=device)
     alphas, betas = scalars[:n_iter], scalars[n_iter:2 * n_iter]
-    coef = scalars[2 * n_iter:3 * n_iter + 1]
-    work = scalars[3 * n_iter + 2:]
+    scratch = scalars[2 * n_iter:4 * n_iter + 3]
+    coef = scalars[4 * n_iter + 3:5 * n_iter + 4]
+    work = scalars[5 * n_iter + 4:]
     _native.axpby(torch.float64, 1.0 / scale, start.data, 0.0, V[0])
     for k in range(n_iter):
         vector = Tensor._wrap(start.structure, V[k])
@@ -347,17 +351,7 @@ def _lanczos_device(action, start: Tensor, n_iter: int, tol: float) -> Tuple[flo
         if image.structure != start.structure:
             raise ValueError('the action has to map the legs of the start vector to themselves')
         w = image.data if image.data.data_ptr() != V[k].data_ptr() else image.data.clone()
-        _native.dot(torch.float64, V[k], w, alphas[k:k + 1])
-        _native.multiaxpy(V[k:k + 1], 1, alphas[k:k + 1], -1.0, w)
-        if k > 0:
-            _native.multiaxpy(V[k - 1:k], 1, betas[k - 1:k], -1.0, w)
-        for _ in range(2):
-            _native.multidot(V, k + 1, w, coef)
-            _native.multiaxpy(V, k + 1, coef, -1.0, w)
-        _native.dot(torch.float64, w, w, work[0:1])
-        torch.sqrt(work[0:1], out=betas[k:k + 1])
-        torch.reciprocal(betas[k:k + 1], out=work[1:2])
-        _native.axpby(torch.float64, 0.0, w, 0.0, V[k + 1], alpha_dev=work[1:2])
+        _native.lanczos_orthonormalize(V, k, w, scratch, alphas[k:k + 1], betas[k:k + 1])
     host = scalars[:2 * n_iter].cpu().numpy()
     a, b = host[:n_iter], host[n_iter:]
     size = n_iter

@@ -164,6 +164,34 @@ def test_krylov_kernels_match_numpy(lib):
     assert np.abs(w.cpu().numpy() - ref_w).max() <= 1e-12 * np.abs(ref_w).max()
 
 
+def test_lanczos_orthonormalization_matches_numpy(lib):
+    '''tt_lanczos_orthonormalize against numpy's two classical Gram-Schmidt passes on a 6-vector
+    orthonormal basis: the projection on the last basis vector, the norm left, and the next basis
+    vector (orthogonal to the basis to 1e-13).'''
+    rng = np.random.default_rng(10)
+    n, k = 50021, 5
+    q, _ = np.linalg.qr(rng.standard_normal((n, k + 1)))
+    w0 = rng.standard_normal(n) + 3.0 * q[:, k] - 2.0 * q[:, k - 1]
+    V = torch.zeros((k + 2, n), dtype=torch.float64, device='cuda')
+    V[:k + 1] = torch.tensor(q.T.copy(), device='cuda')
+    w = torch.tensor(w0, device='cuda')
+    coef = torch.zeros(2 * (k + 1) + 1, dtype=torch.float64, device='cuda')
+    alpha = torch.zeros(1, dtype=torch.float64, device='cuda')
+    beta = torch.zeros(1, dtype=torch.float64, device='cuda')
+    assert lib.tt_lanczos_orthonormalize(n, k, V.data_ptr(), V.stride(0), w.data_ptr(), coef.data_ptr(),
+                                         alpha.data_ptr(), beta.data_ptr(), _stream()) == 0, cabi.error(lib)
+    first = q.T @ w0
+    once = w0 - q @ first
+    second = q.T @ once
+    twice = once - q @ second
+    left = np.linalg.norm(twice)
+    assert abs(alpha.item() - (first[k] + second[k])) <= 1e-12 * abs(first[k])
+    assert abs(beta.item() - left) <= 1e-12 * left
+    nxt = V[k + 1].cpu().numpy()
+    assert np.abs(nxt - twice / left).max() <= 1e-12 * np.abs(twice / left).max()
+    assert np.abs(q.T @ nxt).max() <= 1e-13
+
+
 def test_typed_leg_move_converts_between_fp64_and_fp32(lib):
     '''A composed move that reads FP64 and writes FP32 equals the FP64 move rounded to FP32.'''
     from tentorch_amd._native import engine
