@@ -177,11 +177,6 @@ int64_t tt_eigh_max_order(void);
 int64_t tt_eigh_workspace(const tt_eigh_problem *problems, int64_t count);
 int tt_eigh_batched(const tt_eigh_problem *problems, int64_t count, double *matrices, double *values,
                     double *vectors, double *workspace, void *stream);
-/* The same in the split's order: eigenvalues descending, and the eigenvector of the c-th largest
- * eigenvalue in column c of the row-major n x n at `vectors + matrix_offset` (the isometric factor of
- * the split is then its first r columns, as the per-block SVD of svd.cpp:59 orders them). */
-int tt_eigh_batched_descending(const tt_eigh_problem *problems, int64_t count, double *matrices, double *values,
-                               double *vectors, double *workspace, void *stream);
 
 /* The learned tile schedule's cost model (no counterpart in the reference): a multilayer
  * perceptron x (features) -> relu(W1^T x + b1) (hidden) -> relu(W2^T h1 + b2) (hidden) -> w3 . h2 + b3

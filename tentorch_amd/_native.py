@@ -49,8 +49,6 @@ _lib.tt_eigh_workspace.argtypes = [ctypes.POINTER(EighProblem), _i64]
 _lib.tt_eigh_workspace.restype = ctypes.c_int64
 _lib.tt_eigh_batched.argtypes = [ctypes.POINTER(EighProblem), _i64, _c_void_p, _c_void_p, _c_void_p, _c_void_p,
                                  _c_void_p]
-_lib.tt_eigh_batched_descending.argtypes = [ctypes.POINTER(EighProblem), _i64, _c_void_p, _c_void_p, _c_void_p,
-                                            _c_void_p, _c_void_p]
 _lib.tt_timing_enable.argtypes = [_int]
 _lib.tt_timing_collect.argtypes = [ctypes.POINTER(_dbl), _i64, ctypes.POINTER(_i64)]
 _lib.tt_launch_count.restype = ctypes.c_int64
@@ -71,7 +69,7 @@ EXPORTED = ['tt_gss_plan_create', 'tt_gss_plan_destroy', 'tt_gss_plan_counts', '
             'tt_graph_end', 'tt_graph_launch', 'tt_graph_destroy', 'tt_last_error', 'tt_version',
             'tt_launch_count', 'tt_timing_enable', 'tt_timing_collect', 'tt_mlp_param_count', 'tt_mlp_forward',
             'tt_mlp_gradient', 'tt_mlp_adam', 'tt_eigh_max_order', 'tt_eigh_workspace', 'tt_eigh_batched',
-            'tt_eigh_batched_descending', 'tt_lanczos_orthonormalize']
+            'tt_lanczos_orthonormalize']
 
 
 def library() -> ctypes.CDLL:
@@ -181,14 +179,12 @@ def eigh_workspace(sizes) -> int:
     return size
 
 
-def eigh_batched(matrices: torch.Tensor, sizes, offsets=None, work: Optional[torch.Tensor] = None,
-                 descending: bool = False):
+def eigh_batched(matrices: torch.Tensor, sizes, offsets=None, work: Optional[torch.Tensor] = None):
     '''Eigenvalues (ascending) and eigenvectors of the symmetric FP64 matrices packed in `matrices`
     (matrix b is sizes[b] x sizes[b], at offsets[b], default back to back), on the device.  Returns
     (values, vectors): values packed back to back, vectors with eigenvector j of matrix b as row j
-    of its n x n at the matrix's offset.  With `descending`, the values of each matrix descend and
-    the eigenvector of its c-th largest value is column c of its n x n.  `matrices` is overwritten;
-    `work` may supply the workspace (any contents).'''
+    of its n x n at the matrix's offset.  `matrices` is overwritten; `work` may supply the workspace
+    (any contents).'''
     if matrices.dtype != torch.float64:
         raise TypeError('the batched eigensolver runs in float64')
     require_device(matrices)
@@ -210,9 +206,8 @@ def eigh_batched(matrices: torch.Tensor, sizes, offsets=None, work: Optional[tor
     workspace = torch.empty(max(size, 1), dtype=torch.float64, device=matrices.device) if work is None else work
     if workspace.numel() < size or workspace.dtype != torch.float64:
         raise ValueError(f'the eigensolver workspace holds {workspace.numel()} doubles, {size} needed')
-    entry = _lib.tt_eigh_batched_descending if descending else _lib.tt_eigh_batched
-    check(entry(problems, count, matrices.data_ptr(), values.data_ptr(), vectors.data_ptr(), workspace.data_ptr(),
-                stream()), 'eigh')
+    check(_lib.tt_eigh_batched(problems, count, matrices.data_ptr(), values.data_ptr(), vectors.data_ptr(),
+                               workspace.data_ptr(), stream()), 'eigh')
     return values[:value_at], vectors
 
 

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -484,7 +485,7 @@ extern "C" int tt_convert(int from_type, int to_type, int64_t n, const void *x, 
 
 namespace {
 
-constexpr int kMultiBlocks = 256;
+constexpr int kMultiBlocks = 1024;   // enough waves to keep the memory-bound passes fed
 constexpr int kMultiMax = 32;
 
 std::vector<double *> g_multi_scratch;
@@ -502,22 +503,24 @@ int multi_scratch(double **out) {
 
 // Block b sums V[i, j] w[j] over its strided share of j for every row i < k (fixed order), then a
 // fixed LDS tree per row; partial[i * kMultiBlocks + b].
+// kRows (a compile-time bound on k: 4, 8, 16 or 32) keeps the per-row registers to what k needs.
+template <int kRows>
 __global__ __launch_bounds__(kDotThreads) void multidot_stage1(int64_t n, int k, const double *__restrict__ V, int64_t ldv,
                                                                const double *__restrict__ w, double *__restrict__ partial) {
   __shared__ double sh[kDotThreads];
-  double acc[kMultiMax];
+  double acc[kRows];
 #pragma unroll
-  for (int i = 0; i < kMultiMax; ++i) acc[i] = 0.0;
+  for (int i = 0; i < kRows; ++i) acc[i] = 0.0;
   for (int64_t j = static_cast<int64_t>(blockIdx.x) * kDotThreads + threadIdx.x; j < n;
        j += static_cast<int64_t>(kMultiBlocks) * kDotThreads) {
     const double x = w[j];
 #pragma unroll
-    for (int i = 0; i < kMultiMax; ++i)
+    for (int i = 0; i < kRows; ++i)
       if (i < k) acc[i] += V[i * ldv + j] * x;
   }
-  // unrolled over the largest k (the guard is uniform), so acc stays in registers
+  // unrolled over kRows (the guard is uniform), so acc stays in registers
 #pragma unroll
-  for (int i = 0; i < kMultiMax; ++i) {
+  for (int i = 0; i < kRows; ++i) {
     if (i < k) {
       sh[threadIdx.x] = acc[i];
       __syncthreads();
@@ -559,35 +562,36 @@ __global__ void multiaxpy_kernel(int64_t n, int k, const double *__restrict__ co
 // One Gram-Schmidt pass fused with what follows it: w -= V[0..k) coef, then, over the updated w,
 // either the partial products with V[0..k) (next_rows = k, the next pass's projection) or the
 // partial sum of squares (next_rows = 0, the norm); partials laid out as multidot_stage1's.
+template <int kRows>
 __global__ __launch_bounds__(kDotThreads) void project_kernel(int64_t n, int k, const double *__restrict__ V, int64_t ldv,
                                                               const double *__restrict__ coef, double *__restrict__ w,
                                                               int next_rows, double *__restrict__ partial) {
   __shared__ double sh[kDotThreads];
-  double c[kMultiMax], acc[kMultiMax];
+  double c[kRows], acc[kRows];
 #pragma unroll
-  for (int i = 0; i < kMultiMax; ++i) {
+  for (int i = 0; i < kRows; ++i) {
     c[i] = i < k ? coef[i] : 0.0;
     acc[i] = 0.0;
   }
   double squares = 0.0;
   for (int64_t j = static_cast<int64_t>(blockIdx.x) * kDotThreads + threadIdx.x; j < n;
        j += static_cast<int64_t>(kMultiBlocks) * kDotThreads) {
-    double v[kMultiMax];
+    double v[kRows];
     double x = w[j];
 #pragma unroll
-    for (int i = 0; i < kMultiMax; ++i) {
+    for (int i = 0; i < kRows; ++i) {
       v[i] = i < k ? V[i * ldv + j] : 0.0;
       x -= c[i] * v[i];
     }
     w[j] = x;
     squares += x * x;
 #pragma unroll
-    for (int i = 0; i < kMultiMax; ++i) acc[i] += v[i] * x;
+    for (int i = 0; i < kRows; ++i) acc[i] += v[i] * x;
   }
   const int rows = next_rows > 0 ? next_rows : 1;
-  // unrolled over the largest k (the guard is uniform), so acc stays in registers
+  // unrolled over kRows (the guard is uniform), so acc stays in registers
 #pragma unroll
-  for (int i = 0; i < kMultiMax; ++i) {
+  for (int i = 0; i < kRows; ++i) {
     if (i < rows) {
       sh[threadIdx.x] = next_rows > 0 ? acc[i] : squares;
       __syncthreads();
@@ -627,6 +631,15 @@ __global__ __launch_bounds__(kDotThreads) void lanczos_scalars_kernel(const doub
   }
 }
 
+// The smallest row bound that covers k rows.
+template <typename Launch>
+int with_rows(int k, Launch launch) {
+  if (k <= 4) return launch(std::integral_constant<int, 4>{});
+  if (k <= 8) return launch(std::integral_constant<int, 8>{});
+  if (k <= 16) return launch(std::integral_constant<int, 16>{});
+  return launch(std::integral_constant<int, 32>{});
+}
+
 }  // namespace
 
 extern "C" int tt_lanczos_orthonormalize(int64_t n, int k, double *V, int64_t ldv, double *w, double *coef,
@@ -639,29 +652,32 @@ extern "C" int tt_lanczos_orthonormalize(int64_t n, int k, double *V, int64_t ld
   hipStream_t s = as_stream(stream);
   const int rows = k + 1;
   double *first = coef, *second = coef + rows, *inverse = coef + 2 * rows;
-  // first pass: the projections of w on rows 0..k
-  hipLaunchKernelGGL(multidot_stage1, dim3(kMultiBlocks), dim3(kDotThreads), 0, s, n, rows, V, ldv, w, partial);
-  TT_LAUNCHED();
-  hipLaunchKernelGGL(multidot_stage2, dim3(rows), dim3(kDotThreads), 0, s, partial, first);
-  TT_LAUNCHED();
-  // subtract them, and project the result again
-  hipLaunchKernelGGL(project_kernel, dim3(kMultiBlocks), dim3(kDotThreads), 0, s, n, rows, V, ldv, first, w, rows,
-                     partial);
-  TT_LAUNCHED();
-  hipLaunchKernelGGL(multidot_stage2, dim3(rows), dim3(kDotThreads), 0, s, partial, second);
-  TT_LAUNCHED();
-  // subtract the second projections, with the norm of what remains
-  hipLaunchKernelGGL(project_kernel, dim3(kMultiBlocks), dim3(kDotThreads), 0, s, n, rows, V, ldv, second, w, 0,
-                     partial);
-  TT_LAUNCHED();
-  hipLaunchKernelGGL(lanczos_scalars_kernel, dim3(1), dim3(kDotThreads), 0, s, partial, k, first, second, alpha, beta,
-                     inverse);
-  TT_LAUNCHED();
-  // the next basis vector
-  hipLaunchKernelGGL(axpby_kernel<double>, dim3(elementwise_grid(n)), dim3(256), 0, s, n, 0.0, inverse, w, 0.0,
-                     nullptr, V + rows * ldv);
-  TT_LAUNCHED();
-  return 0;
+  return with_rows(rows, [&](auto bound) -> int {
+    constexpr int R = decltype(bound)::value;
+    // first pass: the projections of w on rows 0..k
+    hipLaunchKernelGGL(multidot_stage1<R>, dim3(kMultiBlocks), dim3(kDotThreads), 0, s, n, rows, V, ldv, w, partial);
+    TT_LAUNCHED();
+    hipLaunchKernelGGL(multidot_stage2, dim3(rows), dim3(kDotThreads), 0, s, partial, first);
+    TT_LAUNCHED();
+    // subtract them, and project the result again
+    hipLaunchKernelGGL(project_kernel<R>, dim3(kMultiBlocks), dim3(kDotThreads), 0, s, n, rows, V, ldv, first, w, rows,
+                       partial);
+    TT_LAUNCHED();
+    hipLaunchKernelGGL(multidot_stage2, dim3(rows), dim3(kDotThreads), 0, s, partial, second);
+    TT_LAUNCHED();
+    // subtract the second projections, with the norm of what remains
+    hipLaunchKernelGGL(project_kernel<R>, dim3(kMultiBlocks), dim3(kDotThreads), 0, s, n, rows, V, ldv, second, w, 0,
+                       partial);
+    TT_LAUNCHED();
+    hipLaunchKernelGGL(lanczos_scalars_kernel, dim3(1), dim3(kDotThreads), 0, s, partial, k, first, second, alpha, beta,
+                       inverse);
+    TT_LAUNCHED();
+    // the next basis vector
+    hipLaunchKernelGGL(axpby_kernel<double>, dim3(elementwise_grid(n)), dim3(256), 0, s, n, 0.0, inverse, w, 0.0,
+                       nullptr, V + rows * ldv);
+    TT_LAUNCHED();
+    return 0;
+  });
 }
 
 extern "C" int tt_multidot(int64_t n, int k, const double *V, int64_t ldv, const double *w, double *out, void *stream) {
@@ -670,8 +686,13 @@ extern "C" int tt_multidot(int64_t n, int k, const double *V, int64_t ldv, const
   if (!V || !w || !out) return fail("tt_multidot: null buffer");
   double *partial = nullptr;
   if (int rc = multi_scratch(&partial)) return rc;
-  hipLaunchKernelGGL(multidot_stage1, dim3(kMultiBlocks), dim3(kDotThreads), 0, as_stream(stream), n, k, V, ldv, w, partial);
-  TT_LAUNCHED();
+  if (int rc = with_rows(k, [&](auto bound) -> int {
+        hipLaunchKernelGGL(multidot_stage1<decltype(bound)::value>, dim3(kMultiBlocks), dim3(kDotThreads), 0,
+                           as_stream(stream), n, k, V, ldv, w, partial);
+        TT_LAUNCHED();
+        return 0;
+      }))
+    return rc;
   hipLaunchKernelGGL(multidot_stage2, dim3(k), dim3(kDotThreads), 0, as_stream(stream), partial, out);
   TT_LAUNCHED();
   return 0;

@@ -1117,8 +1117,7 @@ __global__ __launch_bounds__(kEigThreads) void block_t_kernel(const Batch *__res
 }
 
 __global__ __launch_bounds__(kBackThreads) void back_kernel(const Batch *__restrict__ batch, const double *__restrict__ matrices,
-                                                            double *__restrict__ vectors, const double *__restrict__ work,
-                                                            int descending) {
+                                                            double *__restrict__ vectors, const double *__restrict__ work) {
   // the problem this panel belongs to
   int b = 0;
   while (b + 1 < batch->count && batch->p[b + 1].first_panel <= static_cast<int>(blockIdx.x)) ++b;
@@ -1219,28 +1218,10 @@ __global__ __launch_bounds__(kBackThreads) void back_kernel(const Batch *__restr
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = base_row + 4 * r + lk0;
-        if (row < n) {
-          if (descending)
-            out[int64_t(row) * n + (n - 1 - col)] = y[mb][r];
-          else
-            out[int64_t(col) * n + row] = y[mb][r];
-        }
+        if (row < n) out[int64_t(col) * n + row] = y[mb][r];
       }
       __builtin_amdgcn_sched_barrier(0);
     }
-}
-
-// Descending order: every matrix's eigenvalues reversed in place, one workgroup per matrix (one
-// thread per exchanged pair).
-__global__ __launch_bounds__(kEigThreads) void reverse_values_kernel(const Batch *__restrict__ batch,
-                                                                      double *__restrict__ values) {
-  const Problem pr = batch->p[blockIdx.x];
-  double *lam = values + pr.values;
-  for (int i = threadIdx.x; i < pr.n / 2; i += kEigThreads) {
-    const double a = lam[i], b = lam[pr.n - 1 - i];
-    lam[i] = b;
-    lam[pr.n - 1 - i] = a;
-  }
 }
 
 // The batch table, written once per batch into the workspace, where every launch reads it (a
@@ -1271,7 +1252,7 @@ extern "C" int64_t tt_eigh_workspace(const tt_eigh_problem *problems, int64_t co
 namespace {
 
 int eigh_batched(const tt_eigh_problem *problems, int64_t count, double *matrices, double *values, double *vectors,
-                 double *workspace, void *stream, int descending) {
+                 double *workspace, void *stream) {
   if (count < 0 || (count > 0 && !problems)) return fail("tt_eigh_batched: bad problem list");
   if (count == 0) return 0;
   if (!matrices || !values || !vectors || !workspace) return fail("tt_eigh_batched: null buffer");
@@ -1338,13 +1319,8 @@ int eigh_batched(const tt_eigh_problem *problems, int64_t count, double *matrice
       hipLaunchKernelGGL(block_t_kernel, dim3(blocks), dim3(kEigThreads), 0, s, table, matrices, workspace);
       TT_LAUNCHED();
     }
-    hipLaunchKernelGGL(back_kernel, dim3(panels), dim3(kBackThreads), 0, s, table, matrices, vectors, workspace,
-                       descending);
+    hipLaunchKernelGGL(back_kernel, dim3(panels), dim3(kBackThreads), 0, s, table, matrices, vectors, workspace);
     TT_LAUNCHED();
-    if (descending) {
-      hipLaunchKernelGGL(reverse_values_kernel, dim3(batch.count), dim3(kEigThreads), 0, s, table, values);
-      TT_LAUNCHED();
-    }
   }
   return 0;
 }
@@ -1353,11 +1329,6 @@ int eigh_batched(const tt_eigh_problem *problems, int64_t count, double *matrice
 
 extern "C" int tt_eigh_batched(const tt_eigh_problem *problems, int64_t count, double *matrices, double *values,
                                double *vectors, double *workspace, void *stream) {
-  return eigh_batched(problems, count, matrices, values, vectors, workspace, stream, 0);
-}
-
-extern "C" int tt_eigh_batched_descending(const tt_eigh_problem *problems, int64_t count, double *matrices,
-                                          double *values, double *vectors, double *workspace, void *stream) {
-  return eigh_batched(problems, count, matrices, values, vectors, workspace, stream, 1);
+  return eigh_batched(problems, count, matrices, values, vectors, workspace, stream);
 }
 

@@ -75,15 +75,14 @@ def _gram_factors_device(blocks, left_isometric: bool):
             torch.matmul(b, b.transpose(0, 1), out=gram)
         else:
             torch.matmul(b.transpose(0, 1), b, out=gram)
-    # descending: each block's values from the largest, its eigenvectors as columns in that order,
-    # so a block's factor is a view of the first r columns and its singular values a slice
-    values, vectors = _native.eigh_batched(packed, sizes, descending=True)
-    singular = values.clamp_(min=0).sqrt_()
+    values, vectors = _native.eigh_batched(packed, sizes)
     out = []
     value_at = 0
     for b, g, at in zip(blocks, sizes, offsets):
         r = min(b.shape)
-        out.append((vectors[at:at + g * g].view(g, g)[:, :r], singular[value_at:value_at + r]))
+        lam = values[value_at + g - r:value_at + g].flip(0)
+        rows = vectors[at:at + g * g].view(g, g)[g - r:]      # eigenvector j is row j
+        out.append((rows.flip(0).transpose(0, 1), lam.clamp(min=0).sqrt()))
         value_at += g
     return out
 
@@ -208,25 +207,19 @@ def svd(tensor: Tensor, left: Optional[Sequence[int]] = None, max_bond: Optional
     bond = _bond_leg(cat, mults)
     first = Tensor(cat, arranged.codomain, [bond], arranged.qtotal, arranged.dtype, arranged.device)
     second = Tensor(cat, [bond], arranged.domain, cat.trivial(), arranged.dtype, arranged.device)
-    pieces = []    # (position on the bond, kept values) of every block
+    values = torch.zeros(bond.multiplets(), dtype=arranged.dtype, device=arranged.device)
     for f in factors:
         k = f['kept']
         if not k:
             continue
         r, s, c = f['key']
         v = f['v'][:k]
-        o = f['offset']
-        pieces.append((placed[s] + o, v))
         if gram:
             q = f['q'][:, :k]
             if absorb == 2:
-                # rows o..o+k of the second factor's block are contiguous: the product lands there
-                first.block(r, s, c)[:, o:o + k] = q
-                torch.matmul(q.transpose(0, 1), f['block'], out=second.block(s, s)[o:o + k, :])
+                u, vh = q, q.transpose(0, 1) @ f['block']
             else:
-                first.block(r, s, c)[:, o:o + k] = f['block'] @ q
-                second.block(s, s)[o:o + k, :] = q.transpose(0, 1)
-            continue
+                u, vh = f['block'] @ q, q.transpose(0, 1)
         else:
             u, vh = f['u'][:, :k], f['vh'][:k]
             if absorb == 3:
@@ -236,22 +229,10 @@ def svd(tensor: Tensor, left: Optional[Sequence[int]] = None, max_bond: Optional
                 u = u * v
             elif absorb == 2:
                 vh = v.unsqueeze(1) * vh
+        o = f['offset']
         first.block(r, s, c)[:, o:o + k] = u
         second.block(s, s)[o:o + k, :] = vh
-    # the kept values fill the bond in order, sector by sector, each block's run at its offset: one
-    # concatenation when the runs tile the bond, else each run placed into zeros
-    pieces.sort(key=lambda piece: piece[0])
-    at = 0
-    for position, v in pieces:
-        if position != at:
-            break
-        at += v.numel()
-    if pieces and at == bond.multiplets():
-        values = torch.cat([v for _, v in pieces]).to(arranged.dtype)
-    else:
-        values = torch.zeros(bond.multiplets(), dtype=arranged.dtype, device=arranged.device)
-        for position, v in pieces:
-            values[position:position + v.numel()] = v.to(arranged.dtype)
+        values[placed[s] + o:placed[s] + o + k] = v.to(arranged.dtype)
     return first, values, second
 
 

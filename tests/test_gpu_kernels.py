@@ -298,32 +298,6 @@ def _eigh_cases(rng):
     return cases
 
 
-def test_descending_eigh_is_the_ascending_one_reversed():
-    '''tt_eigh_batched_descending (the split's order) returns exactly what tt_eigh_batched does,
-    reordered: each matrix's eigenvalues reversed, and the eigenvector of its c-th largest
-    eigenvalue as column c.'''
-    from tentorch_amd import _native
-    rng = np.random.default_rng(23)
-    cases = []
-    for n in (1, 7, 100, 260):
-        a = rng.standard_normal((n, n))
-        cases.append(a + a.T)
-    sizes = [c.shape[0] for c in cases]
-    packed = torch.tensor(np.concatenate([c.reshape(-1) for c in cases]), device='cuda')
-    up_values, up_vectors = _native.eigh_batched(packed.clone(), sizes)
-    down_values, down_vectors = _native.eigh_batched(packed.clone(), sizes, descending=True)
-    up_values, up_vectors = up_values.cpu().numpy(), up_vectors.cpu().numpy()
-    down_values, down_vectors = down_values.cpu().numpy(), down_vectors.cpu().numpy()
-    at = vat = 0
-    for n in sizes:
-        assert np.array_equal(down_values[vat:vat + n], up_values[vat:vat + n][::-1]), n
-        rows = up_vectors[at:at + n * n].reshape(n, n)          # eigenvector j is row j, ascending
-        columns = down_vectors[at:at + n * n].reshape(n, n)     # eigenvector c is column c, descending
-        assert np.array_equal(columns, rows[::-1].T), n
-        at += n * n
-        vat += n
-
-
 @pytest.mark.parametrize('largest', [2048, 300, 128])
 def test_batched_eigh_matches_numpy(largest):
     '''tt_eigh_batched on one batch of mixed sizes: eigenvalues within 1e-12 of numpy's (relative to
