@@ -7,7 +7,7 @@
 //      reflectors stay in the matrix's lower part.  While a trailing matrix exceeds 128 rows, one
 //      launch per step spreads every matrix over one workgroup per 16 trailing columns; the last
 //      128 steps of every matrix run in one launch, one workgroup per matrix, in LDS.
-//   2. Cuppen divide and conquer on the tridiagonal: leaves of 16 rows by implicit QL (one wave
+//   2. Cuppen divide and conquer on the tridiagonal: leaves of 8 rows by implicit QL (one wave
 //      each), then merges bottom up, four launches per level for every matrix of the batch: one
 //      workgroup per merge deflates small coupling components and close poles (Givens rotations);
 //      one wave per secular root takes rational ("middle way") steps from the nearer pole; one
@@ -34,7 +34,7 @@ constexpr int kEigWaves = kEigThreads / 64;
 constexpr int kEigMaxN = 2048;      // largest matrix (LDS vectors are sized for it)
 constexpr int kEigBatch = 64;       // problems per launch (kernel-argument table)
 constexpr int kPanel = 16;          // columns per back-transformation workgroup
-constexpr int kLeaf = 16;           // order of the divide-and-conquer leaves
+constexpr int kLeaf = 8;            // order of the divide-and-conquer leaves
 constexpr double kEps = 2.220446049250313e-16;
 
 struct Problem {
