@@ -94,6 +94,8 @@ class Category {
 
   const std::string &name() const { return name_; }
   virtual bool abelian() const { return false; }
+  // Whether every R symbol is the 1x1 one, so an exchange of two legs is a plain relabelling.
+  virtual bool trivial_braiding() const { return false; }
   // Whether every F and R entry is real (the scalar type the reference's `dtype()` reports).
   virtual bool real_symbols() const { return true; }
 
@@ -139,6 +141,7 @@ class AbelianGroup : public Category {
  public:
   using Category::Category;
   bool abelian() const override { return true; }
+  bool trivial_braiding() const override { return true; }
   double q_dim(const Sector &) const override { return 1.0; }
   Sector dual(const Sector &a) const override { return inverse(a); }
   virtual Sector combine(const Sector &a, const Sector &b) const = 0;

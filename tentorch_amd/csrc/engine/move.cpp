@@ -353,21 +353,24 @@ std::vector<int64_t> chain_permutation(int64_t ncod0, int64_t nlegs, const std::
 // exactly one tree pair: an exchange swaps two sectors with the factor R(b, c) (a sign for fermion
 // parity, 1 for the groups), a bend moves the last sector of one side to the end of the other as its
 // dual with F = 1 (regroup.cpp:25-417 reduce to this).  The composed map is therefore the chain
-// replayed on the sector lists of each source pair; only the final structure is built.
-std::vector<Terms> compose_abelian(const StructurePtr &source, const std::vector<Step> &steps, const PairSpace &target) {
+// replayed on the sector lists of each source pair; only the final structure is built.  Where every R
+// is 1 the replay collapses further: target axis j carries the sector of source leg perm[j], taken
+// dual when that leg ends on the other side (the dual is an involution, so two bends cancel).
+// For each source pair: the target pair it lands on (-1 when a dual sector is not on its leg) and the
+// coefficient.
+void compose_abelian(const StructurePtr &source, const std::vector<Step> &steps, const std::vector<int64_t> &perm,
+                     const PairSpace &target, std::vector<int64_t> &to, std::vector<cplx> &coefs) {
   const Category &cat = *source->cat();
   const Structure &T = *target.s;
+  const int64_t ncod0 = static_cast<int64_t>(source->codomain().size());
+  const int64_t ncod1 = static_cast<int64_t>(T.codomain().size());
+  const int64_t nlegs = source->nlegs();
+  const bool replay = !cat.trivial_braiding();
+  std::vector<char> flips(static_cast<size_t>(nlegs));
+  for (int64_t j = 0; j < nlegs; ++j) flips[static_cast<size_t>(j)] = (perm[static_cast<size_t>(j)] < ncod0) != (j < ncod1);
   PairSpace space(source);
-  std::vector<Terms> comp(static_cast<size_t>(space.total));
-  // Target tree lookup by sector list, per side layout.
-  std::map<const SideLayout *, std::map<std::vector<Sector>, int64_t>> lookup;
-  auto index_of = [&](const SideLayout &layout, const std::vector<Sector> &sectors) -> int64_t {
-    auto &table = lookup[&layout];
-    if (table.empty())
-      for (size_t t = 0; t < layout.trees.size(); ++t) table.emplace(layout.trees[t].sectors, static_cast<int64_t>(t));
-    auto found = table.find(sectors);
-    return found == table.end() ? -1 : found->second;
-  };
+  to.assign(static_cast<size_t>(space.total), -1);
+  coefs.assign(static_cast<size_t>(space.total), cplx(1.0, 0.0));
   auto root_of = [&](const std::vector<Sector> &sectors) {
     Sector r = cat.trivial();
     for (const Sector &s : sectors) r = cat.outcomes(r, s).front().sector;
@@ -378,32 +381,45 @@ std::vector<Terms> compose_abelian(const StructurePtr &source, const std::vector
     const BlockInfo &blk = source->blocks()[static_cast<size_t>(b)];
     for (int64_t row = 0; row < static_cast<int64_t>(blk.row_layout->trees.size()); ++row)
       for (int64_t col = 0; col < static_cast<int64_t>(blk.col_layout->trees.size()); ++col) {
-        cod = blk.row_layout->trees[static_cast<size_t>(row)].sectors;
-        dom = blk.col_layout->trees[static_cast<size_t>(col)].sectors;
+        const std::vector<Sector> &rs = blk.row_layout->trees[static_cast<size_t>(row)].sectors;
+        const std::vector<Sector> &cs = blk.col_layout->trees[static_cast<size_t>(col)].sectors;
         cplx coef(1.0, 0.0);
-        for (const Step &step : steps) {
-          if (step.kind == StepKind::exchange) {
-            const size_t p = static_cast<size_t>(step.position);
-            coef *= cat.R(cod[p], cod[p + 1])(0, 0);
-            std::swap(cod[p], cod[p + 1]);
-          } else if (step.kind == StepKind::to_domain) {
-            dom.push_back(cat.dual(cod.back()));
-            cod.pop_back();
-          } else {
-            cod.push_back(cat.dual(dom.back()));
-            dom.pop_back();
+        if (replay) {
+          cod = rs;
+          dom = cs;
+          for (const Step &step : steps) {
+            if (step.kind == StepKind::exchange) {
+              const size_t p = static_cast<size_t>(step.position);
+              coef *= cat.R(cod[p], cod[p + 1])(0, 0);
+              std::swap(cod[p], cod[p + 1]);
+            } else if (step.kind == StepKind::to_domain) {
+              dom.push_back(cat.dual(cod.back()));
+              cod.pop_back();
+            } else {
+              cod.push_back(cat.dual(dom.back()));
+              dom.pop_back();
+            }
+          }
+        } else {
+          cod.clear();
+          dom.clear();
+          for (int64_t j = 0; j < nlegs; ++j) {
+            const int64_t leg = perm[static_cast<size_t>(j)];
+            const Sector &s = leg < ncod0 ? rs[static_cast<size_t>(leg)] : cs[static_cast<size_t>(leg - ncod0)];
+            (j < ncod1 ? cod : dom).push_back(flips[static_cast<size_t>(j)] ? cat.dual(s) : s);
           }
         }
         const int64_t tb = T.find(root_of(cod), root_of(dom), blk.key.copy);
         if (tb < 0) continue;
         const BlockInfo &tblk = T.blocks()[static_cast<size_t>(tb)];
-        const int64_t trow = index_of(*tblk.row_layout, cod);
-        const int64_t tcol = index_of(*tblk.col_layout, dom);
+        const int64_t trow = tblk.row_layout->find_sectors(cod);
+        const int64_t tcol = tblk.col_layout->find_sectors(dom);
         if (trow < 0 || tcol < 0) continue;  // the dual sector is not on the leg (regroup.cpp:383)
-        comp[static_cast<size_t>(space.id(b, row, col))].emplace_back(target.id(tb, trow, tcol), coef);
+        const size_t at = static_cast<size_t>(space.id(b, row, col));
+        to[at] = target.id(tb, trow, tcol);
+        coefs[at] = coef;
       }
   }
-  return comp;
 }
 
 // Compose the steps from `source` and emit one tile per target tree pair.
@@ -411,97 +427,121 @@ void compose(const StructurePtr &source, const std::vector<Step> &steps, const s
              MovePlan &plan, bool per_block_groups) {
   const int64_t nlegs = source->nlegs();
   if (nlegs > TT_MAX_DIMS) throw std::invalid_argument("a leg move handles at most " + std::to_string(TT_MAX_DIMS) + " legs");
-  PairSpace space(source);
-  std::vector<Terms> comp;
-  std::vector<int64_t> perm = chain_permutation(static_cast<int64_t>(source->codomain().size()), nlegs, steps);
-  const StructurePtr *current = &source;
+  const std::vector<int64_t> perm = chain_permutation(static_cast<int64_t>(source->codomain().size()), nlegs, steps);
+  const StructurePtr &target = after.back();
+  const PairSpace source_space(source), target_space(target);
+  // Target-major contributions, each in source-pair order: the sources of target pair t are
+  // from_pair/from_coef[start[t], start[t + 1]).
+  std::vector<int64_t> start(static_cast<size_t>(target_space.total + 1), 0), from_pair;
+  std::vector<cplx> from_coef;
+  auto gather = [&](auto &&for_each_term) {
+    for_each_term([&](int64_t, int64_t t, const cplx &) { ++start[static_cast<size_t>(t + 1)]; });
+    for (size_t t = 1; t < start.size(); ++t) start[t] += start[t - 1];
+    from_pair.resize(static_cast<size_t>(start.back()));
+    from_coef.resize(static_cast<size_t>(start.back()));
+    std::vector<int64_t> fill(start.begin(), start.end() - 1);
+    for_each_term([&](int64_t i, int64_t t, const cplx &c) {
+      const size_t at = static_cast<size_t>(fill[static_cast<size_t>(t)]++);
+      from_pair[at] = i;
+      from_coef[at] = c;
+    });
+  };
+  auto nonzero = [](const cplx &c) { return c.real() != 0.0 || c.imag() != 0.0; };
   if (source->cat()->abelian()) {
-    PairSpace target_space(after.back());
-    comp = compose_abelian(source, steps, target_space);
-    space = std::move(target_space);
-    current = &after.back();
+    std::vector<int64_t> to;
+    std::vector<cplx> coefs;
+    compose_abelian(source, steps, perm, target_space, to, coefs);
+    gather([&](auto &&visit) {
+      for (int64_t i = 0; i < source_space.total; ++i)
+        if (to[static_cast<size_t>(i)] >= 0 && nonzero(coefs[static_cast<size_t>(i)]))
+          visit(i, to[static_cast<size_t>(i)], coefs[static_cast<size_t>(i)]);
+    });
   } else {
-    comp.resize(static_cast<size_t>(space.total));
+    PairSpace space(source);
+    std::vector<Terms> comp(static_cast<size_t>(space.total));
     for (int64_t i = 0; i < space.total; ++i) comp[static_cast<size_t>(i)].emplace_back(i, cplx(1.0, 0.0));
-  }
-  for (size_t k = 0; k < steps.size() && !source->cat()->abelian(); ++k) {
-    PairSpace next(after[k]);
-    const std::vector<Terms> map = step_map(space, next, steps[k]);
-    for (Terms &terms : comp) {
-      Terms out;
-      for (const auto &[pair, coef] : terms)
-        for (const auto &[target, c2] : map[static_cast<size_t>(pair)]) {
-          auto it = std::find_if(out.begin(), out.end(), [&](const auto &e) { return e.first == target; });
-          if (it == out.end())
-            out.emplace_back(target, coef * c2);
-          else
-            it->second += coef * c2;
-        }
-      terms = std::move(out);
+    for (size_t k = 0; k < steps.size(); ++k) {
+      PairSpace next(after[k]);
+      const std::vector<Terms> map = step_map(space, next, steps[k]);
+      for (Terms &terms : comp) {
+        Terms out;
+        for (const auto &[pair, coef] : terms)
+          for (const auto &[target_pair, c2] : map[static_cast<size_t>(pair)]) {
+            auto it = std::find_if(out.begin(), out.end(), [&](const auto &e) { return e.first == target_pair; });
+            if (it == out.end())
+              out.emplace_back(target_pair, coef * c2);
+            else
+              it->second += coef * c2;
+          }
+        terms = std::move(out);
+      }
+      space = std::move(next);
     }
-    space = std::move(next);
-    current = &after[k];
+    gather([&](auto &&visit) {
+      for (int64_t i = 0; i < static_cast<int64_t>(comp.size()); ++i)
+        for (const auto &[pair, coef] : comp[static_cast<size_t>(i)])
+          if (nonzero(coef)) visit(i, pair, coef);
+    });
   }
-  const StructurePtr &target = *current;
-  // Target-major contributions, each in source-pair order.
-  std::vector<Terms> contrib(static_cast<size_t>(space.total));
-  for (int64_t i = 0; i < static_cast<int64_t>(comp.size()); ++i)
-    for (const auto &[pair, coef] : comp[static_cast<size_t>(i)])
-      if (coef.real() != 0.0 || coef.imag() != 0.0) contrib[static_cast<size_t>(pair)].emplace_back(i, coef);
-  PairSpace source_space(source);
-  std::vector<int64_t> text, tstride, sext, sstride;
+  // Buffers reused across tiles: per source, its strides mapped onto the target axes (nlegs each),
+  // and after squeezing and merging, its strides per tile axis (nlegs each).
+  std::vector<int64_t> text, tstride, sext, sstride, soff, sstr, ext, dstr, src;
   plan.tasks.clear();
   plan.sources.clear();
+  plan.tasks.reserve(static_cast<size_t>(target_space.total));
+  plan.sources.reserve(from_pair.size());
+  const size_t L = static_cast<size_t>(nlegs);
   for (int64_t b = 0; b < static_cast<int64_t>(target->blocks().size()); ++b) {
     const BlockInfo &blk = target->blocks()[static_cast<size_t>(b)];
     const int64_t nrows = static_cast<int64_t>(blk.row_layout->trees.size());
     const int64_t ncols = static_cast<int64_t>(blk.col_layout->trees.size());
     for (int64_t row = 0; row < nrows; ++row)
       for (int64_t col = 0; col < ncols; ++col) {
-        const int64_t pair = space.id(b, row, col);
+        const int64_t pair = target_space.id(b, row, col);
         pair_extents(*target, b, row, col, text);
         const int64_t doff = pair_strides(*target, b, row, col, text, tstride);
-        const Terms &list = contrib[static_cast<size_t>(pair)];
-        // Per-source strides, mapped onto the target axes.
-        std::vector<std::vector<int64_t>> sstr;
-        std::vector<int64_t> soff;
-        for (const auto &[spair, coef] : list) {
+        const size_t first = static_cast<size_t>(start[static_cast<size_t>(pair)]);
+        const size_t count = static_cast<size_t>(start[static_cast<size_t>(pair + 1)]) - first;
+        soff.resize(count);
+        sstr.resize(count * L);
+        src.resize(count * L);
+        for (size_t s = 0; s < count; ++s) {
           int64_t sb, sr, sc;
-          source_space.decode(spair, &sb, &sr, &sc);
+          source_space.decode(from_pair[first + s], &sb, &sr, &sc);
           pair_extents(*source, sb, sr, sc, sext);
-          soff.push_back(pair_strides(*source, sb, sr, sc, sext, sstride));
-          std::vector<int64_t> mapped(static_cast<size_t>(nlegs));
-          for (int64_t j = 0; j < nlegs; ++j) {
-            const int64_t from_axis = perm[static_cast<size_t>(j)];
-            if (sext[static_cast<size_t>(from_axis)] != text[static_cast<size_t>(j)])
+          soff[s] = pair_strides(*source, sb, sr, sc, sext, sstride);
+          for (size_t j = 0; j < L; ++j) {
+            const size_t from_axis = static_cast<size_t>(perm[j]);
+            if (sext[from_axis] != text[j])
               throw std::logic_error("a leg moved across sides must carry the same multiplicity in a sector and its dual");
-            mapped[static_cast<size_t>(j)] = sstride[static_cast<size_t>(from_axis)];
+            sstr[s * L + j] = sstride[from_axis];
           }
-          sstr.push_back(std::move(mapped));
         }
         // Squeeze unit axes and merge axes that stay contiguous for the target and every source.
-        std::vector<int64_t> ext, dstr;
-        std::vector<std::vector<int64_t>> src(list.size());
-        for (int64_t j = 0; j < nlegs; ++j) {
-          const int64_t e = text[static_cast<size_t>(j)];
+        ext.clear();
+        dstr.clear();
+        for (size_t j = 0; j < L; ++j) {
+          const int64_t e = text[j];
           if (e == 1) continue;
-          bool merge = !ext.empty() && dstr.back() == tstride[static_cast<size_t>(j)] * e;
-          for (size_t s = 0; merge && s < list.size(); ++s) merge = src[s].back() == sstr[s][static_cast<size_t>(j)] * e;
+          bool merge = !ext.empty() && dstr.back() == tstride[j] * e;
+          const size_t d = merge ? ext.size() - 1 : ext.size();
+          for (size_t s = 0; merge && s < count; ++s) merge = src[s * L + d] == sstr[s * L + j] * e;
           if (merge) {
             ext.back() *= e;
-            dstr.back() = tstride[static_cast<size_t>(j)];
-            for (size_t s = 0; s < list.size(); ++s) src[s].back() = sstr[s][static_cast<size_t>(j)];
+            dstr.back() = tstride[j];
+            for (size_t s = 0; s < count; ++s) src[s * L + d] = sstr[s * L + j];
           } else {
+            const size_t fresh = ext.size();
             ext.push_back(e);
-            dstr.push_back(tstride[static_cast<size_t>(j)]);
-            for (size_t s = 0; s < list.size(); ++s) src[s].push_back(sstr[s][static_cast<size_t>(j)]);
+            dstr.push_back(tstride[j]);
+            for (size_t s = 0; s < count; ++s) src[s * L + fresh] = sstr[s * L + j];
           }
         }
         tt_gss_task task{};
         task.dst_offset = doff;
         task.ndim = static_cast<int32_t>(ext.size());
         task.first_source = static_cast<int32_t>(plan.sources.size());
-        task.num_sources = static_cast<int32_t>(list.size());
+        task.num_sources = static_cast<int32_t>(count);
         task.group = per_block_groups ? static_cast<int32_t>(b) : 0;
         for (size_t d = 0; d < ext.size(); ++d) {
           task.extent[d] = ext[d];
@@ -509,20 +549,20 @@ void compose(const StructurePtr &source, const std::vector<Step> &steps, const s
         }
         // Every address the tile touches stays inside its buffer (checked here, on the host, so a
         // planning mistake raises instead of reaching the device).
-        auto span = [&](int64_t base, const std::vector<int64_t> &strides, int64_t size, const char *what) {
+        auto span = [&](int64_t base, const int64_t *strides, int64_t size, const char *what) {
           int64_t hi = base;
           for (size_t d = 0; d < ext.size(); ++d) hi += (ext[d] - 1) * strides[d];
           if (base < 0 || hi >= size) throw std::logic_error(std::string("a leg move tile leaves its ") + what + " buffer");
         };
-        span(doff, dstr, target->size(), "target");
-        for (size_t s = 0; s < list.size(); ++s) span(soff[s], src[s], source->size(), "source");
+        span(doff, dstr.data(), target->size(), "target");
+        for (size_t s = 0; s < count; ++s) span(soff[s], src.data() + s * L, source->size(), "source");
         plan.tasks.push_back(task);
-        for (size_t s = 0; s < list.size(); ++s) {
+        for (size_t s = 0; s < count; ++s) {
           tt_gss_source source_box{};
           source_box.src_offset = soff[s];
-          for (size_t d = 0; d < src[s].size(); ++d) source_box.src_stride[d] = src[s][d];
-          source_box.coef_re = list[s].second.real();
-          source_box.coef_im = list[s].second.imag();
+          for (size_t d = 0; d < ext.size(); ++d) source_box.src_stride[d] = src[s * L + d];
+          source_box.coef_re = from_coef[first + s].real();
+          source_box.coef_im = from_coef[first + s].imag();
           plan.sources.push_back(source_box);
         }
       }

@@ -167,6 +167,12 @@ bool FusionTree::operator<(const FusionTree &o) const {
   return coupled < o.coupled;
 }
 
+size_t SectorListHash::operator()(const std::vector<Sector> &sectors) const {
+  size_t seed = 0;
+  for (const Sector &s : sectors) mix(seed, SectorHash{}(s));
+  return seed;
+}
+
 size_t TreeHash::operator()(const FusionTree &t) const {
   size_t seed = 0;
   for (const Sector &s : t.sectors) mix(seed, SectorHash{}(s));
@@ -284,6 +290,7 @@ LayoutPtr layout_of(const CategoryPtr &cat, const Legs &legs, const Sector &root
     }
     layout->offsets.push_back(layout->offsets.back() + rows);
     layout->index.emplace(layout->trees[t], static_cast<int64_t>(t));
+    if (cat->abelian()) layout->by_sectors.emplace(layout->trees[t].sectors, static_cast<int64_t>(t));
   }
   table.emplace(std::move(key), layout);
   return layout;

@@ -77,16 +77,26 @@ std::vector<Sector> reachable_charges(const Category &cat, const Legs &legs);
 std::vector<FusionTree> fusion_trees(const Category &cat, const Legs &legs, const Sector &coupled);
 FusionTree trimmed(const Category &cat, const FusionTree &tree);
 
+struct SectorListHash {
+  size_t operator()(const std::vector<Sector> &sectors) const;
+};
+
 struct SideLayout {
   std::vector<FusionTree> trees;
   std::vector<int64_t> offsets;   // size trees + 1
   std::vector<int64_t> mults;     // per tree, the multiplicity each leg carries (trees x legs)
   int64_t nlegs = 0;
   std::unordered_map<FusionTree, int64_t, TreeHash> index;
+  // Abelian categories only: a tree is fixed by its sector list (one outcome per fusion, no copies).
+  std::unordered_map<std::vector<Sector>, int64_t, SectorListHash> by_sectors;
   int64_t rows() const { return offsets.back(); }
   int64_t find(const FusionTree &t) const {
     auto f = index.find(t);
     return f == index.end() ? -1 : f->second;
+  }
+  int64_t find_sectors(const std::vector<Sector> &sectors) const {
+    auto f = by_sectors.find(sectors);
+    return f == by_sectors.end() ? -1 : f->second;
   }
 };
 
