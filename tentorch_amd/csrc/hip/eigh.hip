@@ -89,6 +89,44 @@ __device__ double wave_sum(double v) {
   return __hiloint2double(hi, lo);
 }
 
+// The problem a wave of a batched launch works on, when problem b owns count(n_b) consecutive
+// work items (workgroups, or waves for the leaves) in batch order.  Lane b of the wave loads problem
+// b (the table always holds kEigBatch entries, zeroed past the count), a prefix sum over the lanes'
+// counts and a ballot then give the problem: one memory round trip, where walking the table would
+// be a chain of dependent loads ahead of every launch's own.  All 64 lanes must take part.
+// Returns false past the last problem; `first` is the index of the problem's first work item.
+struct Located {
+  Problem pr;
+  int b, first;
+};
+
+static_assert(kEigBatch == 64, "one lane per problem of a batch");
+
+template <class Count>
+__device__ inline bool locate(const Batch *__restrict__ batch, int item, Count count, Located *out) {
+  const int lane = threadIdx.x & 63;
+  const Problem mine = batch->p[lane];
+  const int problems = batch->count;
+  int incl = lane < problems ? count(mine.n) : 0;
+#pragma unroll
+  for (int offset = 1; offset < 64; offset <<= 1) {
+    const int below = __shfl_up(incl, offset, 64);
+    if (lane >= offset) incl += below;
+  }
+  const int b = __popcll(__ballot(incl <= item));
+  if (b >= problems) return false;
+  out->b = b;
+  out->first = b > 0 ? __shfl(incl, b - 1, 64) : 0;
+  out->pr.matrix = __shfl(static_cast<long long>(mine.matrix), b, 64);
+  out->pr.values = __shfl(static_cast<long long>(mine.values), b, 64);
+  out->pr.work = __shfl(static_cast<long long>(mine.work), b, 64);
+  out->pr.n = __shfl(mine.n, b, 64);
+  out->pr.first_panel = __shfl(mine.first_panel, b, 64);
+  out->pr.first_leaf = __shfl(mine.first_leaf, b, 64);
+  out->pr.pad = 0;
+  return true;
+}
+
 // ---------------------------------------------------------------------------------------------
 // 1. Tridiagonalisation.  A is column-major n x n (symmetric, so either order), in place.
 //    Output: diag[n], off[n-1], tau[n-1]; reflector k is A[k+1.., k] with A[k+1, k] = 1.  The
@@ -132,15 +170,10 @@ __host__ __device__ inline int step_blocks(int n, int k) {
 
 __global__ __launch_bounds__(kStepThreads) void tridiag_step_kernel(const Batch *__restrict__ batch, double *__restrict__ matrices,
                                                                     double *__restrict__ work, int k) {
-  int b = 0, first = 0;
-  for (; b < batch->count; ++b) {
-    const int count = step_blocks(batch->p[b].n, k);
-    if (static_cast<int>(blockIdx.x) < first + count) break;
-    first += count;
-  }
-  if (b == batch->count) return;
-  const Problem pr = batch->p[b];
-  const int n = pr.n, block = static_cast<int>(blockIdx.x) - first;
+  Located at;
+  if (!locate(batch, static_cast<int>(blockIdx.x), [k](int order) { return step_blocks(order, k); }, &at)) return;
+  const Problem &pr = at.pr;
+  const int n = pr.n, block = static_cast<int>(blockIdx.x) - at.first;
   double *__restrict__ A = matrices + pr.matrix;
   double *diag = work + pr.work + 3 * int64_t(n) * n, *off = diag + n, *tau = off + n;
   double *vbuf = tau + n, *pbuf = vbuf + 2 * n;   // [2][n] each, by step parity
@@ -357,21 +390,21 @@ __global__ __launch_bounds__(kTriThreads) void tridiag_lds_kernel(const Batch *_
 //     with Wilkinson-type shifts and eigenvector accumulation, one wave per leaf.  Every lane runs
 //     the scalar recurrence on the wave's copy of the leaf in LDS; lane r applies the rotations to
 //     row r of the leaf's eigenvectors.  The eigenpairs are then sorted ascending and written to the
-//     leaf's diagonal block of Q, whose other entries in the leaf's rows are zeroed (and the whole
-//     rows of the second buffer too).
+//     leaf's diagonal block of Q, one entry per lane (the rest of Q was zeroed with the batch table).
+
+static_assert(kLeaf * kLeaf == 64, "a leaf's eigenvector block is written by one wave, one entry per lane");
 
 __global__ __launch_bounds__(kEigThreads) void leaf_kernel(const Batch *__restrict__ batch, double *__restrict__ values,
                                                             double *__restrict__ work, int32_t leaves) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int leaf = static_cast<int>(blockIdx.x) * kEigWaves + wave;
   __shared__ double dl[kEigWaves][kLeaf], el[kEigWaves][kLeaf], Z[kEigWaves][kLeaf][kLeaf + 1];
-  if (leaf >= leaves) return;
-  int b = 0;
-  while (b + 1 < batch->count && batch->p[b + 1].first_leaf <= leaf) ++b;
-  const Problem pr = batch->p[b];
+  Located at;
+  if (leaf >= leaves || !locate(batch, leaf, [](int order) { return (order + kLeaf - 1) / kLeaf; }, &at)) return;
+  const Problem &pr = at.pr;
   const int n = pr.n;
   const int lo = (leaf - pr.first_leaf) * kLeaf, hi = min(lo + kLeaf, n), m = hi - lo;
-  double *Q = work + pr.work, *Qt = Q + int64_t(n) * n;
+  double *Q = work + pr.work;
   const double *diag = Q + 3 * int64_t(n) * n, *off = diag + n;
   double *d = dl[wave], *e = el[wave];
   double (*z)[kLeaf + 1] = Z[wave];
@@ -449,12 +482,9 @@ __global__ __launch_bounds__(kEigThreads) void leaf_kernel(const Batch *__restri
   }
   double *lam = values + pr.values;
   if (lane < m) lam[lo + lane] = d[lane];
-  // the leaf's rows of both buffers: zero, then the leaf's eigenvectors in its diagonal block
-  for (int c = 0; c < n; ++c)
-    if (lane < m) {
-      Q[lo + lane + int64_t(c) * n] = (c >= lo && c < hi) ? z[lane][c - lo] : 0.0;
-      Qt[lo + lane + int64_t(c) * n] = 0.0;
-    }
+  // the leaf's eigenvectors in its diagonal block of Q (both buffers were zeroed with the batch table)
+  const int r = lane & (kLeaf - 1), c = lane / kLeaf;
+  if (r < m && c < m) Q[lo + r + int64_t(lo + c) * n] = z[r][c];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -517,19 +547,16 @@ __device__ inline DcScratch dc_scratch(double *base, int n) {
   return DcScratch{d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n, i, i + n, i + 2 * n, i + 3 * n};
 }
 
-// unit of workgroup `block` among the units of a level: problem index, lo (-1 past the batch)
+// unit of workgroup `block` among the units of a level: problem, lo (-1 past the batch)
 struct UnitRef {
   int b, lo;
+  Problem pr;
 };
 
-__device__ inline UnitRef find_unit(const Batch &batch, int width, int block) {
-  int first = 0;
-  for (int b = 0; b < batch.count; ++b) {
-    const int units = dc_units(batch.p[b].n, width);
-    if (block < first + units) return UnitRef{b, (block - first) * 2 * width};
-    first += units;
-  }
-  return UnitRef{-1, 0};
+__device__ inline UnitRef find_unit(const Batch *__restrict__ batch, int width, int block) {
+  Located at;
+  if (!locate(batch, block, [width](int order) { return dc_units(order, width); }, &at)) return UnitRef{-1, 0, Problem{}};
+  return UnitRef{at.b, (block - at.first) * 2 * width, at.pr};
 }
 
 // Exclusive prefix sum over the workgroup (kEigThreads threads) of one value per thread; the total
@@ -556,9 +583,9 @@ static_assert(kPerThread * kEigThreads == kEigMaxN, "four list elements per thre
 
 __global__ __launch_bounds__(kEigThreads) void dc_deflate_kernel(const Batch *__restrict__ batch, double *__restrict__ values,
                                                                   double *__restrict__ work, int width, int parity) {
-  const UnitRef u = find_unit(*batch, width, static_cast<int>(blockIdx.x));
+  const UnitRef u = find_unit(batch, width, static_cast<int>(blockIdx.x));
   if (u.b < 0) return;
-  const Problem pr = batch->p[u.b];
+  const Problem &pr = u.pr;
   const int n = pr.n, lo = u.lo;
   double *const base = work + pr.work;
   double *__restrict__ Q = base + (parity ? int64_t(n) * n : 0);
@@ -754,17 +781,13 @@ __device__ inline Secular secular_wave(const double (&d)[kPoleChunks], const dou
 }
 
 __global__ __launch_bounds__(kEigThreads) void dc_secular_kernel(const Batch *__restrict__ batch, double *__restrict__ work, int width) {
-  int b = 0, first = 0;
-  for (; b < batch->count; ++b) {
-    const int blocks = dc_root_blocks(batch->p[b].n, width);
-    if (static_cast<int>(blockIdx.x) < first + blocks) break;
-    first += blocks;
-  }
-  if (b == batch->count) return;
-  const Problem pr = batch->p[b];
+  Located at;
+  if (!locate(batch, static_cast<int>(blockIdx.x), [width](int order) { return dc_root_blocks(order, width); }, &at))
+    return;
+  const Problem &pr = at.pr;
   const int n = pr.n;
   const int units = dc_units(n, width), full = (2 * width + kEigWaves - 1) / kEigWaves;
-  int local = static_cast<int>(blockIdx.x) - first, unit = local / full;
+  int local = static_cast<int>(blockIdx.x) - at.first, unit = local / full;
   if (unit >= units - 1) {
     unit = units - 1;
     local -= (units - 1) * full;
@@ -848,9 +871,9 @@ __global__ __launch_bounds__(kEigThreads) void dc_secular_kernel(const Batch *__
 
 __global__ __launch_bounds__(kEigThreads) void dc_finish_kernel(const Batch *__restrict__ batch, double *__restrict__ values,
                                                                  double *__restrict__ work, int width) {
-  const UnitRef u = find_unit(*batch, width, static_cast<int>(blockIdx.x));
+  const UnitRef u = find_unit(batch, width, static_cast<int>(blockIdx.x));
   if (u.b < 0) return;
-  const Problem pr = batch->p[u.b];
+  const Problem &pr = u.pr;
   const int n = pr.n, lo = u.lo;
   const int mid = lo + width, hi = min(lo + 2 * width, n), s = hi - lo;
   if (mid >= n) return;   // a copy: its maps are written
@@ -945,15 +968,10 @@ __host__ __device__ inline int dc_tiles(int n, int width) {
 
 __global__ __launch_bounds__(kEigThreads) void dc_gemm_kernel(const Batch *__restrict__ batch, double *__restrict__ work, int width,
                                                                int parity) {
-  int b = 0, first = 0;
-  for (; b < batch->count; ++b) {
-    const int tiles = dc_tiles(batch->p[b].n, width);
-    if (static_cast<int>(blockIdx.x) < first + tiles) break;
-    first += tiles;
-  }
-  if (b == batch->count) return;
-  const Problem pr = batch->p[b];
-  const int n = pr.n;
+  Located at;
+  if (!locate(batch, static_cast<int>(blockIdx.x), [width](int order) { return dc_tiles(order, width); }, &at)) return;
+  const Problem &pr = at.pr;
+  const int n = pr.n, first = at.first;
   double *const base = work + pr.work;
   const double *__restrict__ Q = base + (parity ? int64_t(n) * n : 0);
   double *__restrict__ Qt = base + (parity ? 0 : int64_t(n) * n);
@@ -1060,15 +1078,10 @@ __host__ __device__ inline int reflector_blocks(int n) { return n > 1 ? (n - 1 +
 
 __global__ __launch_bounds__(kEigThreads) void block_t_kernel(const Batch *__restrict__ batch, const double *__restrict__ matrices,
                                                                double *__restrict__ work) {
-  int b = 0, first = 0;
-  for (; b < batch->count; ++b) {
-    const int count = reflector_blocks(batch->p[b].n);
-    if (static_cast<int>(blockIdx.x) < first + count) break;
-    first += count;
-  }
-  if (b == batch->count) return;
-  const Problem pr = batch->p[b];
-  const int n = pr.n, q = static_cast<int>(blockIdx.x) - first;
+  Located at;
+  if (!locate(batch, static_cast<int>(blockIdx.x), [](int order) { return reflector_blocks(order); }, &at)) return;
+  const Problem &pr = at.pr;
+  const int n = pr.n, q = static_cast<int>(blockIdx.x) - at.first;
   const int j = q * kRefl, m = min(kRefl, n - 1 - j);
   const double *__restrict__ A = matrices + pr.matrix;
   const double *tau = work + pr.work + 3 * int64_t(n) * n + 2 * n;
@@ -1119,11 +1132,11 @@ __global__ __launch_bounds__(kEigThreads) void block_t_kernel(const Batch *__res
 __global__ __launch_bounds__(kBackThreads) void back_kernel(const Batch *__restrict__ batch, const double *__restrict__ matrices,
                                                             double *__restrict__ vectors, const double *__restrict__ work) {
   // the problem this panel belongs to
-  int b = 0;
-  while (b + 1 < batch->count && batch->p[b + 1].first_panel <= static_cast<int>(blockIdx.x)) ++b;
-  const Problem pr = batch->p[b];
+  Located at;
+  if (!locate(batch, static_cast<int>(blockIdx.x), [](int order) { return (order + kPanel - 1) / kPanel; }, &at)) return;
+  const Problem &pr = at.pr;
   const int n = pr.n;
-  const int c0 = (static_cast<int>(blockIdx.x) - pr.first_panel) * kPanel;
+  const int c0 = (static_cast<int>(blockIdx.x) - at.first) * kPanel;
   const double *__restrict__ A = matrices + pr.matrix;
   const double *__restrict__ Z = work + pr.work + (dc_levels(n) & 1 ? int64_t(n) * n : 0);
   const double *__restrict__ Tall = work + pr.work + 3 * int64_t(n) * n + 7 * n;
@@ -1225,9 +1238,28 @@ __global__ __launch_bounds__(kBackThreads) void back_kernel(const Batch *__restr
 }
 
 // The batch table, written once per batch into the workspace, where every launch reads it (a
-// kernel argument of its size would be copied for each of the many launches of a batch).
-__global__ void store_batch_kernel(Batch batch, Batch *__restrict__ table) {
-  if (threadIdx.x == 0) *table = batch;
+// kernel argument of its size would be copied for each of the many launches of a batch).  The same
+// launch zeroes the two eigenvector buffers of every problem (2 n^2 doubles at its workspace, unused
+// until the leaves): the merges read whole diagonal blocks of them, whose parts outside the solved
+// sub-blocks must be zero.  Grid (kZeroBlocks, problems), coalesced 16-byte stores.
+constexpr int kZeroBlocks = 64;
+constexpr int kZeroThreads = 256;
+
+__global__ __launch_bounds__(kZeroThreads) void store_batch_kernel(Batch batch, Batch *__restrict__ table,
+                                                                  double *__restrict__ work) {
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *table = batch;
+  const Problem &pr = batch.p[blockIdx.y];
+  double *const Q = work + pr.work;
+  const int64_t count = 2 * int64_t(pr.n) * pr.n;
+  const int64_t stride = int64_t(kZeroBlocks) * kZeroThreads;
+  const int64_t first = int64_t(blockIdx.x) * kZeroThreads + threadIdx.x;
+  if ((reinterpret_cast<uintptr_t>(Q) & 15) == 0) {
+    double2 *const Q2 = reinterpret_cast<double2 *>(Q);
+    for (int64_t i = first; i < count / 2; i += stride) Q2[i] = double2{0.0, 0.0};
+    if (first == 0 && (count & 1)) Q[count - 1] = 0.0;
+  } else {
+    for (int64_t i = first; i < count; i += stride) Q[i] = 0.0;
+  }
 }
 
 constexpr int64_t kTableDoubles = (sizeof(Batch) + 255) / 256 * 32;   // one table, 256-byte aligned
@@ -1277,7 +1309,8 @@ int eigh_batched(const tt_eigh_problem *problems, int64_t count, double *matrice
       leaves += static_cast<int32_t>((p.n + kLeaf - 1) / kLeaf);
     }
     hipStream_t s = as_stream(stream);
-    hipLaunchKernelGGL(store_batch_kernel, dim3(1), dim3(64), 0, s, batch, table);
+    hipLaunchKernelGGL(store_batch_kernel, dim3(kZeroBlocks, batch.count), dim3(kZeroThreads), 0, s, batch, table,
+                       workspace);
     TT_LAUNCHED();
     int64_t largest = 0;
     for (int32_t b = 0; b < batch.count; ++b) largest = std::max<int64_t>(largest, batch.p[b].n);
