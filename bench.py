@@ -76,20 +76,20 @@ def time_breakdown(driver):
         return wrapper
 
     import tentorch_amd.linalg as lin
-    saved = (alg.lanczos, alg.svd, alg.einsum, alg.contract, driver._advance_left, driver._advance_right,
-             driver.state.move_center, driver.state.normalize, lin._gram_factors, lin._gram_factors_device,
+    saved = (alg.lanczos, alg._split, alg.einsum, alg.contract, driver._advance_left, driver._advance_right,
+             driver.state.move_center, driver.state.normalize_on_device, lin._gram_factors, lin._gram_factors_device,
              lin._gram_factors_torch)
     lin._gram_factors = timed('svd: gram eigh', lin._gram_factors)
     lin._gram_factors_device = timed('svd: gram eigh on device', lin._gram_factors_device)
     lin._gram_factors_torch = timed('svd: gram eigh by torch', lin._gram_factors_torch)
     alg.lanczos = timed('lanczos', alg.lanczos)
-    alg.svd = timed('svd', alg.svd)
+    alg._split = timed('svd', alg._split)
     alg.einsum = timed('einsum (head, tail, energy)', alg.einsum)
     alg.contract = timed('contract (two-site)', alg.contract)
     driver._advance_left = timed('environments', driver._advance_left)
     driver._advance_right = timed('environments', driver._advance_right)
     driver.state.move_center = timed('canonical moves', driver.state.move_center)
-    driver.state.normalize = timed('normalize', driver.state.normalize)
+    driver.state.normalize_on_device = timed('normalize', driver.state.normalize_on_device)
     try:
         torch.cuda.synchronize()
         start = time.perf_counter()
@@ -97,8 +97,8 @@ def time_breakdown(driver):
         torch.cuda.synchronize()
         totals['sweep'] = time.perf_counter() - start
     finally:
-        (alg.lanczos, alg.svd, alg.einsum, alg.contract, driver._advance_left, driver._advance_right,
-         driver.state.move_center, driver.state.normalize, lin._gram_factors, lin._gram_factors_device,
+        (alg.lanczos, alg._split, alg.einsum, alg.contract, driver._advance_left, driver._advance_right,
+         driver.state.move_center, driver.state.normalize_on_device, lin._gram_factors, lin._gram_factors_device,
          lin._gram_factors_torch) = saved
     return {k: round(v, 3) for k, v in totals.items()}
 

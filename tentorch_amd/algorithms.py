@@ -9,9 +9,10 @@ and `tail` out of the Krylov loop, and runs each matvec as one composed move and
 
 from typing import Dict, Literal
 
+import numpy as np
 import torch
 
-from .linalg import lanczos, svd
+from .linalg import _split, lanczos
 from .models import Model, heisenberg
 from .networks import MPS, identity
 from .tensor import Settings, Tensor, contract, domain_legs, einsum, inner
@@ -87,11 +88,16 @@ class DMRG:
         else:
             energy, vector = lanczos(action, two, self.n_iter)
         factor = 2 if direction == 'right' else 1
-        left, values, right = svd(vector, None, self.max_bond_dim, self.svd_tol, factor)
+        # the split's kept singular values come back on the host with it: the truncated weight and
+        # the zero-norm check need no further synchronisation, nor does the normalisation
+        left, values, right, kept = _split(vector, None, self.max_bond_dim, self.svd_tol, factor)
         self.state.tensors[at], self.state.tensors[at + 1] = left, right.place([0, 1])
         self.center = at + 1 if direction == 'right' else at
-        self.state.normalize()
-        return energy, 1.0 - float(torch.sum(values.abs() ** 2))
+        weight = float(np.sum(kept ** 2))
+        if weight == 0.0:
+            raise ValueError('the split kept no weight: the state cannot be normalised')
+        self.state.normalize_on_device()
+        return energy, 1.0 - weight
 
     def sweep(self):
         self.state.move_center(0)

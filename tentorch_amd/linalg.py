@@ -61,9 +61,12 @@ def _gram_factors(blocks, left_isometric: bool):
 
 
 def _gram_factors_device(blocks, left_isometric: bool):
-    '''For each (m x n) block: the isometric factor (m x r or n x r, r = min(m, n)) and the singular
-    values in descending order, from the device library's batched eigendecomposition of the Gram
-    matrices B B^T (left) or B^T B (right), all packed in one buffer.'''
+    '''For each (m x n) block: the isometric factor (m x r or n x r, r = min(m, n), a view) and the
+    singular values in descending order (host, FP64), from the device library's batched
+    eigendecomposition of the negated Gram matrices -B B^T (left) or -B^T B (right), all packed in
+    one buffer.  Negation turns the solver's ascending order into descending singular values, so the
+    leading r eigenvectors (rows of the output) are the factor's columns as they stand; the
+    eigenvalues come back to the host in one copy, the split's one synchronisation.'''
     if not blocks:
         return []
     sizes = [b.shape[0] if left_isometric else b.shape[1] for b in blocks]
@@ -72,29 +75,31 @@ def _gram_factors_device(blocks, left_isometric: bool):
     for b, g, at in zip(blocks, sizes, offsets):
         gram = packed[at:at + g * g].view(g, g)
         if left_isometric:
-            torch.matmul(b, b.transpose(0, 1), out=gram)
+            torch.addmm(gram, b, b.transpose(0, 1), beta=0.0, alpha=-1.0, out=gram)
         else:
-            torch.matmul(b.transpose(0, 1), b, out=gram)
+            torch.addmm(gram, b.transpose(0, 1), b, beta=0.0, alpha=-1.0, out=gram)
     values, vectors = _native.eigh_batched(packed, sizes)
+    negated = values.cpu().numpy()
     out = []
     value_at = 0
     for b, g, at in zip(blocks, sizes, offsets):
         r = min(b.shape)
-        lam = values[value_at + g - r:value_at + g].flip(0)
-        rows = vectors[at:at + g * g].view(g, g)[g - r:]      # eigenvector j is row j
-        out.append((rows.flip(0).transpose(0, 1), lam.clamp(min=0).sqrt()))
+        singular = np.sqrt(np.maximum(-negated[value_at:value_at + r], 0.0))
+        rows = vectors[at:at + g * g].view(g, g)[:r]      # eigenvector j is row j
+        out.append((rows.transpose(0, 1), singular))
         value_at += g
     return out
 
 
 def _gram_factors_torch(blocks, left_isometric: bool):
     '''For each (m x n) block: the isometric factor Q (m x r or n x r, r = min(m, n)) with the
-    singular values in descending order, from batched eigh of B B^T (left) or B^T B (right).  Blocks
-    are batched by Gram size in buckets that differ by at most 25 %; each Gram matrix is padded with
-    -1 on the diagonal, which the PSD spectrum never reaches, so the padding never enters the top r.'''
+    singular values in descending order (host, FP64), from batched eigh of -B B^T (left) or -B^T B
+    (right).  Blocks are batched by Gram size in buckets that differ by at most 25 %; each Gram
+    matrix is padded with +1 on the diagonal, above the non-positive spectrum of a negated Gram
+    matrix, so the padding never enters the leading r.'''
     grams = []
     for b in blocks:
-        grams.append(b @ b.transpose(0, 1) if left_isometric else b.transpose(0, 1) @ b)
+        grams.append(-(b @ b.transpose(0, 1)) if left_isometric else -(b.transpose(0, 1) @ b))
     order = sorted(range(len(blocks)), key=lambda i: grams[i].shape[0])
     out = [None] * len(blocks)
     at = 0
@@ -109,15 +114,14 @@ def _gram_factors_torch(blocks, left_isometric: bool):
             n = grams[i].shape[0]
             batch[slot, :n, :n] = grams[i]
             if n < size:
-                batch[slot, n:, n:] = -torch.eye(size - n, dtype=batch.dtype, device=batch.device)
+                batch[slot, n:, n:] = torch.eye(size - n, dtype=batch.dtype, device=batch.device)
         values, vectors = torch.linalg.eigh(batch)
+        negated = values.cpu().numpy()
         for slot, i in enumerate(members):
             m, n = blocks[i].shape
             r = min(m, n)
             g = grams[i].shape[0]
-            lam = values[slot, size - r:].flip(0)
-            vec = vectors[slot, :g, size - r:].flip(1)
-            out[i] = (vec, lam.clamp(min=0).sqrt())
+            out[i] = (vectors[slot, :g, :r], np.sqrt(np.maximum(-negated[slot, :r], 0.0)))
         at = end
     return out
 
@@ -126,6 +130,13 @@ def svd(tensor: Tensor, left: Optional[Sequence[int]] = None, max_bond: Optional
         absorb: int = 0) -> Tuple[Tensor, torch.Tensor, Tensor]:
     '''`split_over` (`svd.cpp:41-177`).  Returns (U, values, V) with U over (codomain | bond) and V over
     (bond | domain).'''
+    return _split(tensor, left, max_bond, tol, absorb)[:3]
+
+
+def _split(tensor: Tensor, left: Optional[Sequence[int]], max_bond: Optional[int], tol: float, absorb: int):
+    '''`svd`, plus the kept singular values on the host (FP64, in the order of `values`), which the
+    split reads back anyway to rank them: a caller that needs their sum needs no further
+    synchronisation.'''
     if absorb not in (0, 1, 2, 3):
         raise ValueError(f'absorb has to be 0, 1, 2 or 3, got {absorb}')
     if max_bond is not None and max_bond < 1:
@@ -136,16 +147,19 @@ def svd(tensor: Tensor, left: Optional[Sequence[int]] = None, max_bond: Optional
     items = [(key, block) for key, block in arranged.blocks().items() if block.numel() != 0]
     gram = SvdSettings.method == 'gram' and absorb in (1, 2) and not arranged.dtype.is_complex
     if gram:
-        # absorb 2 keeps U isometric (values go right), absorb 1 keeps V isometric
+        # absorb 2 keeps U isometric (values go right), absorb 1 keeps V isometric; the singular
+        # values arrive on the host
         left_iso = absorb == 2
         for (key, block), (q, v) in zip(items, _gram_factors([b for _, b in items], left_iso)):
             factors.append({'key': key, 'block': block, 'q': q, 'v': v, 'kept': 0, 'offset': 0})
+        host = np.concatenate([f['v'] for f in factors]) if factors else np.zeros(0)
+        sizes = [f['v'].size for f in factors]
     else:
         for key, block in items:
             u, v, vh = torch.linalg.svd(block, full_matrices=False)
             factors.append({'key': key, 'u': u, 'v': v, 'vh': vh, 'kept': 0, 'offset': 0})
-    host = torch.cat([f['v'] for f in factors]).double().cpu().numpy() if factors else np.zeros(0)
-    sizes = [f['v'].numel() for f in factors]
+        host = torch.cat([f['v'] for f in factors]).double().cpu().numpy() if factors else np.zeros(0)
+        sizes = [f['v'].numel() for f in factors]
     # The ranking keeps a leading run of every block, which needs each block's values descending
     # (svd.cpp:80).  torch.linalg.svd on ROCm can return values at the rounding level (~1e-14 of
     # the largest) slightly out of order; such a block is re-sorted with its singular vectors.
@@ -207,33 +221,44 @@ def svd(tensor: Tensor, left: Optional[Sequence[int]] = None, max_bond: Optional
     bond = _bond_leg(cat, mults)
     first = Tensor(cat, arranged.codomain, [bond], arranged.qtotal, arranged.dtype, arranged.device)
     second = Tensor(cat, [bond], arranged.domain, cat.trivial(), arranged.dtype, arranged.device)
-    values = torch.zeros(bond.multiplets(), dtype=arranged.dtype, device=arranged.device)
-    for f in factors:
+    kept_values = np.zeros(bond.multiplets())
+    values = None if gram else torch.zeros(bond.multiplets(), dtype=arranged.dtype, device=arranged.device)
+    starts = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64) if factors else np.zeros(1, dtype=np.int64)
+    for fi, f in enumerate(factors):
         k = f['kept']
         if not k:
             continue
         r, s, c = f['key']
-        v = f['v'][:k]
+        o = f['offset']
+        kept_values[placed[s] + o:placed[s] + o + k] = host[starts[fi]:starts[fi] + k]
         if gram:
+            # two library calls per block, the product written in place: with absorb 2 the rows
+            # o..o+k of V's block are one contiguous range
             q = f['q'][:, :k]
             if absorb == 2:
-                u, vh = q, q.transpose(0, 1) @ f['block']
+                first.block(r, s, c)[:, o:o + k].copy_(q)
+                torch.matmul(q.transpose(0, 1), f['block'], out=second.block(s, s)[o:o + k, :])
             else:
-                u, vh = f['block'] @ q, q.transpose(0, 1)
-        else:
-            u, vh = f['u'][:, :k], f['vh'][:k]
-            if absorb == 3:
-                u = u * v.sqrt()
-                vh = v.sqrt().unsqueeze(1) * vh
-            elif absorb == 1:
-                u = u * v
-            elif absorb == 2:
-                vh = v.unsqueeze(1) * vh
-        o = f['offset']
+                first.block(r, s, c)[:, o:o + k] = f['block'] @ q
+                second.block(s, s)[o:o + k, :].copy_(q.transpose(0, 1))
+            continue
+        v = f['v'][:k]
+        u, vh = f['u'][:, :k], f['vh'][:k]
+        if absorb == 3:
+            u = u * v.sqrt()
+            vh = v.sqrt().unsqueeze(1) * vh
+        elif absorb == 1:
+            u = u * v
+        elif absorb == 2:
+            vh = v.unsqueeze(1) * vh
         first.block(r, s, c)[:, o:o + k] = u
         second.block(s, s)[o:o + k, :] = vh
         values[placed[s] + o:placed[s] + o + k] = v.to(arranged.dtype)
-    return first, values, second
+    if gram:
+        # one copy from pinned memory, queued behind the products without waiting for them
+        staged = torch.from_numpy(kept_values).to(arranged.dtype).pin_memory()
+        values = staged.to(arranged.device, non_blocking=True)
+    return first, values, second, kept_values
 
 
 def svdvals(tensor: Tensor, left: Optional[Sequence[int]] = None) -> torch.Tensor:
@@ -286,9 +311,9 @@ def rq(tensor: Tensor, left: Optional[Sequence[int]] = None) -> Tuple[Tensor, Te
 
 
 class LanczosSettings:
-    '''`device` runs the iteration with the Krylov basis and its scalars on the GPU (one host
-    synchronisation per call besides the start-norm check); `host` follows the reference's loop with
-    one synchronisation per scalar product.'''
+    '''`device` runs the iteration with the Krylov basis and its scalars, the start norm included, on
+    the GPU (one host synchronisation per call); `host` follows the reference's loop with one
+    synchronisation per scalar product.'''
     mode = 'device'
 
 
@@ -313,19 +338,21 @@ def _lanczos_device(action, start: Tensor, n_iter: int, tol: float) -> Tuple[flo
     then runs two modified Gram-Schmidt passes (`lanczos.py:36-45`); the first full pass removes the
     same components, so the two agree up to rounding.  The alphas and betas are read back once; the
     break at beta < tol (`lanczos.py:46-48`) is applied to what was read.'''
-    scale = norm(start)
-    if scale == 0.0:
-        raise ValueError('Lanczos needs a starting vector of non-zero norm.')
     size_n = start.structure.size
     device = start.device
     V = torch.empty((n_iter + 1, size_n), dtype=torch.float64, device=device)
-    # alphas, betas, the step's scratch (2 (k+1) + 1), the Ritz coefficients, two work scalars
+    # alphas, betas, the step's scratch (2 (k+1) + 1), the Ritz coefficients, four work scalars
     scalars = torch.zeros(5 * n_iter + 8, dtype=torch.float64, device=device)
     alphas, betas = scalars[:n_iter], scalars[n_iter:2 * n_iter]
     scratch = scalars[2 * n_iter:4 * n_iter + 3]
     coef = scalars[4 * n_iter + 3:5 * n_iter + 4]
     work = scalars[5 * n_iter + 4:]
-    _native.axpby(torch.float64, 1.0 / scale, start.data, 0.0, V[0])
+    # v_0 = start / |start| with the norm kept on the device (the dot of `norm`, then a correctly
+    # rounded square root and reciprocal, as on the host); a zero start is refused once the
+    # scalars are read back
+    _native.dot(torch.float64, start.data, start.data, work[2:3])
+    torch.reciprocal(torch.sqrt(work[2:3]), out=work[3:4])
+    _native.axpby(torch.float64, 0.0, start.data, 0.0, V[0], alpha_dev=work[3:4])
     for k in range(n_iter):
         vector = Tensor._wrap(start.structure, V[k])
         image = action(vector)
@@ -333,8 +360,10 @@ def _lanczos_device(action, start: Tensor, n_iter: int, tol: float) -> Tuple[flo
             raise ValueError('the action has to map the legs of the start vector to themselves')
         w = image.data if image.data.data_ptr() != V[k].data_ptr() else image.data.clone()
         _native.lanczos_orthonormalize(V, k, w, scratch, alphas[k:k + 1], betas[k:k + 1])
-    host = scalars[:2 * n_iter].cpu().numpy()
-    a, b = host[:n_iter], host[n_iter:]
+    host = scalars.cpu().numpy()
+    if host[5 * n_iter + 6] == 0.0:
+        raise ValueError('Lanczos needs a starting vector of non-zero norm.')
+    a, b = host[:n_iter], host[n_iter:2 * n_iter]
     size = n_iter
     for k in range(n_iter):
         if b[k] < tol:

@@ -13,6 +13,7 @@ from typing import Optional, Sequence
 import numpy as np
 import torch
 
+from . import _native
 from .symmetry import Leg, fusion_trees, reachable_charges
 from .tensor import Tensor, block_keys, contract, domain_legs, einsum, inner
 from .linalg import qr, rq
@@ -177,6 +178,18 @@ class MPS:
 
     def normalize(self):
         self.tensors[self.center] /= self.norm()
+
+    def normalize_on_device(self):
+        '''`normalize` with the norm left on the device: the same sum of squares, square root and
+        reciprocal as `normalize` (both correctly rounded), then the same scaling kernel, without a
+        synchronisation.  A zero norm is not checked here: the caller knows the weight it kept.'''
+        t = self.tensors[self.center]
+        if t.dtype.is_complex:
+            self.normalize()
+            return
+        scale = torch.reciprocal(torch.sqrt((t.data.abs() ** 2).sum()))
+        _native.axpby(t.dtype, 0.0, None, 0.0, t.data, beta_dev=scale)
+        t._version += 1
 
     def expectation(self, operator) -> complex:
         '''`MPS.expectation` (`mps.py:176-195`).'''

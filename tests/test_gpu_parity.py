@@ -199,10 +199,13 @@ def test_complex_recoupling_on_a_real_tensor_is_refused():
         tt.einsum('ABCd,Be->ACde', x, y)
 
 
-def test_split_by_the_device_eigensolver_matches_the_exact_svd():
-    '''The DMRG split (absorb 2: U isometric, values into V) through the batched device eigensolver
-    and through one exact SVD per block: the same singular values and the same reconstruction.'''
-    from tentorch_amd.linalg import SvdSettings
+@pytest.mark.parametrize('absorb', [2, 1])
+def test_split_by_the_device_eigensolver_matches_the_exact_svd(absorb):
+    '''The DMRG split (absorb 2: U isometric, values into V; absorb 1: V isometric, values into U)
+    through the batched device eigensolver and through one exact SVD per block: the same singular
+    values, the same reconstruction, the isometric factor orthonormal, and the kept values the
+    split hands back on the host equal to the device ones.'''
+    from tentorch_amd.linalg import SvdSettings, _split
     rng = np.random.default_rng(31)
     bond = tt.Leg(tt.u1, [(-2,), (0,), (2,)], [9, 14, 9])
     phys = tt.Leg(tt.u1, [(-1,), (1,)], [1, 1])
@@ -211,16 +214,22 @@ def test_split_by_the_device_eigensolver_matches_the_exact_svd():
     for method, eigh in (('gram', 'device'), ('gram', 'torch'), ('exact', 'device')):
         SvdSettings.method, SvdSettings.eigh = method, eigh
         try:
-            results[(method, eigh)] = tt.svd(two, None, 20, 1e-14, 2)
+            results[(method, eigh)] = _split(two, None, 20, 1e-14, absorb)
         finally:
             SvdSettings.method, SvdSettings.eigh = 'gram', 'device'
-    ref_u, ref_s, ref_v = results[('exact', 'device')]
+    ref_u, ref_s, ref_v, _ = results[('exact', 'device')]
     ref_rec = tt.contract(ref_u, ref_v, tt.domain_legs(ref_u), [0]).data
-    for key in (('gram', 'device'), ('gram', 'torch')):
-        u, s, v = results[key]
+    for key in (('gram', 'device'), ('gram', 'torch'), ('exact', 'device')):
+        u, s, v, kept = results[key]
+        assert np.array_equal(kept, s.cpu().numpy()), key
         assert torch.allclose(s.sort().values, ref_s.sort().values, rtol=1e-10, atol=1e-13), key
         rec = tt.contract(u, v, tt.domain_legs(u), [0]).data
         assert (rec - ref_rec).abs().max() <= 1e-12 * ref_rec.abs().max(), key
+        # the isometric factor: orthonormal columns of U's blocks (absorb 2) or rows of V's (absorb 1)
+        for block in (u if absorb == 2 else v).blocks().values():
+            m = block if absorb == 2 else block.transpose(0, 1)
+            eye = torch.eye(m.shape[1], dtype=m.dtype, device=m.device)
+            assert (m.transpose(0, 1) @ m - eye).abs().max() <= 1e-12, key
 
 
 def test_exact_split_reorders_unsorted_singular_values(monkeypatch):
