@@ -37,20 +37,27 @@ int launch(Kernel kernel, dim3 grid, dim3 block, uint32_t lds, hipStream_t strea
   return 0;
 }
 
-// Allocates `bytes` of device memory at *where and copies `data` there through a pinned staging
-// buffer on the library's own non-blocking upload stream, waiting for that copy alone: a plan
-// upload neither drains nor waits for the work queued on the caller's streams.
-int upload(void **where, const void *data, size_t bytes);
-
-// Several host tables copied into one device allocation (each at a 256-byte aligned offset) with
-// one allocation, one copy and one synchronisation of the upload stream: *block receives the
-// allocation (to be freed as a whole), each *table.where its table's address (null when empty).
+// Several host tables of one plan copied to the device together (each at a 256-byte aligned
+// offset): *block receives their device range (handed back with release_tables), each
+// *table.where its table's address (null when empty).  The range is carved from a device arena
+// of large chunks (no allocation per plan) and filled by an asynchronous copy from a pinned ring
+// on the library's own upload stream, which the host does not wait for: every launch that reads
+// plan tables calls await_uploads on its stream first.
 struct Table {
   void **where;
   const void *data;
   size_t bytes;
 };
 int upload_tables(void **block, std::initializer_list<Table> tables);
+
+// Returns a plan's table range to the arena; a chunk that no live plan uses any more, and that
+// takes no new tables, is freed.
+void release_tables(void *block);
+
+// Orders `stream` after every table upload issued so far on the current device: nothing when
+// they have completed, else a device-side wait on the upload stream's event (a host wait for the
+// uploads when `stream` is being captured into a graph).
+int await_uploads(hipStream_t stream);
 
 inline hipStream_t as_stream(void *stream) { return reinterpret_cast<hipStream_t>(stream); }
 

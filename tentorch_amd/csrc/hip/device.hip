@@ -155,7 +155,7 @@ void division_constants(uint32_t d, uint64_t *magic, uint32_t *shift) {
 }  // namespace
 
 struct tt_gss_plan {
-  void *block = nullptr;   // the one device allocation holding the three tables
+  void *block = nullptr;   // the plan's range of the table arena, holding its three tables
   DevTask *tasks = nullptr;
   tt_gss_source *sources = nullptr;
   GssChunk *chunks = nullptr;
@@ -234,7 +234,7 @@ extern "C" int tt_gss_plan_create(const tt_gss_task *tasks, int64_t num_tasks, c
 
 extern "C" int tt_gss_plan_destroy(tt_gss_plan *plan) {
   if (!plan) return 0;
-  if (plan->block) (void)hipFree(plan->block);
+  ttdev::release_tables(plan->block);
   delete plan;
   return 0;
 }
@@ -250,6 +250,7 @@ namespace {
 template <typename TS, typename T>
 int gss_launch(const tt_gss_plan *plan, const void *src, void *dst, hipStream_t stream, int32_t first, int32_t count) {
   if (count <= 0) return 0;
+  if (int rc = ttdev::await_uploads(stream)) return rc;
   if (int rc = ttdev::launch(gss_kernel<TS, T>, dim3(count), dim3(kGssThreads), 0, stream, plan->chunks, plan->tasks,
                              plan->sources, static_cast<const TS *>(src), static_cast<T *>(dst), first))
     return rc;
@@ -770,62 +771,141 @@ void ttdev::note_launch() { g_launches.fetch_add(1, std::memory_order_relaxed); 
 
 namespace {
 
+// One arena chunk of plan tables: bump-allocated, freed when no live plan uses it and it no longer
+// takes new tables.
+struct ArenaChunk {
+  char *base = nullptr;
+  size_t capacity = 0, used = 0;
+  int64_t live = 0;
+};
+
+// Per device: the upload stream, the event recorded after its latest copy, the pinned staging ring,
+// the arena chunks (the last takes new tables), and how many uploads were issued and were last
+// seen complete.
 struct Uploader {
   hipStream_t stream = nullptr;
-  void *staging = nullptr;  // pinned host memory
-  size_t capacity = 0;
+  hipEvent_t done = nullptr;
+  unsigned char *staging = nullptr;
+  size_t staging_capacity = 0, staging_used = 0;
+  std::vector<ArenaChunk> chunks;
+  uint64_t issued = 0, completed = 0;
 };
 std::mutex g_upload_mutex;
 std::vector<Uploader> g_uploaders;  // one per device
 
-}  // namespace
+constexpr size_t kChunkBytes = size_t(16) << 20;
+constexpr size_t kStagingBytes = size_t(8) << 20;
+constexpr size_t kTableAlign = 256;
 
-int ttdev::upload(void **where, const void *data, size_t bytes) {
-  if (!bytes) return 0;
-  TT_HIP(hipMalloc(where, bytes));
-  int device = 0;
-  TT_HIP(hipGetDevice(&device));
-  std::lock_guard<std::mutex> lock(g_upload_mutex);
-  if (g_uploaders.size() <= static_cast<size_t>(device)) g_uploaders.resize(static_cast<size_t>(device) + 1);
-  Uploader &u = g_uploaders[static_cast<size_t>(device)];
-  if (!u.stream) TT_HIP(hipStreamCreateWithFlags(&u.stream, hipStreamNonBlocking));
-  if (u.capacity < bytes) {
-    if (u.staging) TT_HIP(hipHostFree(u.staging));
-    u.staging = nullptr;
-    u.capacity = 0;
-    const size_t grown = std::max<size_t>(bytes, size_t(1) << 20);
-    TT_HIP(hipHostMalloc(&u.staging, grown, hipHostMallocDefault));
-    u.capacity = grown;
-  }
-  std::memcpy(u.staging, data, bytes);
-  TT_HIP(hipMemcpyAsync(*where, u.staging, bytes, hipMemcpyHostToDevice, u.stream));
-  TT_HIP(hipStreamSynchronize(u.stream));
-  return 0;
-}
+size_t aligned(size_t bytes) { return (bytes + kTableAlign - 1) / kTableAlign * kTableAlign; }
+
+}  // namespace
 
 int ttdev::upload_tables(void **block, std::initializer_list<Table> tables) {
   *block = nullptr;
   size_t total = 0;
   for (const Table &t : tables) {
     *t.where = nullptr;
-    total = (total + 255) / 256 * 256 + t.bytes;
+    total = aligned(total) + t.bytes;
   }
   if (!total) return 0;
-  std::vector<unsigned char> packed(total);
-  size_t at = 0;
+  int device = 0;
+  TT_HIP(hipGetDevice(&device));
+  std::lock_guard<std::mutex> lock(g_upload_mutex);
+  if (g_uploaders.size() <= static_cast<size_t>(device)) g_uploaders.resize(static_cast<size_t>(device) + 1);
+  Uploader &u = g_uploaders[static_cast<size_t>(device)];
+  if (!u.stream) TT_HIP(hipStreamCreateWithFlags(&u.stream, hipStreamNonBlocking));
+  if (!u.done) TT_HIP(hipEventCreateWithFlags(&u.done, hipEventDisableTiming));
+  // device range: the current chunk, or a new one
+  const size_t need = aligned(total);
+  if (u.chunks.empty() || u.chunks.back().capacity - u.chunks.back().used < need) {
+    ArenaChunk chunk;
+    chunk.capacity = std::max(kChunkBytes, need);
+    TT_HIP(hipMalloc(reinterpret_cast<void **>(&chunk.base), chunk.capacity));
+    // the chunk that stops taking tables goes now if no live plan uses it
+    if (!u.chunks.empty() && u.chunks.back().live == 0) {
+      TT_HIP(hipFree(u.chunks.back().base));
+      u.chunks.pop_back();
+    }
+    u.chunks.push_back(chunk);
+  }
+  ArenaChunk &chunk = u.chunks.back();
+  char *const target = chunk.base + chunk.used;
+  // staging: the rest of the ring, or (after the copies in flight have finished) its front
+  if (u.staging_capacity < total) {
+    TT_HIP(hipStreamSynchronize(u.stream));
+    if (u.staging) TT_HIP(hipHostFree(u.staging));
+    u.staging = nullptr;
+    u.staging_capacity = u.staging_used = 0;
+    const size_t grown = std::max(kStagingBytes, need);
+    TT_HIP(hipHostMalloc(reinterpret_cast<void **>(&u.staging), grown, hipHostMallocDefault));
+    u.staging_capacity = grown;
+  } else if (u.staging_capacity - u.staging_used < total) {
+    TT_HIP(hipStreamSynchronize(u.stream));
+    u.staging_used = 0;
+  }
+  unsigned char *const stage = u.staging + u.staging_used;
   std::vector<size_t> offsets;
+  size_t at = 0;
   for (const Table &t : tables) {
-    at = (at + 255) / 256 * 256;
+    at = aligned(at);
     offsets.push_back(at);
-    if (t.bytes) std::memcpy(packed.data() + at, t.data, t.bytes);
+    if (t.bytes) std::memcpy(stage + at, t.data, t.bytes);
     at += t.bytes;
   }
-  if (upload(block, packed.data(), total)) return -1;
+  TT_HIP(hipMemcpyAsync(target, stage, total, hipMemcpyHostToDevice, u.stream));
+  TT_HIP(hipEventRecord(u.done, u.stream));
+  u.staging_used = std::min(u.staging_capacity, u.staging_used + need);
+  chunk.used += need;
+  chunk.live += 1;
+  u.issued += 1;
+  *block = target;
   size_t k = 0;
   for (const Table &t : tables) {
-    if (t.bytes) *t.where = static_cast<unsigned char *>(*block) + offsets[k];
+    if (t.bytes) *t.where = target + offsets[k];
     ++k;
   }
+  return 0;
+}
+
+void ttdev::release_tables(void *block) {
+  if (!block) return;
+  const char *const p = static_cast<const char *>(block);
+  std::lock_guard<std::mutex> lock(g_upload_mutex);
+  for (Uploader &u : g_uploaders)
+    for (size_t c = 0; c < u.chunks.size(); ++c) {
+      ArenaChunk &chunk = u.chunks[c];
+      if (p < chunk.base || p >= chunk.base + chunk.capacity) continue;
+      if (--chunk.live == 0 && c + 1 < u.chunks.size()) {
+        (void)hipFree(chunk.base);   // waits for the device, so no kernel still reads these tables
+        u.chunks.erase(u.chunks.begin() + static_cast<std::ptrdiff_t>(c));
+      }
+      return;
+    }
+}
+
+int ttdev::await_uploads(hipStream_t stream) {
+  int device = 0;
+  TT_HIP(hipGetDevice(&device));
+  std::lock_guard<std::mutex> lock(g_upload_mutex);
+  if (g_uploaders.size() <= static_cast<size_t>(device)) return 0;
+  Uploader &u = g_uploaders[static_cast<size_t>(device)];
+  if (u.completed == u.issued) return 0;
+  hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+  TT_HIP(hipStreamIsCapturing(stream, &capture));
+  if (capture != hipStreamCaptureStatusNone) {
+    // a graph cannot wait on an event recorded outside it: finish the uploads on the host
+    TT_HIP(hipStreamSynchronize(u.stream));
+    u.completed = u.issued;
+    return 0;
+  }
+  const hipError_t query = hipEventQuery(u.done);
+  if (query == hipSuccess) {
+    u.completed = u.issued;
+    return 0;
+  }
+  if (query != hipErrorNotReady) return fail(std::string("upload event query failed: ") + hipGetErrorString(query));
+  TT_HIP(hipStreamWaitEvent(stream, u.done, 0));
   return 0;
 }
 

@@ -312,7 +312,7 @@ int choose_shape(const tt_gemm_group *groups, int64_t num_groups, const tt_gemm_
 }  // namespace
 
 struct tt_ggemm_plan {
-  void *block = nullptr;   // the one device allocation holding the three tables
+  void *block = nullptr;   // the plan's range of the table arena, holding its three tables
   tt_gemm_group *groups = nullptr;
   tt_gemm_term *terms = nullptr;
   GemmTile *tiles = nullptr;
@@ -380,7 +380,7 @@ extern "C" int tt_ggemm_plan_create(const tt_gemm_group *groups, int64_t num_gro
 
 extern "C" int tt_ggemm_plan_destroy(tt_ggemm_plan *plan) {
   if (!plan) return 0;
-  if (plan->block) (void)hipFree(plan->block);
+  ttdev::release_tables(plan->block);
   delete plan;
   return 0;
 }
@@ -400,6 +400,7 @@ template <typename TA, typename TB, typename TO, typename TC, int BM, int BN, in
 int ggemm_launch(const tt_ggemm_plan *plan, const void *a, const void *b, void *c, hipStream_t stream, int64_t grid,
                  int32_t plain_group, int32_t plain_term, int32_t accumulate) {
   if (grid <= 0) return 0;
+  if (int rc = ttdev::await_uploads(stream)) return rc;
   if (int rc = ttdev::launch(ggemm_kernel<TA, TB, TO, TC, BM, BN, BK>, dim3(static_cast<uint32_t>(grid)),
                              dim3(kGemmThreads), 0, stream, plan->tiles, plan->groups, plan->terms,
                              static_cast<const TA *>(a), static_cast<const TB *>(b), static_cast<TO *>(c), plain_group,
