@@ -5,6 +5,7 @@
 // autotuner through tt_ggemm_plan_create_config).  gfx950 only.
 #include <algorithm>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -95,7 +96,7 @@ constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 // A BM x BN tile per workgroup: four waves in a 2 x 2 grid, each wave (BM/2) x (BN/2) as
 // 16 x 16 x 4 MFMA tiles.  Operands are staged through LDS (A transposed to k-major) with the next
-// k-slab prefetched into registers while the current one is multiplied; one barrier per slab.
+// two k-slabs in flight into registers while the current one is multiplied; one barrier per slab.
 template <typename TA, typename TB, typename TO, typename TC, int BM, int BN, int BK>
 __global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__restrict__ tiles,
                                                              const tt_gemm_group *__restrict__ groups,
@@ -150,21 +151,22 @@ __global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__r
   int64_t k0 = 0;
   while (term < t_end && terms[term].k <= 0) ++term;
 
-  TC ra[A_PER], rb[B_PER];
-  auto load = [&](int32_t t, int64_t kb) {
+  // Slab (t, kb) of the sum into registers: A element (row a_row, k a_k + j), B element (k b_k,
+  // column b_col + j).
+  auto load = [&](int32_t t, int64_t kb, TC (&xa)[A_PER], TC (&xb)[B_PER]) {
     const tt_gemm_term tm = terms[t];
     const int64_t K = tm.k;
     const int64_t gr = m0 + a_row;
 #pragma unroll
     for (int j = 0; j < A_PER; ++j) {
       const int64_t kk = kb + a_k + j;
-      ra[j] = (gr < m && kk < K) ? Conv<TC>::from(A[tm.a_offset + gr * K + kk]) : Conv<TC>::zero();
+      xa[j] = (gr < m && kk < K) ? Conv<TC>::from(A[tm.a_offset + gr * K + kk]) : Conv<TC>::zero();
     }
     const int64_t kr = kb + b_k;
 #pragma unroll
     for (int j = 0; j < B_PER; ++j) {
       const int64_t gc = n0 + b_col + j;
-      rb[j] = (kr < K && gc < n) ? Conv<TC>::from(B[tm.b_offset + kr * n + gc]) : Conv<TC>::zero();
+      xb[j] = (kr < K && gc < n) ? Conv<TC>::from(B[tm.b_offset + kr * n + gc]) : Conv<TC>::zero();
     }
   };
   auto advance = [&](int32_t &t, int64_t &kb) {
@@ -176,18 +178,36 @@ __global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__r
     }
   };
 
+  // Two slabs in flight: while slab s is multiplied from LDS, slabs s + 1 and s + 2 are in the two
+  // register sets, which alternate roles from one slab to the next (the loop body is unrolled twice
+  // so each set stays in fixed registers).  (t1, k1) is the slab after the current one.  Complex
+  // tiles keep one slab in flight: their largest tile has no registers left for a second.
+  constexpr bool kTwoAhead = !std::is_same<TC, cdouble>::value;
+  TC ra0[A_PER], rb0[B_PER], ra1[A_PER], rb1[B_PER];
+  int32_t t1 = term;
+  int64_t k1 = k0;
+  if (term < t_end) {
+    load(term, k0, ra0, rb0);
+    advance(t1, k1);
+    if constexpr (kTwoAhead) {
+      if (t1 < t_end) load(t1, k1, ra1, rb1);
+    }
+  }
   int buf = 0;
-  if (term < t_end) load(term, k0);
-  while (term < t_end) {
+  auto step = [&](TC (&xa)[A_PER], TC (&xb)[B_PER]) {
 #pragma unroll
-    for (int j = 0; j < A_PER; ++j) As[buf][a_k + j][a_row] = ra[j];
+    for (int j = 0; j < A_PER; ++j) As[buf][a_k + j][a_row] = xa[j];
 #pragma unroll
-    for (int j = 0; j < B_PER; ++j) Bs[buf][b_k][b_col + j] = rb[j];
+    for (int j = 0; j < B_PER; ++j) Bs[buf][b_k][b_col + j] = xb[j];
     __syncthreads();
-    int32_t next_term = term;
-    int64_t next_k = k0;
-    advance(next_term, next_k);
-    if (next_term < t_end) load(next_term, next_k);
+    int32_t t2 = t1;
+    int64_t k2 = k1;
+    if (t2 < t_end) advance(t2, k2);
+    if constexpr (kTwoAhead) {
+      if (t2 < t_end) load(t2, k2, xa, xb);
+    } else {
+      if (t1 < t_end) load(t1, k1, xa, xb);
+    }
 #pragma unroll
     for (int kk = 0; kk < BK / 4; ++kk) {
       const int k = kk * 4 + lk;
@@ -201,9 +221,18 @@ __global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__r
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = M::step(af[i], bf[j], acc[i][j]);
     }
-    term = next_term;
-    k0 = next_k;
+    term = t1;
+    k0 = k1;
+    t1 = t2;
+    k1 = k2;
     buf ^= 1;
+  };
+  while (term < t_end) {
+    step(ra0, rb0);
+    if constexpr (kTwoAhead) {
+      if (term >= t_end) break;
+      step(ra1, rb1);
+    }
   }
 
 #pragma unroll
