@@ -96,7 +96,8 @@ constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 // A BM x BN tile per workgroup: four waves in a 2 x 2 grid, each wave (BM/2) x (BN/2) as
 // 16 x 16 x 4 MFMA tiles.  Operands are staged through LDS (A transposed to k-major) with the next
-// two k-slabs in flight into registers while the current one is multiplied; one barrier per slab.
+// k-slab (two for 32 x 32 tiles) in flight into registers while the current one is multiplied; one
+// barrier per slab.
 template <typename TA, typename TB, typename TO, typename TC, int BM, int BN, int BK>
 __global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__restrict__ tiles,
                                                              const tt_gemm_group *__restrict__ groups,
@@ -180,9 +181,10 @@ __global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__r
 
   // Two slabs in flight: while slab s is multiplied from LDS, slabs s + 1 and s + 2 are in the two
   // register sets, which alternate roles from one slab to the next (the loop body is unrolled twice
-  // so each set stays in fixed registers).  (t1, k1) is the slab after the current one.  Complex
-  // tiles keep one slab in flight: their largest tile has no registers left for a second.
-  constexpr bool kTwoAhead = !std::is_same<TC, cdouble>::value;
+  // so each set stays in fixed registers).  (t1, k1) is the slab after the current one.  Only the
+  // 32 x 32 real tiles keep two slabs in flight: on larger tiles the second register set costs half
+  // the occupancy (measured slower), and complex tiles have no registers left for it.
+  constexpr bool kTwoAhead = !std::is_same<TC, cdouble>::value && BM * BN <= 32 * 32;
   TC ra0[A_PER], rb0[B_PER], ra1[A_PER], rb1[B_PER];
   int32_t t1 = term;
   int64_t k1 = k0;

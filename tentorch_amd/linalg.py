@@ -375,7 +375,8 @@ def _lanczos_device(action, start: Tensor, n_iter: int, tol: float) -> Tuple[flo
     for step in range(size - 1):
         matrix[step, step + 1] = matrix[step + 1, step] = b[step]
     values, vectors = np.linalg.eigh(matrix)
-    coef[:size].copy_(torch.from_numpy(vectors[:, 0].copy()))
+    # from pinned memory, queued without waiting (a pageable copy would synchronise)
+    coef[:size].copy_(torch.from_numpy(vectors[:, 0].copy()).pin_memory(), non_blocking=True)
     out = torch.zeros(size_n, dtype=torch.float64, device=device)
     _native.multiaxpy(V, size, coef, 1.0, out)
     _native.dot(torch.float64, out, out, work[0:1])
