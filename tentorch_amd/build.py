@@ -24,7 +24,9 @@ HIP_LIB = os.path.join(HERE, 'libtentorch_hip.so')
 HIP_SOURCES = ['device.hip', 'ggemm.hip', 'mlp.hip', 'eigh.hip']
 ENGINE_SOURCES = ['category.cpp', 'tree.cpp', 'move.cpp', 'program.cpp', 'bindings.cpp']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
-HIP_FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared',
+# The soname makes the engine record a bare library name, found through its $ORIGIN run path, so
+# the two libraries load from wherever the tree lies.
+HIP_FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared', '-Wl,-soname,libtentorch_hip.so',
              '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
 
 

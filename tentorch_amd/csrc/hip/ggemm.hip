@@ -4,6 +4,7 @@
 // The tile shape is a property of the plan (chosen by the host from the block shapes, or by an
 // autotuner through tt_ggemm_plan_create_config).  gfx950 only.
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -84,6 +85,47 @@ struct GemmTile {
   int32_t pad;
 };
 
+// Device form of a term: A's row stride travels with the depth, so that a piece of a term (a
+// k-range of it, for a product whose depth is split) reads the rows of the whole term.
+struct KTerm {
+  int64_t a_offset;
+  int64_t b_offset;
+  int64_t k;
+  int64_t lda;
+};
+
+// Device form of a group.  A part of a split product (partial != 0) writes its m x n partial sum,
+// in the compute type, at c_offset of the plan's partial-sum buffer instead of into C.
+struct KGroup {
+  int64_t c_offset;
+  int64_t m;
+  int64_t n;
+  int32_t first_term;
+  int32_t num_terms;
+  int32_t partial;
+  int32_t pad;
+};
+
+// A split product: its parts' partial sums lie back to back from p_offset (part s at
+// p_offset + s * elems) and are added in part order into the m x n block at c_offset of C.
+struct RGroup {
+  int64_t c_offset;
+  int64_t elems;
+  int64_t p_offset;
+  int32_t parts;
+  int32_t pad;
+};
+
+// One workgroup of the partial-sum launch: kReduceSpan consecutive elements of one split product.
+struct ReduceTile {
+  int32_t rgroup;
+  int32_t pad;
+  int64_t e0;
+};
+constexpr int kReduceThreads = 256;
+constexpr int kReducePer = 8;
+constexpr int64_t kReduceSpan = kReduceThreads * kReducePer;
+
 constexpr int kGemmThreads = 256;  // four waves in a 2 x 2 grid over the tile
 constexpr int kPad = 1;
 
@@ -100,11 +142,12 @@ constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 // barrier per slab.
 template <typename TA, typename TB, typename TO, typename TC, int BM, int BN, int BK>
 __global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__restrict__ tiles,
-                                                             const tt_gemm_group *__restrict__ groups,
-                                                             const tt_gemm_term *__restrict__ terms,
+                                                             const KGroup *__restrict__ groups,
+                                                             const KTerm *__restrict__ terms,
                                                              const TA *__restrict__ A, const TB *__restrict__ B,
-                                                             TO *__restrict__ C, int32_t plain_group,
-                                                             int32_t plain_term, int32_t accumulate) {
+                                                             TO *__restrict__ C, TC *__restrict__ P,
+                                                             int32_t plain_group, int32_t plain_term,
+                                                             int32_t accumulate) {
   using M = Mfma<TC>;
   constexpr int TM = BM / 2, TN = BN / 2, FM = TM / 16, FN = TN / 16;
   constexpr int A_PER = BM * BK / kGemmThreads, B_PER = BK * BN / kGemmThreads;
@@ -130,7 +173,7 @@ __global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__r
     t_begin = plain_term < 0 ? 0 : plain_term;
     t_end = plain_term < 0 ? 0 : plain_term + 1;
   }
-  const tt_gemm_group group = groups[gi];
+  const KGroup group = groups[gi];
   const int64_t m = group.m, n = group.n;
 
   const int tid = threadIdx.x;
@@ -155,13 +198,13 @@ __global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__r
   // Slab (t, kb) of the sum into registers: A element (row a_row, k a_k + j), B element (k b_k,
   // column b_col + j).
   auto load = [&](int32_t t, int64_t kb, TC (&xa)[A_PER], TC (&xb)[B_PER]) {
-    const tt_gemm_term tm = terms[t];
+    const KTerm tm = terms[t];
     const int64_t K = tm.k;
     const int64_t gr = m0 + a_row;
 #pragma unroll
     for (int j = 0; j < A_PER; ++j) {
       const int64_t kk = kb + a_k + j;
-      xa[j] = (gr < m && kk < K) ? Conv<TC>::from(A[tm.a_offset + gr * K + kk]) : Conv<TC>::zero();
+      xa[j] = (gr < m && kk < K) ? Conv<TC>::from(A[tm.a_offset + gr * tm.lda + kk]) : Conv<TC>::zero();
     }
     const int64_t kr = kb + b_k;
 #pragma unroll
@@ -246,14 +289,46 @@ __global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__r
         const int64_t row = m0 + wm * TM + i * 16 + M::row(lk, r);
         const int64_t col = n0 + wn * TN + j * 16 + li;
         if (row < m && col < n) {
-          TO *out = C + group.c_offset + row * n + col;
           const TC value = M::get(acc[i][j], r);
+          if (group.partial) {
+            P[group.c_offset + row * n + col] = value;
+            continue;
+          }
+          TO *out = C + group.c_offset + row * n + col;
           if (accumulate)
             *out = Conv<TO>::from(add(Conv<TC>::from(*out), value));
           else
             *out = Conv<TO>::from(value);
         }
       }
+}
+
+// The second launch of a plan with split products: each element of a split block is the sum of
+// its parts' partial sums, added in part order (so the result does not depend on the tile shape
+// or on which part finished first), converted to the output type.
+template <typename TO, typename TC>
+__global__ __launch_bounds__(kReduceThreads) void ggemm_reduce_kernel(const ReduceTile *__restrict__ rtiles,
+                                                                      const RGroup *__restrict__ rgroups,
+                                                                      const TC *__restrict__ P, TO *__restrict__ C) {
+  const ReduceTile tile = rtiles[blockIdx.x];
+  const RGroup g = rgroups[tile.rgroup];
+  TC sums[kReducePer];
+  bool live[kReducePer];
+#pragma unroll
+  for (int j = 0; j < kReducePer; ++j) {
+    const int64_t e = tile.e0 + j * kReduceThreads + threadIdx.x;
+    live[j] = e < g.elems;
+    sums[j] = live[j] ? P[g.p_offset + e] : Conv<TC>::zero();
+  }
+  for (int32_t s = 1; s < g.parts; ++s) {
+    const TC *part = P + g.p_offset + s * g.elems;
+#pragma unroll
+    for (int j = 0; j < kReducePer; ++j)
+      if (live[j]) sums[j] = add(sums[j], part[tile.e0 + j * kReduceThreads + threadIdx.x]);
+  }
+#pragma unroll
+  for (int j = 0; j < kReducePer; ++j)
+    if (live[j]) C[g.c_offset + tile.e0 + j * kReduceThreads + threadIdx.x] = Conv<TO>::from(sums[j]);
 }
 
 // How the tiles of one launch are ordered.  Workgroup i of a launch runs on XCD i % 8, and each XCD
@@ -267,19 +342,22 @@ constexpr int kXcds = 8;
 constexpr int kNumPackings = 2;
 
 // Host-side tile list and cost of a group list under one tile shape and packing.
-void tile_list(const tt_gemm_group *groups, int64_t num_groups, const tt_gemm_term *terms, const Shape &shape,
-               int packing, std::vector<GemmTile> &tiles, double *max_cost) {
+// Groups flagged in `skip` (split products, whose parts are launched in their place) get no tile.
+void tile_list(const std::vector<KGroup> &groups, const std::vector<KTerm> &terms, const std::vector<char> &skip,
+               const Shape &shape, int packing, std::vector<GemmTile> &tiles, double *max_cost) {
   struct Costed {
     double cost;
     GemmTile tile;
   };
+  const int64_t num_groups = static_cast<int64_t>(groups.size());
   std::vector<Costed> costed;
   std::vector<double> group_cost(static_cast<size_t>(num_groups), 0.0);
   *max_cost = 0.0;
   for (int64_t g = 0; g < num_groups; ++g) {
-    const tt_gemm_group &grp = groups[g];
+    if (skip[static_cast<size_t>(g)]) continue;
+    const KGroup &grp = groups[static_cast<size_t>(g)];
     int64_t ksum = 0;
-    for (int32_t t = grp.first_term; t < grp.first_term + grp.num_terms; ++t) ksum += terms[t].k;
+    for (int32_t t = grp.first_term; t < grp.first_term + grp.num_terms; ++t) ksum += terms[static_cast<size_t>(t)].k;
     const double cost = static_cast<double>((ksum + shape.bk - 1) / shape.bk) + 1.0;
     *max_cost = std::max(*max_cost, cost);
     for (int64_t mt = 0; mt < grp.m; mt += shape.bm)
@@ -340,14 +418,118 @@ int choose_shape(const tt_gemm_group *groups, int64_t num_groups, const tt_gemm_
   return tiles64 >= 8 * 256 ? 0 : 1;
 }
 
+// Depth split.  A launch whose products are few and deep (the DMRG matvec's second step: 16-29
+// blocks of at most ~350 x 350 with depths up to five times that) has too few output tiles to
+// occupy the chip, and every tile walks its whole depth alone.  Such a product is cut along its
+// depth into parts, each a product of its own that writes a partial sum; a second launch adds the
+// parts in order.  The decision reads only the block shapes, never the tile shape, so every launch
+// configuration of a plan sums in one order and gives the same values bit for bit.
+constexpr int64_t kSplitTiles = 16 * 256;  // a plan with at least this many 32 x 32 tiles is not split
+constexpr int64_t kSplitMinDepth = 512;    // nor is a product shallower than this
+constexpr int64_t kSplitSide = 128;        // a part is about max(m, n, 128) deep
+constexpr int64_t kMaxParts = 8;
+constexpr int64_t kPieceAlign = 32;        // part boundaries at multiples of 32 within a term
+
+// The device tables of a plan: the caller's groups and terms first (the per-block form indexes
+// them), then the parts of split products and their term pieces; the tiles of one launch; the
+// partial-sum records.
+struct DeviceTables {
+  std::vector<KGroup> groups;
+  std::vector<KTerm> terms;
+  std::vector<char> skip;  // split products (their parts are launched instead)
+  std::vector<RGroup> rgroups;
+  std::vector<ReduceTile> rtiles;
+  int64_t partial_elems = 0;
+};
+
+void split_tables(const tt_gemm_group *groups, int64_t num_groups, const tt_gemm_term *terms, int64_t num_terms,
+                  DeviceTables &d) {
+  d.groups.clear();
+  d.terms.clear();
+  for (int64_t t = 0; t < num_terms; ++t) d.terms.push_back(KTerm{terms[t].a_offset, terms[t].b_offset, terms[t].k, terms[t].k});
+  for (int64_t g = 0; g < num_groups; ++g)
+    d.groups.push_back(KGroup{groups[g].c_offset, groups[g].m, groups[g].n, groups[g].first_term, groups[g].num_terms, 0, 0});
+  d.skip.assign(static_cast<size_t>(num_groups), 0);
+  int64_t tiles32 = 0;
+  for (int64_t g = 0; g < num_groups; ++g) tiles32 += ((groups[g].m + 31) / 32) * ((groups[g].n + 31) / 32);
+  if (tiles32 >= kSplitTiles) return;
+  for (int64_t g = 0; g < num_groups; ++g) {
+    const tt_gemm_group &grp = groups[g];
+    int64_t depth = 0;
+    for (int32_t t = grp.first_term; t < grp.first_term + grp.num_terms; ++t) depth += std::max<int64_t>(terms[t].k, 0);
+    const int64_t side = std::max({grp.m, grp.n, kSplitSide});
+    if (grp.m == 0 || grp.n == 0 || depth < kSplitMinDepth || depth < 2 * side) continue;
+    const int64_t want = std::min(kMaxParts, depth / side);
+    const int64_t chunk = ((depth + want - 1) / want + kPieceAlign - 1) / kPieceAlign * kPieceAlign;
+    // Pieces of at most `chunk` along each term, in term order; a part collects consecutive pieces
+    // until it is `chunk` deep.
+    std::vector<std::vector<KTerm>> parts(1);
+    int64_t part_depth = 0;
+    for (int32_t t = grp.first_term; t < grp.first_term + grp.num_terms; ++t) {
+      const tt_gemm_term &tm = terms[t];
+      for (int64_t k0 = 0; k0 < tm.k; k0 += chunk) {
+        if (part_depth >= chunk) {
+          parts.emplace_back();
+          part_depth = 0;
+        }
+        const int64_t len = std::min(chunk, tm.k - k0);
+        parts.back().push_back(KTerm{tm.a_offset + k0, tm.b_offset + k0 * grp.n, len, tm.k});
+        part_depth += len;
+      }
+    }
+    if (parts.size() < 2) continue;
+    const int64_t elems = grp.m * grp.n;
+    d.skip[static_cast<size_t>(g)] = 1;
+    const int32_t rg = static_cast<int32_t>(d.rgroups.size());
+    d.rgroups.push_back(RGroup{grp.c_offset, elems, d.partial_elems, static_cast<int32_t>(parts.size()), 0});
+    for (size_t s = 0; s < parts.size(); ++s) {
+      d.groups.push_back(KGroup{d.partial_elems + static_cast<int64_t>(s) * elems, grp.m, grp.n,
+                                static_cast<int32_t>(d.terms.size()), static_cast<int32_t>(parts[s].size()), 1, 0});
+      d.terms.insert(d.terms.end(), parts[s].begin(), parts[s].end());
+      d.skip.push_back(0);
+    }
+    for (int64_t e0 = 0; e0 < elems; e0 += kReduceSpan) d.rtiles.push_back(ReduceTile{rg, 0, e0});
+    d.partial_elems += static_cast<int64_t>(parts.size()) * elems;
+  }
+}
+
+// The partial-sum buffer: one per device, grown when a plan needs more and never freed, because a
+// plan (or a graph captured from it) keeps the buffer that was current when it was made.  Split
+// launches share it, so the library's contract of one stream per device at a time is what keeps
+// two of them from overlapping.
+constexpr int kMaxDevices = 64;
+std::mutex g_partials_mutex;
+void *g_partials[kMaxDevices] = {};
+size_t g_partials_bytes[kMaxDevices] = {};
+
+int partial_buffer(size_t bytes, void **out) {
+  int device = 0;
+  TT_HIP(hipGetDevice(&device));
+  if (device < 0 || device >= kMaxDevices) return fail("grouped GEMM: device index out of range");
+  std::lock_guard<std::mutex> lock(g_partials_mutex);
+  if (g_partials_bytes[device] < bytes) {
+    size_t want = std::max(bytes, 2 * g_partials_bytes[device]);
+    want = (want + (size_t(1) << 20) - 1) & ~((size_t(1) << 20) - 1);
+    void *fresh = nullptr;
+    TT_HIP(hipMalloc(&fresh, want));
+    g_partials[device] = fresh;
+    g_partials_bytes[device] = want;
+  }
+  *out = g_partials[device];
+  return 0;
+}
+
 }  // namespace
 
 struct tt_ggemm_plan {
-  void *block = nullptr;   // the plan's range of the table arena, holding its three tables
-  tt_gemm_group *groups = nullptr;
-  tt_gemm_term *terms = nullptr;
+  void *block = nullptr;   // the plan's range of the table arena, holding its tables
+  KGroup *groups = nullptr;
+  KTerm *terms = nullptr;
   GemmTile *tiles = nullptr;
-  int64_t num_groups = 0, num_terms = 0, num_tiles = 0;
+  RGroup *rgroups = nullptr;
+  ReduceTile *rtiles = nullptr;
+  void *partials = nullptr;  // partial sums of split products (16 bytes per element: any compute type)
+  int64_t num_groups = 0, num_terms = 0, num_tiles = 0, num_rtiles = 0;
   int config = 0;
   double flops = 0.0;
   std::vector<tt_gemm_group> host_groups;
@@ -382,19 +564,30 @@ extern "C" int tt_ggemm_plan_create_config(const tt_gemm_group *groups, int64_t 
   }
   if (config < 0) config = choose_shape(groups, num_groups, terms);
   if (config >= kNumShapes * kNumPackings) return fail("tt_ggemm_plan_create: no such configuration");
+  DeviceTables d;
+  split_tables(groups, num_groups, terms, num_terms, d);
+  if (d.groups.size() >= (size_t(1) << 31) || d.terms.size() >= (size_t(1) << 31))
+    return fail("tt_ggemm_plan_create: too many blocks");
   std::vector<GemmTile> tiles;
   double max_cost = 0.0;
-  tile_list(groups, num_groups, terms, kShapes[config % kNumShapes], config / kNumShapes, tiles, &max_cost);
+  tile_list(d.groups, d.terms, d.skip, kShapes[config % kNumShapes], config / kNumShapes, tiles, &max_cost);
+  void *partials = nullptr;
+  if (d.partial_elems > 0)
+    if (int rc = partial_buffer(static_cast<size_t>(d.partial_elems) * sizeof(cdouble), &partials)) return rc;
   auto *p = new tt_ggemm_plan();
   p->num_groups = num_groups;
   p->num_terms = num_terms;
   p->num_tiles = static_cast<int64_t>(tiles.size());
+  p->num_rtiles = static_cast<int64_t>(d.rtiles.size());
+  p->partials = partials;
   p->config = config;
   p->flops = flops;
   p->host_groups.assign(groups, groups + num_groups);
-  if (ttdev::upload_tables(&p->block, {{reinterpret_cast<void **>(&p->groups), groups, sizeof(tt_gemm_group) * num_groups},
-                                        {reinterpret_cast<void **>(&p->terms), terms, sizeof(tt_gemm_term) * num_terms},
-                                        {reinterpret_cast<void **>(&p->tiles), tiles.data(), sizeof(GemmTile) * tiles.size()}})) {
+  if (ttdev::upload_tables(&p->block, {{reinterpret_cast<void **>(&p->groups), d.groups.data(), sizeof(KGroup) * d.groups.size()},
+                                        {reinterpret_cast<void **>(&p->terms), d.terms.data(), sizeof(KTerm) * d.terms.size()},
+                                        {reinterpret_cast<void **>(&p->tiles), tiles.data(), sizeof(GemmTile) * tiles.size()},
+                                        {reinterpret_cast<void **>(&p->rgroups), d.rgroups.data(), sizeof(RGroup) * d.rgroups.size()},
+                                        {reinterpret_cast<void **>(&p->rtiles), d.rtiles.data(), sizeof(ReduceTile) * d.rtiles.size()}})) {
     char keep[1024];
     tt_last_error(keep, sizeof(keep));
     tt_ggemm_plan_destroy(p);
@@ -434,8 +627,22 @@ int ggemm_launch(const tt_ggemm_plan *plan, const void *a, const void *b, void *
   if (int rc = ttdev::await_uploads(stream)) return rc;
   if (int rc = ttdev::launch(ggemm_kernel<TA, TB, TO, TC, BM, BN, BK>, dim3(static_cast<uint32_t>(grid)),
                              dim3(kGemmThreads), 0, stream, plan->tiles, plan->groups, plan->terms,
-                             static_cast<const TA *>(a), static_cast<const TB *>(b), static_cast<TO *>(c), plain_group,
-                             plain_term, accumulate))
+                             static_cast<const TA *>(a), static_cast<const TB *>(b), static_cast<TO *>(c),
+                             static_cast<TC *>(plan->partials), plain_group, plain_term, accumulate))
+    return rc;
+  TT_LAUNCHED();
+  ttdev::note_launch();
+  return 0;
+}
+
+// The partial sums of a plan's split products, added into C (after the batched launch, in order on
+// the same stream).
+template <typename TO, typename TC>
+int reduce_launch(const tt_ggemm_plan *plan, void *c, hipStream_t stream) {
+  if (plan->num_rtiles == 0) return 0;
+  if (int rc = ttdev::launch(ggemm_reduce_kernel<TO, TC>, dim3(static_cast<uint32_t>(plan->num_rtiles)),
+                             dim3(kReduceThreads), 0, stream, plan->rtiles, plan->rgroups,
+                             static_cast<const TC *>(plan->partials), static_cast<TO *>(c)))
     return rc;
   TT_LAUNCHED();
   ttdev::note_launch();
@@ -443,7 +650,7 @@ int ggemm_launch(const tt_ggemm_plan *plan, const void *a, const void *b, void *
 }
 
 template <typename TA, typename TB, typename TO, typename TC>
-int ggemm_shape(const tt_ggemm_plan *plan, const void *a, const void *b, void *c, hipStream_t stream, int64_t grid,
+int ggemm_tiles(const tt_ggemm_plan *plan, const void *a, const void *b, void *c, hipStream_t stream, int64_t grid,
                 int32_t pg, int32_t pt, int32_t acc) {
   switch (plan->config % kNumShapes) {
     case 0: return ggemm_launch<TA, TB, TO, TC, 64, 64, 16>(plan, a, b, c, stream, grid, pg, pt, acc);
@@ -455,6 +662,13 @@ int ggemm_shape(const tt_ggemm_plan *plan, const void *a, const void *b, void *c
     case 6: return ggemm_launch<TA, TB, TO, TC, 64, 64, 32>(plan, a, b, c, stream, grid, pg, pt, acc);
     default: return fail("grouped GEMM: no such tile shape");
   }
+}
+
+template <typename TA, typename TB, typename TO, typename TC>
+int ggemm_shape(const tt_ggemm_plan *plan, const void *a, const void *b, void *c, hipStream_t stream, int64_t grid,
+                int32_t pg, int32_t pt, int32_t acc) {
+  if (int rc = ggemm_tiles<TA, TB, TO, TC>(plan, a, b, c, stream, grid, pg, pt, acc)) return rc;
+  return pg < 0 ? reduce_launch<TO, TC>(plan, c, stream) : 0;
 }
 
 int ggemm_dispatch(const tt_ggemm_plan *plan, int at, int bt, int ct, int comp, const void *a, const void *b, void *c,

@@ -57,7 +57,9 @@ def _reference_gemm(a, b, c_size, groups, terms):
     return c
 
 
-SHAPES = [(1, 1, 1), (6, 1, 1), (64, 64, 16), (65, 63, 17), (130, 7, 200), (3, 129, 5), (200, 190, 70), (17, 33, 0)]
+SHAPES = [(1, 1, 1), (6, 1, 1), (64, 64, 16), (65, 63, 17), (130, 7, 200), (3, 129, 5), (200, 190, 70), (17, 33, 0),
+          # deep products, which the library splits along the depth (partial sums added in order)
+          (40, 30, 1500), (5, 70, 900), (300, 260, 700)]
 
 
 @pytest.mark.parametrize('multi_term', [False, True])
@@ -237,6 +239,41 @@ def test_every_launch_configuration_gives_identical_products(lib):
         if first is None:
             first = got
         assert np.array_equal(got, first), config
+
+
+@pytest.mark.parametrize('precision', ['f64', 'f32'])
+def test_deep_products_are_split_and_match_the_per_block_form(lib, precision):
+    '''Products much deeper than they are wide run as depth parts plus one launch that adds the
+    parts' partial sums in order: two launches for the whole plan, twice for the same values, within
+    1e-12 (FP64) of the per-block form and of numpy, with the last rows and columns of ragged
+    blocks written.'''
+    rng = np.random.default_rng(13)
+    shapes = [(40, 30, 1500), (5, 70, 900), (300, 260, 700), (33, 17, 40), (2, 3, 0)]
+    a, b, c_size, groups, terms = _random_gemm(rng, shapes, multi_term=True)
+    ref = _reference_gemm(a, b, c_size, groups, terms)
+    g, t = cabi.gemm_tables(groups, terms)
+    plan = ctypes.c_void_p()
+    assert lib.tt_ggemm_plan_create(g, len(groups), t, len(terms), ctypes.byref(plan)) == 0, cabi.error(lib)
+    dtype = torch.float64 if precision == 'f64' else torch.float32
+    code = cabi.TT_F64 if precision == 'f64' else cabi.TT_F32
+    da, db = torch.tensor(a, dtype=dtype, device='cuda'), torch.tensor(b, dtype=dtype, device='cuda')
+    results = []
+    for _ in range(2):
+        dc = torch.full((c_size,), float('nan'), dtype=dtype, device='cuda')
+        before = lib.tt_launch_count()
+        assert lib.tt_ggemm_execute(plan, code, code, code, code, da.data_ptr(), db.data_ptr(), dc.data_ptr(),
+                                    _stream()) == 0, cabi.error(lib)
+        assert lib.tt_launch_count() - before == 2
+        results.append(dc.double().cpu().numpy())
+    assert np.array_equal(results[0], results[1])
+    plain = torch.full((c_size,), float('nan'), dtype=dtype, device='cuda')
+    assert lib.tt_ggemm_execute_plain(plan, code, da.data_ptr(), db.data_ptr(), plain.data_ptr(), _stream()) == 0, cabi.error(lib)
+    lib.tt_ggemm_plan_destroy(plan)
+    tol = 1e-12 if precision == 'f64' else 2e-5
+    scale = np.abs(ref).max()
+    assert np.all(np.isfinite(results[0]))
+    assert np.abs(results[0] - ref).max() <= tol * scale
+    assert np.abs(results[0] - plain.double().cpu().numpy()).max() <= tol * scale
 
 
 @pytest.mark.parametrize('config', [0, 1, 5, 13])
