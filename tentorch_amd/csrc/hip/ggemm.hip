@@ -77,6 +77,44 @@ template <typename T>
 __device__ T add(T a, T b) { return a + b; }
 __device__ cdouble add(cdouble a, cdouble b) { return cdouble{a.re + b.re, a.im + b.im}; }
 
+// Raw buffer loads.  A buffer resource holds a base address and a byte range; a load at an offset
+// outside the range returns zeros and touches no memory, so an operand slab can be read whole,
+// without a branch or a select per element, and what lies past a block reads as zero.  Word 3 of
+// the resource is the gfx9-family default (no format conversion, raw addressing).
+constexpr int kBufferWord3 = 0x00020000;
+// The resource of one operand slab.  Its fields are uniform over the workgroup; reading them
+// through readfirstlane puts them in scalar registers, where a buffer load needs its resource (the
+// compiler cannot always prove that a value derived from the tile record is uniform, and would
+// otherwise wrap every load in a loop over distinct lane values).
+__device__ inline __amdgpu_buffer_rsrc_t resource(const char *base, uint32_t bytes) {
+  const uint64_t address = reinterpret_cast<uint64_t>(base);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(address));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(address >> 32));
+  const uint32_t range = __builtin_amdgcn_readfirstlane(bytes);
+  void *uniform = reinterpret_cast<void *>((static_cast<uint64_t>(hi) << 32) | lo);
+  return __builtin_amdgcn_make_buffer_rsrc(uniform, static_cast<short>(0), static_cast<int>(range), kBufferWord3);
+}
+template <typename T>
+struct BufferLoad;
+template <>
+struct BufferLoad<float> {
+  static __device__ float at(__amdgpu_buffer_rsrc_t r, uint32_t byte) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte, 0, 0));
+  }
+};
+template <>
+struct BufferLoad<double> {
+  static __device__ double at(__amdgpu_buffer_rsrc_t r, uint32_t byte) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, byte, 0, 0));
+  }
+};
+template <>
+struct BufferLoad<cdouble> {
+  static __device__ cdouble at(__amdgpu_buffer_rsrc_t r, uint32_t byte) {
+    return __builtin_bit_cast(cdouble, __builtin_amdgcn_raw_buffer_load_b128(r, byte, 0, 0));
+  }
+};
+
 // Output tile of one workgroup: which block, and the tile's first row and column.
 struct GemmTile {
   int32_t group;
@@ -127,6 +165,9 @@ constexpr int kReducePer = 8;
 constexpr int64_t kReduceSpan = kReduceThreads * kReducePer;
 
 constexpr int kGemmThreads = 256;  // four waves in a 2 x 2 grid over the tile
+// Limits that keep a tile's byte offsets and the ranges of its operand blocks in 32 bits.
+constexpr int64_t kMaxSide = int64_t(1) << 20;
+constexpr int64_t kMaxBlockElements = int64_t(1) << 27;
 constexpr int kPad = 1;
 
 // The tile shapes the library is built with: (BM, BN, BK).
@@ -191,68 +232,92 @@ __global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__r
   const int a_row = tid / A_TPR, a_k = (tid % A_TPR) * A_PER;
   const int b_k = tid / B_TPR, b_col = (tid % B_TPR) * B_PER;
 
-  int32_t term = t_begin;
-  int64_t k0 = 0;
-  while (term < t_end && terms[term].k <= 0) ++term;
+  // The slabs of the sum: BK-deep k-ranges of each term in list order (terms of depth 0 skipped).
+  while (t_begin < t_end && terms[t_begin].k <= 0) ++t_begin;
+  int32_t slabs = 0;
+  for (int32_t t = t_begin; t < t_end; ++t)
+    if (terms[t].k > 0) slabs += static_cast<int32_t>((terms[t].k + BK - 1) / BK);
 
-  // Slab (t, kb) of the sum into registers: A element (row a_row, k a_k + j), B element (k b_k,
-  // column b_col + j).
-  auto load = [&](int32_t t, int64_t kb, TC (&xa)[A_PER], TC (&xb)[B_PER]) {
-    const KTerm tm = terms[t];
-    const int64_t K = tm.k;
-    const int64_t gr = m0 + a_row;
+  // The load cursor: the slab whose loads are issued next.  Each slab is read through two buffer
+  // resources based at its first element (A at row m0, column kb of the term's block; B at row kb,
+  // column n0) whose ranges end where the term's blocks end (A's at the term's last element: a term
+  // that is a depth range of a split product reads rows of the whole product).  A thread's elements
+  // sit at fixed byte offsets from those bases (computed once per term), so every load is issued
+  // unconditionally and the compiler counts the loads in flight.  Rows of A past m, rows of B past
+  // the term's depth, and every load once the cursor has run past the last slab (range 0) read
+  // zeros.  What a tile reads past n (B) or m (A) inside a range only reaches output rows and
+  // columns that are never stored.  The one in-range overhang that reaches the output, A's columns
+  // past the term's depth in its last slab, is masked when the slab is stored (`klim`).
+  // The cursor's state is uniform: byte pointers to the next slab's first A and B elements, the
+  // bytes from there to the end of each block, and the depth the term has left; a slab advances
+  // them by constant steps.
+  const char *a_ptr = nullptr, *b_ptr = nullptr;
+  uint32_t a_left = 0, b_left = 0;
+  int32_t k_left = 0;
+  uint32_t a_off = 0;
+  const uint32_t b_off = static_cast<uint32_t>((b_k * n + b_col) * sizeof(TB));
+  const uint32_t b_step = static_cast<uint32_t>(BK * n * sizeof(TB));
+  constexpr uint32_t a_step = BK * sizeof(TA);
+  int32_t lt = t_begin;
+  auto enter = [&](const KTerm &tm) {
+    a_ptr = reinterpret_cast<const char *>(A + tm.a_offset + m0 * tm.lda);
+    a_left = static_cast<uint32_t>(((m - 1 - m0) * tm.lda + tm.k) * static_cast<int64_t>(sizeof(TA)));
+    b_ptr = reinterpret_cast<const char *>(B + tm.b_offset + n0);
+    b_left = static_cast<uint32_t>((tm.k * n - n0) * static_cast<int64_t>(sizeof(TB)));
+    k_left = static_cast<int32_t>(tm.k);
+    a_off = static_cast<uint32_t>((a_row * tm.lda + a_k) * sizeof(TA));
+  };
+  if (t_begin < t_end) enter(terms[t_begin]);
+
+  auto issue = [&](TA (&xa)[A_PER], TB (&xb)[B_PER], int32_t &klim) {
+    klim = k_left < BK ? k_left : BK;
+    const __amdgpu_buffer_rsrc_t ra = resource(a_ptr, a_left);
+    const __amdgpu_buffer_rsrc_t rb = resource(b_ptr, b_left);
 #pragma unroll
-    for (int j = 0; j < A_PER; ++j) {
-      const int64_t kk = kb + a_k + j;
-      xa[j] = (gr < m && kk < K) ? Conv<TC>::from(A[tm.a_offset + gr * tm.lda + kk]) : Conv<TC>::zero();
-    }
-    const int64_t kr = kb + b_k;
+    for (int j = 0; j < A_PER; ++j) xa[j] = BufferLoad<TA>::at(ra, a_off + j * sizeof(TA));
 #pragma unroll
-    for (int j = 0; j < B_PER; ++j) {
-      const int64_t gc = n0 + b_col + j;
-      xb[j] = (kr < K && gc < n) ? Conv<TC>::from(B[tm.b_offset + kr * n + gc]) : Conv<TC>::zero();
+    for (int j = 0; j < B_PER; ++j) xb[j] = BufferLoad<TB>::at(rb, b_off + j * sizeof(TB));
+    a_ptr += a_step;
+    a_left = a_left > a_step ? a_left - a_step : 0u;
+    b_ptr += b_step;
+    b_left = b_left > b_step ? b_left - b_step : 0u;
+    k_left -= BK;
+    if (k_left <= 0 && lt < t_end) {
+      ++lt;
+      while (lt < t_end && terms[lt].k <= 0) ++lt;
+      if (lt < t_end) {
+        enter(terms[lt]);
+      } else {
+        // past the last slab: every further load reads zeros, and the slabs are masked whole
+        a_left = b_left = 0u;
+        k_left = 0;
+      }
     }
   };
-  auto advance = [&](int32_t &t, int64_t &kb) {
-    kb += BK;
-    if (kb >= terms[t].k) {
-      kb = 0;
-      ++t;
-      while (t < t_end && terms[t].k <= 0) ++t;
-    }
-  };
 
-  // Two slabs in flight: while slab s is multiplied from LDS, slabs s + 1 and s + 2 are in the two
-  // register sets, which alternate roles from one slab to the next (the loop body is unrolled twice
-  // so each set stays in fixed registers).  (t1, k1) is the slab after the current one.  Only the
-  // 32 x 32 real tiles keep two slabs in flight: on larger tiles the second register set costs half
-  // the occupancy (measured slower), and complex tiles have no registers left for it.
+  // Two slabs in flight (two register sets, alternating) on tiles of at most 32 x 32; one on
+  // larger tiles, where a second set costs occupancy, and on complex tiles.
   constexpr bool kTwoAhead = !std::is_same<TC, cdouble>::value && BM * BN <= 32 * 32;
-  TC ra0[A_PER], rb0[B_PER], ra1[A_PER], rb1[B_PER];
-  int32_t t1 = term;
-  int64_t k1 = k0;
-  if (term < t_end) {
-    load(term, k0, ra0, rb0);
-    advance(t1, k1);
-    if constexpr (kTwoAhead) {
-      if (t1 < t_end) load(t1, k1, ra1, rb1);
-    }
+  TA ra0[A_PER], ra1[A_PER];
+  TB rb0[B_PER], rb1[B_PER];
+  int32_t klim0 = 0, klim1 = 0;
+  if (slabs > 0) {
+    issue(ra0, rb0, klim0);
+    if constexpr (kTwoAhead) issue(ra1, rb1, klim1);
   }
   int buf = 0;
-  auto step = [&](TC (&xa)[A_PER], TC (&xb)[B_PER]) {
+  auto step = [&](TA (&xa)[A_PER], TB (&xb)[B_PER], int32_t &klim) {
+    if (klim == BK) {
 #pragma unroll
-    for (int j = 0; j < A_PER; ++j) As[buf][a_k + j][a_row] = xa[j];
-#pragma unroll
-    for (int j = 0; j < B_PER; ++j) Bs[buf][b_k][b_col + j] = xb[j];
-    __syncthreads();
-    int32_t t2 = t1;
-    int64_t k2 = k1;
-    if (t2 < t_end) advance(t2, k2);
-    if constexpr (kTwoAhead) {
-      if (t2 < t_end) load(t2, k2, xa, xb);
+      for (int j = 0; j < A_PER; ++j) As[buf][a_k + j][a_row] = Conv<TC>::from(xa[j]);
     } else {
-      if (t1 < t_end) load(t1, k1, xa, xb);
+#pragma unroll
+      for (int j = 0; j < A_PER; ++j) As[buf][a_k + j][a_row] = a_k + j < klim ? Conv<TC>::from(xa[j]) : Conv<TC>::zero();
     }
+#pragma unroll
+    for (int j = 0; j < B_PER; ++j) Bs[buf][b_k][b_col + j] = Conv<TC>::from(xb[j]);
+    __syncthreads();
+    issue(xa, xb, klim);
 #pragma unroll
     for (int kk = 0; kk < BK / 4; ++kk) {
       const int k = kk * 4 + lk;
@@ -266,18 +331,17 @@ __global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__r
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = M::step(af[i], bf[j], acc[i][j]);
     }
-    term = t1;
-    k0 = k1;
-    t1 = t2;
-    k1 = k2;
     buf ^= 1;
   };
-  while (term < t_end) {
-    step(ra0, rb0);
-    if constexpr (kTwoAhead) {
-      if (term >= t_end) break;
-      step(ra1, rb1);
+  if constexpr (kTwoAhead) {
+    // An even number of steps, so the loop body is the two steps with no exit between them; an
+    // odd count ends with a step over a slab of zeros, which adds exact zeros.
+    for (int32_t slab = 0; slab < slabs; slab += 2) {
+      step(ra0, rb0, klim0);
+      step(ra1, rb1, klim1);
     }
+  } else {
+    for (int32_t slab = 0; slab < slabs; ++slab) step(ra0, rb0, klim0);
   }
 
 #pragma unroll
@@ -408,14 +472,16 @@ void tile_list(const std::vector<KGroup> &groups, const std::vector<KTerm> &term
   }
 }
 
-// The default tile shape.  Measured on the DMRG matvec products (16 blocks, 0.97 GFLOP per step):
-// with fewer than eight 64 x 64 tiles per CU the launch is short of parallel work and 32 x 32 tiles
-// win (by up to 1.8x when K is long and the blocks are small); with more, 64 x 64 tiles do as well
-// and read each operand fewer times.
-int choose_shape(const tt_gemm_group *groups, int64_t num_groups, const tt_gemm_term *) {
+// The default tile shape, from the number of 64 x 64 tiles of the launch (split products counted
+// by their parts).  Measured on the DMRG matvec products (15-sector bonds, χ = 1024 and 2048,
+// FP32 and FP64 MFMA): from 2048 tiles 64 x 64 is best or within 8 % of it; between 512 and 2048
+// tiles 64 x 32 is best or within 12 % (FP64 deep products: 25 %); below that the launch is short
+// of parallel work and 32 x 32 tiles win.
+int choose_shape(const std::vector<KGroup> &groups, const std::vector<char> &skip) {
   int64_t tiles64 = 0;
-  for (int64_t g = 0; g < num_groups; ++g) tiles64 += ((groups[g].m + 63) / 64) * ((groups[g].n + 63) / 64);
-  return tiles64 >= 8 * 256 ? 0 : 1;
+  for (size_t g = 0; g < groups.size(); ++g)
+    if (!skip[g]) tiles64 += ((groups[g].m + 63) / 64) * ((groups[g].n + 63) / 64);
+  return tiles64 >= 2048 ? 0 : tiles64 >= 512 ? 3 : 1;
 }
 
 // Depth split.  A launch whose products are few and deep (the DMRG matvec's second step: 16-29
@@ -556,16 +622,20 @@ extern "C" int tt_ggemm_plan_create_config(const tt_gemm_group *groups, int64_t 
     if (grp.m < 0 || grp.n < 0) return fail("tt_ggemm_plan_create: negative block shape");
     if (grp.num_terms < 0 || grp.first_term < 0 || grp.first_term + grp.num_terms > num_terms)
       return fail("tt_ggemm_plan_create: group terms out of range");
-    if (grp.m >= (int64_t(1) << 31) || grp.n >= (int64_t(1) << 31)) return fail("tt_ggemm_plan_create: block too large");
+    // The kernel reads a tile's operands at 32-bit byte offsets from the tile's first element.
+    if (grp.m >= kMaxSide || grp.n >= kMaxSide) return fail("tt_ggemm_plan_create: block too large (a side of 2^20 or more)");
     for (int32_t t = grp.first_term; t < grp.first_term + grp.num_terms; ++t) {
       if (terms[t].k < 0) return fail("tt_ggemm_plan_create: negative k");
+      if (terms[t].k >= kMaxSide) return fail("tt_ggemm_plan_create: block too large (a depth of 2^20 or more)");
+      if (grp.m * terms[t].k >= kMaxBlockElements || terms[t].k * grp.n >= kMaxBlockElements)
+        return fail("tt_ggemm_plan_create: operand block too large (2^27 elements or more)");
       flops += 2.0 * static_cast<double>(grp.m) * static_cast<double>(grp.n) * static_cast<double>(terms[t].k);
     }
   }
-  if (config < 0) config = choose_shape(groups, num_groups, terms);
-  if (config >= kNumShapes * kNumPackings) return fail("tt_ggemm_plan_create: no such configuration");
   DeviceTables d;
   split_tables(groups, num_groups, terms, num_terms, d);
+  if (config < 0) config = choose_shape(d.groups, d.skip);
+  if (config >= kNumShapes * kNumPackings) return fail("tt_ggemm_plan_create: no such configuration");
   if (d.groups.size() >= (size_t(1) << 31) || d.terms.size() >= (size_t(1) << 31))
     return fail("tt_ggemm_plan_create: too many blocks");
   std::vector<GemmTile> tiles;

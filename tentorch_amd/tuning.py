@@ -154,13 +154,18 @@ def measure(program, args, op: int, configs=None, reps: int = 3) -> np.ndarray:
             program.set_gemm_config(op, c)
             program.prepare()
             program.run(*base, op, op + 1, *extra)          # warm
+            before = _native.launch_count()
             _native.timing(True)
             try:
                 for _ in range(reps):
                     program.run(*base, op, op + 1, *extra)
             finally:
                 _native.timing(False)
-            times[c] = min(_native.timing_collect()) * 1e3
+            durations = _native.timing_collect()
+            # a product split along its depth runs two launches (parts, then their sum): a run's
+            # time is the sum of its launches
+            per_run = (_native.launch_count() - before) // reps
+            times[c] = min(sum(durations[r * per_run:(r + 1) * per_run]) for r in range(reps)) * 1e3
     finally:
         program.set_gemm_config(op, keep)
     return times
