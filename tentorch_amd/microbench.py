@@ -20,7 +20,7 @@ one grouped MFMA launch per contraction step), and, for the grouped 3-operand fo
 captured hipGraph.  Times are HIP events on the launch stream around `--reps` calls after one warm
 call; launches are counted by the device library.
 
-    python -m tentorch_amd.microbench [--precision fp64 fp32] [--reps 50] [--form 3 5]
+    python -m tentorch_amd.microbench [--precision fp64 fp32 c128] [--reps 50] [--form 3 5]
                                       [--copy-mib 256]
 
 `--copy-mib` adds a plain copy of that size (the calibration kernel for hardware byte counters).
@@ -99,7 +99,7 @@ def fresh(ops):
 
 def case(form: int, precision: str, mode: str, reps: int, seed: int = 0) -> dict:
     '''One measured configuration; `mode` is plain, grouped or graph.'''
-    dtype = torch.float32 if precision == 'fp32' else torch.float64
+    dtype = {'fp32': torch.float32, 'fp64': torch.float64, 'c128': torch.complex128}[precision]
     left, first, second, right, theta = operands(dtype, seed)
     workspaces.clear()
     Settings.mode = 1 if mode == 'plain' else 0
@@ -128,7 +128,7 @@ def case(form: int, precision: str, mode: str, reps: int, seed: int = 0) -> dict
             torch.cuda.synchronize()
         else:
             ms, launches = _timed(lambda: run(program, fresh(ops), out=out), reps)
-        result = out.double().cpu().numpy()
+        result = (out.real if out.is_complex() else out).double().cpu().numpy()
     finally:
         Settings.mode = 0
     gemm_flops = sum(cost[1] for cost in program.op_costs() if cost[0] == 'gemm')
@@ -150,7 +150,7 @@ def copy_case(mib: int) -> dict:
 
 def main(argv=None) -> int:
     parser = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    parser.add_argument('--precision', nargs='+', default=['fp64', 'fp32'], choices=('fp64', 'fp32'))
+    parser.add_argument('--precision', nargs='+', default=['fp64', 'fp32'], choices=('fp64', 'fp32', 'c128'))
     parser.add_argument('--form', nargs='+', type=int, default=[3, 5], choices=(3, 5))
     parser.add_argument('--mode', nargs='+', default=['plain', 'grouped', 'graph'],
                         choices=('plain', 'grouped', 'graph'))

@@ -17,6 +17,7 @@ lists them.  Two ways of choosing one per block-product op:
 `Settings.tiles` selects 'default' (the library's rule from block shapes), 'model' or 'exhaustive'.
 
     python -m tentorch_amd.tuning harvest --out samples.npz    # exhaustive timings of configs 1-3
+                                                               # (FP64, FP32 and complex128)
     python -m tentorch_amd.tuning train --data samples.npz --out cost_model.npz
     python -m tentorch_amd.tuning regret --model cost_model.npz  # SU(2) and Fibonacci contractions
 '''
@@ -276,11 +277,11 @@ def load_samples(path: str):
 
 # -- workloads -------------------------------------------------------------------------------------
 
-def _dmrg_sweeps(length, bond, sweeps, precision):
+def _dmrg_sweeps(length, bond, sweeps, precision, dtype=None):
     from .algorithms import DMRG
     from .models import Model
     from .symmetry import u1
-    driver = DMRG(Model(length, cat=u1), max_bond_dim=bond, n_iter=5, seed=0, precision=precision)
+    driver = DMRG(Model(length, cat=u1), max_bond_dim=bond, n_iter=5, seed=0, precision=precision, dtype=dtype)
     for s in range(sweeps):
         start = time.perf_counter()
         driver.sweep()
@@ -291,7 +292,7 @@ def _dmrg_sweeps(length, bond, sweeps, precision):
 
 def _microbench():
     from . import microbench
-    for precision in ('fp64', 'fp32'):
+    for precision in ('fp64', 'fp32', 'c128'):
         for form in (3, 5):
             microbench.case(form, precision, 'grouped', 2)
     print(f'[tuning] microbench: {len(tuner.samples)} ops measured', file=sys.stderr, flush=True)
@@ -362,6 +363,8 @@ def main(argv=None) -> int:
         from .tensor import Settings
         with Settings.using(tiles='exhaustive'):
             _dmrg_sweeps(16, 32, 2, 'fp64')
+            # the reference engine's default scalar type
+            _dmrg_sweeps(16, 32, 2, 'fp64', dtype=torch.complex128)
             _microbench()
             _dmrg_sweeps(64, 1024, args.sweeps, 'fp64')
             tuner.reset()
