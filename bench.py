@@ -43,6 +43,24 @@ def log(message: str) -> None:
     print(message, file=sys.stderr, flush=True)
 
 
+def measured_traffic(compute):
+    '''HBM bytes per block-product step of the matvec, from the hardware byte counters of two
+    rocprofv3 passes over this bench (FETCH_SIZE, then WRITE_SIZE), each scaled by what the counter
+    reads for a copy of known size in the same run; the summary is committed beside this file's
+    profiles.  None when no summary for this compute type is there.'''
+    path = os.path.join(HERE, 'profiles', 'dmrg_matvec_gemm_L64_chi1024_pmc.json')
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    entry = data.get(compute)
+    if not entry:
+        return None
+    return {'bytes_per_step': entry['hbm_bytes_per_step'],
+            'source': 'profiles/dmrg_matvec_gemm_L64_chi1024_pmc.json (FETCH_SIZE x %.3f + WRITE_SIZE x %.3f, '
+                      'calibrated on a %d MiB copy)' % (data['fetch_scale'], data['write_scale'], data['copy_mib'])}
+
+
 def cpu_baseline(length=16, bond=32, sweeps=3):
     '''The oracle on one host thread: s/sweep of the reference's CPU case.'''
     from threadpoolctl import threadpool_limits
@@ -123,6 +141,9 @@ def main() -> int:
     parser.add_argument('--no-profile', action='store_true', help='do not time launches with events')
     parser.add_argument('--breakdown', action='store_true',
                         help='after the timed sweeps, one extra sweep with a synchronised time breakdown')
+    parser.add_argument('--copy-mib', type=int, default=0,
+                        help='at the end, one plain FP64 copy of this size (the calibration kernel of the '
+                             'hardware byte counters)')
     args = parser.parse_args()
     if args.gpus != 1:
         print(f'bench.py measures one GPU: every box here has one, and this path has no multi-GPU form '
@@ -242,15 +263,23 @@ def main() -> int:
         g = summary['gemm']
         achieved = g['flops'] / (g['ms'] * 1e-3) / 1e12
         kernel = ('ggemm_kernel<float,double,float,float> and <float,float,double,float> (the matvec\'s two '
-                  'FP32-MFMA block-product launches: FP64 operand converted on load, FP64 result)'
-                  if compute == 'f32' else 'ggemm_kernel<double,double,double,double> (matvec block products)')
+                  'FP32-MFMA block-product steps: FP64 operand converted on load, FP64 result), with '
+                  'ggemm_reduce_kernel<double,float> after a step whose products are split along their depth'
+                  if compute == 'f32' else 'ggemm_kernel<double,double,double,double> (matvec block products), with '
+                  'ggemm_reduce_kernel<double,double> after a step whose products are split along their depth')
+        traffic = measured_traffic(compute)
         roofline = {'bound': 'mfma', 'kernel': kernel,
                     'achieved': round(achieved, 3), 'peak': peak, 'unit': 'TFLOP/s',
-                    'frac': round(achieved / peak, 4), 'traffic': None,
-                    'launches': g['launches'], 'avg_launch_us': round(g['ms'] * 1e3 / g['launches'], 2),
-                    'flops_per_launch': g['flops'] / g['launches'],
+                    'frac': round(achieved / peak, 4),
+                    'traffic': None if traffic is None else traffic['bytes_per_step'],
+                    'traffic_source': None if traffic is None else traffic['source'],
+                    'algorithmic_bytes_per_step': g['bytes'] / g['ops'],
+                    'steps': g['ops'], 'launches': g['launches'],
+                    'avg_step_us': round(g['ms'] * 1e3 / g['ops'], 2),
+                    'flops_per_step': g['flops'] / g['ops'],
                     'timing': 'kernel-dispatch events (hipExtLaunchKernelGGL) over one instrumented sweep '
-                              'after the timed region'}
+                              'after the timed region; a step is one grouped launch plus, when its products '
+                              'are split along their depth, the launch that adds their parts'}
     if 'move' in summary and summary['move']['ms'] > 0 and roofline is not None:
         m = summary['move']
         gbs = m['bytes'] / (m['ms'] * 1e-3) / 1e9
@@ -258,6 +287,15 @@ def main() -> int:
                              'peak': PEAK_HBM_GBS, 'unit': 'GB/s', 'frac': round(gbs / PEAK_HBM_GBS, 4),
                              'launches': m['launches'], 'avg_launch_us': round(m['ms'] * 1e3 / m['launches'], 2),
                              'bytes_per_launch': m['bytes'] / m['launches']}
+
+    if args.copy_mib:
+        from tentorch_amd import _native
+        n = args.copy_mib * (1 << 20) // 8
+        x = torch.randn(n, dtype=torch.float64, device='cuda')
+        y = torch.empty_like(x)
+        _native.axpby(torch.float64, 1.0, x, 0.0, y)
+        torch.cuda.synchronize()
+        log(f'[bench] calibration copy: {args.copy_mib} MiB read, {args.copy_mib} MiB written (axpby_kernel<double>)')
 
     baseline = None
     if not args.no_cpu:

@@ -161,8 +161,8 @@ struct ReduceTile {
   int64_t e0;
 };
 constexpr int kReduceThreads = 256;
-constexpr int kReducePer = 8;
-constexpr int64_t kReduceSpan = kReduceThreads * kReducePer;
+constexpr int64_t kReduceSpan = kReduceThreads;  // one element per thread
+constexpr int kMaxLoadedParts = 8;               // the most parts a split product has
 
 constexpr int kGemmThreads = 256;  // four waves in a 2 x 2 grid over the tile
 // Limits that keep a tile's byte offsets and the ranges of its operand blocks in 32 bits.
@@ -369,38 +369,33 @@ __global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__r
 
 // The second launch of a plan with split products: each element of a split block is the sum of
 // its parts' partial sums, added in part order (so the result does not depend on the tile shape
-// or on which part finished first), converted to the output type.
+// or on which part finished first), converted to the output type.  One element per thread, with
+// the loads of all its parts issued before the adds (a part index past the last reads the last
+// part again and is not added), so a thread waits for memory once.
 template <typename TO, typename TC>
 __global__ __launch_bounds__(kReduceThreads) void ggemm_reduce_kernel(const ReduceTile *__restrict__ rtiles,
                                                                       const RGroup *__restrict__ rgroups,
                                                                       const TC *__restrict__ P, TO *__restrict__ C) {
   const ReduceTile tile = rtiles[blockIdx.x];
   const RGroup g = rgroups[tile.rgroup];
-  TC sums[kReducePer];
-  bool live[kReducePer];
+  const int64_t e = tile.e0 + threadIdx.x;
+  if (e >= g.elems) return;
+  TC part[kMaxLoadedParts];
 #pragma unroll
-  for (int j = 0; j < kReducePer; ++j) {
-    const int64_t e = tile.e0 + j * kReduceThreads + threadIdx.x;
-    live[j] = e < g.elems;
-    sums[j] = live[j] ? P[g.p_offset + e] : Conv<TC>::zero();
-  }
-  for (int32_t s = 1; s < g.parts; ++s) {
-    const TC *part = P + g.p_offset + s * g.elems;
+  for (int s = 0; s < kMaxLoadedParts; ++s) part[s] = P[g.p_offset + (s < g.parts ? s : g.parts - 1) * g.elems + e];
+  TC sum = part[0];
 #pragma unroll
-    for (int j = 0; j < kReducePer; ++j)
-      if (live[j]) sums[j] = add(sums[j], part[tile.e0 + j * kReduceThreads + threadIdx.x]);
-  }
-#pragma unroll
-  for (int j = 0; j < kReducePer; ++j)
-    if (live[j]) C[g.c_offset + tile.e0 + j * kReduceThreads + threadIdx.x] = Conv<TO>::from(sums[j]);
+  for (int s = 1; s < kMaxLoadedParts; ++s)
+    if (s < g.parts) sum = add(sum, part[s]);
+  C[g.c_offset + e] = Conv<TO>::from(sum);
 }
 
 // How the tiles of one launch are ordered.  Workgroup i of a launch runs on XCD i % 8, and each XCD
 // has an L2 of its own.
 enum Packing {
   kLongestFirst = 0,  // by decreasing K, so the last wave of the launch is made of short tiles
-  kXcdGrouped = 1,    // every tile of a block product on one XCD (its A and B panels then share an
-                      // L2); block products spread over the XCDs by largest total cost first
+  kXcdBands = 1,      // each block product's tiles cut into row-major bands, one per XCD (a band's
+                      // A rows and the product's B then share that XCD's L2)
 };
 constexpr int kXcds = 8;
 constexpr int kNumPackings = 2;
@@ -415,7 +410,6 @@ void tile_list(const std::vector<KGroup> &groups, const std::vector<KTerm> &term
   };
   const int64_t num_groups = static_cast<int64_t>(groups.size());
   std::vector<Costed> costed;
-  std::vector<double> group_cost(static_cast<size_t>(num_groups), 0.0);
   *max_cost = 0.0;
   for (int64_t g = 0; g < num_groups; ++g) {
     if (skip[static_cast<size_t>(g)]) continue;
@@ -427,7 +421,6 @@ void tile_list(const std::vector<KGroup> &groups, const std::vector<KTerm> &term
     for (int64_t mt = 0; mt < grp.m; mt += shape.bm)
       for (int64_t nt = 0; nt < grp.n; nt += shape.bn) {
         costed.push_back({cost, GemmTile{static_cast<int32_t>(g), static_cast<int32_t>(mt), static_cast<int32_t>(nt), 0}});
-        group_cost[static_cast<size_t>(g)] += cost;
       }
   }
   std::stable_sort(costed.begin(), costed.end(), [](const Costed &a, const Costed &b) { return a.cost > b.cost; });
@@ -436,25 +429,27 @@ void tile_list(const std::vector<KGroup> &groups, const std::vector<KTerm> &term
     for (const Costed &c : costed) tiles.push_back(c.tile);
     return;
   }
-  // XCD-grouped: assign block products to XCDs by largest total cost first (each to the least
-  // loaded XCD), keep each XCD's tiles longest first, then lay them out so that position p holds a
-  // tile of XCD p % 8; when an XCD runs out, the position takes a tile from the fullest one.
-  std::vector<int64_t> order(static_cast<size_t>(num_groups));
-  for (int64_t g = 0; g < num_groups; ++g) order[static_cast<size_t>(g)] = g;
-  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
-    return group_cost[static_cast<size_t>(a)] > group_cost[static_cast<size_t>(b)];
-  });
-  std::vector<int> xcd_of(static_cast<size_t>(num_groups), 0);
-  double load[kXcds] = {0.0};
-  for (int64_t g : order) {
-    int best = 0;
-    for (int x = 1; x < kXcds; ++x)
-      if (load[x] < load[best]) best = x;
-    xcd_of[static_cast<size_t>(g)] = best;
-    load[best] += group_cost[static_cast<size_t>(g)];
+  // XCD bands: the tiles of each block product, in row-major order, are cut into kXcds contiguous
+  // ranges (bands of output rows when the product is taller than wide), range c on XCD
+  // (c + rotation) % kXcds, the rotation advancing with the tiles of earlier products so that
+  // products of few tiles spread over the XCDs.  Each XCD then reads the A rows of its band once
+  // and B once, where round-robin placement has every XCD read all of A; and every XCD gets an
+  // equal share of every product.  Each XCD keeps its tiles longest first, and position p of the
+  // launch holds a tile of XCD p % 8; when an XCD runs out, the position takes a tile from the
+  // fullest one.
+  std::vector<int64_t> count(static_cast<size_t>(num_groups), 0), first(static_cast<size_t>(num_groups), 0);
+  for (const Costed &c : costed) ++count[static_cast<size_t>(c.tile.group)];
+  for (int64_t g = 1; g < num_groups; ++g)
+    first[static_cast<size_t>(g)] = first[static_cast<size_t>(g - 1)] + count[static_cast<size_t>(g - 1)];
+  std::vector<int64_t> seen(static_cast<size_t>(num_groups), 0);
+  std::vector<int> xcd_of_tile(costed.size(), 0);
+  for (size_t i = 0; i < costed.size(); ++i) {
+    const size_t g = static_cast<size_t>(costed[i].tile.group);
+    const int64_t index = seen[g]++;  // row-major: a product's tiles keep their order in `costed`
+    xcd_of_tile[i] = static_cast<int>((first[g] + index * kXcds / count[g]) % kXcds);
   }
   std::vector<std::vector<GemmTile>> bins(kXcds);
-  for (const Costed &c : costed) bins[static_cast<size_t>(xcd_of[static_cast<size_t>(c.tile.group)])].push_back(c.tile);
+  for (size_t i = 0; i < costed.size(); ++i) bins[static_cast<size_t>(xcd_of_tile[i])].push_back(costed[i].tile);
   std::vector<size_t> next(kXcds, 0);
   for (size_t p = 0; p < costed.size(); ++p) {
     int x = static_cast<int>(p % kXcds);
@@ -472,16 +467,16 @@ void tile_list(const std::vector<KGroup> &groups, const std::vector<KTerm> &term
   }
 }
 
-// The default tile shape, from the number of 64 x 64 tiles of the launch (split products counted
-// by their parts).  Measured on the DMRG matvec products (15-sector bonds, χ = 1024 and 2048,
-// FP32 and FP64 MFMA): from 2048 tiles 64 x 64 is best or within 8 % of it; between 512 and 2048
-// tiles 64 x 32 is best or within 12 % (FP64 deep products: 25 %); below that the launch is short
-// of parallel work and 32 x 32 tiles win.
+// The default configuration, from the number of 64 x 64 tiles of the launch (split products
+// counted by their parts).  Measured on the DMRG matvec products (15-sector bonds, χ = 1024 and
+// 2048, FP32 and FP64 MFMA): from 2048 tiles 64 x 64 tiles in XCD bands are best or within 3 % of
+// the best; between 512 and 2048 tiles 64 x 32 tiles in XCD bands are best or within 8 %; below
+// that the launch is short of parallel work and 32 x 32 tiles, longest first, win.
 int choose_shape(const std::vector<KGroup> &groups, const std::vector<char> &skip) {
   int64_t tiles64 = 0;
   for (size_t g = 0; g < groups.size(); ++g)
     if (!skip[g]) tiles64 += ((groups[g].m + 63) / 64) * ((groups[g].n + 63) / 64);
-  return tiles64 >= 2048 ? 0 : tiles64 >= 512 ? 3 : 1;
+  return tiles64 >= 2048 ? kNumShapes * kXcdBands + 0 : tiles64 >= 512 ? kNumShapes * kXcdBands + 3 : 1;
 }
 
 // Depth split.  A launch whose products are few and deep (the DMRG matvec's second step: 16-29
@@ -493,7 +488,7 @@ int choose_shape(const std::vector<KGroup> &groups, const std::vector<char> &ski
 constexpr int64_t kSplitTiles = 16 * 256;  // a plan with at least this many 32 x 32 tiles is not split
 constexpr int64_t kSplitMinDepth = 512;    // nor is a product shallower than this
 constexpr int64_t kSplitSide = 128;        // a part is about max(m, n, 128) deep
-constexpr int64_t kMaxParts = 8;
+constexpr int64_t kMaxParts = 7;             // a split product has at most kMaxParts + 1 parts
 constexpr int64_t kPieceAlign = 32;        // part boundaries at multiples of 32 within a term
 
 // The device tables of a plan: the caller's groups and terms first (the per-block form indexes
@@ -543,7 +538,9 @@ void split_tables(const tt_gemm_group *groups, int64_t num_groups, const tt_gemm
         part_depth += len;
       }
     }
-    if (parts.size() < 2) continue;
+    // Every part but the last is at least `chunk` deep, so there are at most want + 1 parts; the
+    // bound on the sum launch's loads is checked all the same.
+    if (parts.size() < 2 || parts.size() > static_cast<size_t>(kMaxLoadedParts)) continue;
     const int64_t elems = grp.m * grp.n;
     d.skip[static_cast<size_t>(g)] = 1;
     const int32_t rg = static_cast<int32_t>(d.rgroups.size());

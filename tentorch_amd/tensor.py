@@ -421,7 +421,8 @@ class Profiler:
                 continue
             if compute is not None and kind == 'gemm' and kind_compute != compute:
                 continue
-            entry = out.setdefault(kind, {'launches': 0, 'ms': 0.0, 'flops': 0.0, 'bytes': 0.0})
+            entry = out.setdefault(kind, {'ops': 0, 'launches': 0, 'ms': 0.0, 'flops': 0.0, 'bytes': 0.0})
+            entry['ops'] += 1
             entry['launches'] += launches
             entry['ms'] += ms
             entry['flops'] += flops
