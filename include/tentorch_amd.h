@@ -97,6 +97,17 @@ typedef struct tt_gemm_term {
 
 typedef struct tt_ggemm_plan tt_ggemm_plan;
 
+/*
+ * Limits: every block side and depth below 2^20, and every operand block (m k or k n elements) below
+ * 2^27 elements, so that a tile's byte offsets and its operand ranges stay in 32 bits; larger blocks
+ * are refused with an error.  A plan whose launch has few tiles splits its deep products (depth at
+ * least 512 and at least twice max(m, n, 128)) into at most eight depth parts; `tt_ggemm_execute`
+ * then runs a second launch that adds the parts' partial sums in part order, so the result still
+ * does not depend on the configuration.  The partial sums live in a buffer the library keeps per
+ * device, allocated when a plan is created (never while a stream is being captured, if the plan
+ * was created before) and kept until the process ends; launches of split plans on one device share
+ * it and must not overlap (one stream at a time, as everywhere in this library).
+ */
 int tt_ggemm_plan_create(const tt_gemm_group *groups, int64_t num_groups, const tt_gemm_term *terms,
                          int64_t num_terms, tt_ggemm_plan **plan);
 /* The same with an explicit tile shape (0 <= config < tt_ggemm_num_configs(); a negative config lets
@@ -104,9 +115,9 @@ int tt_ggemm_plan_create(const tt_gemm_group *groups, int64_t num_groups, const 
 int tt_ggemm_plan_create_config(const tt_gemm_group *groups, int64_t num_groups, const tt_gemm_term *terms,
                                 int64_t num_terms, int config, tt_ggemm_plan **plan);
 /* Tile shapes the library is built with: rows x cols of the output tile, depth of one k-slab. */
-/* Configurations: c % 7 picks the tile shape (BM, BN, BK), c / 7 the packing of tiles into the
- * launch: 0 longest K first, 1 all tiles of a block product on one XCD (workgroup i runs on XCD
- * i % 8), block products spread over the XCDs by cost. */
+/* Configurations: c % 7 picks the tile shape (BM, BN, BK), c / 7 the placement of tiles in the
+ * launch: 0 longest K first; 1 XCD bands: each block product's tiles, row-major, cut into eight
+ * contiguous bands, one per XCD (workgroup i runs on XCD i % 8), each XCD's tiles longest K first. */
 int tt_ggemm_num_configs(void);
 int tt_ggemm_config_shape(int config, int *rows, int *cols, int *depth);
 /* The tile shape a plan uses. */
@@ -115,9 +126,10 @@ int tt_ggemm_plan_destroy(tt_ggemm_plan *plan);
 /* Real multiply-add count of the plan (2 m n k summed over terms), and tile count of one launch. */
 int tt_ggemm_plan_counts(const tt_ggemm_plan *plan, double *flops, int64_t *tiles);
 /*
- * One launch for every output block.  `a_type`, `b_type`, `c_type` are the element types of the
- * three buffers (TT_F64 or TT_F32); `compute_type` TT_F64 runs FP64 MFMA with FP64 accumulation,
- * TT_F32 runs FP32 MFMA with FP32 accumulation (operands rounded to FP32 on load).
+ * One launch for every output block (two when the plan splits deep products: the parts, then the
+ * sum of their partial sums).  `a_type`, `b_type`, `c_type` are the element types of the three
+ * buffers (TT_F64 or TT_F32); `compute_type` TT_F64 runs FP64 MFMA with FP64 accumulation, TT_F32
+ * runs FP32 MFMA with FP32 accumulation (operands rounded to FP32 when staged).
  */
 int tt_ggemm_execute(const tt_ggemm_plan *plan, int a_type, int b_type, int c_type, int compute_type,
                      const void *a, const void *b, void *c, void *stream);

@@ -59,7 +59,7 @@ def per_kernel(fetch_dir, write_dir, copy_mib):
         rows.sort(key=lambda r: int(r['Dispatch_Id']))
         copies[counter] = float([r for r in rows if COPY in r['Kernel_Name']][-1]['Counter_Value'])
         for r in rows:
-            name = r['Kernel_Name'].split('(anonymous namespace)::')[-1].split('(')[0]
+            name = r['Kernel_Name'].replace('(anonymous namespace)::', '').split('(')[0]
             entry = table.setdefault(name, {'fetch': [], 'write': []})
             entry[counter].append(float(r['Counter_Value']))
     scale = {k: copy_mib * 1024.0 / v for k, v in copies.items()}
