@@ -62,7 +62,11 @@ struct DevTask {
 
 constexpr int kGssThreads = 256;
 constexpr int kGssUnroll = 4;
-constexpr int kGssChunk = kGssThreads * kGssUnroll * 4;
+// One pass per workgroup: a chunk is one element per (thread, unroll slot), so a large move runs as
+// many short workgroups (≈ 2700 for the DMRG matvec's 2.8 M-element move at χ = 1024) and its
+// memory latency is hidden by occupancy, where four passes per workgroup left under three
+// workgroups per CU.
+constexpr int kGssChunk = kGssThreads * kGssUnroll;
 
 template <typename T>
 struct Scale;
@@ -118,27 +122,42 @@ __global__ __launch_bounds__(kGssThreads) void gss_kernel(const GssChunk *__rest
         }
       }
     }
-    T value[kGssUnroll];
+    // Every slot loads, a slot past the chunk's end from the tile's first source element (an
+    // address inside the source), so the loads carry no branch and are all in flight together;
+    // only the stores are masked.
+    bool live[kGssUnroll];
 #pragma unroll
-    for (int u = 0; u < kGssUnroll; ++u)
-      if (at + u * kGssThreads < chunk.count)
-        value[u] = ns > 0 ? Scale<T>::fma(Scale<T>::zero(), task.coef_re, task.coef_im, static_cast<T>(src[soff[u]]))
-                          : Scale<T>::zero();
+    for (int u = 0; u < kGssUnroll; ++u) live[u] = at + u * kGssThreads < chunk.count;
+    T value[kGssUnroll];
+    if (ns > 0) {
+      TS loaded[kGssUnroll];
+#pragma unroll
+      for (int u = 0; u < kGssUnroll; ++u) loaded[u] = src[live[u] ? soff[u] : task.src_offset];
+#pragma unroll
+      for (int u = 0; u < kGssUnroll; ++u)
+        value[u] = Scale<T>::fma(Scale<T>::zero(), task.coef_re, task.coef_im, static_cast<T>(loaded[u]));
+    } else {
+#pragma unroll
+      for (int u = 0; u < kGssUnroll; ++u) value[u] = Scale<T>::zero();
+    }
     for (int s = 1; s < ns; ++s) {
       const tt_gss_source &source = sources[task.first_source + s];
+      TS loaded[kGssUnroll];
 #pragma unroll
       for (int u = 0; u < kGssUnroll; ++u) {
-        if (at + u * kGssThreads >= chunk.count) continue;
         int64_t off = source.src_offset;
 #pragma unroll
         for (int d = 0; d < TT_MAX_DIMS; ++d)
           if (d < nd) off += static_cast<int64_t>(idx[u][d]) * source.src_stride[d];
-        value[u] = Scale<T>::fma(value[u], source.coef_re, source.coef_im, static_cast<T>(src[off]));
+        loaded[u] = src[live[u] ? off : source.src_offset];
       }
+#pragma unroll
+      for (int u = 0; u < kGssUnroll; ++u)
+        value[u] = Scale<T>::fma(value[u], source.coef_re, source.coef_im, static_cast<T>(loaded[u]));
     }
 #pragma unroll
     for (int u = 0; u < kGssUnroll; ++u)
-      if (at + u * kGssThreads < chunk.count) dst[doff[u]] = value[u];
+      if (live[u]) dst[doff[u]] = value[u];
   }
 }
 
