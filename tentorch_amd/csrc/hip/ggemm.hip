@@ -408,8 +408,13 @@ void tile_list(const std::vector<KGroup> &groups, const std::vector<KTerm> &term
     double cost;
     GemmTile tile;
   };
+  // Tiles longest first: every tile of a product costs the same (its slab count), so the products
+  // are ordered by cost (ties in list order) and each emits its tiles row-major, which is the
+  // order a stable sort of all tiles by cost gives, without sorting the tiles.
   const int64_t num_groups = static_cast<int64_t>(groups.size());
-  std::vector<Costed> costed;
+  std::vector<double> group_cost(static_cast<size_t>(num_groups), 0.0);
+  std::vector<int64_t> order;
+  int64_t count_all = 0;
   *max_cost = 0.0;
   for (int64_t g = 0; g < num_groups; ++g) {
     if (skip[static_cast<size_t>(g)]) continue;
@@ -417,14 +422,25 @@ void tile_list(const std::vector<KGroup> &groups, const std::vector<KTerm> &term
     int64_t ksum = 0;
     for (int32_t t = grp.first_term; t < grp.first_term + grp.num_terms; ++t) ksum += terms[static_cast<size_t>(t)].k;
     const double cost = static_cast<double>((ksum + shape.bk - 1) / shape.bk) + 1.0;
+    group_cost[static_cast<size_t>(g)] = cost;
     *max_cost = std::max(*max_cost, cost);
-    for (int64_t mt = 0; mt < grp.m; mt += shape.bm)
-      for (int64_t nt = 0; nt < grp.n; nt += shape.bn) {
-        costed.push_back({cost, GemmTile{static_cast<int32_t>(g), static_cast<int32_t>(mt), static_cast<int32_t>(nt), 0}});
-      }
+    order.push_back(g);
+    count_all += ((grp.m + shape.bm - 1) / shape.bm) * ((grp.n + shape.bn - 1) / shape.bn);
   }
-  std::stable_sort(costed.begin(), costed.end(), [](const Costed &a, const Costed &b) { return a.cost > b.cost; });
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+    return group_cost[static_cast<size_t>(a)] > group_cost[static_cast<size_t>(b)];
+  });
+  std::vector<Costed> costed;
+  costed.reserve(static_cast<size_t>(count_all));
+  for (int64_t g : order) {
+    const KGroup &grp = groups[static_cast<size_t>(g)];
+    for (int64_t mt = 0; mt < grp.m; mt += shape.bm)
+      for (int64_t nt = 0; nt < grp.n; nt += shape.bn)
+        costed.push_back({group_cost[static_cast<size_t>(g)],
+                          GemmTile{static_cast<int32_t>(g), static_cast<int32_t>(mt), static_cast<int32_t>(nt), 0}});
+  }
   tiles.clear();
+  tiles.reserve(costed.size());
   if (packing == kLongestFirst) {
     for (const Costed &c : costed) tiles.push_back(c.tile);
     return;
