@@ -133,6 +133,33 @@ def svd(tensor: Tensor, left: Optional[Sequence[int]] = None, max_bond: Optional
     return _split(tensor, left, max_bond, tol, absorb)[:3]
 
 
+def _paired_prefixes(cand_owner: np.ndarray, cand_position: np.ndarray, sectors, room) -> np.ndarray:
+    '''The re-pairing of `split_over` (`svd.cpp:97-119`): walking the ranked candidates (block
+    `cand_owner[i]`, singular value `cand_position[i]`, largest first), keep a candidate while its
+    block's sector has kept fewer than `room[sector]` (a sector without room keeps none); a block keeps
+    the prefix up to its last kept candidate.  Vectorised: a candidate's rank among the candidates of
+    its sector decides.  Returns the kept count of every block.'''
+    kept = np.zeros(len(sectors), dtype=np.int64)
+    if cand_owner.size == 0:
+        return kept
+    ids, sector_id = {}, np.empty(len(sectors), dtype=np.int64)
+    for fi, s in enumerate(sectors):
+        sector_id[fi] = ids.setdefault(s, len(ids))
+    limit = np.full(len(ids), -1, dtype=np.int64)
+    for s, i in ids.items():
+        if s in room:
+            limit[i] = room[s]
+    cand_sector = sector_id[cand_owner]
+    by_sector = np.argsort(cand_sector, kind='stable')
+    sorted_sector = cand_sector[by_sector]
+    starts = np.searchsorted(sorted_sector, sorted_sector, side='left')
+    rank = np.empty_like(cand_sector)
+    rank[by_sector] = np.arange(sorted_sector.size) - starts
+    keep = rank < limit[cand_sector]
+    np.maximum.at(kept, cand_owner[keep], cand_position[keep] + 1)
+    return kept
+
+
 def _split(tensor: Tensor, left: Optional[Sequence[int]], max_bond: Optional[int], tol: float, absorb: int):
     '''`svd`, plus the kept singular values on the host (FP64, in the order of `values`), which the
     split reads back anyway to rank them: a caller that needs their sum needs no further
@@ -181,9 +208,9 @@ def _split(tensor: Tensor, left: Optional[Sequence[int]], max_bond: Optional[int
     order = order[:above[0] if above.size else order.size]
     if max_bond is not None:
         order = order[:max_bond]
-    candidates = list(zip(owner[order].tolist(), position[order].tolist()))
+    cand_owner, cand_position = owner[order], position[order]
     kept = np.zeros(len(factors), dtype=np.int64)
-    np.maximum.at(kept, owner[order], position[order] + 1)
+    np.maximum.at(kept, cand_owner, cand_position + 1)
     for fi, f in enumerate(factors):
         f['kept'] = int(kept[fi])
     mults = {}
@@ -197,17 +224,10 @@ def _split(tensor: Tensor, left: Optional[Sequence[int]], max_bond: Optional[int
         if other is not None:
             paired[sector] = min(count, other)
     if paired and paired != mults:
-        for f in factors:
-            f['kept'] = 0
-        used = {}
-        for fi, index in candidates:
-            f = factors[fi]
-            s = f['key'].domain
-            room = paired.get(s)
-            if room is None or used.get(s, 0) >= room:
-                continue
-            used[s] = used.get(s, 0) + 1
-            f['kept'] = max(f['kept'], index + 1)
+        sectors = [f['key'].domain for f in factors]
+        kept = _paired_prefixes(cand_owner, cand_position, sectors, paired)
+        for fi, f in enumerate(factors):
+            f['kept'] = int(kept[fi])
         mults = paired
     nxt = {}
     for f in factors:
