@@ -153,6 +153,18 @@ int tt_multidot(int64_t n, int k, const double *V, int64_t ldv, const double *w,
 /* w += scale * sum_{i < k} coef[i] V[i][:], coefficients read from device memory. */
 int tt_multiaxpy(int64_t n, int k, const double *coef, double scale, const double *V, int64_t ldv, double *w,
                  void *stream);
+/* The same over the first min(k, *count) rows, `count` read from device memory (a Lanczos basis past
+ * an early stop may hold non-finite rows, which are then never read). */
+int tt_multiaxpy_counted(int64_t n, int k, const int32_t *count, const double *coef, double scale, const double *V,
+                         int64_t ldv, double *w, void *stream);
+/* The Ritz pair of a Lanczos run from its device scalars, on the device (the host `eigh` of the
+ * tridiagonal and the coefficient extraction of tentorch/linalg/lanczos.py:52-62): size = the first
+ * j + 1 with beta[j] < tol, else k (k <= 32); the tridiagonal of alpha[0..size) and
+ * beta[0..size-1) diagonalised by implicit QL; *ritz = its least eigenvalue, coef[0..k) its
+ * eigenvector (zero past size, first coefficient non-negative), *size = size.  One launch, no host
+ * sync. */
+int tt_lanczos_ritz(int k, const double *alpha, const double *beta, double tol, double *coef, double *ritz,
+                    int32_t *size, void *stream);
 /* One Krylov step's vector work after the action (tentorch/linalg/lanczos.py:36-50): w, the image of
  * row k of V, is orthogonalised against rows 0..k of V by two classical Gram-Schmidt passes, each
  * subtraction fused with what follows it (the second projection, then the norm); *alpha = the two

@@ -54,6 +54,8 @@ _lib.tt_timing_collect.argtypes = [ctypes.POINTER(_dbl), _i64, ctypes.POINTER(_i
 _lib.tt_launch_count.restype = ctypes.c_int64
 _lib.tt_multidot.argtypes = [_i64, _int, _c_void_p, _i64, _c_void_p, _c_void_p, _c_void_p]
 _lib.tt_multiaxpy.argtypes = [_i64, _int, _c_void_p, _dbl, _c_void_p, _i64, _c_void_p, _c_void_p]
+_lib.tt_multiaxpy_counted.argtypes = [_i64, _int, _c_void_p, _c_void_p, _dbl, _c_void_p, _i64, _c_void_p, _c_void_p]
+_lib.tt_lanczos_ritz.argtypes = [_int, _c_void_p, _c_void_p, _dbl, _c_void_p, _c_void_p, _c_void_p, _c_void_p]
 _lib.tt_lanczos_orthonormalize.argtypes = [_i64, _int, _c_void_p, _i64, _c_void_p, _c_void_p, _c_void_p, _c_void_p,
                                            _c_void_p]
 _lib.tt_graph_begin.argtypes = [_c_void_p]
@@ -69,7 +71,7 @@ EXPORTED = ['tt_gss_plan_create', 'tt_gss_plan_destroy', 'tt_gss_plan_counts', '
             'tt_graph_end', 'tt_graph_launch', 'tt_graph_destroy', 'tt_last_error', 'tt_version',
             'tt_launch_count', 'tt_timing_enable', 'tt_timing_collect', 'tt_mlp_param_count', 'tt_mlp_forward',
             'tt_mlp_gradient', 'tt_mlp_adam', 'tt_eigh_max_order', 'tt_eigh_workspace', 'tt_eigh_batched',
-            'tt_lanczos_orthonormalize']
+            'tt_lanczos_orthonormalize', 'tt_lanczos_ritz', 'tt_multiaxpy_counted']
 
 
 def library() -> ctypes.CDLL:
@@ -151,10 +153,23 @@ def multidot(V: torch.Tensor, k: int, w: torch.Tensor, out: torch.Tensor) -> Non
           'multidot')
 
 
-def multiaxpy(V: torch.Tensor, k: int, coef: torch.Tensor, scale: float, w: torch.Tensor) -> None:
-    '''w += scale * coef[:k] @ V[:k] with the coefficients on the device.'''
-    check(_lib.tt_multiaxpy(w.numel(), k, coef.data_ptr(), float(scale), V.data_ptr(), V.stride(0), w.data_ptr(),
-                            stream()), 'multiaxpy')
+def multiaxpy(V: torch.Tensor, k: int, coef: torch.Tensor, scale: float, w: torch.Tensor,
+              count: Optional[torch.Tensor] = None) -> None:
+    '''w += scale * coef[:k] @ V[:k] with the coefficients on the device; with `count` (a device
+    int32), only the first min(k, count) rows of V are read.'''
+    if count is None:
+        check(_lib.tt_multiaxpy(w.numel(), k, coef.data_ptr(), float(scale), V.data_ptr(), V.stride(0), w.data_ptr(),
+                                stream()), 'multiaxpy')
+        return
+    check(_lib.tt_multiaxpy_counted(w.numel(), k, count.data_ptr(), coef.data_ptr(), float(scale), V.data_ptr(),
+                                    V.stride(0), w.data_ptr(), stream()), 'multiaxpy')
+
+
+def lanczos_ritz(k: int, alphas: torch.Tensor, betas: torch.Tensor, tol: float, coef: torch.Tensor,
+                 ritz: torch.Tensor, size: torch.Tensor) -> None:
+    '''The Ritz pair of a Lanczos run from its device scalars, on the device (tt_lanczos_ritz).'''
+    check(_lib.tt_lanczos_ritz(k, alphas.data_ptr(), betas.data_ptr(), float(tol), coef.data_ptr(), ritz.data_ptr(),
+                               size.data_ptr(), stream()), 'lanczos ritz')
 
 
 def lanczos_orthonormalize(V: torch.Tensor, k: int, w: torch.Tensor, coef: torch.Tensor, alpha: torch.Tensor,

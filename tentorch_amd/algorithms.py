@@ -12,7 +12,7 @@ from typing import Dict, Literal
 import numpy as np
 import torch
 
-from .linalg import _split, lanczos
+from .linalg import _split, lanczos_deferred
 from .models import Model, heisenberg
 from .networks import MPS, identity
 from .tensor import Settings, Tensor, contract, domain_legs, einsum, inner
@@ -78,19 +78,28 @@ class DMRG:
         def action(vector: Tensor) -> Tensor:
             return einsum('EPaqj,JRSFb,AQsb->EPrf', head, tail, vector)
 
+        # Lanczos with its tridiagonal solved on the device: no host synchronisation before the
+        # split, whose readback also brings the Ritz values staged here (read once it is done).
         if self.precision == 'fp32':
             # The Krylov steps run their block products on FP32 MFMA; the Ritz vector is then
             # corrected in FP64 by a short Lanczos restart from it with FP64 block products.
             with Settings.using(precision='fp32'):
-                energy, vector = lanczos(action, two, self.n_iter)
+                first_value, vector = lanczos_deferred(action, two, self.n_iter)
+            first_value.stage()
             if self.refine:
-                energy, vector = lanczos(action, vector, self.refine)
+                value, vector = lanczos_deferred(action, vector, self.refine)
+            else:
+                value = first_value
         else:
-            energy, vector = lanczos(action, two, self.n_iter)
+            first_value, vector = lanczos_deferred(action, two, self.n_iter)
+            value = first_value
+        value.stage()
         factor = 2 if direction == 'right' else 1
         # the split's kept singular values come back on the host with it: the truncated weight and
         # the zero-norm check need no further synchronisation, nor does the normalisation
         left, values, right, kept = _split(vector, None, self.max_bond_dim, self.svd_tol, factor)
+        first_value.get()
+        energy = value.get()
         self.state.tensors[at], self.state.tensors[at + 1] = left, right.place([0, 1])
         self.center = at + 1 if direction == 'right' else at
         weight = float(np.sum(kept ** 2))

@@ -569,13 +569,100 @@ __global__ __launch_bounds__(kDotThreads) void multidot_stage2(const double *__r
   if (threadIdx.x == 0) out[i] = sh[0];
 }
 
-__global__ void multiaxpy_kernel(int64_t n, int k, const double *__restrict__ coef, double scale,
-                                 const double *__restrict__ V, int64_t ldv, double *__restrict__ w) {
+// `count`, when given, caps k from device memory: rows of V past it are never read (a Lanczos basis
+// past an early stop may hold non-finite rows).
+__global__ void multiaxpy_kernel(int64_t n, int k, const int32_t *__restrict__ count, const double *__restrict__ coef,
+                                 double scale, const double *__restrict__ V, int64_t ldv, double *__restrict__ w) {
+  const int rows = count ? min(k, *count) : k;
   for (int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < n;
        j += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     double acc = 0.0;
-    for (int i = 0; i < k; ++i) acc += coef[i] * V[i * ldv + j];
+    for (int i = 0; i < rows; ++i) acc += coef[i] * V[i * ldv + j];
     w[j] += scale * acc;
+  }
+}
+
+// The Ritz pair of a Lanczos run from its device scalars (lanczos.py:52-62): the tridiagonal of
+// alpha[0..size) and beta[0..size-1), where size stops at the first beta below `tol` (the
+// reference's break), is diagonalised by implicit QL with Wilkinson-type shifts and eigenvector
+// accumulation (the tql2 iteration of EISPACK).  One wave: every lane runs the scalar recurrence on
+// the wave's copy of the tridiagonal in LDS, and lane r applies the rotations to row r of the
+// eigenvector matrix.  Writes the least eigenvalue, its eigenvector as the coefficients of the
+// basis (zero past size; the sign makes the first coefficient non-negative, the Ritz vector's
+// overlap with the start), and size.
+constexpr int kRitzMax = 32;
+__global__ __launch_bounds__(64) void lanczos_ritz_kernel(int k, const double *__restrict__ alpha,
+                                                          const double *__restrict__ beta, double tol,
+                                                          double *__restrict__ coef, double *__restrict__ ritz,
+                                                          int32_t *__restrict__ size_out) {
+  __shared__ double d[kRitzMax], e[kRitzMax], z[kRitzMax][kRitzMax + 1];
+  const int lane = threadIdx.x;
+  int m = k;
+  for (int j = 0; j < k; ++j)
+    if (beta[j] < tol) {
+      m = j + 1;
+      break;
+    }
+  if (lane < m) {
+    d[lane] = alpha[lane];
+    e[lane] = lane + 1 < m ? beta[lane] : 0.0;
+    for (int c = 0; c < m; ++c) z[lane][c] = lane == c ? 1.0 : 0.0;
+  }
+  __builtin_amdgcn_wave_barrier();
+  constexpr double kEps = 2.220446049250313e-16;
+  for (int l = 0; l < m; ++l) {
+    for (int iter = 0; iter < 60; ++iter) {
+      int mm = l;
+      for (; mm < m - 1; ++mm)
+        if (fabs(e[mm]) <= kEps * (fabs(d[mm]) + fabs(d[mm + 1]))) break;
+      if (mm == l) break;
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = hypot(g, 1.0);
+      g = d[mm] - d[l] + e[l] / (g + copysign(r, g));
+      double sn = 1.0, cs = 1.0, p = 0.0;
+      bool split = false;
+      for (int i = mm - 1; i >= l; --i) {
+        const double f = sn * e[i], bb = cs * e[i];
+        r = hypot(f, g);
+        __builtin_amdgcn_wave_barrier();
+        e[i + 1] = r;
+        if (r == 0.0) {
+          d[i + 1] -= p;
+          e[mm] = 0.0;
+          split = true;
+          break;
+        }
+        sn = f / r;
+        cs = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * sn + 2.0 * cs * bb;
+        p = sn * r;
+        __builtin_amdgcn_wave_barrier();
+        d[i + 1] = g + p;
+        g = cs * r - bb;
+        if (lane < m) {
+          const double zi1 = z[lane][i + 1], zi = z[lane][i];
+          z[lane][i + 1] = sn * zi + cs * zi1;
+          z[lane][i] = cs * zi - sn * zi1;
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (split) continue;
+      d[l] -= p;
+      e[l] = g;
+      e[mm] = 0.0;
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  int least = 0;
+  for (int c = 1; c < m; ++c)
+    if (d[c] < d[least]) least = c;
+  const double sign = z[0][least] < 0.0 ? -1.0 : 1.0;
+  if (lane < k) coef[lane] = lane < m ? sign * z[lane][least] : 0.0;
+  if (lane == 0) {
+    ritz[0] = d[least];
+    size_out[0] = m;
   }
 }
 
@@ -720,11 +807,26 @@ extern "C" int tt_multidot(int64_t n, int k, const double *V, int64_t ldv, const
 
 extern "C" int tt_multiaxpy(int64_t n, int k, const double *coef, double scale, const double *V, int64_t ldv, double *w,
                             void *stream) {
+  return tt_multiaxpy_counted(n, k, nullptr, coef, scale, V, ldv, w, stream);
+}
+
+extern "C" int tt_multiaxpy_counted(int64_t n, int k, const int32_t *count, const double *coef, double scale,
+                                    const double *V, int64_t ldv, double *w, void *stream) {
   if (n < 0 || k < 0) return fail("tt_multiaxpy: negative size");
   if (n == 0 || k == 0) return 0;
   if (!V || !w || !coef) return fail("tt_multiaxpy: null buffer");
-  hipLaunchKernelGGL(multiaxpy_kernel, dim3(elementwise_grid(n)), dim3(256), 0, as_stream(stream), n, k, coef, scale, V,
-                     ldv, w);
+  hipLaunchKernelGGL(multiaxpy_kernel, dim3(elementwise_grid(n)), dim3(256), 0, as_stream(stream), n, k, count, coef,
+                     scale, V, ldv, w);
+  TT_LAUNCHED();
+  return 0;
+}
+
+extern "C" int tt_lanczos_ritz(int k, const double *alpha, const double *beta, double tol, double *coef, double *ritz,
+                               int32_t *size, void *stream) {
+  if (k < 1 || k > kRitzMax) return fail("tt_lanczos_ritz: k must lie in [1, 32]");
+  if (!alpha || !beta || !coef || !ritz || !size) return fail("tt_lanczos_ritz: null buffer");
+  hipLaunchKernelGGL(lanczos_ritz_kernel, dim3(1), dim3(64), 0, as_stream(stream), k, alpha, beta, tol, coef, ritz,
+                     size);
   TT_LAUNCHED();
   return 0;
 }

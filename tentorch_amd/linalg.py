@@ -350,22 +350,17 @@ def lanczos(action: Callable[[Tensor], Tensor], start: Tensor, n_iter: int = 5,
     return _lanczos_host(action, start, n_iter, tol)
 
 
-def _lanczos_device(action, start: Tensor, n_iter: int, tol: float) -> Tuple[float, Tensor]:
-    '''The same recurrence with every scalar on the device.  Each Krylov step: the action, then one
-    library call (`tt_lanczos_orthonormalize`): two classical Gram-Schmidt passes of w = Hv_k against
-    the whole basis, alpha = its projections on v_k (<v_k, Hv_k> plus the second pass's correction),
-    beta = |w|, v_{k+1} = w / beta.  The reference subtracts alpha v_k + beta_{k-1} v_{k-1} first and
-    then runs two modified Gram-Schmidt passes (`lanczos.py:36-45`); the first full pass removes the
-    same components, so the two agree up to rounding.  The alphas and betas are read back once; the
-    break at beta < tol (`lanczos.py:46-48`) is applied to what was read.'''
+def _krylov_basis(action, start: Tensor, n_iter: int):
+    '''The Krylov basis V (n_iter + 1 rows) and the device scalars of a Lanczos run.  `scalars` holds
+    the alphas, the betas, each step's scratch (2 (k+1) + 1), the Ritz coefficients (n_iter + 1) and
+    six work scalars; work[2] is the squared norm of the start.  Each Krylov step is the action, then
+    one library call (`tt_lanczos_orthonormalize`).'''
     size_n = start.structure.size
     device = start.device
     V = torch.empty((n_iter + 1, size_n), dtype=torch.float64, device=device)
-    # alphas, betas, the step's scratch (2 (k+1) + 1), the Ritz coefficients, four work scalars
-    scalars = torch.zeros(5 * n_iter + 8, dtype=torch.float64, device=device)
+    scalars = torch.zeros(5 * n_iter + 10, dtype=torch.float64, device=device)
     alphas, betas = scalars[:n_iter], scalars[n_iter:2 * n_iter]
     scratch = scalars[2 * n_iter:4 * n_iter + 3]
-    coef = scalars[4 * n_iter + 3:5 * n_iter + 4]
     work = scalars[5 * n_iter + 4:]
     # v_0 = start / |start| with the norm kept on the device (the dot of `norm`, then a correctly
     # rounded square root and reciprocal, as on the host); a zero start is refused once the
@@ -380,6 +375,21 @@ def _lanczos_device(action, start: Tensor, n_iter: int, tol: float) -> Tuple[flo
             raise ValueError('the action has to map the legs of the start vector to themselves')
         w = image.data if image.data.data_ptr() != V[k].data_ptr() else image.data.clone()
         _native.lanczos_orthonormalize(V, k, w, scratch, alphas[k:k + 1], betas[k:k + 1])
+    return V, scalars, work
+
+
+def _lanczos_device(action, start: Tensor, n_iter: int, tol: float) -> Tuple[float, Tensor]:
+    '''The same recurrence with every scalar on the device.  Each Krylov step: the action, then one
+    library call (`tt_lanczos_orthonormalize`): two classical Gram-Schmidt passes of w = Hv_k against
+    the whole basis, alpha = its projections on v_k (<v_k, Hv_k> plus the second pass's correction),
+    beta = |w|, v_{k+1} = w / beta.  The reference subtracts alpha v_k + beta_{k-1} v_{k-1} first and
+    then runs two modified Gram-Schmidt passes (`lanczos.py:36-45`); the first full pass removes the
+    same components, so the two agree up to rounding.  The alphas and betas are read back once; the
+    break at beta < tol (`lanczos.py:46-48`) is applied to what was read.'''
+    size_n = start.structure.size
+    device = start.device
+    V, scalars, work = _krylov_basis(action, start, n_iter)
+    coef = scalars[4 * n_iter + 3:5 * n_iter + 4]
     host = scalars.cpu().numpy()
     if host[5 * n_iter + 6] == 0.0:
         raise ValueError('Lanczos needs a starting vector of non-zero norm.')
@@ -403,6 +413,64 @@ def _lanczos_device(action, start: Tensor, n_iter: int, tol: float) -> Tuple[flo
     torch.rsqrt(work[0:1], out=work[1:2])
     _native.axpby(torch.float64, 0.0, out, 0.0, out, alpha_dev=work[1:2])
     return float(values[0]), Tensor._wrap(start.structure, out)
+
+
+class RitzValue:
+    '''A Ritz value that may still be on the device.  `stage()` queues its copy (with the squared
+    norm of the run's start vector) to pinned host memory behind an event; `get()` waits for that
+    event, refuses a zero start as `lanczos` does, and returns the value.'''
+
+    def __init__(self, device_scalars: Optional[torch.Tensor] = None, host_value: Optional[float] = None):
+        self._device = device_scalars      # [ritz value, squared start norm]
+        self._value = host_value
+        self._staged = None
+        self._ready = None
+
+    def stage(self) -> 'RitzValue':
+        if self._device is not None and self._staged is None:
+            self._staged = torch.empty(2, dtype=torch.float64, pin_memory=True)
+            self._staged.copy_(self._device, non_blocking=True)
+            self._ready = torch.cuda.Event()
+            self._ready.record()
+        return self
+
+    def get(self) -> float:
+        if self._value is None:
+            self.stage()
+            self._ready.synchronize()
+            host = self._staged.numpy()
+            if host[1] == 0.0:
+                raise ValueError('Lanczos needs a starting vector of non-zero norm.')
+            self._value = float(host[0])
+        return self._value
+
+
+def lanczos_deferred(action: Callable[[Tensor], Tensor], start: Tensor, n_iter: int = 5,
+                     tol: float = 1e-12) -> Tuple[RitzValue, Tensor]:
+    '''`lanczos` with no host synchronisation: the tridiagonal is solved on the device
+    (`tt_lanczos_ritz`), the Ritz vector formed from the basis rows before the stop
+    (`tt_multiaxpy_counted`) and normalised there, and the Ritz value left on the device until the
+    caller reads it (after a synchronisation it needs anyway).  Same recurrence as `lanczos`; the
+    Ritz vector's sign makes its overlap with the start non-negative.  Runs that the device path does
+    not take (complex vectors, more than 31 steps, host mode) run `lanczos` and return its value.'''
+    if n_iter < 1:
+        raise ValueError(f'Lanczos needs at least one step, not {n_iter}.')
+    if not (LanczosSettings.mode == 'device' and start.dtype == torch.float64 and n_iter + 1 <= 32):
+        value, vector = lanczos(action, start, n_iter, tol)
+        return RitzValue(host_value=value), vector
+    V, scalars, work = _krylov_basis(action, start, n_iter)
+    alphas, betas = scalars[:n_iter], scalars[n_iter:2 * n_iter]
+    coef = scalars[4 * n_iter + 3:5 * n_iter + 4]
+    size = torch.empty(1, dtype=torch.int32, device=start.device)
+    _native.lanczos_ritz(n_iter, alphas, betas, tol, coef, work[4:5], size)
+    out = torch.zeros(start.structure.size, dtype=torch.float64, device=start.device)
+    _native.multiaxpy(V, n_iter, coef, 1.0, out, count=size)
+    _native.dot(torch.float64, out, out, work[0:1])
+    torch.rsqrt(work[0:1], out=work[1:2])
+    _native.axpby(torch.float64, 0.0, out, 0.0, out, alpha_dev=work[1:2])
+    # [ritz value, squared start norm] side by side for one copy to the host
+    work[5:6].copy_(work[2:3])
+    return RitzValue(device_scalars=work[4:6]), Tensor._wrap(start.structure, out)
 
 
 def _lanczos_host(action, start: Tensor, n_iter: int, tol: float) -> Tuple[float, Tensor]:

@@ -64,6 +64,8 @@ def load():
     lib.tt_multidot.argtypes = [i64, i32, vp, i64, vp, vp, vp]
     lib.tt_multiaxpy.argtypes = [i64, i32, vp, ctypes.c_double, vp, i64, vp, vp]
     lib.tt_lanczos_orthonormalize.argtypes = [i64, i32, vp, i64, vp, vp, vp, vp, vp]
+    lib.tt_lanczos_ritz.argtypes = [i32, vp, vp, ctypes.c_double, vp, vp, vp, vp]
+    lib.tt_multiaxpy_counted.argtypes = [i64, i32, vp, vp, ctypes.c_double, vp, i64, vp, vp]
     lib.tt_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
     lib.tt_launch_count.restype = ctypes.c_int64
     return lib

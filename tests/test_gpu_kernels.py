@@ -166,6 +166,56 @@ def test_krylov_kernels_match_numpy(lib):
     assert np.abs(w.cpu().numpy() - ref_w).max() <= 1e-12 * np.abs(ref_w).max()
 
 
+@pytest.mark.parametrize('k,stop', [(1, None), (2, None), (5, None), (7, 3), (13, None), (31, 0), (32, 20)])
+def test_lanczos_ritz_matches_numpy(lib, k, stop):
+    '''tt_lanczos_ritz against numpy's eigh of the tridiagonal the reference builds: the size stops at
+    the first beta below tol, the least eigenvalue to 1e-12, its eigenvector (first component made
+    non-negative) to 1e-10, zero coefficients past the size.'''
+    rng = np.random.default_rng(100 + k)
+    a = rng.standard_normal(k)
+    b = np.abs(rng.standard_normal(k)) + 0.1
+    if stop is not None:
+        b[stop] = 1e-15
+    size = k
+    for j in range(k):
+        if b[j] < 1e-12:
+            size = j + 1
+            break
+    matrix = np.diag(a[:size]) + np.diag(b[:size - 1], 1) + np.diag(b[:size - 1], -1)
+    values, vectors = np.linalg.eigh(matrix)
+    ref = vectors[:, 0] * (1.0 if vectors[0, 0] >= 0 else -1.0)
+    da, db = torch.tensor(a, device='cuda'), torch.tensor(b, device='cuda')
+    coef = torch.full((k + 1,), float('nan'), dtype=torch.float64, device='cuda')
+    ritz = torch.zeros(1, dtype=torch.float64, device='cuda')
+    dsize = torch.zeros(1, dtype=torch.int32, device='cuda')
+    assert lib.tt_lanczos_ritz(k, da.data_ptr(), db.data_ptr(), 1e-12, coef.data_ptr(), ritz.data_ptr(),
+                               dsize.data_ptr(), _stream()) == 0, cabi.error(lib)
+    got = coef.cpu().numpy()
+    assert int(dsize.item()) == size
+    assert abs(ritz.item() - values[0]) <= 1e-12 * max(1.0, np.abs(values).max())
+    assert np.abs(got[:size] - ref).max() <= 1e-10
+    assert np.all(got[size:k] == 0.0)
+
+
+def test_counted_multiaxpy_skips_rows_past_the_count(lib):
+    '''tt_multiaxpy_counted reads only the first min(k, count) rows: rows past it hold NaN and the
+    result stays finite and equal to numpy's.'''
+    rng = np.random.default_rng(14)
+    n, k, count = 10007, 6, 4
+    V = torch.tensor(rng.standard_normal((k, n)), device='cuda')
+    V[count:] = float('nan')
+    coef = torch.tensor(rng.standard_normal(k), device='cuda')
+    w = torch.tensor(rng.standard_normal(n), device='cuda')
+    before = w.cpu().numpy()
+    dcount = torch.tensor([count], dtype=torch.int32, device='cuda')
+    assert lib.tt_multiaxpy_counted(n, k, dcount.data_ptr(), coef.data_ptr(), 0.5, V.data_ptr(), V.stride(0),
+                                    w.data_ptr(), _stream()) == 0, cabi.error(lib)
+    ref = before + 0.5 * (coef.cpu().numpy()[:count] @ V[:count].cpu().numpy())
+    got = w.cpu().numpy()
+    assert np.all(np.isfinite(got))
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
 def test_lanczos_orthonormalization_matches_numpy(lib):
     '''tt_lanczos_orthonormalize against numpy's two classical Gram-Schmidt passes on a 6-vector
     orthonormal basis: the projection on the last basis vector, the norm left, and the next basis

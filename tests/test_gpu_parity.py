@@ -114,6 +114,31 @@ def test_device_lanczos_matches_the_host_loop():
     assert abs(abs(tt.inner(v1, v2)) - 1.0) < 1e-10
 
 
+def test_deferred_lanczos_matches_lanczos():
+    '''The window's Lanczos with the tridiagonal solved on the device (no host synchronisation) gives
+    the Ritz value and vector of `lanczos` on one DMRG window, the vector's sign fixed by a
+    non-negative overlap with the start; a zero start is refused when the value is read.'''
+    from tentorch_amd.linalg import lanczos_deferred
+    driver = tt.DMRG(tt.Model(8, cat=tt.u1), max_bond_dim=16, n_iter=5)
+    driver.state.move_center(0)
+    driver.cache_environments()
+    at = driver.center
+    first, second = driver.state.tensors[at], driver.state.tensors[at + 1]
+    two = tt.contract(first, second, tt.domain_legs(first), [0])
+    head = tt.einsum('EIa,IPqj->EPaqj', driver.left[at], driver.operator.tensors[at])
+    tail = tt.einsum('JRsk,Fkb->JRSFb', driver.operator.tensors[at + 1], driver.right[at + 2])
+    action = lambda v: tt.einsum('EPaqj,JRSFb,AQsb->EPrf', head, tail, v)
+    e1, v1 = tt.lanczos(action, two, 5)
+    value, v2 = lanczos_deferred(action, two, 5)
+    assert abs(value.get() - e1) < 1e-12
+    assert abs(abs(tt.inner(v1, v2)) - 1.0) < 1e-10
+    assert tt.inner(two, v2).real >= 0.0
+    zero = tt.Tensor._wrap(two.structure, torch.zeros_like(two.data))
+    value, _ = lanczos_deferred(action, zero, 5)
+    with pytest.raises(ValueError, match='non-zero norm'):
+        value.get()
+
+
 def test_fp32_form_with_defect_correction_tracks_fp64():
     '''The FP32 form (FP32 block products in Lanczos, FP64 two-step correction) stays within 1e-10 of
     the FP64 sweep energy once the window problem is converged, and reaches ED at L = 8.'''
