@@ -195,8 +195,22 @@ __global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__r
   constexpr int A_TPR = BK / A_PER, B_TPR = BN / B_PER;
   static_assert(FM >= 1 && FN >= 1 && A_PER >= 1 && B_PER >= 1, "tile too small for four waves");
   static_assert(BK % A_PER == 0 && BN % B_PER == 0 && BK % 4 == 0, "tile does not split over the threads");
-  __shared__ TC As[2][BK][BM + kPad];
-  __shared__ TC Bs[2][BK][BN + kPad];
+  // FP32 tiles larger than 32 x 32 store their slabs depth-contiguous (As[row][k], Bs[col][k],
+  // rows padded by four so that sixteen lanes reading sixteen rows hit distinct banks): a lane then
+  // reads the four depths it feeds to four successive MFMAs with one 16-byte LDS read.  Other tiles
+  // keep the depth-major layout (As[k][row], Bs[k][col]), which measured faster on 32 x 32.  Every
+  // FP32 tile sums in one order: within each 16-deep chunk, MFMA kk takes depths 4 lk + kk from the
+  // lane groups lk (A and B permuted alike), so the order depends on the chunk, not the tile shape.
+  constexpr bool kFp32 = std::is_same<TC, float>::value;
+  constexpr bool kDepthRows = kFp32 && FM * FN >= 2;
+  static_assert(!kFp32 || BK % 16 == 0, "FP32 slabs are summed in 16-deep chunks");
+  static_assert(!kDepthRows || (kGemmThreads % BN == 0 && (kGemmThreads / BN) * B_PER == BK),
+                "depth-contiguous B slabs are loaded as whole columns of B_PER depths");
+  constexpr int kRowPad = 4;
+  constexpr int AR = kDepthRows ? BM : BK, AC = kDepthRows ? BK + kRowPad : BM + kPad;
+  constexpr int BR = kDepthRows ? BN : BK, BC = kDepthRows ? BK + kRowPad : BN + kPad;
+  __shared__ __align__(16) TC As[2][AR][AC];
+  __shared__ __align__(16) TC Bs[2][BR][BC];
 
   int32_t gi, m0, n0, t_begin, t_end;
   if (plain_group < 0) {
@@ -230,7 +244,11 @@ __global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__r
     for (int j = 0; j < FN; ++j) acc[i][j] = M::zero();
 
   const int a_row = tid / A_TPR, a_k = (tid % A_TPR) * A_PER;
-  const int b_k = tid / B_TPR, b_col = (tid % B_TPR) * B_PER;
+  // B: depth-major slabs take B_PER consecutive columns of one depth per thread; depth-contiguous
+  // ones take B_PER consecutive depths of one column (lanes run along a row of B either way, and
+  // the thread's store is then one contiguous LDS write)
+  const int b_k = kDepthRows ? (tid / BN) * B_PER : tid / B_TPR;
+  const int b_col = kDepthRows ? tid % BN : (tid % B_TPR) * B_PER;
 
   // The slabs of the sum: BK-deep k-ranges of each term in list order (terms of depth 0 skipped).
   while (t_begin < t_end && terms[t_begin].k <= 0) ++t_begin;
@@ -257,6 +275,7 @@ __global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__r
   uint32_t a_off = 0;
   const uint32_t b_off = static_cast<uint32_t>((b_k * n + b_col) * sizeof(TB));
   const uint32_t b_step = static_cast<uint32_t>(BK * n * sizeof(TB));
+  const uint32_t b_elem = kDepthRows ? static_cast<uint32_t>(n * sizeof(TB)) : static_cast<uint32_t>(sizeof(TB));
   constexpr uint32_t a_step = BK * sizeof(TA);
   int32_t lt = t_begin;
   auto enter = [&](const KTerm &tm) {
@@ -276,7 +295,7 @@ __global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__r
 #pragma unroll
     for (int j = 0; j < A_PER; ++j) xa[j] = BufferLoad<TA>::at(ra, a_off + j * sizeof(TA));
 #pragma unroll
-    for (int j = 0; j < B_PER; ++j) xb[j] = BufferLoad<TB>::at(rb, b_off + j * sizeof(TB));
+    for (int j = 0; j < B_PER; ++j) xb[j] = BufferLoad<TB>::at(rb, b_off + j * b_elem);
     a_ptr += a_step;
     a_left = a_left > a_step ? a_left - a_step : 0u;
     b_ptr += b_step;
@@ -295,53 +314,84 @@ __global__ __launch_bounds__(kGemmThreads) void ggemm_kernel(const GemmTile *__r
     }
   };
 
-  // Two slabs in flight (two register sets, alternating) on tiles of at most 32 x 32; one on
-  // larger tiles, where a second set costs occupancy, and on complex tiles.
-  constexpr bool kTwoAhead = !std::is_same<TC, cdouble>::value && BM * BN <= 32 * 32;
-  TA ra0[A_PER], ra1[A_PER];
-  TB rb0[B_PER], rb1[B_PER];
-  int32_t klim0 = 0, klim1 = 0;
-  if (slabs > 0) {
-    issue(ra0, rb0, klim0);
-    if constexpr (kTwoAhead) issue(ra1, rb1, klim1);
-  }
+  // kAhead slabs in flight, in as many register sets used in turn: two on FP32 tiles up to 64 x 64
+  // (a set costs A_PER + B_PER registers) and on FP64 tiles of at most 32 x 32; one on larger FP64
+  // tiles, where a second set costs occupancy, and on complex tiles.
+  constexpr int kAhead = std::is_same<TC, cdouble>::value ? 1
+                         : std::is_same<TC, float>::value ? (BM * BN <= 64 * 64 && BK == 16 ? 2 : 1)
+                                                          : (BM * BN <= 32 * 32 ? 2 : 1);
+  TA ra[kAhead][A_PER];
+  TB rb[kAhead][B_PER];
+  int32_t klim[kAhead] = {};
+#pragma unroll
+  for (int set = 0; set < kAhead; ++set)
+    if (slabs > 0) issue(ra[set], rb[set], klim[set]);
   int buf = 0;
+  // slot of depth k, row r of the A slab and depth k, column c of the B slab
+  auto a_slot = [&](int k, int r) -> TC & {
+    if constexpr (kDepthRows) return As[buf][r][k];
+    else return As[buf][k][r];
+  };
+  auto b_slot = [&](int k, int c) -> TC & {
+    if constexpr (kDepthRows) return Bs[buf][c][k];
+    else return Bs[buf][k][c];
+  };
   auto step = [&](TA (&xa)[A_PER], TB (&xb)[B_PER], int32_t &klim) {
     if (klim == BK) {
 #pragma unroll
-      for (int j = 0; j < A_PER; ++j) As[buf][a_k + j][a_row] = Conv<TC>::from(xa[j]);
+      for (int j = 0; j < A_PER; ++j) a_slot(a_k + j, a_row) = Conv<TC>::from(xa[j]);
     } else {
 #pragma unroll
-      for (int j = 0; j < A_PER; ++j) As[buf][a_k + j][a_row] = a_k + j < klim ? Conv<TC>::from(xa[j]) : Conv<TC>::zero();
+      for (int j = 0; j < A_PER; ++j) a_slot(a_k + j, a_row) = a_k + j < klim ? Conv<TC>::from(xa[j]) : Conv<TC>::zero();
     }
 #pragma unroll
-    for (int j = 0; j < B_PER; ++j) Bs[buf][b_k][b_col + j] = Conv<TC>::from(xb[j]);
+    for (int j = 0; j < B_PER; ++j) {
+      if constexpr (kDepthRows) b_slot(b_k + j, b_col) = Conv<TC>::from(xb[j]);
+      else b_slot(b_k, b_col + j) = Conv<TC>::from(xb[j]);
+    }
     __syncthreads();
+    // uniform; false only for a padding step over a slab of zeros, which one set never takes
+    const bool live = kAhead == 1 || klim > 0;
     issue(xa, xb, klim);
+    if (!live) {
+      // a padding step past the last slab: nothing to add
+    } else if constexpr (kDepthRows) {
 #pragma unroll
-    for (int kk = 0; kk < BK / 4; ++kk) {
-      const int k = kk * 4 + lk;
-      TC af[FM], bf[FN];
+      for (int chunk = 0; chunk < BK / 16; ++chunk) {
+        f4 af[FM], bf[FN];
 #pragma unroll
-      for (int i = 0; i < FM; ++i) af[i] = As[buf][k][wm * TM + i * 16 + li];
+        for (int i = 0; i < FM; ++i) af[i] = *reinterpret_cast<const f4 *>(&As[buf][wm * TM + i * 16 + li][chunk * 16 + lk * 4]);
 #pragma unroll
-      for (int j = 0; j < FN; ++j) bf[j] = Bs[buf][k][wn * TN + j * 16 + li];
+        for (int j = 0; j < FN; ++j) bf[j] = *reinterpret_cast<const f4 *>(&Bs[buf][wn * TN + j * 16 + li][chunk * 16 + lk * 4]);
 #pragma unroll
-      for (int i = 0; i < FM; ++i)
+        for (int kk = 0; kk < 4; ++kk)
 #pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = M::step(af[i], bf[j], acc[i][j]);
+          for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) acc[i][j] = M::step(af[i][kk], bf[j][kk], acc[i][j]);
+      }
+    } else {
+#pragma unroll
+      for (int kk = 0; kk < BK / 4; ++kk) {
+        const int k = kFp32 ? (kk / 4) * 16 + lk * 4 + kk % 4 : kk * 4 + lk;
+        TC af[FM], bf[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) af[i] = As[buf][k][wm * TM + i * 16 + li];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) bf[j] = Bs[buf][k][wn * TN + j * 16 + li];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) acc[i][j] = M::step(af[i], bf[j], acc[i][j]);
+      }
     }
     buf ^= 1;
   };
-  if constexpr (kTwoAhead) {
-    // An even number of steps, so the loop body is the two steps with no exit between them; an
-    // odd count ends with a step over a slab of zeros, which adds exact zeros.
-    for (int32_t slab = 0; slab < slabs; slab += 2) {
-      step(ra0, rb0, klim0);
-      step(ra1, rb1, klim1);
-    }
-  } else {
-    for (int32_t slab = 0; slab < slabs; ++slab) step(ra0, rb0, klim0);
+  // Whole rounds of kAhead steps, with no exit between them; a count that is not a multiple of
+  // kAhead ends with steps over slabs of zeros (loaded past the last slab), whose MFMAs are skipped.
+  for (int32_t slab = 0; slab < slabs; slab += kAhead) {
+#pragma unroll
+    for (int set = 0; set < kAhead; ++set) step(ra[set], rb[set], klim[set]);
   }
 
 #pragma unroll
