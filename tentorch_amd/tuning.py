@@ -7,7 +7,7 @@ lists them.  Two ways of choosing one per block-product op:
 
 - the exhaustive autotuner runs the op under every configuration, on the program's own data inside
   the program that needs it, and keeps the fastest.  Its timings are also the training data and the
-  judge of the cost model;
+  yardstick of the cost model;
 - the cost model is a multilayer perceptron (40 inputs, two hidden layers of 64, ReLU).  It predicts
   log kernel time from a plan's block shapes and a configuration (`Program.gemm_features`) and picks
   the configuration with the least prediction, for plans it has never seen.  Training (mean squared
