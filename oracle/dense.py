@@ -28,7 +28,47 @@ def _leg_offsets(leg: Leg):
 
 
 def dense(t: Tensor) -> np.ndarray:
-    '''The dense array of an abelian tensor, every leg in the codomain frame.'''
+    '''The dense array of an abelian tensor, every leg in the codomain frame.  Each (codomain tree,
+    domain tree) pair of a block is a box of the dense array (one index range per leg, copies
+    mixed-radix with the last leg fastest), written in one slice assignment.'''
+    legs = t.legs()
+    ncod = len(t.codomain)
+    out = np.zeros([leg.multiplets() for leg in legs], dtype=t.dtype)
+    offs = [_leg_offsets(leg) for leg in legs]
+    for (r, s, c), block in t.blocks.items():
+        row_boxes = _boxes(t.cat, t.codomain, r, offs[:ncod], flip=False)
+        col_boxes = _boxes(t.cat, t.domain, s, offs[ncod:], flip=True)
+        row_at = 0
+        for row_slices, row_shape in row_boxes:
+            nr = int(np.prod(row_shape)) if row_shape else 1
+            col_at = 0
+            for col_slices, col_shape in col_boxes:
+                nc = int(np.prod(col_shape)) if col_shape else 1
+                part = block[row_at:row_at + nr, col_at:col_at + nc]
+                out[tuple(row_slices + col_slices)] = part.reshape(tuple(row_shape) + tuple(col_shape))
+                col_at += nc
+            row_at += nr
+    return out
+
+
+def _boxes(cat, legs: Sequence[Leg], root, offs, flip: bool):
+    '''The rows of one side as boxes, in block row order: one (slices, extents) per fusion tree.'''
+    if not legs:
+        return [([], [])]
+    out = []
+    for tree in fusion_trees(cat, legs, root):
+        slices, extents = [], []
+        for leg, sector, off in zip(legs, tree.sectors, offs):
+            start = off[cat.dual(sector) if flip else sector]
+            m = leg.mult_of(sector)
+            slices.append(slice(start, start + m))
+            extents.append(m)
+        out.append((slices, extents))
+    return out
+
+
+def dense_by_entry(t: Tensor) -> np.ndarray:
+    '''`dense` written entry by entry (the definition the box form is checked against).'''
     legs = t.legs()
     ncod = len(t.codomain)
     shape = [leg.multiplets() for leg in legs]

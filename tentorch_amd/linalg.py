@@ -126,10 +126,60 @@ def _gram_factors_torch(blocks, left_isometric: bool):
     return out
 
 
-def svd(tensor: Tensor, left: Optional[Sequence[int]] = None, max_bond: Optional[int] = None, tol: float = 0.0,
-        absorb: int = 0) -> Tuple[Tensor, torch.Tensor, Tensor]:
-    '''`split_over` (`svd.cpp:41-177`).  Returns (U, values, V) with U over (codomain | bond) and V over
-    (bond | domain).'''
+def _is_index(value) -> bool:
+    return isinstance(value, (int, np.integer)) and not isinstance(value, bool)
+
+
+def _bind_split_arguments(args, kwargs):
+    '''The two call forms of the reference's `svd` binding (`bindings.cpp:367-382`), resolved as its
+    overloads are, first form first:
+      svd(tensor, max_bond=None, tol=0.0, absorb=0)
+      svd(tensor, left, max_bond=None, tol=0.0, absorb=0)
+    Returns (left, max_bond, tol, absorb); a call that fits neither form raises TypeError.'''
+
+    def bind(names, defaults):
+        if len(args) > len(names):
+            return None
+        bound = dict(zip(names, args))
+        for name, value in kwargs.items():
+            if name not in names or name in bound:
+                return None
+            bound[name] = value
+        for name, value in defaults.items():
+            bound.setdefault(name, value)
+        if any(name not in bound for name in names):
+            return None
+        if bound['max_bond'] is not None and not _is_index(bound['max_bond']):
+            return None
+        if not isinstance(bound['tol'], (int, float, np.floating, np.integer)) or isinstance(bound['tol'], bool):
+            return None
+        if not _is_index(bound['absorb']):
+            return None
+        if 'left' in bound:
+            left = bound['left']
+            if left is None or isinstance(left, (str, bytes)) or not all(_is_index(i) for i in left):
+                return None
+        return bound
+
+    first = bind(('max_bond', 'tol', 'absorb'), {'max_bond': None, 'tol': 0.0, 'absorb': 0})
+    if first is not None:
+        return None, first['max_bond'], float(first['tol']), int(first['absorb'])
+    try:
+        second = bind(('left', 'max_bond', 'tol', 'absorb'), {'max_bond': None, 'tol': 0.0, 'absorb': 0})
+    except TypeError:
+        second = None
+    if second is not None:
+        return [int(i) for i in second['left']], second['max_bond'], float(second['tol']), int(second['absorb'])
+    raise TypeError('svd(): incompatible arguments; the supported forms are svd(tensor, max_bond=None, tol=0.0, '
+                    'absorb=0) and svd(tensor, left, max_bond=None, tol=0.0, absorb=0)')
+
+
+def svd(tensor: Tensor, *args, **kwargs) -> Tuple[Tensor, torch.Tensor, Tensor]:
+    '''`split_over` (`svd.cpp:41-177`) under the reference binding's two call forms
+    (`bindings.cpp:367-382`): `svd(tensor, max_bond=None, tol=0.0, absorb=0)` splits the tensor as its
+    sides stand, `svd(tensor, left, max_bond=None, tol=0.0, absorb=0)` first brings the legs `left`
+    to the codomain.  Returns (U, values, V) with U over (codomain | bond) and V over (bond | domain).'''
+    left, max_bond, tol, absorb = _bind_split_arguments(args, kwargs)
     return _split(tensor, left, max_bond, tol, absorb)[:3]
 
 
@@ -282,7 +332,8 @@ def _split(tensor: Tensor, left: Optional[Sequence[int]], max_bond: Optional[int
 
 
 def svdvals(tensor: Tensor, left: Optional[Sequence[int]] = None) -> torch.Tensor:
-    return svd(tensor, left)[1]
+    '''`svdvals` (`svd.cpp:179-187`, binding `bindings.cpp:383-385`): every singular value, no cut.'''
+    return _split(tensor, None if left is None else list(left), None, 0.0, 0)[1]
 
 
 def _split_blocks(tensor: Tensor, pieces) -> Tuple[Tensor, Tensor]:

@@ -314,7 +314,10 @@ def run(program, operands: Sequence[Tensor], out: Optional[torch.Tensor] = None,
     if elements:
         entry = workspaces.get(program, ws_dtype, device, elements)
         hoist = program.hoistable
-        identity = tuple((t._uid, t._version) for t in operands)
+        # torch's in-place counter (shared by every view of the buffer) catches writes through
+        # `block()` views, `to_tuple()` blocks and `.data`; `_version` catches the library's own
+        # writes, which torch does not see; the buffer address catches a replaced `.data`
+        identity = tuple((t._uid, t._version, t.data._version, t.data.data_ptr()) for t in operands)
         if mode == 0 and hoist and entry[1] is not None:
             for i, (now, then) in enumerate(zip(identity, entry[1])):
                 if (hoist >> i) & 1 and now == then:
@@ -404,8 +407,10 @@ class Profiler:
                     _native.timing(False)
             launches = _native.launch_count() - before
             if timed and launches:      # an op with no work launches nothing
+                # a move is typed by the workspace it writes: FP32 in the FP32 form's programs
+                moved = {4: 'f32', 8: 'f64', 16: 'c128'}[inner]
                 self.records.append((kind, flops, written * size(dst) + read * size(a),
-                                     compute if kind == 'gemm' else '', label, launches))
+                                     compute if kind == 'gemm' else moved, label, launches))
 
     @staticmethod
     def _one(program, args, i):
@@ -413,13 +418,13 @@ class Profiler:
         program.run(*base, i, i + 1, *extra)
 
     def summary(self, label=None, compute=None):
-        '''Totals per kind, over every record or over the records of one einsum equation (and, for
-        block products, one compute type).'''
+        '''Totals per kind, over every record or over the records of one einsum equation and one
+        compute type (a block product's MFMA type; a leg move's workspace type).'''
         out = {}
         for (kind, flops, moved, kind_compute, lab, launches), ms in zip(self.records, self.ms):
             if label is not None and lab != label:
                 continue
-            if compute is not None and kind == 'gemm' and kind_compute != compute:
+            if compute is not None and kind_compute != compute:
                 continue
             entry = out.setdefault(kind, {'ops': 0, 'launches': 0, 'ms': 0.0, 'flops': 0.0, 'bytes': 0.0})
             entry['ops'] += 1

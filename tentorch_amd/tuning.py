@@ -298,14 +298,15 @@ def _microbench():
     print(f'[tuning] microbench: {len(tuner.samples)} ops measured', file=sys.stderr, flush=True)
 
 
-def uneven_contractions(seed: int = 0, trials: int = 4):
+def uneven_cases(seed: int = 0, trials: int = 4):
     '''Contractions over uneven blocks: random SU(2) tensors whose legs carry spins j = 0..4
     (labels 2j) with multiplicities ~ Poisson(12 exp(-j/2)), and Fibonacci tensors with 1 and tau
-    multiplicities ~ Poisson(40), N(0, 1) entries in FP64.  Each contraction takes the middle leg of
-    a three-leg codomain, so the operand's move recouples two legs through F and R symbols before the
-    block products.  Fibonacci symbols carry phases, so those tensors are complex128.  Yields (name, thunk); a thunk builds its tensors and runs the einsum.'''
+    multiplicities ~ Poisson(40), N(0, 1) entries.  Each contraction takes the middle leg of a
+    three-leg codomain, so the operand's move recouples two legs through F and R symbols before the
+    block products.  Fibonacci symbols carry phases, so those tensors are complex128.  Yields
+    (name, operands, spec): `operands()` draws the tensors on the device.'''
     from .symmetry import Leg, fibonacci, su2
-    from .tensor import Tensor, einsum
+    from .tensor import Tensor
     rng = np.random.default_rng(seed)
 
     def su2_leg():
@@ -320,26 +321,39 @@ def uneven_contractions(seed: int = 0, trials: int = 4):
     def fib_leg():
         return Leg(fibonacci, [(0,), (1,)], [max(1, int(rng.poisson(40))), max(1, int(rng.poisson(40)))])
 
-    def run(cat, legs, spec, shapes, generator_seed, dtype=torch.float64):
-        def thunk():
+    def drawn(cat, legs, shapes, generator_seed, dtype=torch.float64):
+        def operands():
             g = np.random.default_rng(generator_seed)
-            tensors = [Tensor.random(cat, [legs[i] for i in cod], [legs[i] for i in dom], cat.trivial(), dtype=dtype,
-                                     generator=g) for cod, dom in shapes]
+            return [Tensor.random(cat, [legs[i] for i in cod], [legs[i] for i in dom], cat.trivial(), dtype=dtype,
+                                  generator=g) for cod, dom in shapes]
+        return operands
+
+    for trial in range(trials):
+        legs = [su2_leg() for _ in range(6)]
+        yield (f'SU(2) {trial} ABCd,Be->ACde',
+               drawn(su2, legs, [((0, 1, 2), (3,)), ((1,), (4,))], int(rng.integers(1 << 30))), 'ABCd,Be->ACde')
+        yield (f'SU(2) {trial} ABCd,CEf->AEBdf',
+               drawn(su2, legs, [((0, 1, 2), (3,)), ((2, 4), (5,))], int(rng.integers(1 << 30))), 'ABCd,CEf->AEBdf')
+        legs = [fib_leg() for _ in range(5)]
+        yield (f'Fibonacci {trial} ABCd,Be->ACde',
+               drawn(fibonacci, legs, [((0, 1, 2), (3,)), ((1,), (4,))], int(rng.integers(1 << 30)),
+                     torch.complex128), 'ABCd,Be->ACde')
+
+
+def uneven_contractions(seed: int = 0, trials: int = 4):
+    '''`uneven_cases` as (name, thunk): a thunk draws its tensors and runs the einsum.'''
+    from .tensor import einsum
+
+    def thunk_of(operands, spec):
+        def thunk():
+            tensors = operands()
             out = einsum(spec, *tensors)
             torch.cuda.synchronize()
             del out, tensors
         return thunk
 
-    for trial in range(trials):
-        legs = [su2_leg() for _ in range(6)]
-        yield (f'SU(2) {trial} ABCd,Be->ACde', run(su2, legs, 'ABCd,Be->ACde', [((0, 1, 2), (3,)), ((1,), (4,))],
-                                                  int(rng.integers(1 << 30))))
-        yield (f'SU(2) {trial} ABCd,CEf->AEBdf', run(su2, legs, 'ABCd,CEf->AEBdf',
-                                                     [((0, 1, 2), (3,)), ((2, 4), (5,))], int(rng.integers(1 << 30))))
-        legs = [fib_leg() for _ in range(5)]
-        yield (f'Fibonacci {trial} ABCd,Be->ACde', run(fibonacci, legs, 'ABCd,Be->ACde',
-                                                       [((0, 1, 2), (3,)), ((1,), (4,))], int(rng.integers(1 << 30)),
-                                                       torch.complex128))
+    for name, operands, spec in uneven_cases(seed, trials):
+        yield name, thunk_of(operands, spec)
 
 
 def main(argv=None) -> int:

@@ -1,8 +1,8 @@
 '''
 The HIP path end to end against the oracle: leg moves bit-exact, einsum/contract within 1e-12
-relative (FP64 block products sum in another order than numpy), and the two-site DMRG of the
-reference's tests (`tests/test_dmrg.py:16-23`) against exact diagonalisation within 1e-10, with the
-plain per-block form agreeing with the batched one.
+relative (FP64 block products sum in another order than numpy), the plain per-block form agreeing
+with the batched one, the device Lanczos and split against their host and exact counterparts.  The
+DMRG scenarios of the reference's tests are in test_gpu_dmrg.py.
 '''
 
 import itertools
@@ -67,15 +67,6 @@ def test_contract_and_conj_match_the_oracle():
     assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
     assert np.array_equal(ad.conj().data.cpu().numpy(), ao.conj().flat())
     assert abs(tt.inner(ad, bd).real - ot.inner(ao, bo).real) <= 1e-12 * ot.norm(ao) * ot.norm(bo)
-
-
-@pytest.mark.parametrize('cat', ['u1', 'no_symmetry'])
-@pytest.mark.parametrize('length', [4, 6, 8])
-def test_dmrg_energy_matches_exact_diagonalisation(cat, length):
-    model = tt.Model(length, cat=getattr(tt, cat))
-    driver = tt.DMRG(model, max_bond_dim=64, n_iter=5)
-    driver.run(num_sweeps=6, verbose=False)
-    assert abs(driver.energy().real - exact_energy(length)) < 1e-10
 
 
 def test_plain_and_batched_forms_agree_on_a_sweep():
@@ -269,7 +260,7 @@ def test_exact_split_reorders_unsorted_singular_values(monkeypatch):
     two = tt.Tensor.random(tt.u1, [bond, phys], [phys, bond], (0,), generator=rng)
     SvdSettings.method = 'exact'
     try:
-        ref_u, ref_s, ref_v = tt.svd(two, None, 20, 1e-14, 2)
+        ref_u, ref_s, ref_v = tt.svd(two, 20, 1e-14, 2)
         original = torch.linalg.svd
         shuffle = torch.Generator().manual_seed(5)
 
@@ -278,7 +269,7 @@ def test_exact_split_reorders_unsorted_singular_values(monkeypatch):
             order = torch.randperm(s.numel(), generator=shuffle).to(s.device)
             return u[:, order], s[order], vh[order]
         monkeypatch.setattr(torch.linalg, 'svd', shuffled)
-        u, s, v = tt.svd(two, None, 20, 1e-14, 2)
+        u, s, v = tt.svd(two, 20, 1e-14, 2)
     finally:
         SvdSettings.method = 'gram'
     assert torch.equal(s, ref_s)

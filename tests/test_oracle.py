@@ -232,3 +232,12 @@ def test_lanczos_matches_dense_projection():
     q, _ = np.linalg.qr(np.stack(k, axis=1))
     assert abs(value - np.linalg.eigvalsh(q.T @ m @ q)[0]) < 1e-10
     assert abs(np.linalg.norm(vector.flat()) - 1.0) < 1e-12
+
+
+def test_dense_boxes_equal_the_entry_by_entry_expansion():
+    from oracle.dense import dense_by_entry
+    rng = np.random.default_rng(11)
+    for cod, dom in (([BOND, PHYS], [PHYS, BOND]), ([BOND, MPOL], [BOND]), ([MPOL, PHYS, PHYS, BOND], [BOND]),
+                     ([BOND], [BOND, PHYS, MPOL])):
+        t = _random_u1(rng, cod, dom)
+        assert np.array_equal(dense(t), dense_by_entry(t))
