@@ -201,6 +201,17 @@ int64_t tt_eigh_max_order(void);
 int64_t tt_eigh_workspace(const tt_eigh_problem *problems, int64_t count);
 int tt_eigh_batched(const tt_eigh_problem *problems, int64_t count, double *matrices, double *values,
                     double *vectors, double *workspace, void *stream);
+/* No counterpart in the reference: tt_eigh_batched for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  The same
+ * kernels, with the workspace taken from a static array of TT_EIGH_STATIC_SCRATCH_DOUBLES doubles
+ * (64 MiB) that the library holds on every device it runs on until the process ends; nothing is
+ * allocated or freed.  A batch whose tt_eigh_workspace exceeds that size is refused with an error.
+ * The scratch belongs to the current device (the stream must be on it), keeps the last call's
+ * contents (every call rewrites what it reads before reading it), and is shared by every call on
+ * that device: calls that overlap on one device race. */
+#define TT_EIGH_STATIC_SCRATCH_DOUBLES (INT64_C(8) << 20)
+int tt_eigh_batched_static(const tt_eigh_problem *problems, int64_t count, double *matrices, double *values,
+                           double *vectors, void *stream);
 
 /* The learned tile schedule's cost model (no counterpart in the reference): a multilayer
  * perceptron x (features) -> relu(W1^T x + b1) (hidden) -> relu(W2^T h1 + b2) (hidden) -> w3 . h2 + b3

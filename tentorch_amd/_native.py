@@ -71,7 +71,7 @@ EXPORTED = ['tt_gss_plan_create', 'tt_gss_plan_destroy', 'tt_gss_plan_counts', '
             'tt_graph_end', 'tt_graph_launch', 'tt_graph_destroy', 'tt_last_error', 'tt_version',
             'tt_launch_count', 'tt_timing_enable', 'tt_timing_collect', 'tt_mlp_param_count', 'tt_mlp_forward',
             'tt_mlp_gradient', 'tt_mlp_adam', 'tt_eigh_max_order', 'tt_eigh_workspace', 'tt_eigh_batched',
-            'tt_lanczos_orthonormalize', 'tt_lanczos_ritz', 'tt_multiaxpy_counted']
+            'tt_lanczos_orthonormalize', 'tt_lanczos_ritz', 'tt_multiaxpy_counted', 'tt_eigh_batched_static']
 
 
 def library() -> ctypes.CDLL:

@@ -20,6 +20,7 @@
 // Eigenvalues come out ascending; eigenvector j is row j of the matrix's n x n output.  gfx950 only.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -1362,6 +1363,50 @@ int eigh_batched(const tt_eigh_problem *problems, int64_t count, double *matrice
 
 extern "C" int tt_eigh_batched(const tt_eigh_problem *problems, int64_t count, double *matrices, double *values,
                                double *vectors, double *workspace, void *stream) {
+  return eigh_batched(problems, count, matrices, values, vectors, workspace, stream);
+}
+
+// The workspace of tt_eigh_batched_static: one array per device, in the code object's device
+// memory, held until the process ends.  In host code `scratch` only names the symbol; its device
+// address comes from hipGetSymbolAddress, once per device (each device holds its own copy).
+static __device__ __align__(256) double scratch[TT_EIGH_STATIC_SCRATCH_DOUBLES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+double *scratch_address[kMaxDevices] = {};
+
+// The device address of the current device's scratch array, looked up on the first call there.
+int current_scratch(double **out) {
+  int device = -1;
+  TT_HIP(hipGetDevice(&device));
+  if (device < 0 || device >= kMaxDevices)
+    return fail("tt_eigh_batched_static: device " + std::to_string(device) + " beyond the " +
+                std::to_string(kMaxDevices) + " the library tracks");
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (!scratch_address[device]) {
+    void *address = nullptr;
+    TT_HIP(hipGetSymbolAddress(&address, HIP_SYMBOL(scratch)));
+    scratch_address[device] = static_cast<double *>(address);
+  }
+  *out = scratch_address[device];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int tt_eigh_batched_static(const tt_eigh_problem *problems, int64_t count, double *matrices,
+                                      double *values, double *vectors, void *stream) {
+  const int64_t need = tt_eigh_workspace(problems, count);
+  if (need < 0) return -1;
+  if (need > TT_EIGH_STATIC_SCRATCH_DOUBLES)
+    return fail("tt_eigh_batched_static: the batch needs " + std::to_string(need) +
+                " doubles of workspace, more than the " + std::to_string(TT_EIGH_STATIC_SCRATCH_DOUBLES) +
+                " of the library's static scratch; pass a workspace to tt_eigh_batched instead");
+  if (count == 0) return 0;
+  double *workspace = nullptr;
+  if (int rc = current_scratch(&workspace)) return rc;
   return eigh_batched(problems, count, matrices, values, vectors, workspace, stream);
 }
 
