@@ -213,6 +213,46 @@ int tt_eigh_batched(const tt_eigh_problem *problems, int64_t count, double *matr
 int tt_eigh_batched_static(const tt_eigh_problem *problems, int64_t count, double *matrices, double *values,
                            double *vectors, void *stream);
 
+/* Batched kernels of the graded split (`split_over`, tentorch/linalg/graded/svd.cpp:41-177: the
+ * per-block `at::linalg_svd` and the factor copies into the bond, svd.cpp:59,159-172).  Each takes a
+ * problem table in device memory, `count` problems sorted by `first`, whose work items are numbered
+ * consecutively across problems (`first` is the number of the problem's first item).
+ *
+ * tt_bgemm: for each problem C = alpha op(A) op(B) diag(col_scale) + beta C (col_scale null: 1),
+ * op(A) m x k, op(B) k x n, all row-major with row strides lda, ldb, ldc; trans_a / trans_b set
+ * means the operand is stored transposed (A as k x m, B as n x k).  FP64 on MFMA, one work item per
+ * 64 x 64 tile of C (ceil(m / 64) ceil(n / 64) per problem, row-major); `tiles` is their total.
+ * With beta 0, C is not read. */
+typedef struct tt_bgemm_problem {
+  const double *a;
+  const double *b;
+  double *c;
+  const double *col_scale;
+  int64_t first;
+  int32_t m, n, k;
+  int32_t lda, ldb, ldc;
+  int32_t trans_a, trans_b;
+  double alpha;
+  double beta;
+} tt_bgemm_problem;
+int tt_bgemm(const tt_bgemm_problem *problems, int64_t count, int64_t tiles, void *stream);
+/* One work item per row.  tt_row_norms: dst[i] = the Euclidean norm of row i of src (`cols`
+ * entries, row stride src_ld).  tt_copy_rows: row i of the problem is row index[i] of src (row i
+ * when index is null), written as row i of dst (transpose 0, stride dst_ld) or as its column i
+ * (transpose 1).  `rows` is the total over the problems. */
+typedef struct tt_rows_problem {
+  const double *src;
+  double *dst;
+  const int32_t *index;
+  int64_t first;
+  int32_t rows, cols;
+  int32_t src_ld, dst_ld;
+  int32_t transpose;
+  int32_t reserved;
+} tt_rows_problem;
+int tt_row_norms(const tt_rows_problem *problems, int64_t count, int64_t rows, void *stream);
+int tt_copy_rows(const tt_rows_problem *problems, int64_t count, int64_t rows, void *stream);
+
 /* The learned tile schedule's cost model (no counterpart in the reference): a multilayer
  * perceptron x (features) -> relu(W1^T x + b1) (hidden) -> relu(W2^T h1 + b2) (hidden) -> w3 . h2 + b3
  * in FP32, parameters packed as W1 [features][hidden], b1, W2 [hidden][hidden], b2, w3, b3.

@@ -6,6 +6,8 @@ library is missing, or no GPU is visible when device work is asked for, this mod
 
 import ctypes
 import os
+
+import numpy as np
 from typing import Optional
 
 import torch
@@ -58,6 +60,9 @@ _lib.tt_multiaxpy_counted.argtypes = [_i64, _int, _c_void_p, _c_void_p, _dbl, _c
 _lib.tt_lanczos_ritz.argtypes = [_int, _c_void_p, _c_void_p, _dbl, _c_void_p, _c_void_p, _c_void_p, _c_void_p]
 _lib.tt_lanczos_orthonormalize.argtypes = [_i64, _int, _c_void_p, _i64, _c_void_p, _c_void_p, _c_void_p, _c_void_p,
                                            _c_void_p]
+_lib.tt_bgemm.argtypes = [_c_void_p, _i64, _i64, _c_void_p]
+_lib.tt_row_norms.argtypes = [_c_void_p, _i64, _i64, _c_void_p]
+_lib.tt_copy_rows.argtypes = [_c_void_p, _i64, _i64, _c_void_p]
 _lib.tt_graph_begin.argtypes = [_c_void_p]
 _lib.tt_graph_end.argtypes = [_c_void_p, ctypes.POINTER(_c_void_p)]
 _lib.tt_graph_launch.argtypes = [_c_void_p, _c_void_p]
@@ -71,7 +76,8 @@ EXPORTED = ['tt_gss_plan_create', 'tt_gss_plan_destroy', 'tt_gss_plan_counts', '
             'tt_graph_end', 'tt_graph_launch', 'tt_graph_destroy', 'tt_last_error', 'tt_version',
             'tt_launch_count', 'tt_timing_enable', 'tt_timing_collect', 'tt_mlp_param_count', 'tt_mlp_forward',
             'tt_mlp_gradient', 'tt_mlp_adam', 'tt_eigh_max_order', 'tt_eigh_workspace', 'tt_eigh_batched',
-            'tt_lanczos_orthonormalize', 'tt_lanczos_ritz', 'tt_multiaxpy_counted', 'tt_eigh_batched_static']
+            'tt_lanczos_orthonormalize', 'tt_lanczos_ritz', 'tt_multiaxpy_counted', 'tt_eigh_batched_static',
+            'tt_bgemm', 'tt_row_norms', 'tt_copy_rows']
 
 
 def library() -> ctypes.CDLL:
@@ -194,12 +200,13 @@ def eigh_workspace(sizes) -> int:
     return size
 
 
-def eigh_batched(matrices: torch.Tensor, sizes, offsets=None, work: Optional[torch.Tensor] = None):
+def eigh_batched(matrices: torch.Tensor, sizes, offsets=None, work: Optional[torch.Tensor] = None,
+                 vectors: Optional[torch.Tensor] = None):
     '''Eigenvalues (ascending) and eigenvectors of the symmetric FP64 matrices packed in `matrices`
     (matrix b is sizes[b] x sizes[b], at offsets[b], default back to back), on the device.  Returns
     (values, vectors): values packed back to back, vectors with eigenvector j of matrix b as row j
     of its n x n at the matrix's offset.  `matrices` is overwritten; `work` may supply the workspace
-    (any contents).'''
+    (any contents) and `vectors` the eigenvector buffer.'''
     if matrices.dtype != torch.float64:
         raise TypeError('the batched eigensolver runs in float64')
     require_device(matrices)
@@ -214,7 +221,10 @@ def eigh_batched(matrices: torch.Tensor, sizes, offsets=None, work: Optional[tor
     if offsets is None and at > matrices.numel():
         raise ValueError('the matrices do not fit the buffer')
     values = torch.empty(max(value_at, 1), dtype=torch.float64, device=matrices.device)
-    vectors = torch.empty_like(matrices)
+    if vectors is None:
+        vectors = torch.empty_like(matrices)
+    elif vectors.numel() < matrices.numel() or vectors.dtype != torch.float64:
+        raise ValueError('the eigenvector buffer has to be as large as the matrix buffer')
     size = _lib.tt_eigh_workspace(problems, count)
     if size < 0:
         raise RuntimeError(last_error())
@@ -261,3 +271,114 @@ def capture(fn) -> Graph:
         status = _lib.tt_graph_end(s, ctypes.byref(handle))
     check(status, 'graph capture')
     return Graph(handle)
+
+
+# -- the split's batched kernels: problem tables built on the host, uploaded, then launched --------
+
+BGEMM_PROBLEM = np.dtype([('a', '<u8'), ('b', '<u8'), ('c', '<u8'), ('col_scale', '<u8'), ('first', '<i8'),
+                          ('m', '<i4'), ('n', '<i4'), ('k', '<i4'), ('lda', '<i4'), ('ldb', '<i4'), ('ldc', '<i4'),
+                          ('trans_a', '<i4'), ('trans_b', '<i4'), ('alpha', '<f8'), ('beta', '<f8')])
+ROWS_PROBLEM = np.dtype([('src', '<u8'), ('dst', '<u8'), ('index', '<u8'), ('first', '<i8'), ('rows', '<i4'),
+                         ('cols', '<i4'), ('src_ld', '<i4'), ('dst_ld', '<i4'), ('transpose', '<i4'),
+                         ('reserved', '<i4')])
+assert BGEMM_PROBLEM.itemsize == 88 and ROWS_PROBLEM.itemsize == 56
+
+
+class Extents:
+    '''Device buffers a batch of problems may touch, as (first address, one past the last) byte
+    ranges: every operand a table names is checked to lie inside one of them before the launch.'''
+
+    def __init__(self):
+        self.ranges = []
+
+    def add(self, t: torch.Tensor) -> 'Extents':
+        start = t.data_ptr()
+        self.ranges.append((start, start + t.numel() * t.element_size()))
+        return self
+
+    def check(self, pointer: int, first: int, last: int, what: str) -> None:
+        '''Elements first..last (inclusive, FP64 or int32 by the caller's scaling) of an operand at
+        `pointer` lie in one known buffer.'''
+        lo, hi = pointer + first, pointer + last
+        for start, end in self.ranges:
+            if start <= lo and hi < end:
+                return
+        raise RuntimeError(f'{what}: bytes {lo:#x}..{hi:#x} lie outside every buffer of the launch')
+
+
+def _rows_touched(rows, cols, ld):
+    return 0, 8 * ((rows - 1) * ld + cols - 1) if rows > 0 and cols > 0 else 0
+
+
+def check_bgemm(table: np.ndarray, extents: Extents) -> int:
+    '''Validates a bgemm table against the buffers it names; returns its tile count.'''
+    tiles = 0
+    for p in table:
+        m, n, k = int(p['m']), int(p['n']), int(p['k'])
+        if m < 0 or n < 0 or k < 0 or int(p['first']) != tiles:
+            raise RuntimeError('bgemm: bad problem sizes or tile numbering')
+        a_rows, a_cols = (k, m) if p['trans_a'] else (m, k)
+        b_rows, b_cols = (n, k) if p['trans_b'] else (k, n)
+        if a_cols > int(p['lda']) or b_cols > int(p['ldb']) or n > int(p['ldc']):
+            raise RuntimeError('bgemm: a row stride is shorter than its row')
+        if m and k:
+            extents.check(int(p['a']), *_rows_touched(a_rows, a_cols, int(p['lda'])), 'bgemm A')
+        if n and k:
+            extents.check(int(p['b']), *_rows_touched(b_rows, b_cols, int(p['ldb'])), 'bgemm B')
+        if m and n:
+            extents.check(int(p['c']), *_rows_touched(m, n, int(p['ldc'])), 'bgemm C')
+            if int(p['col_scale']):
+                extents.check(int(p['col_scale']), 0, 8 * (n - 1), 'bgemm column scale')
+        tiles += -(-m // 64) * -(-n // 64)
+    return tiles
+
+
+def check_rows(table: np.ndarray, extents: Extents, source_rows=None, vector_dst: bool = False) -> int:
+    '''Validates a row-problem table; returns its row count.  With an index, `source_rows[i]` is the
+    number of rows problem i's source holds, and the caller has checked that every index is below
+    it.  `vector_dst`: dst receives one value per row (the norms).'''
+    rows = 0
+    for i, p in enumerate(table):
+        r, c = int(p['rows']), int(p['cols'])
+        if r < 0 or c < 0 or int(p['first']) != rows or c > int(p['src_ld']):
+            raise RuntimeError('row problem: bad sizes or row numbering')
+        if r and c:
+            held = r
+            if int(p['index']):
+                extents.check(int(p['index']), 0, 4 * (r - 1), 'row index')
+                held = int(source_rows[i])
+            extents.check(int(p['src']), *_rows_touched(held, c, int(p['src_ld'])), 'row source')
+            if vector_dst:
+                extents.check(int(p['dst']), 0, 8 * (r - 1), 'row destination')
+            elif p['transpose']:
+                extents.check(int(p['dst']), *_rows_touched(c, r, int(p['dst_ld'])), 'row destination')
+            else:
+                extents.check(int(p['dst']), *_rows_touched(r, c, int(p['dst_ld'])), 'row destination')
+        rows += r
+    return rows
+
+
+def upload(*tables: np.ndarray):
+    '''The tables in one device buffer (one copy from pinned memory, queued on the current stream);
+    returns the buffer (keep it alive until the launches are queued) and each table's address.'''
+    sizes = [t.nbytes for t in tables]
+    offsets = np.concatenate([[0], np.cumsum([(n + 255) // 256 * 256 for n in sizes])]).astype(np.int64)
+    host = torch.empty(max(int(offsets[-1]), 1), dtype=torch.uint8, pin_memory=True)
+    view = host.numpy()
+    for t, at in zip(tables, offsets):
+        view[at:at + t.nbytes] = t.view(np.uint8).reshape(-1)
+    device = host.to('cuda', non_blocking=True)
+    base = device.data_ptr()
+    return device, [base + int(at) for at in offsets[:-1]]
+
+
+def bgemm(table: np.ndarray, address: int, tiles: int) -> None:
+    check(_lib.tt_bgemm(address, len(table), tiles, stream()), 'bgemm')
+
+
+def row_norms(table: np.ndarray, address: int, rows: int) -> None:
+    check(_lib.tt_row_norms(address, len(table), rows, stream()), 'row norms')
+
+
+def copy_rows(table: np.ndarray, address: int, rows: int) -> None:
+    check(_lib.tt_copy_rows(address, len(table), rows, stream()), 'copy rows')

@@ -21,7 +21,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, 'csrc')
 BUILD = os.path.join(ROOT, '_build')
 HIP_LIB = os.path.join(HERE, 'libtentorch_hip.so')
-HIP_SOURCES = ['device.hip', 'ggemm.hip', 'mlp.hip', 'eigh.hip']
+HIP_SOURCES = ['device.hip', 'ggemm.hip', 'mlp.hip', 'eigh.hip', 'split.hip']
 ENGINE_SOURCES = ['category.cpp', 'tree.cpp', 'move.cpp', 'program.cpp', 'bindings.cpp']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 # The soname makes the engine record a bare library name, found through its $ORIGIN run path, so

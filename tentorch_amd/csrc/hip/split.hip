@@ -1,0 +1,203 @@
+// Batched kernels of the graded split (tentorch/linalg/graded/svd.cpp:41-177): the Gram products,
+// projections and rotations of every block of a split in one launch each, the row norms the
+// ranking reads, and the copies that assemble the factors.  Every launch takes a problem table in
+// device memory (the caller uploads it), so its work does not depend on how many blocks there are.
+//
+// bgemm_kernel: C = alpha op(A) op(B) diag(s) + beta C for each problem, FP64 MFMA
+// (v_mfma_f64_16x16x4f64).  One workgroup per 64 x 64 tile of one problem; four waves, each a
+// 32 x 32 quarter as 2 x 2 MFMA blocks.  The 16-deep slabs of A and B go through LDS (A as
+// As[depth][row], B as Bs[depth][col]), which is also where a transposed operand is turned:
+// each operand is loaded along its contiguous axis (a row of A or of B^T: 16 consecutive depths;
+// a row of A^T or of B: 64 consecutive rows or columns), the next slab in registers while the MFMAs
+// run on the current one.  The MFMA result layout: lane l holds rows 4 r + l / 16 of a 16 x 16
+// block, column l % 16 (measured on gfx950; the eigensolver's merge products use the same).
+
+#include <algorithm>
+#include <string>
+
+#include "common.h"
+
+using ttdev::as_stream;
+using ttdev::fail;
+
+namespace {
+
+constexpr int kTile = 64;
+constexpr int kSlab = 16;
+constexpr int kThreads = 256;
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// The problem of a global tile index: the last problem whose first tile is at or before it.
+template <typename Problem>
+__device__ int find_problem(const Problem *__restrict__ problems, int count, int64_t tile) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (problems[mid].first <= tile) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kThreads) void bgemm_kernel(const tt_bgemm_problem *__restrict__ problems, int count) {
+  const int index = find_problem(problems, count, blockIdx.x);
+  const tt_bgemm_problem p = problems[index];
+  const int64_t local = int64_t(blockIdx.x) - p.first;
+  const int col_tiles = (p.n + kTile - 1) / kTile;
+  const int r0 = int(local / col_tiles) * kTile, c0 = int(local % col_tiles) * kTile;
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  __shared__ double As[2][kSlab][kTile + 1], Bs[2][kSlab][kTile + 1];
+
+  // staging: each thread brings four elements of each operand per slab
+  double ra[4], rb[4];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int i, kk;
+      if (p.trans_a) { i = t & 63; kk = (t >> 6) + 4 * q; }      // A^T rows: 64 consecutive i
+      else { kk = t & 15; i = (t >> 4) + 16 * q; }              // A rows: 16 consecutive depths
+      const int row = r0 + i, depth = k0 + kk;
+      ra[q] = (row < p.m && depth < p.k)
+                  ? (p.trans_a ? p.a[int64_t(depth) * p.lda + row] : p.a[int64_t(row) * p.lda + depth])
+                  : 0.0;
+      int j, kb;
+      if (p.trans_b) { kb = t & 15; j = (t >> 4) + 16 * q; }      // B^T rows: 16 consecutive depths
+      else { j = t & 63; kb = (t >> 6) + 4 * q; }               // B rows: 64 consecutive columns
+      const int col = c0 + j, depth_b = k0 + kb;
+      rb[q] = (col < p.n && depth_b < p.k)
+                  ? (p.trans_b ? p.b[int64_t(col) * p.ldb + depth_b] : p.b[int64_t(depth_b) * p.ldb + col])
+                  : 0.0;
+    }
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (p.trans_a) As[buf][(t >> 6) + 4 * q][t & 63] = ra[q];
+      else As[buf][t & 15][(t >> 4) + 16 * q] = ra[q];
+      if (p.trans_b) Bs[buf][t & 15][(t >> 4) + 16 * q] = rb[q];
+      else Bs[buf][(t >> 6) + 4 * q][t & 63] = rb[q];
+    }
+  };
+
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32, lr = lane & 15, lk = lane >> 4;
+  d4 acc[2][2];
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = d4{0.0, 0.0, 0.0, 0.0};
+  int buf = 0;
+  if (p.k > 0) fetch(0);
+  for (int k0 = 0; k0 < p.k; k0 += kSlab) {
+    stage(buf);
+    __syncthreads();
+    if (k0 + kSlab < p.k) fetch(k0 + kSlab);
+#pragma unroll
+    for (int k4 = 0; k4 < kSlab; k4 += 4) {
+      double a[2], bv[2];
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi) a[bi] = As[buf][k4 + lk][wr + 16 * bi + lr];
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) bv[bj] = Bs[buf][k4 + lk][wc + 16 * bj + lr];
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+          acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], bv[bj], acc[bi][bj], 0, 0, 0);
+    }
+    buf ^= 1;
+    // the next iteration stages into the other buffer; the one just read is rewritten two slabs on,
+    // after the barrier of the next iteration
+  }
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) {
+      const int col = c0 + wc + 16 * bj + lr;
+      if (col >= p.n) continue;
+      const double scale = p.alpha * (p.col_scale ? p.col_scale[col] : 1.0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = r0 + wr + 16 * bi + 4 * r + lk;
+        if (row >= p.m) continue;
+        double *const out = p.c + int64_t(row) * p.ldc + col;
+        *out = p.beta == 0.0 ? scale * acc[bi][bj][r] : scale * acc[bi][bj][r] + p.beta * *out;
+      }
+    }
+}
+
+// One wave per row: the Euclidean norm of its `cols` entries, summed in a fixed order (lane
+// partial sums over a stride of 64, then a butterfly over the wave).
+constexpr int kRowWaves = 4;
+
+__global__ __launch_bounds__(64 * kRowWaves) void row_norms_kernel(const tt_rows_problem *__restrict__ problems,
+                                                                  int count, int64_t rows) {
+  const int64_t row = int64_t(blockIdx.x) * kRowWaves + (threadIdx.x >> 6);
+  if (row >= rows) return;
+  const int lane = threadIdx.x & 63;
+  const tt_rows_problem p = problems[find_problem(problems, count, row)];
+  const int64_t local = row - p.first;
+  const double *__restrict__ x = p.src + local * p.src_ld;
+  double sum = 0.0;
+  for (int j = lane; j < p.cols; j += 64) sum = fma(x[j], x[j], sum);
+#pragma unroll
+  for (int offset = 32; offset > 0; offset >>= 1) sum += __shfl_xor(sum, offset);
+  if (lane == 0) p.dst[local] = sqrt(sum);
+}
+
+// One wave per destination row: row i of the problem takes source row index[i] (i when there is no
+// index), written as row i of dst (`transpose` 0) or as column i of dst (`transpose` 1).
+__global__ __launch_bounds__(64 * kRowWaves) void copy_rows_kernel(const tt_rows_problem *__restrict__ problems,
+                                                                  int count, int64_t rows) {
+  const int64_t row = int64_t(blockIdx.x) * kRowWaves + (threadIdx.x >> 6);
+  if (row >= rows) return;
+  const int lane = threadIdx.x & 63;
+  const tt_rows_problem p = problems[find_problem(problems, count, row)];
+  const int64_t local = row - p.first;
+  const int64_t from = p.index ? p.index[local] : local;
+  const double *__restrict__ x = p.src + from * p.src_ld;
+  if (p.transpose) {
+    for (int j = lane; j < p.cols; j += 64) p.dst[int64_t(j) * p.dst_ld + local] = x[j];
+  } else {
+    double *__restrict__ y = p.dst + local * p.dst_ld;
+    for (int j = lane; j < p.cols; j += 64) y[j] = x[j];
+  }
+}
+
+template <typename Problem>
+int check_table(const Problem *problems, int64_t count, const char *what) {
+  if (count < 0 || (count > 0 && !problems)) return fail(std::string(what) + ": bad problem table");
+  if (count > (int64_t(1) << 30)) return fail(std::string(what) + ": too many problems");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int tt_bgemm(const tt_bgemm_problem *problems, int64_t count, int64_t tiles, void *stream) {
+  if (int rc = check_table(problems, count, "tt_bgemm")) return rc;
+  if (tiles < 0 || tiles > (int64_t(1) << 31) - 1) return fail("tt_bgemm: tile count out of range");
+  if (count == 0 || tiles == 0) return 0;
+  hipLaunchKernelGGL(bgemm_kernel, dim3(static_cast<uint32_t>(tiles)), dim3(kThreads), 0, as_stream(stream),
+                     problems, static_cast<int>(count));
+  TT_LAUNCHED();
+  return 0;
+}
+
+extern "C" int tt_row_norms(const tt_rows_problem *problems, int64_t count, int64_t rows, void *stream) {
+  if (int rc = check_table(problems, count, "tt_row_norms")) return rc;
+  if (rows < 0) return fail("tt_row_norms: negative row count");
+  if (count == 0 || rows == 0) return 0;
+  hipLaunchKernelGGL(row_norms_kernel, dim3(static_cast<uint32_t>((rows + kRowWaves - 1) / kRowWaves)),
+                     dim3(64 * kRowWaves), 0, as_stream(stream), problems, static_cast<int>(count), rows);
+  TT_LAUNCHED();
+  return 0;
+}
+
+extern "C" int tt_copy_rows(const tt_rows_problem *problems, int64_t count, int64_t rows, void *stream) {
+  if (int rc = check_table(problems, count, "tt_copy_rows")) return rc;
+  if (rows < 0) return fail("tt_copy_rows: negative row count");
+  if (count == 0 || rows == 0) return 0;
+  hipLaunchKernelGGL(copy_rows_kernel, dim3(static_cast<uint32_t>((rows + kRowWaves - 1) / kRowWaves)),
+                     dim3(64 * kRowWaves), 0, as_stream(stream), problems, static_cast<int>(count), rows);
+  TT_LAUNCHED();
+  return 0;
+}

@@ -20,6 +20,7 @@ import torch
 
 from . import _native
 from ._native import engine
+from .gram import GramSplit, choose_tails
 from .tensor import Tensor, inner, inner_device, norm
 
 
@@ -40,89 +41,106 @@ class SvdSettings:
     other factor as the exact projection of the block on it; 'exact' runs one SVD per block.
     `eigh` picks the Gram eigensolver: 'device' (this library's batched tridiagonalisation, divide
     and conquer and back-transformation for all blocks at once, orders up to 2048; larger Gram
-    matrices go to torch) or 'torch' (`torch.linalg.eigh`, batched by size class).'''
+    matrices go to torch) or 'torch' (`torch.linalg.eigh`).  `resolved`: a Gram pass's values are
+    trusted down to this fraction of its largest; below, when the cut can reach there, the rows under
+    `reliable` x the largest go through a further pass (see `tentorch_amd.gram`).'''
     method = 'gram'
     eigh = 'device'
+    reliable = 1e-5
+    resolved = 1e-7
 
 
-def _gram_factors(blocks, left_isometric: bool):
-    if SvdSettings.eigh != 'device':
-        return _gram_factors_torch(blocks, left_isometric)
-    # the device eigensolver takes Gram matrices up to its order limit; larger ones go to torch
-    order = [b.shape[0] if left_isometric else b.shape[1] for b in blocks]
-    small = [i for i, g in enumerate(order) if g <= _native.EIGH_MAX_ORDER]
-    large = [i for i, g in enumerate(order) if g > _native.EIGH_MAX_ORDER]
-    out = [None] * len(blocks)
-    for group, solve in ((small, _gram_factors_device), (large, _gram_factors_torch)):
-        if group:
-            for i, factor in zip(group, solve([blocks[i] for i in group], left_isometric)):
-                out[i] = factor
-    return out
+def _eigenrows(packed: torch.Tensor, sizes, offsets):
+    '''Eigenvectors of the symmetric matrices packed in `packed` (matrix b: sizes[b]^2 entries at
+    offsets[b]) as the rows of one n x n matrix per problem, in ascending eigenvalue order: one call
+    of the device library's batched solver for every matrix up to its order limit, torch beyond it
+    (or for all of them under `SvdSettings.eigh = 'torch'`).  Nothing is read back.'''
+    rows = [None] * len(sizes)
+    device = SvdSettings.eigh == 'device'
+    small = [b for b, g in enumerate(sizes) if device and g <= _native.EIGH_MAX_ORDER]
+    if small:
+        _, vectors = _native.eigh_batched(packed, [sizes[b] for b in small], [offsets[b] for b in small])
+        for b in small:
+            g, at = sizes[b], offsets[b]
+            rows[b] = vectors[at:at + g * g].view(g, g)
+    for b, g in enumerate(sizes):
+        if rows[b] is None:
+            at = offsets[b]
+            _, columns = torch.linalg.eigh(packed[at:at + g * g].view(g, g))
+            rows[b] = columns.transpose(0, 1)
+    return rows
 
 
-def _gram_factors_device(blocks, left_isometric: bool):
-    '''For each (m x n) block: the isometric factor (m x r or n x r, r = min(m, n), a view) and the
-    singular values in descending order (host, FP64), from the device library's batched
-    eigendecomposition of the negated Gram matrices -B B^T (left) or -B^T B (right), all packed in
-    one buffer.  Negation turns the solver's ascending order into descending singular values, so the
-    leading r eigenvectors (rows of the output) are the factor's columns as they stand; the
-    eigenvalues come back to the host in one copy, the split's one synchronisation.'''
-    if not blocks:
-        return []
-    sizes = [b.shape[0] if left_isometric else b.shape[1] for b in blocks]
-    offsets = np.concatenate([[0], np.cumsum([g * g for g in sizes])]).tolist()
-    packed = torch.empty(max(offsets[-1], 1), dtype=torch.float64, device=blocks[0].device)
-    for b, g, at in zip(blocks, sizes, offsets):
+def _negated_grams(sides):
+    '''-C C^T of every (g x n) matrix C, packed back to back in one buffer.'''
+    sizes = [c.shape[0] for c in sides]
+    offsets = np.concatenate([[0], np.cumsum([g * g for g in sizes])]).astype(np.int64).tolist()
+    packed = torch.empty(max(offsets[-1], 1), dtype=torch.float64, device=sides[0].device)
+    for c, g, at in zip(sides, sizes, offsets):
         gram = packed[at:at + g * g].view(g, g)
-        if left_isometric:
-            torch.addmm(gram, b, b.transpose(0, 1), beta=0.0, alpha=-1.0, out=gram)
-        else:
-            torch.addmm(gram, b.transpose(0, 1), b, beta=0.0, alpha=-1.0, out=gram)
-    values, vectors = _native.eigh_batched(packed, sizes)
-    negated = values.cpu().numpy()
+        torch.addmm(gram, c, c.transpose(0, 1), beta=0.0, alpha=-1.0, out=gram)
+    return packed, sizes, offsets[:-1]
+
+
+def _row_norms(matrices) -> torch.Tensor:
+    '''The Euclidean norm of every row of every matrix, concatenated, on the device.'''
+    return torch.cat([torch.linalg.vector_norm(m, dim=1) for m in matrices])
+
+
+def _rotated(x: torch.Tensor, head: torch.Tensor, tail: torch.Tensor):
+    '''(head, tail) rows under the rotation exp([[0, X^T], [-X, 0]]) to second order:
+    head' = (1 - X^T X / 2) head + X^T tail, tail' = -X head + (1 - X X^T / 2) tail; the rotation
+    is orthogonal up to |X|^3.'''
+    xt = x.transpose(0, 1)
+    new_head = head + torch.matmul(xt, tail) - 0.5 * torch.matmul(torch.matmul(xt, x), head)
+    new_tail = tail - torch.matmul(x, head) - 0.5 * torch.matmul(torch.matmul(x, xt), tail)
+    return new_head, new_tail
+
+
+def _gram_split(blocks, left_isometric: bool, cut):
+    '''The passes of `tentorch_amd.gram.GramSplit` (Gram eigenvectors, projected row norms,
+    decoupled tail passes; its module text gives the numerics) in torch operations per block: the
+    path for `SvdSettings.eigh = 'torch'` and for Gram orders above the device eigensolver's limit.
+    With C = B (`left_isometric`) or B^T, returns per block (Q rows [r x g], P = Q C rows [r x n], the
+    singular values on the host), r = min(g, n), in descending value order.'''
+    sides = [b if left_isometric else b.transpose(0, 1) for b in blocks]
+    packed, sizes, offsets = _negated_grams(sides)
+    q = [rows.clone() for rows in _eigenrows(packed, sizes, offsets)]
+    p = [torch.matmul(qb, cb) for qb, cb in zip(q, sides)]
+    ranks = [min(c.shape) for c in sides]
+    norms = _row_norms(p).cpu().numpy()
+    starts = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    values = [norms[starts[b]:starts[b] + ranks[b]].copy() for b in range(len(sides))]
+    first = [0] * len(sides)          # the first row of every block's current pass
+    while True:
+        tails = choose_tails(values, first, cut(values), SvdSettings.reliable, SvdSettings.resolved)
+        if not tails:
+            break
+        for b, h in tails:
+            head, tail = p[b][first[b]:h], p[b][h:]
+            x = torch.matmul(tail, head.transpose(0, 1)) / (head * head).sum(dim=1)
+            p[b] = torch.cat([p[b][:first[b]], *_rotated(x, head, tail)])
+            q[b] = torch.cat([q[b][:first[b]], *_rotated(x, q[b][first[b]:h], q[b][h:])])
+        tail_p = [p[b][h:] for b, h in tails]
+        packed, t_sizes, t_offsets = _negated_grams(tail_p)
+        for (b, h), w in zip(tails, _eigenrows(packed, t_sizes, t_offsets)):
+            p[b] = torch.cat([p[b][:h], torch.matmul(w, p[b][h:])])
+            q[b] = torch.cat([q[b][:h], torch.matmul(w, q[b][h:])])
+        tail_norms = _row_norms([p[b][h:ranks[b]] for b, h in tails]).cpu().numpy()
+        at = 0
+        for b, h in tails:
+            n = ranks[b] - h
+            values[b][h:] = tail_norms[at:at + n]
+            at += n
+            first[b] = h
     out = []
-    value_at = 0
-    for b, g, at in zip(blocks, sizes, offsets):
-        r = min(b.shape)
-        singular = np.sqrt(np.maximum(-negated[value_at:value_at + r], 0.0))
-        rows = vectors[at:at + g * g].view(g, g)[:r]      # eigenvector j is row j
-        out.append((rows.transpose(0, 1), singular))
-        value_at += g
-    return out
-
-
-def _gram_factors_torch(blocks, left_isometric: bool):
-    '''For each (m x n) block: the isometric factor Q (m x r or n x r, r = min(m, n)) with the
-    singular values in descending order (host, FP64), from batched eigh of -B B^T (left) or -B^T B
-    (right).  Blocks are batched by Gram size in buckets that differ by at most 25 %; each Gram
-    matrix is padded with +1 on the diagonal, above the non-positive spectrum of a negated Gram
-    matrix, so the padding never enters the leading r.'''
-    grams = []
-    for b in blocks:
-        grams.append(-(b @ b.transpose(0, 1)) if left_isometric else -(b.transpose(0, 1) @ b))
-    order = sorted(range(len(blocks)), key=lambda i: grams[i].shape[0])
-    out = [None] * len(blocks)
-    at = 0
-    while at < len(order):
-        end = at + 1
-        while end < len(order) and grams[order[end]].shape[0] <= 1.25 * grams[order[at]].shape[0]:
-            end += 1
-        members = order[at:end]
-        size = grams[members[-1]].shape[0]
-        batch = torch.zeros((len(members), size, size), dtype=grams[members[0]].dtype, device=grams[members[0]].device)
-        for slot, i in enumerate(members):
-            n = grams[i].shape[0]
-            batch[slot, :n, :n] = grams[i]
-            if n < size:
-                batch[slot, n:, n:] = torch.eye(size - n, dtype=batch.dtype, device=batch.device)
-        values, vectors = torch.linalg.eigh(batch)
-        negated = values.cpu().numpy()
-        for slot, i in enumerate(members):
-            m, n = blocks[i].shape
-            r = min(m, n)
-            g = grams[i].shape[0]
-            out[i] = (vectors[slot, :g, :r], np.sqrt(np.maximum(-negated[slot, :r], 0.0)))
-        at = end
+    for b, v in enumerate(values):
+        qb, pb = q[b][:ranks[b]], p[b][:ranks[b]]
+        order = np.argsort(-v, kind='stable')
+        if np.any(order != np.arange(v.size)):
+            index = torch.as_tensor(order, device=pb.device)
+            qb, pb, v = qb[index], pb[index], v[order]
+        out.append((qb, pb, v))
     return out
 
 
@@ -223,13 +241,28 @@ def _split(tensor: Tensor, left: Optional[Sequence[int]], max_bond: Optional[int
     factors = []
     items = [(key, block) for key, block in arranged.blocks().items() if block.numel() != 0]
     gram = SvdSettings.method == 'gram' and absorb in (1, 2) and not arranged.dtype.is_complex
-    if gram:
+    device_split = None
+    if gram and items:
         # absorb 2 keeps U isometric (values go right), absorb 1 keeps V isometric; the singular
-        # values arrive on the host
-        left_iso = absorb == 2
-        for (key, block), (q, v) in zip(items, _gram_factors([b for _, b in items], left_iso)):
-            factors.append({'key': key, 'block': block, 'q': q, 'v': v, 'kept': 0, 'offset': 0})
-        host = np.concatenate([f['v'] for f in factors]) if factors else np.zeros(0)
+        # values arrive on the host, each block's in descending order
+        def floor(values):
+            # the smallest value the ranking below can keep: the tolerance, or the first value the cap
+            # drops (a block's tail that lies wholly below it is dropped whatever its exact values)
+            ranked = np.sort(np.concatenate(values))[::-1]
+            alive = ranked[ranked > tol]
+            return max(tol, float(alive[max_bond])) if max_bond is not None and alive.size > max_bond else tol
+
+        blocks = [b for _, b in items]
+        on_device = SvdSettings.eigh == 'device' and max(
+            (b.shape[0] if absorb == 2 else b.shape[1]) for b in blocks) <= _native.EIGH_MAX_ORDER
+        if on_device:
+            device_split = GramSplit(blocks, absorb == 2)
+            for (key, _), v in zip(items, device_split.factor(floor, SvdSettings.reliable, SvdSettings.resolved)):
+                factors.append({'key': key, 'v': v, 'kept': 0, 'offset': 0})
+        else:
+            for (key, _), (q, projected, v) in zip(items, _gram_split(blocks, absorb == 2, floor)):
+                factors.append({'key': key, 'q': q, 'p': projected, 'v': v, 'kept': 0, 'offset': 0})
+        host = np.concatenate([f['v'] for f in factors])
         sizes = [f['v'].size for f in factors]
     else:
         for key, block in items:
@@ -294,6 +327,7 @@ def _split(tensor: Tensor, left: Optional[Sequence[int]], max_bond: Optional[int
     kept_values = np.zeros(bond.multiplets())
     values = None if gram else torch.zeros(bond.multiplets(), dtype=arranged.dtype, device=arranged.device)
     starts = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64) if factors else np.zeros(1, dtype=np.int64)
+    placements = []
     for fi, f in enumerate(factors):
         k = f['kept']
         if not k:
@@ -301,16 +335,19 @@ def _split(tensor: Tensor, left: Optional[Sequence[int]], max_bond: Optional[int
         r, s, c = f['key']
         o = f['offset']
         kept_values[placed[s] + o:placed[s] + o + k] = host[starts[fi]:starts[fi] + k]
+        if gram and device_split is not None:
+            placements.append((fi, k, first.block(r, s, c), o, second.block(s, s), o))
+            continue
         if gram:
-            # two library calls per block, the product written in place: with absorb 2 the rows
-            # o..o+k of V's block are one contiguous range
-            q = f['q'][:, :k]
+            # the isometric factor is Q's leading rows, the other the projection P = Q C already
+            # formed: two copies per block
+            q, projected = f['q'][:k], f['p'][:k]
             if absorb == 2:
-                first.block(r, s, c)[:, o:o + k].copy_(q)
-                torch.matmul(q.transpose(0, 1), f['block'], out=second.block(s, s)[o:o + k, :])
+                first.block(r, s, c)[:, o:o + k].copy_(q.transpose(0, 1))
+                second.block(s, s)[o:o + k, :].copy_(projected)
             else:
-                first.block(r, s, c)[:, o:o + k] = f['block'] @ q
-                second.block(s, s)[o:o + k, :].copy_(q.transpose(0, 1))
+                first.block(r, s, c)[:, o:o + k].copy_(projected.transpose(0, 1))
+                second.block(s, s)[o:o + k, :].copy_(q)
             continue
         v = f['v'][:k]
         u, vh = f['u'][:, :k], f['vh'][:k]
@@ -324,10 +361,15 @@ def _split(tensor: Tensor, left: Optional[Sequence[int]], max_bond: Optional[int
         first.block(r, s, c)[:, o:o + k] = u
         second.block(s, s)[o:o + k, :] = vh
         values[placed[s] + o:placed[s] + o + k] = v.to(arranged.dtype)
+    if gram and device_split is not None:
+        device_split.assemble(placements)
     if gram:
         # one copy from pinned memory, queued behind the products without waiting for them
-        staged = torch.from_numpy(kept_values).to(arranged.dtype).pin_memory()
-        values = staged.to(arranged.device, non_blocking=True)
+        staged = torch.from_numpy(kept_values).to(arranged.dtype)
+        if arranged.device.type == 'cuda':
+            values = staged.pin_memory().to(arranged.device, non_blocking=True)
+        else:
+            values = staged
     return first, values, second, kept_values
 
 

@@ -1,0 +1,107 @@
+'''
+The split's device launches interpreted on the CPU: `tt_bgemm`, `tt_row_norms`, `tt_copy_rows` and
+the batched eigensolver, run by numpy on the host buffers their problem tables address.  With the
+tables pointing at CPU tensors, `tentorch_amd.gram.GramSplit` runs end to end without a GPU, so its
+table construction, its passes and its bounds checks are tested on the CPU; the kernels themselves
+are tested on the GPU against this reading of the header's semantics.
+'''
+
+import contextlib
+import ctypes
+
+import numpy as np
+import torch
+
+from tentorch_amd import _native
+
+
+def _doubles(address: int, count: int) -> np.ndarray:
+    return np.ctypeslib.as_array((ctypes.c_double * max(count, 1)).from_address(address))[:count]
+
+
+def _int32s(address: int, count: int) -> np.ndarray:
+    return np.ctypeslib.as_array((ctypes.c_int32 * max(count, 1)).from_address(address))[:count]
+
+
+def _matrix(address: int, rows: int, cols: int, ld: int) -> np.ndarray:
+    '''A rows x cols row-major view with row stride ld.'''
+    flat = _doubles(address, (rows - 1) * ld + cols if rows and cols else 0)
+    return np.lib.stride_tricks.as_strided(flat, shape=(rows, cols), strides=(8 * ld, 8)) if rows and cols \
+        else np.zeros((rows, cols))
+
+
+class _Tables:
+    '''Uploaded tables: kept on the host, addressed by their numpy buffers.'''
+
+    def __init__(self):
+        self.keep = []
+
+    def upload(self, *tables):
+        copies = [np.ascontiguousarray(t).copy() for t in tables]
+        self.keep.append(copies)
+        return copies, [c.ctypes.data for c in copies]
+
+
+def _find(address, keep):
+    for tables in keep:
+        for t in tables:
+            if t.ctypes.data == address:
+                return t
+    raise KeyError('unknown table address')
+
+
+@contextlib.contextmanager
+def emulated_split():
+    tables = _Tables()
+    saved = (_native.upload, _native.bgemm, _native.row_norms, _native.copy_rows, _native.eigh_batched)
+
+    def bgemm(table, address, tiles):
+        table = _find(address, tables.keep)
+        for p in table:
+            m, n, k = int(p['m']), int(p['n']), int(p['k'])
+            if not (m and n):
+                continue
+            a = _matrix(int(p['a']), *((k, m) if p['trans_a'] else (m, k)), int(p['lda'])) if k else np.zeros((m, 0))
+            b = _matrix(int(p['b']), *((n, k) if p['trans_b'] else (k, n)), int(p['ldb'])) if k else np.zeros((0, n))
+            product = (a.T if p['trans_a'] else a) @ (b.T if p['trans_b'] else b)
+            if int(p['col_scale']):
+                product = product * _doubles(int(p['col_scale']), n)
+            c = _matrix(int(p['c']), m, n, int(p['ldc']))
+            c[...] = p['alpha'] * product + (p['beta'] * c if p['beta'] != 0.0 else 0.0)
+
+    def row_norms(table, address, rows):
+        for p in _find(address, tables.keep):
+            r, c = int(p['rows']), int(p['cols'])
+            if r and c:
+                _doubles(int(p['dst']), r)[...] = np.linalg.norm(_matrix(int(p['src']), r, c, int(p['src_ld'])), axis=1)
+
+    def copy_rows(table, address, rows):
+        for p in _find(address, tables.keep):
+            r, c = int(p['rows']), int(p['cols'])
+            if not (r and c):
+                continue
+            index = _int32s(int(p['index']), r) if int(p['index']) else np.arange(r)
+            held = int(index.max()) + 1
+            source = _matrix(int(p['src']), held, c, int(p['src_ld']))[index]
+            if p['transpose']:
+                _matrix(int(p['dst']), c, r, int(p['dst_ld']))[...] = source.T
+            else:
+                _matrix(int(p['dst']), r, c, int(p['dst_ld']))[...] = source
+
+    def eigh_batched(matrices, sizes, offsets=None, work=None, vectors=None):
+        values = torch.empty(max(sum(sizes), 1), dtype=torch.float64)
+        vectors = torch.empty_like(matrices) if vectors is None else vectors
+        at = 0
+        for n, offset in zip(sizes, offsets):
+            w, v = np.linalg.eigh(matrices[offset:offset + n * n].view(n, n).numpy())
+            values[at:at + n] = torch.from_numpy(w)
+            vectors[offset:offset + n * n] = torch.from_numpy(np.ascontiguousarray(v.T)).reshape(-1)
+            at += n
+        return values[:at], vectors
+
+    _native.upload, _native.bgemm, _native.row_norms, _native.copy_rows, _native.eigh_batched = (
+        tables.upload, bgemm, row_norms, copy_rows, eigh_batched)
+    try:
+        yield
+    finally:
+        (_native.upload, _native.bgemm, _native.row_norms, _native.copy_rows, _native.eigh_batched) = saved

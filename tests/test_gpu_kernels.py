@@ -424,3 +424,66 @@ def test_batched_eigh_matches_numpy(largest):
         assert np.abs(c @ rows.T - rows.T * lam).max() <= 1e-12 * scale, n
         at += n * n
         vat += n
+
+
+def test_batched_gemm_with_transposes_matches_numpy():
+    '''tt_bgemm over one table mixing every transpose pair, ragged sizes (1 to 150, not multiples of
+    the 64 x 64 tile or the 16-deep slab), an empty depth, column scales, alpha and beta.'''
+    from tentorch_amd import _native
+    from tentorch_amd.gram import _Launches, _gemm
+    rng = np.random.default_rng(51)
+    shapes = [(1, 1, 1), (17, 5, 33), (64, 64, 16), (65, 130, 150), (150, 3, 97), (40, 70, 0), (3, 129, 64)]
+    cases, buffers = [], []
+    extents = _native.Extents()
+    for i, (m, n, k) in enumerate(shapes):
+        ta, tb = i % 2, (i // 2) % 2
+        a = rng.standard_normal((k, m) if ta else (m, k))
+        b = rng.standard_normal((n, k) if tb else (k, n))
+        c = rng.standard_normal((m, n))
+        scale = rng.standard_normal(n) if i % 3 == 0 else None
+        alpha, beta = 0.5 + i, (0.0 if i % 2 else -1.5)
+        ref = alpha * ((a.T if ta else a) @ (b.T if tb else b)) * (scale if scale is not None else 1.0) + beta * c
+        da, db, dc = (torch.tensor(x, device='cuda').contiguous() for x in (a, b, c))
+        ds = torch.tensor(scale, device='cuda') if scale is not None else None
+        for t in (da, db, dc) + ((ds,) if ds is not None else ()):
+            extents.add(t)
+        cases.append((dc, ref))
+        buffers.append((da, db, dc, ds))
+        lda, ldb = (m if ta else k), (k if tb else n)
+        buffers[-1] += (_gemm(da.data_ptr(), db.data_ptr(), dc.data_ptr(), m, n, k, max(lda, 1), max(ldb, 1), n,
+                              ta, tb, alpha, beta, ds.data_ptr() if ds is not None else 0),)
+    launches = _Launches(extents)
+    launches.gemm([entry[-1] for entry in buffers])
+    keep = launches.upload()
+    launches.run()
+    torch.cuda.synchronize()
+    for dc, ref in cases:
+        assert np.abs(dc.cpu().numpy() - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+
+
+def test_row_norms_and_row_copies_match_numpy():
+    '''tt_row_norms and tt_copy_rows (with and without a row index, straight and transposed) on
+    matrices with row strides longer than their rows.'''
+    from tentorch_amd import _native
+    from tentorch_amd.gram import _Launches, _rows
+    rng = np.random.default_rng(52)
+    src = torch.tensor(rng.standard_normal((70, 90)), device='cuda')       # rows of 80, stride 90
+    norms = torch.empty(70, dtype=torch.float64, device='cuda')
+    straight = torch.full((70, 85), float('nan'), dtype=torch.float64, device='cuda')
+    turned = torch.full((80, 75), float('nan'), dtype=torch.float64, device='cuda')
+    order = rng.permutation(70)[:33].astype(np.int32)
+    index = torch.tensor(order, device='cuda')
+    extents = _native.Extents().add(src).add(norms).add(straight).add(turned).add(index)
+    launches = _Launches(extents)
+    launches.rows('norms', [_rows(src.data_ptr(), norms.data_ptr(), 70, 80, 90, 1)])
+    launches.rows('copy', [_rows(src.data_ptr(), straight.data_ptr(), 70, 80, 90, 85),
+                           _rows(src.data_ptr(), turned.data_ptr(), 33, 80, 90, 75, 1, index.data_ptr())],
+                  source_rows=[70, 70])
+    keep = launches.upload()
+    launches.run()
+    host = src.cpu().numpy()[:, :80]
+    assert np.abs(norms.cpu().numpy() - np.linalg.norm(host, axis=1)).max() <= 1e-13 * np.abs(host).max() * 10
+    assert np.array_equal(straight.cpu().numpy()[:, :80], host)
+    assert np.all(np.isnan(straight.cpu().numpy()[:, 80:]))
+    assert np.array_equal(turned.cpu().numpy()[:, :33], host[order].T)
+    assert np.all(np.isnan(turned.cpu().numpy()[:, 33:]))

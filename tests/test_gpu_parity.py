@@ -276,3 +276,55 @@ def test_exact_split_reorders_unsorted_singular_values(monkeypatch):
     rec = tt.contract(u, v, tt.domain_legs(u), [0]).data
     ref_rec = tt.contract(ref_u, ref_v, tt.domain_legs(ref_u), [0]).data
     assert (rec - ref_rec).abs().max() <= 1e-12 * ref_rec.abs().max()
+
+
+def _deep_spectrum_tensor(seed):
+    '''A U(1) two-site tensor (bond, phys | phys, bond) whose blocks hold hand-set singular values:
+    one geometric sequence 10^-((k + 1/2) / 4), k = 0, 1, ..., from about 1 down past 1e-16, dealt
+    round-robin to the blocks (so the ranking runs across sectors), each block padded with exact
+    zeros up to its rank, between random orthogonal factors.  Consecutive values differ by a factor
+    1.78, and none lies within a factor 1.3 of the tolerance 1e-14.'''
+    rng = np.random.default_rng(seed)
+    cat = osym.u1
+    bond = osym.Leg(cat, [(-2,), (0,), (2,)], [9, 14, 9])
+    phys = osym.Leg(cat, [(-1,), (1,)], [1, 1])
+    t = ot.Tensor(cat, [bond, phys], [phys, bond], (0,))
+    keys = list(t.blocks)
+    sequence = [10.0 ** (-(k + 0.5) / 4) for k in range(68)]
+    spectra = {key: sequence[i::len(keys)] for i, key in enumerate(keys)}
+    for key, block in t.blocks.items():
+        m, n = block.shape
+        k = min(m, n)
+        values = np.zeros(k)
+        given = np.asarray(spectra[key])[:k]
+        values[:given.size] = given
+        u, _ = np.linalg.qr(rng.standard_normal((m, m)))
+        v, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        block[...] = (u[:, :k] * values) @ v[:, :k].T
+    return t
+
+
+@pytest.mark.parametrize('absorb', [2, 1])
+@pytest.mark.parametrize('cap', [None, 30, 45])
+def test_split_keeps_the_oracles_multiplets_on_a_deep_spectrum(absorb, cap):
+    '''The DMRG split (Gram eigendecompositions on the device, projected row norms, decoupled tail
+    passes) against the oracle's per-block SVD on singular values spanning sixteen decades with exact
+    zeros: the same kept multiplets per sector (bond legs bit-equal) with the cap high in the
+    spectrum, in its deep tail (about 1e-11) or absent (the tolerance then cuts, and the exact zeros
+    must go), the same kept values to 1e-14 of the largest, and the same truncated tensor.'''
+    from tentorch_amd.linalg import _split
+    from bridge import same_legs, to_device
+    ref_u, ref_s, ref_v = ot.svd(_deep_spectrum_tensor(17), None, cap, 1e-14, absorb)
+    two = to_device(_deep_spectrum_tensor(17))
+    u, s, v, kept = _split(two, None, cap, 1e-14, absorb)
+    assert same_legs(u.domain[0], ref_u.domain[0]), (u.domain[0], ref_u.domain[0])
+    assert np.abs(kept - ref_s).max() <= 1e-14 * np.abs(ref_s).max()
+    assert np.all(kept > 1e-14)
+    got = tt.contract(u, v, tt.domain_legs(u), [0]).data.cpu().numpy()
+    ref = ot.contract(ref_u, ref_v, [2], [0]).flat()
+    assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+    # the isometric factor stays orthonormal
+    for block in (u if absorb == 2 else v).blocks().values():
+        m = block if absorb == 2 else block.transpose(0, 1)
+        eye = torch.eye(m.shape[1], dtype=m.dtype, device=m.device)
+        assert (m.transpose(0, 1) @ m - eye).abs().max() <= 1e-12

@@ -241,3 +241,117 @@ def test_dense_boxes_equal_the_entry_by_entry_expansion():
                      ([BOND], [BOND, PHYS, MPOL])):
         t = _random_u1(rng, cod, dom)
         assert np.array_equal(dense(t), dense_by_entry(t))
+
+
+# -- the splittings (`svd.cpp:41-257`), pinned by the dense matrix each abelian tensor is -------------
+
+def _matrix(t):
+    '''The dense array of a tensor as (codomain legs) x (domain legs).'''
+    d = dense(t)
+    rows = int(np.prod([leg.multiplets() for leg in t.codomain]))
+    return d.reshape(rows, -1)
+
+
+def _tensor_with_spectra(rng, cat, codomain, domain, spectra):
+    '''A tensor whose block with domain root s has singular values spectra[s] (padded with zeros).'''
+    t = ot.Tensor(cat, codomain, domain, cat.trivial())
+    for (r, s, c), block in t.blocks.items():
+        m, n = block.shape
+        k = min(m, n)
+        values = np.zeros(k)
+        given = np.asarray(spectra.get(s, []), dtype=np.float64)[:k]
+        values[:given.size] = given
+        u, _ = np.linalg.qr(rng.standard_normal((m, m)))
+        v, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        block[...] = (u[:, :k] * values) @ v[:, :k].T
+    return t
+
+
+@pytest.mark.parametrize('absorb', [0, 1, 2, 3])
+def test_oracle_svd_is_the_truncated_svd_of_the_dense_matrix(absorb):
+    '''No symmetry (one self-dual sector, so no re-pairing): the values are the dense matrix's
+    singular values, the kept count is min(cap, #values > tol), and U V reconstructs the tensor up to
+    exactly the dropped weight (Eckart-Young), whatever factor takes the values.'''
+    rng = np.random.default_rng(12)
+    cat = osym.no_symmetry
+    a, b = osym.Leg(cat, [(0,)], [6]), osym.Leg(cat, [(0,)], [2])
+    t = ot.random_tensor(cat, [a, b], [b, a], (0,), rng)
+    dense_values = np.linalg.svd(_matrix(t), compute_uv=False)
+    all_values = np.sort(ot.svdvals(t))[::-1]
+    assert np.abs(all_values - dense_values).max() < 1e-12
+    tol = dense_values[9] * 0.999            # the tenth value survives the tolerance, the eleventh not
+    for cap, expected in ((None, 10), (4, 4), (40, 10)):
+        u, s, v = ot.svd(t, None, cap, tol, absorb)
+        assert s.size == expected == u.domain[0].multiplets()
+        assert np.abs(np.sort(s)[::-1] - dense_values[:expected]).max() < 1e-12
+        if absorb == 0:       # both factors isometric: the values sit between them
+            rebuilt = _matrix(u) @ np.diag(s) @ _matrix(v)
+        else:
+            rebuilt = _matrix(ot.contract(u, v, [2], [0]))
+        dropped = float(np.sum(dense_values[expected:] ** 2))
+        assert abs(np.linalg.norm(_matrix(t) - rebuilt) ** 2 - dropped) < 1e-10
+        if absorb in (0, 2):
+            m = _matrix(u)
+            assert np.abs(m.T @ m - np.eye(expected)).max() < 1e-12
+        if absorb in (0, 1):
+            m = _matrix(v)
+            assert np.abs(m @ m.T - np.eye(expected)).max() < 1e-12
+
+
+def test_oracle_svd_ranks_across_sectors_and_re_pairs_duals():
+    '''U(1) blocks with hand-chosen values: the cap keeps the largest values over all sectors, then
+    a sector and its dual keep the same count (the smaller), refilled in rank order
+    (`svd.cpp:97-119`); exact zeros never survive, whatever the cap.'''
+    rng = np.random.default_rng(13)
+    cat = osym.u1
+    bond = osym.Leg(cat, [(-2,), (0,), (2,)], [3, 3, 3])
+    one = osym.Leg(cat, [(0,)], [1])
+    spectra = {(-2,): [0.9, 0.5, 0.1], (0,): [0.8, 0.4, 0.05], (2,): [0.7, 0.45, 0.0]}
+    t = _tensor_with_spectra(rng, cat, [bond, one], [bond], spectra)
+    cases = {
+        # cap: (multiplicities by sector, kept values by sector)
+        4: ({(-2,): 1, (0,): 1, (2,): 1}, [0.9, 0.8, 0.7]),     # 0.5 of sector -2 loses its pair
+        5: ({(-2,): 2, (0,): 1, (2,): 2}, [0.9, 0.5, 0.8, 0.7, 0.45]),   # 0.45 pairs 0.5
+        6: ({(-2,): 2, (0,): 2, (2,): 2}, [0.9, 0.5, 0.8, 0.4, 0.7, 0.45]),
+        9: ({(-2,): 2, (0,): 3, (2,): 2}, [0.9, 0.5, 0.8, 0.4, 0.05, 0.7, 0.45]),  # 0.1 unpaired, 0 below tol
+    }
+    for cap, (mults, values) in cases.items():
+        u, s, v = ot.svd(t, None, cap, 1e-14, 2)
+        bond_out = u.domain[0]
+        assert dict(zip(bond_out.sectors, bond_out.multiplicities)) == mults, cap
+        assert np.abs(s - values).max() < 1e-12, cap
+    # rank deficiency: a block of zeros keeps nothing, and the bond still exists
+    zero = _tensor_with_spectra(rng, cat, [bond, one], [bond], {})
+    u, s, v = ot.svd(zero, None, None, 1e-14, 2)
+    assert u.domain[0].sectors == [cat.trivial()] and u.domain[0].multiplets() == 1
+    assert s.size == 1 and s[0] == 0.0
+
+
+@pytest.mark.parametrize('split', ['qr', 'rq'])
+def test_oracle_qr_and_rq_factor_every_block(split):
+    '''qr: Q isometric (Q^T Q = 1) over (codomain | bond), R upper triangular per block, Q R the
+    tensor; rq: the mirror, with the isometry on the right.  The bond carries min(rows, cols) per
+    block of each domain root.'''
+    rng = np.random.default_rng(14)
+    cat = osym.u1
+    bond = osym.Leg(cat, [(-1,), (1,)], [3, 2])
+    phys = osym.Leg(cat, [(-1,), (1,)], [1, 1])
+    out = osym.Leg(cat, [(-2,), (0,), (2,)], [2, 3, 2])
+    t = ot.random_tensor(cat, [bond, phys], [out], (0,), rng)
+    assert len(t.blocks) == 3
+    left, right = (ot.qr if split == 'qr' else ot.rq)(t)
+    assert np.abs(_matrix(ot.contract(left, right, [2], [0])) - _matrix(t)).max() < 1e-12
+    iso = left if split == 'qr' else right
+    m = _matrix(iso)
+    if split == 'qr':
+        assert np.abs(m.T @ m - np.eye(m.shape[1])).max() < 1e-12
+        for block in right.blocks.values():
+            assert np.abs(np.tril(block, -1)).max(initial=0.0) < 1e-15
+    else:
+        assert np.abs(m @ m.T - np.eye(m.shape[0])).max() < 1e-12
+        for block in left.blocks.values():
+            assert np.abs(np.triu(block, 1)).max(initial=0.0) < 1e-15
+    expected = {}
+    for (r, s, c), block in t.blocks.items():
+        expected[s] = expected.get(s, 0) + min(block.shape)
+    assert dict(zip(left.domain[0].sectors, left.domain[0].multiplicities)) == expected
