@@ -328,3 +328,43 @@ def test_split_keeps_the_oracles_multiplets_on_a_deep_spectrum(absorb, cap):
         m = block if absorb == 2 else block.transpose(0, 1)
         eye = torch.eye(m.shape[1], dtype=m.dtype, device=m.device)
         assert (m.transpose(0, 1) @ m - eye).abs().max() <= 1e-12
+
+
+def test_editing_a_block_between_two_identical_contractions_is_seen():
+    '''A program run again on an operand whose buffer was written through a block view, a to_tuple
+    block or `.data` (torch's in-place counter moves, the library's own counter does not) must
+    redo that operand's layout move instead of reusing the hoisted copy.'''
+    rng = np.random.default_rng(61)
+    _, a = _pair(rng, [BOND, PHYS], [PHYS, BOND])
+    _, b = _pair(rng, [BOND, PHYS], [PHYS, BOND])
+    spec = 'ABcd,DCef->ABef'
+    first = tt.einsum(spec, a, b).data.clone()
+    key = a.keys()[0]
+    a.block(key.codomain, key.domain, key.copy).mul_(-3.0)       # through a block view
+    second = tt.einsum(spec, a, b).data.clone()
+    a.to_tuple()[3][key].zero_()                                   # through a to_tuple block
+    third = tt.einsum(spec, a, b).data.clone()
+    a.data[:5] = 7.0                                               # through .data
+    fourth = tt.einsum(spec, a, b).data.clone()
+    for before, after in ((first, second), (second, third), (third, fourth)):
+        assert not torch.equal(before, after)
+    fresh = tt.einsum(spec, tt.Tensor._wrap(a.structure, a.data.clone()), b).data
+    assert torch.equal(fresh, fourth)
+
+
+def test_svd_takes_both_call_forms_of_the_reference():
+    '''`svd(tensor, max_bond, tol, absorb)` (the form `dmrg.py:124` uses) and
+    `svd(tensor, left, max_bond, tol, absorb)`, positionally and by keyword.'''
+    rng = np.random.default_rng(62)
+    _, two = _pair(rng, [BOND, PHYS], [PHYS, BOND])
+    u1, s1, v1 = tt.svd(two, 5, 1e-14, 2)
+    u2, s2, v2 = tt.svd(two, [0, 1], 5, 1e-14, 2)
+    u3, s3, v3 = tt.svd(two, max_bond=5, tol=1e-14, absorb=2)
+    u4, s4, v4 = tt.svd(two, left=[0, 1], max_bond=5, absorb=2, tol=1e-14)
+    for s in (s2, s3, s4):
+        assert torch.equal(s, s1)
+    assert u1.domain[0].multiplets() <= 5      # the cap, then a sector and its dual keep one count
+    full = tt.svd(two)[1]
+    assert full.numel() == tt.svdvals(two).numel() == tt.svdvals(two, [0, 1]).numel()
+    with pytest.raises(TypeError):
+        tt.svd(two, None, 5, 1e-14, 2)            # neither form: the reference's binding refuses it too

@@ -5,6 +5,7 @@ reference writes it, on random rankings.
 '''
 
 import numpy as np
+import pytest
 
 from tentorch_amd.linalg import _paired_prefixes
 
@@ -35,3 +36,20 @@ def test_vectorised_pairing_equals_the_candidate_walk():
         room = {s: int(rng.integers(0, 20)) for s in set(sectors) if rng.random() < 0.8}
         got = _paired_prefixes(owner[order], position[order], sectors, room)
         assert np.array_equal(got, _walk(owner[order], position[order], sectors, room)), trial
+
+
+def test_svd_call_forms_bind_as_the_reference_overloads():
+    '''`bindings.cpp:367-382`: the first overload (tensor, max_bond, tol, absorb) wins whenever it
+    fits; a sequence in second place, or a `left` keyword, selects (tensor, left, max_bond, tol,
+    absorb); anything else is a TypeError.'''
+    from tentorch_amd.linalg import _bind_split_arguments as bind
+    assert bind((), {}) == (None, None, 0.0, 0)
+    assert bind((20, 1e-14, 2), {}) == (None, 20, 1e-14, 2)
+    assert bind((None, 1e-14), {}) == (None, None, 1e-14, 0)
+    assert bind(([0, 1], 20, 1e-14, 2), {}) == ([0, 1], 20, 1e-14, 2)
+    assert bind((), {'left': (1, 0), 'absorb': 1}) == ([1, 0], None, 0.0, 1)
+    assert bind((), {'max_bond': 3}) == (None, 3, 0.0, 0)
+    for args, kwargs in (((None, 20, 1e-14, 2), {}), (('01',), {}), ((1.5,), {}), ((), {'rank': 3}),
+                         ((20,), {'max_bond': 3})):
+        with pytest.raises(TypeError):
+            bind(args, kwargs)
