@@ -219,6 +219,8 @@ int tt_eigh_batched_static(const tt_eigh_problem *problems, int64_t count, doubl
  * consecutively across problems (`first` is the number of the problem's first item).
  *
  * tt_bgemm: for each problem C = alpha op(A) op(B) diag(col_scale) + beta C (col_scale null: 1),
+ * or, when row_shift is set, C_ij = alpha (op(A) op(B))_ij / (col_scale[j] - row_shift[i]) + beta C_ij
+ * (the solution of a Sylvester equation with diagonal coefficients; col_scale is then required),
  * op(A) m x k, op(B) k x n, all row-major with row strides lda, ldb, ldc; trans_a / trans_b set
  * means the operand is stored transposed (A as k x m, B as n x k).  FP64 on MFMA, one work item per
  * 64 x 64 tile of C (ceil(m / 64) ceil(n / 64) per problem, row-major); `tiles` is their total.
@@ -228,6 +230,7 @@ typedef struct tt_bgemm_problem {
   const double *b;
   double *c;
   const double *col_scale;
+  const double *row_shift;
   int64_t first;
   int32_t m, n, k;
   int32_t lda, ldb, ldc;
@@ -252,6 +255,20 @@ typedef struct tt_rows_problem {
 } tt_rows_problem;
 int tt_row_norms(const tt_rows_problem *problems, int64_t count, int64_t rows, void *stream);
 int tt_copy_rows(const tt_rows_problem *problems, int64_t count, int64_t rows, void *stream);
+/* One work item per row: dst = alpha x + beta y + gamma I over `cols` entries of each row (y null:
+ * no y term; I the identity, entry (i, i) of the problem's rows), row strides x_ld, y_ld, dst_ld.
+ * dst may be x or y. */
+typedef struct tt_lincomb_problem {
+  const double *x;
+  const double *y;
+  double *dst;
+  int64_t first;
+  int32_t rows, cols;
+  int32_t x_ld, y_ld, dst_ld;
+  int32_t reserved;
+  double alpha, beta, gamma;
+} tt_lincomb_problem;
+int tt_lincomb_rows(const tt_lincomb_problem *problems, int64_t count, int64_t rows, void *stream);
 
 /* The learned tile schedule's cost model (no counterpart in the reference): a multilayer
  * perceptron x (features) -> relu(W1^T x + b1) (hidden) -> relu(W2^T h1 + b2) (hidden) -> w3 . h2 + b3

@@ -63,6 +63,7 @@ _lib.tt_lanczos_orthonormalize.argtypes = [_i64, _int, _c_void_p, _i64, _c_void_
 _lib.tt_bgemm.argtypes = [_c_void_p, _i64, _i64, _c_void_p]
 _lib.tt_row_norms.argtypes = [_c_void_p, _i64, _i64, _c_void_p]
 _lib.tt_copy_rows.argtypes = [_c_void_p, _i64, _i64, _c_void_p]
+_lib.tt_lincomb_rows.argtypes = [_c_void_p, _i64, _i64, _c_void_p]
 _lib.tt_graph_begin.argtypes = [_c_void_p]
 _lib.tt_graph_end.argtypes = [_c_void_p, ctypes.POINTER(_c_void_p)]
 _lib.tt_graph_launch.argtypes = [_c_void_p, _c_void_p]
@@ -77,7 +78,7 @@ EXPORTED = ['tt_gss_plan_create', 'tt_gss_plan_destroy', 'tt_gss_plan_counts', '
             'tt_launch_count', 'tt_timing_enable', 'tt_timing_collect', 'tt_mlp_param_count', 'tt_mlp_forward',
             'tt_mlp_gradient', 'tt_mlp_adam', 'tt_eigh_max_order', 'tt_eigh_workspace', 'tt_eigh_batched',
             'tt_lanczos_orthonormalize', 'tt_lanczos_ritz', 'tt_multiaxpy_counted', 'tt_eigh_batched_static',
-            'tt_bgemm', 'tt_row_norms', 'tt_copy_rows']
+            'tt_bgemm', 'tt_row_norms', 'tt_copy_rows', 'tt_lincomb_rows']
 
 
 def library() -> ctypes.CDLL:
@@ -275,13 +276,17 @@ def capture(fn) -> Graph:
 
 # -- the split's batched kernels: problem tables built on the host, uploaded, then launched --------
 
-BGEMM_PROBLEM = np.dtype([('a', '<u8'), ('b', '<u8'), ('c', '<u8'), ('col_scale', '<u8'), ('first', '<i8'),
+BGEMM_PROBLEM = np.dtype([('a', '<u8'), ('b', '<u8'), ('c', '<u8'), ('col_scale', '<u8'), ('row_shift', '<u8'),
+                          ('first', '<i8'),
                           ('m', '<i4'), ('n', '<i4'), ('k', '<i4'), ('lda', '<i4'), ('ldb', '<i4'), ('ldc', '<i4'),
                           ('trans_a', '<i4'), ('trans_b', '<i4'), ('alpha', '<f8'), ('beta', '<f8')])
 ROWS_PROBLEM = np.dtype([('src', '<u8'), ('dst', '<u8'), ('index', '<u8'), ('first', '<i8'), ('rows', '<i4'),
                          ('cols', '<i4'), ('src_ld', '<i4'), ('dst_ld', '<i4'), ('transpose', '<i4'),
                          ('reserved', '<i4')])
-assert BGEMM_PROBLEM.itemsize == 88 and ROWS_PROBLEM.itemsize == 56
+LINCOMB_PROBLEM = np.dtype([('x', '<u8'), ('y', '<u8'), ('dst', '<u8'), ('first', '<i8'), ('rows', '<i4'),
+                            ('cols', '<i4'), ('x_ld', '<i4'), ('y_ld', '<i4'), ('dst_ld', '<i4'), ('reserved', '<i4'),
+                            ('alpha', '<f8'), ('beta', '<f8'), ('gamma', '<f8')])
+assert BGEMM_PROBLEM.itemsize == 96 and ROWS_PROBLEM.itemsize == 56 and LINCOMB_PROBLEM.itemsize == 80
 
 
 class Extents:
@@ -329,6 +334,10 @@ def check_bgemm(table: np.ndarray, extents: Extents) -> int:
             extents.check(int(p['c']), *_rows_touched(m, n, int(p['ldc'])), 'bgemm C')
             if int(p['col_scale']):
                 extents.check(int(p['col_scale']), 0, 8 * (n - 1), 'bgemm column scale')
+            if int(p['row_shift']):
+                if not int(p['col_scale']):
+                    raise RuntimeError('bgemm: row shifts need column shifts')
+                extents.check(int(p['row_shift']), 0, 8 * (m - 1), 'bgemm row shift')
         tiles += -(-m // 64) * -(-n // 64)
     return tiles
 
@@ -356,6 +365,28 @@ def check_rows(table: np.ndarray, extents: Extents, source_rows=None, vector_dst
                 extents.check(int(p['dst']), *_rows_touched(r, c, int(p['dst_ld'])), 'row destination')
         rows += r
     return rows
+
+
+def check_lincomb(table: np.ndarray, extents: Extents) -> int:
+    '''Validates a lincomb table; returns its row count.'''
+    rows = 0
+    for p in table:
+        r, c = int(p['rows']), int(p['cols'])
+        if r < 0 or c < 0 or int(p['first']) != rows or c > int(p['x_ld']) or c > int(p['dst_ld']):
+            raise RuntimeError('lincomb: bad sizes or row numbering')
+        if r and c:
+            extents.check(int(p['x']), *_rows_touched(r, c, int(p['x_ld'])), 'lincomb x')
+            if int(p['y']):
+                if c > int(p['y_ld']):
+                    raise RuntimeError('lincomb: y row stride shorter than its row')
+                extents.check(int(p['y']), *_rows_touched(r, c, int(p['y_ld'])), 'lincomb y')
+            extents.check(int(p['dst']), *_rows_touched(r, c, int(p['dst_ld'])), 'lincomb destination')
+        rows += r
+    return rows
+
+
+def lincomb_rows(table: np.ndarray, address: int, rows: int) -> None:
+    check(_lib.tt_lincomb_rows(address, len(table), rows, stream()), 'lincomb rows')
 
 
 def upload(*tables: np.ndarray):

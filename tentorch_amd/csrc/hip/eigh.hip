@@ -90,6 +90,13 @@ __device__ double wave_sum(double v) {
   return __hiloint2double(hi, lo);
 }
 
+// Lane `lane`'s value of a double (uniform lane index).
+__device__ inline double readlane_double(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
 // The problem a wave of a batched launch works on, when problem b owns count(n_b) consecutive
 // work items (workgroups, or waves for the leaves) in batch order.  Lane b of the wave loads problem
 // b (the table always holds kEigBatch entries, zeroed past the count), a prefix sum over the lanes'
@@ -147,102 +154,87 @@ __device__ double tri_block_sum(double v, double *scratch) {
   return total;
 }
 
-// 1'. The steps while trailing matrices are large: one launch per Householder step k, one workgroup per kStepCols trailing columns of every matrix (one column
-//     per wave), so a large matrix is spread over many CUs instead of one.  Every workgroup of a
-//     matrix redoes the step's O(n) part itself: w of step k-1 from that step's p and v, column k
-//     with step k-1's update, the reflector of step k.  Then each wave applies step k-1's update
-//     to its column (rows k+1..) and forms that column's entry of p = tau A v by a column dot
-//     product (A is symmetric).  Between launches the state lives in the workspace: v and p of the
-//     last two steps (by step parity, so a launch never writes what it reads).  The first
-//     workgroup of a matrix writes the step's scalars, v, and the reflector of step k-1 into column
-//     k-1, which no workgroup reads at step k; a last launch (k = n-1, one workgroup) forms the
-//     last diagonal entry and stores the last reflector.
+// 1'. The steps while trailing matrices are large: one launch per Householder step k, one wave per
+//     trailing column of every matrix, four waves a workgroup.  Each wave works alone, with no
+//     barrier and no LDS: its lanes hold rows k + lane + 64 q of the step's vectors (v and p of
+//     step k-1, column k as it stands), so the step's O(n) part (w of step k-1, column k with that
+//     step's update, the reflector of step k) costs two wave reductions, redone by every wave.
+//     Then the wave applies step k-1's update to its column (rows k+1..) and forms the column's
+//     entry of p = tau A v (A is symmetric, so a column dot product).  A step's time is then the
+//     launch and one round trip to memory, where a workgroup-wide prologue paid its barriers on
+//     every step.  Between launches the state lives in the workspace: v and p of the last two
+//     steps (by step parity, so a launch never writes what it reads).  The first wave of a matrix
+//     writes the step's scalars, v, and the reflector of step k-1 into column k-1, which no wave
+//     reads at step k; a last launch (k = n-1, one wave) forms the last diagonal entry and stores
+//     the last reflector.  The kernel is instantiated for 4 to 32 row chunks; each step takes the
+//     smallest that covers its largest trailing matrix.
 
-constexpr int kStepThreads = 1024;
+constexpr int kStepThreads = 256;
 constexpr int kStepCols = kStepThreads / 64;
-constexpr int kStepChunks = kEigMaxN / 64;   // 64-row chunks of a column per lane
-constexpr int kStepRows = kEigMaxN / kStepThreads;   // rows of the step vectors per thread
-static_assert(kStepRows * kStepThreads == kEigMaxN, "whole rows per thread");
+static_assert(32 * 64 >= kEigMaxN, "the largest instance covers every row");
 
 // workgroups of a matrix of order n at step k
 __host__ __device__ inline int step_blocks(int n, int k) {
   return k + 1 < n ? (n - k - 1 + kStepCols - 1) / kStepCols : (k + 1 == n ? 1 : 0);
 }
 
-__global__ __launch_bounds__(kStepThreads) void tridiag_step_kernel(const Batch *__restrict__ batch, double *__restrict__ matrices,
+template <int kChunks>
+__global__ __launch_bounds__(kStepThreads) void tridiag_step_kernel(const Batch *__restrict__ batch,
+                                                                    double *__restrict__ matrices,
                                                                     double *__restrict__ work, int k) {
   Located at;
   if (!locate(batch, static_cast<int>(blockIdx.x), [k](int order) { return step_blocks(order, k); }, &at)) return;
-  const Problem &pr = at.pr;
-  const int n = pr.n, block = static_cast<int>(blockIdx.x) - at.first;
-  double *__restrict__ A = matrices + pr.matrix;
-  double *diag = work + pr.work + 3 * int64_t(n) * n, *off = diag + n, *tau = off + n;
+  const int n = at.pr.n, block = static_cast<int>(blockIdx.x) - at.first;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int c = k + 1 + block * kStepCols + wave;   // this wave's column
+  const bool last = k + 1 == n;
+  if (last ? wave != 0 : c >= n) return;
+  double *__restrict__ A = matrices + at.pr.matrix;
+  double *diag = work + at.pr.work + 3 * int64_t(n) * n, *off = diag + n, *tau = off + n;
   double *vbuf = tau + n, *pbuf = vbuf + 2 * n;   // [2][n] each, by step parity
-  __shared__ double vp[kEigMaxN], wp[kEigMaxN], vc[kEigMaxN];
-  __shared__ double red[kStepThreads / 64];
-  __shared__ double s_alpha, s_diag;
-  const int t = threadIdx.x;
-  // rows k + t + kStepThreads r of the step vectors: v and p of step k-1 (zero before the first
-  // step), and column k as it stands
-  double v[kStepRows], p[kStepRows], a[kStepRows];
+  const double *vprev = vbuf + ((k - 1) & 1) * n, *pprev = pbuf + ((k - 1) & 1) * n;
+  // every load of the step at once: the vectors of step k-1, column k, this wave's column
+  double v[kChunks], p[kChunks], a[kChunks], y[kChunks];
 #pragma unroll
-  for (int r = 0; r < kStepRows; ++r) {
-    const int i = k + t + kStepThreads * r;
-    v[r] = p[r] = a[r] = 0.0;
-    if (i < n) {
-      a[r] = A[i + int64_t(k) * n];
-      if (k > 0) {
-        v[r] = vbuf[((k - 1) & 1) * n + i];
-        p[r] = pbuf[((k - 1) & 1) * n + i];
-      }
-    }
+  for (int q = 0; q < kChunks; ++q) {
+    const int i = k + lane + 64 * q;
+    const bool in = i < n;
+    v[q] = in && k > 0 ? vprev[i] : 0.0;
+    p[q] = in && k > 0 ? pprev[i] : 0.0;
+    a[q] = in ? A[i + int64_t(k) * n] : 0.0;
+    y[q] = in && !last ? A[i + int64_t(c) * n] : 0.0;
   }
   const double tprev = k > 0 ? tau[k - 1] : 0.0;
-  // this wave's trailing column, rows k+1.., loaded now so the loads overlap the step's O(n) part
-  const int wave = t >> 6, lane = t & 63;
-  const int c = k + 1 + block * kStepCols + wave;
-  double y[kStepChunks];
-#pragma unroll
-  for (int q = 0; q < kStepChunks; ++q) {
-    const int r = k + 1 + lane + 64 * q;
-    y[q] = (c < n && r < n) ? A[r + int64_t(c) * n] : 0.0;
-  }
+  const double vcol = k > 0 && !last ? vprev[c] : 0.0, pcol = k > 0 && !last ? pprev[c] : 0.0;
+  // w = p - tau/2 (p . v) v of step k-1, and column k with step k-1's update
   double partial = 0.0;
 #pragma unroll
-  for (int r = 0; r < kStepRows; ++r) partial += p[r] * v[r];
-  const double pv = tri_block_sum(partial, red);
-  double w[kStepRows];
+  for (int q = 0; q < kChunks; ++q) partial += p[q] * v[q];
+  const double half = 0.5 * tprev * wave_sum(partial);
+  double *const w = p, *const x = a;   // w replaces p, and the updated column k replaces a
 #pragma unroll
-  for (int r = 0; r < kStepRows; ++r) {
-    const int i = k + t + kStepThreads * r;
-    w[r] = p[r] - 0.5 * tprev * pv * v[r];
-    if (i < n) {
-      vp[i] = v[r];
-      wp[i] = w[r];
-    }
-  }
-  __syncthreads();
-  double x[kStepRows];
+  for (int q = 0; q < kChunks; ++q) w[q] = p[q] - half * v[q];
+  const double vk = readlane_double(v[0], 0), wk = readlane_double(w[0], 0);   // row k: lane 0, chunk 0
   partial = 0.0;
 #pragma unroll
-  for (int r = 0; r < kStepRows; ++r) {
-    const int i = k + t + kStepThreads * r;
-    x[r] = i < n ? a[r] - (v[r] * wp[k] + w[r] * vp[k]) : 0.0;
-    if (i == k) s_diag = x[r];
-    if (i == k + 1) s_alpha = x[r];
-    if (i < n && i >= k + 2) partial += x[r] * x[r];
+  for (int q = 0; q < kChunks; ++q) {
+    const int i = k + lane + 64 * q;
+    x[q] = i < n ? a[q] - (v[q] * wk + w[q] * vk) : 0.0;
+    if (i >= k + 2) partial += x[q] * x[q];
   }
-  const double sigma = tri_block_sum(partial, red);
-  if (k + 1 == n) {
-    if (t == 0) diag[k] = s_diag;
+  const double sigma = wave_sum(partial);
+  const double dk = readlane_double(x[0], 0);
+  const bool leader = block == 0 && wave == 0;
+  if (last) {
+    if (lane == 0) diag[k] = dk;
 #pragma unroll
-    for (int r = 0; r < kStepRows; ++r) {
-      const int i = k + t + kStepThreads * r;
-      if (k > 0 && i < n) A[i + int64_t(k - 1) * n] = v[r];
+    for (int q = 0; q < kChunks; ++q) {
+      const int i = k + lane + 64 * q;
+      if (k > 0 && i < n) A[i + int64_t(k - 1) * n] = v[q];
     }
     return;
   }
-  const double alpha = s_alpha;
+  const double alpha = readlane_double(x[0], 1);   // row k + 1
   double beta = alpha, tk = 0.0, scale = 0.0;
   if (sigma != 0.0) {
     const double norm = sqrt(alpha * alpha + sigma);
@@ -250,40 +242,32 @@ __global__ __launch_bounds__(kStepThreads) void tridiag_step_kernel(const Batch 
     tk = (beta - alpha) / beta;
     scale = 1.0 / (alpha - beta);
   }
-#pragma unroll
-  for (int r = 0; r < kStepRows; ++r) {
-    const int i = k + t + kStepThreads * r;
-    const double vnew = i == k + 1 ? 1.0 : (i < n && i >= k + 2 ? x[r] * scale : 0.0);
-    if (i < n) {
-      vc[i] = vnew;
-      if (block == 0) {
-        vbuf[(k & 1) * n + i] = vnew;
-        if (k > 0) A[i + int64_t(k - 1) * n] = v[r];
-      }
-    }
-  }
-  if (block == 0 && t == 0) {
-    diag[k] = s_diag;
-    off[k] = beta;
-    tau[k] = tk;
-  }
-  __syncthreads();
-  // this wave's column: step k-1's update on rows k+1.., and its entry of p
-  if (c >= n) return;
-  double *__restrict__ col = A + int64_t(c) * n;
-  const double vpc = vp[c], wpc = wp[c];
+  // the reflector of step k, and this wave's column with step k-1's update, dotted with it
+  const double wcol = pcol - half * vcol;
   double acc = 0.0;
 #pragma unroll
-  for (int q = 0; q < kStepChunks; ++q) {
-    const int r = k + 1 + lane + 64 * q;
-    if (r < n) {
-      const double z = y[q] - (vp[r] * wpc + wp[r] * vpc);
-      col[r] = z;
-      acc += z * vc[r];
+  for (int q = 0; q < kChunks; ++q) {
+    const int i = k + lane + 64 * q;
+    const double vnew = i == k + 1 ? 1.0 : (i >= k + 2 && i < n ? x[q] * scale : 0.0);
+    if (leader && i < n) {
+      vbuf[(k & 1) * n + i] = vnew;
+      if (k > 0) A[i + int64_t(k - 1) * n] = v[q];
+    }
+    if (i >= k + 1 && i < n) {
+      const double z = y[q] - (v[q] * wcol + w[q] * vcol);
+      A[i + int64_t(c) * n] = z;
+      acc += z * vnew;
     }
   }
   acc = wave_sum(acc);
-  if (lane == 0) pbuf[(k & 1) * n + c] = tk * acc;
+  if (lane == 0) {
+    pbuf[(k & 1) * n + c] = tk * acc;
+    if (leader) {
+      diag[k] = dk;
+      off[k] = beta;
+      tau[k] = tk;
+    }
+  }
 }
 
 // 1''. The last kLdsOrder steps of every matrix in one launch, one workgroup per matrix: the
@@ -1321,7 +1305,16 @@ int eigh_batched(const tt_eigh_problem *problems, int64_t count, double *matrice
     for (int k = 0; k < lds_from; ++k) {
       int grid = 0;
       for (int32_t b = 0; b < batch.count; ++b) grid += step_blocks(batch.p[b].n, k);
-      hipLaunchKernelGGL(tridiag_step_kernel, dim3(grid), dim3(kStepThreads), 0, s, table, matrices, workspace, k);
+      const int64_t rows = largest - k;   // the step's longest vectors: rows k..n-1
+      if (rows <= 4 * 64) {
+        hipLaunchKernelGGL(tridiag_step_kernel<4>, dim3(grid), dim3(kStepThreads), 0, s, table, matrices, workspace, k);
+      } else if (rows <= 8 * 64) {
+        hipLaunchKernelGGL(tridiag_step_kernel<8>, dim3(grid), dim3(kStepThreads), 0, s, table, matrices, workspace, k);
+      } else if (rows <= 16 * 64) {
+        hipLaunchKernelGGL(tridiag_step_kernel<16>, dim3(grid), dim3(kStepThreads), 0, s, table, matrices, workspace, k);
+      } else {
+        hipLaunchKernelGGL(tridiag_step_kernel<32>, dim3(grid), dim3(kStepThreads), 0, s, table, matrices, workspace, k);
+      }
       TT_LAUNCHED();
     }
     hipLaunchKernelGGL(tridiag_lds_kernel, dim3(batch.count), dim3(kTriThreads), 0, s, table, matrices, workspace,

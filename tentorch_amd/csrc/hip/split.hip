@@ -3,7 +3,8 @@
 // ranking reads, and the copies that assemble the factors.  Every launch takes a problem table in
 // device memory (the caller uploads it), so its work does not depend on how many blocks there are.
 //
-// bgemm_kernel: C = alpha op(A) op(B) diag(s) + beta C for each problem, FP64 MFMA
+// bgemm_kernel: C = alpha op(A) op(B) diag(s) + beta C for each problem (or, with row shifts r,
+// C_ij = alpha (op(A) op(B))_ij / (s_j - r_i) + beta C_ij), FP64 MFMA
 // (v_mfma_f64_16x16x4f64).  One workgroup per 64 x 64 tile of one problem; four waves, each a
 // 32 x 32 quarter as 2 x 2 MFMA blocks.  The 16-deep slabs of A and B go through LDS (A as
 // As[depth][row], B as Bs[depth][col]), which is also where a transposed operand is turned:
@@ -114,13 +115,16 @@ __global__ __launch_bounds__(kThreads) void bgemm_kernel(const tt_bgemm_problem 
     for (int bj = 0; bj < 2; ++bj) {
       const int col = c0 + wc + 16 * bj + lr;
       if (col >= p.n) continue;
-      const double scale = p.alpha * (p.col_scale ? p.col_scale[col] : 1.0);
+      const double column = p.col_scale ? p.col_scale[col] : 1.0;
+      const double scale = p.alpha * column;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = r0 + wr + 16 * bi + 4 * r + lk;
         if (row >= p.m) continue;
         double *const out = p.c + int64_t(row) * p.ldc + col;
-        *out = p.beta == 0.0 ? scale * acc[bi][bj][r] : scale * acc[bi][bj][r] + p.beta * *out;
+        const double value = p.row_shift ? p.alpha * acc[bi][bj][r] / (column - p.row_shift[row])
+                                         : scale * acc[bi][bj][r];
+        *out = p.beta == 0.0 ? value : value + p.beta * *out;
       }
     }
 }
@@ -163,6 +167,26 @@ __global__ __launch_bounds__(64 * kRowWaves) void copy_rows_kernel(const tt_rows
   }
 }
 
+// One wave per row: dst = alpha x + beta y + gamma I (the identity's entry on the row's diagonal),
+// y and the identity optional.
+__global__ __launch_bounds__(64 * kRowWaves) void lincomb_rows_kernel(const tt_lincomb_problem *__restrict__ problems,
+                                                                     int count, int64_t rows) {
+  const int64_t row = int64_t(blockIdx.x) * kRowWaves + (threadIdx.x >> 6);
+  if (row >= rows) return;
+  const int lane = threadIdx.x & 63;
+  const tt_lincomb_problem p = problems[find_problem(problems, count, row)];
+  const int64_t local = row - p.first;
+  const double *__restrict__ x = p.x + local * p.x_ld;
+  const double *__restrict__ y = p.y ? p.y + local * p.y_ld : nullptr;
+  double *__restrict__ out = p.dst + local * p.dst_ld;
+  for (int j = lane; j < p.cols; j += 64) {
+    double value = p.alpha * x[j];
+    if (y) value += p.beta * y[j];
+    if (j == local) value += p.gamma;
+    out[j] = value;
+  }
+}
+
 template <typename Problem>
 int check_table(const Problem *problems, int64_t count, const char *what) {
   if (count < 0 || (count > 0 && !problems)) return fail(std::string(what) + ": bad problem table");
@@ -197,6 +221,16 @@ extern "C" int tt_copy_rows(const tt_rows_problem *problems, int64_t count, int6
   if (rows < 0) return fail("tt_copy_rows: negative row count");
   if (count == 0 || rows == 0) return 0;
   hipLaunchKernelGGL(copy_rows_kernel, dim3(static_cast<uint32_t>((rows + kRowWaves - 1) / kRowWaves)),
+                     dim3(64 * kRowWaves), 0, as_stream(stream), problems, static_cast<int>(count), rows);
+  TT_LAUNCHED();
+  return 0;
+}
+
+extern "C" int tt_lincomb_rows(const tt_lincomb_problem *problems, int64_t count, int64_t rows, void *stream) {
+  if (int rc = check_table(problems, count, "tt_lincomb_rows")) return rc;
+  if (rows < 0) return fail("tt_lincomb_rows: negative row count");
+  if (count == 0 || rows == 0) return 0;
+  hipLaunchKernelGGL(lincomb_rows_kernel, dim3(static_cast<uint32_t>((rows + kRowWaves - 1) / kRowWaves)),
                      dim3(64 * kRowWaves), 0, as_stream(stream), problems, static_cast<int>(count), rows);
   TT_LAUNCHED();
   return 0;

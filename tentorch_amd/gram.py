@@ -12,12 +12,14 @@ Per block B (absorb 2: C = B, the left factor isometric; absorb 1: C = B^T), all
   second order, (eps s_0^2 / s_i^2)^2 relative, sound down to about 1e-5 s_0.
 - further passes, where the host's ranking could reach a block's values below what a pass resolves
   (`SvdSettings.resolved`, 1e-7 of the pass's largest value), over the block's tail (its rows below
-  `SvdSettings.reliable`, 1e-5, of the pass's largest value, cut at a wide gap): the tail T is decoupled
-  from the pass's head rows H, whose eigenvectors leaked into it (a Gram pass of T alone would turn
-  that leak into spurious values of about eps s_0^2 / s_h), by the rotation
-  exp([[0, X^T], [-X, 0]]), X = T_P H_P^T diag(1 / |H_P|^2) (|X| ~ 1e-6), applied to second order
-  to [P | Q]; then T is diagonalised by the eigenvectors W of -T_P T_P^T: T <- W T.  T's norm is
-  at most about reliable x the pass's largest, so the pass resolves its values to about eps x that.
+  `SvdSettings.reliable`, 1e-5, of the pass's largest value, cut at the widest gap in the decade
+  above): the tail T is decoupled from the pass's head rows H, whose eigenvectors leaked into it (a
+  Gram pass of T alone would turn that leak into spurious values of about eps s_0^2 / s_h), by the
+  rotation exp([[0, X^T], [-X, 0]]), X_ij = (T_P H_P^T)_ij / (|H_j|^2 - |T_i|^2) (|X| ~ 1e-6),
+  applied to third order to [P | Q]; then T is diagonalised by the eigenvectors W of -T_P T_P^T:
+  T <- W T.  T's norm is at most about 10 x reliable x the pass's largest, so the pass resolves its
+  values to about eps x that.  The head rows keep their values, accurate to (eps s_0^2 / s^2)^2
+  relative, which is below eps s_0 absolute for every s above the boundary.
 - the factors: Q's rows and P's rows, in descending order of their norms, copied into the blocks of
   the two new tensors (`tt_copy_rows`).
 
@@ -44,12 +46,21 @@ def _table(rows, dtype, count_field):
     return table, int(items.sum())
 
 
-def _gemm(a, b, c, m, n, k, lda, ldb, ldc, trans_a=0, trans_b=0, alpha=1.0, beta=0.0, scale=0):
-    return (a, b, c, scale, 0, m, n, k, lda, ldb, ldc, trans_a, trans_b, alpha, beta)
+def _gemm(a, b, c, m, n, k, lda, ldb, ldc, trans_a=0, trans_b=0, alpha=1.0, beta=0.0, scale=0, shift=0):
+    return (a, b, c, scale, shift, 0, m, n, k, lda, ldb, ldc, trans_a, trans_b, alpha, beta)
 
 
 def _rows(src, dst, rows, cols, src_ld, dst_ld, transpose=0, index=0):
     return (src, dst, index, 0, rows, cols, src_ld, dst_ld, transpose, 0)
+
+
+def _lincomb(x, y, dst, rows, cols, x_ld, y_ld, dst_ld, alpha, beta, gamma):
+    return (x, y, dst, 0, rows, cols, x_ld, y_ld, dst_ld, 0, alpha, beta, gamma)
+
+
+# rotations that decouple a tail from its head before the tail's Gram pass: one leaves a coupling
+# of order |X|^2 relative, |X| being about eps x (largest / boundary value)^2 / (1 - gap^-2)
+DECOUPLING = 1
 
 
 class _Launches:
@@ -64,6 +75,12 @@ class _Launches:
         if _native.check_bgemm(table, self.extents) != tiles:
             raise RuntimeError('bgemm: tile count')
         self.steps.append(('gemm', table, tiles))
+
+    def lincomb(self, problems):
+        table, rows = _table(problems, _native.LINCOMB_PROBLEM, 'rows')
+        if _native.check_lincomb(table, self.extents) != rows:
+            raise RuntimeError('lincomb: row count')
+        self.steps.append(('lincomb', table, rows))
 
     def rows(self, kind, problems, source_rows=None):
         table, rows = _table(problems, _native.ROWS_PROBLEM, 'rows')
@@ -85,6 +102,8 @@ class _Launches:
                 _native.bgemm(table, address, items)
             elif kind == 'norms':
                 _native.row_norms(table, address, items)
+            elif kind == 'lincomb':
+                _native.lincomb_rows(table, address, items)
             else:
                 _native.copy_rows(table, address, items)
 
@@ -101,11 +120,15 @@ def _ptr(t: torch.Tensor, offset: int = 0) -> int:
     return t.data_ptr() + 8 * int(offset)
 
 
+# the tail of a pass starts at the widest gap among the values within these factors of the level
+WINDOW = (1.0, 10.0)
+
+
 def _tail_start(values: np.ndarray, first_below: int, level: float) -> int:
     '''Where a pass's tail starts: at the widest gap between consecutive values (rows in descending
-    eigenvalue order) among the rows within a factor 10 of `level`, so that no near-degenerate pair
-    straddles the cut (the decoupling divides by the gap).'''
-    window = np.nonzero((values > 0.1 * level) & (values < 10.0 * level))[0]
+    eigenvalue order) among the rows between WINDOW[0] and WINDOW[1] times `level`, so that no
+    near-degenerate pair straddles the cut (the decoupling divides by the gap).'''
+    window = np.nonzero((values > WINDOW[0] * level) & (values < WINDOW[1] * level))[0]
     if not window.size:
         return first_below
     lo, hi = max(1, int(window[0])), min(values.size - 1, int(window[-1]) + 1)
@@ -115,17 +138,26 @@ def _tail_start(values: np.ndarray, first_below: int, level: float) -> int:
     return lo + int(np.argmax(ratios))
 
 
-def choose_tails(values: List[np.ndarray], first: List[int], floor: float, reliable: float, resolved: float):
-    '''The blocks whose current pass needs another: (block, tail start).  A pass's row norms are
-    trusted down to `resolved` x its largest value (there the second-order error is about
-    (eps / resolved^2)^2 = 1e-4 relative); another pass is needed when the smallest value the ranking
-    can keep, `floor`, lies below twice that.  The tail then holds the rows below about reliable x
-    the largest, cut at a wide gap.'''
+def choose_tails(norms: List[np.ndarray], ranks: List[int], first: List[int], floor: float, reliable: float,
+                 resolved: float):
+    '''The blocks whose current pass needs another: (block, tail start).  `norms[b]` holds the row
+    norms of block b's projection, all g rows (its singular values are the first ranks[b] of them
+    once resolved); the current pass covers rows first[b]...  A pass's row norms are trusted down to
+    `resolved` x its largest value (there the second-order error is about (eps / resolved^2)^2 = 1e-4
+    relative); below that nothing is known of a row but its norm.  Another pass is needed when the
+    smallest value the ranking can keep, `floor`, lies below twice that level and the unresolved rows
+    could hold a value above half the floor: their singular values are at most their Frobenius norm
+    (the coupling to the resolved rows only lowers them, by the Schur complement), so a set whose norm
+    is below half the floor is dropped whatever its exact values.  The tail then holds the rows below
+    about reliable x the largest, cut at a wide gap.'''
     tails = []
-    for b, v in enumerate(values):
-        pass_values = v[first[b]:]
+    for b, v in enumerate(norms):
+        pass_values = v[first[b]:ranks[b]]
         top = float(pass_values.max()) if pass_values.size else 0.0
         if pass_values.size < 2 or top == 0.0 or floor >= 2.0 * resolved * top:
+            continue
+        unresolved = v[first[b]:][v[first[b]:] < 2.0 * resolved * top]
+        if float(np.sqrt(np.sum(unresolved ** 2))) < 0.5 * floor:
             continue
         below = np.nonzero(pass_values < reliable * top)[0]
         if below.size:
@@ -160,8 +192,8 @@ class GramSplit:
         sq_at = np.concatenate([[0], np.cumsum([x * x for x in g])]).astype(np.int64)
         grams = torch.empty(max(int(sq_at[-1]), 1), dtype=torch.float64, device=self.device)
         vectors = torch.empty_like(grams)
-        r_at = np.concatenate([[0], np.cumsum(r)]).astype(np.int64)
-        norms = torch.empty(max(int(r_at[-1]), 1), dtype=torch.float64, device=self.device)
+        g_at = np.concatenate([[0], np.cumsum(g)]).astype(np.int64)
+        norms = torch.empty(max(int(g_at[-1]), 1), dtype=torch.float64, device=self.device)
         extents = _native.Extents().add(self.pq).add(grams).add(vectors).add(norms)
         gram_problems, p_problems, q_copies, norm_rows = [], [], [], []
         for b, block in enumerate(self.blocks):
@@ -180,7 +212,7 @@ class GramSplit:
             p_problems.append(_gemm(_ptr(vectors, sq_at[b]), base, _ptr(self.pq, self.pq_at[b]), gb, nb, gb, gb,
                                     cols_b, ld[b], trans_a=0, trans_b=0 if self.left else 1))
             q_copies.append(_rows(_ptr(vectors, sq_at[b]), _ptr(self.pq, self.pq_at[b] + nb), gb, gb, gb, ld[b]))
-            norm_rows.append(_rows(_ptr(self.pq, self.pq_at[b]), _ptr(norms, r_at[b]), r[b], nb, ld[b], 1))
+            norm_rows.append(_rows(_ptr(self.pq, self.pq_at[b]), _ptr(norms, g_at[b]), gb, nb, ld[b], 1))
         launches = _Launches(extents)
         launches.gemm(gram_problems)
         launches.gemm(p_problems)
@@ -192,84 +224,130 @@ class GramSplit:
         launches.run(1)
         host = norms.cpu().numpy()
         del keep
-        return [host[r_at[b]:r_at[b + 1]].copy() for b in range(len(g))]
+        return [host[g_at[b]:g_at[b + 1]].copy() for b in range(len(g))]
 
     # -- a further pass over the tails
     def _tail_pass(self, tails, values):
-        n, ld, r, g = self.n, self.ld, self.r, self.g
+        '''One more pass over the tails: decoupled from their heads (DECOUPLING rotations), then
+        diagonalised; the tails' new row norms are read back into `values`.'''
         heads = [(b, self.first[b], h) for b, h in tails]
+        tail_rows = None
+        for iteration in range(DECOUPLING):
+            tail_rows = self._decouple(heads, values, last=iteration + 1 == DECOUPLING)
+        self._diagonalise(heads, values, tail_rows)
+        for b, f, h in heads:
+            self.first[b] = h
+
+    def _decouple(self, heads, values, last: bool):
+        '''The rotation exp([[0, X^T], [-X, 0]]) of every (head, tail) pair of rows of [P | Q], with
+        X_ij = (T_P H_P^T)_ij / (|H_j|^2 - |T_i|^2) (the rotation's generator to first order: with
+        H_P H_P^T and T_P T_P^T diagonal, it solves X H_P H_P^T - T_P T_P^T X = T_P H_P^T, so the
+        rotated tail is orthogonal to the rotated head), through the third order: with K = X^T X
+        and U = X^T T, head <- (1 - K/2) H + (1 - K/6) U and tail <- T - X ((1 - K/6) H + U/2),
+        orthogonal to |X|^4 / 24 (|X| is below about 1e-5 at a tail boundary chosen at a gap).  The
+        heads go back in place; the tails too, unless `last`, when they stay in the returned buffer
+        for the diagonalisation.'''
+        n, ld, g = self.n, self.ld, self.g
         hh = [h - f for _, f, h in heads]
         t = [g[b] - h for b, _, h in heads]
+        w = [ld[b] for b, _, _ in heads]
+
         def offsets(sizes):
             return np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
-        x_at, xtx_at, sq_at = offsets([a * c for a, c in zip(t, hh)]), offsets([a * a for a in hh]), offsets([a * a for a in t])
-        hn_at = offsets([a * ld[b] for a, (b, _, _) in zip(hh, heads)])
-        tn_at = offsets([a * ld[b] for a, (b, _, _) in zip(t, heads)])
-        inv_at = offsets(hh)
-        nr = [max(r[b] - h, 0) for b, _, h in heads]
-        nr_at = offsets(nr)
+
+        x_at, sq_at = offsets([a * c for a, c in zip(t, hh)]), offsets([a * a for a in hh])
+        hw_at, tw_at = offsets([a * c for a, c in zip(hh, w)]), offsets([a * c for a, c in zip(t, w)])
+        head_at, tail_at = offsets(hh), offsets(t)
+
         def buffer(size):
             return torch.empty(max(int(size), 1), dtype=torch.float64, device=self.device)
-        x, xtx, xxt, hn, tn = buffer(x_at[-1]), buffer(xtx_at[-1]), buffer(sq_at[-1]), buffer(hn_at[-1]), buffer(tn_at[-1])
-        grams, vectors, norms = buffer(sq_at[-1]), buffer(sq_at[-1]), buffer(nr_at[-1])
-        inverse = np.concatenate([1.0 / values[b][f:h] ** 2 for b, f, h in heads])
-        scale = _uploaded(inverse, self.device)
-        extents = _native.Extents().add(self.pq).add(scale)
-        for buf in (x, xtx, xxt, hn, tn, grams, vectors, norms):
+
+        x, u, hn, m, tn = buffer(x_at[-1]), buffer(hw_at[-1]), buffer(hw_at[-1]), buffer(hw_at[-1]), buffer(tw_at[-1])
+        k_mat, a_mat, b_mat = (buffer(sq_at[-1]) for _ in range(3))
+        scale = _uploaded(np.concatenate([values[b][f:h] ** 2 for b, f, h in heads]), self.device)
+        shift = _uploaded(np.concatenate([values[b][h:] ** 2 for b, f, h in heads]), self.device)
+        extents = _native.Extents().add(self.pq).add(scale).add(shift)
+        for buf in (x, u, hn, m, tn, k_mat, a_mat, b_mat):
             extents.add(buf)
-        steps = {k: [] for k in ('x', 'xx', 'copy', 'acc1', 'acc2', 'gram', 'rotate', 'back', 'norms')}
+        steps = {k: [] for k in ('x', 'k', 'init', 'first', 'second', 'tail', 'back')}
         for i, (b, f, h) in enumerate(heads):
             base = self.pq_at[b]
             H, T = _ptr(self.pq, base + f * ld[b]), _ptr(self.pq, base + h * ld[b])
-            X, Hn, Tn = _ptr(x, x_at[i]), _ptr(hn, hn_at[i]), _ptr(tn, tn_at[i])
-            # X = T_P H_P^T diag(1 / |H_P|^2)
-            steps['x'].append(_gemm(T, H, X, t[i], hh[i], n[b], ld[b], ld[b], hh[i], trans_b=1,
-                                    scale=_ptr(scale, inv_at[i])))
-            steps['xx'].append(_gemm(X, X, _ptr(xtx, xtx_at[i]), hh[i], hh[i], t[i], hh[i], hh[i], hh[i], trans_a=1))
-            steps['xx'].append(_gemm(X, X, _ptr(xxt, sq_at[i]), t[i], t[i], hh[i], hh[i], hh[i], t[i], trans_b=1))
-            steps['copy'].append(_rows(H, Hn, hh[i], ld[b], ld[b], ld[b]))
-            steps['copy'].append(_rows(T, Tn, t[i], ld[b], ld[b], ld[b]))
-            # Hn = H + X^T T - X^T X H / 2 and Tn = T - X H - X X^T T / 2, both rows of [P | Q]
-            steps['acc1'].append(_gemm(X, T, Hn, hh[i], ld[b], t[i], hh[i], ld[b], ld[b], trans_a=1, beta=1.0))
-            steps['acc1'].append(_gemm(X, H, Tn, t[i], ld[b], hh[i], hh[i], ld[b], ld[b], alpha=-1.0, beta=1.0))
-            steps['acc2'].append(_gemm(_ptr(xtx, xtx_at[i]), H, Hn, hh[i], ld[b], hh[i], hh[i], ld[b], ld[b],
-                                       alpha=-0.5, beta=1.0))
-            steps['acc2'].append(_gemm(_ptr(xxt, sq_at[i]), T, Tn, t[i], ld[b], t[i], t[i], ld[b], ld[b],
-                                       alpha=-0.5, beta=1.0))
-            # -Tn_P Tn_P^T, diagonalised: T = W Tn; the head goes back in place
-            steps['gram'].append(_gemm(Tn, Tn, _ptr(grams, sq_at[i]), t[i], t[i], n[b], ld[b], ld[b], t[i],
-                                       trans_b=1, alpha=-1.0))
-            steps['rotate'].append(_gemm(_ptr(vectors, sq_at[i]), Tn, T, t[i], ld[b], t[i], t[i], ld[b], ld[b]))
-            steps['back'].append(_rows(Hn, H, hh[i], ld[b], ld[b], ld[b]))
-            steps['norms'].append(_rows(T, _ptr(norms, nr_at[i]), nr[i], n[b], ld[b], 1))
+            X, U, Hn, M, Tn = (_ptr(x, x_at[i]), _ptr(u, hw_at[i]), _ptr(hn, hw_at[i]), _ptr(m, hw_at[i]),
+                               _ptr(tn, tw_at[i]))
+            K, A, B = (_ptr(buf, sq_at[i]) for buf in (k_mat, a_mat, b_mat))
+            e, q = hh[i], t[i]
+            steps['x'].append(_gemm(T, H, X, q, e, n[b], ld[b], ld[b], e, trans_b=1, scale=_ptr(scale, head_at[i]),
+                                    shift=_ptr(shift, tail_at[i])))
+            steps['k'].append(_gemm(X, X, K, e, e, q, e, e, e, trans_a=1))
+            steps['k'].append(_gemm(X, T, U, e, w[i], q, e, ld[b], w[i], trans_a=1))
+            steps['init'].append(_lincomb(K, 0, A, e, e, e, 0, e, -0.5, 0.0, 1.0))
+            steps['init'].append(_lincomb(K, 0, B, e, e, e, 0, e, -1.0 / 6.0, 0.0, 1.0))
+            steps['init'].append(_lincomb(U, 0, M, e, w[i], w[i], 0, w[i], 0.5, 0.0, 0.0))
+            steps['init'].append(_lincomb(T, 0, Tn, q, w[i], ld[b], 0, w[i], 1.0, 0.0, 0.0))
+            steps['first'].append(_gemm(A, H, Hn, e, w[i], e, e, ld[b], w[i]))
+            steps['first'].append(_gemm(B, H, M, e, w[i], e, e, ld[b], w[i], beta=1.0))
+            steps['second'].append(_gemm(B, U, Hn, e, w[i], e, e, w[i], w[i], beta=1.0))
+            steps['tail'].append(_gemm(X, M, Tn, q, w[i], e, e, w[i], w[i], alpha=-1.0, beta=1.0))
+            steps['back'].append(_lincomb(Hn, 0, H, e, w[i], w[i], 0, ld[b], 1.0, 0.0, 0.0))
+            if not last:
+                steps['back'].append(_lincomb(Tn, 0, T, q, w[i], w[i], 0, ld[b], 1.0, 0.0, 0.0))
         launches = _Launches(extents)
-        launches.gemm(steps['x'])          # 0
-        launches.gemm(steps['xx'])         # 1
-        launches.rows('copy', steps['copy'])   # 2
-        launches.gemm(steps['acc1'])       # 3
-        launches.gemm(steps['acc2'])       # 4
-        launches.gemm(steps['gram'])       # 5
-        launches.gemm(steps['rotate'])     # 6
-        launches.rows('copy', steps['back'])   # 7
-        launches.rows('norms', steps['norms'])  # 8
+        for kind in ('x', 'k'):
+            launches.gemm(steps[kind])
+        launches.lincomb(steps['init'])
+        for kind in ('first', 'second', 'tail'):
+            launches.gemm(steps[kind])
+        launches.lincomb(steps['back'])
         keep = launches.upload()
-        launches.run(0, 6)
+        launches.run()
+        del keep
+        self._keep = (scale, shift)
+        return (tn, tw_at) if last else None
+
+    def _diagonalise(self, heads, values, tail_rows):
+        '''Each decoupled tail (in `tail_rows`, t x w per block) diagonalised: W, the eigenvectors of
+        -T_P T_P^T as rows, gives the block's tail rows W T; their norms come back to the host.'''
+        n, ld, g = self.n, self.ld, self.g
+        tn, tw_at = tail_rows
+        t = [g[b] - h for b, _, h in heads]
+        sq_at = np.concatenate([[0], np.cumsum([a * a for a in t])]).astype(np.int64)
+        nr_at = np.concatenate([[0], np.cumsum(t)]).astype(np.int64)
+        grams = torch.empty(max(int(sq_at[-1]), 1), dtype=torch.float64, device=self.device)
+        vectors, norms = torch.empty_like(grams), torch.empty(max(int(nr_at[-1]), 1), dtype=torch.float64,
+                                                              device=self.device)
+        extents = _native.Extents().add(self.pq).add(tn).add(grams).add(vectors).add(norms)
+        gram_steps, rotate_steps, norm_steps = [], [], []
+        for i, (b, f, h) in enumerate(heads):
+            T, Tn, q = _ptr(self.pq, self.pq_at[b] + h * ld[b]), _ptr(tn, tw_at[i]), t[i]
+            gram_steps.append(_gemm(Tn, Tn, _ptr(grams, sq_at[i]), q, q, n[b], ld[b], ld[b], q, trans_b=1,
+                                    alpha=-1.0))
+            rotate_steps.append(_gemm(_ptr(vectors, sq_at[i]), Tn, T, q, ld[b], q, q, ld[b], ld[b]))
+            norm_steps.append(_rows(T, _ptr(norms, nr_at[i]), q, n[b], ld[b], 1))
+        launches = _Launches(extents)
+        launches.gemm(gram_steps)
+        launches.gemm(rotate_steps)
+        launches.rows('norms', norm_steps)
+        keep = launches.upload()
+        launches.run(0, 1)
         _native.eigh_batched(grams, t, sq_at[:-1].tolist(), vectors=vectors)
-        launches.run(6)
+        launches.run(1)
         host = norms.cpu().numpy()
-        del keep, scale
+        del keep
+        self._keep = None
         for i, (b, f, h) in enumerate(heads):
             values[b][h:] = host[nr_at[i]:nr_at[i + 1]]
-            self.first[b] = h
 
     def factor(self, cut: Callable, reliable: float, resolved: float) -> List[np.ndarray]:
-        values = self._first_pass()
+        norms = self._first_pass()          # every row of every block; the first r are the values
         self.first = [0] * len(self.blocks)
         while True:
-            tails = choose_tails(values, self.first, cut(values), reliable, resolved)
+            tails = choose_tails(norms, self.r, self.first, cut([v[:r] for v, r in zip(norms, self.r)]), reliable,
+                                 resolved)
             if not tails:
                 break
-            self._tail_pass(tails, values)
+            self._tail_pass(tails, norms)
+        values = [v[:r] for v, r in zip(norms, self.r)]
         self.order = [np.argsort(-v, kind='stable').astype(np.int32) for v in values]
         return [v[o] for v, o in zip(values, self.order)]
 

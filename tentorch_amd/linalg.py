@@ -20,7 +20,7 @@ import torch
 
 from . import _native
 from ._native import engine
-from .gram import GramSplit, choose_tails
+from .gram import DECOUPLING, GramSplit, choose_tails
 from .tensor import Tensor, inner, inner_device, norm
 
 
@@ -88,12 +88,16 @@ def _row_norms(matrices) -> torch.Tensor:
 
 
 def _rotated(x: torch.Tensor, head: torch.Tensor, tail: torch.Tensor):
-    '''(head, tail) rows under the rotation exp([[0, X^T], [-X, 0]]) to second order:
-    head' = (1 - X^T X / 2) head + X^T tail, tail' = -X head + (1 - X X^T / 2) tail; the rotation
-    is orthogonal up to |X|^3.'''
+    '''(head, tail) rows under the rotation exp([[0, X^T], [-X, 0]]) through the third order, as
+    `tentorch_amd.gram` applies it: with K = X^T X and U = X^T tail,
+    head' = (1 - K/2) head + (1 - K/6) U and tail' = tail - X ((1 - K/6) head + U/2).'''
     xt = x.transpose(0, 1)
-    new_head = head + torch.matmul(xt, tail) - 0.5 * torch.matmul(torch.matmul(xt, x), head)
-    new_tail = tail - torch.matmul(x, head) - 0.5 * torch.matmul(torch.matmul(x, xt), tail)
+    k = torch.matmul(xt, x)
+    eye = torch.eye(k.shape[0], dtype=k.dtype, device=k.device)
+    a, b = eye - k / 2, eye - k / 6
+    u = torch.matmul(xt, tail)
+    new_head = torch.matmul(a, head) + torch.matmul(b, u)
+    new_tail = tail - torch.matmul(x, torch.matmul(b, head) + u / 2)
     return new_head, new_tail
 
 
@@ -110,31 +114,36 @@ def _gram_split(blocks, left_isometric: bool, cut):
     ranks = [min(c.shape) for c in sides]
     norms = _row_norms(p).cpu().numpy()
     starts = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
-    values = [norms[starts[b]:starts[b] + ranks[b]].copy() for b in range(len(sides))]
+    values = [norms[starts[b]:starts[b + 1]].copy() for b in range(len(sides))]    # all g rows
     first = [0] * len(sides)          # the first row of every block's current pass
     while True:
-        tails = choose_tails(values, first, cut(values), SvdSettings.reliable, SvdSettings.resolved)
+        tails = choose_tails(values, ranks, first, cut([v[:r] for v, r in zip(values, ranks)]),
+                             SvdSettings.reliable, SvdSettings.resolved)
         if not tails:
             break
         for b, h in tails:
-            head, tail = p[b][first[b]:h], p[b][h:]
-            x = torch.matmul(tail, head.transpose(0, 1)) / (head * head).sum(dim=1)
-            p[b] = torch.cat([p[b][:first[b]], *_rotated(x, head, tail)])
-            q[b] = torch.cat([q[b][:first[b]], *_rotated(x, q[b][first[b]:h], q[b][h:])])
+            gaps = values[b][first[b]:h][None, :] ** 2 - values[b][h:][:, None] ** 2
+            scale = torch.as_tensor(1.0 / gaps, device=p[b].device)
+            for _ in range(DECOUPLING):
+                head, tail = p[b][first[b]:h], p[b][h:]
+                x = torch.matmul(tail, head.transpose(0, 1)) * scale
+                p[b] = torch.cat([p[b][:first[b]], *_rotated(x, head, tail)])
+                q[b] = torch.cat([q[b][:first[b]], *_rotated(x, q[b][first[b]:h], q[b][h:])])
         tail_p = [p[b][h:] for b, h in tails]
         packed, t_sizes, t_offsets = _negated_grams(tail_p)
         for (b, h), w in zip(tails, _eigenrows(packed, t_sizes, t_offsets)):
             p[b] = torch.cat([p[b][:h], torch.matmul(w, p[b][h:])])
             q[b] = torch.cat([q[b][:h], torch.matmul(w, q[b][h:])])
-        tail_norms = _row_norms([p[b][h:ranks[b]] for b, h in tails]).cpu().numpy()
+        tail_norms = _row_norms([p[b][h:] for b, h in tails]).cpu().numpy()
         at = 0
         for b, h in tails:
-            n = ranks[b] - h
+            n = values[b].size - h
             values[b][h:] = tail_norms[at:at + n]
             at += n
             first[b] = h
     out = []
     for b, v in enumerate(values):
+        v = v[:ranks[b]]
         qb, pb = q[b][:ranks[b]], p[b][:ranks[b]]
         order = np.argsort(-v, kind='stable')
         if np.any(order != np.arange(v.size)):

@@ -1,7 +1,7 @@
 '''
-The split's device launches interpreted on the CPU: `tt_bgemm`, `tt_row_norms`, `tt_copy_rows` and
-the batched eigensolver, run by numpy on the host buffers their problem tables address.  With the
-tables pointing at CPU tensors, `tentorch_amd.gram.GramSplit` runs end to end without a GPU, so its
+The split's device launches interpreted on the CPU: `tt_bgemm`, `tt_row_norms`, `tt_copy_rows`,
+`tt_lincomb_rows` and the batched eigensolver, run by numpy on the host buffers their problem tables
+address.  With the tables pointing at CPU tensors, `tentorch_amd.gram.GramSplit` runs end to end without a GPU, so its
 table construction, its passes and its bounds checks are tested on the CPU; the kernels themselves
 are tested on the GPU against this reading of the header's semantics.
 '''
@@ -53,7 +53,8 @@ def _find(address, keep):
 @contextlib.contextmanager
 def emulated_split():
     tables = _Tables()
-    saved = (_native.upload, _native.bgemm, _native.row_norms, _native.copy_rows, _native.eigh_batched)
+    names = ('upload', 'bgemm', 'row_norms', 'copy_rows', 'lincomb_rows', 'eigh_batched')
+    saved = {name: getattr(_native, name) for name in names}
 
     def bgemm(table, address, tiles):
         table = _find(address, tables.keep)
@@ -64,7 +65,9 @@ def emulated_split():
             a = _matrix(int(p['a']), *((k, m) if p['trans_a'] else (m, k)), int(p['lda'])) if k else np.zeros((m, 0))
             b = _matrix(int(p['b']), *((n, k) if p['trans_b'] else (k, n)), int(p['ldb'])) if k else np.zeros((0, n))
             product = (a.T if p['trans_a'] else a) @ (b.T if p['trans_b'] else b)
-            if int(p['col_scale']):
+            if int(p['row_shift']):
+                product = product / (_doubles(int(p['col_scale']), n)[None, :] - _doubles(int(p['row_shift']), m)[:, None])
+            elif int(p['col_scale']):
                 product = product * _doubles(int(p['col_scale']), n)
             c = _matrix(int(p['c']), m, n, int(p['ldc']))
             c[...] = p['alpha'] * product + (p['beta'] * c if p['beta'] != 0.0 else 0.0)
@@ -88,6 +91,17 @@ def emulated_split():
             else:
                 _matrix(int(p['dst']), r, c, int(p['dst_ld']))[...] = source
 
+    def lincomb_rows(table, address, rows):
+        for p in _find(address, tables.keep):
+            r, c = int(p['rows']), int(p['cols'])
+            if not (r and c):
+                continue
+            value = p['alpha'] * _matrix(int(p['x']), r, c, int(p['x_ld']))
+            if int(p['y']):
+                value = value + p['beta'] * _matrix(int(p['y']), r, c, int(p['y_ld']))
+            value = value + p['gamma'] * np.eye(r, c)
+            _matrix(int(p['dst']), r, c, int(p['dst_ld']))[...] = value
+
     def eigh_batched(matrices, sizes, offsets=None, work=None, vectors=None):
         values = torch.empty(max(sum(sizes), 1), dtype=torch.float64)
         vectors = torch.empty_like(matrices) if vectors is None else vectors
@@ -99,9 +113,12 @@ def emulated_split():
             at += n
         return values[:at], vectors
 
-    _native.upload, _native.bgemm, _native.row_norms, _native.copy_rows, _native.eigh_batched = (
-        tables.upload, bgemm, row_norms, copy_rows, eigh_batched)
+    replacements = dict(upload=tables.upload, bgemm=bgemm, row_norms=row_norms, copy_rows=copy_rows,
+                        lincomb_rows=lincomb_rows, eigh_batched=eigh_batched)
+    for name, function in replacements.items():
+        setattr(_native, name, function)
     try:
         yield
     finally:
-        (_native.upload, _native.bgemm, _native.row_norms, _native.copy_rows, _native.eigh_batched) = saved
+        for name, function in saved.items():
+            setattr(_native, name, function)

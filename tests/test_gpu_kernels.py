@@ -428,7 +428,8 @@ def test_batched_eigh_matches_numpy(largest):
 
 def test_batched_gemm_with_transposes_matches_numpy():
     '''tt_bgemm over one table mixing every transpose pair, ragged sizes (1 to 150, not multiples of
-    the 64 x 64 tile or the 16-deep slab), an empty depth, column scales, alpha and beta.'''
+    the 64 x 64 tile or the 16-deep slab), an empty depth, column scales, the division by column
+    minus row shifts, alpha and beta.'''
     from tentorch_amd import _native
     from tentorch_amd.gram import _Launches, _gemm
     rng = np.random.default_rng(51)
@@ -440,18 +441,27 @@ def test_batched_gemm_with_transposes_matches_numpy():
         a = rng.standard_normal((k, m) if ta else (m, k))
         b = rng.standard_normal((n, k) if tb else (k, n))
         c = rng.standard_normal((m, n))
-        scale = rng.standard_normal(n) if i % 3 == 0 else None
+        scale = rng.standard_normal(n) + 3.0 if i % 3 != 2 else None
+        shift = rng.standard_normal(m) - 3.0 if i % 3 == 1 else None
         alpha, beta = 0.5 + i, (0.0 if i % 2 else -1.5)
-        ref = alpha * ((a.T if ta else a) @ (b.T if tb else b)) * (scale if scale is not None else 1.0) + beta * c
+        product = (a.T if ta else a) @ (b.T if tb else b)
+        if shift is not None:
+            product = product / (scale[None, :] - shift[:, None])
+        elif scale is not None:
+            product = product * scale
+        ref = alpha * product + beta * c
         da, db, dc = (torch.tensor(x, device='cuda').contiguous() for x in (a, b, c))
         ds = torch.tensor(scale, device='cuda') if scale is not None else None
-        for t in (da, db, dc) + ((ds,) if ds is not None else ()):
-            extents.add(t)
+        dr = torch.tensor(shift, device='cuda') if shift is not None else None
+        for t in (da, db, dc, ds, dr):
+            if t is not None:
+                extents.add(t)
         cases.append((dc, ref))
-        buffers.append((da, db, dc, ds))
+        buffers.append((da, db, dc, ds, dr))
         lda, ldb = (m if ta else k), (k if tb else n)
         buffers[-1] += (_gemm(da.data_ptr(), db.data_ptr(), dc.data_ptr(), m, n, k, max(lda, 1), max(ldb, 1), n,
-                              ta, tb, alpha, beta, ds.data_ptr() if ds is not None else 0),)
+                              ta, tb, alpha, beta, ds.data_ptr() if ds is not None else 0,
+                              dr.data_ptr() if dr is not None else 0),)
     launches = _Launches(extents)
     launches.gemm([entry[-1] for entry in buffers])
     keep = launches.upload()
