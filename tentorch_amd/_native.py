@@ -291,98 +291,133 @@ assert BGEMM_PROBLEM.itemsize == 96 and ROWS_PROBLEM.itemsize == 56 and LINCOMB_
 
 class Extents:
     '''Device buffers a batch of problems may touch, as (first address, one past the last) byte
-    ranges: every operand a table names is checked to lie inside one of them before the launch.'''
+    ranges: every operand a table names is checked to lie inside one of them before the launch.
+    Ranges that overlap are views of one allocation and are checked as their union.'''
 
     def __init__(self):
         self.ranges = []
+        self._spans = None
 
     def add(self, t: torch.Tensor) -> 'Extents':
         start = t.data_ptr()
         self.ranges.append((start, start + t.numel() * t.element_size()))
+        self._spans = None
         return self
 
+    def spans(self):
+        '''The ranges sorted by start, overlapping ones merged: (starts, ends) as int64 arrays.'''
+        if self._spans is None:
+            merged = []
+            for start, end in sorted(self.ranges):
+                if merged and start < merged[-1][1]:
+                    merged[-1][1] = max(merged[-1][1], end)
+                else:
+                    merged.append([start, end])
+            spans = np.array(merged, dtype=np.int64).reshape(-1, 2)
+            self._spans = (spans[:, 0].copy(), spans[:, 1].copy())
+        return self._spans
+
+    def check_many(self, pointers, last, mask, what: str) -> None:
+        '''Bytes 0..last[i] (inclusive) from pointers[i] lie in one known range, for every i where
+        mask[i] holds.'''
+        pointers = np.asarray(pointers, dtype=np.int64)[mask]
+        if not pointers.size:
+            return
+        hi = pointers + np.asarray(last, dtype=np.int64)[mask]
+        starts, ends = self.spans()
+        at = np.searchsorted(starts, pointers, side='right') - 1
+        inside = (at >= 0) & (hi < ends[np.maximum(at, 0)])
+        if not inside.all():
+            bad = int(np.argmin(inside))
+            raise RuntimeError(f'{what}: bytes {int(pointers[bad]):#x}..{int(hi[bad]):#x} lie outside every '
+                               'buffer of the launch')
+
     def check(self, pointer: int, first: int, last: int, what: str) -> None:
-        '''Elements first..last (inclusive, FP64 or int32 by the caller's scaling) of an operand at
-        `pointer` lie in one known buffer.'''
-        lo, hi = pointer + first, pointer + last
-        for start, end in self.ranges:
-            if start <= lo and hi < end:
-                return
-        raise RuntimeError(f'{what}: bytes {lo:#x}..{hi:#x} lie outside every buffer of the launch')
+        '''Bytes first..last (inclusive) of an operand at `pointer` lie in one known buffer.'''
+        self.check_many([pointer + first], [last - first], [True], what)
 
 
-def _rows_touched(rows, cols, ld):
-    return 0, 8 * ((rows - 1) * ld + cols - 1) if rows > 0 and cols > 0 else 0
+def _touched(rows, cols, ld):
+    '''The last byte offset a row-major rows x cols operand with row stride ld touches (0 when empty).'''
+    return np.where((rows > 0) & (cols > 0), 8 * ((rows - 1) * ld + cols - 1), 0)
+
+
+def _fields(table, *names):
+    return [table[name].astype(np.int64) for name in names]
+
+
+def _numbered(first, items, what):
+    if np.any(first != np.concatenate([[0], np.cumsum(items)[:-1]])):
+        raise RuntimeError(f'{what}: work items are not numbered consecutively')
 
 
 def check_bgemm(table: np.ndarray, extents: Extents) -> int:
     '''Validates a bgemm table against the buffers it names; returns its tile count.'''
-    tiles = 0
-    for p in table:
-        m, n, k = int(p['m']), int(p['n']), int(p['k'])
-        if m < 0 or n < 0 or k < 0 or int(p['first']) != tiles:
-            raise RuntimeError('bgemm: bad problem sizes or tile numbering')
-        a_rows, a_cols = (k, m) if p['trans_a'] else (m, k)
-        b_rows, b_cols = (n, k) if p['trans_b'] else (k, n)
-        if a_cols > int(p['lda']) or b_cols > int(p['ldb']) or n > int(p['ldc']):
-            raise RuntimeError('bgemm: a row stride is shorter than its row')
-        if m and k:
-            extents.check(int(p['a']), *_rows_touched(a_rows, a_cols, int(p['lda'])), 'bgemm A')
-        if n and k:
-            extents.check(int(p['b']), *_rows_touched(b_rows, b_cols, int(p['ldb'])), 'bgemm B')
-        if m and n:
-            extents.check(int(p['c']), *_rows_touched(m, n, int(p['ldc'])), 'bgemm C')
-            if int(p['col_scale']):
-                extents.check(int(p['col_scale']), 0, 8 * (n - 1), 'bgemm column scale')
-            if int(p['row_shift']):
-                if not int(p['col_scale']):
-                    raise RuntimeError('bgemm: row shifts need column shifts')
-                extents.check(int(p['row_shift']), 0, 8 * (m - 1), 'bgemm row shift')
-        tiles += -(-m // 64) * -(-n // 64)
-    return tiles
+    if not table.size:
+        return 0
+    m, n, k, lda, ldb, ldc, first = _fields(table, 'm', 'n', 'k', 'lda', 'ldb', 'ldc', 'first')
+    a, b, c, scale, shift = _fields(table, 'a', 'b', 'c', 'col_scale', 'row_shift')
+    if np.any((m < 0) | (n < 0) | (k < 0)):
+        raise RuntimeError('bgemm: negative problem sizes')
+    tiles = (-(-m // 64)) * (-(-n // 64))
+    _numbered(first, tiles, 'bgemm')
+    ta, tb = table['trans_a'] != 0, table['trans_b'] != 0
+    a_rows, a_cols = np.where(ta, k, m), np.where(ta, m, k)
+    b_rows, b_cols = np.where(tb, n, k), np.where(tb, k, n)
+    if np.any((a_cols > lda) | (b_cols > ldb) | (n > ldc)):
+        raise RuntimeError('bgemm: a row stride is shorter than its row')
+    out = (m > 0) & (n > 0)
+    extents.check_many(a, _touched(a_rows, a_cols, lda), (m > 0) & (k > 0), 'bgemm A')
+    extents.check_many(b, _touched(b_rows, b_cols, ldb), (n > 0) & (k > 0), 'bgemm B')
+    extents.check_many(c, _touched(m, n, ldc), out, 'bgemm C')
+    extents.check_many(scale, 8 * (n - 1), out & (scale != 0), 'bgemm column scale')
+    if np.any(out & (shift != 0) & (scale == 0)):
+        raise RuntimeError('bgemm: row shifts need column shifts')
+    extents.check_many(shift, 8 * (m - 1), out & (shift != 0), 'bgemm row shift')
+    return int(tiles.sum())
 
 
 def check_rows(table: np.ndarray, extents: Extents, source_rows=None, vector_dst: bool = False) -> int:
     '''Validates a row-problem table; returns its row count.  With an index, `source_rows[i]` is the
     number of rows problem i's source holds, and the caller has checked that every index is below
     it.  `vector_dst`: dst receives one value per row (the norms).'''
-    rows = 0
-    for i, p in enumerate(table):
-        r, c = int(p['rows']), int(p['cols'])
-        if r < 0 or c < 0 or int(p['first']) != rows or c > int(p['src_ld']):
-            raise RuntimeError('row problem: bad sizes or row numbering')
-        if r and c:
-            held = r
-            if int(p['index']):
-                extents.check(int(p['index']), 0, 4 * (r - 1), 'row index')
-                held = int(source_rows[i])
-            extents.check(int(p['src']), *_rows_touched(held, c, int(p['src_ld'])), 'row source')
-            if vector_dst:
-                extents.check(int(p['dst']), 0, 8 * (r - 1), 'row destination')
-            elif p['transpose']:
-                extents.check(int(p['dst']), *_rows_touched(c, r, int(p['dst_ld'])), 'row destination')
-            else:
-                extents.check(int(p['dst']), *_rows_touched(r, c, int(p['dst_ld'])), 'row destination')
-        rows += r
-    return rows
+    if not table.size:
+        return 0
+    r, c, src_ld, dst_ld, first = _fields(table, 'rows', 'cols', 'src_ld', 'dst_ld', 'first')
+    src, dst, index = _fields(table, 'src', 'dst', 'index')
+    if np.any((r < 0) | (c < 0) | (c > src_ld)):
+        raise RuntimeError('row problem: bad sizes')
+    _numbered(first, r, 'row problem')
+    live = (r > 0) & (c > 0)
+    indexed = index != 0
+    if np.any(live & indexed) and source_rows is None:
+        raise RuntimeError('row problem: an index needs the source row counts')
+    held = np.where(indexed, np.asarray(source_rows if source_rows is not None else r, dtype=np.int64), r)
+    extents.check_many(index, 4 * (r - 1), live & indexed, 'row index')
+    extents.check_many(src, _touched(held, c, src_ld), live, 'row source')
+    if vector_dst:
+        extents.check_many(dst, 8 * (r - 1), live, 'row destination')
+    else:
+        turned = table['transpose'] != 0
+        last = np.where(turned, _touched(c, r, dst_ld), _touched(r, c, dst_ld))
+        extents.check_many(dst, last, live, 'row destination')
+    return int(r.sum())
 
 
 def check_lincomb(table: np.ndarray, extents: Extents) -> int:
     '''Validates a lincomb table; returns its row count.'''
-    rows = 0
-    for p in table:
-        r, c = int(p['rows']), int(p['cols'])
-        if r < 0 or c < 0 or int(p['first']) != rows or c > int(p['x_ld']) or c > int(p['dst_ld']):
-            raise RuntimeError('lincomb: bad sizes or row numbering')
-        if r and c:
-            extents.check(int(p['x']), *_rows_touched(r, c, int(p['x_ld'])), 'lincomb x')
-            if int(p['y']):
-                if c > int(p['y_ld']):
-                    raise RuntimeError('lincomb: y row stride shorter than its row')
-                extents.check(int(p['y']), *_rows_touched(r, c, int(p['y_ld'])), 'lincomb y')
-            extents.check(int(p['dst']), *_rows_touched(r, c, int(p['dst_ld'])), 'lincomb destination')
-        rows += r
-    return rows
+    if not table.size:
+        return 0
+    r, c, x_ld, y_ld, dst_ld, first = _fields(table, 'rows', 'cols', 'x_ld', 'y_ld', 'dst_ld', 'first')
+    x, y, dst = _fields(table, 'x', 'y', 'dst')
+    if np.any((r < 0) | (c < 0) | (c > x_ld) | (c > dst_ld) | ((y != 0) & (c > y_ld))):
+        raise RuntimeError('lincomb: bad sizes or row strides')
+    _numbered(first, r, 'lincomb')
+    live = (r > 0) & (c > 0)
+    extents.check_many(x, _touched(r, c, x_ld), live, 'lincomb x')
+    extents.check_many(y, _touched(r, c, y_ld), live & (y != 0), 'lincomb y')
+    extents.check_many(dst, _touched(r, c, dst_ld), live, 'lincomb destination')
+    return int(r.sum())
 
 
 def lincomb_rows(table: np.ndarray, address: int, rows: int) -> None:

@@ -5,8 +5,8 @@
 //   1. Householder tridiagonalisation.  The rank-2 update of step k is deferred into step k+1's
 //      symmetric matrix-vector pass, so each step reads and writes the trailing matrix once.  The
 //      reflectors stay in the matrix's lower part.  While a trailing matrix exceeds 128 rows, one
-//      launch per step spreads every matrix over one workgroup per 16 trailing columns; the last
-//      128 steps of every matrix run in one launch, one workgroup per matrix, in LDS.
+//      launch per step spreads every matrix over one wave per trailing column; the last 128 steps
+//      of every matrix run in one launch, one workgroup per matrix, in LDS.
 //   2. Cuppen divide and conquer on the tridiagonal: leaves of 8 rows by implicit QL (one wave
 //      each), then merges bottom up, four launches per level for every matrix of the batch: one
 //      workgroup per merge deflates small coupling components and close poles (Givens rotations);

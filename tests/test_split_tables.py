@@ -1,0 +1,91 @@
+'''
+The host checks every batched split launch passes before it reaches the device
+(`tentorch_amd._native.check_bgemm`, `check_rows`, `check_lincomb`): a table whose operands reach
+one element past their buffer, whose work items are misnumbered or whose row strides are shorter
+than their rows is refused, and an exact fit is accepted with the right item count.  CPU tensors
+stand in for device buffers (only their addresses are read).
+'''
+
+import numpy as np
+import pytest
+import torch
+
+from tentorch_amd import _native
+from tentorch_amd.gram import _gemm, _lincomb, _rows, _table
+
+
+def _buffers(*sizes):
+    tensors = [torch.zeros(n, dtype=torch.float64) for n in sizes]
+    extents = _native.Extents()
+    for t in tensors:
+        extents.add(t)
+    return tensors, extents
+
+
+def _bgemm(problems):
+    return _table(problems, _native.BGEMM_PROBLEM, 'tiles')[0]
+
+
+def test_bgemm_table_exact_fit_and_one_past():
+    (a, b, c), extents = _buffers(70 * 33, 33 * 129, 70 * 129)
+    exact = _bgemm([_gemm(a.data_ptr(), b.data_ptr(), c.data_ptr(), 70, 129, 33, 33, 129, 129)])
+    assert _native.check_bgemm(exact, extents) == 2 * 3
+    # transposed A stored 33 x 70 fills the same buffer
+    turned = _bgemm([_gemm(a.data_ptr(), b.data_ptr(), c.data_ptr(), 70, 129, 33, 70, 129, 129, trans_a=1)])
+    assert _native.check_bgemm(turned, extents) == 6
+    over = _bgemm([_gemm(a.data_ptr() + 8, b.data_ptr(), c.data_ptr(), 70, 129, 33, 33, 129, 129)])
+    with pytest.raises(RuntimeError, match='bgemm A'):
+        _native.check_bgemm(over, extents)
+    wide = _bgemm([_gemm(a.data_ptr(), b.data_ptr(), c.data_ptr(), 70, 130, 33, 33, 129, 129)])
+    with pytest.raises(RuntimeError, match='row stride'):
+        _native.check_bgemm(wide, extents)
+
+
+def test_bgemm_scales_shifts_and_numbering():
+    (a, b, c, s, r), extents = _buffers(16, 16, 16, 4, 3)
+    ok = _bgemm([_gemm(a.data_ptr(), b.data_ptr(), c.data_ptr(), 4, 4, 4, 4, 4, 4, scale=s.data_ptr()),
+                 _gemm(a.data_ptr(), b.data_ptr(), c.data_ptr(), 3, 4, 4, 4, 4, 4, scale=s.data_ptr(),
+                       shift=r.data_ptr())])
+    assert _native.check_bgemm(ok, extents) == 2
+    no_scale = _bgemm([_gemm(a.data_ptr(), b.data_ptr(), c.data_ptr(), 3, 4, 4, 4, 4, 4, shift=r.data_ptr())])
+    with pytest.raises(RuntimeError, match='column shifts'):
+        _native.check_bgemm(no_scale, extents)
+    long_shift = _bgemm([_gemm(a.data_ptr(), b.data_ptr(), c.data_ptr(), 4, 4, 4, 4, 4, 4, scale=s.data_ptr(),
+                               shift=r.data_ptr())])
+    with pytest.raises(RuntimeError, match='row shift'):
+        _native.check_bgemm(long_shift, extents)
+    renumbered = ok.copy()
+    renumbered['first'][1] = 5
+    with pytest.raises(RuntimeError, match='numbered'):
+        _native.check_bgemm(renumbered, extents)
+
+
+def test_overlapping_views_are_one_buffer_and_adjacent_buffers_are_not():
+    base = torch.zeros(100, dtype=torch.float64)
+    extents = _native.Extents().add(base[:60]).add(base[40:])
+    extents.check(base.data_ptr(), 0, 8 * 99, 'whole')             # the union of two overlapping views
+    first, second = torch.zeros(64, dtype=torch.float64), torch.zeros(64, dtype=torch.float64)
+    apart = _native.Extents().add(first).add(second)
+    lo, hi = sorted([first.data_ptr(), second.data_ptr()])
+    with pytest.raises(RuntimeError):
+        apart.check(lo, 0, 8 * 64, 'one past the first buffer')
+
+
+def test_row_and_lincomb_tables():
+    (src, dst, norms), extents = _buffers(10 * 12, 10 * 12, 10)
+    index = torch.tensor([3, 1, 4], dtype=torch.int32)
+    extents.add(index)
+    rows = _table([_rows(src.data_ptr(), norms.data_ptr(), 10, 11, 12, 1)], _native.ROWS_PROBLEM, 'rows')[0]
+    assert _native.check_rows(rows, extents, vector_dst=True) == 10
+    turned = _table([_rows(src.data_ptr(), dst.data_ptr(), 3, 11, 12, 10, 1, index.data_ptr())],
+                    _native.ROWS_PROBLEM, 'rows')[0]
+    assert _native.check_rows(turned, extents, source_rows=[10]) == 3
+    with pytest.raises(RuntimeError, match='row source'):
+        _native.check_rows(turned, extents, source_rows=[11])
+    comb = _table([_lincomb(src.data_ptr(), dst.data_ptr(), dst.data_ptr(), 10, 12, 12, 12, 12, 1.0, 2.0, 0.0)],
+                  _native.LINCOMB_PROBLEM, 'rows')[0]
+    assert _native.check_lincomb(comb, extents) == 10
+    past = _table([_lincomb(src.data_ptr(), 0, dst.data_ptr() + 8, 10, 12, 12, 0, 12, 1.0, 0.0, 0.0)],
+                  _native.LINCOMB_PROBLEM, 'rows')[0]
+    with pytest.raises(RuntimeError, match='lincomb destination'):
+        _native.check_lincomb(past, extents)
