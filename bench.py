@@ -10,17 +10,19 @@ vector (defect correction); `fp64` keeps every product in FP64.
 
 A step is one sweep.  W warmup sweeps ramp the bond (the state starts with one multiplet per
 sector), then K sweeps are timed between synchronisations, with no instrumentation.  One further
-sweep, outside the timed region, times every block-product and leg-move launch with HIP events
-attached to the kernel dispatch itself (hipExtLaunchKernelGGL), so that each interval is the kernel
-alone; the grouped GEMM of the matvec is the dominant kernel, and its roofline line uses the algorithmic 2mnk of
-its descriptor table over its average launch time.  The plain FP64 form then sweeps once from the
-state the timed sweeps started from, and |dE| between the two forms' first sweep from that state is
-reported (for the FP32 form this is |E - E_FP64|), with the truncated weight of both.  The plain
-form splits the two-site tensor as the reference does, one torch.linalg.svd per block
-(`speedup_vs_plain`); a second plain sweep shares the batched form's device eigensolver for the split,
-so that its ratio isolates the contraction path (`plain_sweeps`).  The CPU
-baseline times the oracle (the numpy restatement of the reference engine) on the reference's own
-CPU case, L = 16, chi = 32, 3 sweeps, on one host thread.
+sweep, outside the timed region, times every block-product, leg-move and tridiagonalisation-step
+launch with HIP events attached to the kernel dispatch itself (hipExtLaunchKernelGGL), so that each
+interval is the kernel alone.  The step of the split's batched eigensolver is the sweep's dominant
+kernel: the roofline line is its algorithmic bytes (trailing block read and written once, plus the
+step's vectors) over its average launch time, with the matvec's grouped GEMM (algorithmic 2mnk of its
+descriptor table) and leg moves beside it.  From the state the timed sweeps started from, the
+batched form then sweeps once in FP64 throughout (`fp64_form`), and the plain FP64 form once with
+each split: one torch.linalg.svd per block as the reference factorises (`speedup_vs_plain`), and the
+batched form's device eigensolver, so that its ratio isolates the contraction path (`plain_sweeps`,
+with the plain block products' TFLOP/s).  |dE| between the forms' first sweep from that state is
+reported (for the FP32 form against FP64 this is |E - E_FP64|), with the truncated weight of each.
+The CPU baseline times the oracle (the numpy restatement of the reference engine) on the
+reference's own CPU case, L = 16, chi = 32, 3 sweeps, on one host thread.
 
 One JSON line goes to stdout; progress goes to stderr.
 '''
@@ -146,6 +148,7 @@ def main() -> int:
               f'(asked for {args.gpus}).', file=sys.stderr)
         return 2
 
+    compute = 'f32' if args.precision == 'fp32' else 'f64'
     import numpy as np
     import torch
     if not torch.cuda.is_available():
@@ -193,8 +196,65 @@ def main() -> int:
     total = time.perf_counter() - total_start
     per_sweep = total / args.steps
 
-    # One more sweep, after the timed region and outside it, with every leg move and block product
-    # of the grouped form timed by events attached to the kernel dispatch (kernel durations only).
+    after = ([t.clone() for t in driver.state.tensors], driver.state.center)
+
+    def from_snapshot(label, mode, precision, method, profiler=None):
+        '''One sweep from the state the timed sweeps started from, in another form: mode 1 is the
+        plain per-block form (per-block launches for every contraction), `precision` the block
+        products' type, `method` the split: 'exact' per block by torch.linalg.svd, as the reference
+        factorises (svd.cpp:59), or 'gram', the batched device eigensolver the batched form uses.'''
+        state = driver.state
+        state.tensors = [t.clone() for t in snapshot[0]]
+        state.center = snapshot[1]
+        workspaces.clear()
+        Settings.mode = mode
+        saved = driver.precision, SvdSettings.method
+        driver.precision, SvdSettings.method = precision, method
+        try:
+            torch.cuda.synchronize()
+            start = time.perf_counter()
+            if profiler is None:
+                energy, lost = driver.sweep()
+            else:
+                with profiler:
+                    energy, lost = driver.sweep()
+            torch.cuda.synchronize()
+            elapsed = time.perf_counter() - start
+        finally:
+            Settings.mode = 0
+            driver.precision, SvdSettings.method = saved
+        log(f'[bench] {label} sweep: {elapsed:.2f} s energy {energy:.12f} |dE| {abs(energy - energies[0]):.2e} '
+            f'truncated {lost:.2e}')
+        return elapsed, energy, lost
+
+    # the batched form in FP64 throughout, from the same state
+    fp64_form = None
+    if compute == 'f32':
+        t64, e64, lost64 = from_snapshot('batched FP64', 0, 'fp64', SvdSettings.method)
+        fp64_form = {'s_per_sweep': round(t64, 4), 'dE_vs_fp32_form': abs(e64 - energies[0]), 'truncated': lost64}
+
+    # the plain form, its block products timed program by program
+    plain, plain_tflops = {}, {}
+    if not args.no_plain:
+        for split in ('reference', 'shared'):
+            if args.plain_split in (split, 'both'):
+                timer = Profiler()
+                plain[split] = from_snapshot(f'plain ({split} split)', 1, 'fp64',
+                                             'exact' if split == 'reference' else 'gram', timer)
+                part = timer.summary().get('plain')
+                if part and part['ms'] > 0:
+                    plain_tflops[split] = round(part['flops'] / (part['ms'] * 1e-3) / 1e12, 3)
+                    log(f'[bench] plain ({split} split) block products: {part["ops"]} programs, '
+                        f'{part["ms"]:.1f} ms, {plain_tflops[split]:.2f} TFLOP/s')
+    first_plain = plain.get('reference') or plain.get('shared')
+    plain_per_sweep, plain_energy, plain_lost = first_plain if first_plain else (None, None, None)
+    delta = None if plain_energy is None else abs(plain_energy - energies[0])
+
+    # One more sweep, after the timed region and outside it, from the state the timed sweeps left
+    # and last on the device (so a profiler's last dispatches are this sweep's), with every leg move,
+    # block product and tridiagonalisation step timed by events attached to the kernel dispatch.
+    driver.state.tensors, driver.state.center = after
+    workspaces.clear()
     profiler = Profiler()
     if not args.no_profile:
         start = time.perf_counter()
@@ -208,53 +268,38 @@ def main() -> int:
         breakdown = time_breakdown(driver)
         log(f'[bench] breakdown of one sweep: {json.dumps(breakdown)}')
 
-    def plain_sweep(split):
-        '''One plain FP64 sweep from the snapshot: per-block launches for every contraction, and the
-        split either per block by torch.linalg.svd, as the reference factorises (svd.cpp:59), or
-        by the batched device eigensolver the batched form uses.'''
-        state = driver.state
-        state.tensors = [t.clone() for t in snapshot[0]]
-        state.center = snapshot[1]
-        workspaces.clear()
-        Settings.mode = 1
-        precision, method = driver.precision, SvdSettings.method
-        driver.precision = 'fp64'
-        SvdSettings.method = 'exact' if split == 'reference' else 'gram'
-        try:
-            torch.cuda.synchronize()
-            start = time.perf_counter()
-            energy, lost = driver.sweep()
-            torch.cuda.synchronize()
-            elapsed = time.perf_counter() - start
-        finally:
-            Settings.mode = 0
-            driver.precision, SvdSettings.method = precision, method
-        log(f'[bench] plain sweep ({split} split): {elapsed:.2f} s energy {energy:.12f} '
-            f'|dE| {abs(energy - energies[0]):.2e} truncated {lost:.2e}')
-        return elapsed, energy, lost
-
-    plain = {}
-    if not args.no_plain:
-        for split in ('reference', 'shared'):
-            if args.plain_split in (split, 'both'):
-                plain[split] = plain_sweep(split)
-    first_plain = plain.get('reference') or plain.get('shared')
-    plain_per_sweep, plain_energy, plain_lost = first_plain if first_plain else (None, None, None)
-    delta = None if plain_energy is None else abs(plain_energy - energies[0])
-
     roofline = None
     matvec = 'EPaqj,JRSFb,AQsb->EPrf'
-    compute = 'f32' if args.precision == 'fp32' else 'f64'
     peak = PEAK_FP32_TFLOPS if compute == 'f32' else PEAK_FP64_TFLOPS
     summary = profiler.summary(matvec, compute) if not args.no_profile else {}
     overall = profiler.summary() if not args.no_profile else {}
     refine = profiler.summary(matvec, 'f64') if not args.no_profile and compute == 'f32' else {}
     for name, part in (('all', overall), ('matvec', summary), ('matvec fp64 refine', refine)):
-        for kind in ('gemm', 'move'):
+        for kind in ('gemm', 'move', 'tridiag'):
             if kind in part and part[kind]['ms'] > 0:
                 k = part[kind]
                 log(f'[bench] {name} {kind}: {k["launches"]} launches, {k["ms"]:.1f} ms, '
                     f'{k["flops"] / (k["ms"] * 1e-3) / 1e12:.2f} TFLOP/s, {k["bytes"] / (k["ms"] * 1e-3) / 1e9:.1f} GB/s')
+    if 'tridiag' in overall and overall['tridiag']['ms'] > 0:
+        # the dominant kernel of the sweep: a Householder step of the split's batched eigensolver
+        tri = overall['tridiag']
+        gbs = tri['bytes'] / (tri['ms'] * 1e-3) / 1e9
+        traffic = measured_traffic('tridiag')
+        roofline = {'bound': 'hbm',
+                    'kernel': 'tridiag_step_kernel<4|8|16|32> (one Householder step of the split\'s batched '
+                              'symmetric eigensolver over every Gram matrix of a pass, one wave per trailing '
+                              'column)',
+                    'achieved': round(gbs, 1), 'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
+                    'frac': round(gbs / PEAK_HBM_GBS, 4),
+                    'traffic': None if traffic is None else traffic['bytes_per_step'],
+                    'traffic_source': None if traffic is None else traffic['source'],
+                    'algorithmic_bytes_per_launch': tri['bytes'] / tri['launches'],
+                    'launches': tri['launches'], 'avg_launch_us': round(tri['ms'] * 1e3 / tri['launches'], 3),
+                    'timing': 'kernel-dispatch events (hipExtLaunchKernelGGL) on every step launch of one '
+                              'instrumented sweep after the timed region; algorithmic bytes per step: the '
+                              'trailing block read and written once plus the step vectors, 8 (2 m^2 + 6 m) per '
+                              'matrix with m trailing columns'}
+    gemm_entry = None
     if 'gemm' in summary and summary['gemm']['ms'] > 0:
         g = summary['gemm']
         achieved = g['flops'] / (g['ms'] * 1e-3) / 1e12
@@ -264,25 +309,34 @@ def main() -> int:
                   if compute == 'f32' else 'ggemm_kernel<double,double,double,double> (matvec block products), with '
                   'ggemm_reduce_kernel<double,double> after a step whose products are split along their depth')
         traffic = measured_traffic(compute)
-        roofline = {'bound': 'mfma', 'kernel': kernel,
-                    'achieved': round(achieved, 3), 'peak': peak, 'unit': 'TFLOP/s',
-                    'frac': round(achieved / peak, 4),
-                    'traffic': None if traffic is None else traffic['bytes_per_step'],
-                    'traffic_source': None if traffic is None else traffic['source'],
-                    'algorithmic_bytes_per_step': g['bytes'] / g['ops'],
-                    'steps': g['ops'], 'launches': g['launches'],
-                    'avg_step_us': round(g['ms'] * 1e3 / g['ops'], 2),
-                    'flops_per_step': g['flops'] / g['ops'],
-                    'timing': 'kernel-dispatch events (hipExtLaunchKernelGGL) over one instrumented sweep '
-                              'after the timed region; a step is one grouped launch plus, when its products '
-                              'are split along their depth, the launch that adds their parts'}
-    if 'move' in summary and summary['move']['ms'] > 0 and roofline is not None:
+        gemm_entry = {'bound': 'mfma', 'kernel': kernel,
+                      'achieved': round(achieved, 3), 'peak': peak, 'unit': 'TFLOP/s',
+                      'frac': round(achieved / peak, 4),
+                      'traffic': None if traffic is None else traffic['bytes_per_step'],
+                      'traffic_source': None if traffic is None else traffic['source'],
+                      'algorithmic_bytes_per_step': g['bytes'] / g['ops'],
+                      'steps': g['ops'], 'launches': g['launches'],
+                      'avg_step_us': round(g['ms'] * 1e3 / g['ops'], 2),
+                      'flops_per_step': g['flops'] / g['ops'],
+                      'timing': 'kernel-dispatch events (hipExtLaunchKernelGGL) over one instrumented sweep '
+                                'after the timed region; a step is one grouped launch plus, when its products '
+                                'are split along their depth, the launch that adds their parts'}
+    moves_entry = None
+    if 'move' in summary and summary['move']['ms'] > 0:
         m = summary['move']
         gbs = m['bytes'] / (m['ms'] * 1e-3) / 1e9
-        roofline['moves'] = {'bound': 'hbm', 'kernel': 'gss_kernel (matvec leg moves)', 'achieved': round(gbs, 1),
-                             'peak': PEAK_HBM_GBS, 'unit': 'GB/s', 'frac': round(gbs / PEAK_HBM_GBS, 4),
-                             'launches': m['launches'], 'avg_launch_us': round(m['ms'] * 1e3 / m['launches'], 2),
-                             'bytes_per_launch': m['bytes'] / m['launches']}
+        moves_entry = {'bound': 'hbm', 'kernel': 'gss_kernel (matvec leg moves into the FP32 form\'s workspace)'
+                       if compute == 'f32' else 'gss_kernel (matvec leg moves)', 'achieved': round(gbs, 1),
+                       'peak': PEAK_HBM_GBS, 'unit': 'GB/s', 'frac': round(gbs / PEAK_HBM_GBS, 4),
+                       'launches': m['launches'], 'avg_launch_us': round(m['ms'] * 1e3 / m['launches'], 2),
+                       'bytes_per_launch': m['bytes'] / m['launches']}
+    if roofline is None and gemm_entry is not None:
+        roofline = dict(gemm_entry)
+    elif roofline is not None:
+        roofline['matvec_gemm'] = gemm_entry
+        roofline['matvec_moves'] = moves_entry
+    if roofline is not None and 'gemm' in overall and overall['gemm']['ms'] > 0:
+        roofline['all_block_products_tflops'] = round(overall['gemm']['flops'] / (overall['gemm']['ms'] * 1e-3) / 1e12, 3)
 
     if args.copy_mib:
         from tentorch_amd import _native
@@ -318,8 +372,10 @@ def main() -> int:
         'speedup_vs_plain': None if plain_per_sweep is None else round(plain_per_sweep / per_sweep, 3),
         'dE_plain_vs_batched': delta,
         'plain_sweeps': {split: {'s_per_sweep': round(t, 4), 'speedup': round(t / per_sweep, 3),
-                                 'dE': abs(e - energies[0]), 'truncated': lost}
+                                 'dE': abs(e - energies[0]), 'truncated': lost,
+                                 'block_product_tflops': plain_tflops.get(split)}
                          for split, (t, e, lost) in plain.items()},
+        'fp64_form': fp64_form,
         'roofline': roofline, 'cpu_baseline': baseline, 'breakdown_s': breakdown,
     }
     print(json.dumps(line), flush=True)

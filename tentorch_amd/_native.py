@@ -201,6 +201,33 @@ def eigh_workspace(sizes) -> int:
     return size
 
 
+# Called with the matrix orders of every tt_eigh_batched call while set (by a profiler); the
+# call's tridiagonalisation step launches are then timed by events attached to their dispatch.
+eigh_observer = None
+
+# orders up to this are tridiagonalised in one launch per batch; larger ones step by step above it
+EIGH_STEPPED_ABOVE = 128
+EIGH_BATCH = 64
+
+
+def tridiagonal_steps(sizes):
+    '''(launches, algorithmic bytes) of the step-by-step tridiagonalisation tt_eigh_batched runs
+    for matrices of these orders: per batch of 64, one launch for each step k below the batch's
+    largest order minus 128.  Step k of a matrix with m = n - k - 1 > 0 trailing columns reads and
+    writes its trailing block once and its vectors (column k, v and p of the previous step, v and p
+    of this one): 8 (2 m^2 + 6 m) bytes.'''
+    launches, moved = 0, 0
+    for first in range(0, len(sizes), EIGH_BATCH):
+        batch = np.asarray(sizes[first:first + EIGH_BATCH], dtype=np.int64)
+        steps = max(0, int(batch.max()) - EIGH_STEPPED_ABOVE)
+        launches += steps
+        for k in range(steps):
+            m = batch - k - 1
+            m = m[m > 0]
+            moved += int(8 * np.sum(2 * m * m + 6 * m))
+    return launches, moved
+
+
 def eigh_batched(matrices: torch.Tensor, sizes, offsets=None, work: Optional[torch.Tensor] = None,
                  vectors: Optional[torch.Tensor] = None):
     '''Eigenvalues (ascending) and eigenvectors of the symmetric FP64 matrices packed in `matrices`
@@ -232,8 +259,17 @@ def eigh_batched(matrices: torch.Tensor, sizes, offsets=None, work: Optional[tor
     workspace = torch.empty(max(size, 1), dtype=torch.float64, device=matrices.device) if work is None else work
     if workspace.numel() < size or workspace.dtype != torch.float64:
         raise ValueError(f'the eigensolver workspace holds {workspace.numel()} doubles, {size} needed')
-    check(_lib.tt_eigh_batched(problems, count, matrices.data_ptr(), values.data_ptr(), vectors.data_ptr(),
-                               workspace.data_ptr(), stream()), 'eigh')
+    observer = eigh_observer
+    if observer is not None:
+        timing(True)
+    try:
+        check(_lib.tt_eigh_batched(problems, count, matrices.data_ptr(), values.data_ptr(), vectors.data_ptr(),
+                                   workspace.data_ptr(), stream()), 'eigh')
+    finally:
+        if observer is not None:
+            timing(False)
+    if observer is not None:
+        observer(list(sizes))
     return values[:value_at], vectors
 
 

@@ -1305,16 +1305,20 @@ int eigh_batched(const tt_eigh_problem *problems, int64_t count, double *matrice
     for (int k = 0; k < lds_from; ++k) {
       int grid = 0;
       for (int32_t b = 0; b < batch.count; ++b) grid += step_blocks(batch.p[b].n, k);
-      const int64_t rows = largest - k;   // the step's longest vectors: rows k..n-1
+      // the step's longest vectors are rows k..n-1; in timing mode each step launch is timed
+      const int64_t rows = largest - k;
+      const dim3 blocks(grid), threads(kStepThreads);
+      int rc;
       if (rows <= 4 * 64) {
-        hipLaunchKernelGGL(tridiag_step_kernel<4>, dim3(grid), dim3(kStepThreads), 0, s, table, matrices, workspace, k);
+        rc = ttdev::launch(tridiag_step_kernel<4>, blocks, threads, 0, s, table, matrices, workspace, k);
       } else if (rows <= 8 * 64) {
-        hipLaunchKernelGGL(tridiag_step_kernel<8>, dim3(grid), dim3(kStepThreads), 0, s, table, matrices, workspace, k);
+        rc = ttdev::launch(tridiag_step_kernel<8>, blocks, threads, 0, s, table, matrices, workspace, k);
       } else if (rows <= 16 * 64) {
-        hipLaunchKernelGGL(tridiag_step_kernel<16>, dim3(grid), dim3(kStepThreads), 0, s, table, matrices, workspace, k);
+        rc = ttdev::launch(tridiag_step_kernel<16>, blocks, threads, 0, s, table, matrices, workspace, k);
       } else {
-        hipLaunchKernelGGL(tridiag_step_kernel<32>, dim3(grid), dim3(kStepThreads), 0, s, table, matrices, workspace, k);
+        rc = ttdev::launch(tridiag_step_kernel<32>, blocks, threads, 0, s, table, matrices, workspace, k);
       }
+      if (rc) return rc;
       TT_LAUNCHED();
     }
     hipLaunchKernelGGL(tridiag_lds_kernel, dim3(batch.count), dim3(kTriThreads), 0, s, table, matrices, workspace,

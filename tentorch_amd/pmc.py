@@ -1,35 +1,42 @@
 '''
 Reduction of rocprofv3 output of `bench.py` to the figures its roofline line uses.
 
-Three profiled runs of the same bench command (the FP32 form, with its instrumented sweep last and a
-256 MiB calibration copy after it):
+Profiled runs of the bench command (the FP32 form, with its instrumented sweep last and a 256 MiB
+calibration copy after it), a kernel trace and two counter passes:
 
-    rocprofv3 --kernel-trace --stats --output-format csv -d T -o bench -- python3 -u bench.py --no-plain --no-cpu > T/bench.json
-    rocprofv3 --pmc FETCH_SIZE --kernel-include-regex "ggemm|axpby" --output-format csv -d F -o fetch -- python3 -u bench.py --no-plain --no-cpu --copy-mib 256
-    rocprofv3 --pmc WRITE_SIZE --kernel-include-regex "ggemm|axpby" --output-format csv -d W -o write -- python3 -u bench.py --no-plain --no-cpu --copy-mib 256
-    python -m tentorch_amd.pmc --trace T --fetch F --write W --bench-line T/bench.json --out OUT
+    rocprofv3 --kernel-trace --stats --output-format csv -d T -o bench -- python3 -u bench.py ARGS > T.json
+    rocprofv3 --pmc FETCH_SIZE --kernel-include-regex "tridiag_step|ggemm|axpby" --output-format csv -d F -o fetch -- python3 -u bench.py ARGS --copy-mib 256 > F.json
+    rocprofv3 --pmc WRITE_SIZE --kernel-include-regex "tridiag_step|ggemm|axpby" --output-format csv -d W -o write -- python3 -u bench.py ARGS --copy-mib 256 > W.json
+    python -m tentorch_amd.pmc --reduce T --bench-line T.json --out trace.json     (and F, W alike)
+    python -m tentorch_amd.pmc --combine --fetch fetch.json --write write.json --out OUT
+
+Each run is reduced where it ran (its raw files are large); the reductions are small JSON files.
 
 `--per-kernel` prints the mean counters of every kernel of two counter passes instead (used for
 the block-einsum microbenchmark, `python -m tentorch_amd.microbench ... --copy-mib 256`).
 
 The runs are deterministic (seed 0), so the last sweep of each is the same instrumented sweep.  Of
-it, the matvec's FP32 block-product steps are taken: the last `steps / 2` dispatches of each of
+it, the step launches of the split's tridiagonalisation are taken (the last `launches` dispatches of
+`tridiag_step_kernel<...>`, the count the bench line's roofline reports), and the matvec's FP32
+block-product steps: the last `steps / 2` dispatches of each of
 the two kernel families (`ggemm_kernel<float, double, float, float, ...>` for the first product,
 `<float, float, double, float, ...>` for the second) and the last `launches - steps` partial-sum
 launches (`ggemm_reduce_kernel<double, float>`), the counts the bench line reports for that sweep.
 The byte counters are scaled by what they read for the copy (read 256 MiB, write 256 MiB), the last
-`axpby_kernel<double>` dispatch.  Writes `dmrg_matvec_gemm_L64_chi1024_pmc.json` (read by bench.py)
-and `reduced.json` (kernel-trace averages and raw counter sums) into OUT.
+`axpby_kernel<double>` dispatch.  `--combine` writes `dmrg_matvec_gemm_L64_chi1024_pmc.json` (read
+by bench.py: HBM bytes per tridiagonalisation launch under `tridiag`, per block-product step under
+`f32`) into OUT.
 '''
 
 import argparse
+import collections
 import csv
 import glob
 import json
 import os
-import shutil
 import sys
 
+TRIDIAG = 'tridiag_step_kernel'
 STEP1 = 'ggemm_kernel<float, double, float, float'
 STEP2 = 'ggemm_kernel<float, float, double, float'
 REDUCE = 'ggemm_reduce_kernel<double, float>'
@@ -74,61 +81,129 @@ def per_kernel(fetch_dir, write_dir, copy_mib):
     return 0
 
 
+def _groups(roof):
+    '''The kernel families of the instrumented sweep the bench line reports: name fragment and
+    dispatch count, per group (the tridiagonalisation steps; the matvec's FP32 block products).'''
+    gemm = roof.get('matvec_gemm') if roof.get('bound') == 'hbm' else roof
+    groups = {}
+    if roof.get('bound') == 'hbm':
+        groups['tridiag'] = {'families': {'tridiag': (TRIDIAG, roof['launches'])}, 'units': roof['launches'],
+                             'algorithmic': roof['algorithmic_bytes_per_launch'], 'unit': 'launch'}
+    if gemm:
+        steps = gemm['steps']
+        groups['f32'] = {'families': {'step1': (STEP1, steps // 2), 'step2': (STEP2, steps // 2),
+                                      'reduce': (REDUCE, gemm['launches'] - steps)},
+                         'units': steps, 'algorithmic': gemm['algorithmic_bytes_per_step'],
+                         'unit': 'block-product step'}
+    return groups
+
+
+def _stream_last(path, families, value):
+    '''The last `count` values of each family (dispatch order) and the last copy dispatch's value,
+    read row by row (a counter or trace file of a whole bench run holds about a million dispatches).'''
+    last = {key: collections.deque(maxlen=max(count, 1)) for key, (_, count) in families.items()}
+    copy, previous = None, -1
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            dispatch = int(row['Dispatch_Id'])
+            if dispatch < previous:
+                raise RuntimeError(f'{path}: dispatches out of order')
+            previous = dispatch
+            name = row['Kernel_Name']
+            if COPY in name:
+                copy = value(row)
+            for key, (fragment, count) in families.items():
+                if count and fragment in name:
+                    last[key].append(value(row))
+    return {key: list(values) for key, values in last.items()}, copy
+
+
+def reduce_pass(directory, bench_line, out):
+    '''One profiled run (kernel trace, or one counter) reduced to its instrumented sweep's figures.'''
+    roof = json.loads(open(bench_line).read().strip().splitlines()[-1])['roofline']
+    groups = _groups(roof)
+    families = {f'{g}:{k}': v for g, spec in groups.items() for k, v in spec['families'].items()}
+    traces = glob.glob(os.path.join(directory, '**', '*kernel_trace.csv'), recursive=True)
+    result = {'roofline': roof}
+    if traces:
+        last, _ = _stream_last(traces[0], families,
+                               lambda r: int(r['End_Timestamp']) - int(r['Start_Timestamp']))
+        result['kind'] = 'trace'
+        result['avg_us'] = {k: sum(v) / max(len(v), 1) / 1e3 for k, v in last.items()}
+        result['dispatches'] = {k: len(v) for k, v in last.items()}
+        for g, spec in groups.items():
+            total = sum(sum(last[f'{g}:{k}']) for k in spec['families'])
+            result.setdefault('avg_us_per_unit', {})[g] = total / spec['units'] / 1e3
+        stats = glob.glob(os.path.join(directory, '**', '*kernel_stats.csv'), recursive=True)
+        if stats:
+            with open(stats[0]) as f:
+                result['stats'] = [r for r in csv.DictReader(f)
+                                   if any(fragment in r['Name'] for fragment, _ in families.values())]
+    else:
+        counters = glob.glob(os.path.join(directory, '**', '*counter_collection.csv'), recursive=True)
+        if not counters:
+            raise FileNotFoundError(f'no kernel trace or counter file under {directory}')
+        names = set()
+
+        def value(row):
+            names.add(row['Counter_Name'])
+            return float(row['Counter_Value'])
+        last, copy = _stream_last(counters[0], families, value)
+        result['kind'] = 'counter'
+        result['counter'] = sorted(names)
+        result['copy_kib'] = copy
+        result['sum_kib'] = {g: sum(sum(last[f'{g}:{k}']) for k in spec['families']) for g, spec in groups.items()}
+        result['dispatches'] = {k: len(v) for k, v in last.items()}
+    with open(out, 'w') as f:
+        json.dump(result, f, indent=1)
+    print(json.dumps({k: v for k, v in result.items() if k not in ('roofline', 'stats')}))
+    return result
+
+
+def combine(fetch_json, write_json, copy_mib, out_dir):
+    '''The two counter passes' figures as HBM bytes per unit of each group, calibrated on the copy.'''
+    fetch, write = json.load(open(fetch_json)), json.load(open(write_json))
+    kib = copy_mib * 1024.0
+    fetch_scale, write_scale = kib / fetch['copy_kib'], kib / write['copy_kib']     # the counters read KiB
+    summary = {'fetch_scale': fetch_scale, 'write_scale': write_scale, 'copy_mib': copy_mib}
+    for g, spec in _groups(fetch['roofline']).items():
+        fetch_bytes = fetch['sum_kib'][g] * fetch_scale * 1024.0 / spec['units']
+        write_bytes = write['sum_kib'][g] * write_scale * 1024.0 / spec['units']
+        summary[g] = {'hbm_bytes_per_step': fetch_bytes + write_bytes, 'fetch_bytes_per_step': fetch_bytes,
+                      'write_bytes_per_step': write_bytes, 'algorithmic_bytes_per_step': spec['algorithmic'],
+                      'units': spec['units'], 'unit': spec['unit']}
+    os.makedirs(out_dir, exist_ok=True)
+    with open(os.path.join(out_dir, 'dmrg_matvec_gemm_L64_chi1024_pmc.json'), 'w') as f:
+        json.dump(summary, f, indent=1)
+    print(json.dumps(summary))
+    return summary
+
+
 def main(argv=None) -> int:
     parser = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     parser.add_argument('--per-kernel', action='store_true',
-                        help='print a per-kernel table of the two counter passes instead (no --trace needed)')
-    for flag in ('--trace', '--fetch', '--write', '--bench-line', '--out'):
+                        help='print a per-kernel table of two counter passes (--fetch, --write directories)')
+    parser.add_argument('--reduce', help='one profiled run\'s output directory, reduced to --out (JSON)')
+    parser.add_argument('--combine', action='store_true',
+                        help='the reduced FETCH_SIZE and WRITE_SIZE passes (--fetch, --write JSON) to --out (a directory)')
+    for flag in ('--fetch', '--write', '--bench-line', '--out'):
         parser.add_argument(flag)
     parser.add_argument('--copy-mib', type=int, default=256)
     args = parser.parse_args(argv)
     if args.per_kernel:
         return per_kernel(args.fetch, args.write, args.copy_mib)
-    if not all((args.trace, args.fetch, args.write, args.bench_line, args.out)):
-        parser.error('--trace, --fetch, --write, --bench-line and --out are needed')
-    os.makedirs(args.out, exist_ok=True)
-    bench = json.loads(open(args.bench_line).read().strip().splitlines()[-1])
-    roof = bench['roofline']
-    steps, reduces = roof['steps'], roof['launches'] - roof['steps']
-    families = {'step1': (STEP1, steps // 2), 'step2': (STEP2, steps // 2), 'reduce': (REDUCE, reduces)}
-
-    trace = _rows(os.path.join(args.trace, '**', '*kernel_trace.csv'))
-    trace.sort(key=lambda r: int(r['Start_Timestamp']))
-    out = {'bench_roofline': roof, 'steps_per_sweep': steps, 'reduce_launches_per_sweep': reduces}
-    total_ns = 0
-    for key, (name, count) in families.items():
-        d = [int(r['End_Timestamp']) - int(r['Start_Timestamp']) for r in _last(trace, name, count)]
-        total_ns += sum(d)
-        out['trace_' + key] = {'dispatches': len(d), 'avg_us': sum(d) / max(len(d), 1) / 1e3}
-    out['trace_avg_step_us'] = total_ns / steps / 1e3
-
-    def counters(directory):
-        rows = _rows(os.path.join(directory, '**', '*counter_collection.csv'))
-        rows.sort(key=lambda r: int(r['Dispatch_Id']))
-        sums = {key: sum(float(r['Counter_Value']) for r in _last(rows, name, count))
-                for key, (name, count) in families.items()}
-        copy = [r for r in rows if COPY in r['Kernel_Name']][-1]
-        return sums, float(copy['Counter_Value'])
-
-    fetch, fetch_copy = counters(args.fetch)
-    write, write_copy = counters(args.write)
-    kib = args.copy_mib * 1024.0
-    fetch_scale, write_scale = kib / fetch_copy, kib / write_copy    # the counters read KiB
-    fetch_bytes = sum(fetch.values()) * fetch_scale * 1024.0 / steps
-    write_bytes = sum(write.values()) * write_scale * 1024.0 / steps
-    out['pmc'] = {'fetch_kib': fetch, 'write_kib': write, 'copy_fetch_kib': fetch_copy, 'copy_write_kib': write_copy}
-    summary = {'f32': {'hbm_bytes_per_step': fetch_bytes + write_bytes, 'fetch_bytes_per_step': fetch_bytes,
-                       'write_bytes_per_step': write_bytes,
-                       'algorithmic_bytes_per_step': roof['algorithmic_bytes_per_step'], 'steps': steps},
-               'fetch_scale': fetch_scale, 'write_scale': write_scale, 'copy_mib': args.copy_mib}
-    with open(os.path.join(args.out, 'reduced.json'), 'w') as f:
-        json.dump(out, f, indent=1)
-    with open(os.path.join(args.out, 'dmrg_matvec_gemm_L64_chi1024_pmc.json'), 'w') as f:
-        json.dump(summary, f, indent=1)
-    for path in glob.glob(os.path.join(args.trace, '**', '*kernel_stats.csv'), recursive=True):
-        shutil.copy(path, os.path.join(args.out, 'kernel_stats.csv'))
-    print(json.dumps(summary))
-    return 0
+    if args.reduce:
+        if not (args.bench_line and args.out):
+            parser.error('--reduce needs --bench-line and --out')
+        reduce_pass(args.reduce, args.bench_line, args.out)
+        return 0
+    if args.combine:
+        if not (args.fetch and args.write and args.out):
+            parser.error('--combine needs --fetch, --write and --out')
+        combine(args.fetch, args.write, args.copy_mib, args.out)
+        return 0
+    parser.error('one of --per-kernel, --reduce, --combine')
+    return 2
 
 
 if __name__ == '__main__':

@@ -348,23 +348,33 @@ def run(program, operands: Sequence[Tensor], out: Optional[torch.Tensor] = None,
 
 class Profiler:
     '''Times every block-product and leg-move launch of the grouped form while active, op by op,
-    with a start and a stop event attached to the kernel dispatch itself (so the interval is the
-    kernel's execution, not the host's launch latency); `summary()` synchronises and returns per-kind
-    totals with their algorithmic work.'''
+    and every step launch of the eigensolver's tridiagonalisation (kind 'tridiag'), with a start and
+    a stop event attached to the kernel dispatch itself (so the interval is the kernel's execution,
+    not the host's launch latency); `summary()` synchronises and returns per-kind totals with their
+    algorithmic work.  Programs of the per-block form (mode 1) are timed as a whole by events on the
+    stream (kind 'plain', with the flops of their block products).'''
 
     active = None
 
     def __init__(self):
         self.records = []   # (kind, flops, bytes moved, compute type, label, launches)
         self.ms = []        # the kernel time of each record
+        self.plain = []     # (flops, start event, stop event) of each per-block program
 
     def __enter__(self):
         Profiler.active = self
+        _native.eigh_observer = self._eigh
         return self
 
     def __exit__(self, *exc):
         Profiler.active = None
+        _native.eigh_observer = None
         self._collect()
+
+    def _eigh(self, sizes):
+        launches, moved = _native.tridiagonal_steps(sizes)
+        if launches:
+            self.records.append(('tridiag', 0.0, moved, 'f64', 'eigh', launches))
 
     def _collect(self):
         durations = _native.timing_collect()
@@ -378,7 +388,14 @@ class Profiler:
 
     def run(self, program, args, skip, mode, label=''):
         if mode != 0:
-            program.run(*args)      # the per-block form is timed as a whole, not launch by launch
+            # the per-block form is timed as a whole, not launch by launch
+            import torch
+            flops = sum(cost[1] for cost in program.op_costs() if cost[0] == 'gemm')
+            start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            start.record()
+            program.run(*args)
+            stop.record()
+            self.plain.append((flops, start, stop))
             return
         program.prepare()
         costs = program.op_costs()
@@ -421,6 +438,10 @@ class Profiler:
         '''Totals per kind, over every record or over the records of one einsum equation and one
         compute type (a block product's MFMA type; a leg move's workspace type).'''
         out = {}
+        if self.plain and label is None and compute is None:
+            ms = sum(start.elapsed_time(stop) for _, start, stop in self.plain)
+            out['plain'] = {'ops': len(self.plain), 'launches': 0, 'ms': ms,
+                            'flops': float(sum(f for f, _, _ in self.plain)), 'bytes': 0.0}
         for (kind, flops, moved, kind_compute, lab, launches), ms in zip(self.records, self.ms):
             if label is not None and lab != label:
                 continue
