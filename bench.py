@@ -21,8 +21,9 @@ each split: one torch.linalg.svd per block as the reference factorises (`speedup
 batched form's device eigensolver, so that its ratio isolates the contraction path (`plain_sweeps`,
 with the plain block products' TFLOP/s).  |dE| between the forms' first sweep from that state is
 reported (for the FP32 form against FP64 this is |E - E_FP64|), with the truncated weight of each.
-The CPU baseline times the oracle (the numpy restatement of the reference engine) on the
-reference's own CPU case, L = 16, chi = 32, 3 sweeps, on one host thread.
+The CPU baseline times the reference's own engine (its C++ sources built into oracle/_ref) under a
+restatement of its DMRG driver on the reference's CPU case, L = 16, chi = 32, 3 sweeps, on one
+thread and on all the process's threads, with the oracle (numpy, one thread) beside it.
 
 One JSON line goes to stdout; progress goes to stderr.
 '''
@@ -63,11 +64,30 @@ def measured_traffic(compute):
                       'calibrated on a %d MiB copy)' % (data['fetch_scale'], data['write_scale'], data['copy_mib'])}
 
 
+def _cpu_model() -> str:
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
 def cpu_baseline(length=16, bond=32, sweeps=3):
-    '''The oracle on one host thread: s/sweep of the reference's CPU case.'''
+    '''The reference's CPU case (L = 16, chi = 32, n_iter = 5, 3 sweeps from seed 0, FP64) on the host:
+    the reference's own engine (built from its sources into oracle/_ref) under the oracle's
+    restatement of its DMRG driver, on one ATen thread and on every thread the process may use; and
+    the oracle (numpy) on one thread beside it.  Returns the baseline object of the bench line.'''
+    import torch
     from threadpoolctl import threadpool_limits
     import oracle.symmetry as osym
+    from oracle import ref_engine
     from oracle.dmrg import DMRG as OracleDMRG, Model as OracleModel
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else os.cpu_count()
+    threads_all = torch.get_num_threads()
+    out = {'unit': 's/sweep', 'nproc': os.cpu_count(), 'usable_cpus': usable, 'cpu_model': _cpu_model()}
     with threadpool_limits(limits=1):
         driver = OracleDMRG(OracleModel(length, cat=osym.u1), max_bond_dim=bond, n_iter=5, seed=0)
         times = []
@@ -75,8 +95,27 @@ def cpu_baseline(length=16, bond=32, sweeps=3):
             start = time.perf_counter()
             energy, _ = driver.sweep()
             times.append(time.perf_counter() - start)
-            log(f'[cpu] oracle sweep {sweep + 1}: {times[-1]:.2f} s energy {energy:.12f}')
-    return sum(times) / len(times), times, energy
+        log(f'[cpu] oracle (numpy), 1 thread: {sum(times) / len(times):.3f} s/sweep, energy {energy:.12f}')
+    port = {'value': round(sum(times) / len(times), 4), 'cores': 1, 'kind': 'port', 'energy': energy}
+    if ref_engine.engine() is None:
+        out.update(port)
+        out['sample'] = ('oracle (numpy restatement of the reference engine), U(1) Heisenberg L=16 chi=32 n_iter=5, '
+                         '3 sweeps from seed 0, one thread (the reference engine is not built here)')
+        return out
+    runs = {}
+    for threads in (1, threads_all):
+        with threadpool_limits(limits=threads):
+            seconds, energies = ref_engine.sweeps(length, bond, sweeps, threads=threads)
+        runs[threads] = (sum(seconds) / len(seconds), energies[-1])
+        log(f'[cpu] reference engine, {threads} thread(s): {runs[threads][0]:.3f} s/sweep, energy {energies[-1]:.12f}')
+    out.update({'value': round(runs[1][0], 4), 'cores': 1, 'kind': 'reference', 'energy': runs[1][1],
+                'all_cores': {'value': round(runs[threads_all][0], 4), 'cores': threads_all},
+                'port': port,
+                'sample': 'the reference engine (its C++ sources built with g++ into oracle/_ref, ATen per block on '
+                          'the CPU) under a restatement of its DMRG driver: U(1) Heisenberg L=16 chi=32 n_iter=5, '
+                          f'{sweeps} sweeps from seed 0, FP64, on 1 and on {threads_all} ATen threads; the '
+                          'reference\'s CPU case, not L=64 chi=1024'})
+    return out
 
 
 def time_breakdown(driver):
@@ -349,10 +388,7 @@ def main() -> int:
 
     baseline = None
     if not args.no_cpu:
-        value, cpu_times, cpu_energy = cpu_baseline()
-        baseline = {'value': round(value, 4), 'unit': 's/sweep', 'cores': 1, 'kind': 'port',
-                    'sample': 'oracle (numpy restatement of the reference engine), U(1) Heisenberg L=16 chi=32 '
-                              'n_iter=5, 3 sweeps from seed 0, one thread; the reference CPU case, not L=64 chi=1024'}
+        baseline = cpu_baseline()
 
     line = {
         'metric': 's/sweep U(1) Heisenberg L=64 chi=1024, batched vs plain, one GPU, |dE|<=1e-10',

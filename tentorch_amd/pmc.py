@@ -29,7 +29,6 @@ by bench.py: HBM bytes per tridiagonalisation launch under `tridiag`, per block-
 '''
 
 import argparse
-import collections
 import csv
 import glob
 import json
@@ -99,23 +98,25 @@ def _groups(roof):
 
 
 def _stream_last(path, families, value):
-    '''The last `count` values of each family (dispatch order) and the last copy dispatch's value,
-    read row by row (a counter or trace file of a whole bench run holds about a million dispatches).'''
-    last = {key: collections.deque(maxlen=max(count, 1)) for key, (_, count) in families.items()}
-    copy, previous = None, -1
+    '''The last `count` values of each family in dispatch order, and the last copy dispatch's value.
+    The file is read row by row and only matching dispatches are kept (a trace or counter file of a
+    whole bench run holds about a million dispatches, not always in dispatch order).'''
+    kept = {key: [] for key in families}
+    copies = []
     with open(path) as f:
         for row in csv.DictReader(f):
-            dispatch = int(row['Dispatch_Id'])
-            if dispatch < previous:
-                raise RuntimeError(f'{path}: dispatches out of order')
-            previous = dispatch
             name = row['Kernel_Name']
             if COPY in name:
-                copy = value(row)
+                copies.append((int(row['Dispatch_Id']), value(row)))
             for key, (fragment, count) in families.items():
                 if count and fragment in name:
-                    last[key].append(value(row))
-    return {key: list(values) for key, values in last.items()}, copy
+                    kept[key].append((int(row['Dispatch_Id']), value(row)))
+    last = {}
+    for key, (_, count) in families.items():
+        ordered = sorted(kept[key])
+        last[key] = [v for _, v in ordered[-count:]] if count else []
+    copy = max(copies)[1] if copies else None
+    return last, copy
 
 
 def reduce_pass(directory, bench_line, out):
