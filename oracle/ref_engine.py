@@ -28,12 +28,25 @@ _module = None
 
 
 def engine():
-    '''The compiled reference engine, or None where it has not been built.'''
+    '''The compiled reference engine, or None where it has not been built.  Its categories are made
+    at import on the device `default_device` picks (`cat.cpp:13-20`): the GPU where torch sees one,
+    unless TENTORCH_DEVICE is "cpu"; the import sets that, so the engine runs its CPU path on a GPU
+    box too.'''
     global _module
     if _module is None and os.path.exists(LIBRARY):
-        spec = importlib.util.spec_from_file_location('symmetry_ext', LIBRARY)
-        module = importlib.util.module_from_spec(spec)
-        spec.loader.exec_module(module)
+        saved = os.environ.get('TENTORCH_DEVICE')
+        os.environ['TENTORCH_DEVICE'] = 'cpu'
+        try:
+            spec = importlib.util.spec_from_file_location('symmetry_ext', LIBRARY)
+            module = importlib.util.module_from_spec(spec)
+            spec.loader.exec_module(module)
+        finally:
+            if saved is None:
+                del os.environ['TENTORCH_DEVICE']
+            else:
+                os.environ['TENTORCH_DEVICE'] = saved
+        if str(module.u1.device) != 'cpu':
+            raise RuntimeError(f'the reference engine made its categories on {module.u1.device}, not the CPU')
         _module = module
     return _module
 
