@@ -52,10 +52,19 @@ def engine():
 
 
 def _category(ext, cat):
+    '''The reference engine's category of an oracle category.'''
     if isinstance(cat, osym.U1Symmetry):
         return ext.u1
     if isinstance(cat, osym.NoSymmetry):
         return ext.no_symmetry
+    if isinstance(cat, osym.SU2Symmetry):
+        return ext.su2
+    if isinstance(cat, osym.FibonacciCategory):
+        return ext.fibonacci
+    if isinstance(cat, osym.FermionParity):
+        return ext.fermion_parity
+    if isinstance(cat, osym.ZNSymmetry):
+        return ext.z_n(cat.modulus)
     raise ValueError(f'no reference category for {cat.name}')
 
 

@@ -64,3 +64,49 @@ def test_matvec_einsum_and_split_agree_with_the_reference_engine():
         product = ot.contract(u, v, ot.domain_legs(u), [0])
         reference_product = ext.contract(ru, rv, ext.domain_legs(ru), [0])
         _same(reference_product, product, 1e-12)
+
+
+# the legs of tests/test_engine.py's placement test, per category
+_LEGS = {
+    'u1': [([(-2,), (0,), (2,)], [2, 3, 2]), ([(-1,), (1,)], [1, 1])],
+    'z3': [([(0,), (1,), (2,)], [2, 1, 1]), ([(0,), (1,), (2,)], [1, 1, 1])],
+    'fermion_parity': [([(0,), (1,)], [2, 2]), ([(0,), (1,)], [1, 1])],
+    'su2': [([(0,), (1,), (2,)], [2, 1, 2]), ([(1,)], [1])],
+    'fibonacci': [([(0,), (1,)], [2, 3]), ([(0,), (1,)], [1, 1])],
+}
+_CATS = {'u1': osym.u1, 'z3': osym.z_n(3), 'fermion_parity': osym.fermion_parity, 'su2': osym.su2,
+         'fibonacci': osym.fibonacci}
+
+
+def _pair(name, cod, dom, seed=0):
+    cat = _CATS[name]
+    dtype = np.complex128 if name == 'fibonacci' else np.float64
+    rng = np.random.default_rng(seed)
+    o = ot.random_tensor(cat, [osym.Leg(cat, *l) for l in cod], [osym.Leg(cat, *l) for l in dom], cat.trivial(), rng,
+                         dtype)
+    return o, ref_engine.to_reference(ext, o)
+
+
+@pytest.mark.parametrize('name', sorted(_LEGS))
+def test_every_placement_agrees_with_the_reference_engine(name):
+    '''All 5 x 24 placements of a 4-leg tensor: exchanges (braiding, `regroup.cpp:25-258`) and
+    bends (`regroup.cpp:260-417`, the dual convention and the F-entry read included) of the oracle
+    against the reference's own `Tensor.place`, for abelian, fermionic, SU(2) and Fibonacci
+    categories: the recoupling gauge of the oracle is the reference's.'''
+    import itertools
+    a, b = _LEGS[name]
+    o, r = _pair(name, [a, b], [b, a])
+    for n_above in range(5):
+        for perm in itertools.permutations(range(4)):
+            above, below = list(perm[:n_above]), list(perm[n_above:])
+            _same(r.place(above, below), o.place(above, below), 1e-12)
+
+
+@pytest.mark.parametrize('name', ['su2', 'fibonacci'])
+def test_recoupling_einsums_agree_with_the_reference_engine(name):
+    '''Contractions whose first move recouples two legs (the learned schedule's uneven cases).'''
+    a, b = _LEGS[name]
+    first, rfirst = _pair(name, [a, b, a], [b], seed=1)
+    second, rsecond = _pair(name, [b], [a], seed=2)
+    for spec, operands, reference in (('ABCd,Be->ACde', (first, second), (rfirst, rsecond)),):
+        _same(ext.einsum(spec, *reference), ot.einsum(spec, *operands), 1e-12)
