@@ -269,6 +269,19 @@ typedef struct tt_lincomb_problem {
   double alpha, beta, gamma;
 } tt_lincomb_problem;
 int tt_lincomb_rows(const tt_lincomb_problem *problems, int64_t count, int64_t rows, void *stream);
+/* Host-side checks of the three table kinds above before they are uploaded (no counterpart in the
+ * reference, whose per-block calls take sized tensors): every operand a problem names must lie in
+ * one of the `spans` device byte ranges [starts[i], ends[i]) (sorted by start, not overlapping),
+ * sizes must be non-negative, row strides no shorter than their rows, and the work items numbered
+ * consecutively from 0 in `first`.  On success `items` receives the launch's work-item count (tiles
+ * or rows).  tt_rows_check: with an index, source_rows[i] is the number of rows problem i's source
+ * holds (the caller keeps every index below it); vector_dst: dst takes one value per row. */
+int tt_bgemm_check(const tt_bgemm_problem *problems, int64_t count, const int64_t *starts, const int64_t *ends,
+                   int64_t spans, int64_t *items);
+int tt_rows_check(const tt_rows_problem *problems, int64_t count, const int64_t *source_rows, int vector_dst,
+                  const int64_t *starts, const int64_t *ends, int64_t spans, int64_t *items);
+int tt_lincomb_check(const tt_lincomb_problem *problems, int64_t count, const int64_t *starts, const int64_t *ends,
+                     int64_t spans, int64_t *items);
 
 /* The learned tile schedule's cost model (no counterpart in the reference): a multilayer
  * perceptron x (features) -> relu(W1^T x + b1) (hidden) -> relu(W2^T h1 + b2) (hidden) -> w3 . h2 + b3

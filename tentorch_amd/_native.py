@@ -64,6 +64,9 @@ _lib.tt_bgemm.argtypes = [_c_void_p, _i64, _i64, _c_void_p]
 _lib.tt_row_norms.argtypes = [_c_void_p, _i64, _i64, _c_void_p]
 _lib.tt_copy_rows.argtypes = [_c_void_p, _i64, _i64, _c_void_p]
 _lib.tt_lincomb_rows.argtypes = [_c_void_p, _i64, _i64, _c_void_p]
+_lib.tt_bgemm_check.argtypes = [_c_void_p, _i64, _c_void_p, _c_void_p, _i64, ctypes.POINTER(_i64)]
+_lib.tt_rows_check.argtypes = [_c_void_p, _i64, _c_void_p, _int, _c_void_p, _c_void_p, _i64, ctypes.POINTER(_i64)]
+_lib.tt_lincomb_check.argtypes = [_c_void_p, _i64, _c_void_p, _c_void_p, _i64, ctypes.POINTER(_i64)]
 _lib.tt_graph_begin.argtypes = [_c_void_p]
 _lib.tt_graph_end.argtypes = [_c_void_p, ctypes.POINTER(_c_void_p)]
 _lib.tt_graph_launch.argtypes = [_c_void_p, _c_void_p]
@@ -78,7 +81,8 @@ EXPORTED = ['tt_gss_plan_create', 'tt_gss_plan_destroy', 'tt_gss_plan_counts', '
             'tt_launch_count', 'tt_timing_enable', 'tt_timing_collect', 'tt_mlp_param_count', 'tt_mlp_forward',
             'tt_mlp_gradient', 'tt_mlp_adam', 'tt_eigh_max_order', 'tt_eigh_workspace', 'tt_eigh_batched',
             'tt_lanczos_orthonormalize', 'tt_lanczos_ritz', 'tt_multiaxpy_counted', 'tt_eigh_batched_static',
-            'tt_bgemm', 'tt_row_norms', 'tt_copy_rows', 'tt_lincomb_rows']
+            'tt_bgemm', 'tt_row_norms', 'tt_copy_rows', 'tt_lincomb_rows', 'tt_bgemm_check', 'tt_rows_check',
+            'tt_lincomb_check']
 
 
 def library() -> ctypes.CDLL:
@@ -373,87 +377,32 @@ class Extents:
         self.check_many([pointer + first], [last - first], [True], what)
 
 
-def _touched(rows, cols, ld):
-    '''The last byte offset a row-major rows x cols operand with row stride ld touches (0 when empty).'''
-    return np.where((rows > 0) & (cols > 0), 8 * ((rows - 1) * ld + cols - 1), 0)
-
-
-def _fields(table, *names):
-    return [table[name].astype(np.int64) for name in names]
-
-
-def _numbered(first, items, what):
-    if np.any(first != np.concatenate([[0], np.cumsum(items)[:-1]])):
-        raise RuntimeError(f'{what}: work items are not numbered consecutively')
+def _checked(function, table, extents, *extra):
+    '''Runs one of the library's table checks; returns the table's work-item count.'''
+    starts, ends = extents.spans()
+    items = _i64(0)
+    check(function(table.ctypes.data if table.size else None, len(table), *extra, starts.ctypes.data, ends.ctypes.data,
+                   len(starts), ctypes.byref(items)), 'table check')
+    return items.value
 
 
 def check_bgemm(table: np.ndarray, extents: Extents) -> int:
-    '''Validates a bgemm table against the buffers it names; returns its tile count.'''
-    if not table.size:
-        return 0
-    m, n, k, lda, ldb, ldc, first = _fields(table, 'm', 'n', 'k', 'lda', 'ldb', 'ldc', 'first')
-    a, b, c, scale, shift = _fields(table, 'a', 'b', 'c', 'col_scale', 'row_shift')
-    if np.any((m < 0) | (n < 0) | (k < 0)):
-        raise RuntimeError('bgemm: negative problem sizes')
-    tiles = (-(-m // 64)) * (-(-n // 64))
-    _numbered(first, tiles, 'bgemm')
-    ta, tb = table['trans_a'] != 0, table['trans_b'] != 0
-    a_rows, a_cols = np.where(ta, k, m), np.where(ta, m, k)
-    b_rows, b_cols = np.where(tb, n, k), np.where(tb, k, n)
-    if np.any((a_cols > lda) | (b_cols > ldb) | (n > ldc)):
-        raise RuntimeError('bgemm: a row stride is shorter than its row')
-    out = (m > 0) & (n > 0)
-    extents.check_many(a, _touched(a_rows, a_cols, lda), (m > 0) & (k > 0), 'bgemm A')
-    extents.check_many(b, _touched(b_rows, b_cols, ldb), (n > 0) & (k > 0), 'bgemm B')
-    extents.check_many(c, _touched(m, n, ldc), out, 'bgemm C')
-    extents.check_many(scale, 8 * (n - 1), out & (scale != 0), 'bgemm column scale')
-    if np.any(out & (shift != 0) & (scale == 0)):
-        raise RuntimeError('bgemm: row shifts need column shifts')
-    extents.check_many(shift, 8 * (m - 1), out & (shift != 0), 'bgemm row shift')
-    return int(tiles.sum())
+    '''Validates a bgemm table against the buffers it names (`tt_bgemm_check`); returns its tile count.'''
+    return _checked(_lib.tt_bgemm_check, table, extents)
 
 
 def check_rows(table: np.ndarray, extents: Extents, source_rows=None, vector_dst: bool = False) -> int:
-    '''Validates a row-problem table; returns its row count.  With an index, `source_rows[i]` is the
-    number of rows problem i's source holds, and the caller has checked that every index is below
-    it.  `vector_dst`: dst receives one value per row (the norms).'''
-    if not table.size:
-        return 0
-    r, c, src_ld, dst_ld, first = _fields(table, 'rows', 'cols', 'src_ld', 'dst_ld', 'first')
-    src, dst, index = _fields(table, 'src', 'dst', 'index')
-    if np.any((r < 0) | (c < 0) | (c > src_ld)):
-        raise RuntimeError('row problem: bad sizes')
-    _numbered(first, r, 'row problem')
-    live = (r > 0) & (c > 0)
-    indexed = index != 0
-    if np.any(live & indexed) and source_rows is None:
-        raise RuntimeError('row problem: an index needs the source row counts')
-    held = np.where(indexed, np.asarray(source_rows if source_rows is not None else r, dtype=np.int64), r)
-    extents.check_many(index, 4 * (r - 1), live & indexed, 'row index')
-    extents.check_many(src, _touched(held, c, src_ld), live, 'row source')
-    if vector_dst:
-        extents.check_many(dst, 8 * (r - 1), live, 'row destination')
-    else:
-        turned = table['transpose'] != 0
-        last = np.where(turned, _touched(c, r, dst_ld), _touched(r, c, dst_ld))
-        extents.check_many(dst, last, live, 'row destination')
-    return int(r.sum())
+    '''Validates a row-problem table (`tt_rows_check`); returns its row count.  With an index,
+    `source_rows[i]` is the number of rows problem i's source holds, and the caller has checked that
+    every index is below it.  `vector_dst`: dst receives one value per row (the norms).'''
+    held = None if source_rows is None else np.ascontiguousarray(source_rows, dtype=np.int64)
+    return _checked(_lib.tt_rows_check, table, extents, None if held is None else held.ctypes.data,
+                    1 if vector_dst else 0)
 
 
 def check_lincomb(table: np.ndarray, extents: Extents) -> int:
-    '''Validates a lincomb table; returns its row count.'''
-    if not table.size:
-        return 0
-    r, c, x_ld, y_ld, dst_ld, first = _fields(table, 'rows', 'cols', 'x_ld', 'y_ld', 'dst_ld', 'first')
-    x, y, dst = _fields(table, 'x', 'y', 'dst')
-    if np.any((r < 0) | (c < 0) | (c > x_ld) | (c > dst_ld) | ((y != 0) & (c > y_ld))):
-        raise RuntimeError('lincomb: bad sizes or row strides')
-    _numbered(first, r, 'lincomb')
-    live = (r > 0) & (c > 0)
-    extents.check_many(x, _touched(r, c, x_ld), live, 'lincomb x')
-    extents.check_many(y, _touched(r, c, y_ld), live & (y != 0), 'lincomb y')
-    extents.check_many(dst, _touched(r, c, dst_ld), live, 'lincomb destination')
-    return int(r.sum())
+    '''Validates a lincomb table (`tt_lincomb_check`); returns its row count.'''
+    return _checked(_lib.tt_lincomb_check, table, extents)
 
 
 def lincomb_rows(table: np.ndarray, address: int, rows: int) -> None:

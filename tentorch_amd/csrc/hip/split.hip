@@ -14,6 +14,7 @@
 // block, column l % 16 (measured on gfx950; the eigensolver's merge products use the same).
 
 #include <algorithm>
+#include <cstdio>
 #include <string>
 
 #include "common.h"
@@ -233,5 +234,123 @@ extern "C" int tt_lincomb_rows(const tt_lincomb_problem *problems, int64_t count
   hipLaunchKernelGGL(lincomb_rows_kernel, dim3(static_cast<uint32_t>((rows + kRowWaves - 1) / kRowWaves)),
                      dim3(64 * kRowWaves), 0, as_stream(stream), problems, static_cast<int>(count), rows);
   TT_LAUNCHED();
+  return 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Host-side table checks.
+
+namespace {
+
+struct Spans {
+  const int64_t *starts, *ends;
+  int64_t count;
+  // bytes [pointer, pointer + last] lie in one span
+  bool holds(const void *pointer, int64_t last) const {
+    const int64_t lo = static_cast<int64_t>(reinterpret_cast<uintptr_t>(pointer)), hi = lo + last;
+    int64_t a = 0, b = count;   // the last span starting at or before lo
+    while (a < b) {
+      const int64_t mid = (a + b) / 2;
+      if (starts[mid] <= lo) a = mid + 1; else b = mid;
+    }
+    return a > 0 && hi < ends[a - 1];
+  }
+};
+
+// the last byte offset a row-major rows x cols operand with row stride ld touches
+int64_t touched(int64_t rows, int64_t cols, int64_t ld) { return 8 * ((rows - 1) * ld + cols - 1); }
+
+int outside(const char *what, const void *pointer) {
+  char text[64];
+  std::snprintf(text, sizeof text, "%p", pointer);
+  return fail(std::string(what) + ": the operand at " + text + " reaches outside every buffer of the launch");
+}
+
+int check_spans(const int64_t *starts, const int64_t *ends, int64_t spans, const char *what) {
+  if (spans < 0 || (spans > 0 && (!starts || !ends))) return fail(std::string(what) + ": bad buffer list");
+  for (int64_t i = 0; i < spans; ++i)
+    if (ends[i] < starts[i] || (i > 0 && starts[i] < ends[i - 1]))
+      return fail(std::string(what) + ": buffer ranges unsorted or overlapping");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int tt_bgemm_check(const tt_bgemm_problem *problems, int64_t count, const int64_t *starts,
+                              const int64_t *ends, int64_t spans, int64_t *items) {
+  if (int rc = check_table(problems, count, "bgemm")) return rc;
+  if (int rc = check_spans(starts, ends, spans, "bgemm")) return rc;
+  const Spans in{starts, ends, spans};
+  int64_t tiles = 0;
+  for (int64_t i = 0; i < count; ++i) {
+    const tt_bgemm_problem &p = problems[i];
+    const int64_t m = p.m, n = p.n, k = p.k;
+    if (m < 0 || n < 0 || k < 0) return fail("bgemm: negative problem sizes");
+    if (p.first != tiles) return fail("bgemm: work items are not numbered consecutively");
+    const int64_t a_rows = p.trans_a ? k : m, a_cols = p.trans_a ? m : k;
+    const int64_t b_rows = p.trans_b ? n : k, b_cols = p.trans_b ? k : n;
+    if (a_cols > p.lda || b_cols > p.ldb || n > p.ldc) return fail("bgemm: a row stride is shorter than its row");
+    if (m > 0 && k > 0 && !in.holds(p.a, touched(a_rows, a_cols, p.lda))) return outside("bgemm A", p.a);
+    if (n > 0 && k > 0 && !in.holds(p.b, touched(b_rows, b_cols, p.ldb))) return outside("bgemm B", p.b);
+    if (m > 0 && n > 0) {
+      if (!in.holds(p.c, touched(m, n, p.ldc))) return outside("bgemm C", p.c);
+      if (p.col_scale && !in.holds(p.col_scale, 8 * (n - 1))) return outside("bgemm column scale", p.col_scale);
+      if (p.row_shift && !p.col_scale) return fail("bgemm: row shifts need column shifts");
+      if (p.row_shift && !in.holds(p.row_shift, 8 * (m - 1))) return outside("bgemm row shift", p.row_shift);
+    }
+    tiles += ((m + kTile - 1) / kTile) * ((n + kTile - 1) / kTile);
+  }
+  if (items) *items = tiles;
+  return 0;
+}
+
+extern "C" int tt_rows_check(const tt_rows_problem *problems, int64_t count, const int64_t *source_rows, int vector_dst,
+                             const int64_t *starts, const int64_t *ends, int64_t spans, int64_t *items) {
+  if (int rc = check_table(problems, count, "row problem")) return rc;
+  if (int rc = check_spans(starts, ends, spans, "row problem")) return rc;
+  const Spans in{starts, ends, spans};
+  int64_t rows = 0;
+  for (int64_t i = 0; i < count; ++i) {
+    const tt_rows_problem &p = problems[i];
+    const int64_t r = p.rows, c = p.cols;
+    if (r < 0 || c < 0 || c > p.src_ld) return fail("row problem: bad sizes");
+    if (p.first != rows) return fail("row problem: work items are not numbered consecutively");
+    if (r > 0 && c > 0) {
+      int64_t held = r;
+      if (p.index) {
+        if (!source_rows) return fail("row problem: an index needs the source row counts");
+        held = source_rows[i];
+        if (!in.holds(p.index, 4 * (r - 1))) return outside("row index", p.index);
+      }
+      if (!in.holds(p.src, touched(held, c, p.src_ld))) return outside("row source", p.src);
+      const int64_t last = vector_dst ? 8 * (r - 1) : (p.transpose ? touched(c, r, p.dst_ld) : touched(r, c, p.dst_ld));
+      if (!in.holds(p.dst, last)) return outside("row destination", p.dst);
+    }
+    rows += r;
+  }
+  if (items) *items = rows;
+  return 0;
+}
+
+extern "C" int tt_lincomb_check(const tt_lincomb_problem *problems, int64_t count, const int64_t *starts,
+                                const int64_t *ends, int64_t spans, int64_t *items) {
+  if (int rc = check_table(problems, count, "lincomb")) return rc;
+  if (int rc = check_spans(starts, ends, spans, "lincomb")) return rc;
+  const Spans in{starts, ends, spans};
+  int64_t rows = 0;
+  for (int64_t i = 0; i < count; ++i) {
+    const tt_lincomb_problem &p = problems[i];
+    const int64_t r = p.rows, c = p.cols;
+    if (r < 0 || c < 0 || c > p.x_ld || c > p.dst_ld || (p.y && c > p.y_ld))
+      return fail("lincomb: bad sizes or row strides");
+    if (p.first != rows) return fail("lincomb: work items are not numbered consecutively");
+    if (r > 0 && c > 0) {
+      if (!in.holds(p.x, touched(r, c, p.x_ld))) return outside("lincomb x", p.x);
+      if (p.y && !in.holds(p.y, touched(r, c, p.y_ld))) return outside("lincomb y", p.y);
+      if (!in.holds(p.dst, touched(r, c, p.dst_ld))) return outside("lincomb destination", p.dst);
+    }
+    rows += r;
+  }
+  if (items) *items = rows;
   return 0;
 }
