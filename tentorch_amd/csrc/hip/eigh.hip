@@ -271,16 +271,23 @@ __global__ __launch_bounds__(kStepThreads) void tridiag_step_kernel(const Batch 
 }
 
 // 1''. The last kLdsOrder steps of every matrix in one launch, one workgroup per matrix: the
-//      trailing matrix (order <= kLdsOrder, rows padded to kLdsStride against bank conflicts) is
-//      loaded into LDS with the state the stepped launches left (v, p and tau of step k0-1, whose
-//      rank-2 update is still pending), and the remaining steps run there with no memory traffic
-//      but the reflectors and scalars they produce.  In the trailing pass eight lanes share a row
-//      and sum their partial products by shuffles.  With k0 = 0 it is the whole reduction of a
-//      batch of small matrices.
+//      trailing matrix (order <= kLdsOrder) is loaded into LDS with the state the stepped launches
+//      left (v, p and tau of step k0-1, whose rank-2 update is still pending), and the remaining
+//      steps run there with no memory traffic but the reflectors and scalars they produce.  A step
+//      takes two barriers: wave 0 forms column kk with the pending update and the step's reflector
+//      (its two sums are wave reductions over the column's rows, two per lane); then every thread
+//      applies the pending update to the trailing rows (eight lanes a row, their partial products
+//      summed by shuffles) and forms p.  The pending update is (v, p, tau) of the previous step in
+//      LDS, double-buffered by step parity; each wave forms w = p - tau/2 (p . v) v where it needs
+//      it, with p . v by a wave reduction of its own.  Rows are padded to 132 doubles, which puts
+//      the 8 rows x 8 columns a wave touches in the trailing pass on distinct bank pairs (2-way,
+//      the least a wave's 64 doubles can take; 129 gave 8-way).  With k0 = 0 it is the whole
+//      reduction of a batch of small matrices.
 
 constexpr int kLdsOrder = 128;
-constexpr int kLdsStride = kLdsOrder + 1;
+constexpr int kLdsStride = kLdsOrder + 4;
 static_assert(kLdsOrder * 8 == kTriThreads, "eight lanes per trailing row");
+static_assert(kLdsOrder <= 2 * 64, "wave 0 holds a column in two rows per lane");
 
 __global__ __launch_bounds__(kTriThreads) void tridiag_lds_kernel(const Batch *__restrict__ batch,
                                                                    double *__restrict__ matrices,
@@ -293,80 +300,103 @@ __global__ __launch_bounds__(kTriThreads) void tridiag_lds_kernel(const Batch *_
   double *diag = work + pr.work + 3 * int64_t(n) * n, *off = diag + n, *tau = off + n;
   const double *vbuf = tau + n, *pbuf = vbuf + 2 * n;
   __shared__ double T[kLdsOrder * kLdsStride];
-  __shared__ double vp[kLdsOrder], wp[kLdsOrder], vc[kLdsOrder], pc[kLdsOrder];
-  __shared__ double red[kTriThreads / 64];
-  __shared__ double s_alpha, s_diag;
-  const int t = threadIdx.x;
+  __shared__ double vl[2][kLdsOrder], pl[2][kLdsOrder], tl[2];
+  const int t = threadIdx.x, lane = t & 63;
+  const bool first_wave = t < 64;
   for (int e = t; e < m * m; e += kTriThreads) {
     const int i = e % m, j = e / m;
     T[i * kLdsStride + j] = A[(k0 + i) + int64_t(k0 + j) * n];
   }
-  // the pending update of step k0-1 (none before the first step); its reflector goes to column k0-1
-  double v = 0.0, p = 0.0;
-  if (t < m && k0 > 0) {
-    v = vbuf[((k0 - 1) & 1) * n + k0 + t];
-    p = pbuf[((k0 - 1) & 1) * n + k0 + t];
-    A[(k0 + t) + int64_t(k0 - 1) * n] = v;
+  // the pending update of step k0-1 (none before the first step) in parity 1; its reflector goes
+  // to column k0-1
+  if (t < kLdsOrder) {
+    const bool on = k0 > 0 && t < m;
+    const double v = on ? vbuf[((k0 - 1) & 1) * n + k0 + t] : 0.0;
+    vl[1][t] = v;
+    pl[1][t] = on ? pbuf[((k0 - 1) & 1) * n + k0 + t] : 0.0;
+    if (on) A[(k0 + t) + int64_t(k0 - 1) * n] = v;
   }
-  const double tprev = k0 > 0 ? tau[k0 - 1] : 0.0;
-  const double pv0 = tri_block_sum(p * v, red);
-  if (t < m) {
-    vp[t] = v;
-    wp[t] = p - 0.5 * tprev * pv0 * v;
-  }
+  if (t == 0) tl[1] = k0 > 0 ? tau[k0 - 1] : 0.0;
   __syncthreads();
+  // half = tau/2 (p . v) of the pending step, by every wave for itself (rows lane, lane + 64)
+  auto half_of = [&](int buf, int from) {
+    double partial = 0.0;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int r = lane + 64 * s;
+      if (r >= from && r < m) partial += pl[buf][r] * vl[buf][r];
+    }
+    return 0.5 * tl[buf] * wave_sum(partial);
+  };
+  double half = half_of(1, 0);
   for (int kk = 0; kk + 1 < m; ++kk) {
-    const int k = k0 + kk;
-    // (a) column kk (rows kk..) with the pending update, and the reflector of this step
-    double x = 0.0;
-    if (t >= kk && t < m) x = T[t * kLdsStride + kk] - (vp[t] * wp[kk] + wp[t] * vp[kk]);
-    if (t == kk) s_diag = x;
-    if (t == kk + 1) s_alpha = x;
-    const double sigma = tri_block_sum(t >= kk + 2 && t < m ? x * x : 0.0, red);
-    const double alpha = s_alpha;
-    double beta = alpha, tk = 0.0, scale = 0.0;
-    if (sigma != 0.0) {
-      const double norm = sqrt(alpha * alpha + sigma);
-      beta = alpha >= 0.0 ? -norm : norm;
-      tk = (beta - alpha) / beta;
-      scale = 1.0 / (alpha - beta);
-    }
-    if (t < m) {
-      const double vnew = t == kk + 1 ? 1.0 : (t >= kk + 2 ? x * scale : 0.0);
-      vc[t] = vnew;
-      if (t >= kk + 1) A[(k0 + t) + int64_t(k) * n] = vnew;
-    }
-    if (t == 0) {
-      diag[k] = s_diag;
-      off[k] = beta;
-      tau[k] = tk;
+    const int k = k0 + kk, par = kk & 1, prev = par ^ 1;
+    // (a) wave 0: column kk (rows kk..) with the pending update, and the reflector of this step
+    if (first_wave) {
+      const double vk = vl[prev][kk], wk = pl[prev][kk] - half * vk;
+      double x[2], partial = 0.0;
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const int r = lane + 64 * s;
+        const double vr = r < m ? vl[prev][r] : 0.0, wr = r < m ? pl[prev][r] - half * vr : 0.0;
+        x[s] = r >= kk && r < m ? T[r * kLdsStride + kk] - (vr * wk + wr * vk) : 0.0;
+        if (r >= kk + 2) partial += x[s] * x[s];
+      }
+      const double sigma = wave_sum(partial);
+      const double dk = readlane_double(kk < 64 ? x[0] : x[1], kk & 63);
+      const double alpha = readlane_double(kk + 1 < 64 ? x[0] : x[1], (kk + 1) & 63);
+      double beta = alpha, tk = 0.0, scale = 0.0;
+      if (sigma != 0.0) {
+        const double norm = sqrt(alpha * alpha + sigma);
+        beta = alpha >= 0.0 ? -norm : norm;
+        tk = (beta - alpha) / beta;
+        scale = 1.0 / (alpha - beta);
+      }
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const int r = lane + 64 * s;
+        if (r < m) {
+          const double vnew = r == kk + 1 ? 1.0 : (r >= kk + 2 ? x[s] * scale : 0.0);
+          vl[par][r] = vnew;
+          if (r >= kk + 1) A[(k0 + r) + int64_t(k) * n] = vnew;
+        }
+      }
+      if (lane == 0) {
+        tl[par] = tk;
+        diag[k] = dk;
+        off[k] = beta;
+        tau[k] = tk;
+      }
     }
     __syncthreads();
     // (b) rows and columns kk+1..: the pending update, and p = tau T v
+    const double tk = tl[par];
     const int row = kk + 1 + (t >> 3), sub = t & 7;
     double acc = 0.0;
     if (row < m) {
-      const double vpr = vp[row], wpr = wp[row];
+      const double vpr = vl[prev][row], wpr = pl[prev][row] - half * vpr;
       for (int j = kk + 1 + sub; j < m; j += 8) {
-        const double y = T[row * kLdsStride + j] - (vpr * wp[j] + wpr * vp[j]);
+        const double vj = vl[prev][j], wj = pl[prev][j] - half * vj;
+        const double y = T[row * kLdsStride + j] - (vpr * wj + wpr * vj);
         T[row * kLdsStride + j] = y;
-        acc += y * vc[j];
+        acc += y * vl[par][j];
       }
     }
     acc += __shfl_xor(acc, 1, 64);
     acc += __shfl_xor(acc, 2, 64);
     acc += __shfl_xor(acc, 4, 64);
-    if (sub == 0 && row < m) pc[row] = tk * acc;
+    if (sub == 0 && row < m) pl[par][row] = tk * acc;
     __syncthreads();
-    // (c) w = p - tau/2 (p . v) v becomes the pending update
-    const double pv = tri_block_sum(t >= kk + 1 && t < m ? pc[t] * vc[t] : 0.0, red);
-    if (t >= kk + 1 && t < m) {
-      wp[t] = pc[t] - 0.5 * tk * pv * vc[t];
-      vp[t] = vc[t];
-    }
-    __syncthreads();
+    // (c) this step's update becomes the pending one
+    half = half_of(par, kk + 1);
   }
-  if (t == 0) diag[n - 1] = T[(m - 1) * kLdsStride + (m - 1)] - 2.0 * vp[m - 1] * wp[m - 1];
+  // the last diagonal entry, with the pending update
+  if (t == 0) {
+    const int last = m - 1, buf = (m - 2) & 1;
+    const double v = m >= 2 ? vl[buf][last] : vl[1][0];
+    const double w = m >= 2 ? pl[buf][last] - half * v : pl[1][0] - half * v;
+    diag[n - 1] = T[last * kLdsStride + last] - 2.0 * v * w;
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
