@@ -409,20 +409,6 @@ def lincomb_rows(table: np.ndarray, address: int, rows: int) -> None:
     check(_lib.tt_lincomb_rows(address, len(table), rows, stream()), 'lincomb rows')
 
 
-def upload(*tables: np.ndarray):
-    '''The tables in one device buffer (one copy from pinned memory, queued on the current stream);
-    returns the buffer (keep it alive until the launches are queued) and each table's address.'''
-    sizes = [t.nbytes for t in tables]
-    offsets = np.concatenate([[0], np.cumsum([(n + 255) // 256 * 256 for n in sizes])]).astype(np.int64)
-    host = torch.empty(max(int(offsets[-1]), 1), dtype=torch.uint8, pin_memory=True)
-    view = host.numpy()
-    for t, at in zip(tables, offsets):
-        view[at:at + t.nbytes] = t.view(np.uint8).reshape(-1)
-    device = host.to('cuda', non_blocking=True)
-    base = device.data_ptr()
-    return device, [base + int(at) for at in offsets[:-1]]
-
-
 def bgemm(table: np.ndarray, address: int, tiles: int) -> None:
     check(_lib.tt_bgemm(address, len(table), tiles, stream()), 'bgemm')
 

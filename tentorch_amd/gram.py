@@ -35,64 +35,85 @@ import torch
 from . import _native
 
 
-def _table(rows, dtype, count_field):
-    '''A problem table from tuples (the `first` field numbered here), and its item count.'''
-    table = np.array(rows, dtype=dtype) if rows else np.zeros(0, dtype=dtype)
-    if count_field == 'tiles':
-        items = (-(-table['m'] // 64)) * (-(-table['n'] // 64))
-    else:
-        items = table['rows'].astype(np.int64)
-    table['first'] = np.concatenate([[0], np.cumsum(items)[:-1]]) if table.size else table['first']
-    return table, int(items.sum())
+def _offsets(sizes) -> np.ndarray:
+    '''Exclusive prefix sums (one more entry: the total).'''
+    return np.concatenate([[0], np.cumsum(np.asarray(sizes, dtype=np.int64))]).astype(np.int64)
 
 
-def _gemm(a, b, c, m, n, k, lda, ldb, ldc, trans_a=0, trans_b=0, alpha=1.0, beta=0.0, scale=0, shift=0):
-    return (a, b, c, scale, shift, 0, m, n, k, lda, ldb, ldc, trans_a, trans_b, alpha, beta)
+def _bgemm(count: int, **fields):
+    '''A bgemm problem table of `count` problems, fields given as scalars or arrays; numbered by
+    64 x 64 tiles.  Returns (table, tiles).'''
+    table = np.zeros(count, dtype=_native.BGEMM_PROBLEM)
+    table['alpha'] = 1.0
+    for name, value in fields.items():
+        table[name] = value
+    tiles = (-(-table['m'].astype(np.int64) // 64)) * (-(-table['n'].astype(np.int64) // 64))
+    table['first'] = _offsets(tiles)[:-1]
+    return table, int(tiles.sum())
 
 
-def _rows(src, dst, rows, cols, src_ld, dst_ld, transpose=0, index=0):
-    return (src, dst, index, 0, rows, cols, src_ld, dst_ld, transpose, 0)
+def _rowwise(dtype, count: int, **fields):
+    '''A row-problem or lincomb table, numbered by rows.  Returns (table, rows).'''
+    table = np.zeros(count, dtype=dtype)
+    for name, value in fields.items():
+        table[name] = value
+    rows = table['rows'].astype(np.int64)
+    table['first'] = _offsets(rows)[:-1]
+    return table, int(rows.sum())
 
 
-def _lincomb(x, y, dst, rows, cols, x_ld, y_ld, dst_ld, alpha, beta, gamma):
-    return (x, y, dst, 0, rows, cols, x_ld, y_ld, dst_ld, 0, alpha, beta, gamma)
+class _Stage:
+    '''The host arrays and launch tables of one phase in one device buffer: the device range is
+    allocated first, so tables can name the addresses of what is staged with them; the host side
+    is pinned memory, sent in one copy behind the phase's earlier work.  On a CPU device the buffer
+    is host memory and the copy a plain one.'''
 
+    def __init__(self, device, sizes):
+        self.offsets = _offsets([(int(n) + 255) // 256 * 256 for n in sizes])
+        total = max(int(self.offsets[-1]), 1)
+        self.device_buffer = torch.empty(total, dtype=torch.uint8, device=device)
+        self.host = torch.empty(total, dtype=torch.uint8, pin_memory=device.type == 'cuda')
+        self.view = self.host.numpy()
+        self.base = self.device_buffer.data_ptr()
 
-# rotations that decouple a tail from its head before the tail's Gram pass: one leaves a coupling
-# of order |X|^2 relative, |X| being about eps x (largest / boundary value)^2 / (1 - gap^-2)
-DECOUPLING = 1
+    def address(self, part: int) -> int:
+        return self.base + int(self.offsets[part])
+
+    def put(self, part: int, array: np.ndarray) -> None:
+        at = int(self.offsets[part])
+        self.view[at:at + array.nbytes] = np.ascontiguousarray(array).view(np.uint8).reshape(-1)
+
+    def send(self) -> None:
+        self.device_buffer.copy_(self.host, non_blocking=True)
 
 
 class _Launches:
-    '''The launches of one phase: tables collected, checked against `extents`, uploaded together.'''
+    '''The launches of one phase: each table checked against `extents` (every buffer its operands may
+    touch), staged with the phase's host arrays, and launched from its device address.'''
 
     def __init__(self, extents: '_native.Extents'):
         self.extents = extents
         self.steps = []
 
-    def gemm(self, problems):
-        table, tiles = _table(problems, _native.BGEMM_PROBLEM, 'tiles')
-        if _native.check_bgemm(table, self.extents) != tiles:
-            raise RuntimeError('bgemm: tile count')
-        self.steps.append(('gemm', table, tiles))
+    def add(self, kind, table_items, source_rows=None):
+        table, items = table_items
+        if kind == 'gemm':
+            checked = _native.check_bgemm(table, self.extents)
+        elif kind == 'lincomb':
+            checked = _native.check_lincomb(table, self.extents)
+        else:
+            checked = _native.check_rows(table, self.extents, source_rows, vector_dst=kind == 'norms')
+        if checked != items:
+            raise RuntimeError(f'{kind}: work-item count')
+        self.steps.append((kind, table, items))
 
-    def lincomb(self, problems):
-        table, rows = _table(problems, _native.LINCOMB_PROBLEM, 'rows')
-        if _native.check_lincomb(table, self.extents) != rows:
-            raise RuntimeError('lincomb: row count')
-        self.steps.append(('lincomb', table, rows))
-
-    def rows(self, kind, problems, source_rows=None):
-        table, rows = _table(problems, _native.ROWS_PROBLEM, 'rows')
-        if _native.check_rows(table, self.extents, source_rows, vector_dst=kind == 'norms') != rows:
-            raise RuntimeError('row problems: row count')
-        self.steps.append((kind, table, rows))
-
-    def upload(self):
-        '''Uploads every table; returns the device buffer (keep it until the launches are queued).'''
-        buffer, addresses = _native.upload(*[table for _, table, _ in self.steps])
-        self.addresses = addresses
-        return buffer
+    def stage(self, stage: '_Stage', first_part: int) -> None:
+        '''Puts the tables into `stage` from part `first_part` on, and sends it.'''
+        self.addresses = []
+        for i, (_, table, _) in enumerate(self.steps):
+            stage.put(first_part + i, table)
+            self.addresses.append(stage.address(first_part + i))
+        stage.send()
 
     def run(self, first: int = 0, last: int = None):
         for (kind, table, items), address in list(zip(self.steps, self.addresses))[first:last]:
@@ -108,16 +129,20 @@ class _Launches:
                 _native.copy_rows(table, address, items)
 
 
-def _uploaded(array: np.ndarray, device) -> torch.Tensor:
-    '''A host array on the device, copied from pinned memory without waiting.'''
-    host = torch.from_numpy(np.ascontiguousarray(array))
-    if device.type == 'cuda':
-        return host.pin_memory().to(device, non_blocking=True)
-    return host.clone()
+def _extents_of(*tensors) -> '_native.Extents':
+    extents = _native.Extents()
+    for t in tensors:
+        extents.add(t)
+    return extents
 
 
-def _ptr(t: torch.Tensor, offset: int = 0) -> int:
-    return t.data_ptr() + 8 * int(offset)
+def _doubles(device, count: int) -> torch.Tensor:
+    return torch.empty(max(int(count), 1), dtype=torch.float64, device=device)
+
+
+# rotations that decouple a tail from its head before the tail's Gram pass: one leaves a coupling
+# of order |X|^2 relative, |X| being about eps x (largest / boundary value)^2 / (1 - gap^-2)
+DECOUPLING = 1
 
 
 # the tail of a pass starts at the widest gap among the values within these factors of the level
@@ -170,61 +195,59 @@ def choose_tails(norms: List[np.ndarray], ranks: List[int], first: List[int], fl
 class GramSplit:
     '''The singular triplets of the blocks `blocks` (2-D row-major device views, FP64) for a split
     that keeps the left factor isometric (`left_isometric`, absorb 2) or the right one (absorb 1).
-    `factor(cut, reliable)` runs the passes, `cut(values)` being the host ranking's floor, and
-    returns each block's singular values (host, descending); `assemble` then writes the kept
+    `factor(cut, reliable, resolved)` runs the passes, `cut(values)` being the host ranking's floor,
+    and returns each block's singular values (host, descending); `assemble` then writes the kept
     triplets into the new tensors' blocks.'''
 
     def __init__(self, blocks, left_isometric: bool):
         self.blocks = blocks
         self.left = left_isometric
-        self.g = [b.shape[0] if left_isometric else b.shape[1] for b in blocks]
-        self.n = [b.shape[1] if left_isometric else b.shape[0] for b in blocks]
-        self.r = [min(g, n) for g, n in zip(self.g, self.n)]
-        self.ld = [n + g for g, n in zip(self.g, self.n)]
-        device = blocks[0].device
-        self.pq_at = np.concatenate([[0], np.cumsum([g * ld for g, ld in zip(self.g, self.ld)])]).astype(np.int64)
-        self.pq = torch.empty(max(int(self.pq_at[-1]), 1), dtype=torch.float64, device=device)
-        self.device = device
+        rows = np.array([b.shape[0] for b in blocks], dtype=np.int64)
+        cols = np.array([b.shape[1] for b in blocks], dtype=np.int64)
+        self.rows, self.cols = rows, cols
+        self.g = rows.copy() if left_isometric else cols.copy()
+        self.n = cols.copy() if left_isometric else rows.copy()
+        self.r = np.minimum(self.g, self.n)
+        self.ld = self.n + self.g
+        self.device = blocks[0].device
+        self.pq_at = _offsets(self.g * self.ld)
+        self.pq = _doubles(self.device, self.pq_at[-1])
+        self.pq_base = self.pq.data_ptr()
+        self.block_ptr = np.array([b.data_ptr() for b in blocks], dtype=np.int64)
+
+    def _row_ptr(self, b, row):
+        '''Address of row `row` of block b's [P | Q] (vectorised over arrays).'''
+        return self.pq_base + 8 * (self.pq_at[b] + row * self.ld[b])
 
     # -- pass 1
     def _first_pass(self):
-        g, n, ld, r = self.g, self.n, self.ld, self.r
-        sq_at = np.concatenate([[0], np.cumsum([x * x for x in g])]).astype(np.int64)
-        grams = torch.empty(max(int(sq_at[-1]), 1), dtype=torch.float64, device=self.device)
-        vectors = torch.empty_like(grams)
-        g_at = np.concatenate([[0], np.cumsum(g)]).astype(np.int64)
-        norms = torch.empty(max(int(g_at[-1]), 1), dtype=torch.float64, device=self.device)
-        extents = _native.Extents().add(self.pq).add(grams).add(vectors).add(norms)
-        gram_problems, p_problems, q_copies, norm_rows = [], [], [], []
-        for b, block in enumerate(self.blocks):
-            extents.add(block)
-            rows_b, cols_b = block.shape
-            base = block.data_ptr()
-            gb, nb = g[b], n[b]
-            # -C C^T with C = B (B B^T) or C = B^T (B^T B); B is stored rows_b x cols_b
-            if self.left:
-                gram_problems.append(_gemm(base, base, _ptr(grams, sq_at[b]), gb, gb, cols_b, cols_b, cols_b, gb,
-                                           trans_a=0, trans_b=1, alpha=-1.0))
-            else:
-                gram_problems.append(_gemm(base, base, _ptr(grams, sq_at[b]), gb, gb, rows_b, cols_b, cols_b, gb,
-                                           trans_a=1, trans_b=0, alpha=-1.0))
-            # P = Q C into [P | Q][:, :n]
-            p_problems.append(_gemm(_ptr(vectors, sq_at[b]), base, _ptr(self.pq, self.pq_at[b]), gb, nb, gb, gb,
-                                    cols_b, ld[b], trans_a=0, trans_b=0 if self.left else 1))
-            q_copies.append(_rows(_ptr(vectors, sq_at[b]), _ptr(self.pq, self.pq_at[b] + nb), gb, gb, gb, ld[b]))
-            norm_rows.append(_rows(_ptr(self.pq, self.pq_at[b]), _ptr(norms, g_at[b]), gb, nb, ld[b], 1))
+        g, n, ld, count = self.g, self.n, self.ld, len(self.blocks)
+        sq_at, g_at = _offsets(g * g), _offsets(g)
+        grams, vectors, norms = _doubles(self.device, sq_at[-1]), _doubles(self.device, sq_at[-1]), \
+            _doubles(self.device, g_at[-1])
+        extents = _extents_of(self.pq, grams, vectors, norms, *self.blocks)
+        gram_at = grams.data_ptr() + 8 * sq_at[:-1]
+        vec_at = vectors.data_ptr() + 8 * sq_at[:-1]
+        p_at = self._row_ptr(np.arange(count), 0)
         launches = _Launches(extents)
-        launches.gemm(gram_problems)
-        launches.gemm(p_problems)
-        launches.rows('copy', q_copies)
-        launches.rows('norms', norm_rows)
-        keep = launches.upload()
+        # -C C^T with C = B (B B^T) or C = B^T (B^T B); B is stored rows x cols
+        launches.add('gemm', _bgemm(count, a=self.block_ptr, b=self.block_ptr, c=gram_at, m=g, n=g,
+                                    k=self.cols if self.left else self.rows, lda=self.cols, ldb=self.cols, ldc=g,
+                                    trans_a=0 if self.left else 1, trans_b=1 if self.left else 0, alpha=-1.0))
+        # P = Q C into [P | Q][:, :n], Q beside it, the row norms of P
+        launches.add('gemm', _bgemm(count, a=vec_at, b=self.block_ptr, c=p_at, m=g, n=n, k=g, lda=g, ldb=self.cols,
+                                    ldc=ld, trans_b=0 if self.left else 1))
+        launches.add('copy', _rowwise(_native.ROWS_PROBLEM, count, src=vec_at, dst=p_at + 8 * n, rows=g, cols=g,
+                                      src_ld=g, dst_ld=ld))
+        launches.add('norms', _rowwise(_native.ROWS_PROBLEM, count, src=p_at, dst=norms.data_ptr() + 8 * g_at[:-1],
+                                       rows=g, cols=n, src_ld=ld, dst_ld=1))
+        stage = _Stage(self.device, [t.nbytes for _, t, _ in launches.steps])
+        launches.stage(stage, 0)
         launches.run(0, 1)
-        _native.eigh_batched(grams, g, sq_at[:-1].tolist(), vectors=vectors)
+        _native.eigh_batched(grams, g.tolist(), sq_at[:-1].tolist(), vectors=vectors)
         launches.run(1)
         host = norms.cpu().numpy()
-        del keep
-        return [host[g_at[b]:g_at[b + 1]].copy() for b in range(len(g))]
+        return [host[g_at[b]:g_at[b + 1]].copy() for b in range(count)]
 
     # -- a further pass over the tails
     def _tail_pass(self, tails, values):
@@ -247,107 +270,96 @@ class GramSplit:
         orthogonal to |X|^4 / 24 (|X| is below about 1e-5 at a tail boundary chosen at a gap).  The
         heads go back in place; the tails too, unless `last`, when they stay in the returned buffer
         for the diagonalisation.'''
-        n, ld, g = self.n, self.ld, self.g
-        hh = [h - f for _, f, h in heads]
-        t = [g[b] - h for b, _, h in heads]
-        w = [ld[b] for b, _, _ in heads]
-
-        def offsets(sizes):
-            return np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
-
-        x_at, sq_at = offsets([a * c for a, c in zip(t, hh)]), offsets([a * a for a in hh])
-        hw_at, tw_at = offsets([a * c for a, c in zip(hh, w)]), offsets([a * c for a, c in zip(t, w)])
-        head_at, tail_at = offsets(hh), offsets(t)
-
-        def buffer(size):
-            return torch.empty(max(int(size), 1), dtype=torch.float64, device=self.device)
-
-        x, u, hn, m, tn = buffer(x_at[-1]), buffer(hw_at[-1]), buffer(hw_at[-1]), buffer(hw_at[-1]), buffer(tw_at[-1])
-        k_mat, a_mat, b_mat = (buffer(sq_at[-1]) for _ in range(3))
-        scale = _uploaded(np.concatenate([values[b][f:h] ** 2 for b, f, h in heads]), self.device)
-        shift = _uploaded(np.concatenate([values[b][h:] ** 2 for b, f, h in heads]), self.device)
-        extents = _native.Extents().add(self.pq).add(scale).add(shift)
-        for buf in (x, u, hn, m, tn, k_mat, a_mat, b_mat):
-            extents.add(buf)
-        steps = {k: [] for k in ('x', 'k', 'init', 'first', 'second', 'tail', 'back')}
-        for i, (b, f, h) in enumerate(heads):
-            base = self.pq_at[b]
-            H, T = _ptr(self.pq, base + f * ld[b]), _ptr(self.pq, base + h * ld[b])
-            X, U, Hn, M, Tn = (_ptr(x, x_at[i]), _ptr(u, hw_at[i]), _ptr(hn, hw_at[i]), _ptr(m, hw_at[i]),
-                               _ptr(tn, tw_at[i]))
-            K, A, B = (_ptr(buf, sq_at[i]) for buf in (k_mat, a_mat, b_mat))
-            e, q = hh[i], t[i]
-            steps['x'].append(_gemm(T, H, X, q, e, n[b], ld[b], ld[b], e, trans_b=1, scale=_ptr(scale, head_at[i]),
-                                    shift=_ptr(shift, tail_at[i])))
-            steps['k'].append(_gemm(X, X, K, e, e, q, e, e, e, trans_a=1))
-            steps['k'].append(_gemm(X, T, U, e, w[i], q, e, ld[b], w[i], trans_a=1))
-            steps['init'].append(_lincomb(K, 0, A, e, e, e, 0, e, -0.5, 0.0, 1.0))
-            steps['init'].append(_lincomb(K, 0, B, e, e, e, 0, e, -1.0 / 6.0, 0.0, 1.0))
-            steps['init'].append(_lincomb(U, 0, M, e, w[i], w[i], 0, w[i], 0.5, 0.0, 0.0))
-            steps['init'].append(_lincomb(T, 0, Tn, q, w[i], ld[b], 0, w[i], 1.0, 0.0, 0.0))
-            steps['first'].append(_gemm(A, H, Hn, e, w[i], e, e, ld[b], w[i]))
-            steps['first'].append(_gemm(B, H, M, e, w[i], e, e, ld[b], w[i], beta=1.0))
-            steps['second'].append(_gemm(B, U, Hn, e, w[i], e, e, w[i], w[i], beta=1.0))
-            steps['tail'].append(_gemm(X, M, Tn, q, w[i], e, e, w[i], w[i], alpha=-1.0, beta=1.0))
-            steps['back'].append(_lincomb(Hn, 0, H, e, w[i], w[i], 0, ld[b], 1.0, 0.0, 0.0))
-            if not last:
-                steps['back'].append(_lincomb(Tn, 0, T, q, w[i], w[i], 0, ld[b], 1.0, 0.0, 0.0))
+        b = np.array([x[0] for x in heads], dtype=np.int64)
+        f = np.array([x[1] for x in heads], dtype=np.int64)
+        h = np.array([x[2] for x in heads], dtype=np.int64)
+        count = len(heads)
+        e, q, w, n = h - f, self.g[b] - h, self.ld[b], self.n[b]
+        # one work buffer: X (q x e), U, H', M (e x w), T' (q x w), K, A, B (e x e), per head
+        sizes = np.stack([q * e, e * w, e * w, e * w, q * w, e * e, e * e, e * e])
+        at = _offsets(sizes.T.reshape(-1)).reshape(-1)[:-1].reshape(count, 8)
+        work = _doubles(self.device, sizes.sum())
+        ptr = work.data_ptr() + 8 * at
+        X, U, Hn, M, Tn, K, A, B = (ptr[:, j] for j in range(8))
+        H, T = self._row_ptr(b, f), self._row_ptr(b, h)
+        scale = np.concatenate([values[bb][ff:hh] ** 2 for bb, ff, hh in heads])
+        shift = np.concatenate([values[bb][hh:] ** 2 for bb, ff, hh in heads])
+        head_at, tail_at = _offsets(e)[:-1], _offsets(q)[:-1]
+        copies = count if last else 2 * count
+        counts = [count, 2 * count, 4 * count, 2 * count, count, count, copies]
+        kinds = [(_native.BGEMM_PROBLEM, c) for c in counts[:2]] + [(_native.LINCOMB_PROBLEM, counts[2])] + \
+            [(_native.BGEMM_PROBLEM, c) for c in counts[3:6]] + [(_native.LINCOMB_PROBLEM, counts[6])]
+        stage = _Stage(self.device, [scale.nbytes, shift.nbytes] + [dt.itemsize * c for dt, c in kinds])
+        stage.put(0, scale)
+        stage.put(1, shift)
+        scale_ptr, shift_ptr = stage.address(0) + 8 * head_at, stage.address(1) + 8 * tail_at
+        extents = _extents_of(self.pq, work, stage.device_buffer)
+        cat = np.concatenate
         launches = _Launches(extents)
-        for kind in ('x', 'k'):
-            launches.gemm(steps[kind])
-        launches.lincomb(steps['init'])
-        for kind in ('first', 'second', 'tail'):
-            launches.gemm(steps[kind])
-        launches.lincomb(steps['back'])
-        keep = launches.upload()
+        launches.add('gemm', _bgemm(count, a=T, b=H, c=X, m=q, n=e, k=n, lda=w, ldb=w, ldc=e, trans_b=1,
+                                    col_scale=scale_ptr, row_shift=shift_ptr))
+        launches.add('gemm', _bgemm(2 * count, a=cat([X, X]), b=cat([X, T]), c=cat([K, U]), m=cat([e, e]),
+                                    n=cat([e, w]), k=cat([q, q]), lda=cat([e, e]), ldb=cat([e, w]), ldc=cat([e, w]),
+                                    trans_a=1))
+        launches.add('lincomb', _rowwise(_native.LINCOMB_PROBLEM, 4 * count, x=cat([K, K, U, T]), dst=cat([A, B, M, Tn]),
+                                         rows=cat([e, e, e, q]), cols=cat([e, e, w, w]), x_ld=cat([e, e, w, w]),
+                                         dst_ld=cat([e, e, w, w]),
+                                         alpha=np.repeat([-0.5, -1.0 / 6.0, 0.5, 1.0], count),
+                                         gamma=np.repeat([1.0, 1.0, 0.0, 0.0], count)))
+        launches.add('gemm', _bgemm(2 * count, a=cat([A, B]), b=cat([H, H]), c=cat([Hn, M]), m=cat([e, e]),
+                                    n=cat([w, w]), k=cat([e, e]), lda=cat([e, e]), ldb=cat([w, w]), ldc=cat([w, w]),
+                                    beta=np.repeat([0.0, 1.0], count)))
+        launches.add('gemm', _bgemm(count, a=B, b=U, c=Hn, m=e, n=w, k=e, lda=e, ldb=w, ldc=w, beta=1.0))
+        launches.add('gemm', _bgemm(count, a=X, b=M, c=Tn, m=q, n=w, k=e, lda=e, ldb=w, ldc=w, alpha=-1.0, beta=1.0))
+        back_src, back_dst = (Hn, H) if last else (cat([Hn, Tn]), cat([H, T]))
+        back_rows = e if last else cat([e, q])
+        launches.add('lincomb', _rowwise(_native.LINCOMB_PROBLEM, copies, x=back_src, dst=back_dst, rows=back_rows,
+                                         cols=np.tile(w, copies // count), x_ld=np.tile(w, copies // count),
+                                         dst_ld=np.tile(w, copies // count), alpha=1.0))
+        launches.stage(stage, 2)
         launches.run()
-        del keep
-        self._keep = (scale, shift)
-        return (tn, tw_at) if last else None
+        self._keep = (work, stage)
+        return (Tn, q, work) if last else None
 
     def _diagonalise(self, heads, values, tail_rows):
-        '''Each decoupled tail (in `tail_rows`, t x w per block) diagonalised: W, the eigenvectors of
+        '''Each decoupled tail (q x w per block, at `Tn`) diagonalised: W, the eigenvectors of
         -T_P T_P^T as rows, gives the block's tail rows W T; their norms come back to the host.'''
-        n, ld, g = self.n, self.ld, self.g
-        tn, tw_at = tail_rows
-        t = [g[b] - h for b, _, h in heads]
-        sq_at = np.concatenate([[0], np.cumsum([a * a for a in t])]).astype(np.int64)
-        nr_at = np.concatenate([[0], np.cumsum(t)]).astype(np.int64)
-        grams = torch.empty(max(int(sq_at[-1]), 1), dtype=torch.float64, device=self.device)
-        vectors, norms = torch.empty_like(grams), torch.empty(max(int(nr_at[-1]), 1), dtype=torch.float64,
-                                                              device=self.device)
-        extents = _native.Extents().add(self.pq).add(tn).add(grams).add(vectors).add(norms)
-        gram_steps, rotate_steps, norm_steps = [], [], []
-        for i, (b, f, h) in enumerate(heads):
-            T, Tn, q = _ptr(self.pq, self.pq_at[b] + h * ld[b]), _ptr(tn, tw_at[i]), t[i]
-            gram_steps.append(_gemm(Tn, Tn, _ptr(grams, sq_at[i]), q, q, n[b], ld[b], ld[b], q, trans_b=1,
+        Tn, q, work = tail_rows
+        b = np.array([x[0] for x in heads], dtype=np.int64)
+        h = np.array([x[2] for x in heads], dtype=np.int64)
+        count, w, n = len(heads), self.ld[b], self.n[b]
+        sq_at, nr_at = _offsets(q * q), _offsets(q)
+        grams, vectors, norms = _doubles(self.device, sq_at[-1]), _doubles(self.device, sq_at[-1]), \
+            _doubles(self.device, nr_at[-1])
+        T = self._row_ptr(b, h)
+        gram_ptr, vec_ptr = grams.data_ptr() + 8 * sq_at[:-1], vectors.data_ptr() + 8 * sq_at[:-1]
+        launches = _Launches(_extents_of(self.pq, work, grams, vectors, norms))
+        launches.add('gemm', _bgemm(count, a=Tn, b=Tn, c=gram_ptr, m=q, n=q, k=n, lda=w, ldb=w, ldc=q, trans_b=1,
                                     alpha=-1.0))
-            rotate_steps.append(_gemm(_ptr(vectors, sq_at[i]), Tn, T, q, ld[b], q, q, ld[b], ld[b]))
-            norm_steps.append(_rows(T, _ptr(norms, nr_at[i]), q, n[b], ld[b], 1))
-        launches = _Launches(extents)
-        launches.gemm(gram_steps)
-        launches.gemm(rotate_steps)
-        launches.rows('norms', norm_steps)
-        keep = launches.upload()
+        launches.add('gemm', _bgemm(count, a=vec_ptr, b=Tn, c=T, m=q, n=w, k=q, lda=q, ldb=w, ldc=w))
+        launches.add('norms', _rowwise(_native.ROWS_PROBLEM, count, src=T, dst=norms.data_ptr() + 8 * nr_at[:-1],
+                                       rows=q, cols=n, src_ld=w, dst_ld=1))
+        stage = _Stage(self.device, [t.nbytes for _, t, _ in launches.steps])
+        launches.stage(stage, 0)
         launches.run(0, 1)
-        _native.eigh_batched(grams, t, sq_at[:-1].tolist(), vectors=vectors)
+        _native.eigh_batched(grams, q.tolist(), sq_at[:-1].tolist(), vectors=vectors)
         launches.run(1)
         host = norms.cpu().numpy()
-        del keep
         self._keep = None
-        for i, (b, f, h) in enumerate(heads):
-            values[b][h:] = host[nr_at[i]:nr_at[i + 1]]
+        for i, (bb, ff, hh) in enumerate(heads):
+            values[bb][hh:] = host[nr_at[i]:nr_at[i + 1]]
 
     def factor(self, cut: Callable, reliable: float, resolved: float) -> List[np.ndarray]:
         norms = self._first_pass()          # every row of every block; the first r are the values
         self.first = [0] * len(self.blocks)
+        ranks = self.r.tolist()
         while True:
-            tails = choose_tails(norms, self.r, self.first, cut([v[:r] for v, r in zip(norms, self.r)]), reliable,
+            tails = choose_tails(norms, ranks, self.first, cut([v[:r] for v, r in zip(norms, ranks)]), reliable,
                                  resolved)
             if not tails:
                 break
             self._tail_pass(tails, norms)
-        values = [v[:r] for v, r in zip(norms, self.r)]
+        values = [v[:r] for v, r in zip(norms, ranks)]
         self.order = [np.argsort(-v, kind='stable').astype(np.int32) for v in values]
         return [v[o] for v, o in zip(values, self.order)]
 
@@ -358,28 +370,32 @@ class GramSplit:
         value order) into the two new tensors' blocks.'''
         if not placements:
             return
-        index_host = np.concatenate([self.order[b][:k] for b, k, *_ in placements])
-        index = _uploaded(index_host, self.device)
-        at = np.concatenate([[0], np.cumsum([k for _, k, *_ in placements])]).astype(np.int64)
-        extents = _native.Extents().add(self.pq).add(index)
-        problems, sources = [], []
-        for i, (b, k, first_block, column, second_block, row) in enumerate(placements):
-            extents.add(first_block).add(second_block)
-            base, n, g, ld = self.pq_at[b], self.n[b], self.g[b], self.ld[b]
-            rows_pointer = index.data_ptr() + 4 * int(at[i])
-            p_rows, q_rows = _ptr(self.pq, base), _ptr(self.pq, base + n)
-            first_cols = first_block.shape[1]
-            second_cols = second_block.shape[1]
-            # absorb 2: first = Q^T (columns), second = P (rows); absorb 1: first = P^T, second = Q
-            first_src, first_width = (q_rows, g) if self.left else (p_rows, n)
-            second_src, second_width = (p_rows, n) if self.left else (q_rows, g)
-            problems.append(_rows(first_src, _ptr(first_block, column), k, first_width, ld, first_cols, 1,
-                                  rows_pointer))
-            problems.append(_rows(second_src, _ptr(second_block, row * second_cols), k, second_width, ld,
-                                  second_cols, 0, rows_pointer))
-            sources += [self.r[b], self.r[b]]
+        count = len(placements)
+        b = np.array([x[0] for x in placements], dtype=np.int64)
+        k = np.array([x[1] for x in placements], dtype=np.int64)
+        index = np.concatenate([self.order[bb][:kk] for bb, kk in zip(b, k)]).astype(np.int32)
+        first_ptr = np.array([x[2].data_ptr() + 8 * x[3] for x in placements], dtype=np.int64)
+        second_ptr = np.array([x[4].data_ptr() + 8 * x[5] * x[4].shape[1] for x in placements], dtype=np.int64)
+        first_cols = np.array([x[2].shape[1] for x in placements], dtype=np.int64)
+        second_cols = np.array([x[4].shape[1] for x in placements], dtype=np.int64)
+        n, g, ld = self.n[b], self.g[b], self.ld[b]
+        p_rows, q_rows = self._row_ptr(b, 0), self._row_ptr(b, 0) + 8 * n
+        # absorb 2: first = Q^T (columns), second = P (rows); absorb 1: first = P^T, second = Q
+        first_src, first_width = (q_rows, g) if self.left else (p_rows, n)
+        second_src, second_width = (p_rows, n) if self.left else (q_rows, g)
+        table_bytes = 2 * count * _native.ROWS_PROBLEM.itemsize
+        stage = _Stage(self.device, [index.nbytes, table_bytes])
+        stage.put(0, index)
+        rows_ptr = stage.address(0) + 4 * _offsets(k)[:-1]
+        extents = _extents_of(self.pq, stage.device_buffer, *[x[2] for x in placements], *[x[4] for x in placements])
+        cat = np.concatenate
         launches = _Launches(extents)
-        launches.rows('copy', problems, sources)
-        keep = launches.upload()
+        launches.add('copy', _rowwise(_native.ROWS_PROBLEM, 2 * count, src=cat([first_src, second_src]),
+                                      dst=cat([first_ptr, second_ptr]), index=cat([rows_ptr, rows_ptr]), rows=cat([k, k]),
+                                      cols=cat([first_width, second_width]), src_ld=cat([ld, ld]),
+                                      dst_ld=cat([first_cols, second_cols]),
+                                      transpose=np.repeat([1, 0], count)),
+                     source_rows=cat([self.r[b], self.r[b]]))
+        launches.stage(stage, 1)
         launches.run()
-        del keep
+        self._keep = stage

@@ -1,7 +1,7 @@
 '''
 The split's device launches interpreted on the CPU: `tt_bgemm`, `tt_row_norms`, `tt_copy_rows`,
 `tt_lincomb_rows` and the batched eigensolver, run by numpy on the host buffers their problem tables
-address.  With the tables pointing at CPU tensors, `tentorch_amd.gram.GramSplit` runs end to end without a GPU, so its
+address (each table read from where it was staged).  With the tables pointing at CPU tensors, `tentorch_amd.gram.GramSplit` runs end to end without a GPU, so its
 table construction, its passes and its bounds checks are tested on the CPU; the kernels themselves
 are tested on the GPU against this reading of the header's semantics.
 '''
@@ -30,35 +30,23 @@ def _matrix(address: int, rows: int, cols: int, ld: int) -> np.ndarray:
         else np.zeros((rows, cols))
 
 
-class _Tables:
-    '''Uploaded tables: kept on the host, addressed by their numpy buffers.'''
-
-    def __init__(self):
-        self.keep = []
-
-    def upload(self, *tables):
-        copies = [np.ascontiguousarray(t).copy() for t in tables]
-        self.keep.append(copies)
-        return copies, [c.ctypes.data for c in copies]
-
-
-def _find(address, keep):
-    for tables in keep:
-        for t in tables:
-            if t.ctypes.data == address:
-                return t
-    raise KeyError('unknown table address')
+def _staged(address: int, table: np.ndarray) -> np.ndarray:
+    '''The table as the launch reads it: len(table) records at `address` (the staged copy).'''
+    if not len(table):
+        return table
+    raw = (ctypes.c_char * (len(table) * table.dtype.itemsize)).from_address(address)
+    staged = np.frombuffer(raw, dtype=table.dtype).copy()
+    assert staged.tobytes() == np.ascontiguousarray(table).tobytes(), 'staged table differs from the host table'
+    return staged
 
 
 @contextlib.contextmanager
 def emulated_split():
-    tables = _Tables()
-    names = ('upload', 'bgemm', 'row_norms', 'copy_rows', 'lincomb_rows', 'eigh_batched')
+    names = ('bgemm', 'row_norms', 'copy_rows', 'lincomb_rows', 'eigh_batched')
     saved = {name: getattr(_native, name) for name in names}
 
     def bgemm(table, address, tiles):
-        table = _find(address, tables.keep)
-        for p in table:
+        for p in _staged(address, table):
             m, n, k = int(p['m']), int(p['n']), int(p['k'])
             if not (m and n):
                 continue
@@ -73,13 +61,13 @@ def emulated_split():
             c[...] = p['alpha'] * product + (p['beta'] * c if p['beta'] != 0.0 else 0.0)
 
     def row_norms(table, address, rows):
-        for p in _find(address, tables.keep):
+        for p in _staged(address, table):
             r, c = int(p['rows']), int(p['cols'])
             if r and c:
                 _doubles(int(p['dst']), r)[...] = np.linalg.norm(_matrix(int(p['src']), r, c, int(p['src_ld'])), axis=1)
 
     def copy_rows(table, address, rows):
-        for p in _find(address, tables.keep):
+        for p in _staged(address, table):
             r, c = int(p['rows']), int(p['cols'])
             if not (r and c):
                 continue
@@ -92,7 +80,7 @@ def emulated_split():
                 _matrix(int(p['dst']), r, c, int(p['dst_ld']))[...] = source
 
     def lincomb_rows(table, address, rows):
-        for p in _find(address, tables.keep):
+        for p in _staged(address, table):
             r, c = int(p['rows']), int(p['cols'])
             if not (r and c):
                 continue
@@ -113,7 +101,7 @@ def emulated_split():
             at += n
         return values[:at], vectors
 
-    replacements = dict(upload=tables.upload, bgemm=bgemm, row_norms=row_norms, copy_rows=copy_rows,
+    replacements = dict(bgemm=bgemm, row_norms=row_norms, copy_rows=copy_rows,
                         lincomb_rows=lincomb_rows, eigh_batched=eigh_batched)
     for name, function in replacements.items():
         setattr(_native, name, function)

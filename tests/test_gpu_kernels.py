@@ -431,7 +431,7 @@ def test_batched_gemm_with_transposes_matches_numpy():
     the 64 x 64 tile or the 16-deep slab), an empty depth, column scales, the division by column
     minus row shifts, alpha and beta.'''
     from tentorch_amd import _native
-    from tentorch_amd.gram import _Launches, _gemm
+    from tentorch_amd.gram import _Launches, _Stage, _bgemm
     rng = np.random.default_rng(51)
     shapes = [(1, 1, 1), (17, 5, 33), (64, 64, 16), (65, 130, 150), (150, 3, 97), (40, 70, 0), (3, 129, 64)]
     cases, buffers = [], []
@@ -459,12 +459,15 @@ def test_batched_gemm_with_transposes_matches_numpy():
         cases.append((dc, ref))
         buffers.append((da, db, dc, ds, dr))
         lda, ldb = (m if ta else k), (k if tb else n)
-        buffers[-1] += (_gemm(da.data_ptr(), db.data_ptr(), dc.data_ptr(), m, n, k, max(lda, 1), max(ldb, 1), n,
-                              ta, tb, alpha, beta, ds.data_ptr() if ds is not None else 0,
-                              dr.data_ptr() if dr is not None else 0),)
+        buffers[-1] += (dict(a=da.data_ptr(), b=db.data_ptr(), c=dc.data_ptr(), m=m, n=n, k=k, lda=max(lda, 1),
+                             ldb=max(ldb, 1), ldc=n, trans_a=ta, trans_b=tb, alpha=alpha, beta=beta,
+                             col_scale=ds.data_ptr() if ds is not None else 0,
+                             row_shift=dr.data_ptr() if dr is not None else 0),)
+    problems = [entry[-1] for entry in buffers]
     launches = _Launches(extents)
-    launches.gemm([entry[-1] for entry in buffers])
-    keep = launches.upload()
+    launches.add('gemm', _bgemm(len(problems), **{key: [p[key] for p in problems] for key in problems[0]}))
+    stage = _Stage(torch.device('cuda'), [launches.steps[0][1].nbytes])
+    launches.stage(stage, 0)
     launches.run()
     torch.cuda.synchronize()
     for dc, ref in cases:
@@ -475,7 +478,7 @@ def test_row_norms_and_row_copies_match_numpy():
     '''tt_row_norms and tt_copy_rows (with and without a row index, straight and transposed) on
     matrices with row strides longer than their rows.'''
     from tentorch_amd import _native
-    from tentorch_amd.gram import _Launches, _rows
+    from tentorch_amd.gram import _Launches, _Stage, _rowwise
     rng = np.random.default_rng(52)
     src = torch.tensor(rng.standard_normal((70, 90)), device='cuda')       # rows of 80, stride 90
     norms = torch.empty(70, dtype=torch.float64, device='cuda')
@@ -485,11 +488,14 @@ def test_row_norms_and_row_copies_match_numpy():
     index = torch.tensor(order, device='cuda')
     extents = _native.Extents().add(src).add(norms).add(straight).add(turned).add(index)
     launches = _Launches(extents)
-    launches.rows('norms', [_rows(src.data_ptr(), norms.data_ptr(), 70, 80, 90, 1)])
-    launches.rows('copy', [_rows(src.data_ptr(), straight.data_ptr(), 70, 80, 90, 85),
-                           _rows(src.data_ptr(), turned.data_ptr(), 33, 80, 90, 75, 1, index.data_ptr())],
-                  source_rows=[70, 70])
-    keep = launches.upload()
+    launches.add('norms', _rowwise(_native.ROWS_PROBLEM, 1, src=src.data_ptr(), dst=norms.data_ptr(), rows=70, cols=80,
+                                   src_ld=90, dst_ld=1))
+    launches.add('copy', _rowwise(_native.ROWS_PROBLEM, 2, src=src.data_ptr(), dst=[straight.data_ptr(), turned.data_ptr()],
+                                  rows=[70, 33], cols=80, src_ld=90, dst_ld=[85, 75], transpose=[0, 1],
+                                  index=[0, index.data_ptr()]),
+                 source_rows=[70, 70])
+    stage = _Stage(torch.device('cuda'), [table.nbytes for _, table, _ in launches.steps])
+    launches.stage(stage, 0)
     launches.run()
     host = src.cpu().numpy()[:, :80]
     assert np.abs(norms.cpu().numpy() - np.linalg.norm(host, axis=1)).max() <= 1e-13 * np.abs(host).max() * 10

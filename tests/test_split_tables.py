@@ -11,7 +11,7 @@ import pytest
 import torch
 
 from tentorch_amd import _native
-from tentorch_amd.gram import _gemm, _lincomb, _rows, _table
+from tentorch_amd.gram import _bgemm as _bgemm_fields, _rowwise
 
 
 def _buffers(*sizes):
@@ -22,8 +22,31 @@ def _buffers(*sizes):
     return tensors, extents
 
 
+GEMM = ('a', 'b', 'c', 'm', 'n', 'k', 'lda', 'ldb', 'ldc', 'trans_a', 'trans_b', 'alpha', 'beta', 'col_scale',
+        'row_shift')
+ROWS = ('src', 'dst', 'rows', 'cols', 'src_ld', 'dst_ld', 'transpose', 'index')
+LINCOMB = ('x', 'y', 'dst', 'rows', 'cols', 'x_ld', 'y_ld', 'dst_ld', 'alpha', 'beta', 'gamma')
+
+
+def _gemm(a, b, c, m, n, k, lda, ldb, ldc, trans_a=0, trans_b=0, alpha=1.0, beta=0.0, scale=0, shift=0):
+    return (a, b, c, m, n, k, lda, ldb, ldc, trans_a, trans_b, alpha, beta, scale, shift)
+
+
+def _columns(names, problems):
+    return {name: [p[i] for p in problems] for i, name in enumerate(names)}
+
+
 def _bgemm(problems):
-    return _table(problems, _native.BGEMM_PROBLEM, 'tiles')[0]
+    return _bgemm_fields(len(problems), **_columns(GEMM, problems))[0]
+
+
+def _rows_table(*problems):
+    padded = [p + (0,) * (len(ROWS) - len(p)) for p in problems]
+    return _rowwise(_native.ROWS_PROBLEM, len(padded), **_columns(ROWS, padded))[0]
+
+
+def _lincomb_table(*problems):
+    return _rowwise(_native.LINCOMB_PROBLEM, len(problems), **_columns(LINCOMB, problems))[0]
 
 
 def test_bgemm_table_exact_fit_and_one_past():
@@ -75,18 +98,15 @@ def test_row_and_lincomb_tables():
     (src, dst, norms), extents = _buffers(10 * 12, 10 * 12, 10)
     index = torch.tensor([3, 1, 4], dtype=torch.int32)
     extents.add(index)
-    rows = _table([_rows(src.data_ptr(), norms.data_ptr(), 10, 11, 12, 1)], _native.ROWS_PROBLEM, 'rows')[0]
+    rows = _rows_table((src.data_ptr(), norms.data_ptr(), 10, 11, 12, 1))
     assert _native.check_rows(rows, extents, vector_dst=True) == 10
-    turned = _table([_rows(src.data_ptr(), dst.data_ptr(), 3, 11, 12, 10, 1, index.data_ptr())],
-                    _native.ROWS_PROBLEM, 'rows')[0]
+    turned = _rows_table((src.data_ptr(), dst.data_ptr(), 3, 11, 12, 10, 1, index.data_ptr()))
     assert _native.check_rows(turned, extents, source_rows=[10]) == 3
     with pytest.raises(RuntimeError, match='row source'):
         _native.check_rows(turned, extents, source_rows=[11])
-    comb = _table([_lincomb(src.data_ptr(), dst.data_ptr(), dst.data_ptr(), 10, 12, 12, 12, 12, 1.0, 2.0, 0.0)],
-                  _native.LINCOMB_PROBLEM, 'rows')[0]
+    comb = _lincomb_table((src.data_ptr(), dst.data_ptr(), dst.data_ptr(), 10, 12, 12, 12, 12, 1.0, 2.0, 0.0))
     assert _native.check_lincomb(comb, extents) == 10
-    past = _table([_lincomb(src.data_ptr(), 0, dst.data_ptr() + 8, 10, 12, 12, 0, 12, 1.0, 0.0, 0.0)],
-                  _native.LINCOMB_PROBLEM, 'rows')[0]
+    past = _lincomb_table((src.data_ptr(), 0, dst.data_ptr() + 8, 10, 12, 12, 0, 12, 1.0, 0.0, 0.0))
     with pytest.raises(RuntimeError, match='lincomb destination'):
         _native.check_lincomb(past, extents)
 
