@@ -139,6 +139,13 @@ struct Builder {
     p->move_elements += plan->elements();
     return d;
   }
+  // A leg move planned elsewhere (a partial trace): same op, same hoisting rules.
+  int32_t apply(int32_t v, const MovePlanPtr &plan) {
+    const int32_t d = fresh(plan->target, deps[static_cast<size_t>(v)]);
+    p->ops.push_back(Op{OpKind::move, plan, nullptr, v, -1, d, deps[static_cast<size_t>(v)]});
+    p->move_elements += plan->elements();
+    return d;
+  }
   int32_t gemm(int32_t a, int32_t b, const GemmPlanPtr &g) {
     const uint32_t dep = deps[static_cast<size_t>(a)] | deps[static_cast<size_t>(b)];
     const int32_t d = fresh(g->out, dep);
@@ -331,6 +338,102 @@ std::vector<int32_t> combine(Builder &bld, const std::vector<int32_t> &lefts, co
   return folded;
 }
 
+// The partial trace of a tensor over its last codomain leg and its last domain leg, which name one
+// space (einsum.cpp:95-159), as gather-scale-scatter tables: one per charge the kept roots close to.
+// A source tree pair whose two traced sectors agree feeds the tile of its two trimmed trees; the
+// traced copy digit is the fastest radix of both trees, so the sum over the copies is `copies`
+// source boxes, each the diagonal offset (c, c) with stride `copies` along both axes.  Tiles nothing
+// feeds are tasks with no source, which the kernel writes with zeros.
+std::vector<MovePlanPtr> plan_trace(const StructurePtr &source) {
+  const CategoryPtr &catp = source->cat();
+  const Category &cat = *catp;
+  const Legs out_codomain(source->codomain().begin(), source->codomain().end() - 1);
+  const Legs out_domain(source->domain().begin(), source->domain().end() - 1);
+  struct Box {
+    int64_t offset, row_stride, col_stride;
+  };
+  // per closing charge: its structure and, per target tile (block, row tree, column tree), the boxes
+  std::map<Sector, StructurePtr> targets;
+  std::map<Sector, std::map<std::tuple<int64_t, int64_t, int64_t>, std::vector<Box>>> feeds;
+  for (const BlockInfo &b : source->blocks()) {
+    require_single_closing(cat, b.key.codomain, b.key.domain, source->qtotal(), "the roots", "a trace");
+    const SideLayout &rows = *b.row_layout, &cols = *b.col_layout;
+    for (size_t ra = 0; ra < rows.trees.size(); ++ra) {
+      const FusionTree &above_tree = rows.trees[ra];
+      const int64_t row_span = rows.offsets[ra + 1] - rows.offsets[ra];
+      const FusionTree kept_above = trimmed(cat, above_tree);
+      for (size_t ca = 0; ca < cols.trees.size(); ++ca) {
+        const FusionTree &under_tree = cols.trees[ca];
+        if (!(above_tree.sectors.back() == under_tree.sectors.back())) continue;
+        const int64_t col_span = cols.offsets[ca + 1] - cols.offsets[ca];
+        const FusionTree kept_under = trimmed(cat, under_tree);
+        const LayoutPtr out_rows = layout_of(catp, out_codomain, kept_above.coupled);
+        const LayoutPtr out_cols = layout_of(catp, out_domain, kept_under.coupled);
+        const int64_t new_row = out_rows->find(kept_above), new_col = out_cols->find(kept_under);
+        if (new_row < 0 || new_col < 0)
+          throw std::logic_error("a kept tree of a trace is not in the layout it was trimmed from");
+        const int64_t inside_rows = out_rows->offsets[static_cast<size_t>(new_row) + 1] - out_rows->offsets[static_cast<size_t>(new_row)];
+        const int64_t inside_cols = out_cols->offsets[static_cast<size_t>(new_col) + 1] - out_cols->offsets[static_cast<size_t>(new_col)];
+        const int64_t copies = row_span / inside_rows;
+        if (copies * inside_rows != row_span || copies * inside_cols != col_span)
+          throw std::logic_error("the traced legs of a tree pair carry different multiplicities");
+        for (const Outcome &closing : cat.outcomes(kept_above.coupled, cat.dual(kept_under.coupled))) {
+          require_single_closing(cat, kept_above.coupled, kept_under.coupled, closing.sector, "the kept roots", "a trace");
+          auto slot = targets.find(closing.sector);
+          if (slot == targets.end())
+            slot = targets.emplace(closing.sector, Structure::make(catp, out_codomain, out_domain, closing.sector)).first;
+          const int64_t tb = slot->second->find(kept_above.coupled, kept_under.coupled, 0);
+          if (tb < 0) throw std::logic_error("a trace has no result block for its kept roots");
+          auto &boxes = feeds[closing.sector][{tb, new_row, new_col}];
+          for (int64_t c = 0; c < copies; ++c)
+            boxes.push_back(Box{b.offset + (rows.offsets[ra] + c) * b.cols + cols.offsets[ca] + c, copies * b.cols, copies});
+        }
+      }
+    }
+  }
+  std::vector<MovePlanPtr> out;
+  for (const auto &[charge, target] : targets) {
+    auto plan = std::make_shared<MovePlan>();
+    plan->source = source;
+    plan->target = target;
+    const auto &fed = feeds[charge];
+    for (int64_t tb = 0; tb < static_cast<int64_t>(target->blocks().size()); ++tb) {
+      const BlockInfo &t = target->blocks()[static_cast<size_t>(tb)];
+      const SideLayout &rows = *t.row_layout, &cols = *t.col_layout;
+      for (size_t ra = 0; ra < rows.trees.size(); ++ra)
+        for (size_t ca = 0; ca < cols.trees.size(); ++ca) {
+          const int64_t nr = rows.offsets[ra + 1] - rows.offsets[ra], nc = cols.offsets[ca + 1] - cols.offsets[ca];
+          if (nr == 0 || nc == 0) continue;
+          tt_gss_task task{};
+          task.dst_offset = t.offset + rows.offsets[ra] * t.cols + cols.offsets[ca];
+          task.ndim = 2;
+          task.first_source = static_cast<int32_t>(plan->sources.size());
+          task.extent[0] = nr;
+          task.extent[1] = nc;
+          task.dst_stride[0] = t.cols;
+          task.dst_stride[1] = 1;
+          const auto found = fed.find({tb, static_cast<int64_t>(ra), static_cast<int64_t>(ca)});
+          if (found != fed.end())
+            for (const Box &box : found->second) {
+              // the box's last element lies inside the source
+              if (box.offset < 0 || box.offset + (nr - 1) * box.row_stride + (nc - 1) * box.col_stride >= source->size())
+                throw std::logic_error("a trace reads past its source");
+              tt_gss_source s{};
+              s.src_offset = box.offset;
+              s.src_stride[0] = box.row_stride;
+              s.src_stride[1] = box.col_stride;
+              s.coef_re = 1.0;
+              plan->sources.push_back(s);
+            }
+          task.num_sources = static_cast<int32_t>(plan->sources.size()) - task.first_source;
+          plan->tasks.push_back(task);
+        }
+    }
+    out.push_back(plan);
+  }
+  return out;
+}
+
 struct ProgramKey {
   int kind;
   std::string spec;
@@ -424,13 +527,53 @@ ProgramPtr build_einsum(const std::string &spec, const std::vector<StructurePtr>
     std::vector<char> lab;
     for (int64_t leg : above) lab.push_back(groups[at].labels[static_cast<size_t>(leg)]);
     for (int64_t leg : below) lab.push_back(groups[at].labels[static_cast<size_t>(leg)]);
-    for (size_t i = 0; i < lab.size(); ++i)
-      for (size_t j = i + 1; j < lab.size(); ++j)
-        if (lab[i] == lab[j])
-          throw std::invalid_argument(std::string("the label '") + lab[i] + "' of '" + spec +
-                                      "' is traced within one operand, which this backend does not run yet");
-    prepared.push_back({bld.move(v, above, below)});
+    // Labels named twice within the operand are traced away, one pair at a time, the first repeated
+    // label first (einsum.cpp:386-439); a trace can leave one tensor per charge its kept roots close
+    // to, and every later step runs on each of them.
+    std::vector<int32_t> alternatives{bld.move(v, above, below)};
+    while (true) {
+      int64_t first = -1, second = -1;
+      for (size_t one = 0; one < lab.size() && first < 0; ++one)
+        for (size_t other = one + 1; other < lab.size(); ++other)
+          if (lab[one] == lab[other]) {
+            first = static_cast<int64_t>(one);
+            second = static_cast<int64_t>(other);
+            break;
+          }
+      if (first < 0) break;
+      const StructurePtr &shape = bld.structure(alternatives.front());
+      const Legs legs = shape->legs();
+      if (legs[static_cast<size_t>(first)] != legs[static_cast<size_t>(second)])
+        throw std::invalid_argument(std::string("the label '") + lab[static_cast<size_t>(first)] + "' of '" + spec +
+                                    "' names two different spaces in the operand " + std::to_string(at) +
+                                    ", and a trace sums a space with itself");
+      const int64_t ncodomain = static_cast<int64_t>(shape->codomain().size());
+      const int64_t upper = first < ncodomain ? first : second, lower = first < ncodomain ? second : first;
+      std::vector<int64_t> keep_above, keep_below;
+      for (int64_t leg = 0; leg < shape->nlegs(); ++leg)
+        if (leg != upper && leg != lower) (leg < ncodomain ? keep_above : keep_below).push_back(leg);
+      keep_above.push_back(upper);
+      keep_below.push_back(lower);
+      std::vector<int32_t> traced;
+      for (int32_t alternative : alternatives) {
+        const int32_t placed = bld.move(alternative, keep_above, keep_below);
+        for (const MovePlanPtr &plan : plan_trace(bld.structure(placed))) traced.push_back(bld.apply(placed, plan));
+      }
+      alternatives = std::move(traced);
+      lab.erase(lab.begin() + second);
+      lab.erase(lab.begin() + first);
+    }
+    if (alternatives.empty())
+      throw std::invalid_argument("the traces of '" + spec + "' leave nothing to sum");
+    prepared.push_back(std::move(alternatives));
     labels.push_back(lab);
+  }
+  size_t combinations = 1;
+  for (const auto &alternatives : prepared) {
+    if (static_cast<int64_t>(alternatives.size()) > kMaxCharges / static_cast<int64_t>(combinations))
+      throw std::invalid_argument("the traces of '" + spec + "' leave more than " + std::to_string(kMaxCharges) +
+                                  " combinations of charges, which is more than this sums");
+    combinations *= alternatives.size();
   }
   // The output legs have to close to the charge (einsum.cpp:523-544).
   Legs above_legs, below_legs;
