@@ -269,6 +269,28 @@ typedef struct tt_lincomb_problem {
   double alpha, beta, gamma;
 } tt_lincomb_problem;
 int tt_lincomb_rows(const tt_lincomb_problem *problems, int64_t count, int64_t rows, void *stream);
+/* Batched Householder QR in FP64, replacing the per-block `at::linalg_qr` of `qr_over` and
+ * `rq_over` (tentorch/linalg/graded/svd.cpp:225-249).  Problem i factors A (m x n) = its source
+ * (row-major, row stride src_ld) or, when `trans` is set, the source's transpose (stored n x m), as
+ * A = Q R with k = min(m, n) reflectors in LAPACK's convention (geqr2 / org2r: beta = -sign(alpha)
+ * times the column norm, a column with nothing below its diagonal left as it is), so the signs of
+ * Q's columns and R's rows are LAPACK's.  Q (m x k) goes to q with row stride q_ld, or as its
+ * transpose (k x m) when q_trans; R (k x n, zeros below the diagonal) to r with row stride r_ld, or
+ * transposed (n x k) when r_trans.  `work` holds m n + k doubles for the problem (column-major
+ * matrix and reflector factors); the outputs must not overlap the source.  One workgroup per
+ * problem.  tt_qr_check validates a table against device byte ranges as tt_bgemm_check does. */
+typedef struct tt_qr_problem {
+  const double *src;
+  double *q;
+  double *r;
+  double *work;
+  int32_t m, n;
+  int32_t src_ld, q_ld, r_ld;
+  int32_t trans, q_trans, r_trans;
+} tt_qr_problem;
+int tt_qr_batched(const tt_qr_problem *problems, int64_t count, void *stream);
+int tt_qr_check(const tt_qr_problem *problems, int64_t count, const int64_t *starts, const int64_t *ends,
+                int64_t spans);
 /* Host-side checks of the three table kinds above before they are uploaded (no counterpart in the
  * reference, whose per-block calls take sized tensors): every operand a problem names must lie in
  * one of the `spans` device byte ranges [starts[i], ends[i]) (sorted by start, not overlapping),

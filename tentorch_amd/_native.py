@@ -64,6 +64,8 @@ _lib.tt_bgemm.argtypes = [_c_void_p, _i64, _i64, _c_void_p]
 _lib.tt_row_norms.argtypes = [_c_void_p, _i64, _i64, _c_void_p]
 _lib.tt_copy_rows.argtypes = [_c_void_p, _i64, _i64, _c_void_p]
 _lib.tt_lincomb_rows.argtypes = [_c_void_p, _i64, _i64, _c_void_p]
+_lib.tt_qr_batched.argtypes = [_c_void_p, _i64, _c_void_p]
+_lib.tt_qr_check.argtypes = [_c_void_p, _i64, _c_void_p, _c_void_p, _i64]
 _lib.tt_bgemm_check.argtypes = [_c_void_p, _i64, _c_void_p, _c_void_p, _i64, ctypes.POINTER(_i64)]
 _lib.tt_rows_check.argtypes = [_c_void_p, _i64, _c_void_p, _int, _c_void_p, _c_void_p, _i64, ctypes.POINTER(_i64)]
 _lib.tt_lincomb_check.argtypes = [_c_void_p, _i64, _c_void_p, _c_void_p, _i64, ctypes.POINTER(_i64)]
@@ -327,6 +329,10 @@ LINCOMB_PROBLEM = np.dtype([('x', '<u8'), ('y', '<u8'), ('dst', '<u8'), ('first'
                             ('cols', '<i4'), ('x_ld', '<i4'), ('y_ld', '<i4'), ('dst_ld', '<i4'), ('reserved', '<i4'),
                             ('alpha', '<f8'), ('beta', '<f8'), ('gamma', '<f8')])
 assert BGEMM_PROBLEM.itemsize == 96 and ROWS_PROBLEM.itemsize == 56 and LINCOMB_PROBLEM.itemsize == 80
+QR_PROBLEM = np.dtype([('src', '<u8'), ('q', '<u8'), ('r', '<u8'), ('work', '<u8'), ('m', '<i4'), ('n', '<i4'),
+                       ('src_ld', '<i4'), ('q_ld', '<i4'), ('r_ld', '<i4'), ('trans', '<i4'), ('q_trans', '<i4'),
+                       ('r_trans', '<i4')])
+assert QR_PROBLEM.itemsize == 64
 
 
 class Extents:
@@ -403,6 +409,17 @@ def check_rows(table: np.ndarray, extents: Extents, source_rows=None, vector_dst
 def check_lincomb(table: np.ndarray, extents: Extents) -> int:
     '''Validates a lincomb table (`tt_lincomb_check`); returns its row count.'''
     return _checked(_lib.tt_lincomb_check, table, extents)
+
+
+def check_qr(table: np.ndarray, extents: Extents) -> None:
+    '''Validates a QR table against the buffers it names (`tt_qr_check`).'''
+    starts, ends = extents.spans()
+    check(_lib.tt_qr_check(table.ctypes.data if table.size else None, len(table), starts.ctypes.data, ends.ctypes.data,
+                           len(starts)), 'table check')
+
+
+def qr_batched(table: np.ndarray, address: int) -> None:
+    check(_lib.tt_qr_batched(address, len(table), stream()), 'batched QR')
 
 
 def lincomb_rows(table: np.ndarray, address: int, rows: int) -> None:

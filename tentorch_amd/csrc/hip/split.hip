@@ -188,6 +188,117 @@ __global__ __launch_bounds__(64 * kRowWaves) void lincomb_rows_kernel(const tt_l
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Batched Householder QR (qr_over / rq_over, svd.cpp:225-249, which run at::linalg_qr per block).
+// One workgroup per problem.  The matrix is copied into the problem's column-major workspace and
+// reduced column by column as LAPACK's geqr2 does: wave 0 forms the column's reflector (dlarfg's
+// convention: beta = -sign(alpha) ||(alpha, x)||, tau = (beta - alpha) / beta, v = x / (alpha - beta),
+// a column with nothing below its diagonal left as it is), then the trailing columns are dealt out
+// one per wave, each a dot product with the reflector (rows over the lanes) and an update.  R is
+// written out, and Q is formed in place from the reflectors, last first (org2r).  Every step's
+// column work is one wave's, so a step costs two barriers.
+
+constexpr int kQrThreads = 1024;
+constexpr int kQrWaves = kQrThreads / 64;
+
+// Sum over the wave's 64 lanes by butterfly shuffles: every lane ends with the same total, summed
+// in the same order whatever the data.
+__device__ inline double wave_total(double v) {
+#pragma unroll
+  for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_xor(v, offset, 64);
+  return v;
+}
+
+// Apply the reflector of column j (v = 1 at row j, W[j+1.., j] below it, factor tj) to columns
+// [from, to) of W over rows j..m-1, one column per wave in turn.
+__device__ inline void qr_apply(double *__restrict__ W, int m, int j, double tj, int from, int to, int wave, int lane) {
+  const double *__restrict__ v = W + int64_t(j) * m;
+  for (int c = from + wave; c < to; c += kQrWaves) {
+    double *__restrict__ y = W + int64_t(c) * m;
+    double s = lane == 0 ? y[j] : 0.0;
+    for (int i = j + 1 + lane; i < m; i += 64) s += v[i] * y[i];
+    const double f = tj * wave_total(s);
+    if (lane == 0) y[j] -= f;
+    for (int i = j + 1 + lane; i < m; i += 64) y[i] -= f * v[i];
+  }
+}
+
+__global__ __launch_bounds__(kQrThreads) void qr_kernel(const tt_qr_problem *__restrict__ problems) {
+  const tt_qr_problem p = problems[blockIdx.x];
+  const int m = p.m, n = p.n, k = m < n ? m : n;
+  if (m == 0 || n == 0) return;
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  double *__restrict__ W = p.work;
+  double *__restrict__ tau = p.work + int64_t(m) * n;
+  const int64_t size = int64_t(m) * n;
+  // W[i + j m] = A(i, j): A is the source (rows x cols, m = rows) or its transpose (m = cols); the
+  // source is read along its rows
+  for (int64_t e = t; e < size; e += kQrThreads) {
+    if (!p.trans) {
+      const int64_t i = e / n, j = e % n;
+      W[i + j * m] = p.src[i * p.src_ld + j];
+    } else {
+      const int64_t j = e / m, i = e % m;
+      W[i + j * m] = p.src[j * p.src_ld + i];
+    }
+  }
+  __syncthreads();
+  for (int j = 0; j < k; ++j) {
+    if (wave == 0) {
+      double *__restrict__ col = W + int64_t(j) * m;
+      double s = 0.0;
+      for (int i = j + 1 + lane; i < m; i += 64) s += col[i] * col[i];
+      const double sigma = wave_total(s), alpha = col[j];
+      double tj = 0.0, beta = alpha, scale = 1.0;
+      if (sigma != 0.0) {
+        const double norm = sqrt(alpha * alpha + sigma);
+        beta = alpha >= 0.0 ? -norm : norm;
+        tj = (beta - alpha) / beta;
+        scale = 1.0 / (alpha - beta);
+      }
+      for (int i = j + 1 + lane; i < m; i += 64) col[i] *= scale;
+      if (lane == 0) {
+        col[j] = beta;
+        tau[j] = tj;
+      }
+    }
+    __syncthreads();
+    const double tj = tau[j];
+    if (tj != 0.0) qr_apply(W, m, j, tj, j + 1, n, wave, lane);
+    __syncthreads();
+  }
+  // R (k x n, zeros below the diagonal), or its transpose
+  for (int64_t e = t; e < int64_t(k) * n; e += kQrThreads) {
+    if (!p.r_trans) {
+      const int64_t i = e / n, c = e % n;
+      p.r[i * p.r_ld + c] = c >= i ? W[i + c * m] : 0.0;
+    } else {
+      const int64_t c = e / k, i = e % k;
+      p.r[c * p.r_ld + i] = c >= i ? W[i + c * m] : 0.0;
+    }
+  }
+  __syncthreads();
+  // Q = H_0 ... H_{k-1} [I; 0] in the first k columns, the last reflector first (org2r)
+  for (int j = k - 1; j >= 0; --j) {
+    const double tj = tau[j];
+    if (tj != 0.0) qr_apply(W, m, j, tj, j + 1, k, wave, lane);
+    __syncthreads();
+    double *__restrict__ col = W + int64_t(j) * m;
+    for (int i = t; i < m; i += kQrThreads) col[i] = i < j ? 0.0 : (i == j ? 1.0 - tj : -tj * col[i]);
+    __syncthreads();
+  }
+  // Q (m x k), or its transpose
+  for (int64_t e = t; e < int64_t(m) * k; e += kQrThreads) {
+    if (!p.q_trans) {
+      const int64_t i = e / k, c = e % k;
+      p.q[i * p.q_ld + c] = W[i + c * m];
+    } else {
+      const int64_t c = e / m, i = e % m;
+      p.q[c * p.q_ld + i] = W[i + c * m];
+    }
+  }
+}
+
 template <typename Problem>
 int check_table(const Problem *problems, int64_t count, const char *what) {
   if (count < 0 || (count > 0 && !problems)) return fail(std::string(what) + ": bad problem table");
@@ -233,6 +344,15 @@ extern "C" int tt_lincomb_rows(const tt_lincomb_problem *problems, int64_t count
   if (count == 0 || rows == 0) return 0;
   hipLaunchKernelGGL(lincomb_rows_kernel, dim3(static_cast<uint32_t>((rows + kRowWaves - 1) / kRowWaves)),
                      dim3(64 * kRowWaves), 0, as_stream(stream), problems, static_cast<int>(count), rows);
+  TT_LAUNCHED();
+  return 0;
+}
+
+extern "C" int tt_qr_batched(const tt_qr_problem *problems, int64_t count, void *stream) {
+  if (int rc = check_table(problems, count, "tt_qr_batched")) return rc;
+  if (count > (int64_t(1) << 31) - 1) return fail("tt_qr_batched: too many problems");
+  if (count == 0) return 0;
+  hipLaunchKernelGGL(qr_kernel, dim3(static_cast<uint32_t>(count)), dim3(kQrThreads), 0, as_stream(stream), problems);
   TT_LAUNCHED();
   return 0;
 }
@@ -352,5 +472,28 @@ extern "C" int tt_lincomb_check(const tt_lincomb_problem *problems, int64_t coun
     rows += r;
   }
   if (items) *items = rows;
+  return 0;
+}
+
+extern "C" int tt_qr_check(const tt_qr_problem *problems, int64_t count, const int64_t *starts, const int64_t *ends,
+                           int64_t spans) {
+  if (int rc = check_table(problems, count, "qr")) return rc;
+  if (int rc = check_spans(starts, ends, spans, "qr")) return rc;
+  const Spans in{starts, ends, spans};
+  for (int64_t i = 0; i < count; ++i) {
+    const tt_qr_problem &p = problems[i];
+    const int64_t m = p.m, n = p.n, k = std::min(m, n);
+    if (m < 0 || n < 0) return fail("qr: negative problem sizes");
+    if (m == 0 || n == 0) continue;
+    // the source is rows x cols as stored: m x n, or n x m when it is factored transposed
+    const int64_t rows = p.trans ? n : m, cols = p.trans ? m : n;
+    if (cols > p.src_ld) return fail("qr: the source's row stride is shorter than its row");
+    if ((p.q_trans ? m : k) > p.q_ld || (p.r_trans ? k : n) > p.r_ld)
+      return fail("qr: an output row stride is shorter than its row");
+    if (!in.holds(p.src, touched(rows, cols, p.src_ld))) return outside("qr source", p.src);
+    if (!in.holds(p.q, p.q_trans ? touched(k, m, p.q_ld) : touched(m, k, p.q_ld))) return outside("qr Q", p.q);
+    if (!in.holds(p.r, p.r_trans ? touched(n, k, p.r_ld) : touched(k, n, p.r_ld))) return outside("qr R", p.r);
+    if (!in.holds(p.work, 8 * (m * n + k - 1))) return outside("qr workspace", p.work);
+  }
   return 0;
 }

@@ -20,7 +20,7 @@ import torch
 
 from . import _native
 from ._native import engine
-from .gram import DECOUPLING, GramSplit, choose_tails
+from .gram import DECOUPLING, GramSplit, _Stage, choose_tails
 from .tensor import Tensor, inner, inner_device, norm
 
 
@@ -387,49 +387,92 @@ def svdvals(tensor: Tensor, left: Optional[Sequence[int]] = None) -> torch.Tenso
     return _split(tensor, None if left is None else list(left), None, 0.0, 0)[1]
 
 
-def _split_blocks(tensor: Tensor, pieces) -> Tuple[Tensor, Tensor]:
-    '''`split_blocks` (`svd.cpp:196-223`).'''
+def _bond_factors(tensor: Tensor, pieces):
+    """The two factors of `split_blocks` (`svd.cpp:196-223`), zero-filled: `pieces` lists (key, kept)
+    per block in block order; returns the two tensors and, per piece, the column offset of its
+    multiplets among those of its domain sector on the new bond."""
     cat = tensor.cat
-    mults, nxt, placed = {}, {}, []
-    for key, lft, rgt, kept in pieces:
-        if not kept:
-            continue
+    mults, nxt, offsets = {}, {}, []
+    for key, kept in pieces:
         s = key.domain
-        offset = nxt.get(s, 0)
-        nxt[s] = offset + kept
-        mults[s] = mults.get(s, 0) + kept
-        placed.append((key, lft, rgt, kept, offset))
+        offsets.append(nxt.get(s, 0))
+        if kept:
+            nxt[s] = offsets[-1] + kept
+            mults[s] = mults.get(s, 0) + kept
     bond = _bond_leg(cat, mults)
     first = Tensor(cat, tensor.codomain, [bond], tensor.qtotal, tensor.dtype, tensor.device)
     second = Tensor(cat, [bond], tensor.domain, cat.trivial(), tensor.dtype, tensor.device)
-    for (r, s, c), lft, rgt, kept, offset in placed:
-        first.block(r, s, c)[:, offset:offset + kept] = lft
-        second.block(s, s)[offset:offset + kept, :] = rgt
+    return first, second, offsets
+
+
+def _triangular_split(tensor: Tensor, left: Optional[Sequence[int]], mirrored: bool) -> Tuple[Tensor, Tensor]:
+    """`qr_over` (`svd.cpp:225-236`) and, `mirrored`, `rq_over` (`svd.cpp:238-249`): each block B
+    factored as Q R (qr: B = Q R, Q into the first factor; rq: B^T = Q R, so B = R^T Q^T with R^T into
+    the first factor and Q^T into the second), k = min(rows, cols) multiplets per block on the bond.
+    Real FP64 blocks go through one launch of the library's batched Householder QR, which writes
+    straight into the two factors; complex blocks take `torch.linalg.qr` per block on the device."""
+    arranged = _arranged(tensor, left)
+    items = [(key, block) for key, block in arranged.blocks().items() if block.numel() != 0]
+    kept = [min(block.shape) for _, block in items]
+    first, second, offsets = _bond_factors(arranged, [(key, k) for (key, _), k in zip(items, kept)])
+    if not items:
+        return first, second
+    if arranged.dtype.is_complex:
+        for (key, block), o, k in zip(items, offsets, kept):
+            r, s, c = key
+            if mirrored:
+                q, rr = torch.linalg.qr(block.transpose(0, 1), mode='reduced')
+                lft, rgt = rr.transpose(0, 1), q.transpose(0, 1)
+            else:
+                lft, rgt = torch.linalg.qr(block, mode='reduced')
+            first.block(r, s, c)[:, o:o + k] = lft
+            second.block(s, s)[o:o + k, :] = rgt
+        return first, second
+    if arranged.dtype != torch.float64:
+        raise ValueError(f'qr and rq run on float64 or complex128 tensors, got {arranged.dtype}')
+    count = len(items)
+    rows = np.array([b.shape[0] for _, b in items], dtype=np.int64)
+    cols = np.array([b.shape[1] for _, b in items], dtype=np.int64)
+    m, n = (cols, rows) if mirrored else (rows, cols)
+    work_at = np.concatenate([[0], np.cumsum(m * n + np.minimum(m, n))]).astype(np.int64)
+    work = torch.empty(max(int(work_at[-1]), 1), dtype=torch.float64, device=arranged.device)
+    table = np.zeros(count, dtype=_native.QR_PROBLEM)
+    lefts = [first.block(key.codomain, key.domain, key.copy) for key, _ in items]
+    rights = [second.block(key.domain, key.domain) for key, _ in items]
+    left_ptr = np.array([b.data_ptr() + 8 * o for b, o in zip(lefts, offsets)], dtype=np.int64)
+    right_ptr = np.array([b.data_ptr() + 8 * o * b.shape[1] for b, o in zip(rights, offsets)], dtype=np.int64)
+    left_ld = np.array([b.shape[1] for b in lefts], dtype=np.int64)
+    right_ld = np.array([b.shape[1] for b in rights], dtype=np.int64)
+    table['src'] = [b.data_ptr() for _, b in items]
+    table['work'] = work.data_ptr() + 8 * work_at[:-1]
+    table['m'], table['n'], table['src_ld'] = m, n, cols
+    table['trans'] = 1 if mirrored else 0
+    # qr: Q (rows x k) is the first factor's columns, R (k x cols) the second's rows; rq: R^T
+    # (rows x k) is the first factor's columns, Q^T (k x cols) the second's rows
+    table['q'], table['q_ld'] = (right_ptr, right_ld) if mirrored else (left_ptr, left_ld)
+    table['r'], table['r_ld'] = (left_ptr, left_ld) if mirrored else (right_ptr, right_ld)
+    table['q_trans'] = table['r_trans'] = 1 if mirrored else 0
+    extents = _native.Extents()
+    for t in (work, first.data, second.data, arranged.data):
+        extents.add(t)
+    _native.check_qr(table, extents)
+    stage = _Stage(arranged.device, [table.nbytes])
+    stage.put(0, table)
+    stage.send()
+    # the workspace and the staged table are stream-ordered: freed after this, they are reused only by
+    # work queued behind the launch
+    _native.qr_batched(table, stage.address(0))
     return first, second
 
 
 def qr(tensor: Tensor, left: Optional[Sequence[int]] = None) -> Tuple[Tensor, Tensor]:
-    '''`qr_over` (`svd.cpp:225-236`).'''
-    arranged = _arranged(tensor, left)
-    pieces = []
-    for key, block in arranged.blocks().items():
-        if block.numel() == 0:
-            continue
-        q, r = torch.linalg.qr(block, mode='reduced')
-        pieces.append((key, q, r, q.shape[1]))
-    return _split_blocks(arranged, pieces)
+    """`qr_over` (`svd.cpp:225-236`)."""
+    return _triangular_split(tensor, left, False)
 
 
 def rq(tensor: Tensor, left: Optional[Sequence[int]] = None) -> Tuple[Tensor, Tensor]:
-    '''`rq_over` (`svd.cpp:238-249`).'''
-    arranged = _arranged(tensor, left)
-    pieces = []
-    for key, block in arranged.blocks().items():
-        if block.numel() == 0:
-            continue
-        q, r = torch.linalg.qr(block.transpose(0, 1), mode='reduced')
-        pieces.append((key, r.transpose(0, 1), q.transpose(0, 1), q.shape[1]))
-    return _split_blocks(arranged, pieces)
+    """`rq_over` (`svd.cpp:238-249`)."""
+    return _triangular_split(tensor, left, True)
 
 
 class LanczosSettings:

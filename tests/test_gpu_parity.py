@@ -368,3 +368,37 @@ def test_svd_takes_both_call_forms_of_the_reference():
     assert full.numel() == tt.svdvals(two).numel() == tt.svdvals(two, [0, 1]).numel()
     with pytest.raises(TypeError):
         tt.svd(two, None, 5, 1e-14, 2)            # neither form: the reference's binding refuses it too
+
+
+@pytest.mark.parametrize('split', ['qr', 'rq'])
+@pytest.mark.parametrize('bond,out', [
+    (([(-1,), (1,)], [3, 2]), ([(-2,), (0,), (2,)], [2, 3, 2])),
+    (([(-2,), (0,), (2,)], [20, 35, 20]), ([(-3,), (-1,), (1,), (3,)], [30, 41, 29, 7])),
+])
+def test_qr_and_rq_match_the_oracle(split, bond, out):
+    '''The batched Householder QR against the oracle's per-block LAPACK QR (`svd.cpp:225-249`): the
+    same reflector convention gives the same Q and R, signs included, on tall, wide and square
+    blocks; the bond carries min(rows, cols) multiplets per block.'''
+    rng = np.random.default_rng(21)
+    o, d = _pair(rng, [bond, PHYS], [out], (0,))
+    ref_left, ref_right = (ot.qr if split == 'qr' else ot.rq)(o)
+    left, right = (tt.qr if split == 'qr' else tt.rq)(d)
+    assert list(left.domain[0].sectors) == [tuple(s) for s in ref_left.domain[0].sectors]
+    assert list(left.domain[0].multiplicities) == list(ref_left.domain[0].multiplicities)
+    for mine, ref in ((left, ref_left), (right, ref_right)):
+        got, want = mine.data.cpu().numpy(), ref.flat()
+        assert got.shape == want.shape
+        assert np.abs(got - want).max() <= 1e-12 * max(1.0, np.abs(want).max())
+
+
+def test_qr_of_a_zero_block_gives_unit_columns_and_a_zero_r():
+    '''A column with nothing to reflect keeps its reflector the identity (LAPACK's dlarfg): a zero
+    block factors as unit columns times zero, as numpy's QR gives.'''
+    rng = np.random.default_rng(22)
+    o, d = _pair(rng, [BOND, PHYS], [BOND])
+    d.data.zero_()
+    o = ot.Tensor(o.cat, o.codomain, o.domain, o.qtotal, o.dtype)
+    ref_left, ref_right = ot.qr(o)
+    left, right = tt.qr(d)
+    assert np.array_equal(left.data.cpu().numpy(), ref_left.flat())
+    assert np.array_equal(right.data.cpu().numpy(), ref_right.flat())
