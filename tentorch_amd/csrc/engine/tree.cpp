@@ -5,6 +5,8 @@
 #include <map>
 #include <numeric>
 #include <set>
+#include <unordered_map>
+#include <unordered_set>
 
 namespace tt {
 
@@ -26,19 +28,29 @@ std::map<LegKey, LegPtr> &leg_table() {
   return table;
 }
 
+size_t hash_legs(size_t seed, const std::vector<const Leg *> &legs) {
+  for (const Leg *l : legs) mix(seed, std::hash<const void *>()(l));
+  mix(seed, legs.size());
+  return seed;
+}
+
 struct LayoutKey {
   const Category *cat;
   std::vector<const Leg *> legs;
   Sector root;
-  bool operator<(const LayoutKey &o) const {
-    if (cat != o.cat) return cat < o.cat;
-    if (legs != o.legs) return legs < o.legs;
-    return root < o.root;
+  bool operator==(const LayoutKey &o) const { return cat == o.cat && legs == o.legs && root == o.root; }
+};
+
+struct LayoutKeyHash {
+  size_t operator()(const LayoutKey &k) const {
+    size_t seed = hash_legs(std::hash<const void *>()(k.cat), k.legs);
+    mix(seed, SectorHash()(k.root));
+    return seed;
   }
 };
 
-std::map<LayoutKey, LayoutPtr> &layout_table() {
-  static std::map<LayoutKey, LayoutPtr> table;
+std::unordered_map<LayoutKey, LayoutPtr, LayoutKeyHash> &layout_table() {
+  static std::unordered_map<LayoutKey, LayoutPtr, LayoutKeyHash> table;
   return table;
 }
 
@@ -46,16 +58,21 @@ struct StructureKey {
   const Category *cat;
   std::vector<const Leg *> codomain, domain;
   Sector q;
-  bool operator<(const StructureKey &o) const {
-    if (cat != o.cat) return cat < o.cat;
-    if (codomain != o.codomain) return codomain < o.codomain;
-    if (domain != o.domain) return domain < o.domain;
-    return q < o.q;
+  bool operator==(const StructureKey &o) const {
+    return cat == o.cat && codomain == o.codomain && domain == o.domain && q == o.q;
   }
 };
 
-std::map<StructureKey, StructurePtr> &structure_table() {
-  static std::map<StructureKey, StructurePtr> table;
+struct StructureKeyHash {
+  size_t operator()(const StructureKey &k) const {
+    size_t seed = hash_legs(hash_legs(std::hash<const void *>()(k.cat), k.codomain), k.domain);
+    mix(seed, SectorHash()(k.q));
+    return seed;
+  }
+};
+
+std::unordered_map<StructureKey, StructurePtr, StructureKeyHash> &structure_table() {
+  static std::unordered_map<StructureKey, StructurePtr, StructureKeyHash> table;
   return table;
 }
 
@@ -183,16 +200,25 @@ size_t TreeHash::operator()(const FusionTree &t) const {
 }
 
 namespace {
-std::map<std::pair<const Category *, std::vector<const Leg *>>, std::vector<Sector>> &reachable_table() {
-  static std::map<std::pair<const Category *, std::vector<const Leg *>>, std::vector<Sector>> table;
+struct ReachableKeyHash {
+  size_t operator()(const std::pair<const Category *, std::vector<const Leg *>> &k) const {
+    return hash_legs(std::hash<const void *>()(k.first), k.second);
+  }
+};
+
+std::unordered_map<std::pair<const Category *, std::vector<const Leg *>>, std::vector<Sector>, ReachableKeyHash> &
+reachable_table() {
+  static std::unordered_map<std::pair<const Category *, std::vector<const Leg *>>, std::vector<Sector>, ReachableKeyHash>
+      table;
   return table;
 }
 std::vector<Sector> compute_reachable(const Category &cat, const Legs &legs);
 }  // namespace
 
 // Insertion order of first appearance (tree.cpp:375-398), kept per (category, legs).
-std::vector<Sector> reachable_charges(const Category &cat, const Legs &legs) {
+const std::vector<Sector> &reachable_charges(const Category &cat, const Legs &legs) {
   std::pair<const Category *, std::vector<const Leg *>> key{&cat, {}};
+  key.second.reserve(legs.size());
   for (const LegPtr &l : legs) key.second.push_back(l.get());
   auto &table = reachable_table();
   auto found = table.find(key);
@@ -216,25 +242,30 @@ std::vector<Sector> compute_reachable(const Category &cat, const Legs &legs) {
 }
 }  // namespace
 
-// Depth first over leg blocks in sector order and outcomes in category order (tree.cpp:400-471).
-std::vector<FusionTree> fusion_trees(const Category &cat, const Legs &legs, const Sector &coupled) {
+namespace {
+// Every fusion tree of `legs`, depth first over leg blocks in sector order and outcomes in category
+// order (tree.cpp:400-471), handed to `emit` as it is completed; `coupled`, when given, keeps only
+// the trees of that root (the last fusion is filtered, the walk above it is shared).
+template <class Emit>
+void walk_trees(const Category &cat, const Legs &legs, const Sector *coupled, Emit &&emit) {
   const int64_t count = static_cast<int64_t>(legs.size());
   if (count == 0) {
-    if (coupled == cat.trivial()) return {FusionTree{{}, {}, {}, coupled}};
-    return {};
+    if (!coupled || *coupled == cat.trivial()) emit(FusionTree{{}, {}, {}, cat.trivial()});
+    return;
   }
-  std::vector<FusionTree> out;
   FusionTree tree;
   tree.sectors.resize(static_cast<size_t>(count));
   tree.inner.resize(static_cast<size_t>(count >= 2 ? count - 2 : 0));
   tree.copies.resize(static_cast<size_t>(count >= 2 ? count - 1 : 0));
-  tree.coupled = coupled;
   std::function<void(int64_t, const Sector &)> walk = [&](int64_t leg, const Sector &previous) {
     if (leg == 0) {
       for (const Sector &s : legs[0]->sectors()) {
         tree.sectors[0] = s;
         if (count == 1) {
-          if (s == coupled) out.push_back(tree);
+          if (!coupled || s == *coupled) {
+            tree.coupled = s;
+            emit(tree);
+          }
         } else {
           walk(1, s);
         }
@@ -244,19 +275,27 @@ std::vector<FusionTree> fusion_trees(const Category &cat, const Legs &legs, cons
     for (const Sector &s : legs[static_cast<size_t>(leg)]->sectors()) {
       tree.sectors[static_cast<size_t>(leg)] = s;
       for (const Outcome &o : cat.outcomes(previous, s)) {
-        if (leg + 1 == count && o.sector != coupled) continue;
+        if (leg + 1 == count && coupled && o.sector != *coupled) continue;
         if (leg + 1 < count) tree.inner[static_cast<size_t>(leg - 1)] = o.sector;
         for (int64_t copy = 0; copy < o.multiplicity; ++copy) {
           tree.copies[static_cast<size_t>(leg - 1)] = copy;
-          if (leg + 1 == count)
-            out.push_back(tree);
-          else
+          if (leg + 1 == count) {
+            tree.coupled = o.sector;
+            emit(tree);
+          } else {
             walk(leg + 1, o.sector);
+          }
         }
       }
     }
   };
   walk(0, cat.trivial());
+}
+}  // namespace
+
+std::vector<FusionTree> fusion_trees(const Category &cat, const Legs &legs, const Sector &coupled) {
+  std::vector<FusionTree> out;
+  walk_trees(cat, legs, &coupled, [&](const FusionTree &t) { out.push_back(t); });
   return out;
 }
 
@@ -272,6 +311,7 @@ FusionTree trimmed(const Category &cat, const FusionTree &tree) {
 
 LayoutPtr layout_of(const CategoryPtr &cat, const Legs &legs, const Sector &root) {
   LayoutKey key{cat.get(), {}, root};
+  key.legs.reserve(legs.size());
   for (const LegPtr &l : legs) key.legs.push_back(l.get());
   auto &table = layout_table();
   auto found = table.find(key);
@@ -279,8 +319,13 @@ LayoutPtr layout_of(const CategoryPtr &cat, const Legs &legs, const Sector &root
   auto layout = std::make_shared<SideLayout>();
   layout->trees = fusion_trees(*cat, legs, root);
   layout->nlegs = static_cast<int64_t>(legs.size());
+  layout->abelian = cat->abelian();
+  layout->offsets.reserve(layout->trees.size() + 1);
   layout->offsets.push_back(0);
   layout->mults.reserve(layout->trees.size() * legs.size());
+  // an abelian tree is fixed by its sector list, so one index serves both lookups
+  if (layout->abelian) layout->by_sectors.reserve(layout->trees.size());
+  else layout->index.reserve(layout->trees.size());
   for (size_t t = 0; t < layout->trees.size(); ++t) {
     int64_t rows = 1;
     for (size_t axis = 0; axis < legs.size(); ++axis) {
@@ -289,8 +334,8 @@ LayoutPtr layout_of(const CategoryPtr &cat, const Legs &legs, const Sector &root
       rows *= m;
     }
     layout->offsets.push_back(layout->offsets.back() + rows);
-    layout->index.emplace(layout->trees[t], static_cast<int64_t>(t));
-    if (cat->abelian()) layout->by_sectors.emplace(layout->trees[t].sectors, static_cast<int64_t>(t));
+    if (layout->abelian) layout->by_sectors.emplace(layout->trees[t].sectors, static_cast<int64_t>(t));
+    else layout->index.emplace(layout->trees[t], static_cast<int64_t>(t));
   }
   table.emplace(std::move(key), layout);
   return layout;
@@ -298,8 +343,18 @@ LayoutPtr layout_of(const CategoryPtr &cat, const Legs &legs, const Sector &root
 
 std::vector<BlockKey> block_keys(const Category &cat, const Legs &codomain, const Legs &domain, const Sector &q) {
   std::vector<BlockKey> keys;
-  for (const Sector &r : reachable_charges(cat, codomain))
-    for (const Sector &s : reachable_charges(cat, domain))
+  const std::vector<Sector> &rows = reachable_charges(cat, codomain), &cols = reachable_charges(cat, domain);
+  if (cat.abelian()) {
+    // r x dual(s) = q has the one solution s = r x dual(q), with no copies
+    std::unordered_set<Sector, SectorHash> reached(cols.begin(), cols.end());
+    const Sector back = cat.dual(q);
+    for (const Sector &r : rows)
+      for (const Outcome &o : cat.outcomes(r, back))
+        if (reached.count(o.sector)) keys.push_back(BlockKey{r, o.sector, 0});
+    return keys;
+  }
+  for (const Sector &r : rows)
+    for (const Sector &s : cols)
       for (int64_t copy = 0; copy < cat.n_symbol(r, cat.dual(s), q); ++copy) keys.push_back(BlockKey{r, s, copy});
   return keys;
 }
@@ -309,9 +364,14 @@ Structure::Structure(CategoryPtr cat, Legs codomain, Legs domain, Sector qtotal,
   std::vector<BlockKey> keys = block_keys(*cat_, codomain_, domain_, qtotal_);
   std::sort(keys.begin(), keys.end());
   int64_t offset = 0;
+  // a root's layout is shared by every block over it
+  std::unordered_map<Sector, LayoutPtr, SectorHash> rows_at, cols_at;
   for (const BlockKey &k : keys) {
-    LayoutPtr rl = layout_of(cat_, codomain_, k.codomain);
-    LayoutPtr cl = layout_of(cat_, domain_, k.domain);
+    auto r = rows_at.find(k.codomain);
+    if (r == rows_at.end()) r = rows_at.emplace(k.codomain, layout_of(cat_, codomain_, k.codomain)).first;
+    auto c = cols_at.find(k.domain);
+    if (c == cols_at.end()) c = cols_at.emplace(k.domain, layout_of(cat_, domain_, k.domain)).first;
+    const LayoutPtr &rl = r->second, &cl = c->second;
     BlockInfo info{k, rl->rows(), cl->rows(), offset, rl, cl};
     index_.emplace(k, static_cast<int64_t>(blocks_.size()));
     blocks_.push_back(info);

@@ -73,7 +73,7 @@ struct TreeHash {
   size_t operator()(const FusionTree &t) const;
 };
 
-std::vector<Sector> reachable_charges(const Category &cat, const Legs &legs);
+const std::vector<Sector> &reachable_charges(const Category &cat, const Legs &legs);
 std::vector<FusionTree> fusion_trees(const Category &cat, const Legs &legs, const Sector &coupled);
 FusionTree trimmed(const Category &cat, const FusionTree &tree);
 
@@ -86,11 +86,18 @@ struct SideLayout {
   std::vector<int64_t> offsets;   // size trees + 1
   std::vector<int64_t> mults;     // per tree, the multiplicity each leg carries (trees x legs)
   int64_t nlegs = 0;
+  bool abelian = false;
+  // Non-abelian categories: the position of each tree.
   std::unordered_map<FusionTree, int64_t, TreeHash> index;
-  // Abelian categories only: a tree is fixed by its sector list (one outcome per fusion, no copies).
+  // Abelian categories: a tree is fixed by its sector list (one outcome per fusion, no copies), so
+  // this index alone serves both lookups.
   std::unordered_map<std::vector<Sector>, int64_t, SectorListHash> by_sectors;
   int64_t rows() const { return offsets.back(); }
   int64_t find(const FusionTree &t) const {
+    if (abelian) {
+      const int64_t at = find_sectors(t.sectors);
+      return at >= 0 && trees[static_cast<size_t>(at)] == t ? at : -1;
+    }
     auto f = index.find(t);
     return f == index.end() ? -1 : f->second;
   }
