@@ -346,7 +346,11 @@ class Extents:
 
     def add(self, t: torch.Tensor) -> 'Extents':
         start = t.data_ptr()
-        self.ranges.append((start, start + t.numel() * t.element_size()))
+        return self.add_range(start, start + t.numel() * t.element_size())
+
+    def add_range(self, start: int, end: int) -> 'Extents':
+        '''The bytes [start, end) of a device buffer.'''
+        self.ranges.append((start, end))
         self._spans = None
         return self
 

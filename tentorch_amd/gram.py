@@ -364,20 +364,17 @@ class GramSplit:
         return [v[o] for v, o in zip(values, self.order)]
 
     # -- the factors
-    def assemble(self, placements):
-        '''`placements`: (block index, kept k, first-factor block view, its column offset, second-factor
-        block view, its row offset) per kept block.  Writes Q's and P's leading k rows (in descending
-        value order) into the two new tensors' blocks.'''
+    def assemble(self, placements, spans):
+        '''`placements`: (block index, kept k, address of the first factor's block at its column offset,
+        that block's row length, address of the second factor's block at its row offset, that block's
+        row length) per kept block; `spans` the byte ranges of those blocks, which bound the copies.
+        Writes Q's and P's leading k rows (in descending value order) into the two new tensors' blocks.'''
         if not placements:
             return
         count = len(placements)
-        b = np.array([x[0] for x in placements], dtype=np.int64)
-        k = np.array([x[1] for x in placements], dtype=np.int64)
+        columns = np.array(placements, dtype=np.int64).T
+        b, k, first_ptr, first_cols, second_ptr, second_cols = columns
         index = np.concatenate([self.order[bb][:kk] for bb, kk in zip(b, k)]).astype(np.int32)
-        first_ptr = np.array([x[2].data_ptr() + 8 * x[3] for x in placements], dtype=np.int64)
-        second_ptr = np.array([x[4].data_ptr() + 8 * x[5] * x[4].shape[1] for x in placements], dtype=np.int64)
-        first_cols = np.array([x[2].shape[1] for x in placements], dtype=np.int64)
-        second_cols = np.array([x[4].shape[1] for x in placements], dtype=np.int64)
         n, g, ld = self.n[b], self.g[b], self.ld[b]
         p_rows, q_rows = self._row_ptr(b, 0), self._row_ptr(b, 0) + 8 * n
         # absorb 2: first = Q^T (columns), second = P (rows); absorb 1: first = P^T, second = Q
@@ -387,7 +384,9 @@ class GramSplit:
         stage = _Stage(self.device, [index.nbytes, table_bytes])
         stage.put(0, index)
         rows_ptr = stage.address(0) + 4 * _offsets(k)[:-1]
-        extents = _extents_of(self.pq, stage.device_buffer, *[x[2] for x in placements], *[x[4] for x in placements])
+        extents = _extents_of(self.pq, stage.device_buffer)
+        for start, end in spans:
+            extents.add_range(start, end)
         cat = np.concatenate
         launches = _Launches(extents)
         launches.add('copy', _rowwise(_native.ROWS_PROBLEM, 2 * count, src=cat([first_src, second_src]),

@@ -337,6 +337,12 @@ def _split(tensor: Tensor, left: Optional[Sequence[int]], max_bond: Optional[int
     values = None if gram else torch.zeros(bond.multiplets(), dtype=arranged.dtype, device=arranged.device)
     starts = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64) if factors else np.zeros(1, dtype=np.int64)
     placements = []
+    if gram and device_split is not None:
+        # the blocks' places in the two new buffers, read once from their structures
+        first_at = {key: (rows, cols, off) for key, rows, cols, off in first.structure.blocks()}
+        second_at = {key[0]: (rows, cols, off) for key, rows, cols, off in second.structure.blocks()}
+        first_base, second_base = first.data.data_ptr(), second.data.data_ptr()
+        spans = []      # the byte range of every block a placement writes
     for fi, f in enumerate(factors):
         k = f['kept']
         if not k:
@@ -345,7 +351,13 @@ def _split(tensor: Tensor, left: Optional[Sequence[int]], max_bond: Optional[int
         o = f['offset']
         kept_values[placed[s] + o:placed[s] + o + k] = host[starts[fi]:starts[fi] + k]
         if gram and device_split is not None:
-            placements.append((fi, k, first.block(r, s, c), o, second.block(s, s), o))
+            # (block, kept, first block's address of column o, its row length, second block's address
+            # of row o, its row length)
+            frows, fcols, foff = first_at[(r, s, c)]
+            srows, scols, soff = second_at[s]
+            placements.append((fi, k, first_base + 8 * (foff + o), fcols, second_base + 8 * (soff + o * scols), scols))
+            spans += [(first_base + 8 * foff, first_base + 8 * (foff + frows * fcols)),
+                      (second_base + 8 * soff, second_base + 8 * (soff + srows * scols))]
             continue
         if gram:
             # the isometric factor is Q's leading rows, the other the projection P = Q C already
@@ -371,7 +383,7 @@ def _split(tensor: Tensor, left: Optional[Sequence[int]], max_bond: Optional[int
         second.block(s, s)[o:o + k, :] = vh
         values[placed[s] + o:placed[s] + o + k] = v.to(arranged.dtype)
     if gram and device_split is not None:
-        device_split.assemble(placements)
+        device_split.assemble(placements, spans)
     if gram:
         # one copy from pinned memory, queued behind the products without waiting for them
         staged = torch.from_numpy(kept_values).to(arranged.dtype)
