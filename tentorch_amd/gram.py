@@ -200,20 +200,35 @@ class GramSplit:
     triplets into the new tensors' blocks.'''
 
     def __init__(self, blocks, left_isometric: bool):
-        self.blocks = blocks
-        self.left = left_isometric
         rows = np.array([b.shape[0] for b in blocks], dtype=np.int64)
         cols = np.array([b.shape[1] for b in blocks], dtype=np.int64)
+        pointers = np.array([b.data_ptr() for b in blocks], dtype=np.int64)
+        self._setup(rows, cols, pointers, blocks[0].device, pointers + 8 * rows * cols, left_isometric)
+
+    @classmethod
+    def of_buffer(cls, data: torch.Tensor, rows, cols, offsets, left_isometric: bool) -> 'GramSplit':
+        '''The split of the blocks of one flat FP64 buffer: block b is rows[b] x cols[b], row-major, at
+        element offset offsets[b].'''
+        self = cls.__new__(cls)
+        rows, cols = np.asarray(rows, dtype=np.int64), np.asarray(cols, dtype=np.int64)
+        pointers = data.data_ptr() + 8 * np.asarray(offsets, dtype=np.int64)
+        self._setup(rows, cols, pointers, data.device, pointers + 8 * rows * cols, left_isometric)
+        return self
+
+    def _setup(self, rows, cols, pointers, device, ends, left_isometric: bool):
+        self.count = len(rows)
+        self.left = left_isometric
         self.rows, self.cols = rows, cols
         self.g = rows.copy() if left_isometric else cols.copy()
         self.n = cols.copy() if left_isometric else rows.copy()
         self.r = np.minimum(self.g, self.n)
         self.ld = self.n + self.g
-        self.device = blocks[0].device
+        self.device = device
         self.pq_at = _offsets(self.g * self.ld)
         self.pq = _doubles(self.device, self.pq_at[-1])
         self.pq_base = self.pq.data_ptr()
-        self.block_ptr = np.array([b.data_ptr() for b in blocks], dtype=np.int64)
+        self.block_ptr = pointers
+        self.block_spans = list(zip(pointers.tolist(), ends.tolist()))
 
     def _row_ptr(self, b, row):
         '''Address of row `row` of block b's [P | Q] (vectorised over arrays).'''
@@ -221,11 +236,13 @@ class GramSplit:
 
     # -- pass 1
     def _first_pass(self):
-        g, n, ld, count = self.g, self.n, self.ld, len(self.blocks)
+        g, n, ld, count = self.g, self.n, self.ld, self.count
         sq_at, g_at = _offsets(g * g), _offsets(g)
         grams, vectors, norms = _doubles(self.device, sq_at[-1]), _doubles(self.device, sq_at[-1]), \
             _doubles(self.device, g_at[-1])
-        extents = _extents_of(self.pq, grams, vectors, norms, *self.blocks)
+        extents = _extents_of(self.pq, grams, vectors, norms)
+        for start, end in self.block_spans:
+            extents.add_range(start, end)
         gram_at = grams.data_ptr() + 8 * sq_at[:-1]
         vec_at = vectors.data_ptr() + 8 * sq_at[:-1]
         p_at = self._row_ptr(np.arange(count), 0)
@@ -351,7 +368,7 @@ class GramSplit:
 
     def factor(self, cut: Callable, reliable: float, resolved: float) -> List[np.ndarray]:
         norms = self._first_pass()          # every row of every block; the first r are the values
-        self.first = [0] * len(self.blocks)
+        self.first = [0] * self.count
         ranks = self.r.tolist()
         while True:
             tails = choose_tails(norms, ranks, self.first, cut([v[:r] for v, r in zip(norms, ranks)]), reliable,

@@ -21,7 +21,7 @@ import torch
 from . import _native
 from ._native import engine
 from .gram import DECOUPLING, GramSplit, _Stage, choose_tails
-from .tensor import Tensor, inner, inner_device, norm
+from .tensor import BlockKey, Tensor, inner, inner_device, norm
 
 
 def _arranged(tensor: Tensor, left: Optional[Sequence[int]]) -> Tensor:
@@ -248,10 +248,15 @@ def _split(tensor: Tensor, left: Optional[Sequence[int]], max_bond: Optional[int
     arranged = _arranged(tensor, left)
     cat = arranged.cat
     factors = []
-    items = [(key, block) for key, block in arranged.blocks().items() if block.numel() != 0]
+    # (key, rows, cols, element offset) of every non-empty block, without building views
+    layout = [(BlockKey(*key), rows, cols, off) for key, rows, cols, off in arranged.structure.blocks() if rows * cols]
     gram = SvdSettings.method == 'gram' and absorb in (1, 2) and not arranged.dtype.is_complex
     device_split = None
-    if gram and items:
+
+    def views():
+        return [(key, arranged.data[off:off + rows * cols].view(rows, cols)) for key, rows, cols, off in layout]
+
+    if gram and layout:
         # absorb 2 keeps U isometric (values go right), absorb 1 keeps V isometric; the singular
         # values arrive on the host, each block's in descending order
         def floor(values):
@@ -261,20 +266,21 @@ def _split(tensor: Tensor, left: Optional[Sequence[int]], max_bond: Optional[int
             alive = ranked[ranked > tol]
             return max(tol, float(alive[max_bond])) if max_bond is not None and alive.size > max_bond else tol
 
-        blocks = [b for _, b in items]
         on_device = SvdSettings.eigh == 'device' and max(
-            (b.shape[0] if absorb == 2 else b.shape[1]) for b in blocks) <= _native.EIGH_MAX_ORDER
+            (rows if absorb == 2 else cols) for _, rows, cols, _ in layout) <= _native.EIGH_MAX_ORDER
         if on_device:
-            device_split = GramSplit(blocks, absorb == 2)
-            for (key, _), v in zip(items, device_split.factor(floor, SvdSettings.reliable, SvdSettings.resolved)):
+            device_split = GramSplit.of_buffer(arranged.data, [x[1] for x in layout], [x[2] for x in layout],
+                                               [x[3] for x in layout], absorb == 2)
+            for (key, _, _, _), v in zip(layout, device_split.factor(floor, SvdSettings.reliable, SvdSettings.resolved)):
                 factors.append({'key': key, 'v': v, 'kept': 0, 'offset': 0})
         else:
-            for (key, _), (q, projected, v) in zip(items, _gram_split(blocks, absorb == 2, floor)):
+            items = views()
+            for (key, _), (q, projected, v) in zip(items, _gram_split([b for _, b in items], absorb == 2, floor)):
                 factors.append({'key': key, 'q': q, 'p': projected, 'v': v, 'kept': 0, 'offset': 0})
         host = np.concatenate([f['v'] for f in factors])
         sizes = [f['v'].size for f in factors]
     else:
-        for key, block in items:
+        for key, block in views():
             u, v, vh = torch.linalg.svd(block, full_matrices=False)
             factors.append({'key': key, 'u': u, 'v': v, 'vh': vh, 'kept': 0, 'offset': 0})
         host = torch.cat([f['v'] for f in factors]).double().cpu().numpy() if factors else np.zeros(0)
