@@ -175,20 +175,32 @@ def choose_tails(norms: List[np.ndarray], ranks: List[int], first: List[int], fl
     (the coupling to the resolved rows only lowers them, by the Schur complement), so a set whose norm
     is below half the floor is dropped whatever its exact values.  The tail then holds the rows below
     about reliable x the largest, cut at a wide gap.'''
+    count = len(norms)
+    if not count:
+        return []
+    first = np.asarray(first, dtype=np.int64)
+    pass_sizes = np.maximum(np.asarray(ranks, dtype=np.int64) - first, 0)
+    # the largest value of every block's pass, in one reduction over the passes back to back
+    tops = np.zeros(count)
+    live = np.nonzero(pass_sizes > 0)[0]
+    if live.size:
+        values = np.concatenate([norms[b][first[b]:first[b] + pass_sizes[b]] for b in live.tolist()])
+        starts = np.concatenate([[0], np.cumsum(pass_sizes[live])[:-1]]).astype(np.int64)
+        tops[live] = np.maximum.reduceat(values, starts)
+    candidate = (pass_sizes >= 2) & (tops > 0.0) & (floor < 2.0 * resolved * tops)
     tails = []
-    for b, v in enumerate(norms):
-        pass_values = v[first[b]:ranks[b]]
-        top = float(pass_values.max()) if pass_values.size else 0.0
-        if pass_values.size < 2 or top == 0.0 or floor >= 2.0 * resolved * top:
-            continue
-        unresolved = v[first[b]:][v[first[b]:] < 2.0 * resolved * top]
+    for b in np.nonzero(candidate)[0].tolist():
+        v = norms[b][first[b]:]
+        top = tops[b]
+        unresolved = v[v < 2.0 * resolved * top]
         if float(np.sqrt(np.sum(unresolved ** 2))) < 0.5 * floor:
             continue
+        pass_values = v[:pass_sizes[b]]
         below = np.nonzero(pass_values < reliable * top)[0]
         if below.size:
             start = _tail_start(pass_values, int(below[0]), reliable * top)
             if start > 0:        # a pass whose first row is already in the tail has no head to leave
-                tails.append((b, first[b] + start))
+                tails.append((b, int(first[b]) + start))
     return tails
 
 
