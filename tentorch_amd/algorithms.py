@@ -27,7 +27,7 @@ from typing import Dict, Iterator, Literal, Tuple
 
 import numpy as np
 
-from .linalg import _split, lanczos_deferred
+from .linalg import EinsumAction, _split, lanczos_deferred
 from .models import Model, heisenberg
 from .networks import MPS, identity
 from .tensor import Settings, Tensor, contract, domain_legs, einsum, inner
@@ -160,8 +160,7 @@ class DMRG:
     # -- one window
     def _solve(self, head: Tensor, tail: Tensor, start: Tensor):
         '''The window's ground state: Ritz value (still on the device) and Ritz vector.'''
-        def matvec(vector: Tensor) -> Tensor:
-            return einsum(MATVEC, head, tail, vector)
+        matvec = EinsumAction(MATVEC, [head, tail], 2)
 
         if self.precision == 'fp64':
             value, vector = lanczos_deferred(matvec, start, self.n_iter)

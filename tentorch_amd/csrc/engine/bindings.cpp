@@ -11,6 +11,10 @@
 namespace py = pybind11;
 using namespace tt;
 
+namespace tt {
+std::string device_error();
+}
+
 namespace {
 
 Sector to_sector(const py::handle &h) {
@@ -298,6 +302,40 @@ PYBIND11_MODULE(_engine, m) {
            py::arg("dtype"), py::arg("operands"), py::arg("out"), py::arg("workspace"), py::arg("stream"),
            py::arg("mode") = 0, py::arg("skip") = 0, py::arg("first") = 0, py::arg("last") = -1,
            py::arg("ws_dtype") = -1, py::arg("compute") = -1)
+      .def("krylov",
+           [](const PyProgram &p, const std::vector<uintptr_t> &operands, int64_t slot, uintptr_t basis, int64_t ldv,
+              int64_t steps, uintptr_t image, uintptr_t scratch, uintptr_t alphas, uintptr_t betas, uintptr_t workspace,
+              uintptr_t stream, uint32_t skip, int ws_dtype, int compute) {
+             // The Krylov steps of a Lanczos run whose action is this program with the basis row in
+             // operand `slot` (tentorch/linalg/lanczos.py:27-49): for k < steps, image = program(v_k),
+             // then tt_lanczos_orthonormalize makes v_{k+1} and the step's alpha and beta, all on the
+             // stream with no synchronisation.  The operands other than `slot` do not change between
+             // steps, so after the first step their hoisted moves are skipped.
+             if (slot < 0 || slot >= static_cast<int64_t>(operands.size()))
+               throw std::invalid_argument("krylov: the basis operand is out of range");
+             if (steps < 1 || steps > 31) throw std::invalid_argument("krylov: 1..31 steps");
+             const int64_t n = p.p->operands[static_cast<size_t>(slot)]->size();
+             if (p.p->out->size() != n) throw std::invalid_argument("krylov: the action has to keep the vector's size");
+             if (ldv < n) throw std::invalid_argument("krylov: basis rows shorter than the vector");
+             std::vector<const void *> ptrs;
+             for (uintptr_t x : operands) ptrs.push_back(reinterpret_cast<const void *>(x));
+             double *V = reinterpret_cast<double *>(basis);
+             const uint32_t fixed = p.p->hoistable() & ~(1u << slot);
+             py::gil_scoped_release release;
+             for (int64_t k = 0; k < steps; ++k) {
+               ptrs[static_cast<size_t>(slot)] = V + k * ldv;
+               p.p->run(TT_F64, ptrs, reinterpret_cast<void *>(image), reinterpret_cast<void *>(workspace),
+                        reinterpret_cast<void *>(stream), Mode::batched, k == 0 ? skip : (skip | fixed), 0, -1,
+                        ws_dtype, compute);
+               if (tt_lanczos_orthonormalize(n, static_cast<int>(k), V, ldv, reinterpret_cast<double *>(image),
+                                             reinterpret_cast<double *>(scratch), reinterpret_cast<double *>(alphas) + k,
+                                             reinterpret_cast<double *>(betas) + k, reinterpret_cast<void *>(stream)) != 0)
+                 throw std::runtime_error("lanczos orthonormalize: " + device_error());
+             }
+           },
+           py::arg("operands"), py::arg("slot"), py::arg("basis"), py::arg("ldv"), py::arg("steps"), py::arg("image"),
+           py::arg("scratch"), py::arg("alphas"), py::arg("betas"), py::arg("workspace"), py::arg("stream"),
+           py::arg("skip") = 0, py::arg("ws_dtype") = -1, py::arg("compute") = -1)
       .def("prepare", [](const PyProgram &p) { p.p->prepare(); })
       .def("gemm_config",
            [](const PyProgram &p, int64_t op) {

@@ -21,7 +21,7 @@ import torch
 from . import _native
 from ._native import engine
 from .gram import DECOUPLING, GramSplit, _Stage, choose_tails
-from .tensor import BlockKey, Tensor, inner, inner_device, norm
+from .tensor import BlockKey, Tensor, einsum, einsum_program, inner, inner_device, krylov_steps, norm
 
 
 def _arranged(tensor: Tensor, left: Optional[Sequence[int]]) -> Tensor:
@@ -513,6 +513,23 @@ def lanczos(action: Callable[[Tensor], Tensor], start: Tensor, n_iter: int = 5,
     return _lanczos_host(action, start, n_iter, tol)
 
 
+class EinsumAction:
+    '''The action v -> einsum(spec, *operands with v in position `slot`) as a callable, which a
+    Lanczos run recognises: its Krylov steps then go to the engine in one call (`krylov_steps`)
+    instead of one einsum and one orthonormalisation per step from Python.'''
+
+    def __init__(self, spec: str, operands: Sequence[Tensor], slot: int):
+        self.spec, self.operands, self.slot = spec, list(operands), slot
+
+    def with_vector(self, vector: Tensor) -> List[Tensor]:
+        out = list(self.operands)
+        out.insert(self.slot, vector)
+        return out
+
+    def __call__(self, vector: Tensor) -> Tensor:
+        return einsum(self.spec, *self.with_vector(vector))
+
+
 def _krylov_basis(action, start: Tensor, n_iter: int):
     '''The Krylov basis V (n_iter + 1 rows) and the device scalars of a Lanczos run.  `scalars` holds
     the alphas, the betas, each step's scratch (2 (k+1) + 1), the Ritz coefficients (n_iter + 1) and
@@ -531,6 +548,13 @@ def _krylov_basis(action, start: Tensor, n_iter: int):
     _native.dot(torch.float64, start.data, start.data, work[2:3])
     torch.reciprocal(torch.sqrt(work[2:3]), out=work[3:4])
     _native.axpby(torch.float64, 0.0, start.data, 0.0, V[0], alpha_dev=work[3:4])
+    if isinstance(action, EinsumAction) and n_iter + 1 <= 32:
+        operands = action.with_vector(start)
+        program = einsum_program(action.spec, operands)
+        if program.out == start.structure:
+            image = torch.empty(size_n, dtype=torch.float64, device=device)
+            if krylov_steps(program, operands, action.slot, V, n_iter, image, scratch, alphas, betas):
+                return V, scalars, work
     for k in range(n_iter):
         vector = Tensor._wrap(start.structure, V[k])
         image = action(vector)

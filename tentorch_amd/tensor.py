@@ -346,6 +346,40 @@ def run(program, operands: Sequence[Tensor], out: Optional[torch.Tensor] = None,
     return Tensor._wrap(result, out)
 
 
+def krylov_steps(program, operands: Sequence[Tensor], slot: int, V: torch.Tensor, steps: int, image: torch.Tensor,
+                 scratch: torch.Tensor, alphas: torch.Tensor, betas: torch.Tensor) -> bool:
+    '''The Krylov steps of a Lanczos run whose action is `program` with the basis row in operand `slot`,
+    in one engine call (`Program.krylov`): each step runs the program on V[k] into `image`, then the
+    library's orthonormalisation makes V[k + 1], alpha and beta.  The workspace and the hoisting of the
+    other operands' moves follow `run`.  Returns False, with nothing queued, when the run has to go
+    step by step instead (the per-block form, a profiler, a tile autotuner, other than FP64 vectors).'''
+    first = operands[0]
+    if Settings.mode != 0 or Profiler.active is not None or Settings.tiles != 'default' or \
+            any(t.dtype != torch.float64 for t in operands) or V.dtype != torch.float64:
+        return False
+    for t in operands:
+        _native.require_device(t.data)
+    mixed = Settings.precision == 'fp32'
+    ws_dtype = torch.float32 if mixed else torch.float64
+    elements = program.workspace
+    skip, ws = 0, 0
+    if elements:
+        entry = workspaces.get(program, ws_dtype, first.device, elements)
+        hoist = program.hoistable
+        identity = tuple((t._uid, t._version, t.data._version, t.data.data_ptr()) for t in operands)
+        if hoist and entry[1] is not None:
+            for i, (now, then) in enumerate(zip(identity, entry[1])):
+                if i != slot and (hoist >> i) & 1 and now == then:
+                    skip |= 1 << i
+        # the basis operand changes every step; the others keep their moves in this workspace
+        entry[1] = tuple(x if i != slot else None for i, x in enumerate(identity))
+        ws = entry[0].data_ptr()
+    code = _native.TT_F32 if mixed else -1
+    program.krylov([t.data.data_ptr() for t in operands], slot, V.data_ptr(), V.stride(0), steps, image.data_ptr(),
+                   scratch.data_ptr(), alphas.data_ptr(), betas.data_ptr(), ws, _native.stream(), skip, code, code)
+    return True
+
+
 class Profiler:
     '''Times every block-product and leg-move launch of the grouped form while active, op by op,
     and every step launch of the eigensolver's tridiagonalisation (kind 'tridiag'), with a start and
@@ -486,14 +520,18 @@ def clear_caches() -> None:
     engine.clear_caches()
 
 
-def einsum(spec: str, *tensors: Tensor, outcharge=None) -> Tensor:
-    '''`einsum` (`tentorch/linalg/graded/einsum.cpp:444-598`, binding `bindings.cpp:347-361`).'''
+def einsum_program(spec: str, tensors: Sequence[Tensor], outcharge=None):
+    '''The planned program of an einsum over these operands' structures (cached).'''
     if not tensors:
         raise ValueError('an einsum needs at least one operand')
     charge = None if outcharge is None else tuple(outcharge)
     key = ('einsum', spec, tuple(t.structure.id for t in tensors), charge)
-    program = _Plans.get(key, lambda: engine.plan_einsum(spec, [t.structure for t in tensors], charge))
-    return run(program, tensors, label=spec)
+    return _Plans.get(key, lambda: engine.plan_einsum(spec, [t.structure for t in tensors], charge))
+
+
+def einsum(spec: str, *tensors: Tensor, outcharge=None) -> Tensor:
+    '''`einsum` (`tentorch/linalg/graded/einsum.cpp:444-598`, binding `bindings.cpp:347-361`).'''
+    return run(einsum_program(spec, tensors, outcharge), tensors, label=spec)
 
 
 def contract(left: Tensor, right: Tensor, left_legs, right_legs, charge=None) -> Tensor:
