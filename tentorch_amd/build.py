@@ -22,7 +22,7 @@ CSRC = os.path.join(HERE, 'csrc')
 BUILD = os.path.join(ROOT, '_build')
 HIP_LIB = os.path.join(HERE, 'libtentorch_hip.so')
 HIP_SOURCES = ['device.hip', 'ggemm.hip', 'mlp.hip', 'eigh.hip', 'split.hip']
-ENGINE_SOURCES = ['category.cpp', 'tree.cpp', 'move.cpp', 'program.cpp', 'bindings.cpp']
+ENGINE_SOURCES = ['category.cpp', 'tree.cpp', 'move.cpp', 'program.cpp', 'split.cpp', 'bindings.cpp']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 # The soname makes the engine record a bare library name, found through its $ORIGIN run path, so
 # the two libraries load from wherever the tree lies.

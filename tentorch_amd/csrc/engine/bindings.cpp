@@ -7,6 +7,7 @@
 #include <pybind11/stl.h>
 
 #include "program.h"
+#include "split.h"
 
 namespace py = pybind11;
 using namespace tt;
@@ -365,6 +366,32 @@ PYBIND11_MODULE(_engine, m) {
              }
              return out;
            });
+
+  py::class_<TailPass>(m, "TailPass")
+      .def(py::init<const std::vector<int64_t> &, const std::vector<int64_t> &, const std::vector<int64_t> &,
+                    const std::vector<int64_t> &, const std::vector<int64_t> &, const std::vector<int64_t> &,
+                    const std::vector<int64_t> &>(),
+           py::arg("g"), py::arg("n"), py::arg("ld"), py::arg("pq_at"), py::arg("block"), py::arg("first"),
+           py::arg("tail"))
+      .def_property_readonly("work_doubles", &TailPass::work_doubles)
+      .def_property_readonly("gram_doubles", &TailPass::gram_doubles)
+      .def_property_readonly("norm_doubles", &TailPass::norm_doubles)
+      .def_property_readonly("stage_bytes", &TailPass::stage_bytes)
+      .def_property_readonly("orders", &TailPass::orders)
+      .def_property_readonly("gram_offsets", &TailPass::gram_offsets)
+      .def("fill",
+           [](TailPass &t, uintptr_t pq, int64_t pq_bytes, uintptr_t work, uintptr_t grams, uintptr_t vectors,
+              uintptr_t norms, uintptr_t stage_device, uintptr_t stage_host,
+              py::array_t<double, py::array::c_style | py::array::forcecast> values,
+              py::array_t<int64_t, py::array::c_style | py::array::forcecast> values_at) {
+             t.fill(pq, pq_bytes, work, grams, vectors, norms, stage_device, reinterpret_cast<void *>(stage_host),
+                    values.data(), values_at.data());
+           })
+      .def("tables", [](const TailPass &t) {
+        py::list out;
+        for (const TailPass::Table &x : t.tables()) out.append(py::make_tuple(x.kind, x.count, x.items, x.at));
+        return out;
+      });
 
   m.def("plan_einsum", [](const std::string &spec, const std::vector<PyStructure> &ops, py::object charge) {
     std::vector<StructurePtr> ss;

@@ -1,0 +1,59 @@
+// Host side of the split's further passes (the two-site split, tentorch/linalg/graded/svd.cpp:41-177):
+// the launch tables of one tail pass over [P | Q] -- the decoupling rotation of each tail from its
+// head rows, then the Gram product whose eigenvectors diagonalise the tail, the projection on them
+// and its row norms -- laid out in one staging buffer, filled, and checked against the buffers they
+// name; the caller copies the buffer to the device and launches them in order.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "../../../include/tentorch_amd.h"
+
+namespace tt {
+
+class TailPass {
+ public:
+  // Blocks: g (rows of [P | Q]), n (columns of P), ld (row length of [P | Q]) and the element offset
+  // of each block's [P | Q]; heads: (block, first row of the pass, first row of the tail).
+  TailPass(const std::vector<int64_t> &g, const std::vector<int64_t> &n, const std::vector<int64_t> &ld,
+           const std::vector<int64_t> &pq_at, const std::vector<int64_t> &block, const std::vector<int64_t> &first,
+           const std::vector<int64_t> &tail);
+
+  // What the caller allocates: the decoupling's work doubles, the Gram matrices' doubles (their
+  // eigenvectors take as many again), the norms' doubles, and the staging bytes of the tables.
+  int64_t work_doubles() const { return work_total_; }
+  int64_t gram_doubles() const { return gram_total_; }
+  int64_t norm_doubles() const { return norm_total_; }
+  int64_t stage_bytes() const { return stage_total_; }
+  // Per head: the tail's order and the offset of its Gram matrix.
+  const std::vector<int64_t> &orders() const { return q_; }
+  const std::vector<int64_t> &gram_offsets() const { return sq_at_; }
+
+  // Fill the staging buffer (host memory, later copied to `stage_device`) with the values the
+  // rotation needs (squared row norms of heads and tails, from `norms` at `norms_at[block]`) and
+  // every table, addressed at the given device buffers, and check each table against the byte
+  // ranges of those buffers.  Throws std::runtime_error when a table reaches outside them.
+  void fill(uintptr_t pq, int64_t pq_bytes, uintptr_t work, uintptr_t grams, uintptr_t vectors, uintptr_t norms,
+            uintptr_t stage_device, void *stage_host, const double *norm_values, const int64_t *norms_at);
+
+  // The tables in launch order: the decoupling's seven, the Gram products, then (after the
+  // eigensolver) the projections and their row norms.
+  struct Table {
+    int kind;            // 0 bgemm, 1 lincomb, 2 row norms
+    int64_t count = 0;   // problems
+    int64_t items = 0;   // tiles or rows, as the library's check counted them
+    int64_t at = 0;      // byte offset in the staging buffer
+  };
+  const std::vector<Table> &tables() const { return tables_; }
+
+ private:
+  std::vector<int64_t> b_, f_, h_, e_, q_, w_, n_, pq_at_;
+  std::vector<int64_t> work_at_;  // 8 per head
+  std::vector<int64_t> sq_at_, nr_at_, head_at_, tail_at_;
+  int64_t work_total_ = 0, gram_total_ = 0, norm_total_ = 0, stage_total_ = 0;
+  int64_t scale_at_ = 0, shift_at_ = 0;
+  std::vector<Table> tables_;
+};
+
+}  // namespace tt

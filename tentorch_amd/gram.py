@@ -281,14 +281,64 @@ class GramSplit:
     # -- a further pass over the tails
     def _tail_pass(self, tails, values):
         '''One more pass over the tails: decoupled from their heads (DECOUPLING rotations), then
-        diagonalised; the tails' new row norms are read back into `values`.'''
+        diagonalised; the tails' new row norms are read back into `values`.  With one rotation the
+        launch tables come from the engine (`TailPass`: laid out, filled and checked in one call);
+        the launches, in the order of `_decouple` and `_diagonalise`, are the same.'''
         heads = [(b, self.first[b], h) for b, h in tails]
-        tail_rows = None
-        for iteration in range(DECOUPLING):
-            tail_rows = self._decouple(heads, values, last=iteration + 1 == DECOUPLING)
-        self._diagonalise(heads, values, tail_rows)
+        if DECOUPLING == 1:
+            self._tail_pass_tables(heads, values)
+        else:
+            tail_rows = None
+            for iteration in range(DECOUPLING):
+                tail_rows = self._decouple(heads, values, last=iteration + 1 == DECOUPLING)
+            self._diagonalise(heads, values, tail_rows)
         for b, f, h in heads:
             self.first[b] = h
+
+    def _tail_pass_tables(self, heads, values):
+        '''`_decouple` (one rotation) and `_diagonalise` from the engine's tables: one staging
+        buffer, ten table launches around the eigensolver, one readback of the tails' norms.'''
+        b = [x[0] for x in heads]
+        plan = _native.engine.TailPass(self.g.tolist(), self.n.tolist(), self.ld.tolist(), self.pq_at[:-1].tolist(),
+                                       b, [x[1] for x in heads], [x[2] for x in heads])
+        work = _doubles(self.device, plan.work_doubles)
+        grams = _doubles(self.device, 2 * plan.gram_doubles + plan.norm_doubles)
+        vectors, norms = grams[plan.gram_doubles:2 * plan.gram_doubles], grams[2 * plan.gram_doubles:]
+        stage = _Stage(self.device, [plan.stage_bytes])
+        flat = np.concatenate(values)
+        at = _offsets([v.size for v in values])
+        plan.fill(self.pq_base, self.pq.numel() * 8, work.data_ptr(), grams.data_ptr(), vectors.data_ptr(),
+                  norms.data_ptr(), stage.base, stage.host.data_ptr(), flat, at)
+        stage.send()
+        tables = plan.tables()
+        dtypes = (_native.BGEMM_PROBLEM, _native.LINCOMB_PROBLEM, _native.ROWS_PROBLEM)
+
+        def launch(kind, count, items, offset):
+            if not count or not items:
+                return
+            # the launch reads the staged table; the host copy stands in for its length
+            table = np.empty(count, dtype=dtypes[kind])
+            table.view(np.uint8)[:] = stage.view[offset:offset + table.nbytes]
+            address = stage.base + offset
+            if kind == 0:
+                _native.bgemm(table, address, items)
+            elif kind == 1:
+                _native.lincomb_rows(table, address, items)
+            else:
+                _native.row_norms(table, address, items)
+
+        for entry in tables[:8]:
+            launch(*entry)
+        orders = plan.orders
+        _native.eigh_batched(grams[:max(plan.gram_doubles, 1)], orders, plan.gram_offsets, vectors=vectors)
+        for entry in tables[8:]:
+            launch(*entry)
+        host = norms.cpu().numpy()
+        self._keep = (work, grams, stage)
+        start = 0
+        for (bb, ff, hh), q in zip(heads, orders):
+            values[bb][hh:] = host[start:start + q]
+            start += q
 
     def _decouple(self, heads, values, last: bool):
         '''The rotation exp([[0, X^T], [-X, 0]]) of every (head, tail) pair of rows of [P | Q], with
