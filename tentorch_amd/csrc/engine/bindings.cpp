@@ -367,6 +367,18 @@ PYBIND11_MODULE(_engine, m) {
              return out;
            });
 
+  m.def("choose_tails",
+        [](py::array_t<double, py::array::c_style | py::array::forcecast> norms,
+           py::array_t<int64_t, py::array::c_style | py::array::forcecast> norms_at,
+           py::array_t<int64_t, py::array::c_style | py::array::forcecast> ranks,
+           py::array_t<int64_t, py::array::c_style | py::array::forcecast> first, double floor, double reliable,
+           double resolved) {
+          const int64_t blocks = static_cast<int64_t>(ranks.size());
+          if (norms_at.size() != blocks + 1 || first.size() != blocks)
+            throw std::invalid_argument("choose_tails: arrays of unequal length");
+          return choose_tails(norms.data(), norms_at.data(), blocks, ranks.data(), first.data(), floor, reliable,
+                              resolved);
+        });
   py::class_<TailPass>(m, "TailPass")
       .def(py::init<const std::vector<int64_t> &, const std::vector<int64_t> &, const std::vector<int64_t> &,
                     const std::vector<int64_t> &, const std::vector<int64_t> &, const std::vector<int64_t> &,

@@ -1,7 +1,9 @@
 #include "split.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
+#include <limits>
 #include <stdexcept>
 #include <string>
 
@@ -48,7 +50,57 @@ tt_lincomb_problem lincomb(uintptr_t x, uintptr_t dst, int64_t rows, int64_t col
   return p;
 }
 
+// The tail of a pass starts at the widest gap between consecutive values (descending) among the
+// values within (1, 10) times `level`, so that no near-degenerate pair straddles the cut; with no
+// value there, at `first_below`.
+int64_t tail_start(const double *values, int64_t size, int64_t first_below, double level) {
+  int64_t lo_index = -1, hi_index = -1;
+  for (int64_t i = 0; i < size; ++i)
+    if (values[i] > level && values[i] < 10.0 * level) {
+      if (lo_index < 0) lo_index = i;
+      hi_index = i;
+    }
+  if (lo_index < 0) return first_below;
+  const int64_t lo = std::max<int64_t>(1, lo_index), hi = std::min<int64_t>(size - 1, hi_index + 1);
+  if (hi < lo) return first_below;
+  int64_t best = lo;
+  double widest = -1.0;
+  for (int64_t i = lo; i <= hi; ++i) {
+    const double ratio = values[i - 1] / std::max(values[i], std::numeric_limits<double>::min());
+    if (ratio > widest) {
+      widest = ratio;
+      best = i;
+    }
+  }
+  return best;
+}
+
 }  // namespace
+
+std::vector<std::pair<int64_t, int64_t>> choose_tails(const double *norms, const int64_t *norms_at, int64_t blocks,
+                                                      const int64_t *ranks, const int64_t *first, double floor,
+                                                      double reliable, double resolved) {
+  std::vector<std::pair<int64_t, int64_t>> tails;
+  for (int64_t b = 0; b < blocks; ++b) {
+    const double *v = norms + norms_at[b] + first[b];
+    const int64_t rows = norms_at[b + 1] - norms_at[b] - first[b], size = std::max<int64_t>(ranks[b] - first[b], 0);
+    if (size < 2) continue;
+    double top = 0.0;
+    for (int64_t i = 0; i < size; ++i) top = std::max(top, v[i]);
+    if (top == 0.0 || floor >= 2.0 * resolved * top) continue;
+    double unresolved = 0.0;
+    for (int64_t i = 0; i < rows; ++i)
+      if (v[i] < 2.0 * resolved * top) unresolved += v[i] * v[i];
+    if (std::sqrt(unresolved) < 0.5 * floor) continue;
+    int64_t below = -1;
+    for (int64_t i = 0; i < size && below < 0; ++i)
+      if (v[i] < reliable * top) below = i;
+    if (below < 0) continue;
+    const int64_t start = tail_start(v, size, below, reliable * top);
+    if (start > 0) tails.emplace_back(b, first[b] + start);
+  }
+  return tails;
+}
 
 TailPass::TailPass(const std::vector<int64_t> &g, const std::vector<int64_t> &n, const std::vector<int64_t> &ld,
                    const std::vector<int64_t> &pq_at, const std::vector<int64_t> &block,

@@ -6,11 +6,20 @@
 #pragma once
 
 #include <cstdint>
+#include <utility>
 #include <vector>
 
 #include "../../../include/tentorch_amd.h"
 
 namespace tt {
+
+// The blocks whose current pass needs another, as (block, first row of the tail): the rule of
+// tentorch_amd.gram.choose_tails (its docstring gives the reasoning) over every block's row norms
+// `norms` (block b's at norms_at[b] .. norms_at[b + 1]), the pass of block b covering rows first[b]..
+// and its values being the first ranks[b] rows.
+std::vector<std::pair<int64_t, int64_t>> choose_tails(const double *norms, const int64_t *norms_at, int64_t blocks,
+                                                      const int64_t *ranks, const int64_t *first, double floor,
+                                                      double reliable, double resolved);
 
 class TailPass {
  public:

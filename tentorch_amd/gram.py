@@ -275,8 +275,9 @@ class GramSplit:
         launches.run(0, 1)
         _native.eigh_batched(grams, g.tolist(), sq_at[:-1].tolist(), vectors=vectors)
         launches.run(1)
-        host = norms.cpu().numpy()
-        return [host[g_at[b]:g_at[b + 1]].copy() for b in range(count)]
+        host = norms.cpu().numpy().copy()
+        self.norms_flat, self.norms_at = host, g_at
+        return [host[g_at[b]:g_at[b + 1]] for b in range(count)]     # views: updates reach the flat array
 
     # -- a further pass over the tails
     def _tail_pass(self, tails, values):
@@ -432,9 +433,13 @@ class GramSplit:
         norms = self._first_pass()          # every row of every block; the first r are the values
         self.first = [0] * self.count
         ranks = self.r.tolist()
+        rank_array = np.asarray(ranks, dtype=np.int64)
         while True:
-            tails = choose_tails(norms, ranks, self.first, cut([v[:r] for v, r in zip(norms, ranks)]), reliable,
-                                 resolved)
+            # the rule of choose_tails, run by the engine over the flat row norms
+            floor = cut([v[:r] for v, r in zip(norms, ranks)])
+            tails = [tuple(t) for t in _native.engine.choose_tails(
+                self.norms_flat, self.norms_at, rank_array, np.asarray(self.first, dtype=np.int64), floor, reliable,
+                resolved)]
             if not tails:
                 break
             self._tail_pass(tails, norms)

@@ -123,3 +123,32 @@ def test_tridiagonal_step_launches_and_bytes():
     # 65 matrices make two batches: the second (one matrix of order 131) steps three times
     launches, _ = _native.tridiagonal_steps([200] + [10] * 63 + [131])
     assert launches == 72 + 3
+
+
+def test_the_engine_chooses_the_same_tails_as_the_rule():
+    '''The split's further passes are decided by the engine's port of `gram.choose_tails`: on random
+    descending spectra (exact zeros, ties, passes already started, caps anywhere) both pick the same
+    blocks and the same tail starts.'''
+    import numpy as np
+    from tentorch_amd import gram
+    from tentorch_amd._native import engine
+    rng = np.random.default_rng(1)
+    for trial in range(2000):
+        norms, ranks, first = [], [], []
+        for _ in range(int(rng.integers(1, 12))):
+            g = int(rng.integers(0, 40))
+            v = np.sort(np.exp(-rng.uniform(0, 37, g)))[::-1] * rng.uniform(0.1, 2)
+            if rng.random() < 0.2 and g:
+                v[rng.integers(0, g):] = 0.0
+            if rng.random() < 0.1 and g > 3:
+                v[1] = v[2]
+            r = int(rng.integers(0, g + 1))
+            norms.append(v)
+            ranks.append(r)
+            first.append(int(rng.integers(0, r + 1)) if rng.random() < 0.3 else 0)
+        floor = float(np.exp(-rng.uniform(0, 37)))
+        want = gram.choose_tails(norms, ranks, first, floor, 1e-5, 1e-7)
+        at = np.concatenate([[0], np.cumsum([v.size for v in norms])]).astype(np.int64)
+        got = engine.choose_tails(np.concatenate(norms), at, np.array(ranks, dtype=np.int64),
+                                  np.array(first, dtype=np.int64), floor, 1e-5, 1e-7)
+        assert [tuple(x) for x in got] == want, trial
