@@ -187,7 +187,8 @@ class DMRG:
         value, vector = self._solve(head, tail, start)
         # absorb 2 leaves the left factor isometric (the centre moves right), absorb 1 the right one
         absorb = 2 if direction == 'right' else 1
-        isometry_left, _, rest, kept = _split(vector, None, self.max_bond_dim, self.svd_tol, absorb)
+        isometry_left, _, rest, kept = _split(vector, None, self.max_bond_dim, self.svd_tol, absorb,
+                                              device_values=False)
         energy = value.get()
         self.state.tensors[at], self.state.tensors[at + 1] = isometry_left, rest.place([0, 1])
         envs.changed(at, at + 1)

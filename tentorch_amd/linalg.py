@@ -237,10 +237,12 @@ def _paired_prefixes(cand_owner: np.ndarray, cand_position: np.ndarray, sectors,
     return kept
 
 
-def _split(tensor: Tensor, left: Optional[Sequence[int]], max_bond: Optional[int], tol: float, absorb: int):
+def _split(tensor: Tensor, left: Optional[Sequence[int]], max_bond: Optional[int], tol: float, absorb: int,
+           device_values: bool = True):
     '''`svd`, plus the kept singular values on the host (FP64, in the order of `values`), which the
     split reads back anyway to rank them: a caller that needs their sum needs no further
-    synchronisation.'''
+    synchronisation.  `device_values` False leaves `values` None for a caller that uses only the host
+    copy (no upload).'''
     if absorb not in (0, 1, 2, 3):
         raise ValueError(f'absorb has to be 0, 1, 2 or 3, got {absorb}')
     if max_bond is not None and max_bond < 1:
@@ -390,7 +392,7 @@ def _split(tensor: Tensor, left: Optional[Sequence[int]], max_bond: Optional[int
         values[placed[s] + o:placed[s] + o + k] = v.to(arranged.dtype)
     if gram and device_split is not None:
         device_split.assemble(placements, spans)
-    if gram:
+    if gram and device_values:
         # one copy from pinned memory, queued behind the products without waiting for them
         staged = torch.from_numpy(kept_values).to(arranged.dtype)
         if arranged.device.type == 'cuda':
