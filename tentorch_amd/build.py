@@ -26,8 +26,11 @@ ENGINE_SOURCES = ['category.cpp', 'tree.cpp', 'move.cpp', 'program.cpp', 'split.
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 # The soname makes the engine record a bare library name, found through its $ORIGIN run path, so
 # the two libraries load from wherever the tree lies.
+# Kernel arguments are preloaded into scalar registers (gfx950 supports up to 16 dwords): a launch
+# then starts its first memory access without waiting for a load of its arguments, which shortens
+# every latency-bound launch (the eigensolver's steps most of all).
 HIP_FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared', '-Wl,-soname,libtentorch_hip.so',
-             '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
+             '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops', '-mllvm', '-amdgpu-kernarg-preload-count=16']
 
 
 def engine_path() -> str:

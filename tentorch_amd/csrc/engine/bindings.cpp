@@ -81,6 +81,15 @@ py::object tree_object(const FusionTree &t) {
 
 }  // namespace
 
+// (kept per block, bond offset per block, bond sectors, bond multiplicities, kept values) of a split ranking.
+py::tuple ranking_tuple(const SplitRanking &r) {
+  py::list sectors;
+  for (const Sector &x : r.sectors) sectors.append(from_sector(x));
+  py::array_t<double> kept_values(static_cast<py::ssize_t>(r.values.size()));
+  std::copy(r.values.begin(), r.values.end(), kept_values.mutable_data());
+  return py::make_tuple(r.kept, r.offset, sectors, r.mults, kept_values);
+}
+
 PYBIND11_MODULE(_engine, m) {
   m.doc() = "TenTorch MI355X host engine";
 
@@ -378,6 +387,22 @@ PYBIND11_MODULE(_engine, m) {
             throw std::invalid_argument("choose_tails: arrays of unequal length");
           return choose_tails(norms.data(), norms_at.data(), blocks, ranks.data(), first.data(), floor, reliable,
                               resolved);
+        });
+  m.def("rank_split",
+        [](const PyStructure &s, const std::vector<int64_t> &blocks,
+           py::array_t<double, py::array::c_style | py::array::forcecast> values, const std::vector<int64_t> &at,
+           int64_t max_bond, double tol) {
+          if (values.size() < (at.empty() ? 0 : at.back())) throw std::invalid_argument("rank_split: too few values");
+          return ranking_tuple(rank_split(*s.s, blocks, values.data(), at, max_bond, tol));
+        });
+  m.def("rank_split",
+        [](const PyCategory &c, const py::list &sectors,
+           py::array_t<double, py::array::c_style | py::array::forcecast> values, const std::vector<int64_t> &at,
+           int64_t max_bond, double tol) {
+          if (values.size() < (at.empty() ? 0 : at.back())) throw std::invalid_argument("rank_split: too few values");
+          std::vector<Sector> ss;
+          for (const auto &h : sectors) ss.push_back(to_sector(h));
+          return ranking_tuple(rank_split(*c.cat, ss, values.data(), at, max_bond, tol));
         });
   py::class_<TailPass>(m, "TailPass")
       .def(py::init<const std::vector<int64_t> &, const std::vector<int64_t> &, const std::vector<int64_t> &,

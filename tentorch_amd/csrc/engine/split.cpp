@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <stdexcept>
 #include <string>
 
@@ -100,6 +101,93 @@ std::vector<std::pair<int64_t, int64_t>> choose_tails(const double *norms, const
     if (start > 0) tails.emplace_back(b, first[b] + start);
   }
   return tails;
+}
+
+SplitRanking rank_split(const Structure &structure, const std::vector<int64_t> &blocks, const double *values,
+                        const std::vector<int64_t> &at, int64_t max_bond, double tol) {
+  std::vector<Sector> sector(blocks.size());
+  for (size_t i = 0; i < blocks.size(); ++i) {
+    const int64_t b = blocks[i];
+    if (b < 0 || b >= static_cast<int64_t>(structure.blocks().size())) throw std::invalid_argument("rank_split: block");
+    sector[i] = structure.blocks()[static_cast<size_t>(b)].key.domain;
+  }
+  return rank_split(*structure.cat(), sector, values, at, max_bond, tol);
+}
+
+SplitRanking rank_split(const Category &cat, const std::vector<Sector> &sector, const double *values,
+                        const std::vector<int64_t> &at, int64_t max_bond, double tol) {
+  const int64_t count = static_cast<int64_t>(sector.size());
+  if (static_cast<int64_t>(at.size()) != count + 1 || at.front() != 0)
+    throw std::invalid_argument("rank_split: value offsets");
+  for (int64_t i = 0; i < count; ++i)
+    if (at[static_cast<size_t>(i) + 1] < at[static_cast<size_t>(i)]) throw std::invalid_argument("rank_split: value offsets");
+  const int64_t total = at.back();
+  for (int64_t i = 0; i < total; ++i)
+    if (std::isnan(values[i])) throw std::invalid_argument("a singular value is not a number");
+  // candidates, largest first; a stable sort keeps ties in (block, position) order
+  std::vector<int64_t> order(static_cast<size_t>(total));
+  for (int64_t i = 0; i < total; ++i) order[static_cast<size_t>(i)] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return values[x] > values[y]; });
+  int64_t live = 0;
+  while (live < total && !(values[order[static_cast<size_t>(live)]] <= tol)) ++live;
+  if (max_bond >= 0) live = std::min(live, max_bond);
+  std::vector<int64_t> owner(static_cast<size_t>(total));
+  for (int64_t i = 0; i < count; ++i)
+    for (int64_t j = at[static_cast<size_t>(i)]; j < at[static_cast<size_t>(i) + 1]; ++j) owner[static_cast<size_t>(j)] = i;
+  SplitRanking out;
+  out.kept.assign(static_cast<size_t>(count), 0);
+  for (int64_t c = 0; c < live; ++c) {
+    const int64_t j = order[static_cast<size_t>(c)], i = owner[static_cast<size_t>(j)];
+    out.kept[static_cast<size_t>(i)] = std::max(out.kept[static_cast<size_t>(i)], j - at[static_cast<size_t>(i)] + 1);
+  }
+  std::map<Sector, int64_t> mults;
+  for (int64_t i = 0; i < count; ++i)
+    if (out.kept[static_cast<size_t>(i)]) mults[sector[static_cast<size_t>(i)]] += out.kept[static_cast<size_t>(i)];
+  std::map<Sector, int64_t> paired;
+  for (const auto &[s, m] : mults) {
+    const auto other = mults.find(cat.dual(s));
+    if (other != mults.end()) paired[s] = std::min(m, other->second);
+  }
+  if (!paired.empty() && paired != mults) {
+    // a sector keeps its first candidates, up to its paired count (none without a dual)
+    std::map<Sector, int64_t> taken;
+    std::fill(out.kept.begin(), out.kept.end(), 0);
+    for (int64_t c = 0; c < live; ++c) {
+      const int64_t j = order[static_cast<size_t>(c)], i = owner[static_cast<size_t>(j)];
+      const Sector &s = sector[static_cast<size_t>(i)];
+      const auto room = paired.find(s);
+      const int64_t rank = taken[s]++;
+      if (room != paired.end() && rank < room->second)
+        out.kept[static_cast<size_t>(i)] = std::max(out.kept[static_cast<size_t>(i)], j - at[static_cast<size_t>(i)] + 1);
+    }
+    mults = paired;
+  }
+  std::map<Sector, int64_t> next, placed;
+  out.offset.resize(static_cast<size_t>(count));
+  for (int64_t i = 0; i < count; ++i) {
+    int64_t &n = next[sector[static_cast<size_t>(i)]];
+    out.offset[static_cast<size_t>(i)] = n;
+    n += out.kept[static_cast<size_t>(i)];
+  }
+  int64_t run = 0;
+  for (const auto &[s, m] : mults) {
+    placed[s] = run;
+    run += m;
+    out.sectors.push_back(s);
+    out.mults.push_back(m);
+  }
+  out.values.assign(static_cast<size_t>(run == 0 ? 1 : run), 0.0);
+  for (int64_t i = 0; i < count; ++i) {
+    const int64_t k = out.kept[static_cast<size_t>(i)];
+    if (!k) continue;
+    const int64_t to = placed[sector[static_cast<size_t>(i)]] + out.offset[static_cast<size_t>(i)];
+    std::copy(values + at[static_cast<size_t>(i)], values + at[static_cast<size_t>(i)] + k, out.values.begin() + to);
+  }
+  if (mults.empty()) {
+    out.sectors.push_back(cat.trivial());
+    out.mults.push_back(1);
+  }
+  return out;
 }
 
 TailPass::TailPass(const std::vector<int64_t> &g, const std::vector<int64_t> &n, const std::vector<int64_t> &ld,

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../../include/tentorch_amd.h"
+#include "tree.h"
 
 namespace tt {
 
@@ -20,6 +21,28 @@ namespace tt {
 std::vector<std::pair<int64_t, int64_t>> choose_tails(const double *norms, const int64_t *norms_at, int64_t blocks,
                                                       const int64_t *ranks, const int64_t *first, double floor,
                                                       double reliable, double resolved);
+
+// The ranking of split_over (tentorch/linalg/graded/svd.cpp:72-136) over the singular values of
+// blocks with domain sectors `sectors` (values[at[i] .. at[i+1]) descending for block i; a NaN is an
+// error): candidates largest
+// first (ties in block and position order), the ones at or below `tol` dropped, the first `max_bond`
+// kept (all when negative), each block keeping the prefix up to its last candidate; where a bond
+// sector's dual is on the bond too, both keep min of their counts, re-walking the candidates so a
+// sector keeps its first ones.  Fills the kept count and the bond offset of every block, the bond's
+// sectors and multiplicities (sorted; the trivial sector once when nothing is kept), and the kept
+// values in bond order.
+struct SplitRanking {
+  std::vector<int64_t> kept, offset;
+  std::vector<Sector> sectors;
+  std::vector<int64_t> mults;
+  std::vector<double> values;
+};
+SplitRanking rank_split(const Category &cat, const std::vector<Sector> &sectors, const double *values,
+                        const std::vector<int64_t> &at, int64_t max_bond, double tol);
+// The same over the blocks `blocks` (indices into structure.blocks()), each block's sector its key's
+// domain sector.
+SplitRanking rank_split(const Structure &structure, const std::vector<int64_t> &blocks, const double *values,
+                        const std::vector<int64_t> &at, int64_t max_bond, double tol);
 
 class TailPass {
  public:

@@ -210,33 +210,6 @@ def svd(tensor: Tensor, *args, **kwargs) -> Tuple[Tensor, torch.Tensor, Tensor]:
     return _split(tensor, left, max_bond, tol, absorb)[:3]
 
 
-def _paired_prefixes(cand_owner: np.ndarray, cand_position: np.ndarray, sectors, room) -> np.ndarray:
-    '''The re-pairing of `split_over` (`svd.cpp:97-119`): walking the ranked candidates (block
-    `cand_owner[i]`, singular value `cand_position[i]`, largest first), keep a candidate while its
-    block's sector has kept fewer than `room[sector]` (a sector without room keeps none); a block keeps
-    the prefix up to its last kept candidate.  Vectorised: a candidate's rank among the candidates of
-    its sector decides.  Returns the kept count of every block.'''
-    kept = np.zeros(len(sectors), dtype=np.int64)
-    if cand_owner.size == 0:
-        return kept
-    ids, sector_id = {}, np.empty(len(sectors), dtype=np.int64)
-    for fi, s in enumerate(sectors):
-        sector_id[fi] = ids.setdefault(s, len(ids))
-    limit = np.full(len(ids), -1, dtype=np.int64)
-    for s, i in ids.items():
-        if s in room:
-            limit[i] = room[s]
-    cand_sector = sector_id[cand_owner]
-    by_sector = np.argsort(cand_sector, kind='stable')
-    sorted_sector = cand_sector[by_sector]
-    starts = np.searchsorted(sorted_sector, sorted_sector, side='left')
-    rank = np.empty_like(cand_sector)
-    rank[by_sector] = np.arange(sorted_sector.size) - starts
-    keep = rank < limit[cand_sector]
-    np.maximum.at(kept, cand_owner[keep], cand_position[keep] + 1)
-    return kept
-
-
 def _split(tensor: Tensor, left: Optional[Sequence[int]], max_bond: Optional[int], tol: float, absorb: int,
            device_values: bool = True):
     '''`svd`, plus the kept singular values on the host (FP64, in the order of `values`), which the
@@ -251,7 +224,9 @@ def _split(tensor: Tensor, left: Optional[Sequence[int]], max_bond: Optional[int
     cat = arranged.cat
     factors = []
     # (key, rows, cols, element offset) of every non-empty block, without building views
-    layout = [(BlockKey(*key), rows, cols, off) for key, rows, cols, off in arranged.structure.blocks() if rows * cols]
+    blocks = arranged.structure.blocks()
+    nonempty = [i for i, (_, rows, cols, _) in enumerate(blocks) if rows * cols]
+    layout = [(BlockKey(*blocks[i][0]),) + tuple(blocks[i][1:]) for i in nonempty]
     gram = SvdSettings.method == 'gram' and absorb in (1, 2) and not arranged.dtype.is_complex
     device_split = None
 
@@ -300,50 +275,19 @@ def _split(tensor: Tensor, left: Optional[Sequence[int]], max_bond: Optional[int
                 f['u'], f['v'], f['vh'] = f['u'][:, index], f['v'][index], f['vh'][index]
                 host[at:at + size] = segment[order]
             at += size
-    # global ranking, largest first; a stable sort keeps ties in (block, index) order
-    owner = np.repeat(np.arange(len(factors)), sizes)
-    position = np.concatenate([np.arange(n) for n in sizes]) if factors else np.zeros(0, dtype=np.int64)
-    order = np.argsort(-host, kind='stable')
-    above = np.nonzero(host[order] <= tol)[0]
-    order = order[:above[0] if above.size else order.size]
-    if max_bond is not None:
-        order = order[:max_bond]
-    cand_owner, cand_position = owner[order], position[order]
-    kept = np.zeros(len(factors), dtype=np.int64)
-    np.maximum.at(kept, cand_owner, cand_position + 1)
+    # the global ranking (svd.cpp:80-119): candidates largest first, ties in (block, index) order,
+    # cut at the tolerance and the cap, dual sectors re-paired; the engine returns every block's kept
+    # count and bond offset, the bond and the kept values in bond order
+    starts = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64) if factors else np.zeros(1, dtype=np.int64)
+    kept, offsets, bond_sectors, bond_mults, kept_values = engine.rank_split(
+        arranged.structure, nonempty, host, starts.tolist(), -1 if max_bond is None else max_bond, tol)
     for fi, f in enumerate(factors):
-        f['kept'] = int(kept[fi])
-    mults = {}
-    for f in factors:
-        if f['kept']:
-            s = f['key'].domain
-            mults[s] = mults.get(s, 0) + f['kept']
-    paired = {}
-    for sector, count in mults.items():
-        other = mults.get(cat.dual(sector))
-        if other is not None:
-            paired[sector] = min(count, other)
-    if paired and paired != mults:
-        sectors = [f['key'].domain for f in factors]
-        kept = _paired_prefixes(cand_owner, cand_position, sectors, paired)
-        for fi, f in enumerate(factors):
-            f['kept'] = int(kept[fi])
-        mults = paired
-    nxt = {}
-    for f in factors:
-        s = f['key'].domain
-        f['offset'] = nxt.get(s, 0)
-        nxt[s] = f['offset'] + f['kept']
-    placed, run = {}, 0
-    for sector in sorted(mults):
-        placed[sector] = run
-        run += mults[sector]
-    bond = _bond_leg(cat, mults)
+        f['kept'], f['offset'] = kept[fi], offsets[fi]
+    placed = dict(zip(bond_sectors, np.concatenate([[0], np.cumsum(bond_mults)[:-1]]).tolist()))
+    bond = engine.Leg(cat, bond_sectors, bond_mults)
     first = Tensor(cat, arranged.codomain, [bond], arranged.qtotal, arranged.dtype, arranged.device)
     second = Tensor(cat, [bond], arranged.domain, cat.trivial(), arranged.dtype, arranged.device)
-    kept_values = np.zeros(bond.multiplets())
     values = None if gram else torch.zeros(bond.multiplets(), dtype=arranged.dtype, device=arranged.device)
-    starts = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64) if factors else np.zeros(1, dtype=np.int64)
     placements = []
     if gram and device_split is not None:
         # the blocks' places in the two new buffers, read once from their structures
@@ -357,7 +301,6 @@ def _split(tensor: Tensor, left: Optional[Sequence[int]], max_bond: Optional[int
             continue
         r, s, c = f['key']
         o = f['offset']
-        kept_values[placed[s] + o:placed[s] + o + k] = host[starts[fi]:starts[fi] + k]
         if gram and device_split is not None:
             # (block, kept, first block's address of column o, its row length, second block's address
             # of row o, its row length)
