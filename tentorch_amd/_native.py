@@ -212,14 +212,14 @@ def eigh_workspace(sizes) -> int:
 eigh_observer = None
 
 # orders up to this are tridiagonalised in one launch per batch; larger ones step by step above it
-EIGH_STEPPED_ABOVE = 128
+EIGH_STEPPED_ABOVE = 192
 EIGH_BATCH = 64
 
 
 def tridiagonal_steps(sizes):
     '''(launches, algorithmic bytes) of the step-by-step tridiagonalisation tt_eigh_batched runs
     for matrices of these orders: per batch of 64, one launch for each step k below the batch's
-    largest order minus 128.  Step k of a matrix with m = n - k - 1 > 0 trailing columns reads and
+    largest order minus 192.  Step k of a matrix with m = n - k - 1 > 0 trailing columns reads and
     writes its trailing block once and its vectors (column k, v and p of the previous step, v and p
     of this one): 8 (2 m^2 + 6 m) bytes.'''
     launches, moved = 0, 0

@@ -4,9 +4,10 @@
 //
 //   1. Householder tridiagonalisation.  The rank-2 update of step k is deferred into step k+1's
 //      symmetric matrix-vector pass, so each step reads and writes the trailing matrix once.  The
-//      reflectors stay in the matrix's lower part.  While a trailing matrix exceeds 128 rows, one
-//      launch per step spreads every matrix over one wave per trailing column; the last 128 steps
-//      of every matrix run in one launch, one workgroup per matrix, in LDS.
+//      reflectors stay in the matrix's lower part.  While a trailing matrix exceeds 192 rows, one
+//      launch per step spreads every matrix over one wave per trailing column; the last 192 steps
+//      of every matrix run in one launch, one workgroup per matrix, the trailing matrix held in
+//      the workgroup's registers.
 //   2. Cuppen divide and conquer on the tridiagonal: leaves of 8 rows by implicit QL (one wave
 //      each), then merges bottom up, four launches per level for every matrix of the batch: one
 //      workgroup per merge deflates small coupling components and close poles (Givens rotations);
@@ -139,20 +140,9 @@ __device__ inline bool locate(const Batch *__restrict__ batch, int item, Count c
 // 1. Tridiagonalisation.  A is column-major n x n (symmetric, so either order), in place.
 //    Output: diag[n], off[n-1], tau[n-1]; reflector k is A[k+1.., k] with A[k+1, k] = 1.  The
 //    rank-2 update of step k is deferred into step k+1's pass over the trailing matrix.  While a
-//    trailing matrix exceeds kLdsOrder rows the steps run one launch each over many workgroups
-//    (1'); the last kLdsOrder steps of every matrix run in one launch in LDS (1'').
-
-constexpr int kTriThreads = 1024;
-
-__device__ double tri_block_sum(double v, double *scratch) {
-  v = wave_sum(v);
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) scratch[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double total = 0.0;
-  for (int w = 0; w < kTriThreads / 64; ++w) total += scratch[w];
-  return total;
-}
+//    trailing matrix exceeds kRegOrder rows the steps run one launch each over many workgroups
+//    (1'); the last kRegOrder steps of every matrix run in one launch, one workgroup per matrix
+//    with the trailing matrix in registers (1'').
 
 // 1'. The steps while trailing matrices are large: one launch per Householder step k, one wave per
 //     trailing column of every matrix, four waves a workgroup.  Each wave works alone, with no
@@ -270,81 +260,137 @@ __global__ __launch_bounds__(kStepThreads) void tridiag_step_kernel(const Batch 
   }
 }
 
-// 1''. The last kLdsOrder steps of every matrix in one launch, one workgroup per matrix: the
-//      trailing matrix (order <= kLdsOrder) is loaded into LDS with the state the stepped launches
-//      left (v, p and tau of step k0-1, whose rank-2 update is still pending), and the remaining
-//      steps run there with no memory traffic but the reflectors and scalars they produce.  A step
-//      takes two barriers: wave 0 forms column kk with the pending update and the step's reflector
-//      (its two sums are wave reductions over the column's rows, two per lane); then every thread
-//      applies the pending update to the trailing rows (eight lanes a row, their partial products
-//      summed by shuffles) and forms p.  The pending update is (v, p, tau) of the previous step in
-//      LDS, double-buffered by step parity; each wave forms w = p - tau/2 (p . v) v where it needs
-//      it, with p . v by a wave reduction of its own.  Rows are padded to 132 doubles, which puts
-//      the 8 rows x 8 columns a wave touches in the trailing pass on distinct bank pairs (2-way,
-//      the least a wave's 64 doubles can take; 129 gave 8-way).  With k0 = 0 it is the whole
-//      reduction of a batch of small matrices.
+// 1''. The last kRegOrder steps of every matrix in one launch, one workgroup of eight waves per
+//      matrix, with the trailing matrix in registers.  Wave w holds rows 24w..24w+23 of it: lane
+//      (row group g = lane / 16, column lane l = lane % 16) holds rows 24w + 6g + a (a < 6) of
+//      columns l + 16b (b < 12), 72 entries.  Each step takes three barriers: the lanes holding
+//      column kk write it, with the pending update, to LDS; every wave then forms the step's
+//      reflector from it by itself (one wave reduction), applies the pending update to its
+//      entries and writes its rows' partial products with the reflector to LDS, one per column
+//      lane; after the second barrier one thread per row sums its 16 partials in order (p = tau
+//      A v), and after the third every wave forms p . v by itself.  The pending update (v, p and
+//      tau/2 (p . v) of the last step) sits in LDS, double-buffered by step parity.  Finished rows
+//      and columns carry zero vectors, so the update leaves them unchanged; padding past the
+//      matrix's order is zero, and the update skips waves and column groups wholly finished or
+//      past the order.  Against 64 stepped launches and the 128-step LDS launch this kernel
+//      replaces, a converged chi = 1024 sweep ran 3.4 % faster (same saved state, same box).
+//      Takes over from the stepped launches with the state they left (k0 > 0), or reduces a whole
+//      matrix of order <= kRegOrder (k0 = 0).
 
-constexpr int kLdsOrder = 128;
-constexpr int kLdsStride = kLdsOrder + 4;
-static_assert(kLdsOrder * 8 == kTriThreads, "eight lanes per trailing row");
-static_assert(kLdsOrder <= 2 * 64, "wave 0 holds a column in two rows per lane");
+constexpr int kRegOrder = 192;
+constexpr int kRegWaves = 8;
+constexpr int kRegThreads = 64 * kRegWaves;
+constexpr int kRegRows = 6;                   // rows per lane
+constexpr int kRegCols = 12;                  // columns per lane
+static_assert(kRegWaves * 4 * kRegRows == kRegOrder, "the waves' four row groups cover the rows");
+static_assert(16 * kRegCols == kRegOrder, "sixteen column lanes cover the columns");
+constexpr int kRegSpan = 3 * 64;              // LDS vector length: a wave reads three entries a lane
+static_assert(kRegOrder <= kRegSpan, "a wave holds a column in three rows per lane");
 
-__global__ __launch_bounds__(kTriThreads) void tridiag_lds_kernel(const Batch *__restrict__ batch,
+// The workgroup barrier of the register kernel: its waves share only LDS, so the fences cover LDS
+// alone and the barrier does not wait for the reflectors' global stores.
+__device__ inline void reg_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+__global__ __launch_bounds__(kRegThreads) void tridiag_reg_kernel(const Batch *__restrict__ batch,
                                                                    double *__restrict__ matrices,
                                                                    double *__restrict__ work, int k0) {
   const Problem pr = batch->p[blockIdx.x];
   const int n = pr.n;
   if (n - 1 < k0) return;   // finished by the stepped launches
-  const int m = n - k0;     // at most kLdsOrder (the host starts this launch late enough)
+  const int m = n - k0;     // at most kRegOrder (the host starts this launch late enough)
   double *__restrict__ A = matrices + pr.matrix;
   double *diag = work + pr.work + 3 * int64_t(n) * n, *off = diag + n, *tau = off + n;
   const double *vbuf = tau + n, *pbuf = vbuf + 2 * n;
-  __shared__ double T[kLdsOrder * kLdsStride];
-  __shared__ double vl[2][kLdsOrder], pl[2][kLdsOrder], tl[2];
-  const int t = threadIdx.x, lane = t & 63;
-  const bool first_wave = t < 64;
-  for (int e = t; e < m * m; e += kTriThreads) {
-    const int i = e % m, j = e / m;
-    T[i * kLdsStride + j] = A[(k0 + i) + int64_t(k0 + j) * n];
+  __shared__ double X[2][kRegSpan], V[2][kRegSpan], P[2][kRegSpan];
+  __shared__ double S[kRegOrder][16];   // the rows' partial products, by column lane
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  const int group = lane >> 4, col_lane = lane & 15;
+  const int row0 = wave * 4 * kRegRows + group * kRegRows;   // this lane's first row
+  // the trailing matrix, zero past its order (entries past it load the first entry, then drop it:
+  // the loads stay unconditional)
+  double T[kRegRows][kRegCols];
+#pragma unroll
+  for (int b = 0; b < kRegCols; ++b) {
+    const int c = col_lane + 16 * b;
+#pragma unroll
+    for (int a = 0; a < kRegRows; ++a) {
+      const int r = row0 + a;
+      const bool in = r < m && c < m;
+      const double y = A[in ? (k0 + r) + int64_t(k0 + c) * n : k0 + int64_t(k0) * n];
+      T[a][b] = in ? y : 0.0;
+    }
   }
-  // the pending update of step k0-1 (none before the first step) in parity 1; its reflector goes
-  // to column k0-1
-  if (t < kLdsOrder) {
-    const bool on = k0 > 0 && t < m;
-    const double v = on ? vbuf[((k0 - 1) & 1) * n + k0 + t] : 0.0;
-    vl[1][t] = v;
-    pl[1][t] = on ? pbuf[((k0 - 1) & 1) * n + k0 + t] : 0.0;
-    if (on) A[(k0 + t) + int64_t(k0 - 1) * n] = v;
+  // the pending update of step k0-1 (none before the first step) in parity 1, zero past the order
+  // (the vectors' entries up to kRegSpan are read); its reflector goes to column k0-1
+  for (int r = t; r < kRegSpan; r += kRegThreads) {
+    const bool on = k0 > 0 && r < m;
+    const double v = on ? vbuf[((k0 - 1) & 1) * n + k0 + r] : 0.0;
+    V[1][r] = v;
+    P[1][r] = on ? pbuf[((k0 - 1) & 1) * n + k0 + r] : 0.0;
+    V[0][r] = 0.0;
+    P[0][r] = 0.0;
+    X[0][r] = 0.0;
+    X[1][r] = 0.0;
+    if (on) A[(k0 + r) + int64_t(k0 - 1) * n] = v;
   }
-  if (t == 0) tl[1] = k0 > 0 ? tau[k0 - 1] : 0.0;
+  for (int e = t; e < kRegOrder * 16; e += kRegThreads) S[e >> 4][e & 15] = 0.0;
+  const double tprev = k0 > 0 ? tau[k0 - 1] : 0.0;
   __syncthreads();
-  // half = tau/2 (p . v) of the pending step, by every wave for itself (rows lane, lane + 64)
-  auto half_of = [&](int buf, int from) {
+  // tau/2 (p . v) of the pending step in buffer `buf`, over rows `from`.., by every wave for itself
+  // (every LDS entry holds a finite value, so the reads are unconditional and the rows masked)
+  auto half_of = [&](int buf, int from, double tk) {
     double partial = 0.0;
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < 3; ++s) {
       const int r = lane + 64 * s;
-      if (r >= from && r < m) partial += pl[buf][r] * vl[buf][r];
+      const double term = P[buf][r] * V[buf][r];
+      partial += r >= from ? term : 0.0;
     }
-    return 0.5 * tl[buf] * wave_sum(partial);
+    return 0.5 * tk * wave_sum(partial);
   };
-  double half = half_of(1, 0);
-  for (int kk = 0; kk + 1 < m; ++kk) {
-    const int k = k0 + kk, par = kk & 1, prev = par ^ 1;
-    // (a) wave 0: column kk (rows kk..) with the pending update, and the reflector of this step
-    if (first_wave) {
-      const double vk = vl[prev][kk], wk = pl[prev][kk] - half * vk;
-      double x[2], partial = 0.0;
+  double half = half_of(1, 0, tprev);
+  int pending = 1;   // the parity of the pending update
+  const int wave_end = (wave + 1) * 4 * kRegRows;   // one past this wave's last row
+  // The steps in blocks of 16, one per column group B: the column of step kk sits at b = B in the
+  // lanes with l = kk % 16, and the groups before B are finished, so with B a constant (the loop
+  // over B is unrolled) the column is a plain register and the update skips the finished groups.
 #pragma unroll
-      for (int s = 0; s < 2; ++s) {
+  for (int B = 0; B < kRegCols; ++B) {
+    const int block_end = min(16 * B + 16, m - 1);
+    for (int kk = 16 * B; kk < block_end; ++kk) {
+      const int k = k0 + kk, par = kk & 1, prev = pending;
+      // the pending vectors at this lane's rows, and at column kk
+      double vr[kRegRows], wr[kRegRows];
+#pragma unroll
+      for (int a = 0; a < kRegRows; ++a) {
+        vr[a] = V[prev][row0 + a];
+        wr[a] = P[prev][row0 + a] - half * vr[a];
+      }
+      const double vk = V[prev][kk], wk = P[prev][kk] - half * vk;
+      // (a) the lanes holding column kk write it, rows kk.., with the pending update
+      if (col_lane == (kk & 15)) {
+#pragma unroll
+        for (int a = 0; a < kRegRows; ++a) {
+          const int r = row0 + a;
+          if (r >= kk && r < m) X[par][r] = T[a][B] - (vr[a] * wk + wr[a] * vk);
+        }
+      }
+      reg_barrier();
+      // (b) every wave: the reflector of this step from column kk
+      double x[3], partial = 0.0;
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
         const int r = lane + 64 * s;
-        const double vr = r < m ? vl[prev][r] : 0.0, wr = r < m ? pl[prev][r] - half * vr : 0.0;
-        x[s] = r >= kk && r < m ? T[r * kLdsStride + kk] - (vr * wk + wr * vk) : 0.0;
-        if (r >= kk + 2) partial += x[s] * x[s];
+        const double y = X[par][r];
+        x[s] = r >= kk ? y : 0.0;
+        partial += r >= kk + 2 ? y * y : 0.0;
       }
       const double sigma = wave_sum(partial);
-      const double dk = readlane_double(kk < 64 ? x[0] : x[1], kk & 63);
-      const double alpha = readlane_double(kk + 1 < 64 ? x[0] : x[1], (kk + 1) & 63);
+      const double dk = X[par][kk], alpha = X[par][kk + 1];
       double beta = alpha, tk = 0.0, scale = 0.0;
       if (sigma != 0.0) {
         const double norm = sqrt(alpha * alpha + sigma);
@@ -352,50 +398,91 @@ __global__ __launch_bounds__(kTriThreads) void tridiag_lds_kernel(const Batch *_
         tk = (beta - alpha) / beta;
         scale = 1.0 / (alpha - beta);
       }
+      if (wave == 0) {
 #pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        const int r = lane + 64 * s;
-        if (r < m) {
-          const double vnew = r == kk + 1 ? 1.0 : (r >= kk + 2 ? x[s] * scale : 0.0);
-          vl[par][r] = vnew;
-          if (r >= kk + 1) A[(k0 + r) + int64_t(k) * n] = vnew;
+        for (int s = 0; s < 3; ++s) {
+          const int r = lane + 64 * s;
+          if (r < m) {
+            const double vnew = r == kk + 1 ? 1.0 : (r >= kk + 2 ? x[s] * scale : 0.0);
+            V[par][r] = vnew;
+            if (r >= kk + 1) A[(k0 + r) + int64_t(k) * n] = vnew;
+          }
+        }
+        if (lane == 0) {
+          diag[k] = dk;
+          off[k] = beta;
+          tau[k] = tk;
         }
       }
-      if (lane == 0) {
-        tl[par] = tk;
-        diag[k] = dk;
-        off[k] = beta;
-        tau[k] = tk;
+      // (c) the pending update on this lane's entries of the live column groups, and their
+      //     products with the reflector; a wave whose rows are all finished (their entries are
+      //     never read again) or all past the order skips it, and so do column groups past the
+      //     order.  The columns go in pairs, each pair's reads fenced off from the next so the
+      //     compiler cannot hoist them all (their registers would push the entries out).
+      if (wave_end > kk + 1 && wave_end - 4 * kRegRows < m) {
+        double acc[kRegRows];
+#pragma unroll
+        for (int a = 0; a < kRegRows; ++a) acc[a] = 0.0;
+#pragma unroll
+        for (int b0 = 0; b0 < kRegCols; b0 += 2) {
+          if (b0 + 1 < B || 16 * b0 >= m) continue;   // finished, or past the order
+          double vc[2], wc[2], vn[2];
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            const int c = col_lane + 16 * (b0 + j);
+            // the reflector's entry c as arithmetic on 0/1 factors, so the read stays
+            // unconditional (a select around it becomes a branch and a wait per column)
+            const double xc = X[par][c];
+            vc[j] = V[prev][c];
+            wc[j] = P[prev][c] - half * vc[j];
+            const int d = c - (kk + 1);   // 0 at the reflector's leading 1, positive below it
+            const double below = static_cast<double>(min(max(d, 0), 1));
+            const double lead = static_cast<double>(1 - min(abs(d), 1));
+            vn[j] = fma(xc * scale, below, lead);
+          }
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            if (b0 + j < B) continue;
+#pragma unroll
+            for (int a = 0; a < kRegRows; ++a) {
+              const double y = fma(-vr[a], wc[j], fma(-wr[a], vc[j], T[a][b0 + j]));
+              T[a][b0 + j] = y;
+              acc[a] += y * vn[j];
+            }
+          }
+          __builtin_amdgcn_sched_barrier(0);
+        }
+        // (d) the rows' partial sums to LDS, one per column lane
+#pragma unroll
+        for (int a = 0; a < kRegRows; ++a) S[row0 + a][col_lane] = acc[a];
       }
-    }
-    __syncthreads();
-    // (b) rows and columns kk+1..: the pending update, and p = tau T v
-    const double tk = tl[par];
-    const int row = kk + 1 + (t >> 3), sub = t & 7;
-    double acc = 0.0;
-    if (row < m) {
-      const double vpr = vl[prev][row], wpr = pl[prev][row] - half * vpr;
-      for (int j = kk + 1 + sub; j < m; j += 8) {
-        const double vj = vl[prev][j], wj = pl[prev][j] - half * vj;
-        const double y = T[row * kLdsStride + j] - (vpr * wj + wpr * vj);
-        T[row * kLdsStride + j] = y;
-        acc += y * vl[par][j];
+      reg_barrier();
+      // p of this step at rows kk+1.., each row's 16 partials summed in order, zero elsewhere (the
+      // rows of waves that skipped the update hold stale partials, and are finished)
+      if (t < kRegOrder) {
+        double total = 0.0;
+#pragma unroll
+        for (int l = 0; l < 16; ++l) total += S[t][l];
+        P[par][t] = t >= kk + 1 && t < m ? tk * total : 0.0;
       }
+      reg_barrier();
+      // (e) this step's update becomes the pending one
+      half = half_of(par, kk + 1, tk);
+      pending = par;
     }
-    acc += __shfl_xor(acc, 1, 64);
-    acc += __shfl_xor(acc, 2, 64);
-    acc += __shfl_xor(acc, 4, 64);
-    if (sub == 0 && row < m) pl[par][row] = tk * acc;
-    __syncthreads();
-    // (c) this step's update becomes the pending one
-    half = half_of(par, kk + 1);
   }
-  // the last diagonal entry, with the pending update
-  if (t == 0) {
-    const int last = m - 1, buf = (m - 2) & 1;
-    const double v = m >= 2 ? vl[buf][last] : vl[1][0];
-    const double w = m >= 2 ? pl[buf][last] - half * v : pl[1][0] - half * v;
-    diag[n - 1] = T[last * kLdsStride + last] - 2.0 * v * w;
+  // the last diagonal entry, with the pending update, from the lane holding it
+  const int last = m - 1;
+  if (col_lane == (last & 15) && last >= row0 && last < row0 + kRegRows) {
+    const int bk = last >> 4, ak = last - row0;
+    double y = 0.0;
+#pragma unroll
+    for (int a = 0; a < kRegRows; ++a)
+#pragma unroll
+      for (int b = 0; b < kRegCols; ++b)
+        if (a == ak && b == bk) y = T[a][b];
+    const double v = V[pending][last], w = P[pending][last] - half * v;
+    diag[n - 1] = y - 2.0 * v * w;
   }
 }
 
@@ -1329,10 +1416,10 @@ int eigh_batched(const tt_eigh_problem *problems, int64_t count, double *matrice
     TT_LAUNCHED();
     int64_t largest = 0;
     for (int32_t b = 0; b < batch.count; ++b) largest = std::max<int64_t>(largest, batch.p[b].n);
-    // steps over many workgroups while the largest trailing matrix exceeds kLdsOrder rows, then
-    // the rest of every matrix in LDS
-    const int lds_from = static_cast<int>(std::max<int64_t>(0, largest - kLdsOrder));
-    for (int k = 0; k < lds_from; ++k) {
+    // steps over many workgroups while the largest trailing matrix exceeds kRegOrder rows, then
+    // the rest of every matrix in one workgroup each
+    const int reg_from = static_cast<int>(std::max<int64_t>(0, largest - kRegOrder));
+    for (int k = 0; k < reg_from; ++k) {
       int grid = 0;
       for (int32_t b = 0; b < batch.count; ++b) grid += step_blocks(batch.p[b].n, k);
       // the step's longest vectors are rows k..n-1; in timing mode each step launch is timed
@@ -1351,8 +1438,8 @@ int eigh_batched(const tt_eigh_problem *problems, int64_t count, double *matrice
       if (rc) return rc;
       TT_LAUNCHED();
     }
-    hipLaunchKernelGGL(tridiag_lds_kernel, dim3(batch.count), dim3(kTriThreads), 0, s, table, matrices, workspace,
-                       lds_from);
+    hipLaunchKernelGGL(tridiag_reg_kernel, dim3(batch.count), dim3(kRegThreads), 0, s, table, matrices, workspace,
+                       reg_from);
     TT_LAUNCHED();
     hipLaunchKernelGGL(leaf_kernel, dim3((leaves + kEigWaves - 1) / kEigWaves), dim3(kEigThreads), 0, s, table, values,
                        workspace, leaves);

@@ -113,15 +113,15 @@ def test_row_and_lincomb_tables():
 
 def test_tridiagonal_step_launches_and_bytes():
     '''The step launches tt_eigh_batched makes (one per step below the batch's largest order minus
-    128, per batch of 64 matrices) and their algorithmic bytes, 8 (2 m^2 + 6 m) per matrix with
+    192, per batch of 64 matrices) and their algorithmic bytes, 8 (2 m^2 + 6 m) per matrix with
     m > 0 trailing columns.'''
-    assert _native.tridiagonal_steps([128, 100, 5]) == (0, 0)
-    launches, moved = _native.tridiagonal_steps([130, 129, 3])
+    assert _native.tridiagonal_steps([192, 100, 5]) == (0, 0)
+    launches, moved = _native.tridiagonal_steps([194, 193, 3])
     per = lambda m: 8 * (2 * m * m + 6 * m)
     assert launches == 2
-    assert moved == per(129) + per(128) + per(2) + per(128) + per(127) + per(1)
-    # 65 matrices make two batches: the second (one matrix of order 131) steps three times
-    launches, _ = _native.tridiagonal_steps([200] + [10] * 63 + [131])
+    assert moved == per(193) + per(192) + per(2) + per(192) + per(191) + per(1)
+    # 65 matrices make two batches: the second (one matrix of order 195) steps three times
+    launches, _ = _native.tridiagonal_steps([264] + [10] * 63 + [195])
     assert launches == 72 + 3
 
 
