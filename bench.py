@@ -169,7 +169,7 @@ def main() -> int:
     parser.add_argument('--iterations', type=int, default=5)
     parser.add_argument('--precision', choices=('fp64', 'fp32'), default='fp32',
                         help='block products of the batched form: fp32 (with FP64 defect correction) or fp64')
-    parser.add_argument('--refine', type=int, default=2, help='FP64 Lanczos steps after the FP32 solve')
+    parser.add_argument('--refine', type=int, default=3, help='FP64 Lanczos steps after the FP32 solve')
     parser.add_argument('--no-plain', action='store_true', help='skip the plain per-block sweeps')
     parser.add_argument('--plain-split', choices=('reference', 'shared', 'both'), default='both',
                         help="plain sweeps' split: per-block torch SVD as the reference, the batched "

@@ -119,7 +119,7 @@ class DMRG:
     steps from the Ritz vector; 'fp64' keeps every product in FP64.'''
 
     def __init__(self, model: Model, max_bond_dim: int = 50, svd_tol: float = 1e-14, n_iter: int = 5,
-                 seed: int = 0, dtype=None, device=None, precision: str = 'fp64', refine: int = 2) -> None:
+                 seed: int = 0, dtype=None, device=None, precision: str = 'fp64', refine: int = 3) -> None:
         if precision not in ('fp64', 'fp32'):
             raise ValueError(f"precision has to be 'fp64' or 'fp32', not {precision!r}")
         self.model = model
