@@ -185,7 +185,8 @@ int tt_graph_destroy(tt_graph *graph);
 /* Batched symmetric eigendecomposition in FP64 (the split of the two-site tensor: the Gram matrix
  * of each block).  Replaces the per-block `at::linalg_svd` of `split_over`
  * (tentorch/linalg/graded/svd.cpp:41-177) with one eigendecomposition of all blocks at once:
- * tridiagonalisation (one launch per Householder step when a matrix exceeds order 128),
+ * tridiagonalisation (one launch per Householder step while a trailing matrix exceeds order 192,
+ * the last 192 steps in one launch with each trailing matrix in a workgroup's registers),
  * divide-and-conquer leaves and merge levels, blocked back-transformation; stream-ordered, no host
  * synchronisation, no allocation.  Matrix b is the symmetric n x n at
  * `matrices + matrix_offset` (overwritten: it keeps the Householder reflectors); its eigenvalues go
