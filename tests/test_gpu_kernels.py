@@ -397,13 +397,14 @@ def test_batched_eigh_matches_numpy(largest):
     The tridiagonalisation runs one launch per step over many workgroups until the largest trailing
     matrix has 192 rows, then finishes every matrix in one workgroup with the trailing matrix in
     registers: a batch reaching order 2048 (the largest taken), one reaching 300 (matrices of order
-    260 and 200 hand over part-way), one reaching exactly 192 (no stepped launch, the register
-    workgroup's full size), one reaching 160 and one with nothing above 128.'''
+    260, 200 and 193 hand over part-way, 193 after one stepped launch), one reaching exactly 192 (no
+    stepped launch, the register workgroup's full size), one reaching 160 and one with nothing above
+    128 (orders 16, 33 and 177 end a sixteen-column group exactly, just past one, and part-way).'''
     from tentorch_amd import _native
     assert _native.EIGH_MAX_ORDER == 2048
     rng = np.random.default_rng(21)
     cases = [c for c in _eigh_cases(rng) if c.shape[0] <= largest]
-    for n in {2048: (1024, 2048), 300: (300,), 192: (192, 191), 160: (160, 159), 128: ()}[largest]:
+    for n in {2048: (1024, 2048), 300: (300, 193), 192: (192, 191, 177), 160: (160, 159, 33), 128: (16,)}[largest]:
         a = rng.standard_normal((n, n))
         cases.append(a + a.T)
     packed = torch.tensor(np.concatenate([c.reshape(-1) for c in cases]), device='cuda')
