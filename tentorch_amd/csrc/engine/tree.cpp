@@ -54,6 +54,32 @@ std::unordered_map<LayoutKey, LayoutPtr, LayoutKeyHash> &layout_table() {
   return table;
 }
 
+// Tree sets by (category, the legs' sector lists, root): two keys are equal when every leg has the
+// same sectors, whatever its multiplicities.
+struct TreeSetKeyEq {
+  bool operator()(const LayoutKey &a, const LayoutKey &b) const {
+    if (a.cat != b.cat || a.legs.size() != b.legs.size() || !(a.root == b.root)) return false;
+    for (size_t i = 0; i < a.legs.size(); ++i)
+      if (a.legs[i] != b.legs[i] && a.legs[i]->sectors() != b.legs[i]->sectors()) return false;
+    return true;
+  }
+};
+
+struct TreeSetKeyHash {
+  size_t operator()(const LayoutKey &k) const {
+    size_t seed = std::hash<const void *>()(k.cat);
+    for (const Leg *l : k.legs) mix(seed, l->sectors_hash());
+    mix(seed, k.legs.size());
+    mix(seed, SectorHash()(k.root));
+    return seed;
+  }
+};
+
+std::unordered_map<LayoutKey, std::shared_ptr<const TreeSet>, TreeSetKeyHash, TreeSetKeyEq> &tree_set_table() {
+  static std::unordered_map<LayoutKey, std::shared_ptr<const TreeSet>, TreeSetKeyHash, TreeSetKeyEq> table;
+  return table;
+}
+
 struct StructureKey {
   const Category *cat;
   std::vector<const Leg *> codomain, domain;
@@ -106,20 +132,22 @@ LegPtr Leg::make(CategoryPtr cat, std::vector<Sector> sectors, std::vector<int64
     if (cat->q_dim(s) <= 0.0)
       throw std::invalid_argument("The sector " + render(s) + " has no components in " + cat->name() + ".");
   LegKey key{cat.get(), {}};
-  size_t hash = std::hash<std::string>{}(cat->name());
+  size_t hash = std::hash<std::string>{}(cat->name()), sectors_hash = hash;
   for (size_t i = 0; i < ss.size(); ++i) {
     key.data.push_back(ss[i].n);
     for (int j = 0; j < ss[i].n; ++j) {
       key.data.push_back(ss[i].v[j]);
       mix(hash, std::hash<int64_t>{}(ss[i].v[j]));
+      mix(sectors_hash, std::hash<int64_t>{}(ss[i].v[j]));
     }
+    mix(sectors_hash, static_cast<size_t>(ss[i].n));
     key.data.push_back(ms[i]);
     mix(hash, std::hash<int64_t>{}(ms[i]));
   }
   auto &table = leg_table();
   auto found = table.find(key);
   if (found != table.end()) return found->second;
-  LegPtr leg = std::make_shared<Leg>(cat, std::move(ss), std::move(ms), hash);
+  LegPtr leg = std::make_shared<Leg>(cat, std::move(ss), std::move(ms), hash, sectors_hash);
   table.emplace(std::move(key), leg);
   return leg;
 }
@@ -200,16 +228,30 @@ size_t TreeHash::operator()(const FusionTree &t) const {
 }
 
 namespace {
+// Reachable charges depend on the legs' sectors only: keys are equal when every leg has the same
+// sectors, whatever its multiplicities.
+using ReachableKey = std::pair<const Category *, std::vector<const Leg *>>;
+
 struct ReachableKeyHash {
-  size_t operator()(const std::pair<const Category *, std::vector<const Leg *>> &k) const {
-    return hash_legs(std::hash<const void *>()(k.first), k.second);
+  size_t operator()(const ReachableKey &k) const {
+    size_t seed = std::hash<const void *>()(k.first);
+    for (const Leg *l : k.second) mix(seed, l->sectors_hash());
+    mix(seed, k.second.size());
+    return seed;
   }
 };
 
-std::unordered_map<std::pair<const Category *, std::vector<const Leg *>>, std::vector<Sector>, ReachableKeyHash> &
-reachable_table() {
-  static std::unordered_map<std::pair<const Category *, std::vector<const Leg *>>, std::vector<Sector>, ReachableKeyHash>
-      table;
+struct ReachableKeyEq {
+  bool operator()(const ReachableKey &a, const ReachableKey &b) const {
+    if (a.first != b.first || a.second.size() != b.second.size()) return false;
+    for (size_t i = 0; i < a.second.size(); ++i)
+      if (a.second[i] != b.second[i] && a.second[i]->sectors() != b.second[i]->sectors()) return false;
+    return true;
+  }
+};
+
+std::unordered_map<ReachableKey, std::vector<Sector>, ReachableKeyHash, ReachableKeyEq> &reachable_table() {
+  static std::unordered_map<ReachableKey, std::vector<Sector>, ReachableKeyHash, ReachableKeyEq> table;
   return table;
 }
 std::vector<Sector> compute_reachable(const Category &cat, const Legs &legs);
@@ -316,16 +358,28 @@ LayoutPtr layout_of(const CategoryPtr &cat, const Legs &legs, const Sector &root
   auto &table = layout_table();
   auto found = table.find(key);
   if (found != table.end()) return found->second;
-  auto layout = std::make_shared<SideLayout>();
-  layout->trees = fusion_trees(*cat, legs, root);
+  // the trees and their lookup, shared with every layout over legs of the same sectors
+  auto &sets = tree_set_table();
+  auto known = sets.find(key);
+  if (known == sets.end()) {
+    auto set = std::make_shared<TreeSet>();
+    set->trees = fusion_trees(*cat, legs, root);
+    set->abelian = cat->abelian();
+    // an abelian tree is fixed by its sector list, so one index serves both lookups
+    if (set->abelian) {
+      set->by_sectors.reserve(set->trees.size());
+      for (size_t t = 0; t < set->trees.size(); ++t) set->by_sectors.emplace(set->trees[t].sectors, static_cast<int64_t>(t));
+    } else {
+      set->index.reserve(set->trees.size());
+      for (size_t t = 0; t < set->trees.size(); ++t) set->index.emplace(set->trees[t], static_cast<int64_t>(t));
+    }
+    known = sets.emplace(key, std::move(set)).first;
+  }
+  auto layout = std::make_shared<SideLayout>(known->second);
   layout->nlegs = static_cast<int64_t>(legs.size());
-  layout->abelian = cat->abelian();
   layout->offsets.reserve(layout->trees.size() + 1);
   layout->offsets.push_back(0);
   layout->mults.reserve(layout->trees.size() * legs.size());
-  // an abelian tree is fixed by its sector list, so one index serves both lookups
-  if (layout->abelian) layout->by_sectors.reserve(layout->trees.size());
-  else layout->index.reserve(layout->trees.size());
   for (size_t t = 0; t < layout->trees.size(); ++t) {
     int64_t rows = 1;
     for (size_t axis = 0; axis < legs.size(); ++axis) {
@@ -334,8 +388,6 @@ LayoutPtr layout_of(const CategoryPtr &cat, const Legs &legs, const Sector &root
       rows *= m;
     }
     layout->offsets.push_back(layout->offsets.back() + rows);
-    if (layout->abelian) layout->by_sectors.emplace(layout->trees[t].sectors, static_cast<int64_t>(t));
-    else layout->index.emplace(layout->trees[t], static_cast<int64_t>(t));
   }
   table.emplace(std::move(key), layout);
   return layout;
@@ -412,6 +464,7 @@ void clear_engine_caches() {
   reachable_table().clear();
   structure_table().clear();
   layout_table().clear();
+  tree_set_table().clear();
 }
 
 }  // namespace tt

@@ -39,19 +39,22 @@ class Leg {
   int64_t block_of(const Sector &s) const;
   int64_t mult_of(const Sector &s) const { return mults_[static_cast<size_t>(block_of(s))]; }
   size_t hash() const { return hash_; }
+  // A hash of the category and the sectors alone (legs that differ only in multiplicities share it).
+  size_t sectors_hash() const { return sectors_hash_; }
 
   LegPtr fuse(const Leg &other) const;
   LegPtr conj() const;
   LegPtr capped(int64_t max_multiplets) const;
 
-  Leg(CategoryPtr cat, std::vector<Sector> sectors, std::vector<int64_t> mults, size_t hash)
-      : cat_(std::move(cat)), sectors_(std::move(sectors)), mults_(std::move(mults)), hash_(hash) {}
+  Leg(CategoryPtr cat, std::vector<Sector> sectors, std::vector<int64_t> mults, size_t hash, size_t sectors_hash)
+      : cat_(std::move(cat)), sectors_(std::move(sectors)), mults_(std::move(mults)), hash_(hash),
+        sectors_hash_(sectors_hash) {}
 
  private:
   CategoryPtr cat_;
   std::vector<Sector> sectors_;
   std::vector<int64_t> mults_;
-  size_t hash_;
+  size_t hash_, sectors_hash_;
 };
 
 using Legs = std::vector<LegPtr>;
@@ -81,29 +84,38 @@ struct SectorListHash {
   size_t operator()(const std::vector<Sector> &sectors) const;
 };
 
-struct SideLayout {
+// The fusion trees of one side at one root, in order, with their lookup.  They depend on the legs'
+// sectors only, so the layouts of legs that differ in multiplicities share one set.
+struct TreeSet {
   std::vector<FusionTree> trees;
-  std::vector<int64_t> offsets;   // size trees + 1
-  std::vector<int64_t> mults;     // per tree, the multiplicity each leg carries (trees x legs)
-  int64_t nlegs = 0;
   bool abelian = false;
   // Non-abelian categories: the position of each tree.
   std::unordered_map<FusionTree, int64_t, TreeHash> index;
   // Abelian categories: a tree is fixed by its sector list (one outcome per fusion, no copies), so
   // this index alone serves both lookups.
   std::unordered_map<std::vector<Sector>, int64_t, SectorListHash> by_sectors;
+};
+
+struct SideLayout {
+  explicit SideLayout(std::shared_ptr<const TreeSet> s) : set(std::move(s)), trees(set->trees), abelian(set->abelian) {}
+  std::shared_ptr<const TreeSet> set;
+  const std::vector<FusionTree> &trees;   // the set's trees
+  std::vector<int64_t> offsets;   // size trees + 1
+  std::vector<int64_t> mults;     // per tree, the multiplicity each leg carries (trees x legs)
+  int64_t nlegs = 0;
+  bool abelian = false;
   int64_t rows() const { return offsets.back(); }
   int64_t find(const FusionTree &t) const {
     if (abelian) {
       const int64_t at = find_sectors(t.sectors);
       return at >= 0 && trees[static_cast<size_t>(at)] == t ? at : -1;
     }
-    auto f = index.find(t);
-    return f == index.end() ? -1 : f->second;
+    auto f = set->index.find(t);
+    return f == set->index.end() ? -1 : f->second;
   }
   int64_t find_sectors(const std::vector<Sector> &sectors) const {
-    auto f = by_sectors.find(sectors);
-    return f == by_sectors.end() ? -1 : f->second;
+    auto f = set->by_sectors.find(sectors);
+    return f == set->by_sectors.end() ? -1 : f->second;
   }
 };
 
