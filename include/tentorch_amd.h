@@ -182,6 +182,14 @@ int tt_graph_end(void *stream, tt_graph **graph);
 int tt_graph_launch(const tt_graph *graph, void *stream);
 int tt_graph_destroy(tt_graph *graph);
 
+/* Table uploads in one copy.  No counterpart in the reference (its per-block calls upload
+ * nothing).  Plans created between tt_upload_batch_begin and tt_upload_batch_end (on the current
+ * device; batches nest) have their device tables copied together when the outermost batch ends,
+ * one copy and one event instead of one per plan; a launch inside a batch issues what is staged
+ * first.  For a caller that creates several plans at once (an einsum's moves and products). */
+int tt_upload_batch_begin(void);
+int tt_upload_batch_end(void);
+
 /* Batched symmetric eigendecomposition in FP64 (the split of the two-site tensor: the Gram matrix
  * of each block).  Replaces the per-block `at::linalg_svd` of `split_over`
  * (tentorch/linalg/graded/svd.cpp:41-177) with one eigendecomposition of all blocks at once:

@@ -813,10 +813,16 @@ uint32_t Program::hoistable() const {
 }
 
 void Program::prepare() const {
+  // every table of the program in one upload
+  struct Batch {
+    Batch() { check(tt_upload_batch_begin(), "table upload"); }
+    ~Batch() { (void)tt_upload_batch_end(); }
+  } batch;
   for (const Op &op : ops) {
     if (op.kind == OpKind::move) op.move->device();
     if (op.kind == OpKind::gemm) op.gemm->device();
   }
+  prepared_ = true;
 }
 
 void Program::run(int dtype, const std::vector<const void *> &operand_ptrs, void *out_ptr, void *workspace_ptr,
@@ -844,6 +850,7 @@ void Program::run(int dtype, const std::vector<const void *> &operand_ptrs, void
   };
   auto type_of = [&](int32_t v) { return slots[static_cast<size_t>(v)].kind == Slot::workspace ? ws_dtype : dtype; };
   if (workspace > 0 && !workspace_ptr) throw std::invalid_argument("a program needs its workspace");
+  if (mode == Mode::batched && !prepared_) prepare();
   const int64_t end = last < 0 ? static_cast<int64_t>(ops.size()) : std::min<int64_t>(last, static_cast<int64_t>(ops.size()));
   for (int64_t at = std::max<int64_t>(first, 0); at < end; ++at) {
     const Op &op = ops[static_cast<size_t>(at)];

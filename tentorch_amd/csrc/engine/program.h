@@ -93,6 +93,9 @@ class Program {
   // Uploads the device tables of every grouped op now (they are otherwise uploaded on first run):
   // before stream capture, and before timing single launches.
   void prepare() const;
+
+ private:
+  mutable bool prepared_ = false;   // every device table uploaded (prepare ran)
 };
 using ProgramPtr = std::shared_ptr<Program>;
 

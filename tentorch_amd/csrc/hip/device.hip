@@ -910,6 +910,12 @@ struct Uploader {
   size_t staging_capacity = 0, staging_used = 0;
   std::vector<ArenaChunk> chunks;
   uint64_t issued = 0, completed = 0;
+  // Inside tt_upload_batch_begin/end: staged tables whose copy is not issued yet, one contiguous
+  // range of staging and of the arena (consecutive uploads advance both by the same bytes).
+  int batch_depth = 0;
+  unsigned char *pending_stage = nullptr;
+  char *pending_target = nullptr;
+  size_t pending_bytes = 0;
 };
 std::mutex g_upload_mutex;
 std::vector<Uploader> g_uploaders;  // one per device
@@ -920,7 +926,49 @@ constexpr size_t kTableAlign = 256;
 
 size_t aligned(size_t bytes) { return (bytes + kTableAlign - 1) / kTableAlign * kTableAlign; }
 
+// Issues the copy of the pending range and records the upload event after it.
+int flush_pending(Uploader &u) {
+  if (!u.pending_bytes) return 0;
+  TT_HIP(hipMemcpyAsync(u.pending_target, u.pending_stage, u.pending_bytes, hipMemcpyHostToDevice, u.stream));
+  TT_HIP(hipEventRecord(u.done, u.stream));
+  u.issued += 1;
+  u.pending_stage = nullptr;
+  u.pending_target = nullptr;
+  u.pending_bytes = 0;
+  return 0;
+}
+
+Uploader *current_uploader(int *rc) {
+  int device = 0;
+  *rc = 0;
+  if (hipGetDevice(&device) != hipSuccess) {
+    *rc = fail("upload: no current device");
+    return nullptr;
+  }
+  if (g_uploaders.size() <= static_cast<size_t>(device)) g_uploaders.resize(static_cast<size_t>(device) + 1);
+  return &g_uploaders[static_cast<size_t>(device)];
+}
+
 }  // namespace
+
+extern "C" int tt_upload_batch_begin(void) {
+  std::lock_guard<std::mutex> lock(g_upload_mutex);
+  int rc = 0;
+  Uploader *u = current_uploader(&rc);
+  if (!u) return rc;
+  u->batch_depth += 1;
+  return 0;
+}
+
+extern "C" int tt_upload_batch_end(void) {
+  std::lock_guard<std::mutex> lock(g_upload_mutex);
+  int rc = 0;
+  Uploader *u = current_uploader(&rc);
+  if (!u) return rc;
+  if (u->batch_depth <= 0) return fail("tt_upload_batch_end: no batch is open");
+  if (--u->batch_depth == 0) return flush_pending(*u);
+  return 0;
+}
 
 int ttdev::upload_tables(void **block, std::initializer_list<Table> tables) {
   *block = nullptr;
@@ -940,6 +988,7 @@ int ttdev::upload_tables(void **block, std::initializer_list<Table> tables) {
   // device range: the current chunk, or a new one
   const size_t need = aligned(total);
   if (u.chunks.empty() || u.chunks.back().capacity - u.chunks.back().used < need) {
+    if (int rc = flush_pending(u)) return rc;
     ArenaChunk chunk;
     chunk.capacity = std::max(kChunkBytes, need);
     TT_HIP(hipMalloc(reinterpret_cast<void **>(&chunk.base), chunk.capacity));
@@ -954,6 +1003,7 @@ int ttdev::upload_tables(void **block, std::initializer_list<Table> tables) {
   char *const target = chunk.base + chunk.used;
   // staging: the rest of the ring, or (after the copies in flight have finished) its front
   if (u.staging_capacity < total) {
+    if (int rc = flush_pending(u)) return rc;
     TT_HIP(hipStreamSynchronize(u.stream));
     if (u.staging) TT_HIP(hipHostFree(u.staging));
     u.staging = nullptr;
@@ -962,6 +1012,7 @@ int ttdev::upload_tables(void **block, std::initializer_list<Table> tables) {
     TT_HIP(hipHostMalloc(reinterpret_cast<void **>(&u.staging), grown, hipHostMallocDefault));
     u.staging_capacity = grown;
   } else if (u.staging_capacity - u.staging_used < total) {
+    if (int rc = flush_pending(u)) return rc;
     TT_HIP(hipStreamSynchronize(u.stream));
     u.staging_used = 0;
   }
@@ -974,12 +1025,19 @@ int ttdev::upload_tables(void **block, std::initializer_list<Table> tables) {
     if (t.bytes) std::memcpy(stage + at, t.data, t.bytes);
     at += t.bytes;
   }
-  TT_HIP(hipMemcpyAsync(target, stage, total, hipMemcpyHostToDevice, u.stream));
-  TT_HIP(hipEventRecord(u.done, u.stream));
+  // the copy: now, or (inside a batch) with the pending range it extends
+  if (u.pending_bytes && (u.pending_stage + u.pending_bytes != stage || u.pending_target + u.pending_bytes != target))
+    if (int rc = flush_pending(u)) return rc;
+  if (!u.pending_bytes) {
+    u.pending_stage = stage;
+    u.pending_target = target;
+  }
+  u.pending_bytes += need;
+  if (u.batch_depth == 0)
+    if (int rc = flush_pending(u)) return rc;
   u.staging_used = std::min(u.staging_capacity, u.staging_used + need);
   chunk.used += need;
   chunk.live += 1;
-  u.issued += 1;
   *block = target;
   size_t k = 0;
   for (const Table &t : tables) {
@@ -1011,6 +1069,8 @@ int ttdev::await_uploads(hipStream_t stream) {
   std::lock_guard<std::mutex> lock(g_upload_mutex);
   if (g_uploaders.size() <= static_cast<size_t>(device)) return 0;
   Uploader &u = g_uploaders[static_cast<size_t>(device)];
+  // a launch inside a batch needs the tables staged so far
+  if (int rc = flush_pending(u)) return rc;
   if (u.completed == u.issued) return 0;
   hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
   TT_HIP(hipStreamIsCapturing(stream, &capture));
