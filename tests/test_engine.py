@@ -66,6 +66,31 @@ def test_structures_and_layouts_match_the_oracle(name):
             assert [(tuple(a), tuple(b), tuple(c), d) for a, b, c, d in trees_p] == [t.key() for t in trees_o]
 
 
+@pytest.mark.parametrize('name', sorted(CATS))
+def test_legs_with_the_same_sectors_and_other_multiplicities_lay_out_alike(name):
+    '''Legs that differ only in multiplicities share the planner's fusion-tree sets and reachable
+    charges; each structure must still get its own sizes and offsets, and its moves the oracle's.'''
+    cat_p, cat_o = CATS[name]
+    dtype = np.complex128 if name == 'fibonacci' else np.float64
+    exact = cat_o.abelian
+    for bump in (0, 1, 3, 0, 2):
+        legs = [(sectors, [m + bump for m in mults]) for sectors, mults in LEGS[name]]
+        s, o = both(name, [legs[0], legs[1]], [legs[1], legs[0]], seed=bump, dtype=dtype)
+        assert s.size == o.numel()
+        assert [k for k, _, _, _ in s.blocks()] == list(o.blocks.keys())
+        offset = 0
+        for (k, rows, cols, off), ob in zip(s.blocks(), o.blocks.values()):
+            assert (rows, cols) == ob.shape and off == offset
+            offset += rows * cols
+        for above, below in (([3, 1], [0, 2]), ([0, 1, 2], [3]), ([2], [3, 0, 1])):
+            got = run_program(engine.plan_place(s, above, below), [o.flat()], dtype)
+            ref = o.place(above, below).flat()
+            if exact:
+                assert np.array_equal(got, ref), (bump, above, below)
+            else:
+                assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), (bump, above, below)
+
+
 @pytest.mark.parametrize('name', ['su2', 'fibonacci', 'fermion_parity', 'u1'])
 def test_symbols_match_the_oracle(name):
     cat_p, cat_o = CATS[name]
