@@ -267,6 +267,57 @@ def test_typed_leg_move_converts_between_fp64_and_fp32(lib):
     lib.tt_gss_plan_destroy(plan)
 
 
+def test_plans_created_in_an_upload_batch_run_like_plans_created_alone(lib):
+    '''Leg-move plans created inside nested tt_upload_batch_begin/end have their tables copied
+    together when the outer batch ends. One run while both batches are open (its staged tables are
+    issued first), one between the two ends and one after them give the bits of the same plans
+    created outside any batch. An end with no batch open is refused.'''
+    from tentorch_amd._native import engine
+    from tentorch_amd import u1, Leg
+    rng = np.random.default_rng(11)
+    bond = ([(-2,), (0,), (2,)], [2, 3, 2])
+    phys = ([(-1,), (1,)], [1, 1])
+    s = engine.Structure(u1, [Leg(u1, *bond), Leg(u1, *phys)], [Leg(u1, *phys), Leg(u1, *bond)], (0,))
+    src = torch.tensor(rng.standard_normal(s.size), device='cuda')
+    moves = []
+    for above, below in (([3, 0], [1, 2]), ([1], [3, 2, 0]), ([2, 3, 1], [0])):
+        program = engine.plan_place(s, above, below)
+        assert 'move' in program.op_kinds()
+        tasks, sources = program.move_tables(0)
+        moves.append((cabi.gss_tables(tasks, sources), len(tasks), len(sources), program.out.size))
+
+    def create(move):
+        (tasks, sources), num_tasks, num_sources, _ = move
+        plan = ctypes.c_void_p()
+        assert lib.tt_gss_plan_create(tasks, num_tasks, sources, num_sources, ctypes.byref(plan)) == 0, cabi.error(lib)
+        return plan
+
+    def run(plan, move):
+        dst = torch.full((move[3],), float('nan'), dtype=torch.float64, device='cuda')
+        assert lib.tt_gss_execute(plan, cabi.TT_F64, src.data_ptr(), dst.data_ptr(), _stream()) == 0, cabi.error(lib)
+        return dst.cpu()
+
+    alone = []
+    for move in moves:
+        plan = create(move)
+        alone.append(run(plan, move))
+        lib.tt_gss_plan_destroy(plan)
+    assert lib.tt_upload_batch_end() != 0
+    assert 'no batch is open' in cabi.error(lib)
+    assert lib.tt_upload_batch_begin() == 0
+    assert lib.tt_upload_batch_begin() == 0
+    plans = [create(move) for move in moves]
+    got = [run(plans[0], moves[0])]
+    assert lib.tt_upload_batch_end() == 0
+    got.append(run(plans[1], moves[1]))
+    assert lib.tt_upload_batch_end() == 0
+    got.append(run(plans[2], moves[2]))
+    for a, b in zip(alone, got):
+        assert torch.equal(a, b)
+    for plan in plans:
+        lib.tt_gss_plan_destroy(plan)
+
+
 @pytest.mark.parametrize('precision', ['f64', 'f32'])
 def test_every_launch_configuration_gives_identical_products(lib, precision):
     '''All tile shapes and both packings: the same values bit for bit (each output element sums its
